@@ -1,0 +1,172 @@
+"""Benchmark of the DG residual / time-step path on one MI355X.
+
+Workload: the Riemann-box configuration (Euler P1, uniform octree level 7 =
+128^3 = 2,097,152 hexes, HLLC flux + Barth-Jespersen limiter, SSP-RK3,
+transmissive boundaries, dt = 0.3 h / max(|u| + c) fixed at t = 0).  Both the
+FP32-bulk and the FP64 plain paths run K steps from the same initial state;
+value = DOF-updates/s of the FP32 path (E * 4 * V * 3 stages * K / wall).
+
+    python bench.py [--steps K] [--warmup W] [--level L] [--no-fp64] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "DOF-updates/s, FP32-mix vs FP64 at L2≤1e-6; %HBM roofline; adjoint/fwd"
+HBM_PEAK_GBS = 8000.0
+
+
+def stage_kernel_bytes(nv, real_bytes):
+    """Algorithmic HBM bytes per element of one stage-kernel launch: read the
+    stage state and write the new state (4V coefficients each), six int32
+    neighbour indices and four geometry words."""
+    return real_bytes * 4 * nv * 2 + 4 * 6 + 4 * 4
+
+
+def step_bytes(nv, real_bytes):
+    """Algorithmic bytes per element and RK stage of a whole step: state read
+    and write, the extra U^n read of stages 2-3 (2/3 on average), indices and
+    geometry: 4 (4V)(2 + 2/3) + 24 + 16 for FP32."""
+    return real_bytes * 4 * nv * (2 + 2.0 / 3.0) + 4 * 6 + 4 * 4
+
+
+def run_path(CA, W, level, precision, steps, warmup, u0=None):
+    cfg, leaves, uic, periodic = W.c2(level=level, precision=precision)
+    if u0 is None:
+        u0 = uic
+    ctx = CA.GpuContext(cfg)
+    ce, cf = CA.recommended_chunks(cfg)
+    ctx.upload_mesh(CA.Forest.from_leaves(leaves, periodic).flatten(ce, cf))
+    ctx.set_state(u0)
+    dt = ctx.get_dt(mode=1)
+    m0 = ctx.conservation_sums()
+    if warmup:
+        ctx.run(dt, warmup)
+        ctx.set_state(u0)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.run(dt, steps)
+    wall = time.perf_counter() - t0
+    dev_ms = ctx.last_ms()
+    state = ctx.get_state()
+    m1 = ctx.conservation_sums()
+    return dict(ctx=ctx, cfg=cfg, leaves=leaves, u0=u0, dt=dt, wall=wall, dev_ms=dev_ms,
+                state=state, m0=m0, m1=m1)
+
+
+def cpu_baseline(W, budget_s):
+    """The FP64 oracle (single thread) on the same workload at level 4
+    (4,096 hexes), steps until about budget_s of CPU time."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as OL
+    import charm_3d_amd as CA
+    cfg, leaves, u0, _ = W.c2(level=4, precision=CA.FP64)
+    ocfg = OL.make_config(flux="HLLC", limiter=True, model="NS", bnd=("OUTLET",) * 6)
+    sim = OL.OracleSim(ocfg, leaves)
+    sim.set_state(u0)
+    dt = sim.cfl_dt(1)
+    steps, t = 0, 0.0
+    t0 = time.perf_counter()
+    while t < budget_s and steps < 10000:
+        sim.run(dt, 1)
+        steps += 1
+        t = time.perf_counter() - t0
+    dofs = W.dof_updates(len(leaves), 5, 3, steps)
+    return {"value": dofs / t, "unit": "DOF-updates/s", "cores": 1, "kind": "port",
+            "sample": "FP64 oracle (C restatement of the reference numerics, AoS records, "
+                      "callback face traversal), Riemann box at level 4 (4,096 hexes), "
+                      "HLLC + BJ, SSP-RK3, %d steps in %.1f s on one host core" % (steps, t)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--level", type=int, default=7)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--kernel-reps", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import charm_3d_amd as CA
+    import charm_3d_amd.workloads as W
+
+    print("[bench] FP32 bulk path, level %d, %d steps" % (args.level, args.steps), file=sys.stderr, flush=True)
+    r32 = run_path(CA, W, args.level, CA.FP32, args.steps, args.warmup)
+    n = len(r32["leaves"])
+    nv = 5
+    dofs = W.dof_updates(n, nv, 3, args.steps)
+    value = dofs / r32["wall"]
+
+    # dominant kernel: the stage kernel, timed with HIP events on its stream
+    ctx = r32["ctx"]
+    ctx.set_state(r32["u0"])
+    stage_ms, lim_ms = ctx.time_kernels(r32["dt"], args.kernel_reps)
+    kb = stage_kernel_bytes(nv, 4) * n
+    achieved = kb / (stage_ms * 1e-3) / 1e9
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "DOF-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * r32["wall"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "Riemann box (rho,p)=(1,1)|(0.125,0.1) at x=0.5 with seeded "
+                               "+-1% cell-mean perturbation, Euler P1 Taylor basis, uniform "
+                               "octree level %d (%d hexes), HLLC flux + Barth-Jespersen limiter, "
+                               "SSP-RK3, transmissive boundaries, dt = 0.3 h / max(|u|+c)" % (args.level, n),
+                   "elements": n, "vars": nv, "stages_per_step": 3, "dt": r32["dt"]},
+        "roofline": {"bound": "hbm", "kernel": "k_stage (fused face flux + volume + gather + update)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "stage_ms": stage_ms, "limiter_ms": lim_ms,
+                     "bytes_per_elem": stage_kernel_bytes(nv, 4)},
+        "whole_step": {"device_ms_per_step": r32["dev_ms"] / args.steps,
+                       "algorithmic_GBs": step_bytes(nv, 4) * n * 3 * args.steps / r32["wall"] / 1e9,
+                       "hbm_frac": step_bytes(nv, 4) * n * 3 * args.steps / r32["wall"] / 1e9 / HBM_PEAK_GBS},
+        "adjoint_over_forward": None,
+    }
+    mass0 = r32["m0"][4]
+    out["fp32"] = {"mass_drift": abs(r32["m1"][4] - mass0) / abs(mass0)}
+
+    if not args.no_fp64:
+        print("[bench] FP64 plain path", file=sys.stderr, flush=True)
+        r64 = run_path(CA, W, args.level, CA.FP64, args.steps, min(args.warmup, 2), u0=r32["u0"])
+        d = r32["state"] - r64["state"]
+        l2 = float(np.linalg.norm(d) / np.linalg.norm(r64["state"]))
+        linf = float(np.abs(d).max() / np.abs(r64["state"]).max())
+        out["fp64"] = {"value": dofs / r64["wall"], "ms_per_step": 1e3 * r64["wall"] / args.steps,
+                       "mass_drift": abs(r64["m1"][4] - r64["m0"][4]) / abs(r64["m0"][4])}
+        out["errors_vs_fp64"] = {"l2_rel": l2, "linf_rel": linf,
+                                 "l2_rel_means": float(np.linalg.norm(d[:, :, 0]) / np.linalg.norm(r64["state"][:, :, 0]))}
+        out["speedup_fp32_over_fp64"] = r64["wall"] / r32["wall"]
+        del r64
+
+    if not args.no_cpu:
+        print("[bench] CPU baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(W, args.cpu_seconds)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,332 @@
+"""charm_3d_amd -- MI355X-native DG residual / time-step path of CHARM_3D.
+
+The product is the C-ABI library libcharm3d.so (include/charm_gpu.h): a C
+host octree with its flattening plus hand-written gfx950 HIP kernels.  This
+module is a thin ctypes binding over that ABI for tests and benchmarks; it
+never computes the path itself and raises if the library is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libcharm3d.so")
+
+MAX_COMP = 8
+FP64, FP32 = 0, 1
+FLUX = {"LF": 0, "HLLC": 1, "GODUNOV": 2, "CD": 3}
+BND = {"INLET": 0, "OUTLET": 1, "WALL_SLIP": 2, "WALL_NO_SLIP": 3, "MASS_FLOW": 4,
+       "PERIODIC": 100}
+MODEL = {"EULER": 0, "NS": 1}
+EOS = {"IDEAL": 0, "MIX": 1}
+GAS_R = 8.31446261815324
+ROOT_LEN = 1 << 19
+
+
+class CharmError(RuntimeError):
+    pass
+
+
+class CompDesc(C.Structure):
+    _fields_ = [("m", C.c_double), ("cp_type", C.c_int), ("ml_type", C.c_int),
+                ("n_cp", C.c_int), ("cp", C.c_double * 8), ("ml0", C.c_double),
+                ("kp0", C.c_double), ("t0", C.c_double), ("ts", C.c_double), ("h0", C.c_double)]
+
+
+class Config(C.Structure):
+    _fields_ = [("precision", C.c_int), ("model", C.c_int), ("flux", C.c_int),
+                ("limiter", C.c_int), ("eos", C.c_int), ("n_comp", C.c_int),
+                ("comp", CompDesc * MAX_COMP), ("bnd_type", C.c_int * 6),
+                ("bnd_param", (C.c_double * 16) * 6), ("grav", C.c_double * 3),
+                ("cfl", C.c_double), ("dt", C.c_double), ("tmax", C.c_double),
+                ("log_period", C.c_int), ("chunk_elems", C.c_int), ("chunk_faces", C.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libcharm3d.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise CharmError("libcharm3d.so not built: run `python -m charm_3d_amd.build` "
+                         "(or __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    vp, i64, i32p, dp = C.c_void_p, C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int)
+    sig = {
+        "charm_gpu_last_error": (C.c_char_p, []),
+        "charm_gpu_device_count": (C.c_int, [ip]),
+        "charm_forest_new_uniform": (C.c_int, [C.c_int, ip, C.POINTER(vp)]),
+        "charm_forest_new_leaves": (C.c_int, [i64, i32p, ip, C.POINTER(vp)]),
+        "charm_forest_destroy": (C.c_int, [vp]),
+        "charm_forest_num_leaves": (i64, [vp]),
+        "charm_forest_get_leaves": (C.c_int, [vp, i32p]),
+        "charm_forest_adapt": (C.c_int, [vp, C.POINTER(C.c_int8), C.c_int, C.c_int,
+                                         C.POINTER(C.POINTER(C.c_int64)), C.POINTER(C.c_int64)]),
+        "charm_forest_balance": (C.c_int, [vp]),
+        "charm_forest_flatten": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(vp)]),
+        "charm_mesh_destroy": (C.c_int, [vp]),
+        "charm_mesh_num_faces": (i64, [vp]),
+        "charm_mesh_num_chunks": (i64, [vp]),
+        "charm_mesh_get_faces": (C.c_int, [vp, i32p]),
+        "charm_mesh_get_neighbors": (C.c_int, [vp, i32p]),
+        "charm_gpu_create": (C.c_int, [C.POINTER(Config), C.POINTER(vp)]),
+        "charm_gpu_destroy": (C.c_int, [vp]),
+        "charm_gpu_upload_mesh": (C.c_int, [vp, vp]),
+        "charm_gpu_set_state": (C.c_int, [vp, dp, i64]),
+        "charm_gpu_get_state": (C.c_int, [vp, dp, i64]),
+        "charm_gpu_residual": (C.c_int, [vp, dp, i64]),
+        "charm_gpu_limiter_bj": (C.c_int, [vp]),
+        "charm_gpu_run": (C.c_int, [vp, C.c_double, C.c_int]),
+        "charm_gpu_step": (C.c_int, [vp, C.c_int, C.c_double, C.c_int]),
+        "charm_gpu_recommended_chunks": (C.c_int, [C.POINTER(Config), ip, ip]),
+        "charm_gpu_time_kernels": (C.c_int, [vp, C.c_double, C.c_int, dp, dp]),
+        "charm_gpu_get_dt": (C.c_int, [vp, C.c_int, dp]),
+        "charm_gpu_conservation_sums": (C.c_int, [vp, dp]),
+        "charm_gpu_state_flag": (C.c_int, [vp, ip]),
+        "charm_gpu_synchronize": (C.c_int, [vp]),
+        "charm_gpu_last_timing": (C.c_int, [vp, dp, dp]),
+        "charm_gpu_flux_batch": (C.c_int, [vp, C.c_int, i64, dp, dp, dp, dp]),
+        "charm_ctx_create": (C.c_int, [vp, C.POINTER(Config), C.POINTER(vp)]),
+        "charm_ctx_destroy": (C.c_int, [vp]),
+        "charm_ctx_gpu": (vp, [vp]),
+        "charm_model_ns_timestep_single": (None, [vp, dp, vp, vp]),
+        "charm_model_euler_timestep_single": (None, [vp, dp, vp, vp]),
+        "charm_model_ns_get_dt": (C.c_double, [vp]),
+        "charm_model_euler_get_dt": (C.c_double, [vp]),
+        "charm_timesteps": (None, [vp]),
+        "charm_limiter": (None, [vp, vp, vp]),
+        "charm_limiter_bj": (None, [vp, vp, vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().charm_gpu_last_error()
+        raise CharmError("charm error %d: %s" % (rc, msg.decode() if msg else "?"))
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _i32ptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def _periodic(per):
+    arr = (C.c_int * 3)(*[1 if p else 0 for p in per])
+    return arr
+
+
+def make_config(precision=FP32, model="NS", flux="HLLC", limiter=False, eos="IDEAL",
+                comp_m=(GAS_R,), comp_cp=(3.5,), bnd=("OUTLET",) * 6, bnd_param=None,
+                grav=(0.0, 0.0, 0.0), cfl=0.3, dt=0.0, tmax=0.0, chunk_elems=0, chunk_faces=0):
+    """charm_config_t from task.yaml-like keys (constant cp per species)."""
+    cfg = Config()
+    cfg.precision = precision
+    cfg.model = MODEL[model]
+    cfg.flux = FLUX[flux]
+    cfg.limiter = 1 if limiter else 0
+    cfg.eos = EOS[eos]
+    cfg.n_comp = len(comp_m)
+    for i, (m, cp) in enumerate(zip(comp_m, comp_cp)):
+        cfg.comp[i].m = m
+        cfg.comp[i].n_cp = 1
+        cfg.comp[i].cp[0] = cp
+    for i in range(6):
+        cfg.bnd_type[i] = BND[bnd[i]]
+        if bnd_param is not None and bnd_param[i] is not None:
+            for j, v in enumerate(bnd_param[i]):
+                cfg.bnd_param[i][j] = v
+    for i in range(3):
+        cfg.grav[i] = grav[i]
+    cfg.cfl, cfg.dt, cfg.tmax = cfl, dt, tmax
+    cfg.chunk_elems, cfg.chunk_faces = chunk_elems, chunk_faces
+    return cfg
+
+
+def recommended_chunks(cfg):
+    ce, cf = C.c_int(), C.c_int()
+    check(lib().charm_gpu_recommended_chunks(C.byref(cfg), C.byref(ce), C.byref(cf)))
+    return ce.value, cf.value
+
+
+class Mesh:
+    """Flattened leaves (charm_mesh_t)."""
+
+    def __init__(self, handle, n):
+        self.h = handle
+        self.n = n
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().charm_mesh_destroy(self.h)
+            self.h = None
+
+    @property
+    def num_faces(self):
+        return lib().charm_mesh_num_faces(self.h)
+
+    @property
+    def num_chunks(self):
+        return lib().charm_mesh_num_chunks(self.h)
+
+    def faces(self):
+        out = np.zeros((self.num_faces, 4), dtype=np.int32)
+        check(lib().charm_mesh_get_faces(self.h, _i32ptr(out)))
+        return out
+
+    def neighbors(self):
+        out = np.zeros((self.n, 6), dtype=np.int32)
+        check(lib().charm_mesh_get_neighbors(self.h, _i32ptr(out)))
+        return out
+
+
+class Forest:
+    """Host linear octree on the unit cube (charm_forest_t)."""
+
+    def __init__(self, handle, periodic):
+        self.h = handle
+        self.periodic = tuple(bool(p) for p in periodic)
+
+    @classmethod
+    def uniform(cls, level, periodic=(False, False, False)):
+        h = C.c_void_p()
+        check(lib().charm_forest_new_uniform(level, _periodic(periodic), C.byref(h)))
+        return cls(h, periodic)
+
+    @classmethod
+    def from_leaves(cls, leaves, periodic=(False, False, False)):
+        lv = np.ascontiguousarray(leaves, dtype=np.int32)
+        h = C.c_void_p()
+        check(lib().charm_forest_new_leaves(len(lv), _i32ptr(lv), _periodic(periodic), C.byref(h)))
+        return cls(h, periodic)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().charm_forest_destroy(self.h)
+            self.h = None
+
+    @property
+    def n(self):
+        return lib().charm_forest_num_leaves(self.h)
+
+    def leaves(self):
+        out = np.zeros((self.n, 4), dtype=np.int32)
+        check(lib().charm_forest_get_leaves(self.h, _i32ptr(out)))
+        return out
+
+    def balance(self):
+        check(lib().charm_forest_balance(self.h))
+
+    def adapt(self, flags, max_level, min_level=0):
+        fl = np.ascontiguousarray(flags, dtype=np.int8)
+        mp = C.POINTER(C.c_int64)()
+        nm = C.c_int64()
+        check(lib().charm_forest_adapt(self.h, fl.ctypes.data_as(C.POINTER(C.c_int8)),
+                                       max_level, min_level, C.byref(mp), C.byref(nm)))
+        out = np.ctypeslib.as_array(mp, shape=(nm.value,)).copy()
+        C.CDLL(None).free(C.cast(mp, C.c_void_p))
+        return out
+
+    def flatten(self, chunk_elems=128, chunk_faces=512):
+        h = C.c_void_p()
+        check(lib().charm_forest_flatten(self.h, chunk_elems, chunk_faces, C.byref(h)))
+        return Mesh(h, self.n)
+
+
+class GpuContext:
+    """charm_gpu_ctx_t: one HIP stream, mesh and state on the device."""
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        self.nv = 4 + cfg.n_comp
+        self.h = C.c_void_p()
+        check(lib().charm_gpu_create(C.byref(cfg), C.byref(self.h)))
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().charm_gpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def upload_mesh(self, mesh):
+        check(lib().charm_gpu_upload_mesh(self.h, mesh.h))
+        self.n = mesh.n
+
+    def set_state(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        assert u.shape == (self.n, self.nv, 4), u.shape
+        check(lib().charm_gpu_set_state(self.h, _dptr(u), self.n))
+
+    def get_state(self):
+        out = np.zeros((self.n, self.nv, 4))
+        check(lib().charm_gpu_get_state(self.h, _dptr(out), self.n))
+        return out
+
+    def residual(self):
+        out = np.zeros((self.n, self.nv, 4))
+        check(lib().charm_gpu_residual(self.h, _dptr(out), self.n))
+        return out
+
+    def limiter_bj(self):
+        check(lib().charm_gpu_limiter_bj(self.h))
+
+    def run(self, dt, nsteps):
+        check(lib().charm_gpu_run(self.h, dt, nsteps))
+
+    def step(self, model, dt, nsteps):
+        check(lib().charm_gpu_step(self.h, MODEL[model], dt, nsteps))
+
+    def get_dt(self, mode=1):
+        dt = C.c_double()
+        check(lib().charm_gpu_get_dt(self.h, mode, C.byref(dt)))
+        return dt.value
+
+    def conservation_sums(self):
+        out = np.zeros(self.nv)
+        check(lib().charm_gpu_conservation_sums(self.h, _dptr(out)))
+        return out
+
+    def time_kernels(self, dt, reps):
+        a, b = C.c_double(), C.c_double()
+        check(lib().charm_gpu_time_kernels(self.h, dt, reps, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def last_ms(self):
+        a, b = C.c_double(), C.c_double()
+        check(lib().charm_gpu_last_timing(self.h, C.byref(a), C.byref(b)))
+        return a.value
+
+    def synchronize(self):
+        check(lib().charm_gpu_synchronize(self.h))
+
+    def flux_batch(self, kind, prim_l, prim_r, normals):
+        pl = np.ascontiguousarray(prim_l, dtype=np.float64)
+        pr = np.ascontiguousarray(prim_r, dtype=np.float64)
+        nn = np.ascontiguousarray(normals, dtype=np.float64)
+        n = len(pl)
+        out = np.zeros((n, self.nv))
+        check(lib().charm_gpu_flux_batch(self.h, FLUX[kind], n, _dptr(pl), _dptr(pr),
+                                         _dptr(nn), _dptr(out)))
+        return out
+
+
+def device_count():
+    n = C.c_int()
+    rc = lib().charm_gpu_device_count(C.byref(n))
+    return n.value if rc == 0 else 0

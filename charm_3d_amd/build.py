@@ -1,0 +1,58 @@
+"""Build libcharm3d.so in-tree: host C (gcc, C99) + gfx950 HIP kernels (hipcc).
+
+    python -m charm_3d_amd.build      # or charm_3d_amd.build.build()
+"""
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+SRC = os.path.join(PKG, "csrc")
+INC = os.path.join(ROOT, "include")
+OBJ = os.path.join(ROOT, "_build", "charm3d")
+LIB = os.path.join(PKG, "libcharm3d.so")
+
+C_SOURCES = ["charm_forest.c", "charm_model.c"]
+HIP_SOURCES = ["charm_kernels.hip", "charm_gpu.hip"]
+ARCH = os.environ.get("CHARM_OFFLOAD_ARCH", "gfx950")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+C_FLAGS = ["-O2", "-fPIC", "-std=gnu99", "-Wall", "-Wno-unused-function", "-I", INC, "-I", SRC]
+HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17",
+             # packed FP32 VALU forms are kept out of the generated code
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             # FP32 '/' and sqrt lower to the hardware approximations (FP32 bulk path)
+             "-fno-hip-fp32-correctly-rounded-divide-sqrt",
+             "-Wno-unused-result", "-I", INC, "-I", SRC]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build failed: %s\n%s%s" % (" ".join(cmd), r.stdout, r.stderr))
+    return r
+
+
+def build(verbose=False):
+    os.makedirs(OBJ, exist_ok=True)
+    jobs = []
+    for s in C_SOURCES:
+        o = os.path.join(OBJ, s + ".o")
+        jobs.append((["gcc"] + C_FLAGS + ["-c", os.path.join(SRC, s), "-o", o], o))
+    for s in HIP_SOURCES:
+        o = os.path.join(OBJ, s + ".o")
+        jobs.append(([HIPCC] + HIP_FLAGS + ["-c", os.path.join(SRC, s), "-o", o], o))
+    with ThreadPoolExecutor(max_workers=4) as ex:
+        for r in ex.map(lambda j: _run(j[0]), jobs):
+            if verbose and (r.stdout or r.stderr):
+                print(r.stdout, r.stderr, file=sys.stderr)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + [o for _, o in jobs])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,339 @@
+// charm_device.hpp -- pointwise device numerics of the DG path (gfx950).
+//
+// Templated on the arithmetic type R (float: the bulk path, double: the
+// plain FP64 path) and on the species count NC.  Each function states the
+// reference function whose arithmetic it reproduces.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace charm {
+
+constexpr int kMaxComp = 8;
+constexpr double kGasR = 8.31446261815324;       // src/charm_eos.c:8
+constexpr double kEps = 1.e-12;                   // CHARM_EPS
+enum { FLUX_LF = 0, FLUX_HLLC = 1, FLUX_GODUNOV = 2, FLUX_CD = 3 };
+enum { BND_INLET = 0, BND_OUTLET = 1, BND_WALL_SLIP = 2, BND_WALL_NO_SLIP = 3 };
+enum { EOS_IDEAL = 0, EOS_MIX = 1 };
+enum { FACE_CONF = 0, FACE_HANG0 = 1, FACE_HANG1 = 2, FACE_BND = 3 };
+
+// Physical constants shared by every kernel of a context (passed by value).
+struct DevParams {
+    int    nc;
+    int    eos;
+    double gam, cv, m;                 // single-component ideal gas
+    double comp_m[kMaxComp];           // mixture: molar masses
+    double comp_cp[kMaxComp];          //          constant cp per species
+    int    bnd_type[6];
+    double bnd_param[6][5];
+    double grav[3];
+    double cfl;
+};
+
+template <typename R> __device__ __forceinline__ R rcp(R x) { return R(1) / x; }
+template <typename R> __device__ __forceinline__ R dsqrt(R x) { return sqrt(x); }
+template <> __device__ __forceinline__ float dsqrt<float>(float x) { return __builtin_sqrtf(x); }
+
+template <typename R, int NC>
+struct Prim {
+    R r, u, v, w, e_tot, p, cz, gam, e, cv, m;
+    R c[NC];
+};
+
+// EOS closure: gamma, cv, molar mass of the state (src/charm_eos.c:98-148
+// for a constant cp per species).
+template <typename R, int NC>
+__device__ __forceinline__ void eos_coeffs(const DevParams &P, Prim<R, NC> &p)
+{
+    if (P.eos == EOS_IDEAL || NC == 1) {
+        p.gam = R(P.gam); p.cv = R(P.cv); p.m = R(P.m);
+    } else {
+        R cp = R(0), M_ = R(0);
+        #pragma unroll
+        for (int i = 0; i < NC; i++) {
+            M_ += p.c[i] / R(P.comp_m[i]);
+            cp += p.c[i] * R(P.comp_cp[i]);
+        }
+        p.m = R(1) / M_;
+        p.cv = cp - R(kGasR) / p.m;
+        p.gam = cp / p.cv;
+    }
+}
+
+// charm_param_cons_to_prim + EOS flag 4, src/charm_globals.c:183-208 and
+// src/charm_eos.c:86-91: velocities and energy use the unfloored density,
+// pressure and sound speed the floored one.
+template <typename R, int NC>
+__device__ __forceinline__ void cons_to_prim(const DevParams &P, const R *u, Prim<R, NC> &p)
+{
+    R r = R(0);
+    #pragma unroll
+    for (int i = 0; i < NC; i++) r += u[4 + i];
+    const R ri = rcp(r);
+    p.u = u[0] * ri; p.v = u[1] * ri; p.w = u[2] * ri;
+    p.e_tot = u[3] * ri;
+    p.e = p.e_tot - R(0.5) * (p.u * p.u + p.v * p.v + p.w * p.w);
+    #pragma unroll
+    for (int i = 0; i < NC; i++) {
+        R c = u[4 + i] * ri;
+        c = c < R(0) ? R(0) : c;
+        c = c > R(1) ? R(1) : c;
+        p.c[i] = c;
+    }
+    eos_coeffs<R, NC>(P, p);
+    if (r < R(kEps)) r = R(kEps);
+    p.r = r;
+    p.p = r * p.e * (p.gam - R(1));
+    p.cz = dsqrt(p.gam * p.p / r);
+}
+
+// Exterior state of a boundary face, src/charm_bnd_cond.c:21-93: the BC
+// function edits (u, v, w, T, p), then EOS flag 3 rebuilds (r, cz, e) from
+// (T, p) and e_tot is recomputed.  n = s * e_axis (axis-aligned octree).
+template <typename R, int NC>
+__device__ __forceinline__ void bnd_state(const DevParams &P, int face, int axis, R s,
+                                          const Prim<R, NC> &in, Prim<R, NC> &out)
+{
+    out = in;
+    R t = in.e / in.cv;                  // T from flag 4 (src/charm_eos.c:90)
+    R pr = in.p;
+    R vel[3] = {in.u, in.v, in.w};
+    const int type = P.bnd_type[face];
+    if (type == BND_INLET) {
+        vel[0] = R(P.bnd_param[face][0]); vel[1] = R(P.bnd_param[face][1]);
+        vel[2] = R(P.bnd_param[face][2]);
+        t = R(P.bnd_param[face][3]); pr = R(P.bnd_param[face][4]);
+    } else if (type == BND_WALL_SLIP) {
+        R vn = vel[axis] * s;
+        vel[axis] = (vel[axis] - s * vn) - s * vn;
+    } else if (type == BND_WALL_NO_SLIP) {
+        R vn = vel[axis] * s;
+        vel[axis] = vel[axis] - s * vn;
+        t = R(P.bnd_param[face][0]);
+    }
+    out.u = vel[0]; out.v = vel[1]; out.w = vel[2];
+    eos_coeffs<R, NC>(P, out);
+    if (pr < R(kEps)) pr = R(kEps);
+    out.p = pr;
+    out.r = pr * out.m / (t * R(kGasR));
+    out.cz = dsqrt(out.gam * pr / out.r);
+    out.e = pr / (out.r * (out.gam - R(1)));
+    out.e_tot = out.e + R(0.5) * (out.u * out.u + out.v * out.v + out.w * out.w);
+}
+
+// ----------------------------------------------------------------- fluxes
+// A flux evaluation works in the face frame (un, ut1, ut2): for the
+// axis-aligned normal n = s e_a the reference's tangent choice
+// (src/charm_fluxes.c:458-488) is a signed permutation, and every
+// tangential flux is linear in its tangential velocity, so evaluating with
+// the unsigned tangential components gives the reference's result exactly.
+template <typename R, int NC>
+struct Frame {
+    R r, un, ut1, ut2, e_tot, p, cz, gam;
+    R c[NC];
+};
+
+template <typename R, int NC>
+__device__ __forceinline__ Frame<R, NC> to_frame(const Prim<R, NC> &p, R n0, R n1, R n2,
+                                                 R t10, R t11, R t12, R t20, R t21, R t22)
+{
+    Frame<R, NC> f;
+    f.r = p.r; f.e_tot = p.e_tot; f.p = p.p; f.cz = p.cz; f.gam = p.gam;
+    f.un = p.u * n0 + p.v * n1 + p.w * n2;
+    f.ut1 = p.u * t10 + p.v * t11 + p.w * t12;
+    f.ut2 = p.u * t20 + p.v * t21 + p.w * t22;
+    #pragma unroll
+    for (int i = 0; i < NC; i++) f.c[i] = p.c[i];
+    return f;
+}
+
+// x-split HLLC, src/charm_fluxes.c:319-437; q = (qn, qt1, qt2, qe, qc[]).
+// Returns false where the reference aborts (:354-357).
+template <typename R, int NC>
+__device__ __forceinline__ bool hllc_x(const Frame<R, NC> &L, const Frame<R, NC> &Rt, R *q)
+{
+    R p_pvrs = R(0.5) * (L.p + Rt.p) - R(0.5) * (Rt.un - L.un) * R(0.25) * (L.r + Rt.r) * (L.cz + Rt.cz);
+    R p_star = p_pvrs > R(0) ? p_pvrs : R(0);
+    R ql = (p_star <= L.p) ? R(1) : dsqrt(R(1) + (L.gam + R(1)) * (p_star / L.p - R(1)) / (R(2) * L.gam));
+    R qr = (p_star <= Rt.p) ? R(1) : dsqrt(R(1) + (Rt.gam + R(1)) * (p_star / Rt.p - R(1)) / (R(2) * Rt.gam));
+    R sl = L.un - L.cz * ql;
+    R sr = Rt.un + Rt.cz * qr;
+    if (sl > sr) { R t = sl; sl = sr; sr = t; }
+    R s_star = Rt.p - L.p;
+    s_star += L.r * L.un * (sl - L.un);
+    s_star -= Rt.r * Rt.un * (sr - Rt.un);
+    s_star /= (L.r * (sl - L.un) - Rt.r * (sr - Rt.un));
+    if (s_star < sl) s_star = sl;
+    if (s_star > sr) s_star = sr;
+    const bool ok = (sl <= s_star) && (s_star <= sr);
+    if (sl >= R(0) || sr <= R(0)) {
+        const Frame<R, NC> &K = (sl >= R(0)) ? L : Rt;
+        q[0] = K.r * K.un * K.un + K.p;
+        q[1] = K.r * K.ut1 * K.un;
+        q[2] = K.r * K.ut2 * K.un;
+        q[3] = (K.r * K.e_tot + K.p) * K.un;
+        #pragma unroll
+        for (int i = 0; i < NC; i++) q[4 + i] = K.r * K.un * K.c[i];
+    } else {
+        const bool left = s_star >= R(0);
+        const Frame<R, NC> &K = left ? L : Rt;
+        const R sk = left ? sl : sr;
+        // F_HLLC_U / _V / _E macros, src/charm_fluxes.c:319-321
+        const R inv = R(1) / (sk - s_star);
+        const R pk = K.p + K.r * (sk - K.un) * (s_star - K.un);
+        q[0] = (s_star * (sk * (K.r * K.un) - (K.r * K.un * K.un + K.p)) + sk * pk) * inv;
+        q[1] = (s_star * (sk * (K.r * K.ut1) - K.r * K.un * K.ut1)) * inv;
+        q[2] = (s_star * (sk * (K.r * K.ut2) - K.r * K.un * K.ut2)) * inv;
+        q[3] = (s_star * (sk * (K.r * K.e_tot) - (K.r * K.e_tot + K.p) * K.un) + sk * pk * s_star) * inv;
+        #pragma unroll
+        for (int i = 0; i < NC; i++)
+            q[4 + i] = (s_star * (sk * (K.r * K.c[i]) - K.r * K.c[i] * K.un)) * inv;
+    }
+    return ok;
+}
+
+// Lax-Friedrichs in the face frame, src/charm_fluxes.c:255-315
+template <typename R, int NC>
+__device__ __forceinline__ void lf_x(const Frame<R, NC> &L, const Frame<R, NC> &Rt, R *q, bool central)
+{
+    const R alpha = central ? R(0) : fmax(fabs(L.un) + L.cz, fabs(Rt.un) + Rt.cz);
+    const R fL0 = L.r * L.un * L.un + L.p, fR0 = Rt.r * Rt.un * Rt.un + Rt.p;
+    q[0] = R(0.5) * (fR0 + fL0 - alpha * (Rt.r * Rt.un - L.r * L.un));
+    q[1] = R(0.5) * (Rt.r * Rt.un * Rt.ut1 + L.r * L.un * L.ut1 - alpha * (Rt.r * Rt.ut1 - L.r * L.ut1));
+    q[2] = R(0.5) * (Rt.r * Rt.un * Rt.ut2 + L.r * L.un * L.ut2 - alpha * (Rt.r * Rt.ut2 - L.r * L.ut2));
+    q[3] = R(0.5) * ((Rt.r * Rt.e_tot + Rt.p) * Rt.un + (L.r * L.e_tot + L.p) * L.un
+                     - alpha * (Rt.r * Rt.e_tot - L.r * L.e_tot));
+    #pragma unroll
+    for (int i = 0; i < NC; i++)
+        q[4 + i] = R(0.5) * (Rt.r * Rt.un * Rt.c[i] + L.r * L.un * L.c[i]
+                             - alpha * (Rt.r * Rt.c[i] - L.r * L.c[i]));
+}
+
+// Exact Riemann solver (Newton on p* and sampling at xi = 0 of rim_orig,
+// src/charm_fluxes.c:8-180) with one gamma per side; the flux is built as in
+// the disabled body of charm_calc_flux_godunov (:184-252).
+template <typename R, int NC>
+__device__ __forceinline__ void godunov_x(const Frame<R, NC> &Lf, const Frame<R, NC> &Rf, R *q,
+                                          int max_steps)
+{
+    const R RB = Lf.r, PB = Lf.p, UB = Lf.un, RE = Rf.r, PE = Rf.p, UE = Rf.un;
+    const R gb = Lf.gam, ge = Rf.gam;
+    const R AB = gb - R(1), AE = ge - R(1);
+    const R DB = R(2) / AB, DE = R(2) / AE;
+    const R GB = dsqrt(gb * AB), GE = dsqrt(ge * AE);
+    const R HB = AB / R(2), HE = AE / R(2);
+    const R FB = R(3) * gb - R(1), FE = R(3) * ge - R(1);
+    const R OB = AB / (R(2) * gb), OE = AE / (R(2) * ge);
+    const R QB = gb + R(1), QE = ge + R(1);
+    const R PGB = QB / (R(2) * gb), PGE = QE / (R(2) * ge);
+    const R RGB = R(4) * gb, RGE = R(4) * ge;
+    const R SB = gb * AB, SE = ge * AE;
+    const R TB = QB / R(2), TE = QE / R(2);
+    const R eps = R(1.e-5);
+    const R CB = dsqrt(gb * PB / RB), CE = dsqrt(ge * PE / RE);
+    const R EB = CB * CB / SB, EE = CE * CE / SE;
+    const R RCB = RB * CB, RCE = RE * CE;
+    const R DU = UB - UE;
+    R RF = 0, RS = 0, EF = 0, ES = 0, UF = 0, US = 0, SBL, SFL, SSL, SEL;
+    if (DU < -(R(2) * CB / AB + R(2) * CE / AE)) {
+        SBL = UB - CB; SFL = UB + R(2) * CB / AB; SSL = UE - R(2) * CE / AE; SEL = UE + CE;
+    } else {
+        R P = (PB * RCE + PE * RCB + DU * RCB * RCE) / (RCB + RCE);
+        R F1, F2, FS1, FS2;
+        int step = 0;
+        do {
+            if (P < eps) P = eps;
+            const R PPB = P / PB, PPE = P / PE;
+            if (PB <= P) {
+                const R PKB = PGB * PPB + OB, ZNB = RCB * dsqrt(PKB);
+                F1 = (P - PB) / ZNB;
+                FS1 = (QB * PPB + FB) / (RGB * ZNB * PKB);
+            } else {
+                const R ZFB = CB * exp(log(PPB) * OB);
+                F1 = DB * (ZFB - CB);
+                FS1 = ZFB / (gb * P);
+            }
+            if (PE <= P) {
+                const R PKE = PGE * PPE + OE, ZNE = RCE * dsqrt(PKE);
+                F2 = (P - PE) / ZNE;
+                FS2 = (QE * PPE + FE) / (RGE * ZNE * PKE);
+            } else {
+                const R ZFE = CE * exp(log(PPE) * OE);
+                F2 = DE * (ZFE - CE);
+                FS2 = ZFE / (ge * P);
+            }
+            P = P + (DU - F1 - F2) / (FS1 + FS2);
+        } while ((fabs(DU - F1 - F2) > eps) && (++step < max_steps));
+        const R PPB = P / PB, PPE = P / PE;
+        const R ZFB = CB * exp(log(PPB) * OB), ZFE = CE * exp(log(PPE) * OE);
+        if (PB <= P) {
+            const R D = UB - dsqrt((TB * P + HB * PB) / RB);
+            const R UBD = UB - D, RUBD = RB * UBD;
+            RF = RUBD * RUBD / (PB - P + RUBD * UBD);
+            UF = D + RUBD / RF;
+            EF = P / (AB * RF);
+            SBL = D; SFL = D;
+        } else {
+            EF = ZFB * ZFB / SB;
+            UF = UB + DB * (CB - ZFB);
+            RF = P / (AB * EF);
+            SBL = UB - CB; SFL = UF - ZFB;
+        }
+        if (PE <= P) {
+            const R D = UE + dsqrt((TE * P + HE * PE) / RE);
+            const R UED = UE - D, RUED = RE * UED;
+            RS = RUED * RUED / (PE - P + RUED * UED);
+            US = D + RUED / RS;
+            ES = P / (AE * RS);
+            SEL = D; SSL = D;
+        } else {
+            ES = ZFE * ZFE / SE;
+            US = UE - DE * (CE - ZFE);
+            RS = P / (AE * ES);
+            SSL = US + ZFE; SEL = UE + CE;
+        }
+    }
+    R RI, EI, PI, UI, VI, WI;
+    int side;
+    if (SEL <= R(0)) {
+        RI = RE; EI = EE; UI = UE; VI = Rf.ut1; WI = Rf.ut2; PI = AE * EI * RI; side = 1;
+    } else if (SBL >= R(0)) {
+        RI = RB; EI = EB; UI = UB; VI = Lf.ut1; WI = Lf.ut2; PI = AB * EI * RI; side = 0;
+    } else if ((SSL >= R(0)) && (SFL <= R(0))) {
+        if (US >= R(0)) { RI = RF; EI = EF; UI = UF; VI = Lf.ut1; WI = Lf.ut2; PI = AB * EI * RI; side = 0; }
+        else { RI = RS; EI = ES; UI = US; VI = Rf.ut1; WI = Rf.ut2; PI = AE * EI * RI; side = 1; }
+    } else if (SFL > R(0)) {
+        UI = (UB + DB * GB * dsqrt(EB)) / (R(1) + DB);
+        VI = Lf.ut1; WI = Lf.ut2;
+        EI = (UI * UI) / SB;
+        RI = RB * exp(log(EI / EB) * (R(1) / AB));
+        PI = AB * EI * RI; side = 0;
+    } else {
+        UI = (UE - DE * GE * dsqrt(EE)) / (R(1) + DE);
+        VI = Rf.ut1; WI = Rf.ut2;
+        EI = (UI * UI) / SE;
+        RI = RE * exp(log(EI / EE) * (R(1) / AE));
+        PI = AE * EI * RI; side = 1;
+    }
+    const R qr = RI * UI;
+    q[0] = qr * UI + PI;
+    q[1] = qr * VI;
+    q[2] = qr * WI;
+    q[3] = (RI * (EI + R(0.5) * (UI * UI + VI * VI + WI * WI)) + PI) * UI;
+    #pragma unroll
+    for (int i = 0; i < NC; i++) q[4 + i] = qr * (side ? Rf.c[i] : Lf.c[i]);
+}
+
+template <typename R, int NC, int FLUX>
+__device__ __forceinline__ bool flux_x(const Frame<R, NC> &L, const Frame<R, NC> &Rt, R *q)
+{
+    if (FLUX == FLUX_HLLC) return hllc_x<R, NC>(L, Rt, q);
+    if (FLUX == FLUX_GODUNOV) {
+        godunov_x<R, NC>(L, Rt, q, sizeof(R) == 8 ? 5000 : 64);
+        return true;
+    }
+    lf_x<R, NC>(L, Rt, q, FLUX == FLUX_CD);
+    return true;
+}
+
+} // namespace charm

@@ -1,0 +1,601 @@
+/*
+ * charm_forest.c -- host linear octree on the unit cube (replaces the part
+ * of p4est this path uses: p4est_new_ext uniform fill, refine/coarsen,
+ * 2:1 face balance, face neighbour queries) and the flattening of the leaves
+ * into the element / face / chunk arrays the device kernels consume.
+ *
+ * Leaves are kept as a Morton-sorted array of anchors; a point query is a
+ * binary search on the anchors' Morton keys (a leaf at level l owns the key
+ * interval [key, key + 8^(19-l))).
+ */
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "charm_internal.h"
+
+/* ------------------------------------------------------------ errors */
+static __thread char charm_err_buf[512];
+
+void charm_set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(charm_err_buf, sizeof(charm_err_buf), fmt, ap);
+    va_end(ap);
+}
+
+const char *charm_host_last_error(void) { return charm_err_buf; }
+
+/* ------------------------------------------------------------ Morton */
+static uint64_t spread3(uint32_t v)
+{
+    uint64_t x = v & 0x1fffff;
+    x = (x | x << 32) & 0x1f00000000ffffULL;
+    x = (x | x << 16) & 0x1f0000ff0000ffULL;
+    x = (x | x << 8)  & 0x100f00f00f00f00fULL;
+    x = (x | x << 4)  & 0x10c30c30c30c30c3ULL;
+    x = (x | x << 2)  & 0x1249249249249249ULL;
+    return x;
+}
+
+static uint32_t compact3(uint64_t x)
+{
+    x &= 0x1249249249249249ULL;
+    x = (x ^ (x >> 2))  & 0x10c30c30c30c30c3ULL;
+    x = (x ^ (x >> 4))  & 0x100f00f00f00f00fULL;
+    x = (x ^ (x >> 8))  & 0x1f0000ff0000ffULL;
+    x = (x ^ (x >> 16)) & 0x1f00000000ffffULL;
+    x = (x ^ (x >> 32)) & 0x1fffff;
+    return (uint32_t) x;
+}
+
+uint64_t charm_morton(int32_t x, int32_t y, int32_t z)
+{
+    return spread3((uint32_t) x) | (spread3((uint32_t) y) << 1) | (spread3((uint32_t) z) << 2);
+}
+
+static uint64_t span(int level) { return 1ULL << (3 * (CHARM_MAXLEVEL - level)); }
+
+int64_t charm_forest_find(const charm_forest_t *f, int32_t x, int32_t y, int32_t z)
+{
+    uint64_t k = charm_morton(x, y, z);
+    int64_t lo = 0, hi = f->n;            /* first index with key > k */
+    while (lo < hi) {
+        int64_t mid = lo + (hi - lo) / 2;
+        if (f->key[mid] <= k) lo = mid + 1; else hi = mid;
+    }
+    lo -= 1;
+    if (lo < 0) return -1;
+    if (k >= f->key[lo] + span(f->q[4*lo+3])) return -1;
+    return lo;
+}
+
+/* ------------------------------------------------------------ lifecycle */
+static int forest_alloc(charm_forest_t *f, int64_t n)
+{
+    f->q = (int32_t *) malloc((size_t) n * 4 * sizeof(int32_t));
+    f->key = (uint64_t *) malloc((size_t) n * sizeof(uint64_t));
+    if (!f->q || !f->key) {
+        free(f->q); free(f->key); f->q = NULL; f->key = NULL;
+        charm_set_error("forest: out of memory for %lld leaves", (long long) n);
+        return CHARM_ERR_MEM;
+    }
+    f->n = n;
+    return CHARM_OK;
+}
+
+int charm_forest_new_uniform(int level, const int periodic[3], charm_forest_t **out)
+{
+    charm_forest_t *f;
+    int64_t n, i;
+    int32_t len;
+    if (!out || level < 0 || level > 10) {
+        charm_set_error("forest_new_uniform: level %d out of range [0, 10]", level);
+        return CHARM_ERR_ARG;
+    }
+    f = (charm_forest_t *) calloc(1, sizeof(*f));
+    if (!f) { charm_set_error("forest: out of memory"); return CHARM_ERR_MEM; }
+    n = 1LL << (3 * level);
+    if (forest_alloc(f, n)) { free(f); return CHARM_ERR_MEM; }
+    len = CHARM_ROOT_LEN >> level;
+    for (i = 0; i < n; i++) {
+        uint64_t m = (uint64_t) i;
+        f->q[4*i+0] = (int32_t) compact3(m) * len;
+        f->q[4*i+1] = (int32_t) compact3(m >> 1) * len;
+        f->q[4*i+2] = (int32_t) compact3(m >> 2) * len;
+        f->q[4*i+3] = level;
+        f->key[i] = charm_morton(f->q[4*i], f->q[4*i+1], f->q[4*i+2]);
+    }
+    for (i = 0; i < 3; i++) f->periodic[i] = periodic ? (periodic[i] != 0) : 0;
+    *out = f;
+    return CHARM_OK;
+}
+
+typedef struct { uint64_t k; int64_t i; } kidx_t;
+static int cmp_kidx(const void *a, const void *b)
+{
+    uint64_t x = ((const kidx_t *) a)->k, y = ((const kidx_t *) b)->k;
+    return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+static int forest_validate(const charm_forest_t *f)
+{
+    int64_t i;
+    uint64_t expect = 0;
+    for (i = 0; i < f->n; i++) {
+        int l = f->q[4*i+3];
+        int32_t len = CHARM_ROOT_LEN >> l;
+        int a;
+        if (l < 0 || l > CHARM_MAXLEVEL) { charm_set_error("forest: leaf %lld has level %d", (long long) i, l); return CHARM_ERR_MESH; }
+        for (a = 0; a < 3; a++)
+            if (f->q[4*i+a] < 0 || f->q[4*i+a] % len || f->q[4*i+a] + len > CHARM_ROOT_LEN) {
+                charm_set_error("forest: leaf %lld is not aligned to its level", (long long) i);
+                return CHARM_ERR_MESH;
+            }
+        if (f->key[i] != expect) {
+            charm_set_error("forest: leaves do not tile the cube (gap or overlap at leaf %lld)", (long long) i);
+            return CHARM_ERR_MESH;
+        }
+        expect += span(l);
+    }
+    if (expect != span(0)) { charm_set_error("forest: leaves do not cover the cube"); return CHARM_ERR_MESH; }
+    return CHARM_OK;
+}
+
+int charm_forest_new_leaves(int64_t n, const int32_t *leaves, const int periodic[3],
+                            charm_forest_t **out)
+{
+    charm_forest_t *f;
+    kidx_t *ord;
+    int64_t i;
+    int rc;
+    if (!out || n <= 0 || !leaves) { charm_set_error("forest_new_leaves: bad arguments"); return CHARM_ERR_ARG; }
+    f = (charm_forest_t *) calloc(1, sizeof(*f));
+    ord = (kidx_t *) malloc((size_t) n * sizeof(kidx_t));
+    if (!f || !ord || forest_alloc(f, n)) { free(f); free(ord); charm_set_error("forest: out of memory"); return CHARM_ERR_MEM; }
+    for (i = 0; i < n; i++) {
+        ord[i].k = charm_morton(leaves[4*i], leaves[4*i+1], leaves[4*i+2]);
+        ord[i].i = i;
+    }
+    qsort(ord, (size_t) n, sizeof(kidx_t), cmp_kidx);
+    for (i = 0; i < n; i++) {
+        memcpy(&f->q[4*i], &leaves[4*ord[i].i], 4 * sizeof(int32_t));
+        f->key[i] = ord[i].k;
+    }
+    free(ord);
+    for (i = 0; i < 3; i++) f->periodic[i] = periodic ? (periodic[i] != 0) : 0;
+    rc = forest_validate(f);
+    if (rc) { charm_forest_destroy(f); return rc; }
+    *out = f;
+    return CHARM_OK;
+}
+
+int charm_forest_destroy(charm_forest_t *f)
+{
+    if (!f) return CHARM_OK;
+    free(f->q); free(f->key); free(f);
+    return CHARM_OK;
+}
+
+int64_t charm_forest_num_leaves(const charm_forest_t *f) { return f ? f->n : 0; }
+
+int charm_forest_get_leaves(const charm_forest_t *f, int32_t *out)
+{
+    if (!f || !out) { charm_set_error("forest_get_leaves: bad arguments"); return CHARM_ERR_ARG; }
+    memcpy(out, f->q, (size_t) f->n * 4 * sizeof(int32_t));
+    return CHARM_OK;
+}
+
+void *charm_forest_user_pointer(const charm_forest_t *f) { return f ? f->user : NULL; }
+void charm_forest_set_user_pointer(charm_forest_t *f, void *p) { if (f) f->user = p; }
+
+/* ------------------------------------------------------------ refinement */
+
+/* rebuild with per-leaf actions: +1 refine into 8 children, -1 coarsen (the
+ * caller guarantees whole families, first child carries the action), 0 keep,
+ * -2 drop (other members of a coarsened family) */
+static int forest_rebuild(charm_forest_t *f, const int8_t *act, int64_t **map_out, int64_t *n_map)
+{
+    int64_t i, n_new = 0, o = 0;
+    int32_t *q;
+    uint64_t *key;
+    int64_t *map = NULL;
+    for (i = 0; i < f->n; i++)
+        n_new += act[i] == 1 ? 8 : (act[i] == -2 ? 0 : 1);
+    q = (int32_t *) malloc((size_t) n_new * 4 * sizeof(int32_t));
+    key = (uint64_t *) malloc((size_t) n_new * sizeof(uint64_t));
+    if (map_out) map = (int64_t *) malloc((size_t) n_new * sizeof(int64_t));
+    if (!q || !key || (map_out && !map)) {
+        free(q); free(key); free(map);
+        charm_set_error("forest: out of memory");
+        return CHARM_ERR_MEM;
+    }
+    for (i = 0; i < f->n; i++) {
+        const int32_t *p = &f->q[4*i];
+        if (act[i] == 1) {
+            int32_t h = (CHARM_ROOT_LEN >> p[3]) / 2;
+            int c;
+            for (c = 0; c < 8; c++) {
+                q[4*o+0] = p[0] + (c & 1) * h;
+                q[4*o+1] = p[1] + ((c >> 1) & 1) * h;
+                q[4*o+2] = p[2] + ((c >> 2) & 1) * h;
+                q[4*o+3] = p[3] + 1;
+                key[o] = charm_morton(q[4*o], q[4*o+1], q[4*o+2]);
+                if (map) map[o] = i;
+                o++;
+            }
+        } else if (act[i] == -1) {
+            int32_t h = (CHARM_ROOT_LEN >> (p[3] - 1));
+            q[4*o+0] = (p[0] / h) * h;
+            q[4*o+1] = (p[1] / h) * h;
+            q[4*o+2] = (p[2] / h) * h;
+            q[4*o+3] = p[3] - 1;
+            key[o] = charm_morton(q[4*o], q[4*o+1], q[4*o+2]);
+            if (map) map[o] = -1 - i;
+            o++;
+        } else if (act[i] == 0) {
+            memcpy(&q[4*o], p, 4 * sizeof(int32_t));
+            key[o] = f->key[i];
+            if (map) map[o] = i;
+            o++;
+        }
+    }
+    free(f->q); free(f->key);
+    f->q = q; f->key = key; f->n = n_new;
+    if (map_out) { *map_out = map; *n_map = n_new; }
+    return CHARM_OK;
+}
+
+/* 2:1 face balance by ripple refinement: a leaf at level l forces the leaf
+ * across each of its faces to level >= l - 1 */
+static int forest_balance_map(charm_forest_t *f, int64_t **map)
+{
+    for (;;) {
+        int64_t i, nmark = 0;
+        int8_t *act = (int8_t *) calloc((size_t) f->n, 1);
+        if (!act) { charm_set_error("forest: out of memory"); return CHARM_ERR_MEM; }
+        for (i = 0; i < f->n; i++) {
+            const int32_t *p = &f->q[4*i];
+            int32_t len = CHARM_ROOT_LEN >> p[3];
+            int a, s;
+            if (p[3] < 2) continue;
+            for (a = 0; a < 3; a++)
+                for (s = 0; s < 2; s++) {
+                    int32_t x[3] = {p[0], p[1], p[2]};
+                    int64_t nb;
+                    x[a] += s ? len : -1;
+                    if (x[a] < 0 || x[a] >= CHARM_ROOT_LEN) {
+                        if (!f->periodic[a]) continue;
+                        x[a] = (x[a] + CHARM_ROOT_LEN) % CHARM_ROOT_LEN;
+                    }
+                    nb = charm_forest_find(f, x[0], x[1], x[2]);
+                    if (nb >= 0 && f->q[4*nb+3] < p[3] - 1 && !act[nb]) { act[nb] = 1; nmark++; }
+                }
+        }
+        if (nmark == 0) { free(act); return CHARM_OK; }
+        {
+            int64_t *m2 = NULL, nm2, j;
+            int rc = forest_rebuild(f, act, map ? &m2 : NULL, &nm2);
+            free(act);
+            if (rc) return rc;
+            if (map && *map) {           /* compose maps: new -> old original */
+                for (j = 0; j < nm2; j++) m2[j] = (*map)[m2[j]];
+                free(*map);
+            }
+            if (map) *map = m2;
+        }
+    }
+}
+
+int charm_forest_balance(charm_forest_t *f)
+{
+    if (!f) { charm_set_error("forest_balance: null forest"); return CHARM_ERR_ARG; }
+    return forest_balance_map(f, NULL);
+}
+
+int charm_forest_adapt(charm_forest_t *f, const int8_t *flags, int max_level, int min_level,
+                       int64_t **map, int64_t *n_map)
+{
+    int64_t i, nm = 0;
+    int8_t *act;
+    int64_t *m = NULL;
+    int rc;
+    if (!f || !flags) { charm_set_error("forest_adapt: bad arguments"); return CHARM_ERR_ARG; }
+    act = (int8_t *) calloc((size_t) f->n, 1);
+    if (!act) { charm_set_error("forest: out of memory"); return CHARM_ERR_MEM; }
+    for (i = 0; i < f->n; i++) {
+        const int32_t *p = &f->q[4*i];
+        if (act[i]) continue;
+        if (flags[i] > 0 && p[3] < max_level) { act[i] = 1; continue; }
+        if (flags[i] < 0 && p[3] > min_level && p[3] > 0) {
+            /* a family is 8 consecutive leaves of the same level whose first
+             * member is child 0 of the parent */
+            int32_t ph = CHARM_ROOT_LEN >> (p[3] - 1);
+            int c, ok = (i + 8 <= f->n) && p[0] % ph == 0 && p[1] % ph == 0 && p[2] % ph == 0;
+            for (c = 0; ok && c < 8; c++)
+                ok = f->q[4*(i+c)+3] == p[3] && flags[i+c] < 0 &&
+                     f->key[i+c] == f->key[i] + (uint64_t) c * span(p[3]);
+            if (ok) {
+                act[i] = -1;
+                for (c = 1; c < 8; c++) act[i+c] = -2;
+                i += 7;
+            }
+        }
+    }
+    rc = forest_rebuild(f, act, map ? &m : NULL, &nm);
+    free(act);
+    if (rc) return rc;
+    rc = forest_balance_map(f, map ? &m : NULL);
+    if (rc) { free(m); return rc; }
+    if (map) { *map = m; *n_map = f->n; }
+    return CHARM_OK;
+}
+
+/* ------------------------------------------------------------ flattening */
+
+static int mesh_faces(const charm_forest_t *f, charm_mesh_t *m)
+{
+    int64_t e, cap = f->n * 3 + 64, nf = 0;
+    int32_t *f4 = (int32_t *) malloc((size_t) cap * 4 * sizeof(int32_t));
+    if (!f4) goto oom;
+#define PUSH(E0, E1, INFO) do {                                              \
+        if (nf == cap) {                                                     \
+            int32_t *t_; cap *= 2;                                           \
+            t_ = (int32_t *) realloc(f4, (size_t) cap * 4 * sizeof(int32_t)); \
+            if (!t_) goto oom;                                               \
+            f4 = t_;                                                         \
+        }                                                                    \
+        f4[4*nf] = (E0); f4[4*nf+1] = (E1); f4[4*nf+2] = (INFO); f4[4*nf+3] = 0; nf++; \
+    } while (0)
+    for (e = 0; e < f->n; e++) {
+        const int32_t *p = &f->q[4*e];
+        int l = p[3], a;
+        int32_t len = CHARM_ROOT_LEN >> l;
+        for (a = 0; a < 3; a++) {
+            int t1 = (a == 0) ? 1 : 0, t2 = (a == 2) ? 1 : 2;
+            int32_t x[3] = {p[0], p[1], p[2]};
+            int64_t nb;
+            if (p[a] == 0 && !f->periodic[a])
+                PUSH((int32_t) e, -1 - 2*a, CHARM_FACE_INFO(2*a, CHARM_FACE_BND, 0));
+            x[a] += len;
+            if (x[a] == CHARM_ROOT_LEN) {
+                if (!f->periodic[a]) {
+                    PUSH((int32_t) e, -1 - (2*a+1), CHARM_FACE_INFO(2*a+1, CHARM_FACE_BND, 0));
+                    continue;
+                }
+                x[a] = 0;
+            }
+            nb = charm_forest_find(f, x[0], x[1], x[2]);
+            if (nb < 0) { charm_set_error("flatten: no leaf across face"); free(f4); return CHARM_ERR_MESH; }
+            {
+                const int32_t *r = &f->q[4*nb];
+                if (r[3] == l) {
+                    PUSH((int32_t) e, (int32_t) nb, CHARM_FACE_INFO(2*a+1, CHARM_FACE_CONF, 0));
+                } else if (r[3] == l - 1) {
+                    int sub = ((p[t1] - r[t1]) / len) | (((p[t2] - r[t2]) / len) << 1);
+                    PUSH((int32_t) e, (int32_t) nb, CHARM_FACE_INFO(2*a+1, CHARM_FACE_HANG0, sub));
+                } else if (r[3] == l + 1) {
+                    int j;
+                    for (j = 0; j < 4; j++) {
+                        int32_t y[3] = {x[0], x[1], x[2]};
+                        int64_t sm;
+                        y[t1] += (j & 1) * (len / 2);
+                        y[t2] += (j >> 1) * (len / 2);
+                        sm = charm_forest_find(f, y[0], y[1], y[2]);
+                        if (sm < 0 || f->q[4*sm+3] != l + 1) {
+                            charm_set_error("flatten: forest is not 2:1 face balanced");
+                            free(f4); return CHARM_ERR_MESH;
+                        }
+                        PUSH((int32_t) e, (int32_t) sm, CHARM_FACE_INFO(2*a+1, CHARM_FACE_HANG1, j));
+                    }
+                } else {
+                    charm_set_error("flatten: forest is not 2:1 face balanced (levels %d / %d)", l, r[3]);
+                    free(f4); return CHARM_ERR_MESH;
+                }
+            }
+        }
+    }
+#undef PUSH
+    m->nf = nf;
+    m->f4 = f4;
+    return CHARM_OK;
+oom:
+    free(f4);
+    charm_set_error("flatten: out of memory");
+    return CHARM_ERR_MEM;
+}
+
+static int mesh_neighbors(charm_mesh_t *m)
+{
+    int64_t i, naux = 0, cap = 64;
+    m->nbr = (int32_t *) malloc((size_t) m->n * 6 * sizeof(int32_t));
+    m->aux = (int32_t *) malloc((size_t) cap * 4 * sizeof(int32_t));
+    if (!m->nbr || !m->aux) { charm_set_error("flatten: out of memory"); return CHARM_ERR_MEM; }
+    for (i = 0; i < m->n * 6; i++) m->nbr[i] = INT32_MIN;
+    for (i = 0; i < m->nf; i++) {
+        int32_t e0 = m->f4[4*i], e1 = m->f4[4*i+1], info = m->f4[4*i+2];
+        int d0 = info & 7, kind = (info >> 3) & 3, sub = (info >> 5) & 3;
+        int32_t big = -1, small = -1, bdir = 0;
+        switch (kind) {
+        case CHARM_FACE_BND:  m->nbr[6*e0 + d0] = e1; break;
+        case CHARM_FACE_CONF: m->nbr[6*e0 + d0] = e1; m->nbr[6*e1 + (d0 ^ 1)] = e0; break;
+        case CHARM_FACE_HANG0: small = e0; big = e1; bdir = d0 ^ 1; m->nbr[6*e0 + d0] = e1; break;
+        case CHARM_FACE_HANG1: small = e1; big = e0; bdir = d0; m->nbr[6*e1 + (d0 ^ 1)] = e0; break;
+        }
+        if (big >= 0) {
+            int32_t *slot = &m->nbr[6*big + bdir];
+            if (*slot == INT32_MIN) {
+                if (naux == cap) {
+                    int32_t *t;
+                    cap *= 2;
+                    t = (int32_t *) realloc(m->aux, (size_t) cap * 4 * sizeof(int32_t));
+                    if (!t) { charm_set_error("flatten: out of memory"); return CHARM_ERR_MEM; }
+                    m->aux = t;
+                }
+                *slot = -CHARM_NBR_AUX_BASE - (int32_t) naux;
+                m->aux[4*naux] = m->aux[4*naux+1] = m->aux[4*naux+2] = m->aux[4*naux+3] = -1;
+                naux++;
+            }
+            m->aux[4 * (-CHARM_NBR_AUX_BASE - *slot) + sub] = small;
+        }
+    }
+    for (i = 0; i < m->n * 6; i++)
+        if (m->nbr[i] == INT32_MIN) { charm_set_error("flatten: element face %lld not covered", (long long) i); return CHARM_ERR_MESH; }
+    for (i = 0; i < naux * 4; i++)
+        if (m->aux[i] < 0) { charm_set_error("flatten: hanging face group %lld incomplete", (long long) (i / 4)); return CHARM_ERR_MESH; }
+    m->naux = naux;
+    return CHARM_OK;
+}
+
+static int mesh_chunks(charm_mesh_t *m, int ch_e, int ch_f)
+{
+    int64_t n = m->n, nf = m->nf, i, e;
+    int32_t *inc_ptr = NULL, *inc = NULL, *stamp = NULL, *loc = NULL;
+    int64_t nch = 0, ncf = 0, nslot = 0, cap_ch, cap_cf;
+    int rc = CHARM_ERR_MEM;
+
+    inc_ptr = (int32_t *) calloc((size_t) n + 1, sizeof(int32_t));
+    if (!inc_ptr) goto done;
+    for (i = 0; i < nf; i++) {
+        inc_ptr[m->f4[4*i] + 1]++;
+        if (m->f4[4*i+1] >= 0) inc_ptr[m->f4[4*i+1] + 1]++;
+    }
+    for (e = 0; e < n; e++) inc_ptr[e+1] += inc_ptr[e];
+    inc = (int32_t *) malloc((size_t) inc_ptr[n] * sizeof(int32_t));
+    stamp = (int32_t *) malloc((size_t) nf * sizeof(int32_t));
+    loc = (int32_t *) malloc((size_t) nf * sizeof(int32_t));
+    if (!inc || !stamp || !loc) goto done;
+    {
+        int32_t *fill = (int32_t *) malloc((size_t) n * sizeof(int32_t));
+        if (!fill) goto done;
+        memcpy(fill, inc_ptr, (size_t) n * sizeof(int32_t));
+        for (i = 0; i < nf; i++) {
+            inc[fill[m->f4[4*i]]++] = (int32_t) i;
+            if (m->f4[4*i+1] >= 0) inc[fill[m->f4[4*i+1]]++] = (int32_t) i;
+        }
+        free(fill);
+    }
+    for (i = 0; i < nf; i++) stamp[i] = -1;
+
+    cap_ch = n / (ch_e > 0 ? ch_e : 1) + 16;
+    cap_cf = nf * 2 + 64;
+    m->ch_ebeg = (int32_t *) malloc((size_t) (cap_ch + 1) * sizeof(int32_t));
+    m->ch_fbeg = (int32_t *) malloc((size_t) (cap_ch + 1) * sizeof(int32_t));
+    m->cf4 = (int32_t *) malloc((size_t) cap_cf * 4 * sizeof(int32_t));
+    m->slot_ptr = (int32_t *) malloc((size_t) (n + 1) * sizeof(int32_t));
+    m->slots = (uint16_t *) malloc((size_t) inc_ptr[n] * sizeof(uint16_t));
+    if (!m->ch_ebeg || !m->ch_fbeg || !m->cf4 || !m->slot_ptr || !m->slots) goto done;
+
+    {
+        int64_t ebeg = 0, cur = 0;
+        m->ch_ebeg[0] = 0; m->ch_fbeg[0] = 0;
+        for (e = 0; e < n; e++) {
+            int64_t nnew = 0, k;
+            for (k = inc_ptr[e]; k < inc_ptr[e+1]; k++) if (stamp[inc[k]] != nch) nnew++;
+            if (e > ebeg && (e - ebeg >= ch_e || cur + nnew > ch_f)) {
+                nch++;
+                if (nch >= cap_ch) {
+                    int32_t *a, *b;
+                    cap_ch *= 2;
+                    a = (int32_t *) realloc(m->ch_ebeg, (size_t) (cap_ch + 1) * sizeof(int32_t));
+                    if (!a) goto done;
+                    m->ch_ebeg = a;
+                    b = (int32_t *) realloc(m->ch_fbeg, (size_t) (cap_ch + 1) * sizeof(int32_t));
+                    if (!b) goto done;
+                    m->ch_fbeg = b;
+                }
+                m->ch_ebeg[nch] = (int32_t) e;
+                m->ch_fbeg[nch] = (int32_t) ncf;
+                ebeg = e; cur = 0;
+                nnew = inc_ptr[e+1] - inc_ptr[e];
+            }
+            if (nnew > ch_f) { charm_set_error("flatten: chunk_faces %d below one element's faces", ch_f); rc = CHARM_ERR_ARG; goto done; }
+            m->slot_ptr[e] = (int32_t) nslot;
+            for (k = inc_ptr[e]; k < inc_ptr[e+1]; k++) {
+                int32_t fi = inc[k];
+                if (stamp[fi] != nch) {
+                    stamp[fi] = (int32_t) nch;
+                    loc[fi] = (int32_t) cur;
+                    if (ncf == cap_cf) {
+                        int32_t *t;
+                        cap_cf *= 2;
+                        t = (int32_t *) realloc(m->cf4, (size_t) cap_cf * 4 * sizeof(int32_t));
+                        if (!t) goto done;
+                        m->cf4 = t;
+                    }
+                    memcpy(&m->cf4[4*ncf], &m->f4[4*fi], 4 * sizeof(int32_t));
+                    ncf++; cur++;
+                }
+                m->slots[nslot++] = (uint16_t) ((loc[fi] << 1) | (m->f4[4*fi] == e ? 0 : 1));
+            }
+        }
+        nch++;
+        m->ch_ebeg[nch] = (int32_t) n;
+        m->ch_fbeg[nch] = (int32_t) ncf;
+        m->slot_ptr[n] = (int32_t) nslot;
+    }
+    m->nch = nch;
+    m->ncf = ncf;
+    m->nslot = nslot;
+    m->chunk_elems = ch_e;
+    m->chunk_faces = ch_f;
+    rc = CHARM_OK;
+done:
+    if (rc == CHARM_ERR_MEM) charm_set_error("flatten: out of memory");
+    free(inc_ptr); free(inc); free(stamp); free(loc);
+    return rc;
+}
+
+int charm_forest_flatten(const charm_forest_t *f, int chunk_elems, int chunk_faces,
+                         charm_mesh_t **out)
+{
+    charm_mesh_t *m;
+    int64_t i;
+    int rc;
+    if (!f || !out) { charm_set_error("flatten: bad arguments"); return CHARM_ERR_ARG; }
+    if (chunk_elems <= 0) chunk_elems = 128;
+    if (chunk_faces <= 0) chunk_faces = 512;
+    if (chunk_faces > 32767 || chunk_elems > 1024) { charm_set_error("flatten: chunk sizes too large"); return CHARM_ERR_ARG; }
+    m = (charm_mesh_t *) calloc(1, sizeof(*m));
+    if (!m) { charm_set_error("flatten: out of memory"); return CHARM_ERR_MEM; }
+    m->n = f->n;
+    for (i = 0; i < 3; i++) m->periodic[i] = f->periodic[i];
+    m->level = (uint8_t *) malloc((size_t) f->n);
+    m->q = (int32_t *) malloc((size_t) f->n * 4 * sizeof(int32_t));
+    if (!m->level || !m->q) { charm_mesh_destroy(m); charm_set_error("flatten: out of memory"); return CHARM_ERR_MEM; }
+    memcpy(m->q, f->q, (size_t) f->n * 4 * sizeof(int32_t));
+    for (i = 0; i < f->n; i++) m->level[i] = (uint8_t) f->q[4*i+3];
+    if ((rc = mesh_faces(f, m)) || (rc = mesh_neighbors(m)) || (rc = mesh_chunks(m, chunk_elems, chunk_faces))) {
+        charm_mesh_destroy(m);
+        return rc;
+    }
+    *out = m;
+    return CHARM_OK;
+}
+
+int charm_mesh_destroy(charm_mesh_t *m)
+{
+    if (!m) return CHARM_OK;
+    free(m->level); free(m->q); free(m->f4); free(m->nbr); free(m->aux);
+    free(m->ch_ebeg); free(m->ch_fbeg); free(m->cf4); free(m->slot_ptr); free(m->slots);
+    free(m);
+    return CHARM_OK;
+}
+
+int64_t charm_mesh_num_faces(const charm_mesh_t *m) { return m ? m->nf : 0; }
+int64_t charm_mesh_num_chunks(const charm_mesh_t *m) { return m ? m->nch : 0; }
+
+int charm_mesh_get_faces(const charm_mesh_t *m, int32_t *out)
+{
+    if (!m || !out) { charm_set_error("mesh_get_faces: bad arguments"); return CHARM_ERR_ARG; }
+    memcpy(out, m->f4, (size_t) m->nf * 4 * sizeof(int32_t));
+    return CHARM_OK;
+}
+
+int charm_mesh_get_neighbors(const charm_mesh_t *m, int32_t *out)
+{
+    if (!m || !out) { charm_set_error("mesh_get_neighbors: bad arguments"); return CHARM_ERR_ARG; }
+    memcpy(out, m->nbr, (size_t) m->n * 6 * sizeof(int32_t));
+    return CHARM_OK;
+}

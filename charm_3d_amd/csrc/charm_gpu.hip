@@ -1,0 +1,488 @@
+// charm_gpu.hip -- device context and the charm_gpu_* C-ABI entry points.
+//
+// A context owns one HIP stream (default priority), the flattened mesh on
+// the device and three state buffers B0 (u^n, the state the caller sees),
+// B1, B2 (stage results).  Every call returns CHARM_OK or a negative code;
+// the message of a failure is kept per host thread (charm_gpu_last_error).
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "charm_gpu.h"
+#include "charm_internal.h"
+#include "charm_kernels.h"
+
+using namespace charm;
+
+extern "C" const char *charm_host_last_error(void);
+
+struct charm_gpu_ctx {
+    charm_config_t cfg;
+    int prec, nc, nv;
+    size_t esz;
+    DevParams P;
+    hipStream_t st;
+    hipEvent_t ev0, ev1;
+    float last_ms;
+    // mesh
+    int64_t n, nch, ncf, nslot, naux;
+    int max_faces;
+    uint8_t *d_level;
+    int4 *d_cf4, *d_aux;
+    int *d_ch_ebeg, *d_ch_fbeg, *d_slot_ptr, *d_nbr;
+    uint16_t *d_slots;
+    // state
+    void *B[3];
+    double *d_stage64;        // FP64 staging for host transfers
+    double *d_partial, *d_result;
+    int *d_err;
+};
+
+#define HIPCHK(x)                                                                  \
+    do {                                                                           \
+        hipError_t e_ = (x);                                                       \
+        if (e_ != hipSuccess) {                                                    \
+            charm_set_error("%s failed: %s", #x, hipGetErrorString(e_));           \
+            return CHARM_ERR_HIP;                                                  \
+        }                                                                          \
+    } while (0)
+
+extern "C" const char *charm_gpu_last_error(void) { return charm_host_last_error(); }
+
+extern "C" int charm_gpu_device_count(int *n)
+{
+    if (!n) { charm_set_error("device_count: null argument"); return CHARM_ERR_ARG; }
+    HIPCHK(hipGetDeviceCount(n));
+    return CHARM_OK;
+}
+
+static void free_mesh(charm_gpu_ctx_t *c)
+{
+    void **p[] = {(void **) &c->d_level, (void **) &c->d_cf4, (void **) &c->d_aux,
+                  (void **) &c->d_ch_ebeg, (void **) &c->d_ch_fbeg, (void **) &c->d_slot_ptr,
+                  (void **) &c->d_nbr, (void **) &c->d_slots, &c->B[0], &c->B[1], &c->B[2],
+                  (void **) &c->d_stage64, (void **) &c->d_partial};
+    for (size_t i = 0; i < sizeof(p) / sizeof(p[0]); i++)
+        if (*p[i]) { (void) hipFree(*p[i]); *p[i] = NULL; }
+    c->n = 0;
+}
+
+extern "C" int charm_gpu_create(const charm_config_t *cfg, charm_gpu_ctx_t **out)
+{
+    if (!cfg || !out) { charm_set_error("gpu_create: null argument"); return CHARM_ERR_ARG; }
+    if (cfg->n_comp < 1 || cfg->n_comp > 4) {
+        charm_set_error("gpu_create: n_comp %d not in [1, 4]", cfg->n_comp);
+        return CHARM_ERR_UNSUPPORTED;
+    }
+    if (cfg->flux < 0 || cfg->flux > 3 || (cfg->precision != CHARM_FP64 && cfg->precision != CHARM_FP32)) {
+        charm_set_error("gpu_create: bad flux %d or precision %d", cfg->flux, cfg->precision);
+        return CHARM_ERR_ARG;
+    }
+    for (int i = 0; i < cfg->n_comp; i++)
+        if (cfg->comp[i].cp_type != 0) {
+            charm_set_error("gpu_create: polynomial cp needs the FP64 EOS Newton solve, not built");
+            return CHARM_ERR_UNSUPPORTED;
+        }
+    for (int i = 0; i < 6; i++) {
+        int t = cfg->bnd_type[i];
+        if (t == CHARM_BND_MASS_FLOW) { charm_set_error("gpu_create: BOUND_MASS_FLOW not built"); return CHARM_ERR_UNSUPPORTED; }
+        if (t != CHARM_BND_PERIODIC && (t < 0 || t > 3)) { charm_set_error("gpu_create: bad boundary type %d", t); return CHARM_ERR_ARG; }
+    }
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (ndev < 1) { charm_set_error("gpu_create: no HIP device"); return CHARM_ERR_HIP; }
+    charm_gpu_ctx_t *c = (charm_gpu_ctx_t *) calloc(1, sizeof(charm_gpu_ctx_t));
+    if (!c) { charm_set_error("gpu_create: out of memory"); return CHARM_ERR_MEM; }
+    c->cfg = *cfg;
+    c->prec = cfg->precision;
+    c->nc = cfg->n_comp;
+    c->nv = 4 + cfg->n_comp;
+    c->esz = c->prec ? sizeof(float) : sizeof(double);
+    DevParams &P = c->P;
+    memset(&P, 0, sizeof(P));
+    P.nc = c->nc;
+    P.eos = cfg->eos;
+    // ideal gas from component 0 (src/charm_eos.c:113-116)
+    const double cp0 = cfg->comp[0].cp[0];
+    P.m = cfg->comp[0].m;
+    P.cv = cp0 - kGasR / P.m;
+    P.gam = cp0 / P.cv;
+    for (int i = 0; i < c->nc; i++) { P.comp_m[i] = cfg->comp[i].m; P.comp_cp[i] = cfg->comp[i].cp[0]; }
+    for (int f = 0; f < 6; f++) {
+        P.bnd_type[f] = cfg->bnd_type[f];
+        for (int j = 0; j < 5; j++) P.bnd_param[f][j] = cfg->bnd_param[f][j];
+    }
+    for (int d = 0; d < 3; d++) P.grav[d] = cfg->grav[d];
+    P.cfl = cfg->cfl;
+    hipError_t e;
+    if ((e = hipStreamCreate(&c->st)) != hipSuccess || (e = hipEventCreate(&c->ev0)) != hipSuccess ||
+        (e = hipEventCreate(&c->ev1)) != hipSuccess || (e = hipMalloc(&c->d_err, sizeof(int))) != hipSuccess ||
+        (e = hipMalloc(&c->d_result, 16 * sizeof(double))) != hipSuccess ||
+        (e = hipMemsetAsync(c->d_err, 0, sizeof(int), c->st)) != hipSuccess) {
+        charm_set_error("gpu_create: %s", hipGetErrorString(e));
+        charm_gpu_destroy(c);
+        return CHARM_ERR_HIP;
+    }
+    *out = c;
+    return CHARM_OK;
+}
+
+extern "C" int charm_gpu_destroy(charm_gpu_ctx_t *c)
+{
+    if (!c) return CHARM_OK;
+    if (c->st) (void) hipStreamSynchronize(c->st);
+    free_mesh(c);
+    if (c->d_err) (void) hipFree(c->d_err);
+    if (c->d_result) (void) hipFree(c->d_result);
+    if (c->ev0) (void) hipEventDestroy(c->ev0);
+    if (c->ev1) (void) hipEventDestroy(c->ev1);
+    if (c->st) (void) hipStreamDestroy(c->st);
+    free(c);
+    return CHARM_OK;
+}
+
+#define DMALLOC(ptr, bytes)                                                         \
+    do {                                                                            \
+        size_t b_ = (bytes) ? (bytes) : 16;                                         \
+        hipError_t e_ = hipMalloc((void **) &(ptr), b_);                            \
+        if (e_ != hipSuccess) {                                                     \
+            charm_set_error("upload_mesh: hipMalloc(%zu) failed: %s", b_, hipGetErrorString(e_)); \
+            free_mesh(c);                                                           \
+            return CHARM_ERR_MEM;                                                   \
+        }                                                                           \
+    } while (0)
+
+extern "C" int charm_gpu_upload_mesh(charm_gpu_ctx_t *c, const charm_mesh_t *m)
+{
+    if (!c || !m) { charm_set_error("upload_mesh: null argument"); return CHARM_ERR_ARG; }
+    if (m->chunk_elems < 1 || m->chunk_elems > 128) {
+        charm_set_error("upload_mesh: chunk_elems %d outside [1, 128] (one thread per element)", m->chunk_elems);
+        return CHARM_ERR_ARG;
+    }
+    const size_t smem = stage_smem_bytes(c->prec, c->nc, m->chunk_faces);
+    if (smem > 65536) {
+        charm_set_error("upload_mesh: chunk_faces %d needs %zu B of LDS per workgroup (limit 65536)",
+                        m->chunk_faces, smem);
+        return CHARM_ERR_ARG;
+    }
+    for (int a = 0; a < 3; a++) {
+        const bool per = c->cfg.bnd_type[2 * a] == CHARM_BND_PERIODIC;
+        if (per != (m->periodic[a] != 0) || per != (c->cfg.bnd_type[2 * a + 1] == CHARM_BND_PERIODIC)) {
+            charm_set_error("upload_mesh: periodic links of the forest do not match the boundary types");
+            return CHARM_ERR_ARG;
+        }
+    }
+    HIPCHK(hipStreamSynchronize(c->st));
+    free_mesh(c);
+    const size_t ssz = (size_t) m->n * 4 * c->nv * c->esz;
+    DMALLOC(c->d_level, (size_t) m->n);
+    DMALLOC(c->d_cf4, (size_t) m->ncf * sizeof(int4));
+    DMALLOC(c->d_ch_ebeg, (size_t) (m->nch + 1) * sizeof(int));
+    DMALLOC(c->d_ch_fbeg, (size_t) (m->nch + 1) * sizeof(int));
+    DMALLOC(c->d_slot_ptr, (size_t) (m->n + 1) * sizeof(int));
+    DMALLOC(c->d_slots, (size_t) m->nslot * sizeof(uint16_t));
+    DMALLOC(c->d_nbr, (size_t) m->n * 6 * sizeof(int));
+    DMALLOC(c->d_aux, (size_t) m->naux * sizeof(int4));
+    for (int i = 0; i < 3; i++) DMALLOC(c->B[i], ssz);
+    DMALLOC(c->d_stage64, (size_t) m->n * 4 * c->nv * sizeof(double));
+    DMALLOC(c->d_partial, (size_t) ((m->n + 255) / 256 + 1) * (c->nv + 1) * sizeof(double));
+    HIPCHK(hipMemcpyAsync(c->d_level, m->level, (size_t) m->n, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_cf4, m->cf4, (size_t) m->ncf * sizeof(int4), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_ch_ebeg, m->ch_ebeg, (size_t) (m->nch + 1) * sizeof(int), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_ch_fbeg, m->ch_fbeg, (size_t) (m->nch + 1) * sizeof(int), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_slot_ptr, m->slot_ptr, (size_t) (m->n + 1) * sizeof(int), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_slots, m->slots, (size_t) m->nslot * sizeof(uint16_t), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_nbr, m->nbr, (size_t) m->n * 6 * sizeof(int), hipMemcpyHostToDevice, c->st));
+    if (m->naux)
+        HIPCHK(hipMemcpyAsync(c->d_aux, m->aux, (size_t) m->naux * sizeof(int4), hipMemcpyHostToDevice, c->st));
+    for (int i = 0; i < 3; i++) HIPCHK(hipMemsetAsync(c->B[i], 0, ssz, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->n = m->n; c->nch = m->nch; c->ncf = m->ncf; c->nslot = m->nslot; c->naux = m->naux;
+    c->max_faces = m->chunk_faces;
+    return CHARM_OK;
+}
+
+static int need_mesh(const charm_gpu_ctx_t *c, int64_t n)
+{
+    if (!c) { charm_set_error("null context"); return CHARM_ERR_ARG; }
+    if (c->n == 0) { charm_set_error("no mesh uploaded"); return CHARM_ERR_ARG; }
+    if (n >= 0 && n != c->n) { charm_set_error("element count %lld != mesh %lld", (long long) n, (long long) c->n); return CHARM_ERR_ARG; }
+    return CHARM_OK;
+}
+
+extern "C" int charm_gpu_set_state(charm_gpu_ctx_t *c, const double *u, int64_t n)
+{
+    int rc = need_mesh(c, n);
+    if (rc) return rc;
+    if (!u) { charm_set_error("set_state: null state"); return CHARM_ERR_ARG; }
+    const int64_t cnt = n * 4 * c->nv;
+    HIPCHK(hipMemcpyAsync(c->d_stage64, u, (size_t) cnt * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(launch_convert(c->prec, 0, c->d_stage64, c->B[0], cnt, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return CHARM_OK;
+}
+
+static int copy_out(charm_gpu_ctx_t *c, const void *src, double *u, int64_t n)
+{
+    const int64_t cnt = n * 4 * c->nv;
+    HIPCHK(launch_convert(c->prec, 1, src, c->d_stage64, cnt, c->st));
+    HIPCHK(hipMemcpyAsync(u, c->d_stage64, (size_t) cnt * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return CHARM_OK;
+}
+
+extern "C" int charm_gpu_get_state(charm_gpu_ctx_t *c, double *u, int64_t n)
+{
+    int rc = need_mesh(c, n);
+    if (rc) return rc;
+    if (!u) { charm_set_error("get_state: null state"); return CHARM_ERR_ARG; }
+    return copy_out(c, c->B[0], u, n);
+}
+
+static StageArgs stage_args(charm_gpu_ctx_t *c, const void *U, const void *U0, void *out, int mode,
+                            int combine, double dt)
+{
+    StageArgs A;
+    A.U = U; A.U0 = U0; A.out = out;
+    A.level = c->d_level; A.cf4 = c->d_cf4; A.ch_ebeg = c->d_ch_ebeg; A.ch_fbeg = c->d_ch_fbeg;
+    A.slot_ptr = c->d_slot_ptr; A.slots = c->d_slots; A.err = c->d_err;
+    A.n_chunks = (int) c->nch; A.max_faces = c->max_faces;
+    A.mode = mode; A.combine = combine; A.dt = dt;
+    return A;
+}
+
+static LimArgs lim_args(charm_gpu_ctx_t *c, const void *W, const void *U0, void *out, int limit, int combine)
+{
+    LimArgs A;
+    A.W = W; A.U0 = U0; A.out = out; A.nbr = c->d_nbr; A.aux = c->d_aux;
+    A.n = c->n; A.limit = limit; A.combine = combine;
+    return A;
+}
+
+static int check_flag(charm_gpu_ctx_t *c)
+{
+    int flag = 0;
+    HIPCHK(hipMemcpyAsync(&flag, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (flag) {
+        HIPCHK(hipMemsetAsync(c->d_err, 0, sizeof(int), c->st));
+        charm_set_error("HLLC: inequality SL <= S* <= SR is FALSE (NaN or invalid state)");
+        return CHARM_ERR_STATE;
+    }
+    return CHARM_OK;
+}
+
+extern "C" int charm_gpu_residual(charm_gpu_ctx_t *c, double *out, int64_t n)
+{
+    int rc = need_mesh(c, n);
+    if (rc) return rc;
+    StageArgs A = stage_args(c, c->B[0], c->B[0], c->B[1], 0, 0, 0.0);
+    HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, A, c->P, c->st));
+    rc = check_flag(c);
+    if (rc) return rc;
+    return copy_out(c, c->B[1], out, n);
+}
+
+extern "C" int charm_gpu_limiter_bj(charm_gpu_ctx_t *c)
+{
+    int rc = need_mesh(c, -1);
+    if (rc) return rc;
+    LimArgs A = lim_args(c, c->B[0], c->B[0], c->B[1], 1, 0);
+    HIPCHK(launch_limiter(c->prec, c->nc, A, c->P, c->st));
+    void *t = c->B[0]; c->B[0] = c->B[1]; c->B[1] = t;
+    HIPCHK(hipStreamSynchronize(c->st));
+    return CHARM_OK;
+}
+
+// one time step of the configured model, state in and out of B[0]
+static int step_once(charm_gpu_ctx_t *c, double dt)
+{
+    const int lim = c->cfg.limiter;
+    void *B0 = c->B[0], *B1 = c->B[1], *B2 = c->B[2];
+    if (c->cfg.model == CHARM_MODEL_EULER) {
+        // forward Euler + limiter, src/models/euler/charm_model.c:519-535
+        StageArgs A = stage_args(c, B0, B0, B1, 1, 0, dt);
+        HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, A, c->P, c->st));
+        if (lim) {
+            LimArgs L = lim_args(c, B1, B0, B0, 1, 0);
+            HIPCHK(launch_limiter(c->prec, c->nc, L, c->P, c->st));
+        } else {
+            c->B[0] = B1; c->B[1] = B0;
+        }
+        return CHARM_OK;
+    }
+    // SSP-RK3 with the limiter inside each stage and the combines after it,
+    // src/models/ns/charm_model.c:244-266
+    if (lim) {
+        StageArgs A = stage_args(c, B0, B0, B1, 1, 0, dt);
+        HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, A, c->P, c->st));
+        LimArgs L = lim_args(c, B1, B0, B2, 1, 0);
+        HIPCHK(launch_limiter(c->prec, c->nc, L, c->P, c->st));
+        A = stage_args(c, B2, B0, B1, 1, 0, dt);
+        HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, A, c->P, c->st));
+        L = lim_args(c, B1, B0, B2, 1, 1);
+        HIPCHK(launch_limiter(c->prec, c->nc, L, c->P, c->st));
+        A = stage_args(c, B2, B0, B1, 1, 0, dt);
+        HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, A, c->P, c->st));
+        L = lim_args(c, B1, B0, B0, 1, 2);
+        HIPCHK(launch_limiter(c->prec, c->nc, L, c->P, c->st));
+    } else {
+        StageArgs A = stage_args(c, B0, B0, B1, 1, 0, dt);
+        HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, A, c->P, c->st));
+        A = stage_args(c, B1, B0, B2, 1, 1, dt);
+        HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, A, c->P, c->st));
+        A = stage_args(c, B2, B0, B0, 1, 2, dt);
+        HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, A, c->P, c->st));
+    }
+    return CHARM_OK;
+}
+
+extern "C" int charm_gpu_step(charm_gpu_ctx_t *c, int model, double dt, int nsteps)
+{
+    int rc = need_mesh(c, -1);
+    if (rc) return rc;
+    if (nsteps < 0 || !(dt >= 0) || (model != CHARM_MODEL_EULER && model != CHARM_MODEL_NS)) {
+        charm_set_error("step: bad model, dt or step count");
+        return CHARM_ERR_ARG;
+    }
+    const int keep = c->cfg.model;
+    c->cfg.model = model;
+    HIPCHK(hipEventRecord(c->ev0, c->st));
+    for (int s = 0; s < nsteps && !rc; s++) rc = step_once(c, dt);
+    c->cfg.model = keep;
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(c->ev1, c->st));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    HIPCHK(hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
+    return check_flag(c);
+}
+
+extern "C" int charm_gpu_run(charm_gpu_ctx_t *c, double dt, int nsteps)
+{
+    if (!c) { charm_set_error("null context"); return CHARM_ERR_ARG; }
+    return charm_gpu_step(c, c->cfg.model, dt, nsteps);
+}
+
+extern "C" int charm_gpu_recommended_chunks(const charm_config_t *cfg, int *ce, int *cf)
+{
+    if (!cfg || !ce || !cf || cfg->n_comp < 1 || cfg->n_comp > 4) {
+        charm_set_error("recommended_chunks: bad arguments");
+        return CHARM_ERR_ARG;
+    }
+    int faces = 512;
+    while (faces > 64 && stage_smem_bytes(cfg->precision, cfg->n_comp, faces) > 65536) faces /= 2;
+    *cf = faces;
+    *ce = faces >= 512 ? 128 : faces / 4;
+    return CHARM_OK;
+}
+
+extern "C" int charm_gpu_time_kernels(charm_gpu_ctx_t *c, double dt, int reps, double *stage_ms,
+                                      double *limiter_ms)
+{
+    int rc = need_mesh(c, -1);
+    if (rc) return rc;
+    if (reps < 1) { charm_set_error("time_kernels: reps < 1"); return CHARM_ERR_ARG; }
+    float ms = 0.f;
+    StageArgs A = stage_args(c, c->B[0], c->B[0], c->B[1], 1, 0, dt);
+    HIPCHK(hipEventRecord(c->ev0, c->st));
+    for (int r = 0; r < reps; r++) HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, A, c->P, c->st));
+    HIPCHK(hipEventRecord(c->ev1, c->st));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (stage_ms) *stage_ms = ms / reps;
+    LimArgs L = lim_args(c, c->B[1], c->B[0], c->B[2], 1, 1);
+    HIPCHK(hipEventRecord(c->ev0, c->st));
+    for (int r = 0; r < reps; r++) HIPCHK(launch_limiter(c->prec, c->nc, L, c->P, c->st));
+    HIPCHK(hipEventRecord(c->ev1, c->st));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (limiter_ms) *limiter_ms = ms / reps;
+    return check_flag(c);
+}
+
+extern "C" int charm_gpu_last_timing(charm_gpu_ctx_t *c, double *stage_ms, double *limiter_ms)
+{
+    if (!c) { charm_set_error("null context"); return CHARM_ERR_ARG; }
+    if (stage_ms) *stage_ms = c->last_ms;
+    if (limiter_ms) *limiter_ms = 0.0;
+    return CHARM_OK;
+}
+
+static int reduce(charm_gpu_ctx_t *c, int mode, double *host)
+{
+    RedArgs A;
+    A.U = c->B[0]; A.level = c->d_level; A.partial = c->d_partial; A.result = c->d_result;
+    A.n = c->n; A.mode = mode;
+    HIPCHK(launch_reduce(c->prec, c->nc, A, c->P, c->st));
+    HIPCHK(hipMemcpyAsync(host, c->d_result, (size_t) (c->nv + 1) * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return CHARM_OK;
+}
+
+extern "C" int charm_gpu_get_dt(charm_gpu_ctx_t *c, int mode, double *dt)
+{
+    int rc = need_mesh(c, -1);
+    if (rc) return rc;
+    if (!dt || (mode != 0 && mode != 1)) { charm_set_error("get_dt: bad arguments"); return CHARM_ERR_ARG; }
+    double r[16];
+    if ((rc = reduce(c, mode, r))) return rc;
+    *dt = r[c->nv];
+    return CHARM_OK;
+}
+
+extern "C" int charm_gpu_conservation_sums(charm_gpu_ctx_t *c, double *out)
+{
+    int rc = need_mesh(c, -1);
+    if (rc) return rc;
+    if (!out) { charm_set_error("conservation_sums: null output"); return CHARM_ERR_ARG; }
+    double r[16];
+    if ((rc = reduce(c, 1, r))) return rc;
+    memcpy(out, r, (size_t) c->nv * sizeof(double));
+    return CHARM_OK;
+}
+
+extern "C" int charm_gpu_state_flag(charm_gpu_ctx_t *c, int *flag)
+{
+    if (!c || !flag) { charm_set_error("state_flag: null argument"); return CHARM_ERR_ARG; }
+    HIPCHK(hipMemcpyAsync(flag, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemsetAsync(c->d_err, 0, sizeof(int), c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return CHARM_OK;
+}
+
+extern "C" int charm_gpu_synchronize(charm_gpu_ctx_t *c)
+{
+    if (!c) { charm_set_error("null context"); return CHARM_ERR_ARG; }
+    HIPCHK(hipStreamSynchronize(c->st));
+    return CHARM_OK;
+}
+
+extern "C" int charm_gpu_flux_batch(charm_gpu_ctx_t *c, int flux, int64_t n, const double *prim_l,
+                                    const double *prim_r, const double *normals, double *out)
+{
+    if (!c || n < 0 || (n > 0 && (!prim_l || !prim_r || !normals || !out)) || flux < 0 || flux > 3) {
+        charm_set_error("flux_batch: bad arguments");
+        return CHARM_ERR_ARG;
+    }
+    if (n == 0) return CHARM_OK;
+    const size_t np = (size_t) n * (5 + c->nc) * sizeof(double);
+    const size_t nn = (size_t) n * 3 * sizeof(double), no = (size_t) n * c->nv * sizeof(double);
+    double *dl = NULL, *dr = NULL, *dn = NULL, *dq = NULL;
+    hipError_t e = hipMalloc(&dl, np);
+    if (e == hipSuccess) e = hipMalloc(&dr, np);
+    if (e == hipSuccess) e = hipMalloc(&dn, nn);
+    if (e == hipSuccess) e = hipMalloc(&dq, no);
+    if (e == hipSuccess) e = hipMemcpyAsync(dl, prim_l, np, hipMemcpyHostToDevice, c->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dr, prim_r, np, hipMemcpyHostToDevice, c->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dn, normals, nn, hipMemcpyHostToDevice, c->st);
+    if (e == hipSuccess) {
+        FluxArgs A;
+        A.prim_l = dl; A.prim_r = dr; A.normals = dn; A.out = dq; A.n = n;
+        e = launch_flux_points(c->prec, c->nc, flux, A, c->P, c->st);
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dq, no, hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    (void) hipFree(dl); (void) hipFree(dr); (void) hipFree(dn); (void) hipFree(dq);
+    if (e != hipSuccess) { charm_set_error("flux_batch: %s", hipGetErrorString(e)); return CHARM_ERR_HIP; }
+    return CHARM_OK;
+}

@@ -1,0 +1,61 @@
+/*
+ * charm_internal.h -- private host types of libcharm3d (C99 side).
+ */
+#ifndef CHARM_INTERNAL_H
+#define CHARM_INTERNAL_H
+
+#include <stdint.h>
+#include "charm_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CHARM_MAXLEVEL 19                 /* root length 2^19 as p8est */
+#define CHARM_ROOT_LEN (1 << CHARM_MAXLEVEL)
+
+/* face info word: dir0 (3 bits) | kind << 3 (2 bits) | subface << 5 (2 bits) */
+enum { CHARM_FACE_CONF = 0, CHARM_FACE_HANG0 = 1, CHARM_FACE_HANG1 = 2, CHARM_FACE_BND = 3 };
+#define CHARM_FACE_INFO(dir, kind, sub) ((dir) | ((kind) << 3) | ((sub) << 5))
+#define CHARM_NBR_AUX_BASE 8              /* nbr <= -8: hanging group -(8 + g) */
+
+struct charm_forest {
+    int64_t   n;
+    int32_t  *q;            /* [n][4] x y z level, Morton order */
+    uint64_t *key;          /* Morton key of each anchor */
+    int       periodic[3];
+    void     *user;
+};
+
+struct charm_mesh {
+    int64_t   n;
+    uint8_t  *level;        /* [n] */
+    int32_t  *q;            /* [n][4] */
+    int       periodic[3];
+    /* all (sub)faces, each once, side 0 = lower-coordinate element */
+    int64_t   nf;
+    int32_t  *f4;           /* [nf][4] e0 e1 info 0 */
+    /* limiter neighbour table */
+    int32_t  *nbr;          /* [n][6] */
+    int64_t   naux;
+    int32_t  *aux;          /* [naux][4] */
+    /* workgroup chunks */
+    int       chunk_elems, chunk_faces;
+    int64_t   nch;
+    int32_t  *ch_ebeg;      /* [nch+1] */
+    int32_t  *ch_fbeg;      /* [nch+1] */
+    int64_t   ncf;
+    int32_t  *cf4;          /* [ncf][4] chunk-local copies of face records */
+    int32_t  *slot_ptr;     /* [n+1] */
+    int64_t   nslot;
+    uint16_t *slots;        /* local face index << 1 | side */
+};
+
+uint64_t charm_morton(int32_t x, int32_t y, int32_t z);
+int64_t  charm_forest_find(const charm_forest_t *f, int32_t x, int32_t y, int32_t z);
+void     charm_set_error(const char *fmt, ...);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,56 @@
+// charm_kernels.h -- launch interface between the context code and the
+// kernels (C++ linkage, internal to libcharm3d).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "charm_device.hpp"
+
+namespace charm {
+
+struct StageArgs {
+    const void *U;            // state the residual is evaluated on
+    const void *U0;           // stage-start state for the SSP-RK combine
+    void *out;                // residual (mode 0) or updated state (mode 1)
+    const uint8_t *level;
+    const int4 *cf4;          // chunk face records (e0, e1, info, 0)
+    const int *ch_ebeg, *ch_fbeg, *slot_ptr;
+    const uint16_t *slots;
+    int *err;
+    int n_chunks, max_faces;
+    int mode;                 // 0: write int_*; 1: c - dt A^-1 int
+    int combine;              // 0: none, 1: w/4 + 3 u0/4, 2: 2w/3 + u0/3
+    double dt;
+};
+
+struct LimArgs {
+    const void *W, *U0;
+    void *out;
+    const int *nbr;
+    const int4 *aux;
+    int64_t n;
+    int limit;                // apply Barth-Jespersen
+    int combine;              // as StageArgs::combine
+};
+
+struct RedArgs {
+    const void *U;
+    const uint8_t *level;
+    double *partial, *result;
+    int64_t n;
+    int mode;                 // dt length: 0 volume, 1 edge
+};
+
+struct FluxArgs {
+    const double *prim_l, *prim_r, *normals;
+    double *out;
+    int64_t n;
+};
+
+hipError_t launch_stage(int prec, int nc, int flux, const StageArgs &A, const DevParams &P, hipStream_t st);
+size_t stage_smem_bytes(int prec, int nc, int max_faces);
+hipError_t launch_limiter(int prec, int nc, const LimArgs &A, const DevParams &P, hipStream_t st);
+hipError_t launch_reduce(int prec, int nc, const RedArgs &A, const DevParams &P, hipStream_t st);
+hipError_t launch_flux_points(int prec, int nc, int flux, const FluxArgs &A, const DevParams &P, hipStream_t st);
+hipError_t launch_convert(int prec, int to_f64, const void *src, void *dst, int64_t n, hipStream_t st);
+
+} // namespace charm

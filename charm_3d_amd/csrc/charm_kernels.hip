@@ -1,0 +1,632 @@
+// charm_kernels.hip -- gfx950 kernels of the DG residual / time-step path.
+//
+// State layout: U[e][v][k], the 4V coefficients of an element contiguous
+// (v = ru rv rw re rc[..], k = Taylor basis 1, (x-xc)/h, (y-yc)/h, (z-zc)/h).
+//
+// One RK stage = k_stage over workgroup chunks (<= 128 Morton-consecutive
+// elements each):
+//   phase 1  one thread per (sub)face touching the chunk: both traces at the
+//            six face Gauss points, the Riemann flux, and the three flux
+//            moments per variable (sum q wJ, sum q phi_t1 wJ, sum q phi_t2 wJ in
+//            the small side's face frame) written to LDS;
+//   phase 2  one thread per element: volume integral at the 8 Gauss points,
+//            gather of its face moments from LDS (no atomics: a fixed slot
+//            list per element), A^-1 (diagonal on the cube octree) and the
+//            SSP-RK update / combine, one write of the new state.
+// Faces shared by two chunks are evaluated by both (each keeps its side).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "charm_device.hpp"
+#include "charm_kernels.h"
+
+namespace charm {
+
+// Tangential Gauss-point offsets of the six-point face rule (two triangles
+// (v0,v1,v2), (v1,v2,v3) of the face, barycentric (2/3,1/6,1/6)
+// permutations, src/charm_geom.c:443-494) relative to the face centre in
+// units of the element edge; t1 < t2 are the two tangential axes.  Literal
+// switches so that the unrolled loops fold them into immediates.
+template <typename R>
+__device__ __forceinline__ R gp_t1(int k)
+{
+    switch (k) { case 0: return R(-1.0/3); case 1: return R(1.0/6); case 2: return R(-1.0/3);
+                 case 3: return R(1.0/3); case 4: return R(-1.0/6); default: return R(1.0/3); }
+}
+template <typename R>
+__device__ __forceinline__ R gp_t2(int k)
+{
+    switch (k) { case 0: return R(-1.0/3); case 1: return R(-1.0/3); case 2: return R(1.0/6);
+                 case 3: return R(-1.0/6); case 4: return R(1.0/3); default: return R(1.0/3); }
+}
+
+// SSP-RK combines rk_1 (c = c/4 + 3 c_old/4) and rk_2 (c = 2c/3 + c_old/3)
+// in the reference's operation order, src/models/ns/charm_model.c:136-184
+template <typename R>
+__device__ __forceinline__ R rk_combine(int mode, R w, R u0)
+{
+    if (mode == 1) { R c = w * R(0.25); return c + R(0.75) * u0; }
+    R c = w * R(2);
+    c /= R(3);
+    return c + u0 / R(3);
+}
+
+template <typename R, int NV>
+__device__ __forceinline__ void load_elem(const R *__restrict__ U, int e, R *u)
+{
+    const R *p = U + (size_t) e * (4 * NV);
+    #pragma unroll
+    for (int i = 0; i < 4 * NV; i++) u[i] = p[i];
+}
+
+template <typename R, int NV>
+__device__ __forceinline__ void trace(const R *u, R px, R py, R pz, R *t)
+{
+    #pragma unroll
+    for (int v = 0; v < NV; v++)
+        t[v] = u[4*v] + px * u[4*v+1] + py * u[4*v+2] + pz * u[4*v+3];
+}
+
+__device__ __forceinline__ int tang1(int a) { return a == 0 ? 1 : 0; }
+__device__ __forceinline__ int tang2(int a) { return a == 2 ? 1 : 2; }
+
+// spread (normal, t1, t2) components onto (x, y, z)
+template <typename R>
+__device__ __forceinline__ void to_xyz(int a, R n, R t1, R t2, R &x, R &y, R &z)
+{
+    x = (a == 0) ? n : t1;
+    y = (a == 1) ? n : ((a == 0) ? t1 : t2);
+    z = (a == 2) ? n : t2;
+}
+
+// moments of one face: out[3][NV] = wJ * (sum q, sum q phi_t1, sum q phi_t2)
+template <typename R, int NC, int FLUX>
+__device__ __forceinline__ bool face_moments(const DevParams &P, const R *__restrict__ U,
+                                             const uint8_t *__restrict__ level, int4 rec,
+                                             R *out /* [3][NV] */)
+{
+    constexpr int NV = 4 + NC;
+    const int e0 = rec.x, e1 = rec.y, info = rec.z;
+    const int dir0 = info & 7, kind = (info >> 3) & 3, sub = (info >> 5) & 3;
+    const int a = dir0 >> 1;
+    const R s = (dir0 & 1) ? R(1) : R(-1);              // outward normal of side 0
+    const int lev0 = level[e0];
+    const int lown = (kind == FACE_HANG1) ? lev0 + 1 : lev0;
+    const R h = ldexp(R(1), -lown);
+    const R wj = h * h / R(6);
+    R u0[4 * NV], u1[4 * NV];
+    load_elem<R, NV>(U, e0, u0);
+    if (kind != FACE_BND) load_elem<R, NV>(U, e1, u1);
+    // normal-basis values on the face: side 0 at +-1/2 by its face parity
+    const R pa0 = R(0.5) * s, pa1 = -pa0;
+    // the big side of a hanging face sees the small face's points at
+    // delta + offset / 2 in its own units
+    const R d1 = (sub & 1) ? R(0.25) : R(-0.25);
+    const R d2 = (sub & 2) ? R(0.25) : R(-0.25);
+    const bool big0 = (kind == FACE_HANG1), big1 = (kind == FACE_HANG0);
+    R S0[NV], S1[NV], S2[NV];
+    #pragma unroll
+    for (int v = 0; v < NV; v++) { S0[v] = R(0); S1[v] = R(0); S2[v] = R(0); }
+    bool ok = true;
+    #pragma unroll
+    for (int k = 0; k < 6; k++) {
+        const R g1 = gp_t1<R>(k), g2 = gp_t2<R>(k);
+        const R a1_0 = big0 ? d1 + R(0.5) * g1 : g1, a2_0 = big0 ? d2 + R(0.5) * g2 : g2;
+        R px, py, pz;
+        to_xyz<R>(a, pa0, a1_0, a2_0, px, py, pz);
+        R t[NV];
+        trace<R, NV>(u0, px, py, pz, t);
+        Prim<R, NC> pl, pr;
+        cons_to_prim<R, NC>(P, t, pl);
+        if (kind == FACE_BND) {
+            bnd_state<R, NC>(P, dir0, a, s, pl, pr);
+        } else {
+            const R a1_1 = big1 ? d1 + R(0.5) * g1 : g1, a2_1 = big1 ? d2 + R(0.5) * g2 : g2;
+            to_xyz<R>(a, pa1, a1_1, a2_1, px, py, pz);
+            trace<R, NV>(u1, px, py, pz, t);
+            cons_to_prim<R, NC>(P, t, pr);
+        }
+        // face frame (s e_a, e_t1, e_t2)
+        const int t1 = tang1(a), t2 = tang2(a);
+        Frame<R, NC> fl, fr;
+        {
+            const R vl[3] = {pl.u, pl.v, pl.w}, vr[3] = {pr.u, pr.v, pr.w};
+            fl.r = pl.r; fl.e_tot = pl.e_tot; fl.p = pl.p; fl.cz = pl.cz; fl.gam = pl.gam;
+            fr.r = pr.r; fr.e_tot = pr.e_tot; fr.p = pr.p; fr.cz = pr.cz; fr.gam = pr.gam;
+            fl.un = s * ((a == 0) ? vl[0] : (a == 1 ? vl[1] : vl[2]));
+            fr.un = s * ((a == 0) ? vr[0] : (a == 1 ? vr[1] : vr[2]));
+            fl.ut1 = (t1 == 0) ? vl[0] : vl[1];
+            fr.ut1 = (t1 == 0) ? vr[0] : vr[1];
+            fl.ut2 = (t2 == 1) ? vl[1] : vl[2];
+            fr.ut2 = (t2 == 1) ? vr[1] : vr[2];
+            #pragma unroll
+            for (int i = 0; i < NC; i++) { fl.c[i] = pl.c[i]; fr.c[i] = pr.c[i]; }
+        }
+        R q[NV];
+        ok &= flux_x<R, NC, FLUX>(fl, fr, q);
+        R qx, qy, qz;
+        to_xyz<R>(a, s * q[0], q[1], q[2], qx, qy, qz);
+        q[0] = qx; q[1] = qy; q[2] = qz;
+        #pragma unroll
+        for (int v = 0; v < NV; v++) {
+            S0[v] += q[v];
+            S1[v] += q[v] * g1;
+            S2[v] += q[v] * g2;
+        }
+    }
+    #pragma unroll
+    for (int v = 0; v < NV; v++) {
+        out[v] = S0[v] * wj;
+        out[NV + v] = S1[v] * wj;
+        out[2 * NV + v] = S2[v] * wj;
+    }
+    return ok;
+}
+
+// volume integral, src/models/euler/charm_model.c:66-132: with the Taylor
+// basis on a cube, grad phi_k = e_k / h and phi_k(gp) = +-1/(2 sqrt 3), so
+// int[v][1+d] -= (wJ/h) sum_gp F_d,v and the gravity term adds
+// -g_v sum_gp rho phi_k wJ.
+template <typename R, int NC>
+__device__ __forceinline__ void volume_integral(const DevParams &P, const R *u, R h, R *res)
+{
+    constexpr int NV = 4 + NC;
+    const R ph = R(0.28867513459481288225);           // 1/(2 sqrt 3)
+    const R wj = h * h * h / R(8);
+    R SF[3][NV];
+    R G[4] = {R(0), R(0), R(0), R(0)};
+    #pragma unroll
+    for (int d = 0; d < 3; d++)
+        #pragma unroll
+        for (int v = 0; v < NV; v++) SF[d][v] = R(0);
+    #pragma unroll
+    for (int g = 0; g < 8; g++) {
+        const R px = (g & 1) ? ph : -ph, py = (g & 2) ? ph : -ph, pz = (g & 4) ? ph : -ph;
+        R t[NV];
+        trace<R, NV>(u, px, py, pz, t);
+        Prim<R, NC> p;
+        cons_to_prim<R, NC>(P, t, p);
+        const R m[3] = {t[0], t[1], t[2]};
+        const R vel[3] = {p.u, p.v, p.w};
+        #pragma unroll
+        for (int d = 0; d < 3; d++) {
+            SF[d][0] += m[d] * p.u + (d == 0 ? p.p : R(0));
+            SF[d][1] += m[d] * p.v + (d == 1 ? p.p : R(0));
+            SF[d][2] += m[d] * p.w + (d == 2 ? p.p : R(0));
+            SF[d][3] += m[d] * p.e_tot + p.p * vel[d];
+            #pragma unroll
+            for (int i = 0; i < NC; i++) SF[d][4 + i] += m[d] * p.c[i];
+        }
+        G[0] += p.r; G[1] += p.r * px; G[2] += p.r * py; G[3] += p.r * pz;
+    }
+    const R f = wj / h;
+    #pragma unroll
+    for (int v = 0; v < NV; v++) {
+        res[4*v+0] = R(0);
+        res[4*v+1] = -SF[0][v] * f;
+        res[4*v+2] = -SF[1][v] * f;
+        res[4*v+3] = -SF[2][v] * f;
+    }
+    #pragma unroll
+    for (int v = 0; v < 3; v++) {
+        const R gv = R(P.grav[v]);
+        #pragma unroll
+        for (int k = 0; k < 4; k++) res[4*v+k] -= gv * G[k] * wj;
+    }
+}
+
+template <typename R, int NC, int FLUX>
+__global__ __launch_bounds__(128)
+void k_stage(StageArgs A, DevParams P)
+{
+    constexpr int NV = 4 + NC;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    R *mom = reinterpret_cast<R *>(smem_raw);                       // [maxf][3][NV]
+    int *finfo = reinterpret_cast<int *>(smem_raw + (size_t) A.max_faces * 3 * NV * sizeof(R));
+    const R *__restrict__ U = reinterpret_cast<const R *>(A.U);
+    const int c = blockIdx.x;
+    const int fb = A.ch_fbeg[c], fe = A.ch_fbeg[c + 1];
+    const int eb = A.ch_ebeg[c], ee = A.ch_ebeg[c + 1];
+    bool ok = true;
+    for (int f = fb + (int) threadIdx.x; f < fe; f += blockDim.x) {
+        const int4 rec = A.cf4[f];
+        ok &= face_moments<R, NC, FLUX>(P, U, A.level, rec, mom + (size_t) (f - fb) * 3 * NV);
+        finfo[f - fb] = rec.z;
+    }
+    __syncthreads();
+    const int e = eb + (int) threadIdx.x;
+    if (e < ee) {
+        R u[4 * NV], res[4 * NV];
+        load_elem<R, NV>(U, e, u);
+        const int lev = A.level[e];
+        const R h = ldexp(R(1), -lev);
+        volume_integral<R, NC>(P, u, h, res);
+        const int s_end = A.slot_ptr[e + 1];
+        for (int si = A.slot_ptr[e]; si < s_end; si++) {
+            const int sl = A.slots[si];
+            const int fl = sl >> 1, side = sl & 1;
+            const int info = finfo[fl];
+            const int dir0 = info & 7, kind = (info >> 3) & 3, sub = (info >> 5) & 3;
+            const int a = dir0 >> 1;
+            const R sg = side ? R(-1) : R(1);
+            const R pa = ((dir0 & 1) ^ side) ? R(0.5) : R(-0.5);
+            const bool big = (kind == FACE_HANG0 && side == 1) || (kind == FACE_HANG1 && side == 0);
+            const R mu = big ? R(0.5) : R(1);
+            const R d1 = big ? ((sub & 1) ? R(0.25) : R(-0.25)) : R(0);
+            const R d2 = big ? ((sub & 2) ? R(0.25) : R(-0.25)) : R(0);
+            const R *m = mom + (size_t) fl * 3 * NV;
+            #pragma unroll
+            for (int v = 0; v < NV; v++) {
+                const R M0 = sg * m[v], M1 = sg * m[NV + v], M2 = sg * m[2 * NV + v];
+                R cx, cy, cz;
+                to_xyz<R>(a, pa * M0, mu * M1 + d1 * M0, mu * M2 + d2 * M0, cx, cy, cz);
+                res[4*v+0] += M0;
+                res[4*v+1] += cx;
+                res[4*v+2] += cy;
+                res[4*v+3] += cz;
+            }
+        }
+        R *out = reinterpret_cast<R *>(A.out) + (size_t) e * (4 * NV);
+        if (A.mode == 0) {
+            #pragma unroll
+            for (int i = 0; i < 4 * NV; i++) out[i] = res[i];
+        } else {
+            // c -= dt * A^-1 int with A^-1 = diag(1, 12, 12, 12) / h^3
+            // (src/models/euler/charm_model.c:402-433), then the SSP-RK
+            // combine ca * U0 + cb * W when no limiter sits in between
+            const R vinv = R(1) / (h * h * h);
+            const R dt = R(A.dt);
+            const R *__restrict__ U0 = reinterpret_cast<const R *>(A.U0) + (size_t) e * (4 * NV);
+            #pragma unroll
+            for (int v = 0; v < NV; v++)
+                #pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const R ai = (k == 0) ? vinv : R(12) * vinv;
+                    const R w = u[4*v+k] - dt * (ai * res[4*v+k]);
+                    out[4*v+k] = A.combine ? rk_combine<R>(A.combine, w, U0[4*v+k]) : w;
+                }
+        }
+    }
+    if (!ok) A.err[0] = 1;
+}
+
+// Barth-Jespersen limiter, src/limiters/charm_limiter_bj.c:11-332, fused
+// with the SSP-RK combine that follows it (src/models/ns/charm_model.c:
+// 136-184).  Neighbour means are coefficient 0 (the Taylor basis has zero
+// mean for k > 0); a big element's hanging face contributes the volume
+// average of its four small neighbours, a boundary face the mean pushed
+// through the boundary condition.  Only coefficient 0 of neighbours is read,
+// so the slopes of W may be limited while other threads read means.
+template <typename R, int NC>
+__global__ __launch_bounds__(256)
+void k_limiter(LimArgs A, DevParams P)
+{
+    constexpr int NV = 4 + NC;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= A.n) return;
+    const R *__restrict__ W = reinterpret_cast<const R *>(A.W);
+    R u[4 * NV];
+    load_elem<R, NV>(W, e, u);
+    R umin[NV], umax[NV];
+    #pragma unroll
+    for (int v = 0; v < NV; v++) { umin[v] = u[4*v]; umax[v] = u[4*v]; }
+    if (A.limit) {
+        for (int d = 0; d < 6; d++) {
+            const int nb = A.nbr[6 * e + d];
+            R m[NV];
+            if (nb >= 0) {
+                #pragma unroll
+                for (int v = 0; v < NV; v++) m[v] = W[(size_t) nb * 4 * NV + 4 * v];
+            } else if (nb > -8) {
+                // boundary: mean -> prim -> BC -> cons (charm_limiter_bj.c:79-91)
+                R t[NV];
+                #pragma unroll
+                for (int v = 0; v < NV; v++) t[v] = u[4*v];
+                Prim<R, NC> pi, po;
+                cons_to_prim<R, NC>(P, t, pi);
+                const int face = -1 - nb, a = face >> 1;
+                bnd_state<R, NC>(P, face, a, (face & 1) ? R(1) : R(-1), pi, po);
+                m[0] = po.r * po.u; m[1] = po.r * po.v; m[2] = po.r * po.w;
+                m[3] = po.r * (po.e + R(0.5) * (po.u * po.u + po.v * po.v + po.w * po.w));
+                #pragma unroll
+                for (int i = 0; i < NC; i++) m[4 + i] = po.r * po.c[i];
+            } else {
+                const int4 g = A.aux[-8 - nb];
+                const int q[4] = {g.x, g.y, g.z, g.w};
+                #pragma unroll
+                for (int v = 0; v < NV; v++) m[v] = R(0);
+                for (int j = 0; j < 4; j++)
+                    #pragma unroll
+                    for (int v = 0; v < NV; v++) m[v] += W[(size_t) q[j] * 4 * NV + 4 * v];
+                #pragma unroll
+                for (int v = 0; v < NV; v++) m[v] *= R(0.25);
+            }
+            #pragma unroll
+            for (int v = 0; v < NV; v++) {
+                umin[v] = m[v] < umin[v] ? m[v] : umin[v];
+                umax[v] = m[v] > umax[v] ? m[v] : umax[v];
+            }
+        }
+        #pragma unroll
+        for (int v = 0; v < NV; v++) {
+            const R u0 = u[4*v];
+            R psi = R(1);
+            #pragma unroll
+            for (int vt = 0; vt < 8; vt++) {
+                const R f = u0 + ((vt & 1) ? R(0.5) : R(-0.5)) * u[4*v+1]
+                               + ((vt & 2) ? R(0.5) : R(-0.5)) * u[4*v+2]
+                               + ((vt & 4) ? R(0.5) : R(-0.5)) * u[4*v+3];
+                const R df = f - u0;
+                R pt = R(1);
+                if (fabs(df) > R(kEps)) {
+                    if (df > R(0)) pt = fmin(R(1), (umax[v] - u0) / df);
+                    else if (df < R(0)) pt = fmin(R(1), (umin[v] - u0) / df);
+                }
+                psi = pt < psi ? pt : psi;
+            }
+            u[4*v+1] *= psi; u[4*v+2] *= psi; u[4*v+3] *= psi;
+        }
+    }
+    R *out = reinterpret_cast<R *>(A.out) + (size_t) e * (4 * NV);
+    if (A.combine) {
+        const R *__restrict__ U0 = reinterpret_cast<const R *>(A.U0) + (size_t) e * (4 * NV);
+        #pragma unroll
+        for (int i = 0; i < 4 * NV; i++) out[i] = rk_combine<R>(A.combine, u[i], U0[i]);
+    } else {
+        #pragma unroll
+        for (int i = 0; i < 4 * NV; i++) out[i] = u[i];
+    }
+}
+
+// per-element CFL step and FP64 conservation sums, reduced per block
+template <typename R, int NC>
+__global__ __launch_bounds__(256)
+void k_reduce(RedArgs A, DevParams P)
+{
+    constexpr int NV = 4 + NC;
+    __shared__ double sh[256][NV + 1];
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    double loc[NV + 1];
+    #pragma unroll
+    for (int v = 0; v <= NV; v++) loc[v] = 0.0;
+    loc[NV] = 1.e300;
+    if (e < A.n) {
+        const R *u = reinterpret_cast<const R *>(A.U) + (size_t) e * (4 * NV);
+        const double h = ldexp(1.0, -(int) A.level[e]);
+        const double vol = h * h * h;
+        #pragma unroll
+        for (int v = 0; v < NV; v++) loc[v] = vol * (double) u[4 * v];
+        R t[NV];
+        #pragma unroll
+        for (int v = 0; v < NV; v++) t[v] = u[4 * v];
+        Prim<R, NC> p;
+        cons_to_prim<R, NC>(P, t, p);
+        const double len = A.mode == 0 ? vol : h;
+        const double spd = sqrt((double) (p.u * p.u + p.v * p.v + p.w * p.w)) + (double) p.cz;
+        loc[NV] = P.cfl * len / spd;
+    }
+    #pragma unroll
+    for (int v = 0; v <= NV; v++) sh[threadIdx.x][v] = loc[v];
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int) threadIdx.x < s) {
+            #pragma unroll
+            for (int v = 0; v < NV; v++) sh[threadIdx.x][v] += sh[threadIdx.x + s][v];
+            sh[threadIdx.x][NV] = fmin(sh[threadIdx.x][NV], sh[threadIdx.x + s][NV]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        #pragma unroll
+        for (int v = 0; v <= NV; v++) A.partial[(size_t) blockIdx.x * (NV + 1) + v] = sh[0][v];
+    }
+}
+
+// final pass of k_reduce over the block partials (one block)
+__global__ __launch_bounds__(256)
+void k_reduce_final(double *partial, int nblk, int nv, double *out)
+{
+    __shared__ double sh[256];
+    for (int v = 0; v <= nv; v++) {
+        const bool isdt = (v == nv);
+        double acc = isdt ? 1.e300 : 0.0;
+        for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+            const double x = partial[(size_t) b * (nv + 1) + v];
+            acc = isdt ? fmin(acc, x) : acc + x;
+        }
+        sh[threadIdx.x] = acc;
+        __syncthreads();
+        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+            if ((int) threadIdx.x < s)
+                sh[threadIdx.x] = isdt ? fmin(sh[threadIdx.x], sh[threadIdx.x + s])
+                                       : sh[threadIdx.x] + sh[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[v] = sh[0];
+        __syncthreads();
+    }
+}
+
+// pointwise flux with an arbitrary unit normal: the reference's tangent
+// frame (src/charm_fluxes.c:458-488), the frame flux, rotation back (:507-509)
+template <typename R, int NC, int FLUX>
+__global__ void k_flux_points(FluxArgs A, DevParams P)
+{
+    constexpr int NV = 4 + NC;
+    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n) return;
+    Prim<R, NC> pp[2];
+    for (int s = 0; s < 2; s++) {
+        const double *row = (s ? A.prim_r : A.prim_l) + i * (5 + NC);
+        Prim<R, NC> &p = pp[s];
+        p.r = R(row[0]); p.u = R(row[1]); p.v = R(row[2]); p.w = R(row[3]); p.p = R(row[4]);
+        #pragma unroll
+        for (int c = 0; c < NC; c++) p.c[c] = R(row[5 + c]);
+        eos_coeffs<R, NC>(P, p);
+        p.e = p.p / (p.r * (p.gam - R(1)));
+        p.e_tot = p.e + R(0.5) * (p.u * p.u + p.v * p.v + p.w * p.w);
+        p.cz = dsqrt(p.gam * p.p / p.r);
+    }
+    const double *nn = A.normals + 3 * i;
+    double nt[3][3];
+    nt[0][0] = nn[0]; nt[0][1] = nn[1]; nt[0][2] = nn[2];
+    nt[1][0] = nt[1][1] = nt[1][2] = 0.0;
+    double ri = sqrt(nn[0] * nn[0] + nn[1] * nn[1]);
+    if (ri > kEps) { nt[1][0] = -nn[1] / ri; nt[1][1] = nn[0] / ri; }
+    else {
+        ri = sqrt(nn[1] * nn[1] + nn[2] * nn[2]);
+        if (ri > kEps) { nt[1][1] = -nn[2] / ri; nt[1][2] = nn[1] / ri; }
+        else { ri = sqrt(nn[0] * nn[0] + nn[2] * nn[2]); if (ri > kEps) { nt[1][0] = -nn[2] / ri; nt[1][2] = nn[0] / ri; } }
+    }
+    nt[2][0] = nt[0][1] * nt[1][2] - nt[0][2] * nt[1][1];
+    nt[2][1] = -nt[0][0] * nt[1][2] + nt[0][2] * nt[1][0];
+    nt[2][2] = nt[0][0] * nt[1][1] - nt[0][1] * nt[1][0];
+    Frame<R, NC> f[2];
+    for (int s = 0; s < 2; s++)
+        f[s] = to_frame<R, NC>(pp[s], R(nt[0][0]), R(nt[0][1]), R(nt[0][2]), R(nt[1][0]), R(nt[1][1]),
+                               R(nt[1][2]), R(nt[2][0]), R(nt[2][1]), R(nt[2][2]));
+    R q[NV];
+    flux_x<R, NC, FLUX>(f[0], f[1], q);
+    double *o = A.out + i * NV;
+    for (int d = 0; d < 3; d++)
+        o[d] = (double) (q[0] * R(nt[0][d]) + q[1] * R(nt[1][d]) + q[2] * R(nt[2][d]));
+    for (int v = 3; v < NV; v++) o[v] = (double) q[v];
+}
+
+// precision conversions between the FP64 host layout and the device state
+template <typename R>
+__global__ void k_from_f64(const double *src, R *dst, int64_t n)
+{
+    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (R) src[i];
+}
+template <typename R>
+__global__ void k_to_f64(const R *src, double *dst, int64_t n)
+{
+    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (double) src[i];
+}
+
+// ---------------------------------------------------------------- launchers
+template <typename R, int NC>
+static hipError_t launch_stage_nc(int flux, const StageArgs &A, const DevParams &P, hipStream_t st)
+{
+    constexpr int NV = 4 + NC;
+    const size_t sh = (size_t) A.max_faces * (3 * NV * sizeof(R) + sizeof(int));
+    const dim3 grid(A.n_chunks), block(128);
+    switch (flux) {
+    case FLUX_LF: hipLaunchKernelGGL((k_stage<R, NC, FLUX_LF>), grid, block, sh, st, A, P); break;
+    case FLUX_HLLC: hipLaunchKernelGGL((k_stage<R, NC, FLUX_HLLC>), grid, block, sh, st, A, P); break;
+    case FLUX_GODUNOV: hipLaunchKernelGGL((k_stage<R, NC, FLUX_GODUNOV>), grid, block, sh, st, A, P); break;
+    case FLUX_CD: hipLaunchKernelGGL((k_stage<R, NC, FLUX_CD>), grid, block, sh, st, A, P); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <typename R>
+static hipError_t launch_stage_r(int nc, int flux, const StageArgs &A, const DevParams &P, hipStream_t st)
+{
+    switch (nc) {
+    case 1: return launch_stage_nc<R, 1>(flux, A, P, st);
+    case 2: return launch_stage_nc<R, 2>(flux, A, P, st);
+    case 3: return launch_stage_nc<R, 3>(flux, A, P, st);
+    case 4: return launch_stage_nc<R, 4>(flux, A, P, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_stage(int prec, int nc, int flux, const StageArgs &A, const DevParams &P, hipStream_t st)
+{
+    return prec ? launch_stage_r<float>(nc, flux, A, P, st) : launch_stage_r<double>(nc, flux, A, P, st);
+}
+
+size_t stage_smem_bytes(int prec, int nc, int max_faces)
+{
+    return (size_t) max_faces * (3 * (4 + nc) * (prec ? 4 : 8) + sizeof(int));
+}
+
+template <typename R>
+static hipError_t launch_limiter_r(int nc, const LimArgs &A, const DevParams &P, hipStream_t st)
+{
+    const dim3 grid((unsigned) ((A.n + 255) / 256)), block(256);
+    switch (nc) {
+    case 1: hipLaunchKernelGGL((k_limiter<R, 1>), grid, block, 0, st, A, P); break;
+    case 2: hipLaunchKernelGGL((k_limiter<R, 2>), grid, block, 0, st, A, P); break;
+    case 3: hipLaunchKernelGGL((k_limiter<R, 3>), grid, block, 0, st, A, P); break;
+    case 4: hipLaunchKernelGGL((k_limiter<R, 4>), grid, block, 0, st, A, P); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_limiter(int prec, int nc, const LimArgs &A, const DevParams &P, hipStream_t st)
+{
+    return prec ? launch_limiter_r<float>(nc, A, P, st) : launch_limiter_r<double>(nc, A, P, st);
+}
+
+template <typename R>
+static hipError_t launch_reduce_r(int nc, const RedArgs &A, const DevParams &P, hipStream_t st)
+{
+    const dim3 grid((unsigned) ((A.n + 255) / 256)), block(256);
+    switch (nc) {
+    case 1: hipLaunchKernelGGL((k_reduce<R, 1>), grid, block, 0, st, A, P); break;
+    case 2: hipLaunchKernelGGL((k_reduce<R, 2>), grid, block, 0, st, A, P); break;
+    case 3: hipLaunchKernelGGL((k_reduce<R, 3>), grid, block, 0, st, A, P); break;
+    case 4: hipLaunchKernelGGL((k_reduce<R, 4>), grid, block, 0, st, A, P); break;
+    default: return hipErrorInvalidValue;
+    }
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, st, A.partial, (int) grid.x, 4 + nc, A.result);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce(int prec, int nc, const RedArgs &A, const DevParams &P, hipStream_t st)
+{
+    return prec ? launch_reduce_r<float>(nc, A, P, st) : launch_reduce_r<double>(nc, A, P, st);
+}
+
+template <typename R, int NC>
+static hipError_t launch_flux_nc(int flux, const FluxArgs &A, const DevParams &P, hipStream_t st)
+{
+    const dim3 grid((unsigned) ((A.n + 127) / 128)), block(128);
+    switch (flux) {
+    case FLUX_LF: hipLaunchKernelGGL((k_flux_points<R, NC, FLUX_LF>), grid, block, 0, st, A, P); break;
+    case FLUX_HLLC: hipLaunchKernelGGL((k_flux_points<R, NC, FLUX_HLLC>), grid, block, 0, st, A, P); break;
+    case FLUX_GODUNOV: hipLaunchKernelGGL((k_flux_points<R, NC, FLUX_GODUNOV>), grid, block, 0, st, A, P); break;
+    case FLUX_CD: hipLaunchKernelGGL((k_flux_points<R, NC, FLUX_CD>), grid, block, 0, st, A, P); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <typename R>
+static hipError_t launch_flux_r(int nc, int flux, const FluxArgs &A, const DevParams &P, hipStream_t st)
+{
+    switch (nc) {
+    case 1: return launch_flux_nc<R, 1>(flux, A, P, st);
+    case 2: return launch_flux_nc<R, 2>(flux, A, P, st);
+    case 3: return launch_flux_nc<R, 3>(flux, A, P, st);
+    case 4: return launch_flux_nc<R, 4>(flux, A, P, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_flux_points(int prec, int nc, int flux, const FluxArgs &A, const DevParams &P, hipStream_t st)
+{
+    return prec ? launch_flux_r<float>(nc, flux, A, P, st) : launch_flux_r<double>(nc, flux, A, P, st);
+}
+
+hipError_t launch_convert(int prec, int to_f64, const void *src, void *dst, int64_t n, hipStream_t st)
+{
+    const dim3 grid((unsigned) ((n + 255) / 256)), block(256);
+    if (n <= 0) return hipSuccess;
+    if (to_f64) {
+        if (prec) hipLaunchKernelGGL(k_to_f64<float>, grid, block, 0, st, (const float *) src, (double *) dst, n);
+        else hipLaunchKernelGGL(k_to_f64<double>, grid, block, 0, st, (const double *) src, (double *) dst, n);
+    } else {
+        if (prec) hipLaunchKernelGGL(k_from_f64<float>, grid, block, 0, st, (const double *) src, (float *) dst, n);
+        else hipLaunchKernelGGL(k_from_f64<double>, grid, block, 0, st, (const double *) src, (double *) dst, n);
+    }
+    return hipGetLastError();
+}
+
+} // namespace charm

@@ -1,0 +1,197 @@
+/*
+ * charm_model.c -- the reference's plugin entry points over the device
+ * context: charm_model_{euler,ns}_timestep_single, charm_model_*_get_dt,
+ * charm_timesteps, charm_limiter[_bj], charm_calc_flux_*.
+ *
+ * The forest's user pointer is a charm_ctx_t, as p4est->user_pointer is the
+ * reference's charm_ctx_t (src/charm_3d.c:36-37).  Single rank: the ghost
+ * arguments are accepted and passed through untouched.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "charm_internal.h"
+
+struct charm_ctx {
+    charm_config_t   cfg;
+    charm_forest_t  *forest;
+    charm_gpu_ctx_t *gpu;
+    double           t;          /* ctx->t */
+    int              timestep;   /* ctx->timestep */
+    int              err;
+};
+
+int charm_ctx_create(charm_forest_t *f, const charm_config_t *cfg, charm_ctx_t **out)
+{
+    charm_ctx_t *c;
+    charm_mesh_t *m = NULL;
+    int ce = cfg ? cfg->chunk_elems : 0, cf = cfg ? cfg->chunk_faces : 0, rc;
+    if (!f || !cfg || !out) { charm_set_error("ctx_create: null argument"); return CHARM_ERR_ARG; }
+    c = (charm_ctx_t *) calloc(1, sizeof(*c));
+    if (!c) { charm_set_error("ctx_create: out of memory"); return CHARM_ERR_MEM; }
+    c->cfg = *cfg;
+    c->forest = f;
+    if (ce <= 0 || cf <= 0) {
+        if ((rc = charm_gpu_recommended_chunks(cfg, &ce, &cf))) { free(c); return rc; }
+    }
+    if ((rc = charm_gpu_create(cfg, &c->gpu)) ||
+        (rc = charm_forest_flatten(f, ce, cf, &m)) ||
+        (rc = charm_gpu_upload_mesh(c->gpu, m))) {
+        charm_mesh_destroy(m);
+        charm_gpu_destroy(c->gpu);
+        free(c);
+        return rc;
+    }
+    charm_mesh_destroy(m);
+    charm_forest_set_user_pointer(f, c);
+    *out = c;
+    return CHARM_OK;
+}
+
+int charm_ctx_destroy(charm_ctx_t *c)
+{
+    if (!c) return CHARM_OK;
+    if (c->forest && charm_forest_user_pointer(c->forest) == c)
+        charm_forest_set_user_pointer(c->forest, NULL);
+    charm_gpu_destroy(c->gpu);
+    free(c);
+    return CHARM_OK;
+}
+
+charm_gpu_ctx_t *charm_ctx_gpu(charm_ctx_t *c) { return c ? c->gpu : NULL; }
+
+static charm_ctx_t *ctx_of(charm_forest_t *p4est)
+{
+    charm_ctx_t *c = (charm_ctx_t *) charm_forest_user_pointer(p4est);
+    if (!c) charm_set_error("forest has no charm_ctx_t user pointer");
+    return c;
+}
+
+/* get_dt returns TAU: the CFL branch after the early return is disabled in
+ * the reference (src/models/euler/charm_model.c:37-54,
+ * src/models/ns/charm_model.c:43-60); charm_gpu_get_dt exposes it */
+charm_real_t charm_model_euler_get_dt(charm_forest_t *p4est)
+{
+    charm_ctx_t *c = ctx_of(p4est);
+    return c ? c->cfg.dt : 0.;
+}
+
+charm_real_t charm_model_ns_get_dt(charm_forest_t *p4est)
+{
+    charm_ctx_t *c = ctx_of(p4est);
+    return c ? c->cfg.dt : 0.;
+}
+
+static void timestep(charm_forest_t *p4est, int model, charm_real_t *dt)
+{
+    charm_ctx_t *c = ctx_of(p4est);
+    int rc;
+    if (!c || !dt) return;
+    *dt = model == CHARM_MODEL_EULER ? charm_model_euler_get_dt(p4est) : charm_model_ns_get_dt(p4est);
+    rc = charm_gpu_step(c->gpu, model, *dt, 1);
+    if (rc) c->err = rc;
+}
+
+/* src/models/euler/charm_model.c:454-540 (adapt / partition / write hooks
+ * are outside this path) */
+void charm_model_euler_timestep_single(charm_forest_t *p4est, charm_real_t *dt,
+                                       charm_ghost_t **ghost, charm_data_t **ghost_data)
+{
+    (void) ghost; (void) ghost_data;
+    timestep(p4est, CHARM_MODEL_EULER, dt);
+}
+
+/* src/models/ns/charm_model.c:187-273 */
+void charm_model_ns_timestep_single(charm_forest_t *p4est, charm_real_t *dt,
+                                    charm_ghost_t **ghost, charm_data_t **ghost_data)
+{
+    (void) ghost; (void) ghost_data;
+    timestep(p4est, CHARM_MODEL_NS, dt);
+}
+
+/* src/charm_timesteps.c:23-51: the loop advances t by dt until t >= TMAX
+ * and logs every LOG_OUTPUT_STEP steps */
+void charm_timesteps(charm_forest_t *p4est)
+{
+    charm_ctx_t *c = ctx_of(p4est);
+    charm_real_t dt;
+    if (!c) return;
+    dt = c->cfg.model == CHARM_MODEL_EULER ? charm_model_euler_get_dt(p4est) : charm_model_ns_get_dt(p4est);
+    for (c->t = 0., c->timestep = 0; c->t < c->cfg.tmax; c->t += dt, c->timestep++) {
+        if (c->cfg.model == CHARM_MODEL_EULER) charm_model_euler_timestep_single(p4est, &dt, NULL, NULL);
+        else charm_model_ns_timestep_single(p4est, &dt, NULL, NULL);
+        if (c->err) return;
+        if (c->cfg.log_period > 0 && c->timestep % c->cfg.log_period == 0) {
+            double ms = 0.;
+            charm_gpu_last_timing(c->gpu, &ms, NULL);
+            printf("STEP = %8d\tTIME = %16.8e\tTAU = %16.8e\tELAPSED_TIME = %16.8e\n",
+                   c->timestep, c->t, dt, ms * 1e-3);
+        }
+    }
+}
+
+/* src/limiters/charm_limiter.c:9-16 */
+void charm_limiter(charm_forest_t *p4est, charm_ghost_t *ghost, charm_data_t *ghost_data)
+{
+    charm_ctx_t *c = ctx_of(p4est);
+    if (c && c->cfg.limiter) charm_limiter_bj(p4est, ghost, ghost_data);
+}
+
+void charm_limiter_bj(charm_forest_t *p4est, charm_ghost_t *ghost, charm_data_t *ghost_data)
+{
+    charm_ctx_t *c = ctx_of(p4est);
+    int rc;
+    (void) ghost; (void) ghost_data;
+    if (!c) return;
+    rc = charm_gpu_limiter_bj(c->gpu);
+    if (rc) c->err = rc;
+}
+
+/* pointwise fluxes through the device flux kernel (one point) with the
+ * reference's argument roles (include/charm_fluxes.h:19-32) */
+static void flux_point(charm_forest_t *p4est, int kind, charm_prim_t prim[2], charm_real_t *qu,
+                       charm_real_t *qv, charm_real_t *qw, charm_real_t *qe, charm_real_t qc[],
+                       charm_real_t n[3])
+{
+    charm_ctx_t *c = ctx_of(p4est);
+    double row[2][5 + CHARM_MAX_COMP], q[4 + CHARM_MAX_COMP];
+    int s, i, nc;
+    if (!c) return;
+    nc = c->cfg.n_comp;
+    for (s = 0; s < 2; s++) {
+        row[s][0] = prim[s].r; row[s][1] = prim[s].u; row[s][2] = prim[s].v;
+        row[s][3] = prim[s].w; row[s][4] = prim[s].p;
+        for (i = 0; i < nc; i++) row[s][5 + i] = prim[s].c[i];
+    }
+    if (charm_gpu_flux_batch(c->gpu, kind, 1, row[0], row[1], n, q)) { c->err = CHARM_ERR_HIP; return; }
+    *qu = q[0]; *qv = q[1]; *qw = q[2]; *qe = q[3];
+    for (i = 0; i < nc; i++) qc[i] = q[4 + i];
+}
+
+void charm_calc_flux_lf(charm_forest_t *p4est, charm_prim_t prim[2], charm_real_t *qu,
+                        charm_real_t *qv, charm_real_t *qw, charm_real_t *qe,
+                        charm_real_t qc[], charm_real_t n[3])
+{
+    flux_point(p4est, CHARM_FLUX_LF, prim, qu, qv, qw, qe, qc, n);
+}
+
+void charm_calc_flux_hllc(charm_forest_t *p4est, charm_prim_t prim[2], charm_real_t *qu,
+                          charm_real_t *qv, charm_real_t *qw, charm_real_t *qe,
+                          charm_real_t qc[], charm_real_t n[3])
+{
+    flux_point(p4est, CHARM_FLUX_HLLC, prim, qu, qv, qw, qe, qc, n);
+}
+
+void charm_calc_flux_godunov(charm_forest_t *p4est, charm_prim_t prim[2], charm_real_t *qu,
+                             charm_real_t *qv, charm_real_t *qw, charm_real_t *qe,
+                             charm_real_t qc[], charm_real_t n[3])
+{
+    flux_point(p4est, CHARM_FLUX_GODUNOV, prim, qu, qv, qw, qe, qc, n);
+}
+
+void charm_calc_flux_cd(charm_forest_t *p4est, charm_prim_t prim[2], charm_real_t *qu,
+                        charm_real_t *qv, charm_real_t *qw, charm_real_t *qe,
+                        charm_real_t qc[], charm_real_t n[3])
+{
+    flux_point(p4est, CHARM_FLUX_CD, prim, qu, qv, qw, qe, qc, n);
+}

@@ -1,0 +1,95 @@
+"""Synthetic workloads of the benchmark configurations (initial states, BCs).
+
+Initial conditions are FP64 L2 projections onto the P1 Taylor basis with the
+8-point Gauss rule (the projection template of charm_replace_quads,
+src/charm_amr.c:279-304); the reference only sets coefficient 0 from
+piecewise-constant regions (src/charm_init.c:46-68).
+"""
+import numpy as np
+
+from . import GAS_R, ROOT_LEN, Forest, make_config
+
+GAMMA = 1.4
+S3 = 1.0 / np.sqrt(3.0)
+QGP = np.array([[sx, sy, sz] for sz in (-S3, S3) for sy in (-S3, S3) for sx in (-S3, S3)])
+
+
+def centers(leaves):
+    lv = np.asarray(leaves, dtype=np.float64)
+    h = (ROOT_LEN >> lv[:, 3].astype(np.int64)) / ROOT_LEN
+    return lv[:, :3] / ROOT_LEN + 0.5 * h[:, None], h
+
+
+def project_p1(leaves, fn, nv):
+    """u[e][v][k] = A^-1 sum_gp wJ phi_k f(x_gp) with A = V diag(1, 1/12, 1/12, 1/12)."""
+    c, h = centers(leaves)
+    n = len(c)
+    acc = np.zeros((n, nv, 4))
+    for g in QGP:
+        x = c + 0.5 * h[:, None] * g[None, :]
+        f = fn(x)
+        acc[:, :, 0] += f
+        acc[:, :, 1:] += f[:, :, None] * (0.5 * g)[None, None, :]
+    u = np.empty_like(acc)
+    u[:, :, 0] = acc[:, :, 0] / 8.0
+    u[:, :, 1:] = acc[:, :, 1:] * (12.0 / 8.0)
+    return u
+
+
+def prim_to_cons(rho, vel, p, gamma=GAMMA):
+    out = np.zeros(rho.shape + (5,))
+    out[..., 0:3] = rho[..., None] * vel
+    out[..., 3] = p / (gamma - 1) + 0.5 * rho * (vel ** 2).sum(-1)
+    out[..., 4] = rho
+    return out
+
+
+def c1_exact(x, t=0.0):
+    """rho = 1 + 0.2 sin(2 pi (x + y + z - 1.75 t)), u = (1, 0.5, 0.25), p = 1."""
+    rho = 1 + 0.2 * np.sin(2 * np.pi * (x[:, 0] + x[:, 1] + x[:, 2] - 1.75 * t))
+    vel = np.broadcast_to(np.array([1.0, 0.5, 0.25]), x.shape)
+    return prim_to_cons(rho, vel, np.ones_like(rho))
+
+
+def c1(level=4, precision=1):
+    """Euler ideal gas, periodic unit cube, LF, no limiter, SSP-RK3."""
+    leaves = Forest.uniform(level).leaves()
+    cfg = make_config(precision=precision, model="NS", flux="LF", limiter=False,
+                      bnd=("PERIODIC",) * 6, comp_m=(GAS_R,), comp_cp=(3.5,), cfl=0.3)
+    return cfg, leaves, project_p1(leaves, c1_exact, 5), (True, True, True)
+
+
+def c2(level=7, precision=1, seed=2):
+    """Riemann box: (rho, p) = (1, 1) | (0.125, 0.1) at x = 0.5, u = 0, cell means
+    of rho and p scaled by 1 + 0.01 U(-1, 1) (seeded), P1 slopes 0,
+    transmissive boundaries, HLLC + Barth-Jespersen, SSP-RK3."""
+    leaves = Forest.uniform(level).leaves()
+    c, _ = centers(leaves)
+    rng = np.random.default_rng(seed)
+    left = c[:, 0] < 0.5
+    rho = np.where(left, 1.0, 0.125) * (1 + 0.01 * rng.uniform(-1, 1, len(c)))
+    p = np.where(left, 1.0, 0.1) * (1 + 0.01 * rng.uniform(-1, 1, len(c)))
+    u = np.zeros((len(c), 5, 4))
+    u[:, :, 0] = prim_to_cons(rho, np.zeros((len(c), 3)), p)
+    cfg = make_config(precision=precision, model="NS", flux="HLLC", limiter=True,
+                      bnd=("OUTLET",) * 6, comp_m=(GAS_R,), comp_cp=(3.5,), cfl=0.3)
+    return cfg, leaves, u, (False, False, False)
+
+
+def c5_pulse(level=6, precision=1):
+    """Gaussian density pulse rho = 1 + 0.1 exp(-|x - x0|^2 / 0.1^2), x0 = 0.3."""
+    leaves = Forest.uniform(level).leaves()
+
+    def fn(x):
+        r2 = ((x - 0.3) ** 2).sum(1)
+        rho = 1 + 0.1 * np.exp(-r2 / 0.01)
+        return prim_to_cons(rho, np.zeros((len(x), 3)), np.ones(len(x)))
+
+    cfg = make_config(precision=precision, model="NS", flux="HLLC", limiter=False,
+                      bnd=("OUTLET",) * 6, comp_m=(GAS_R,), comp_cp=(3.5,), cfl=0.3)
+    return cfg, leaves, project_p1(leaves, fn, 5), (False, False, False)
+
+
+def dof_updates(n_elem, nv, stages, steps):
+    """one DOF advanced through one RK stage: E * 4 * V * stages * steps"""
+    return n_elem * 4 * nv * stages * steps

@@ -1,0 +1,220 @@
+/*
+ * charm_gpu.h -- C-ABI of the MI355X-native CHARM_3D DG residual and
+ * time-step path (libcharm3d.so).
+ *
+ * Three layers, all plain C types (no torch, no C++ in the signatures):
+ *
+ *  1. charm_forest_*  the host linear octree that replaces p4est for this
+ *                     path (uniform fill, 2:1 face balance, flattening);
+ *  2. charm_gpu_*     device context: mesh upload after every adapt/balance,
+ *                     state transfer, per-operator launches, diagnostics;
+ *  3. the reference's own entry points (charm_model_*_timestep_single,
+ *     charm_model_*_get_dt, charm_timesteps, charm_limiter[_bj],
+ *     charm_calc_flux_*) with p4est_t* replaced by charm_forest_t*.
+ *
+ * Every charm_forest_* / charm_gpu_* function returns CHARM_OK (0) or a
+ * negative error code and never aborts; charm_gpu_last_error() returns the
+ * message of the last failure on the calling thread.  The reference-named
+ * functions keep the reference's void/real return types; on failure they
+ * record the error (charm_gpu_last_error) instead of calling charm_abort
+ * (reference: src/charm_globals.c:459-470).
+ *
+ * Threading: one host thread per context; every context owns one HIP stream
+ * (default priority).  All device work of a context is ordered on it.
+ */
+#ifndef CHARM_GPU_H
+#define CHARM_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef double charm_real_t;       /* include/charm_globals.h:16 */
+
+enum {
+    CHARM_OK = 0,
+    CHARM_ERR_ARG = -1,
+    CHARM_ERR_HIP = -2,
+    CHARM_ERR_MEM = -3,
+    CHARM_ERR_MESH = -4,
+    CHARM_ERR_STATE = -5,      /* NaN / non-positive pressure detected */
+    CHARM_ERR_UNSUPPORTED = -6
+};
+
+/* precision of the bulk arithmetic */
+enum { CHARM_FP64 = 0, CHARM_FP32 = 1 };
+/* FLUX_TYPE (src/charm_init_yaml.cpp:352-379) */
+enum { CHARM_FLUX_LF = 0, CHARM_FLUX_HLLC = 1, CHARM_FLUX_GODUNOV = 2, CHARM_FLUX_CD = 3 };
+/* boundary types (include/charm_globals.h:432-439) plus periodic links */
+enum { CHARM_BND_INLET = 0, CHARM_BND_OUTLET = 1, CHARM_BND_WALL_SLIP = 2,
+       CHARM_BND_WALL_NO_SLIP = 3, CHARM_BND_MASS_FLOW = 4, CHARM_BND_PERIODIC = 100 };
+/* EOS (src/charm_eos.c:98-154) */
+enum { CHARM_EOS_IDEAL = 0, CHARM_EOS_MIX = 1 };
+/* MODEL name (src/charm_init_yaml.cpp:394-413) */
+enum { CHARM_MODEL_EULER = 0, CHARM_MODEL_NS = 1 };
+
+#define CHARM_MAX_COMP 8
+
+/* one species, the fields of charm_comp_t used by this path
+ * (include/charm_globals.h:40-56) */
+typedef struct {
+    double m;                  /* molar mass */
+    int    cp_type;            /* 0 const, 1 polynomial */
+    int    ml_type;            /* 0 const, 1 Sutherland */
+    int    n_cp;
+    double cp[8];              /* cp = sum cp[i] T^i */
+    double ml0, kp0, t0, ts, h0;
+} charm_comp_desc_t;
+
+/* the task.yaml keys this path consumes (src/charm_init_yaml.cpp:333-426) */
+typedef struct {
+    int    precision;          /* CHARM_FP64 / CHARM_FP32 */
+    int    model;              /* CHARM_MODEL_* */
+    int    flux;               /* CHARM_FLUX_* */
+    int    limiter;            /* 0 NONE, 1 BJ */
+    int    eos;                /* CHARM_EOS_* */
+    int    n_comp;
+    charm_comp_desc_t comp[CHARM_MAX_COMP];
+    int    bnd_type[6];        /* per cube face -x,+x,-y,+y,-z,+z */
+    double bnd_param[6][16];
+    double grav[3];
+    double cfl;                /* CFL */
+    double dt;                 /* TAU */
+    double tmax;               /* TMAX */
+    int    log_period;         /* LOG_OUTPUT_STEP (0: no log) */
+    int    chunk_elems;        /* elements per workgroup chunk (0: default) */
+    int    chunk_faces;        /* max faces per chunk (0: default) */
+} charm_config_t;
+
+/* ------------------------------------------------------------ forest */
+typedef struct charm_forest charm_forest_t;   /* replaces p4est_t */
+typedef struct charm_mesh   charm_mesh_t;     /* flattened leaves + faces */
+
+/* uniform fill at `level` of the unit cube (p4est_new_ext(..., min_level,
+ * fill_uniform = 1, ...), src/charm_3d.c:36-37) */
+int charm_forest_new_uniform(int level, const int periodic[3], charm_forest_t **out);
+/* forest from explicit leaves (x, y, z, level) in root units (2^19) */
+int charm_forest_new_leaves(int64_t n, const int32_t *leaves, const int periodic[3],
+                            charm_forest_t **out);
+int charm_forest_destroy(charm_forest_t *f);
+int64_t charm_forest_num_leaves(const charm_forest_t *f);
+int charm_forest_get_leaves(const charm_forest_t *f, int32_t *out /* [n][4] */);
+/* refine (+1) / coarsen (-1, whole families only) by per-leaf flags, then
+ * 2:1 face balance (p4est_refine_ext / coarsen_ext / balance_ext(FACE),
+ * src/charm_amr.c:601-608).  `map` (may be NULL) receives for every new leaf
+ * the index of the old leaf it came from (refined children: the parent;
+ * coarsened parent: -1 - first child). */
+int charm_forest_adapt(charm_forest_t *f, const int8_t *flags, int max_level, int min_level,
+                       int64_t **map, int64_t *n_map);
+int charm_forest_balance(charm_forest_t *f);
+void *charm_forest_user_pointer(const charm_forest_t *f);
+void charm_forest_set_user_pointer(charm_forest_t *f, void *p);
+
+/* flatten the leaves into the element/face arrays the kernels consume */
+int charm_forest_flatten(const charm_forest_t *f, int chunk_elems, int chunk_faces,
+                         charm_mesh_t **out);
+int charm_mesh_destroy(charm_mesh_t *m);
+int64_t charm_mesh_num_faces(const charm_mesh_t *m);
+int64_t charm_mesh_num_chunks(const charm_mesh_t *m);
+/* faces: [nf][4] = (e0, e1 (or -1-domain_face), info, 0) with info =
+ * dir0 | kind << 3 | subface << 5; kind 0 conforming, 1 side 0 hanging
+ * (small), 2 side 1 hanging (small), 3 boundary */
+int charm_mesh_get_faces(const charm_mesh_t *m, int32_t *out);
+/* limiter neighbour table: [n][6] */
+int charm_mesh_get_neighbors(const charm_mesh_t *m, int32_t *out);
+
+/* ------------------------------------------------------------ device */
+typedef struct charm_gpu_ctx charm_gpu_ctx_t;
+
+const char *charm_gpu_last_error(void);
+int charm_gpu_device_count(int *n);
+int charm_gpu_create(const charm_config_t *cfg, charm_gpu_ctx_t **out);
+int charm_gpu_destroy(charm_gpu_ctx_t *ctx);
+/* call after every adapt/balance: uploads the flattened mesh */
+int charm_gpu_upload_mesh(charm_gpu_ctx_t *ctx, const charm_mesh_t *mesh);
+/* state as FP64 [n][V][4] on the host (V = 4 + n_comp, order ru rv rw re
+ * rc[0..]); converted to the context precision on the device */
+int charm_gpu_set_state(charm_gpu_ctx_t *ctx, const double *u, int64_t n);
+int charm_gpu_get_state(charm_gpu_ctx_t *ctx, double *u, int64_t n);
+/* int_* integrals after the zero + convective pass (the value the
+ * reference's update consumes, src/models/euler/charm_model.c:519-525),
+ * FP64 [n][V][4] */
+int charm_gpu_residual(charm_gpu_ctx_t *ctx, double *out, int64_t n);
+/* Barth-Jespersen limiter on the current state */
+int charm_gpu_limiter_bj(charm_gpu_ctx_t *ctx);
+/* whole time steps of the configured model with a fixed dt */
+int charm_gpu_run(charm_gpu_ctx_t *ctx, double dt, int nsteps);
+/* as charm_gpu_run with an explicit model (CHARM_MODEL_EULER: forward Euler
+ * + limiter; CHARM_MODEL_NS: SSP-RK3 with the limiter inside each stage) */
+int charm_gpu_step(charm_gpu_ctx_t *ctx, int model, double dt, int nsteps);
+/* chunk sizes (elements, faces) whose LDS footprint fits a workgroup for
+ * this configuration */
+int charm_gpu_recommended_chunks(const charm_config_t *cfg, int *chunk_elems, int *chunk_faces);
+/* average device time of `reps` back-to-back launches of the stage kernel
+ * and of the limiter kernel on the current state (HIP events, ms) */
+int charm_gpu_time_kernels(charm_gpu_ctx_t *ctx, double dt, int reps, double *stage_ms,
+                           double *limiter_ms);
+/* CFL time step: mode 0 = CFL*volume/(|u|+c) (the reference's disabled
+ * branch, src/models/euler/charm_model.c:15-30), mode 1 = CFL*h/(|u|+c) */
+int charm_gpu_get_dt(charm_gpu_ctx_t *ctx, int mode, double *dt);
+/* FP64 sums_e V_e U[e][v][0] */
+int charm_gpu_conservation_sums(charm_gpu_ctx_t *ctx, double *out);
+/* 1 if a NaN / non-positive pressure was met since the last call */
+int charm_gpu_state_flag(charm_gpu_ctx_t *ctx, int *flag);
+int charm_gpu_synchronize(charm_gpu_ctx_t *ctx);
+/* device time of the last charm_gpu_run, split per kernel family (ms) */
+int charm_gpu_last_timing(charm_gpu_ctx_t *ctx, double *stage_ms, double *limiter_ms);
+
+/* pointwise flux on the device for a batch of point pairs: prim rows are
+ * (r, u, v, w, p, c[0..n_comp-1]); normals [n][3]; out [n][4 + n_comp] */
+int charm_gpu_flux_batch(charm_gpu_ctx_t *ctx, int flux, int64_t n, const double *prim_l,
+                         const double *prim_r, const double *normals, double *out);
+
+/* ------------------------------------------------ reference entry points
+ * (include/charm_models.h:15-23, charm_limiter.h:13-15, charm_fluxes.h:19-32)
+ * The forest's user pointer is a charm_ctx_t built by charm_ctx_create. */
+typedef struct charm_ctx charm_ctx_t;
+typedef struct charm_ghost charm_ghost_t;     /* p4est_ghost_t: single rank, unused */
+typedef struct charm_data  charm_data_t;      /* ghost records: unused */
+
+/* the reference's prim record (include/charm_globals.h:59-76) */
+typedef struct {
+    charm_real_t r, u, v, w, e, e_tot, p, t, cz, gam, cp, cv, m;
+    int          mat_id;
+    charm_real_t c[128];
+} charm_prim_t;
+
+int  charm_ctx_create(charm_forest_t *f, const charm_config_t *cfg, charm_ctx_t **out);
+int  charm_ctx_destroy(charm_ctx_t *c);
+charm_gpu_ctx_t *charm_ctx_gpu(charm_ctx_t *c);
+
+void         charm_model_euler_timestep_single(charm_forest_t *p4est, charm_real_t *dt,
+                                               charm_ghost_t **ghost, charm_data_t **ghost_data);
+charm_real_t charm_model_euler_get_dt(charm_forest_t *p4est);
+void         charm_model_ns_timestep_single(charm_forest_t *p4est, charm_real_t *dt,
+                                            charm_ghost_t **ghost, charm_data_t **ghost_data);
+charm_real_t charm_model_ns_get_dt(charm_forest_t *p4est);
+/* runs steps until t >= TMAX like src/charm_timesteps.c:23-51 */
+void charm_timesteps(charm_forest_t *p4est);
+void charm_limiter(charm_forest_t *p4est, charm_ghost_t *ghost, charm_data_t *ghost_data);
+void charm_limiter_bj(charm_forest_t *p4est, charm_ghost_t *ghost, charm_data_t *ghost_data);
+void charm_calc_flux_lf(charm_forest_t *p4est, charm_prim_t prim[2], charm_real_t *qu,
+                        charm_real_t *qv, charm_real_t *qw, charm_real_t *qe,
+                        charm_real_t qc[], charm_real_t n[3]);
+void charm_calc_flux_hllc(charm_forest_t *p4est, charm_prim_t prim[2], charm_real_t *qu,
+                          charm_real_t *qv, charm_real_t *qw, charm_real_t *qe,
+                          charm_real_t qc[], charm_real_t n[3]);
+void charm_calc_flux_godunov(charm_forest_t *p4est, charm_prim_t prim[2], charm_real_t *qu,
+                             charm_real_t *qv, charm_real_t *qw, charm_real_t *qe,
+                             charm_real_t qc[], charm_real_t n[3]);
+void charm_calc_flux_cd(charm_forest_t *p4est, charm_prim_t prim[2], charm_real_t *qu,
+                        charm_real_t *qv, charm_real_t *qw, charm_real_t *qe,
+                        charm_real_t qc[], charm_real_t n[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,152 @@
+"""CPU tests of libcharm3d.so: every symbol declared in include/charm_gpu.h is
+exported, and the host forest/flattening agrees with the oracle's own face
+traversal (content of the face list: sides, hanging sub-faces, boundaries)."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import charm_3d_amd as CA
+import mesh_util as MU
+import oracle_lib as OL
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    hdr = open(os.path.join(ROOT, "include", "charm_gpu.h")).read()
+    hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
+    names = set()
+    for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?\b(charm_\w+)\s*\(", hdr, flags=re.M):
+        names.add(m.group(1))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    CA.lib()
+    out = subprocess.run(["nm", "-D", "--defined-only", CA.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(line.split()[-1] for line in out.splitlines() if line.strip())
+    decl = declared_functions()
+    assert len(decl) > 40
+    missing = [d for d in decl if d not in exported]
+    assert not missing, missing
+
+
+def oracle_records(faces):
+    recs = set()
+    for r in faces:
+        ns, f0, f1, h0, h1 = (int(x) for x in r[:5])
+        q0, q1 = r[5:9], r[9:13]
+        bnd = int(r[13])
+        if ns == 1:
+            recs.add((int(q0[0]), -1 - bnd, f0 | (3 << 3)))
+        elif h0:
+            for j in range(4):
+                recs.add((int(q0[j]), int(q1[0]), f0 | (1 << 3) | (j << 5)))
+        elif h1:
+            for j in range(4):
+                recs.add((int(q0[0]), int(q1[j]), f0 | (2 << 3) | (j << 5)))
+        else:
+            recs.add((int(q0[0]), int(q1[0]), f0))
+    return recs
+
+
+def product_records(mesh):
+    return set((int(a), int(b), int(c)) for a, b, c, _ in mesh.faces())
+
+
+MESHES = [
+    ("uniform3", lambda: MU.uniform_leaves(3), (False,) * 3),
+    ("uniform3p", lambda: MU.uniform_leaves(3), (True,) * 3),
+    ("adapt_a", lambda: MU.random_adapted(1, 3, seed=11), (False,) * 3),
+    ("adapt_b", lambda: MU.random_adapted(1, 4, seed=12, frac=0.2, periodic=(True, False, True)),
+     (True, False, True)),
+    ("adapt_c", lambda: MU.random_adapted(2, 4, seed=13, frac=0.15, periodic=(True,) * 3), (True,) * 3),
+]
+
+
+@pytest.mark.parametrize("name,gen,periodic", MESHES, ids=[m[0] for m in MESHES])
+def test_flatten_matches_oracle_traversal(name, gen, periodic):
+    leaves = gen()
+    f = CA.Forest.from_leaves(leaves, periodic)
+    assert np.array_equal(f.leaves(), leaves)          # same Morton order
+    mesh = f.flatten(128, 512)
+    bnd = tuple("PERIODIC" if periodic[a // 2] else "OUTLET" for a in range(6))
+    osim = OL.OracleSim(OL.make_config(bnd=bnd), leaves)
+    assert product_records(mesh) == oracle_records(osim.faces())
+    assert mesh.num_faces == len(product_records(mesh))
+
+
+@pytest.mark.parametrize("level", [2, 3, 4])
+def test_uniform_forest_and_counts(level):
+    f = CA.Forest.uniform(level)
+    assert np.array_equal(f.leaves(), MU.uniform_leaves(level))
+    N = 1 << level
+    faces = f.flatten().faces()
+    kind = (faces[:, 2] >> 3) & 3
+    assert (kind == 0).sum() == 3 * N * N * (N - 1)
+    assert (kind == 3).sum() == 6 * N * N
+
+
+def test_balance_matches_bruteforce():
+    rng = np.random.default_rng(21)
+    leaves = [tuple(int(v) for v in q) for q in MU.uniform_leaves(1)]
+    for _ in range(3):                         # unbalanced random refinement
+        nxt = []
+        for q in leaves:
+            if rng.random() < 0.35 and q[3] < 5:
+                nxt.extend(MU._children(q))
+            else:
+                nxt.append(q)
+        leaves = nxt
+    raw = MU.sort_morton(np.array(leaves))
+    f = CA.Forest.from_leaves(raw)
+    f.balance()
+    assert np.array_equal(f.leaves(), MU.balance(raw))
+
+
+def test_neighbor_table_covers_every_face():
+    leaves = MU.random_adapted(1, 3, seed=31)
+    mesh = CA.Forest.from_leaves(leaves).flatten()
+    nbr = mesh.neighbors()
+    assert (nbr != np.iinfo(np.int32).min).all()
+    c, h = MU.centers(leaves)
+    for e in range(len(leaves)):
+        for d in range(6):
+            v = nbr[e, d]
+            a, s = d // 2, (1 if d % 2 else -1)
+            if -8 < v < 0:
+                assert v == -1 - d
+                edge = c[e, a] + s * 0.5 * h[e]
+                assert abs(edge - (1.0 if s > 0 else 0.0)) < 1e-12
+            elif v >= 0:
+                # the neighbour touches the face plane
+                assert abs((c[v, a] - s * 0.5 * h[v]) - (c[e, a] + s * 0.5 * h[e])) < 1e-12
+
+
+def test_adapt_refine_coarsen_and_map():
+    f = CA.Forest.uniform(2)
+    flags = np.zeros(f.n, dtype=np.int8)
+    flags[0] = 1
+    flags[8:16] = -1                     # one whole family
+    mp = f.adapt(flags, max_level=4, min_level=0)
+    lv = f.leaves()
+    assert len(mp) == len(lv)
+    assert (lv[:, 3] == 3).sum() == 8 and (lv[:, 3] == 1).sum() == 1
+    assert (mp[:8] == 0).all() and (mp[8:15] == np.arange(1, 8)).all() and mp[15] == -1 - 8
+    # the result is 2:1 balanced and tiles the cube
+    assert np.array_equal(lv, MU.balance(lv))
+
+
+def test_errors_are_reported_not_fatal():
+    with pytest.raises(CA.CharmError, match="tile|cover|aligned"):
+        CA.Forest.from_leaves(np.array([[0, 0, 0, 1]], dtype=np.int32))
+    unbalanced = MU.sort_morton(np.array(
+        [q for q in MU.uniform_leaves(1).tolist() if q[:3] != [0, 0, 0]] +
+        [q for q in MU.uniform_leaves(3).tolist() if max(q[:3]) < (1 << 18)]))
+    f = CA.Forest.from_leaves(unbalanced)
+    with pytest.raises(CA.CharmError, match="balanced"):
+        f.flatten()

@@ -1,0 +1,220 @@
+"""GPU parity: the HIP path (through the C-ABI) against the FP64 oracle.
+
+FP64 device results must match the oracle to 1e-12 relative (the bar the
+FP64 path is held to); the FP32 bulk path is compared to the FP64 device
+path with the tolerances written in each test.
+"""
+import numpy as np
+import pytest
+
+import charm_3d_amd as CA
+import charm_3d_amd.workloads as W
+import mesh_util as MU
+import oracle_lib as OL
+
+pytestmark = pytest.mark.gpu
+
+REL64 = 1e-12
+
+
+def oracle_cfg(cfg):
+    kinds = {v: k for k, v in CA.BND.items()}
+    bnd = tuple(kinds[cfg.bnd_type[i]] for i in range(6))
+    params = [[cfg.bnd_param[i][j] for j in range(6)] for i in range(6)]
+    return OL.make_config(n_comp=cfg.n_comp, comp_m=[cfg.comp[i].m for i in range(cfg.n_comp)],
+                          comp_cp=[(cfg.comp[i].cp[0],) for i in range(cfg.n_comp)],
+                          eos="IDEAL" if cfg.eos == 0 else "MIX",
+                          flux={v: k for k, v in CA.FLUX.items()}[cfg.flux],
+                          limiter=bool(cfg.limiter), model="NS" if cfg.model == 1 else "EULER",
+                          bnd=bnd, bnd_param=params, grav=[cfg.grav[i] for i in range(3)],
+                          cfl=cfg.cfl)
+
+
+def device(cfg, leaves, periodic, u0):
+    print(f"[gpu] n={len(leaves)} prec={cfg.precision} flux={cfg.flux} lim={cfg.limiter}", flush=True)
+    ctx = CA.GpuContext(cfg)
+    f = CA.Forest.from_leaves(leaves, periodic)
+    ce, cf = CA.recommended_chunks(cfg)
+    ctx.upload_mesh(f.flatten(ce, cf))
+    ctx.set_state(u0)
+    return ctx
+
+
+def relerr(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def random_state(leaves, rng, amp=0.1, nc=1):
+    n = len(leaves)
+    rho = 1.0 + amp * rng.uniform(-1, 1, n)
+    vel = 0.3 * rng.normal(size=(n, 3))
+    p = 1.0 + amp * rng.uniform(-1, 1, n)
+    u = np.zeros((n, 4 + nc, 4))
+    base = W.prim_to_cons(rho, vel, p)
+    u[:, :4, 0] = base[:, :4]
+    if nc == 1:
+        u[:, 4, 0] = rho
+    else:
+        frac = rng.dirichlet(np.ones(nc), size=n)
+        u[:, 4:, 0] = rho[:, None] * frac
+    u[:, :, 1:] = 0.2 * amp * rng.normal(size=(n, 4 + nc, 3)) * np.abs(u[:, :, :1])
+    return u
+
+
+MESHES = {
+    "uniform3": (lambda: MU.uniform_leaves(3), (False,) * 3),
+    "uniform3p": (lambda: MU.uniform_leaves(3), (True,) * 3),
+    "adapt": (lambda: MU.random_adapted(1, 3, seed=11), (False,) * 3),
+    "adapt_p": (lambda: MU.random_adapted(2, 4, seed=13, frac=0.15, periodic=(True,) * 3), (True,) * 3),
+}
+
+
+@pytest.mark.parametrize("flux", ["LF", "HLLC", "CD", "GODUNOV"])
+def test_pointwise_flux_fp64(flux):
+    cfg = CA.make_config(precision=CA.FP64, flux=flux)
+    ctx = CA.GpuContext(cfg)
+    ocfg = oracle_cfg(cfg)
+    rng = np.random.default_rng(7)
+    n = 256
+    pl = np.column_stack([rng.uniform(0.3, 2, n), rng.normal(size=(n, 3)), rng.uniform(0.3, 3, n), np.ones(n)])
+    pr = np.column_stack([rng.uniform(0.3, 2, n), rng.normal(size=(n, 3)), rng.uniform(0.3, 3, n), np.ones(n)])
+    nn = rng.normal(size=(n, 3))
+    nn[:32] = np.eye(3)[rng.integers(0, 3, 32)] * rng.choice([-1, 1], (32, 1))
+    nn /= np.linalg.norm(nn, axis=1)[:, None]
+    got = ctx.flux_batch(flux, pl, pr, nn)
+    for i in range(n):
+        a = OL.prim_from(ocfg, *pl[i, :5])
+        b = OL.prim_from(ocfg, *pr[i, :5])
+        ref = OL.flux(ocfg, flux, a, b, nn[i])
+        tol = 1e-9 if flux == "GODUNOV" else REL64
+        assert np.allclose(got[i], ref, rtol=tol, atol=tol * np.abs(ref).max()), (i, got[i], ref)
+
+
+@pytest.mark.parametrize("mesh", list(MESHES))
+@pytest.mark.parametrize("flux", ["LF", "HLLC", "GODUNOV"])
+def test_residual_fp64_matches_oracle(mesh, flux):
+    gen, periodic = MESHES[mesh]
+    leaves = gen()
+    bnd = ("PERIODIC",) * 6 if periodic[0] else ("OUTLET", "WALL_SLIP", "OUTLET", "INLET", "WALL_NO_SLIP", "OUTLET")
+    params = [None, None, None, (0.1, -0.2, 0.05, 1.1, 1.2), (0.9,), None]
+    cfg = CA.make_config(precision=CA.FP64, flux=flux, bnd=bnd, bnd_param=params, grav=(0.1, -0.3, 0.2))
+    u0 = random_state(leaves, np.random.default_rng(3))
+    ctx = device(cfg, leaves, periodic, u0)
+    got = ctx.residual()
+    sim = OL.OracleSim(oracle_cfg(cfg), leaves)
+    sim.set_state(u0)
+    ref = sim.residual()
+    tol = 1e-9 if flux == "GODUNOV" else REL64
+    assert relerr(got, ref) < tol
+
+
+@pytest.mark.parametrize("mesh", ["uniform3", "adapt", "adapt_p"])
+def test_limiter_fp64_matches_oracle(mesh):
+    gen, periodic = MESHES[mesh]
+    leaves = gen()
+    bnd = ("PERIODIC",) * 6 if periodic[0] else ("OUTLET", "WALL_SLIP", "OUTLET", "INLET", "OUTLET", "OUTLET")
+    params = [None, None, None, (0.1, -0.2, 0.05, 1.1, 1.2), None, None]
+    cfg = CA.make_config(precision=CA.FP64, limiter=True, bnd=bnd, bnd_param=params)
+    u0 = random_state(leaves, np.random.default_rng(5), amp=0.3)
+    ctx = device(cfg, leaves, periodic, u0)
+    ctx.limiter_bj()
+    sim = OL.OracleSim(oracle_cfg(cfg), leaves)
+    sim.set_state(u0)
+    sim.limiter()
+    ref = sim.get_state()
+    got = ctx.get_state()
+    assert relerr(got, ref) < REL64
+    assert (np.abs(got[:, :, 1:]) < np.abs(u0[:, :, 1:]) - 1e-14).any()
+
+
+@pytest.mark.parametrize("case", ["c1_l3", "c2_l4", "c2_adapt", "euler_lim"])
+def test_steps_fp64_match_oracle(case):
+    if case == "c1_l3":
+        cfg, leaves, u0, periodic = W.c1(level=3, precision=CA.FP64)
+        nsteps = 10
+    elif case == "c2_l4":
+        cfg, leaves, u0, periodic = W.c2(level=4, precision=CA.FP64)
+        nsteps = 10
+    elif case == "c2_adapt":
+        leaves = MU.random_adapted(2, 4, seed=17, frac=0.1)
+        cfg = CA.make_config(precision=CA.FP64, flux="HLLC", limiter=True)
+        c, _ = W.centers(leaves)
+        left = c[:, 0] < 0.5
+        u0 = np.zeros((len(leaves), 5, 4))
+        u0[:, :, 0] = W.prim_to_cons(np.where(left, 1.0, 0.125), np.zeros((len(c), 3)), np.where(left, 1.0, 0.1))
+        periodic = (False,) * 3
+        nsteps = 8
+    else:
+        cfg, leaves, u0, periodic = W.c2(level=3, precision=CA.FP64)
+        cfg.model = CA.MODEL["EULER"]
+        nsteps = 10
+    ctx = device(cfg, leaves, periodic, u0)
+    dt = ctx.get_dt(mode=1)
+    sim = OL.OracleSim(oracle_cfg(cfg), leaves)
+    sim.set_state(u0)
+    assert abs(dt - sim.cfl_dt(1)) <= REL64 * dt
+    ctx.run(dt, nsteps)
+    sim.run(dt, nsteps)
+    ref = sim.get_state()
+    assert relerr(ctx.get_state(), ref) < 1e-11
+    sums = ctx.conservation_sums()
+    assert np.allclose(sums, sim.conservation_sums(), rtol=1e-12, atol=1e-13)
+
+
+def test_fp32_bulk_vs_fp64_small_c2():
+    """FP32 bulk against FP64 on the Riemann box (level 4, 20 steps):
+    relative L2 over all coefficients below 1e-5 here; the full-size figure
+    is reported by bench.py."""
+    cfg64, leaves, u0, periodic = W.c2(level=4, precision=CA.FP64)
+    cfg32, _, _, _ = W.c2(level=4, precision=CA.FP32)
+    a = device(cfg64, leaves, periodic, u0)
+    b = device(cfg32, leaves, periodic, u0)
+    dt = a.get_dt(1)
+    a.run(dt, 20)
+    b.run(dt, 20)
+    x, y = a.get_state(), b.get_state()
+    l2 = np.linalg.norm(y - x) / np.linalg.norm(x)
+    assert l2 < 1e-5, l2
+    m0 = (u0[:, 4, 0]).sum()
+    assert abs(b.conservation_sums()[4] - a.conservation_sums()[4]) < 1e-5 * abs(a.conservation_sums()[4])
+    assert m0 > 0
+
+
+def test_fp32_residual_close_to_fp64():
+    leaves = MU.random_adapted(1, 3, seed=11)
+    u0 = random_state(leaves, np.random.default_rng(9))
+    r64 = device(CA.make_config(precision=CA.FP64), leaves, (False,) * 3, u0).residual()
+    r32 = device(CA.make_config(precision=CA.FP32), leaves, (False,) * 3, u0).residual()
+    assert relerr(r32, r64) < 2e-5
+
+
+def test_nan_state_is_reported():
+    leaves = MU.uniform_leaves(2)
+    u0 = random_state(leaves, np.random.default_rng(1))
+    u0[5, 3, 0] = np.nan
+    ctx = device(CA.make_config(precision=CA.FP32), leaves, (False,) * 3, u0)
+    with pytest.raises(CA.CharmError, match="SL <= S"):
+        ctx.run(1e-3, 1)
+
+
+def test_reference_entry_points_drive_the_device():
+    """charm_model_ns_timestep_single / charm_timesteps through the forest's
+    user pointer give the same state as charm_gpu_step."""
+    import ctypes as C
+    cfg, leaves, u0, periodic = W.c2(level=3, precision=CA.FP64)
+    L = CA.lib()
+    f = CA.Forest.from_leaves(leaves, periodic)
+    dt = 1e-3
+    cfg.dt, cfg.tmax = dt, 2.5 * dt
+    h = C.c_void_p()
+    CA.check(L.charm_ctx_create(f.h, C.byref(cfg), C.byref(h)))
+    gpu = L.charm_ctx_gpu(h)
+    u = np.ascontiguousarray(u0)
+    CA.check(L.charm_gpu_set_state(gpu, u.ctypes.data_as(C.POINTER(C.c_double)), len(u)))
+    L.charm_timesteps(f.h)                       # t = 0, dt, 2dt -> 3 steps
+    got = np.zeros_like(u)
+    CA.check(L.charm_gpu_get_state(gpu, got.ctypes.data_as(C.POINTER(C.c_double)), len(u)))
+    L.charm_ctx_destroy(h)
+    ref = device(cfg, leaves, periodic, u0)
+    ref.run(dt, 3)
+    assert np.array_equal(got, ref.get_state())
