@@ -43,6 +43,8 @@ def run_path(CA, W, level, precision, steps, warmup, u0=None):
         u0 = uic
     ctx = CA.GpuContext(cfg)
     ce, cf = CA.recommended_chunks(cfg)
+    if os.environ.get("CHARM_CHUNKS"):
+        ce, cf = (int(x) for x in os.environ["CHARM_CHUNKS"].split(","))
     ctx.upload_mesh(CA.Forest.from_leaves(leaves, periodic).flatten(ce, cf))
     ctx.set_state(u0)
     dt = ctx.get_dt(mode=1)
@@ -115,6 +117,7 @@ def main():
     ctx = r32["ctx"]
     ctx.set_state(r32["u0"])
     stage_ms, lim_ms = ctx.time_kernels(r32["dt"], args.kernel_reps)
+    copy_ms, copy_bytes = ctx.time_copy(args.kernel_reps)
     kb = stage_kernel_bytes(nv, 4) * n
     achieved = kb / (stage_ms * 1e-3) / 1e9
 
@@ -132,7 +135,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "Riemann box (rho,p)=(1,1)|(0.125,0.1) at x=0.5 with seeded "
-                               "+-1% cell-mean perturbation, Euler P1 Taylor basis, uniform "
+                               "+-1%% cell-mean perturbation, Euler P1 Taylor basis, uniform "
                                "octree level %d (%d hexes), HLLC flux + Barth-Jespersen limiter, "
                                "SSP-RK3, transmissive boundaries, dt = 0.3 h / max(|u|+c)" % (args.level, n),
                    "elements": n, "vars": nv, "stages_per_step": 3, "dt": r32["dt"]},
@@ -140,7 +143,8 @@ def main():
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "stage_ms": stage_ms, "limiter_ms": lim_ms,
-                     "bytes_per_elem": stage_kernel_bytes(nv, 4)},
+                     "bytes_per_elem": stage_kernel_bytes(nv, 4),
+                     "copy_GBs": copy_bytes / (copy_ms * 1e-3) / 1e9},
         "whole_step": {"device_ms_per_step": r32["dev_ms"] / args.steps,
                        "algorithmic_GBs": step_bytes(nv, 4) * n * 3 * args.steps / r32["wall"] / 1e9,
                        "hbm_frac": step_bytes(nv, 4) * n * 3 * args.steps / r32["wall"] / 1e9 / HBM_PEAK_GBS},

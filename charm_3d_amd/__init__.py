@@ -85,6 +85,7 @@ def lib():
         "charm_gpu_step": (C.c_int, [vp, C.c_int, C.c_double, C.c_int]),
         "charm_gpu_recommended_chunks": (C.c_int, [C.POINTER(Config), ip, ip]),
         "charm_gpu_time_kernels": (C.c_int, [vp, C.c_double, C.c_int, dp, dp]),
+        "charm_gpu_time_copy": (C.c_int, [vp, C.c_int, dp, dp]),
         "charm_gpu_get_dt": (C.c_int, [vp, C.c_int, dp]),
         "charm_gpu_conservation_sums": (C.c_int, [vp, dp]),
         "charm_gpu_state_flag": (C.c_int, [vp, ip]),
@@ -305,6 +306,11 @@ class GpuContext:
     def time_kernels(self, dt, reps):
         a, b = C.c_double(), C.c_double()
         check(lib().charm_gpu_time_kernels(self.h, dt, reps, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def time_copy(self, reps):
+        a, b = C.c_double(), C.c_double()
+        check(lib().charm_gpu_time_copy(self.h, reps, C.byref(a), C.byref(b)))
         return a.value, b.value
 
     def last_ms(self):

@@ -17,7 +17,8 @@ enum { BND_INLET = 0, BND_OUTLET = 1, BND_WALL_SLIP = 2, BND_WALL_NO_SLIP = 3 };
 enum { EOS_IDEAL = 0, EOS_MIX = 1 };
 enum { FACE_CONF = 0, FACE_HANG0 = 1, FACE_HANG1 = 2, FACE_BND = 3 };
 
-// Physical constants shared by every kernel of a context (passed by value).
+// Physical constants shared by every kernel of a context (passed by value),
+// in FP64 and, for the FP32 path, pre-rounded to FP32 on the host.
 struct DevParams {
     int    nc;
     int    eos;
@@ -28,11 +29,25 @@ struct DevParams {
     double bnd_param[6][5];
     double grav[3];
     double cfl;
+    float  gam_f, cv_f, m_f;
+    float  comp_m_f[kMaxComp], comp_cp_f[kMaxComp];
+    float  bnd_param_f[6][5];
+    float  grav_f[3];
 };
 
+template <typename R> __device__ __forceinline__ R cst(double d, float f);
+template <> __device__ __forceinline__ double cst<double>(double d, float) { return d; }
+template <> __device__ __forceinline__ float cst<float>(double, float f) { return f; }
+
+// Reciprocal, quotient and square root.  The FP32 bulk path uses the
+// hardware v_rcp_f32 / v_sqrt_f32 (about 1 ulp; operands here are normal
+// numbers), the FP64 path IEEE division and sqrt.
 template <typename R> __device__ __forceinline__ R rcp(R x) { return R(1) / x; }
+template <> __device__ __forceinline__ float rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+template <typename R> __device__ __forceinline__ R dv(R a, R b) { return a / b; }
+template <> __device__ __forceinline__ float dv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 template <typename R> __device__ __forceinline__ R dsqrt(R x) { return sqrt(x); }
-template <> __device__ __forceinline__ float dsqrt<float>(float x) { return __builtin_sqrtf(x); }
+template <> __device__ __forceinline__ float dsqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 template <typename R, int NC>
 struct Prim {
@@ -46,17 +61,17 @@ template <typename R, int NC>
 __device__ __forceinline__ void eos_coeffs(const DevParams &P, Prim<R, NC> &p)
 {
     if (P.eos == EOS_IDEAL || NC == 1) {
-        p.gam = R(P.gam); p.cv = R(P.cv); p.m = R(P.m);
+        p.gam = cst<R>(P.gam, P.gam_f); p.cv = cst<R>(P.cv, P.cv_f); p.m = cst<R>(P.m, P.m_f);
     } else {
         R cp = R(0), M_ = R(0);
         #pragma unroll
         for (int i = 0; i < NC; i++) {
-            M_ += p.c[i] / R(P.comp_m[i]);
-            cp += p.c[i] * R(P.comp_cp[i]);
+            M_ += dv(p.c[i], cst<R>(P.comp_m[i], P.comp_m_f[i]));
+            cp += p.c[i] * cst<R>(P.comp_cp[i], P.comp_cp_f[i]);
         }
-        p.m = R(1) / M_;
-        p.cv = cp - R(kGasR) / p.m;
-        p.gam = cp / p.cv;
+        p.m = rcp(M_);
+        p.cv = cp - dv(R(kGasR), p.m);
+        p.gam = dv(cp, p.cv);
     }
 }
 
@@ -84,40 +99,60 @@ __device__ __forceinline__ void cons_to_prim(const DevParams &P, const R *u, Pri
     if (r < R(kEps)) r = R(kEps);
     p.r = r;
     p.p = r * p.e * (p.gam - R(1));
-    p.cz = dsqrt(p.gam * p.p / r);
+    p.cz = dsqrt(p.gam * p.p * rcp(r));
 }
 
 // Exterior state of a boundary face, src/charm_bnd_cond.c:21-93: the BC
 // function edits (u, v, w, T, p), then EOS flag 3 rebuilds (r, cz, e) from
 // (T, p) and e_tot is recomputed.  n = s * e_axis (axis-aligned octree).
+// With frame_axis >= 0 the velocity components are stored in the rotated
+// order (frame_axis, t1, t2) and the inlet velocity is rotated alike.
 template <typename R, int NC>
 __device__ __forceinline__ void bnd_state(const DevParams &P, int face, int axis, R s,
-                                          const Prim<R, NC> &in, Prim<R, NC> &out)
+                                          const Prim<R, NC> &in, Prim<R, NC> &out, int frame_axis)
 {
     out = in;
-    R t = in.e / in.cv;                  // T from flag 4 (src/charm_eos.c:90)
+    R t = dv(in.e, in.cv);               // T from flag 4 (src/charm_eos.c:90)
     R pr = in.p;
     R vel[3] = {in.u, in.v, in.w};
     const int type = P.bnd_type[face];
     if (type == BND_INLET) {
-        vel[0] = R(P.bnd_param[face][0]); vel[1] = R(P.bnd_param[face][1]);
-        vel[2] = R(P.bnd_param[face][2]);
-        t = R(P.bnd_param[face][3]); pr = R(P.bnd_param[face][4]);
+        R pv[3];
+        #pragma unroll
+        for (int i = 0; i < 3; i++) pv[i] = cst<R>(P.bnd_param[face][i], P.bnd_param_f[face][i]);
+        if (frame_axis >= 0) {
+            const int a = frame_axis;
+            vel[0] = (a == 0) ? pv[0] : ((a == 1) ? pv[1] : pv[2]);
+            vel[1] = (a == 0) ? pv[1] : pv[0];
+            vel[2] = (a == 2) ? pv[1] : pv[2];
+        } else {
+            vel[0] = pv[0]; vel[1] = pv[1]; vel[2] = pv[2];
+        }
+        t = cst<R>(P.bnd_param[face][3], P.bnd_param_f[face][3]);
+        pr = cst<R>(P.bnd_param[face][4], P.bnd_param_f[face][4]);
     } else if (type == BND_WALL_SLIP) {
-        R vn = vel[axis] * s;
-        vel[axis] = (vel[axis] - s * vn) - s * vn;
+        R va = (axis == 0) ? vel[0] : ((axis == 1) ? vel[1] : vel[2]);
+        R vn = va * s;
+        va = (va - s * vn) - s * vn;
+        vel[0] = (axis == 0) ? va : vel[0];
+        vel[1] = (axis == 1) ? va : vel[1];
+        vel[2] = (axis == 2) ? va : vel[2];
     } else if (type == BND_WALL_NO_SLIP) {
-        R vn = vel[axis] * s;
-        vel[axis] = vel[axis] - s * vn;
-        t = R(P.bnd_param[face][0]);
+        R va = (axis == 0) ? vel[0] : ((axis == 1) ? vel[1] : vel[2]);
+        R vn = va * s;
+        va = va - s * vn;
+        vel[0] = (axis == 0) ? va : vel[0];
+        vel[1] = (axis == 1) ? va : vel[1];
+        vel[2] = (axis == 2) ? va : vel[2];
+        t = cst<R>(P.bnd_param[face][0], P.bnd_param_f[face][0]);
     }
     out.u = vel[0]; out.v = vel[1]; out.w = vel[2];
     eos_coeffs<R, NC>(P, out);
     if (pr < R(kEps)) pr = R(kEps);
     out.p = pr;
-    out.r = pr * out.m / (t * R(kGasR));
-    out.cz = dsqrt(out.gam * pr / out.r);
-    out.e = pr / (out.r * (out.gam - R(1)));
+    out.r = dv(pr * out.m, t * R(kGasR));
+    out.cz = dsqrt(dv(out.gam * pr, out.r));
+    out.e = dv(pr, out.r * (out.gam - R(1)));
     out.e_tot = out.e + R(0.5) * (out.u * out.u + out.v * out.v + out.w * out.w);
 }
 
@@ -154,15 +189,17 @@ __device__ __forceinline__ bool hllc_x(const Frame<R, NC> &L, const Frame<R, NC>
 {
     R p_pvrs = R(0.5) * (L.p + Rt.p) - R(0.5) * (Rt.un - L.un) * R(0.25) * (L.r + Rt.r) * (L.cz + Rt.cz);
     R p_star = p_pvrs > R(0) ? p_pvrs : R(0);
-    R ql = (p_star <= L.p) ? R(1) : dsqrt(R(1) + (L.gam + R(1)) * (p_star / L.p - R(1)) / (R(2) * L.gam));
-    R qr = (p_star <= Rt.p) ? R(1) : dsqrt(R(1) + (Rt.gam + R(1)) * (p_star / Rt.p - R(1)) / (R(2) * Rt.gam));
+    // (gamma + 1) / (2 gamma) folded to one factor per side
+    const R gl = dv(L.gam + R(1), R(2) * L.gam), gr = dv(Rt.gam + R(1), R(2) * Rt.gam);
+    R ql = (p_star <= L.p) ? R(1) : dsqrt(R(1) + gl * (dv(p_star, L.p) - R(1)));
+    R qr = (p_star <= Rt.p) ? R(1) : dsqrt(R(1) + gr * (dv(p_star, Rt.p) - R(1)));
     R sl = L.un - L.cz * ql;
     R sr = Rt.un + Rt.cz * qr;
     if (sl > sr) { R t = sl; sl = sr; sr = t; }
     R s_star = Rt.p - L.p;
     s_star += L.r * L.un * (sl - L.un);
     s_star -= Rt.r * Rt.un * (sr - Rt.un);
-    s_star /= (L.r * (sl - L.un) - Rt.r * (sr - Rt.un));
+    s_star = dv(s_star, L.r * (sl - L.un) - Rt.r * (sr - Rt.un));
     if (s_star < sl) s_star = sl;
     if (s_star > sr) s_star = sr;
     const bool ok = (sl <= s_star) && (s_star <= sr);
@@ -179,7 +216,7 @@ __device__ __forceinline__ bool hllc_x(const Frame<R, NC> &L, const Frame<R, NC>
         const Frame<R, NC> &K = left ? L : Rt;
         const R sk = left ? sl : sr;
         // F_HLLC_U / _V / _E macros, src/charm_fluxes.c:319-321
-        const R inv = R(1) / (sk - s_star);
+        const R inv = rcp(sk - s_star);
         const R pk = K.p + K.r * (sk - K.un) * (s_star - K.un);
         q[0] = (s_star * (sk * (K.r * K.un) - (K.r * K.un * K.un + K.p)) + sk * pk) * inv;
         q[1] = (s_star * (sk * (K.r * K.ut1) - K.r * K.un * K.ut1)) * inv;
@@ -219,77 +256,77 @@ __device__ __forceinline__ void godunov_x(const Frame<R, NC> &Lf, const Frame<R,
     const R RB = Lf.r, PB = Lf.p, UB = Lf.un, RE = Rf.r, PE = Rf.p, UE = Rf.un;
     const R gb = Lf.gam, ge = Rf.gam;
     const R AB = gb - R(1), AE = ge - R(1);
-    const R DB = R(2) / AB, DE = R(2) / AE;
+    const R DB = dv(R(2), AB), DE = dv(R(2), AE);
     const R GB = dsqrt(gb * AB), GE = dsqrt(ge * AE);
-    const R HB = AB / R(2), HE = AE / R(2);
+    const R HB = (AB * R(0.5)), HE = (AE * R(0.5));
     const R FB = R(3) * gb - R(1), FE = R(3) * ge - R(1);
-    const R OB = AB / (R(2) * gb), OE = AE / (R(2) * ge);
+    const R OB = dv(AB, R(2) * gb), OE = dv(AE, R(2) * ge);
     const R QB = gb + R(1), QE = ge + R(1);
-    const R PGB = QB / (R(2) * gb), PGE = QE / (R(2) * ge);
+    const R PGB = dv(QB, R(2) * gb), PGE = dv(QE, R(2) * ge);
     const R RGB = R(4) * gb, RGE = R(4) * ge;
     const R SB = gb * AB, SE = ge * AE;
-    const R TB = QB / R(2), TE = QE / R(2);
+    const R TB = (QB * R(0.5)), TE = (QE * R(0.5));
     const R eps = R(1.e-5);
-    const R CB = dsqrt(gb * PB / RB), CE = dsqrt(ge * PE / RE);
-    const R EB = CB * CB / SB, EE = CE * CE / SE;
+    const R CB = dsqrt(dv(gb * PB, RB)), CE = dsqrt(dv(ge * PE, RE));
+    const R EB = dv(CB * CB, SB), EE = dv(CE * CE, SE);
     const R RCB = RB * CB, RCE = RE * CE;
     const R DU = UB - UE;
     R RF = 0, RS = 0, EF = 0, ES = 0, UF = 0, US = 0, SBL, SFL, SSL, SEL;
-    if (DU < -(R(2) * CB / AB + R(2) * CE / AE)) {
-        SBL = UB - CB; SFL = UB + R(2) * CB / AB; SSL = UE - R(2) * CE / AE; SEL = UE + CE;
+    if (DU < -(dv(R(2) * CB, AB) + dv(R(2) * CE, AE))) {
+        SBL = UB - CB; SFL = UB + dv(R(2) * CB, AB); SSL = UE - dv(R(2) * CE, AE); SEL = UE + CE;
     } else {
-        R P = (PB * RCE + PE * RCB + DU * RCB * RCE) / (RCB + RCE);
+        R P = dv(PB * RCE + PE * RCB + DU * RCB * RCE, RCB + RCE);
         R F1, F2, FS1, FS2;
         int step = 0;
         do {
             if (P < eps) P = eps;
-            const R PPB = P / PB, PPE = P / PE;
+            const R PPB = dv(P, PB), PPE = dv(P, PE);
             if (PB <= P) {
                 const R PKB = PGB * PPB + OB, ZNB = RCB * dsqrt(PKB);
-                F1 = (P - PB) / ZNB;
-                FS1 = (QB * PPB + FB) / (RGB * ZNB * PKB);
+                F1 = dv(P - PB, ZNB);
+                FS1 = dv(QB * PPB + FB, RGB * ZNB * PKB);
             } else {
                 const R ZFB = CB * exp(log(PPB) * OB);
                 F1 = DB * (ZFB - CB);
-                FS1 = ZFB / (gb * P);
+                FS1 = dv(ZFB, gb * P);
             }
             if (PE <= P) {
                 const R PKE = PGE * PPE + OE, ZNE = RCE * dsqrt(PKE);
-                F2 = (P - PE) / ZNE;
-                FS2 = (QE * PPE + FE) / (RGE * ZNE * PKE);
+                F2 = dv(P - PE, ZNE);
+                FS2 = dv(QE * PPE + FE, RGE * ZNE * PKE);
             } else {
                 const R ZFE = CE * exp(log(PPE) * OE);
                 F2 = DE * (ZFE - CE);
-                FS2 = ZFE / (ge * P);
+                FS2 = dv(ZFE, ge * P);
             }
-            P = P + (DU - F1 - F2) / (FS1 + FS2);
+            P = P + dv(DU - F1 - F2, FS1 + FS2);
         } while ((fabs(DU - F1 - F2) > eps) && (++step < max_steps));
-        const R PPB = P / PB, PPE = P / PE;
+        const R PPB = dv(P, PB), PPE = dv(P, PE);
         const R ZFB = CB * exp(log(PPB) * OB), ZFE = CE * exp(log(PPE) * OE);
         if (PB <= P) {
-            const R D = UB - dsqrt((TB * P + HB * PB) / RB);
+            const R D = UB - dsqrt(dv(TB * P + HB * PB, RB));
             const R UBD = UB - D, RUBD = RB * UBD;
-            RF = RUBD * RUBD / (PB - P + RUBD * UBD);
-            UF = D + RUBD / RF;
-            EF = P / (AB * RF);
+            RF = dv(RUBD * RUBD, PB - P + RUBD * UBD);
+            UF = D + dv(RUBD, RF);
+            EF = dv(P, AB * RF);
             SBL = D; SFL = D;
         } else {
-            EF = ZFB * ZFB / SB;
+            EF = dv(ZFB * ZFB, SB);
             UF = UB + DB * (CB - ZFB);
-            RF = P / (AB * EF);
+            RF = dv(P, AB * EF);
             SBL = UB - CB; SFL = UF - ZFB;
         }
         if (PE <= P) {
-            const R D = UE + dsqrt((TE * P + HE * PE) / RE);
+            const R D = UE + dsqrt(dv(TE * P + HE * PE, RE));
             const R UED = UE - D, RUED = RE * UED;
-            RS = RUED * RUED / (PE - P + RUED * UED);
-            US = D + RUED / RS;
-            ES = P / (AE * RS);
+            RS = dv(RUED * RUED, PE - P + RUED * UED);
+            US = D + dv(RUED, RS);
+            ES = dv(P, AE * RS);
             SEL = D; SSL = D;
         } else {
-            ES = ZFE * ZFE / SE;
+            ES = dv(ZFE * ZFE, SE);
             US = UE - DE * (CE - ZFE);
-            RS = P / (AE * ES);
+            RS = dv(P, AE * ES);
             SSL = US + ZFE; SEL = UE + CE;
         }
     }
@@ -303,16 +340,16 @@ __device__ __forceinline__ void godunov_x(const Frame<R, NC> &Lf, const Frame<R,
         if (US >= R(0)) { RI = RF; EI = EF; UI = UF; VI = Lf.ut1; WI = Lf.ut2; PI = AB * EI * RI; side = 0; }
         else { RI = RS; EI = ES; UI = US; VI = Rf.ut1; WI = Rf.ut2; PI = AE * EI * RI; side = 1; }
     } else if (SFL > R(0)) {
-        UI = (UB + DB * GB * dsqrt(EB)) / (R(1) + DB);
+        UI = dv(UB + DB * GB * dsqrt(EB), R(1) + DB);
         VI = Lf.ut1; WI = Lf.ut2;
-        EI = (UI * UI) / SB;
-        RI = RB * exp(log(EI / EB) * (R(1) / AB));
+        EI = dv(UI * UI, SB);
+        RI = RB * exp(log(dv(EI, EB)) * rcp(AB));
         PI = AB * EI * RI; side = 0;
     } else {
-        UI = (UE - DE * GE * dsqrt(EE)) / (R(1) + DE);
+        UI = dv(UE - DE * GE * dsqrt(EE), R(1) + DE);
         VI = Rf.ut1; WI = Rf.ut2;
-        EI = (UI * UI) / SE;
-        RI = RE * exp(log(EI / EE) * (R(1) / AE));
+        EI = dv(UI * UI, SE);
+        RI = RE * exp(log(dv(EI, EE)) * rcp(AE));
         PI = AE * EI * RI; side = 1;
     }
     const R qr = RI * UI;

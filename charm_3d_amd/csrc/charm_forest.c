@@ -448,10 +448,38 @@ static int mesh_neighbors(charm_mesh_t *m)
     return CHARM_OK;
 }
 
+/* Close a chunk: order its faces by (axis, kind) so that a wave of the
+ * face phase mostly sees one face orientation, give each face its local
+ * index and translate the chunk's element slots to local indices. */
+static void finish_chunk(charm_mesh_t *m, const int32_t *cf_face, int64_t fb, int64_t fe,
+                         int64_t eb, int64_t ee, const int32_t *slot_face, const uint8_t *slot_side,
+                         int32_t *loc, int32_t *sorted)
+{
+    int64_t cnt[13] = {0}, i, e, s;
+    for (i = fb; i < fe; i++) {
+        int info = m->f4[4 * cf_face[i] + 2];
+        cnt[1 + ((info & 7) >> 1) * 4 + ((info >> 3) & 3)]++;
+    }
+    for (i = 1; i < 13; i++) cnt[i] += cnt[i - 1];
+    for (i = fb; i < fe; i++) {
+        int info = m->f4[4 * cf_face[i] + 2];
+        sorted[cnt[((info & 7) >> 1) * 4 + ((info >> 3) & 3)]++] = cf_face[i];
+    }
+    for (i = 0; i < fe - fb; i++) {
+        loc[sorted[i]] = (int32_t) i;
+        memcpy(&m->cf4[4 * (fb + i)], &m->f4[4 * sorted[i]], 4 * sizeof(int32_t));
+    }
+    for (e = eb; e < ee; e++)
+        for (s = m->slot_ptr[e]; s < m->slot_ptr[e + 1]; s++)
+            m->slots[s] = (uint16_t) ((loc[slot_face[s]] << 1) | slot_side[s]);
+}
+
 static int mesh_chunks(charm_mesh_t *m, int ch_e, int ch_f)
 {
     int64_t n = m->n, nf = m->nf, i, e;
-    int32_t *inc_ptr = NULL, *inc = NULL, *stamp = NULL, *loc = NULL;
+    int32_t *inc_ptr = NULL, *inc = NULL, *stamp = NULL, *loc = NULL, *cf_face = NULL;
+    int32_t *slot_face = NULL, *sorted = NULL;
+    uint8_t *slot_side = NULL;
     int64_t nch = 0, ncf = 0, nslot = 0, cap_ch, cap_cf;
     int rc = CHARM_ERR_MEM;
 
@@ -465,7 +493,10 @@ static int mesh_chunks(charm_mesh_t *m, int ch_e, int ch_f)
     inc = (int32_t *) malloc((size_t) inc_ptr[n] * sizeof(int32_t));
     stamp = (int32_t *) malloc((size_t) nf * sizeof(int32_t));
     loc = (int32_t *) malloc((size_t) nf * sizeof(int32_t));
-    if (!inc || !stamp || !loc) goto done;
+    slot_face = (int32_t *) malloc((size_t) inc_ptr[n] * sizeof(int32_t));
+    slot_side = (uint8_t *) malloc((size_t) inc_ptr[n]);
+    sorted = (int32_t *) malloc((size_t) (ch_f + 1) * sizeof(int32_t));
+    if (!inc || !stamp || !loc || !slot_face || !slot_side || !sorted) goto done;
     {
         int32_t *fill = (int32_t *) malloc((size_t) n * sizeof(int32_t));
         if (!fill) goto done;
@@ -483,9 +514,10 @@ static int mesh_chunks(charm_mesh_t *m, int ch_e, int ch_f)
     m->ch_ebeg = (int32_t *) malloc((size_t) (cap_ch + 1) * sizeof(int32_t));
     m->ch_fbeg = (int32_t *) malloc((size_t) (cap_ch + 1) * sizeof(int32_t));
     m->cf4 = (int32_t *) malloc((size_t) cap_cf * 4 * sizeof(int32_t));
+    cf_face = (int32_t *) malloc((size_t) cap_cf * sizeof(int32_t));
     m->slot_ptr = (int32_t *) malloc((size_t) (n + 1) * sizeof(int32_t));
     m->slots = (uint16_t *) malloc((size_t) inc_ptr[n] * sizeof(uint16_t));
-    if (!m->ch_ebeg || !m->ch_fbeg || !m->cf4 || !m->slot_ptr || !m->slots) goto done;
+    if (!m->ch_ebeg || !m->ch_fbeg || !m->cf4 || !cf_face || !m->slot_ptr || !m->slots) goto done;
 
     {
         int64_t ebeg = 0, cur = 0;
@@ -494,6 +526,8 @@ static int mesh_chunks(charm_mesh_t *m, int ch_e, int ch_f)
             int64_t nnew = 0, k;
             for (k = inc_ptr[e]; k < inc_ptr[e+1]; k++) if (stamp[inc[k]] != nch) nnew++;
             if (e > ebeg && (e - ebeg >= ch_e || cur + nnew > ch_f)) {
+                m->slot_ptr[e] = (int32_t) nslot;
+                finish_chunk(m, cf_face, m->ch_fbeg[nch], ncf, ebeg, e, slot_face, slot_side, loc, sorted);
                 nch++;
                 if (nch >= cap_ch) {
                     int32_t *a, *b;
@@ -516,24 +550,29 @@ static int mesh_chunks(charm_mesh_t *m, int ch_e, int ch_f)
                 int32_t fi = inc[k];
                 if (stamp[fi] != nch) {
                     stamp[fi] = (int32_t) nch;
-                    loc[fi] = (int32_t) cur;
                     if (ncf == cap_cf) {
-                        int32_t *t;
+                        int32_t *t, *t2;
                         cap_cf *= 2;
                         t = (int32_t *) realloc(m->cf4, (size_t) cap_cf * 4 * sizeof(int32_t));
                         if (!t) goto done;
                         m->cf4 = t;
+                        t2 = (int32_t *) realloc(cf_face, (size_t) cap_cf * sizeof(int32_t));
+                        if (!t2) goto done;
+                        cf_face = t2;
                     }
-                    memcpy(&m->cf4[4*ncf], &m->f4[4*fi], 4 * sizeof(int32_t));
-                    ncf++; cur++;
+                    cf_face[ncf++] = fi;
+                    cur++;
                 }
-                m->slots[nslot++] = (uint16_t) ((loc[fi] << 1) | (m->f4[4*fi] == e ? 0 : 1));
+                slot_face[nslot] = fi;
+                slot_side[nslot] = (uint8_t) (m->f4[4*fi] == e ? 0 : 1);
+                nslot++;
             }
         }
+        m->slot_ptr[n] = (int32_t) nslot;
+        finish_chunk(m, cf_face, m->ch_fbeg[nch], ncf, ebeg, n, slot_face, slot_side, loc, sorted);
         nch++;
         m->ch_ebeg[nch] = (int32_t) n;
         m->ch_fbeg[nch] = (int32_t) ncf;
-        m->slot_ptr[n] = (int32_t) nslot;
     }
     m->nch = nch;
     m->ncf = ncf;
@@ -543,7 +582,8 @@ static int mesh_chunks(charm_mesh_t *m, int ch_e, int ch_f)
     rc = CHARM_OK;
 done:
     if (rc == CHARM_ERR_MEM) charm_set_error("flatten: out of memory");
-    free(inc_ptr); free(inc); free(stamp); free(loc);
+    free(inc_ptr); free(inc); free(stamp); free(loc); free(cf_face);
+    free(slot_face); free(slot_side); free(sorted);
     return rc;
 }
 

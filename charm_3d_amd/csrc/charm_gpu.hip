@@ -26,7 +26,8 @@ struct charm_gpu_ctx {
     float last_ms;
     // mesh
     int64_t n, nch, ncf, nslot, naux;
-    int max_faces;
+    int max_faces, chunk_elems;
+    int n_xcd;
     uint8_t *d_level;
     int4 *d_cf4, *d_aux;
     int *d_ch_ebeg, *d_ch_fbeg, *d_slot_ptr, *d_nbr;
@@ -98,6 +99,10 @@ extern "C" int charm_gpu_create(const charm_config_t *cfg, charm_gpu_ctx_t **out
     c->nc = cfg->n_comp;
     c->nv = 4 + cfg->n_comp;
     c->esz = c->prec ? sizeof(float) : sizeof(double);
+    // MI355X: 8 XCDs; only the workgroup-to-L2 placement depends on it
+    c->n_xcd = 8;
+    if (getenv("CHARM_NUM_XCD")) c->n_xcd = atoi(getenv("CHARM_NUM_XCD"));
+    if (c->n_xcd < 1 || c->n_xcd > 64) c->n_xcd = 8;
     DevParams &P = c->P;
     memset(&P, 0, sizeof(P));
     P.nc = c->nc;
@@ -114,6 +119,11 @@ extern "C" int charm_gpu_create(const charm_config_t *cfg, charm_gpu_ctx_t **out
     }
     for (int d = 0; d < 3; d++) P.grav[d] = cfg->grav[d];
     P.cfl = cfg->cfl;
+    P.gam_f = (float) P.gam; P.cv_f = (float) P.cv; P.m_f = (float) P.m;
+    for (int i = 0; i < kMaxComp; i++) { P.comp_m_f[i] = (float) P.comp_m[i]; P.comp_cp_f[i] = (float) P.comp_cp[i]; }
+    for (int f = 0; f < 6; f++)
+        for (int j = 0; j < 5; j++) P.bnd_param_f[f][j] = (float) P.bnd_param[f][j];
+    for (int d = 0; d < 3; d++) P.grav_f[d] = (float) P.grav[d];
     hipError_t e;
     if ((e = hipStreamCreate(&c->st)) != hipSuccess || (e = hipEventCreate(&c->ev0)) != hipSuccess ||
         (e = hipEventCreate(&c->ev1)) != hipSuccess || (e = hipMalloc(&c->d_err, sizeof(int))) != hipSuccess ||
@@ -155,8 +165,8 @@ extern "C" int charm_gpu_destroy(charm_gpu_ctx_t *c)
 extern "C" int charm_gpu_upload_mesh(charm_gpu_ctx_t *c, const charm_mesh_t *m)
 {
     if (!c || !m) { charm_set_error("upload_mesh: null argument"); return CHARM_ERR_ARG; }
-    if (m->chunk_elems < 1 || m->chunk_elems > 128) {
-        charm_set_error("upload_mesh: chunk_elems %d outside [1, 128] (one thread per element)", m->chunk_elems);
+    if (m->chunk_elems < 1 || m->chunk_elems > 256) {
+        charm_set_error("upload_mesh: chunk_elems %d outside [1, 256] (one thread per element)", m->chunk_elems);
         return CHARM_ERR_ARG;
     }
     const size_t smem = stage_smem_bytes(c->prec, c->nc, m->chunk_faces);
@@ -199,6 +209,7 @@ extern "C" int charm_gpu_upload_mesh(charm_gpu_ctx_t *c, const charm_mesh_t *m)
     HIPCHK(hipStreamSynchronize(c->st));
     c->n = m->n; c->nch = m->nch; c->ncf = m->ncf; c->nslot = m->nslot; c->naux = m->naux;
     c->max_faces = m->chunk_faces;
+    c->chunk_elems = m->chunk_elems;
     return CHARM_OK;
 }
 
@@ -217,7 +228,7 @@ extern "C" int charm_gpu_set_state(charm_gpu_ctx_t *c, const double *u, int64_t 
     if (!u) { charm_set_error("set_state: null state"); return CHARM_ERR_ARG; }
     const int64_t cnt = n * 4 * c->nv;
     HIPCHK(hipMemcpyAsync(c->d_stage64, u, (size_t) cnt * sizeof(double), hipMemcpyHostToDevice, c->st));
-    HIPCHK(launch_convert(c->prec, 0, c->d_stage64, c->B[0], cnt, c->st));
+    HIPCHK(launch_convert(c->prec, 0, c->d_stage64, c->B[0], n, 4 * c->nv, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     return CHARM_OK;
 }
@@ -225,7 +236,7 @@ extern "C" int charm_gpu_set_state(charm_gpu_ctx_t *c, const double *u, int64_t 
 static int copy_out(charm_gpu_ctx_t *c, const void *src, double *u, int64_t n)
 {
     const int64_t cnt = n * 4 * c->nv;
-    HIPCHK(launch_convert(c->prec, 1, src, c->d_stage64, cnt, c->st));
+    HIPCHK(launch_convert(c->prec, 1, src, c->d_stage64, n, 4 * c->nv, c->st));
     HIPCHK(hipMemcpyAsync(u, c->d_stage64, (size_t) cnt * sizeof(double), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     return CHARM_OK;
@@ -246,7 +257,8 @@ static StageArgs stage_args(charm_gpu_ctx_t *c, const void *U, const void *U0, v
     A.U = U; A.U0 = U0; A.out = out;
     A.level = c->d_level; A.cf4 = c->d_cf4; A.ch_ebeg = c->d_ch_ebeg; A.ch_fbeg = c->d_ch_fbeg;
     A.slot_ptr = c->d_slot_ptr; A.slots = c->d_slots; A.err = c->d_err;
-    A.n_chunks = (int) c->nch; A.max_faces = c->max_faces;
+    A.n_elem = c->n; A.n_chunks = (int) c->nch; A.max_faces = c->max_faces; A.chunk_elems = c->chunk_elems;
+    A.n_xcd = c->n_xcd;
     A.mode = mode; A.combine = combine; A.dt = dt;
     return A;
 }
@@ -255,7 +267,7 @@ static LimArgs lim_args(charm_gpu_ctx_t *c, const void *W, const void *U0, void 
 {
     LimArgs A;
     A.W = W; A.U0 = U0; A.out = out; A.nbr = c->d_nbr; A.aux = c->d_aux;
-    A.n = c->n; A.limit = limit; A.combine = combine;
+    A.n = c->n; A.limit = limit; A.combine = combine; A.n_xcd = c->n_xcd;
     return A;
 }
 
@@ -369,10 +381,29 @@ extern "C" int charm_gpu_recommended_chunks(const charm_config_t *cfg, int *ce, 
         charm_set_error("recommended_chunks: bad arguments");
         return CHARM_ERR_ARG;
     }
-    int faces = 512;
-    while (faces > 64 && stage_smem_bytes(cfg->precision, cfg->n_comp, faces) > 65536) faces /= 2;
+    // about three faces per element on a uniform octree; keep the LDS of a
+    // workgroup near 48 KB so that three workgroups share a CU
+    int faces = 768;
+    while (faces > 96 && stage_smem_bytes(cfg->precision, cfg->n_comp, faces) > 49152) faces -= faces / 4;
     *cf = faces;
-    *ce = faces >= 512 ? 128 : faces / 4;
+    *ce = faces / 3 > 256 ? 256 : faces / 3;
+    return CHARM_OK;
+}
+
+extern "C" int charm_gpu_time_copy(charm_gpu_ctx_t *c, int reps, double *copy_ms, double *bytes)
+{
+    int rc = need_mesh(c, -1);
+    if (rc) return rc;
+    if (reps < 1) { charm_set_error("time_copy: reps < 1"); return CHARM_ERR_ARG; }
+    const size_t ssz = (size_t) c->n * 4 * c->nv * c->esz;
+    float ms = 0.f;
+    HIPCHK(hipEventRecord(c->ev0, c->st));
+    for (int r = 0; r < reps; r++) HIPCHK(launch_copy(c->B[0], c->B[2], ssz, c->st));
+    HIPCHK(hipEventRecord(c->ev1, c->st));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (copy_ms) *copy_ms = ms / reps;
+    if (bytes) *bytes = 2.0 * (double) ssz;
     return CHARM_OK;
 }
 

@@ -16,7 +16,9 @@ struct StageArgs {
     const int *ch_ebeg, *ch_fbeg, *slot_ptr;
     const uint16_t *slots;
     int *err;
-    int n_chunks, max_faces;
+    int64_t n_elem;
+    int n_chunks, max_faces, chunk_elems;
+    int n_xcd;                // XCDs the workgroups are spread over
     int mode;                 // 0: write int_*; 1: c - dt A^-1 int
     int combine;              // 0: none, 1: w/4 + 3 u0/4, 2: 2w/3 + u0/3
     double dt;
@@ -29,6 +31,7 @@ struct LimArgs {
     const int4 *aux;
     int64_t n;
     int limit;                // apply Barth-Jespersen
+    int n_xcd;
     int combine;              // as StageArgs::combine
 };
 
@@ -51,6 +54,7 @@ size_t stage_smem_bytes(int prec, int nc, int max_faces);
 hipError_t launch_limiter(int prec, int nc, const LimArgs &A, const DevParams &P, hipStream_t st);
 hipError_t launch_reduce(int prec, int nc, const RedArgs &A, const DevParams &P, hipStream_t st);
 hipError_t launch_flux_points(int prec, int nc, int flux, const FluxArgs &A, const DevParams &P, hipStream_t st);
-hipError_t launch_convert(int prec, int to_f64, const void *src, void *dst, int64_t n, hipStream_t st);
+hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st);
+hipError_t launch_convert(int prec, int to_f64, const void *src, void *dst, int64_t n, int nvk, hipStream_t st);
 
 } // namespace charm

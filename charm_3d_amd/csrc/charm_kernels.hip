@@ -1,7 +1,8 @@
 // charm_kernels.hip -- gfx950 kernels of the DG residual / time-step path.
 //
-// State layout: U[e][v][k], the 4V coefficients of an element contiguous
-// (v = ru rv rw re rc[..], k = Taylor basis 1, (x-xc)/h, (y-yc)/h, (z-zc)/h).
+// State layout (structure of arrays): U[(4 v + k) n + e] for element e of n,
+// variable v = ru rv rw re rc[..], Taylor coefficient k (1, (x-xc)/h,
+// (y-yc)/h, (z-zc)/h): a wave's per-element accesses are unit-stride.
 //
 // One RK stage = k_stage over workgroup chunks (<= 128 Morton-consecutive
 // elements each):
@@ -39,6 +40,16 @@ __device__ __forceinline__ R gp_t2(int k)
                  case 3: return R(-1.0/6); case 4: return R(1.0/3); default: return R(1.0/3); }
 }
 
+// Workgroups are dispatched round-robin over the XCDs (each with its own
+// L2): workgroup b runs on XCD b % n_xcd.  Renumber so that every XCD walks a
+// contiguous range of Morton-ordered chunks / element blocks and neighbours
+// share its L2.  The grid is n_xcd * ceil(n / n_xcd) workgroups.
+__device__ __forceinline__ int xcd_block(int n_xcd)
+{
+    const int per = (int) gridDim.x / n_xcd;
+    return ((int) blockIdx.x % n_xcd) * per + (int) blockIdx.x / n_xcd;
+}
+
 // SSP-RK combines rk_1 (c = c/4 + 3 c_old/4) and rk_2 (c = 2c/3 + c_old/3)
 // in the reference's operation order, src/models/ns/charm_model.c:136-184
 template <typename R>
@@ -50,12 +61,13 @@ __device__ __forceinline__ R rk_combine(int mode, R w, R u0)
     return c + u0 / R(3);
 }
 
+// per-(v, k) array bases are wave-uniform (scalar registers); the element
+// index is the 32-bit per-lane offset
 template <typename R, int NV>
-__device__ __forceinline__ void load_elem(const R *__restrict__ U, int e, R *u)
+__device__ __forceinline__ void load_elem(const R *__restrict__ U, int64_t n, int e, R *u)
 {
-    const R *p = U + (size_t) e * (4 * NV);
     #pragma unroll
-    for (int i = 0; i < 4 * NV; i++) u[i] = p[i];
+    for (int i = 0; i < 4 * NV; i++) u[i] = (U + i * n)[e];
 }
 
 template <typename R, int NV>
@@ -66,9 +78,6 @@ __device__ __forceinline__ void trace(const R *u, R px, R py, R pz, R *t)
         t[v] = u[4*v] + px * u[4*v+1] + py * u[4*v+2] + pz * u[4*v+3];
 }
 
-__device__ __forceinline__ int tang1(int a) { return a == 0 ? 1 : 0; }
-__device__ __forceinline__ int tang2(int a) { return a == 2 ? 1 : 2; }
-
 // spread (normal, t1, t2) components onto (x, y, z)
 template <typename R>
 __device__ __forceinline__ void to_xyz(int a, R n, R t1, R t2, R &x, R &y, R &z)
@@ -78,24 +87,20 @@ __device__ __forceinline__ void to_xyz(int a, R n, R t1, R t2, R &x, R &y, R &z)
     z = (a == 2) ? n : t2;
 }
 
-// moments of one face: out[3][NV] = wJ * (sum q, sum q phi_t1, sum q phi_t2)
-template <typename R, int NC, int FLUX>
-__device__ __forceinline__ bool face_moments(const DevParams &P, const R *__restrict__ U,
-                                             const uint8_t *__restrict__ level, int4 rec,
-                                             R *out /* [3][NV] */)
+// Moments of one face normal to axis AX from the two element states u0, u1
+// (u1 unused on a boundary face):
+// out[3][NV] = wJ * (sum q, sum q phi_t1, sum q phi_t2) over the six points,
+// phi_t the tangential Taylor coordinates of the small (or side-0) face.
+template <typename R, int NC, int FLUX, int AX, bool BND>
+__device__ __forceinline__ bool face_core(const DevParams &P, const R *u0, const R *u1, int info,
+                                          int lown, R *out, int bnd_axis)
 {
     constexpr int NV = 4 + NC;
-    const int e0 = rec.x, e1 = rec.y, info = rec.z;
+    constexpr int T1 = (AX == 0) ? 1 : 0, T2 = (AX == 2) ? 1 : 2;
     const int dir0 = info & 7, kind = (info >> 3) & 3, sub = (info >> 5) & 3;
-    const int a = dir0 >> 1;
     const R s = (dir0 & 1) ? R(1) : R(-1);              // outward normal of side 0
-    const int lev0 = level[e0];
-    const int lown = (kind == FACE_HANG1) ? lev0 + 1 : lev0;
     const R h = ldexp(R(1), -lown);
-    const R wj = h * h / R(6);
-    R u0[4 * NV], u1[4 * NV];
-    load_elem<R, NV>(U, e0, u0);
-    if (kind != FACE_BND) load_elem<R, NV>(U, e1, u1);
+    const R wj = h * h * R(1.0 / 6.0);
     // normal-basis values on the face: side 0 at +-1/2 by its face parity
     const R pa0 = R(0.5) * s, pa1 = -pa0;
     // the big side of a hanging face sees the small face's points at
@@ -103,54 +108,47 @@ __device__ __forceinline__ bool face_moments(const DevParams &P, const R *__rest
     const R d1 = (sub & 1) ? R(0.25) : R(-0.25);
     const R d2 = (sub & 2) ? R(0.25) : R(-0.25);
     const bool big0 = (kind == FACE_HANG1), big1 = (kind == FACE_HANG0);
+    const R m0 = big0 ? R(0.5) : R(1), m1 = big1 ? R(0.5) : R(1);
+    const R o01 = big0 ? d1 : R(0), o02 = big0 ? d2 : R(0);
+    const R o11 = big1 ? d1 : R(0), o12 = big1 ? d2 : R(0);
     R S0[NV], S1[NV], S2[NV];
-    #pragma unroll
-    for (int v = 0; v < NV; v++) { S0[v] = R(0); S1[v] = R(0); S2[v] = R(0); }
     bool ok = true;
     #pragma unroll
     for (int k = 0; k < 6; k++) {
         const R g1 = gp_t1<R>(k), g2 = gp_t2<R>(k);
-        const R a1_0 = big0 ? d1 + R(0.5) * g1 : g1, a2_0 = big0 ? d2 + R(0.5) * g2 : g2;
-        R px, py, pz;
-        to_xyz<R>(a, pa0, a1_0, a2_0, px, py, pz);
-        R t[NV];
-        trace<R, NV>(u0, px, py, pz, t);
+        R ph[3], t[NV];
+        ph[AX] = pa0; ph[T1] = o01 + m0 * g1; ph[T2] = o02 + m0 * g2;
+        trace<R, NV>(u0, ph[0], ph[1], ph[2], t);
         Prim<R, NC> pl, pr;
         cons_to_prim<R, NC>(P, t, pl);
-        if (kind == FACE_BND) {
-            bnd_state<R, NC>(P, dir0, a, s, pl, pr);
+        if (BND) {
+            bnd_state<R, NC>(P, dir0, AX, s, pl, pr, bnd_axis);
         } else {
-            const R a1_1 = big1 ? d1 + R(0.5) * g1 : g1, a2_1 = big1 ? d2 + R(0.5) * g2 : g2;
-            to_xyz<R>(a, pa1, a1_1, a2_1, px, py, pz);
-            trace<R, NV>(u1, px, py, pz, t);
+            ph[AX] = pa1; ph[T1] = o11 + m1 * g1; ph[T2] = o12 + m1 * g2;
+            trace<R, NV>(u1, ph[0], ph[1], ph[2], t);
             cons_to_prim<R, NC>(P, t, pr);
         }
-        // face frame (s e_a, e_t1, e_t2)
-        const int t1 = tang1(a), t2 = tang2(a);
+        // face frame (s e_AX, e_T1, e_T2)
         Frame<R, NC> fl, fr;
         {
             const R vl[3] = {pl.u, pl.v, pl.w}, vr[3] = {pr.u, pr.v, pr.w};
             fl.r = pl.r; fl.e_tot = pl.e_tot; fl.p = pl.p; fl.cz = pl.cz; fl.gam = pl.gam;
             fr.r = pr.r; fr.e_tot = pr.e_tot; fr.p = pr.p; fr.cz = pr.cz; fr.gam = pr.gam;
-            fl.un = s * ((a == 0) ? vl[0] : (a == 1 ? vl[1] : vl[2]));
-            fr.un = s * ((a == 0) ? vr[0] : (a == 1 ? vr[1] : vr[2]));
-            fl.ut1 = (t1 == 0) ? vl[0] : vl[1];
-            fr.ut1 = (t1 == 0) ? vr[0] : vr[1];
-            fl.ut2 = (t2 == 1) ? vl[1] : vl[2];
-            fr.ut2 = (t2 == 1) ? vr[1] : vr[2];
+            fl.un = s * vl[AX]; fr.un = s * vr[AX];
+            fl.ut1 = vl[T1]; fr.ut1 = vr[T1];
+            fl.ut2 = vl[T2]; fr.ut2 = vr[T2];
             #pragma unroll
             for (int i = 0; i < NC; i++) { fl.c[i] = pl.c[i]; fr.c[i] = pr.c[i]; }
         }
-        R q[NV];
+        R q[NV], qg[NV];
         ok &= flux_x<R, NC, FLUX>(fl, fr, q);
-        R qx, qy, qz;
-        to_xyz<R>(a, s * q[0], q[1], q[2], qx, qy, qz);
-        q[0] = qx; q[1] = qy; q[2] = qz;
+        qg[AX] = s * q[0]; qg[T1] = q[1]; qg[T2] = q[2];
+        #pragma unroll
+        for (int v = 3; v < NV; v++) qg[v] = q[v];
         #pragma unroll
         for (int v = 0; v < NV; v++) {
-            S0[v] += q[v];
-            S1[v] += q[v] * g1;
-            S2[v] += q[v] * g2;
+            if (k == 0) { S0[v] = qg[v]; S1[v] = qg[v] * g1; S2[v] = qg[v] * g2; }
+            else { S0[v] += qg[v]; S1[v] += qg[v] * g1; S2[v] += qg[v] * g2; }
         }
     }
     #pragma unroll
@@ -158,6 +156,66 @@ __device__ __forceinline__ bool face_moments(const DevParams &P, const R *__rest
         out[v] = S0[v] * wj;
         out[NV + v] = S1[v] * wj;
         out[2 * NV + v] = S2[v] * wj;
+    }
+    return ok;
+}
+
+template <typename R, int NC, int FLUX, int AX>
+__device__ __forceinline__ bool face_moments(const DevParams &P, const R *__restrict__ U, int64_t n,
+                                             const uint8_t *__restrict__ level, int4 rec, R *out)
+{
+    constexpr int NV = 4 + NC;
+    const int kind = (rec.z >> 3) & 3;
+    const int lev0 = level[rec.x];
+    R u0[4 * NV], u1[4 * NV];
+    load_elem<R, NV>(U, n, rec.x, u0);
+    load_elem<R, NV>(U, n, rec.y, u1);
+    return face_core<R, NC, FLUX, AX, false>(P, u0, u1, rec.z, kind == FACE_HANG1 ? lev0 + 1 : lev0, out, AX);
+}
+
+// Boundary faces: the element state is rotated into the frame (a, t1, t2)
+// of its face (Taylor slopes and momentum components permuted), evaluated
+// with the x-normal code, and the momentum moments are rotated back; one
+// code path serves the three axes.
+template <typename R, int NC, int FLUX>
+__device__ __forceinline__ bool face_moments_bnd(const DevParams &P, const R *__restrict__ U, int64_t n,
+                                                 const uint8_t *__restrict__ level, int4 rec, R *out)
+{
+    constexpr int NV = 4 + NC;
+    const int ax = (rec.z & 7) >> 1;
+    R u[4 * NV], w[4 * NV];
+    load_elem<R, NV>(U, n, rec.x, u);
+    #pragma unroll
+    for (int v = 0; v < NV; v++) {
+        // source variable: momentum components follow the axis permutation
+        R c[4];
+        #pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (v < 3) {
+                const R x0 = u[4 * 0 + k], x1 = u[4 * 1 + k], x2 = u[4 * 2 + k];
+                // frame var v -> global var: v=0 -> ax, v=1 -> t1, v=2 -> t2
+                const int gv = (v == 0) ? ax : ((v == 1) ? (ax == 0 ? 1 : 0) : (ax == 2 ? 1 : 2));
+                c[k] = (gv == 0) ? x0 : ((gv == 1) ? x1 : x2);
+            } else {
+                c[k] = u[4 * v + k];
+            }
+        }
+        const R ca = (ax == 0) ? c[1] : ((ax == 1) ? c[2] : c[3]);
+        const R ct1 = (ax == 0) ? c[2] : c[1];
+        const R ct2 = (ax == 2) ? c[2] : c[3];
+        w[4 * v] = c[0]; w[4 * v + 1] = ca; w[4 * v + 2] = ct1; w[4 * v + 3] = ct2;
+    }
+    R o[3 * NV];
+    const bool ok = face_core<R, NC, FLUX, 0, true>(P, w, w, rec.z, level[rec.x], o, ax);
+    #pragma unroll
+    for (int m = 0; m < 3; m++) {
+        const R f0 = o[m * NV + 0], f1 = o[m * NV + 1], f2 = o[m * NV + 2];
+        // global var g takes frame var: g == ax -> 0, g == t1 -> 1, g == t2 -> 2
+        out[m * NV + 0] = (ax == 0) ? f0 : f1;
+        out[m * NV + 1] = (ax == 1) ? f0 : ((ax == 0) ? f1 : f2);
+        out[m * NV + 2] = (ax == 2) ? f0 : f2;
+        #pragma unroll
+        for (int v = 3; v < NV; v++) out[m * NV + v] = o[m * NV + v];
     }
     return ok;
 }
@@ -171,7 +229,8 @@ __device__ __forceinline__ void volume_integral(const DevParams &P, const R *u, 
 {
     constexpr int NV = 4 + NC;
     const R ph = R(0.28867513459481288225);           // 1/(2 sqrt 3)
-    const R wj = h * h * h / R(8);
+    const R hinv = rcp(h);
+    const R wj = h * h * h * R(0.125);
     R SF[3][NV];
     R G[4] = {R(0), R(0), R(0), R(0)};
     #pragma unroll
@@ -198,7 +257,7 @@ __device__ __forceinline__ void volume_integral(const DevParams &P, const R *u, 
         }
         G[0] += p.r; G[1] += p.r * px; G[2] += p.r * py; G[3] += p.r * pz;
     }
-    const R f = wj / h;
+    const R f = wj * hinv;
     #pragma unroll
     for (int v = 0; v < NV; v++) {
         res[4*v+0] = R(0);
@@ -208,14 +267,14 @@ __device__ __forceinline__ void volume_integral(const DevParams &P, const R *u, 
     }
     #pragma unroll
     for (int v = 0; v < 3; v++) {
-        const R gv = R(P.grav[v]);
+        const R gv = cst<R>(P.grav[v], P.grav_f[v]);
         #pragma unroll
         for (int k = 0; k < 4; k++) res[4*v+k] -= gv * G[k] * wj;
     }
 }
 
-template <typename R, int NC, int FLUX>
-__global__ __launch_bounds__(128)
+template <typename R, int NC, int FLUX, int BS>
+__global__ __launch_bounds__(BS)
 void k_stage(StageArgs A, DevParams P)
 {
     constexpr int NV = 4 + NC;
@@ -223,20 +282,33 @@ void k_stage(StageArgs A, DevParams P)
     R *mom = reinterpret_cast<R *>(smem_raw);                       // [maxf][3][NV]
     int *finfo = reinterpret_cast<int *>(smem_raw + (size_t) A.max_faces * 3 * NV * sizeof(R));
     const R *__restrict__ U = reinterpret_cast<const R *>(A.U);
-    const int c = blockIdx.x;
+    const int c = xcd_block(A.n_xcd);
+    if (c >= A.n_chunks) return;                 // whole workgroup exits together
     const int fb = A.ch_fbeg[c], fe = A.ch_fbeg[c + 1];
     const int eb = A.ch_ebeg[c], ee = A.ch_ebeg[c + 1];
     bool ok = true;
     for (int f = fb + (int) threadIdx.x; f < fe; f += blockDim.x) {
         const int4 rec = A.cf4[f];
-        ok &= face_moments<R, NC, FLUX>(P, U, A.level, rec, mom + (size_t) (f - fb) * 3 * NV);
+        R *dst = mom + (size_t) (f - fb) * 3 * NV;
+        // faces of a chunk are sorted by (axis, kind): waves are mostly
+        // uniform in the branch taken here
+        const int ax = (rec.z & 7) >> 1;
+        if (((rec.z >> 3) & 3) == FACE_BND) {
+            ok &= face_moments_bnd<R, NC, FLUX>(P, U, A.n_elem, A.level, rec, dst);
+        } else if (ax == 0) {
+            ok &= face_moments<R, NC, FLUX, 0>(P, U, A.n_elem, A.level, rec, dst);
+        } else if (ax == 1) {
+            ok &= face_moments<R, NC, FLUX, 1>(P, U, A.n_elem, A.level, rec, dst);
+        } else {
+            ok &= face_moments<R, NC, FLUX, 2>(P, U, A.n_elem, A.level, rec, dst);
+        }
         finfo[f - fb] = rec.z;
     }
     __syncthreads();
     const int e = eb + (int) threadIdx.x;
     if (e < ee) {
         R u[4 * NV], res[4 * NV];
-        load_elem<R, NV>(U, e, u);
+        load_elem<R, NV>(U, A.n_elem, e, u);
         const int lev = A.level[e];
         const R h = ldexp(R(1), -lev);
         volume_integral<R, NC>(P, u, h, res);
@@ -265,24 +337,25 @@ void k_stage(StageArgs A, DevParams P)
                 res[4*v+3] += cz;
             }
         }
-        R *out = reinterpret_cast<R *>(A.out) + (size_t) e * (4 * NV);
+        R *out = reinterpret_cast<R *>(A.out);
+        const int64_t n = A.n_elem;
         if (A.mode == 0) {
             #pragma unroll
-            for (int i = 0; i < 4 * NV; i++) out[i] = res[i];
+            for (int i = 0; i < 4 * NV; i++) (out + i * n)[e] = res[i];
         } else {
             // c -= dt * A^-1 int with A^-1 = diag(1, 12, 12, 12) / h^3
             // (src/models/euler/charm_model.c:402-433), then the SSP-RK
             // combine ca * U0 + cb * W when no limiter sits in between
-            const R vinv = R(1) / (h * h * h);
+            const R vinv = rcp(h * h * h);
             const R dt = R(A.dt);
-            const R *__restrict__ U0 = reinterpret_cast<const R *>(A.U0) + (size_t) e * (4 * NV);
+            const R *__restrict__ U0 = reinterpret_cast<const R *>(A.U0);
             #pragma unroll
             for (int v = 0; v < NV; v++)
                 #pragma unroll
                 for (int k = 0; k < 4; k++) {
                     const R ai = (k == 0) ? vinv : R(12) * vinv;
                     const R w = u[4*v+k] - dt * (ai * res[4*v+k]);
-                    out[4*v+k] = A.combine ? rk_combine<R>(A.combine, w, U0[4*v+k]) : w;
+                    (out + (4*v+k) * n)[e] = A.combine ? rk_combine<R>(A.combine, w, (U0 + (4*v+k) * n)[e]) : w;
                 }
         }
     }
@@ -301,11 +374,12 @@ __global__ __launch_bounds__(256)
 void k_limiter(LimArgs A, DevParams P)
 {
     constexpr int NV = 4 + NC;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = xcd_block(A.n_xcd) * blockDim.x + threadIdx.x;
     if (e >= A.n) return;
     const R *__restrict__ W = reinterpret_cast<const R *>(A.W);
+    const int64_t n = A.n;
     R u[4 * NV];
-    load_elem<R, NV>(W, e, u);
+    load_elem<R, NV>(W, n, e, u);
     R umin[NV], umax[NV];
     #pragma unroll
     for (int v = 0; v < NV; v++) { umin[v] = u[4*v]; umax[v] = u[4*v]; }
@@ -315,7 +389,7 @@ void k_limiter(LimArgs A, DevParams P)
             R m[NV];
             if (nb >= 0) {
                 #pragma unroll
-                for (int v = 0; v < NV; v++) m[v] = W[(size_t) nb * 4 * NV + 4 * v];
+                for (int v = 0; v < NV; v++) m[v] = (W + 4 * v * n)[nb];
             } else if (nb > -8) {
                 // boundary: mean -> prim -> BC -> cons (charm_limiter_bj.c:79-91)
                 R t[NV];
@@ -324,7 +398,7 @@ void k_limiter(LimArgs A, DevParams P)
                 Prim<R, NC> pi, po;
                 cons_to_prim<R, NC>(P, t, pi);
                 const int face = -1 - nb, a = face >> 1;
-                bnd_state<R, NC>(P, face, a, (face & 1) ? R(1) : R(-1), pi, po);
+                bnd_state<R, NC>(P, face, a, (face & 1) ? R(1) : R(-1), pi, po, -1);
                 m[0] = po.r * po.u; m[1] = po.r * po.v; m[2] = po.r * po.w;
                 m[3] = po.r * (po.e + R(0.5) * (po.u * po.u + po.v * po.v + po.w * po.w));
                 #pragma unroll
@@ -336,7 +410,7 @@ void k_limiter(LimArgs A, DevParams P)
                 for (int v = 0; v < NV; v++) m[v] = R(0);
                 for (int j = 0; j < 4; j++)
                     #pragma unroll
-                    for (int v = 0; v < NV; v++) m[v] += W[(size_t) q[j] * 4 * NV + 4 * v];
+                    for (int v = 0; v < NV; v++) m[v] += (W + 4 * v * n)[q[j]];
                 #pragma unroll
                 for (int v = 0; v < NV; v++) m[v] *= R(0.25);
             }
@@ -358,22 +432,22 @@ void k_limiter(LimArgs A, DevParams P)
                 const R df = f - u0;
                 R pt = R(1);
                 if (fabs(df) > R(kEps)) {
-                    if (df > R(0)) pt = fmin(R(1), (umax[v] - u0) / df);
-                    else if (df < R(0)) pt = fmin(R(1), (umin[v] - u0) / df);
+                    if (df > R(0)) pt = fmin(R(1), dv(umax[v] - u0, df));
+                    else if (df < R(0)) pt = fmin(R(1), dv(umin[v] - u0, df));
                 }
                 psi = pt < psi ? pt : psi;
             }
             u[4*v+1] *= psi; u[4*v+2] *= psi; u[4*v+3] *= psi;
         }
     }
-    R *out = reinterpret_cast<R *>(A.out) + (size_t) e * (4 * NV);
+    R *out = reinterpret_cast<R *>(A.out);
     if (A.combine) {
-        const R *__restrict__ U0 = reinterpret_cast<const R *>(A.U0) + (size_t) e * (4 * NV);
+        const R *__restrict__ U0 = reinterpret_cast<const R *>(A.U0);
         #pragma unroll
-        for (int i = 0; i < 4 * NV; i++) out[i] = rk_combine<R>(A.combine, u[i], U0[i]);
+        for (int i = 0; i < 4 * NV; i++) (out + i * n)[e] = rk_combine<R>(A.combine, u[i], (U0 + i * n)[e]);
     } else {
         #pragma unroll
-        for (int i = 0; i < 4 * NV; i++) out[i] = u[i];
+        for (int i = 0; i < 4 * NV; i++) (out + i * n)[e] = u[i];
     }
 }
 
@@ -390,14 +464,14 @@ void k_reduce(RedArgs A, DevParams P)
     for (int v = 0; v <= NV; v++) loc[v] = 0.0;
     loc[NV] = 1.e300;
     if (e < A.n) {
-        const R *u = reinterpret_cast<const R *>(A.U) + (size_t) e * (4 * NV);
+        const R *u = reinterpret_cast<const R *>(A.U) + e;
         const double h = ldexp(1.0, -(int) A.level[e]);
         const double vol = h * h * h;
         #pragma unroll
-        for (int v = 0; v < NV; v++) loc[v] = vol * (double) u[4 * v];
+        for (int v = 0; v < NV; v++) loc[v] = vol * (double) u[4 * v * A.n];
         R t[NV];
         #pragma unroll
-        for (int v = 0; v < NV; v++) t[v] = u[4 * v];
+        for (int v = 0; v < NV; v++) t[v] = u[4 * v * A.n];
         Prim<R, NC> p;
         cons_to_prim<R, NC>(P, t, p);
         const double len = A.mode == 0 ? vol : h;
@@ -492,35 +566,67 @@ __global__ void k_flux_points(FluxArgs A, DevParams P)
     for (int v = 3; v < NV; v++) o[v] = (double) q[v];
 }
 
-// precision conversions between the FP64 host layout and the device state
+// precision and layout conversions between the FP64 host layout [e][v][k]
+// and the device state [(4 v + k) n + e]
 template <typename R>
-__global__ void k_from_f64(const double *src, R *dst, int64_t n)
+__global__ void k_from_f64(const double *src, R *dst, int64_t n, int nvk)
 {
-    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = (R) src[i];
+    const int64_t e = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    for (int i = 0; i < nvk; i++) dst[i * n + e] = (R) src[e * nvk + i];
 }
 template <typename R>
-__global__ void k_to_f64(const R *src, double *dst, int64_t n)
+__global__ void k_to_f64(const R *src, double *dst, int64_t n, int nvk)
 {
-    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = (double) src[i];
+    const int64_t e = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    for (int i = 0; i < nvk; i++) dst[e * nvk + i] = (double) src[i * n + e];
+}
+
+// plain device copy in 16-byte words (state checkpoints; also the
+// known-size reference for the HBM byte counters)
+__global__ __launch_bounds__(256)
+void k_copy16(const int4 *__restrict__ src, int4 *__restrict__ dst, int64_t n16)
+{
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+         i += (int64_t) gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st)
+{
+    if (bytes % 16) return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st);
+    const int64_t n16 = (int64_t) (bytes / 16);
+    const unsigned blocks = (unsigned) ((n16 + 255) / 256 < 8192 ? (n16 + 255) / 256 : 8192);
+    if (n16 == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(256), 0, st, (const int4 *) src, (int4 *) dst, n16);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------- launchers
-template <typename R, int NC>
-static hipError_t launch_stage_nc(int flux, const StageArgs &A, const DevParams &P, hipStream_t st)
+template <typename R, int NC, int BS>
+static hipError_t launch_stage_bs(int flux, const StageArgs &A, const DevParams &P, hipStream_t st)
 {
     constexpr int NV = 4 + NC;
     const size_t sh = (size_t) A.max_faces * (3 * NV * sizeof(R) + sizeof(int));
-    const dim3 grid(A.n_chunks), block(128);
+    const dim3 grid((unsigned) (((A.n_chunks + A.n_xcd - 1) / A.n_xcd) * A.n_xcd)), block(BS);
     switch (flux) {
-    case FLUX_LF: hipLaunchKernelGGL((k_stage<R, NC, FLUX_LF>), grid, block, sh, st, A, P); break;
-    case FLUX_HLLC: hipLaunchKernelGGL((k_stage<R, NC, FLUX_HLLC>), grid, block, sh, st, A, P); break;
-    case FLUX_GODUNOV: hipLaunchKernelGGL((k_stage<R, NC, FLUX_GODUNOV>), grid, block, sh, st, A, P); break;
-    case FLUX_CD: hipLaunchKernelGGL((k_stage<R, NC, FLUX_CD>), grid, block, sh, st, A, P); break;
+    case FLUX_LF: hipLaunchKernelGGL((k_stage<R, NC, FLUX_LF, BS>), grid, block, sh, st, A, P); break;
+    case FLUX_HLLC: hipLaunchKernelGGL((k_stage<R, NC, FLUX_HLLC, BS>), grid, block, sh, st, A, P); break;
+    case FLUX_GODUNOV: hipLaunchKernelGGL((k_stage<R, NC, FLUX_GODUNOV, BS>), grid, block, sh, st, A, P); break;
+    case FLUX_CD: hipLaunchKernelGGL((k_stage<R, NC, FLUX_CD, BS>), grid, block, sh, st, A, P); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+// one thread per chunk element: 128- or 256-element chunks
+template <typename R, int NC>
+static hipError_t launch_stage_nc(int flux, const StageArgs &A, const DevParams &P, hipStream_t st)
+{
+    if (A.chunk_elems <= 128) return launch_stage_bs<R, NC, 128>(flux, A, P, st);
+    if (A.chunk_elems <= 256) return launch_stage_bs<R, NC, 256>(flux, A, P, st);
+    return hipErrorInvalidValue;
 }
 
 template <typename R>
@@ -548,7 +654,8 @@ size_t stage_smem_bytes(int prec, int nc, int max_faces)
 template <typename R>
 static hipError_t launch_limiter_r(int nc, const LimArgs &A, const DevParams &P, hipStream_t st)
 {
-    const dim3 grid((unsigned) ((A.n + 255) / 256)), block(256);
+    const int64_t nb = (A.n + 255) / 256;
+    const dim3 grid((unsigned) (((nb + A.n_xcd - 1) / A.n_xcd) * A.n_xcd)), block(256);
     switch (nc) {
     case 1: hipLaunchKernelGGL((k_limiter<R, 1>), grid, block, 0, st, A, P); break;
     case 2: hipLaunchKernelGGL((k_limiter<R, 2>), grid, block, 0, st, A, P); break;
@@ -615,16 +722,16 @@ hipError_t launch_flux_points(int prec, int nc, int flux, const FluxArgs &A, con
     return prec ? launch_flux_r<float>(nc, flux, A, P, st) : launch_flux_r<double>(nc, flux, A, P, st);
 }
 
-hipError_t launch_convert(int prec, int to_f64, const void *src, void *dst, int64_t n, hipStream_t st)
+hipError_t launch_convert(int prec, int to_f64, const void *src, void *dst, int64_t n, int nvk, hipStream_t st)
 {
     const dim3 grid((unsigned) ((n + 255) / 256)), block(256);
     if (n <= 0) return hipSuccess;
     if (to_f64) {
-        if (prec) hipLaunchKernelGGL(k_to_f64<float>, grid, block, 0, st, (const float *) src, (double *) dst, n);
-        else hipLaunchKernelGGL(k_to_f64<double>, grid, block, 0, st, (const double *) src, (double *) dst, n);
+        if (prec) hipLaunchKernelGGL(k_to_f64<float>, grid, block, 0, st, (const float *) src, (double *) dst, n, nvk);
+        else hipLaunchKernelGGL(k_to_f64<double>, grid, block, 0, st, (const double *) src, (double *) dst, n, nvk);
     } else {
-        if (prec) hipLaunchKernelGGL(k_from_f64<float>, grid, block, 0, st, (const double *) src, (float *) dst, n);
-        else hipLaunchKernelGGL(k_from_f64<double>, grid, block, 0, st, (const double *) src, (double *) dst, n);
+        if (prec) hipLaunchKernelGGL(k_from_f64<float>, grid, block, 0, st, (const double *) src, (float *) dst, n, nvk);
+        else hipLaunchKernelGGL(k_from_f64<double>, grid, block, 0, st, (const double *) src, (double *) dst, n, nvk);
     }
     return hipGetLastError();
 }

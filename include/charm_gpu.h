@@ -157,6 +157,9 @@ int charm_gpu_recommended_chunks(const charm_config_t *cfg, int *chunk_elems, in
  * and of the limiter kernel on the current state (HIP events, ms) */
 int charm_gpu_time_kernels(charm_gpu_ctx_t *ctx, double dt, int reps, double *stage_ms,
                            double *limiter_ms);
+/* average device time of a plain copy of the state (read + write bytes in
+ * *bytes): the bandwidth reference for the kernels above */
+int charm_gpu_time_copy(charm_gpu_ctx_t *ctx, int reps, double *copy_ms, double *bytes);
 /* CFL time step: mode 0 = CFL*volume/(|u|+c) (the reference's disabled
  * branch, src/models/euler/charm_model.c:15-30), mode 1 = CFL*h/(|u|+c) */
 int charm_gpu_get_dt(charm_gpu_ctx_t *ctx, int mode, double *dt);
