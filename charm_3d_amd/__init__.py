@@ -60,6 +60,7 @@ def lib():
     sig = {
         "charm_gpu_last_error": (C.c_char_p, []),
         "charm_gpu_device_count": (C.c_int, [ip]),
+        "charm_level_mass_inverse": (C.c_int, [C.c_int, dp, dp]),
         "charm_forest_new_uniform": (C.c_int, [C.c_int, ip, C.POINTER(vp)]),
         "charm_forest_new_leaves": (C.c_int, [i64, i32p, ip, C.POINTER(vp)]),
         "charm_forest_destroy": (C.c_int, [vp]),
@@ -155,6 +156,13 @@ def make_config(precision=FP32, model="NS", flux="HLLC", limiter=False, eos="IDE
     cfg.cfl, cfg.dt, cfg.tmax = cfl, dt, tmax
     cfg.chunk_elems, cfg.chunk_faces = chunk_elems, chunk_faces
     return cfg
+
+
+def level_mass_inverse(level):
+    """FP64 mass matrix of a level-`level` cube and its Gauss-Jordan inverse."""
+    a, ai = np.zeros((4, 4)), np.zeros((4, 4))
+    check(lib().charm_level_mass_inverse(level, _dptr(a), _dptr(ai)))
+    return a, ai
 
 
 def recommended_chunks(cfg):

@@ -29,6 +29,8 @@ struct DevParams {
     double bnd_param[6][5];
     double grav[3];
     double cfl;
+    double ainv0[4];                   // diagonal of A^-1 at level 0 (FP64 Gauss-Jordan)
+    float  ainv0_f[4];
     float  gam_f, cv_f, m_f;
     float  comp_m_f[kMaxComp], comp_cp_f[kMaxComp];
     float  bnd_param_f[6][5];

@@ -5,6 +5,7 @@
 // B1, B2 (stage results).  Every call returns CHARM_OK or a negative code;
 // the message of a failure is kept per host thread (charm_gpu_last_error).
 #include <hip/hip_runtime.h>
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -119,6 +120,19 @@ extern "C" int charm_gpu_create(const charm_config_t *cfg, charm_gpu_ctx_t **out
     }
     for (int d = 0; d < 3; d++) P.grav[d] = cfg->grav[d];
     P.cfl = cfg->cfl;
+    {
+        // FP64 mass-matrix inverse (level 0); the octree's matrices are
+        // diagonal, which is checked here
+        double a[4][4], ai[4][4];
+        if (charm_level_mass_inverse(0, a, ai)) return CHARM_ERR_ARG;
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++)
+                if (i != j && fabs(ai[i][j]) > 1e-14 * fabs(ai[i][i])) {
+                    charm_set_error("gpu_create: mass matrix inverse is not diagonal");
+                    return CHARM_ERR_ARG;
+                }
+        for (int k = 0; k < 4; k++) { P.ainv0[k] = ai[k][k]; P.ainv0_f[k] = (float) ai[k][k]; }
+    }
     P.gam_f = (float) P.gam; P.cv_f = (float) P.cv; P.m_f = (float) P.m;
     for (int i = 0; i < kMaxComp; i++) { P.comp_m_f[i] = (float) P.comp_m[i]; P.comp_cp_f[i] = (float) P.comp_cp[i]; }
     for (int f = 0; f < 6; f++)

@@ -343,17 +343,18 @@ void k_stage(StageArgs A, DevParams P)
             #pragma unroll
             for (int i = 0; i < 4 * NV; i++) (out + i * n)[e] = res[i];
         } else {
-            // c -= dt * A^-1 int with A^-1 = diag(1, 12, 12, 12) / h^3
-            // (src/models/euler/charm_model.c:402-433), then the SSP-RK
-            // combine ca * U0 + cb * W when no limiter sits in between
-            const R vinv = rcp(h * h * h);
+            // c -= dt * A^-1 int (src/models/euler/charm_model.c:402-433); A^-1
+            // of a level-l cube is the FP64 level-0 inverse times 8^l (exact
+            // power-of-two scaling), diagonal on the cube octree; then the
+            // SSP-RK combine when no limiter sits in between
+            const R sc = ldexp(R(1), 3 * lev);
             const R dt = R(A.dt);
             const R *__restrict__ U0 = reinterpret_cast<const R *>(A.U0);
             #pragma unroll
             for (int v = 0; v < NV; v++)
                 #pragma unroll
                 for (int k = 0; k < 4; k++) {
-                    const R ai = (k == 0) ? vinv : R(12) * vinv;
+                    const R ai = cst<R>(P.ainv0[k], P.ainv0_f[k]) * sc;
                     const R w = u[4*v+k] - dt * (ai * res[4*v+k]);
                     (out + (4*v+k) * n)[e] = A.combine ? rk_combine<R>(A.combine, w, (U0 + (4*v+k) * n)[e]) : w;
                 }

@@ -126,6 +126,11 @@ int charm_mesh_get_faces(const charm_mesh_t *m, int32_t *out);
 /* limiter neighbour table: [n][6] */
 int charm_mesh_get_neighbors(const charm_mesh_t *m, int32_t *out);
 
+/* FP64 Taylor-basis mass matrix of a level-`level` cube by the 8-point
+ * Gauss rule and its Gauss-Jordan inverse (charm_geom_quad_calc,
+ * src/charm_geom.c:496-530; charm_matr_inv, src/charm_globals.c:285-394) */
+int charm_level_mass_inverse(int level, double a[4][4], double ainv[4][4]);
+
 /* ------------------------------------------------------------ device */
 typedef struct charm_gpu_ctx charm_gpu_ctx_t;
 

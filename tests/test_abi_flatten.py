@@ -150,3 +150,17 @@ def test_errors_are_reported_not_fatal():
     f = CA.Forest.from_leaves(unbalanced)
     with pytest.raises(CA.CharmError, match="balanced"):
         f.flatten()
+
+
+@pytest.mark.parametrize("level", [0, 3, 7, 12])
+def test_level_mass_inverse_matches_oracle(level):
+    a, ai = CA.level_mass_inverse(level)
+    h = 2.0 ** -level
+    assert np.allclose(a @ ai, np.eye(4), atol=1e-14)
+    assert np.allclose(ai, np.diag([1, 12, 12, 12]) / h ** 3, rtol=1e-14, atol=1e-14 / h ** 3)
+    # power-of-two scaling of the level-0 inverse is exact
+    assert np.array_equal(ai, CA.level_mass_inverse(0)[1] * 8.0 ** level)
+    if level <= 3:
+        leaves = MU.uniform_leaves(level)
+        g = OL.OracleSim(OL.make_config(), leaves).geom(len(leaves) // 2)
+        assert np.allclose(g["a_inv"], ai, rtol=1e-13, atol=1e-13 * ai.max())
