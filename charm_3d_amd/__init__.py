@@ -40,7 +40,9 @@ class Config(C.Structure):
                 ("comp", CompDesc * MAX_COMP), ("bnd_type", C.c_int * 6),
                 ("bnd_param", (C.c_double * 16) * 6), ("grav", C.c_double * 3),
                 ("cfl", C.c_double), ("dt", C.c_double), ("tmax", C.c_double),
-                ("log_period", C.c_int), ("chunk_elems", C.c_int), ("chunk_faces", C.c_int)]
+                ("log_period", C.c_int), ("chunk_elems", C.c_int), ("chunk_faces", C.c_int),
+                ("max_err", C.c_double), ("min_level", C.c_int), ("max_level", C.c_int),
+                ("refine_period", C.c_int)]
 
 
 _lib = None
@@ -93,6 +95,11 @@ def lib():
         "charm_gpu_synchronize": (C.c_int, [vp]),
         "charm_gpu_last_timing": (C.c_int, [vp, dp, dp]),
         "charm_gpu_flux_batch": (C.c_int, [vp, C.c_int, i64, dp, dp, dp, dp]),
+        "charm_gpu_amr_indicator": (C.c_int, [vp, dp, dp, i64]),
+        "charm_gpu_remesh": (C.c_int, [vp, vp, C.POINTER(C.c_int64), i64]),
+        "charm_amr_flags": (C.c_int, [i64, i32p, dp, dp, C.c_double, C.c_int, C.c_int,
+                                      C.POINTER(C.c_int8)]),
+        "charm_adapt": (None, [vp, vp, vp]),
         "charm_ctx_create": (C.c_int, [vp, C.POINTER(Config), C.POINTER(vp)]),
         "charm_ctx_destroy": (C.c_int, [vp]),
         "charm_ctx_gpu": (vp, [vp]),
@@ -133,7 +140,8 @@ def _periodic(per):
 
 def make_config(precision=FP32, model="NS", flux="HLLC", limiter=False, eos="IDEAL",
                 comp_m=(GAS_R,), comp_cp=(3.5,), bnd=("OUTLET",) * 6, bnd_param=None,
-                grav=(0.0, 0.0, 0.0), cfl=0.3, dt=0.0, tmax=0.0, chunk_elems=0, chunk_faces=0):
+                grav=(0.0, 0.0, 0.0), cfl=0.3, dt=0.0, tmax=0.0, chunk_elems=0, chunk_faces=0,
+                max_err=0.0, min_level=0, max_level=0, refine_period=0):
     """charm_config_t from task.yaml-like keys (constant cp per species)."""
     cfg = Config()
     cfg.precision = precision
@@ -155,6 +163,8 @@ def make_config(precision=FP32, model="NS", flux="HLLC", limiter=False, eos="IDE
         cfg.grav[i] = grav[i]
     cfg.cfl, cfg.dt, cfg.tmax = cfl, dt, tmax
     cfg.chunk_elems, cfg.chunk_faces = chunk_elems, chunk_faces
+    cfg.max_err, cfg.min_level, cfg.max_level = max_err, min_level, max_level
+    cfg.refine_period = refine_period
     return cfg
 
 
@@ -338,6 +348,78 @@ class GpuContext:
         check(lib().charm_gpu_flux_batch(self.h, FLUX[kind], n, _dptr(pl), _dptr(pr),
                                          _dptr(nn), _dptr(out)))
         return out
+
+
+    def amr_indicator(self):
+        """(err2, rho_c) per leaf: the adaptation's FP64 error estimate."""
+        err2, rho_c = np.zeros(self.n), np.zeros(self.n)
+        check(lib().charm_gpu_amr_indicator(self.h, _dptr(err2), _dptr(rho_c), self.n))
+        return err2, rho_c
+
+    def remesh(self, mesh, amap):
+        """Upload the adapted mesh and project the state with the old->new map."""
+        mp = np.ascontiguousarray(amap, dtype=np.int64)
+        check(lib().charm_gpu_remesh(self.h, mesh.h, mp.ctypes.data_as(C.POINTER(C.c_int64)), len(mp)))
+        self.n = mesh.n
+
+
+def amr_flags(leaves, err2, rho_c, max_err, min_level, max_level):
+    """Refine / coarsen / keep flags of the reference's criteria (host C)."""
+    lv = np.ascontiguousarray(leaves, dtype=np.int32)
+    e2 = np.ascontiguousarray(err2, dtype=np.float64)
+    rc = np.ascontiguousarray(rho_c, dtype=np.float64)
+    out = np.zeros(len(lv), dtype=np.int8)
+    check(lib().charm_amr_flags(len(lv), _i32ptr(lv), _dptr(e2), _dptr(rc), max_err, min_level,
+                                max_level, out.ctypes.data_as(C.POINTER(C.c_int8))))
+    return out
+
+
+class Solver:
+    """charm_ctx_t over a forest: the reference-named entry points
+    (charm_timesteps, charm_model_*_timestep_single, charm_adapt)."""
+
+    def __init__(self, forest, cfg):
+        self.forest, self.cfg = forest, cfg
+        self.nv = 4 + cfg.n_comp
+        self.h = C.c_void_p()
+        check(lib().charm_ctx_create(forest.h, C.byref(cfg), C.byref(self.h)))
+        self.gpu = lib().charm_ctx_gpu(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().charm_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self):
+        msg = lib().charm_gpu_last_error()
+        return msg.decode() if msg else ""
+
+    def set_state(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        assert u.shape == (self.forest.n, self.nv, 4), u.shape
+        check(lib().charm_gpu_set_state(self.gpu, _dptr(u), len(u)))
+
+    def get_state(self):
+        out = np.zeros((self.forest.n, self.nv, 4))
+        check(lib().charm_gpu_get_state(self.gpu, _dptr(out), len(out)))
+        return out
+
+    def timestep(self):
+        """One charm_model_{euler,ns}_timestep_single; returns dt."""
+        dt = C.c_double()
+        fn = (lib().charm_model_euler_timestep_single if self.cfg.model == MODEL["EULER"]
+              else lib().charm_model_ns_timestep_single)
+        fn(self.forest.h, C.byref(dt), None, None)
+        return dt.value
+
+    def adapt(self):
+        lib().charm_adapt(self.forest.h, None, None)
+
+    def timesteps(self):
+        lib().charm_timesteps(self.forest.h)
 
 
 def device_count():

@@ -14,8 +14,8 @@ INC = os.path.join(ROOT, "include")
 OBJ = os.path.join(ROOT, "_build", "charm3d")
 LIB = os.path.join(PKG, "libcharm3d.so")
 
-C_SOURCES = ["charm_forest.c", "charm_geom.c", "charm_model.c"]
-HIP_SOURCES = ["charm_kernels.hip", "charm_gpu.hip"]
+C_SOURCES = ["charm_forest.c", "charm_geom.c", "charm_model.c", "charm_amr.c"]
+HIP_SOURCES = ["charm_kernels.hip", "charm_gpu.hip", "charm_amr.hip"]
 ARCH = os.environ.get("CHARM_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -33,6 +33,8 @@ struct charm_gpu_ctx {
     int4 *d_cf4, *d_aux;
     int *d_ch_ebeg, *d_ch_fbeg, *d_slot_ptr, *d_nbr;
     uint16_t *d_slots;
+    int4 *d_q;                // leaves (x, y, z, level): adaptation
+    int32_t *hq;              // host copy of the leaves of the uploaded mesh
     // state
     void *B[3];
     double *d_stage64;        // FP64 staging for host transfers
@@ -63,9 +65,11 @@ static void free_mesh(charm_gpu_ctx_t *c)
     void **p[] = {(void **) &c->d_level, (void **) &c->d_cf4, (void **) &c->d_aux,
                   (void **) &c->d_ch_ebeg, (void **) &c->d_ch_fbeg, (void **) &c->d_slot_ptr,
                   (void **) &c->d_nbr, (void **) &c->d_slots, &c->B[0], &c->B[1], &c->B[2],
-                  (void **) &c->d_stage64, (void **) &c->d_partial};
+                  (void **) &c->d_stage64, (void **) &c->d_partial, (void **) &c->d_q};
     for (size_t i = 0; i < sizeof(p) / sizeof(p[0]); i++)
         if (*p[i]) { (void) hipFree(*p[i]); *p[i] = NULL; }
+    free(c->hq);
+    c->hq = NULL;
     c->n = 0;
 }
 
@@ -210,6 +214,11 @@ extern "C" int charm_gpu_upload_mesh(charm_gpu_ctx_t *c, const charm_mesh_t *m)
     for (int i = 0; i < 3; i++) DMALLOC(c->B[i], ssz);
     DMALLOC(c->d_stage64, (size_t) m->n * 4 * c->nv * sizeof(double));
     DMALLOC(c->d_partial, (size_t) ((m->n + 255) / 256 + 1) * (c->nv + 1) * sizeof(double));
+    DMALLOC(c->d_q, (size_t) m->n * sizeof(int4));
+    c->hq = (int32_t *) malloc((size_t) (m->n ? m->n : 1) * 4 * sizeof(int32_t));
+    if (!c->hq) { charm_set_error("upload_mesh: out of memory"); free_mesh(c); return CHARM_ERR_MEM; }
+    memcpy(c->hq, m->q, (size_t) m->n * 4 * sizeof(int32_t));
+    HIPCHK(hipMemcpyAsync(c->d_q, m->q, (size_t) m->n * sizeof(int4), hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(c->d_level, m->level, (size_t) m->n, hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(c->d_cf4, m->cf4, (size_t) m->ncf * sizeof(int4), hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(c->d_ch_ebeg, m->ch_ebeg, (size_t) (m->nch + 1) * sizeof(int), hipMemcpyHostToDevice, c->st));
@@ -529,5 +538,107 @@ extern "C" int charm_gpu_flux_batch(charm_gpu_ctx_t *c, int flux, int64_t n, con
     if (e == hipSuccess) e = hipStreamSynchronize(c->st);
     (void) hipFree(dl); (void) hipFree(dr); (void) hipFree(dn); (void) hipFree(dq);
     if (e != hipSuccess) { charm_set_error("flux_batch: %s", hipGetErrorString(e)); return CHARM_ERR_HIP; }
+    return CHARM_OK;
+}
+
+// ------------------------------------------------------------ adaptation
+
+extern "C" int charm_gpu_amr_indicator(charm_gpu_ctx_t *c, double *err2, double *rho_c, int64_t n)
+{
+    int rc = need_mesh(c, n);
+    if (rc) return rc;
+    if (!err2 || !rho_c) { charm_set_error("amr_indicator: null output"); return CHARM_ERR_ARG; }
+    AmrArgs A;
+    A.U = c->B[0]; A.level = c->d_level; A.nbr = c->d_nbr; A.aux = c->d_aux;
+    A.err2 = c->d_stage64; A.rho_c = c->d_stage64 + n; A.n = n;
+    HIPCHK(launch_amr_indicator(c->prec, c->nc, A, c->P, c->st));
+    HIPCHK(hipMemcpyAsync(err2, A.err2, (size_t) n * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(rho_c, A.rho_c, (size_t) n * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return CHARM_OK;
+}
+
+// leaf q (x, y, z, level) lies inside leaf p
+static bool leaf_inside(const int32_t *q, const int32_t *p)
+{
+    if (q[3] < p[3] || p[3] < 0 || q[3] > CHARM_MAXLEVEL) return false;
+    const int32_t len = CHARM_ROOT_LEN >> p[3];
+    for (int d = 0; d < 3; d++)
+        if (q[d] < p[d] || q[d] >= p[d] + len) return false;
+    return true;
+}
+
+// every index the projection kernel reads is checked here, on the host
+static int check_map(const int32_t *oq, int64_t n_old, const int32_t *nq, int64_t n_new, const int64_t *map)
+{
+    for (int64_t i = 0; i < n_new; i++) {
+        const int64_t m = map[i];
+        if (m >= 0) {
+            if (m >= n_old || !leaf_inside(&nq[4 * i], &oq[4 * m])) {
+                charm_set_error("remesh: map[%lld] = %lld does not contain the new leaf", (long long) i, (long long) m);
+                return CHARM_ERR_ARG;
+            }
+            continue;
+        }
+        const int64_t f = -1 - m;
+        if (f + 8 > n_old || oq[4 * f + 3] < 1) {
+            charm_set_error("remesh: map[%lld] names no family of 8 old leaves", (long long) i);
+            return CHARM_ERR_ARG;
+        }
+        const int32_t par[4] = {oq[4 * f], oq[4 * f + 1], oq[4 * f + 2], oq[4 * f + 3] - 1};
+        const int32_t plen = CHARM_ROOT_LEN >> par[3];
+        bool ok = par[0] % plen == 0 && par[1] % plen == 0 && par[2] % plen == 0 && leaf_inside(&nq[4 * i], par);
+        int seen = 0;
+        for (int ch = 0; ok && ch < 8; ch++) {
+            const int32_t *cq = &oq[4 * (f + ch)];
+            ok = cq[3] == par[3] + 1 && leaf_inside(cq, par);
+            if (ok) {
+                const int32_t h = plen / 2;
+                const int bit = ((cq[0] - par[0]) / h) | (((cq[1] - par[1]) / h) << 1) | (((cq[2] - par[2]) / h) << 2);
+                ok = !(seen & (1 << bit));
+                seen |= 1 << bit;
+            }
+        }
+        if (!ok) {
+            charm_set_error("remesh: old leaves %lld..%lld are not the family of the new leaf %lld",
+                            (long long) f, (long long) (f + 7), (long long) i);
+            return CHARM_ERR_ARG;
+        }
+    }
+    return CHARM_OK;
+}
+
+extern "C" int charm_gpu_remesh(charm_gpu_ctx_t *c, const charm_mesh_t *m, const int64_t *map, int64_t n_map)
+{
+    int rc = need_mesh(c, -1);
+    if (rc) return rc;
+    if (!m || !map || n_map != m->n) { charm_set_error("remesh: map length must equal the new leaf count"); return CHARM_ERR_ARG; }
+    if ((rc = check_map(c->hq, c->n, m->q, m->n, map))) return rc;
+    HIPCHK(hipStreamSynchronize(c->st));
+    // keep the old state and leaves while the new mesh is uploaded
+    void *u_old = c->B[0];
+    int4 *q_old = c->d_q;
+    const int64_t n_old = c->n;
+    c->B[0] = NULL; c->d_q = NULL;
+    int64_t *d_map = NULL;
+    hipError_t e = hipSuccess;
+    rc = charm_gpu_upload_mesh(c, m);
+    if (!rc) {
+        e = hipMalloc((void **) &d_map, (size_t) (m->n ? m->n : 1) * sizeof(int64_t));
+        if (e == hipSuccess) e = hipMemcpyAsync(d_map, map, (size_t) m->n * sizeof(int64_t), hipMemcpyHostToDevice, c->st);
+        if (e == hipSuccess) {
+            ProjArgs A;
+            A.Uold = u_old; A.Unew = c->B[0]; A.old_q = q_old; A.new_q = c->d_q; A.map = d_map;
+            A.n_old = n_old; A.n_new = m->n; A.nvk = 4 * c->nv;
+            e = launch_amr_project(c->prec, A, c->st);
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    }
+    (void) hipStreamSynchronize(c->st);
+    if (d_map) (void) hipFree(d_map);
+    (void) hipFree(u_old);
+    (void) hipFree(q_old);
+    if (rc) return rc;
+    if (e != hipSuccess) { charm_set_error("remesh: %s", hipGetErrorString(e)); return CHARM_ERR_HIP; }
     return CHARM_OK;
 }

@@ -51,6 +51,18 @@ struct charm_mesh {
     uint16_t *slots;        /* local face index << 1 | side */
 };
 
+/* the forest's user pointer (the reference's charm_ctx_t) */
+struct charm_ctx {
+    charm_config_t   cfg;
+    charm_forest_t  *forest;
+    charm_gpu_ctx_t *gpu;
+    int              chunk_elems, chunk_faces;   /* flattening of every (re)upload */
+    double           t;          /* ctx->t */
+    int              timestep;   /* ctx->timestep */
+    int              err;
+};
+
+charm_ctx_t *charm_ctx_of(charm_forest_t *f);
 uint64_t charm_morton(int32_t x, int32_t y, int32_t z);
 int64_t  charm_forest_find(const charm_forest_t *f, int32_t x, int32_t y, int32_t z);
 void     charm_set_error(const char *fmt, ...);

@@ -49,12 +49,36 @@ struct FluxArgs {
     int64_t n;
 };
 
+// density-gradient error estimator of the adaptation (one thread per leaf)
+struct AmrArgs {
+    const void *U;
+    const uint8_t *level;
+    const int *nbr;
+    const int4 *aux;
+    double *err2, *rho_c;     // [n] each
+    int64_t n;
+};
+
+// projection of the state onto a new leaf set: map[i] >= 0 names the old
+// leaf that contains new leaf i, map[i] = -1 - f names the coarsened family
+// whose first member is old leaf f (new leaf i is that parent or inside it)
+struct ProjArgs {
+    const void *Uold;
+    void *Unew;
+    const int4 *old_q, *new_q;   // (x, y, z, level) in root units
+    const int64_t *map;
+    int64_t n_old, n_new;
+    int nvk;                     // 4 V coefficients per leaf
+};
+
 hipError_t launch_stage(int prec, int nc, int flux, const StageArgs &A, const DevParams &P, hipStream_t st);
 size_t stage_smem_bytes(int prec, int nc, int max_faces);
 hipError_t launch_limiter(int prec, int nc, const LimArgs &A, const DevParams &P, hipStream_t st);
 hipError_t launch_reduce(int prec, int nc, const RedArgs &A, const DevParams &P, hipStream_t st);
 hipError_t launch_flux_points(int prec, int nc, int flux, const FluxArgs &A, const DevParams &P, hipStream_t st);
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st);
+hipError_t launch_amr_indicator(int prec, int nc, const AmrArgs &A, const DevParams &P, hipStream_t st);
+hipError_t launch_amr_project(int prec, const ProjArgs &A, hipStream_t st);
 hipError_t launch_convert(int prec, int to_f64, const void *src, void *dst, int64_t n, int nvk, hipStream_t st);
 
 } // namespace charm

@@ -22,24 +22,6 @@
 
 namespace charm {
 
-// Tangential Gauss-point offsets of the six-point face rule (two triangles
-// (v0,v1,v2), (v1,v2,v3) of the face, barycentric (2/3,1/6,1/6)
-// permutations, src/charm_geom.c:443-494) relative to the face centre in
-// units of the element edge; t1 < t2 are the two tangential axes.  Literal
-// switches so that the unrolled loops fold them into immediates.
-template <typename R>
-__device__ __forceinline__ R gp_t1(int k)
-{
-    switch (k) { case 0: return R(-1.0/3); case 1: return R(1.0/6); case 2: return R(-1.0/3);
-                 case 3: return R(1.0/3); case 4: return R(-1.0/6); default: return R(1.0/3); }
-}
-template <typename R>
-__device__ __forceinline__ R gp_t2(int k)
-{
-    switch (k) { case 0: return R(-1.0/3); case 1: return R(-1.0/3); case 2: return R(1.0/6);
-                 case 3: return R(-1.0/6); case 4: return R(1.0/3); default: return R(1.0/3); }
-}
-
 // Workgroups are dispatched round-robin over the XCDs (each with its own
 // L2): workgroup b runs on XCD b % n_xcd.  Renumber so that every XCD walks a
 // contiguous range of Morton-ordered chunks / element blocks and neighbours

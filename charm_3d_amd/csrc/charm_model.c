@@ -12,15 +12,6 @@
 #include <string.h>
 #include "charm_internal.h"
 
-struct charm_ctx {
-    charm_config_t   cfg;
-    charm_forest_t  *forest;
-    charm_gpu_ctx_t *gpu;
-    double           t;          /* ctx->t */
-    int              timestep;   /* ctx->timestep */
-    int              err;
-};
-
 int charm_ctx_create(charm_forest_t *f, const charm_config_t *cfg, charm_ctx_t **out)
 {
     charm_ctx_t *c;
@@ -43,6 +34,8 @@ int charm_ctx_create(charm_forest_t *f, const charm_config_t *cfg, charm_ctx_t *
         return rc;
     }
     charm_mesh_destroy(m);
+    c->chunk_elems = ce;
+    c->chunk_faces = cf;
     charm_forest_set_user_pointer(f, c);
     *out = c;
     return CHARM_OK;
@@ -60,7 +53,7 @@ int charm_ctx_destroy(charm_ctx_t *c)
 
 charm_gpu_ctx_t *charm_ctx_gpu(charm_ctx_t *c) { return c ? c->gpu : NULL; }
 
-static charm_ctx_t *ctx_of(charm_forest_t *p4est)
+charm_ctx_t *charm_ctx_of(charm_forest_t *p4est)
 {
     charm_ctx_t *c = (charm_ctx_t *) charm_forest_user_pointer(p4est);
     if (!c) charm_set_error("forest has no charm_ctx_t user pointer");
@@ -72,28 +65,34 @@ static charm_ctx_t *ctx_of(charm_forest_t *p4est)
  * src/models/ns/charm_model.c:43-60); charm_gpu_get_dt exposes it */
 charm_real_t charm_model_euler_get_dt(charm_forest_t *p4est)
 {
-    charm_ctx_t *c = ctx_of(p4est);
+    charm_ctx_t *c = charm_ctx_of(p4est);
     return c ? c->cfg.dt : 0.;
 }
 
 charm_real_t charm_model_ns_get_dt(charm_forest_t *p4est)
 {
-    charm_ctx_t *c = ctx_of(p4est);
+    charm_ctx_t *c = charm_ctx_of(p4est);
     return c ? c->cfg.dt : 0.;
 }
 
 static void timestep(charm_forest_t *p4est, int model, charm_real_t *dt)
 {
-    charm_ctx_t *c = ctx_of(p4est);
+    charm_ctx_t *c = charm_ctx_of(p4est);
     int rc;
     if (!c || !dt) return;
+    /* adapt when REFINE_PERIOD divides a nonzero step number, then take dt
+     * (src/models/euler/charm_model.c:468-490, src/models/ns/charm_model.c:199-215) */
+    if (c->cfg.refine_period > 0 && c->timestep > 0 && c->timestep % c->cfg.refine_period == 0) {
+        charm_adapt(p4est, NULL, NULL);
+        if (c->err) return;
+    }
     *dt = model == CHARM_MODEL_EULER ? charm_model_euler_get_dt(p4est) : charm_model_ns_get_dt(p4est);
     rc = charm_gpu_step(c->gpu, model, *dt, 1);
     if (rc) c->err = rc;
 }
 
-/* src/models/euler/charm_model.c:454-540 (adapt / partition / write hooks
- * are outside this path) */
+/* src/models/euler/charm_model.c:454-540 (the partition and write hooks are
+ * outside this path) */
 void charm_model_euler_timestep_single(charm_forest_t *p4est, charm_real_t *dt,
                                        charm_ghost_t **ghost, charm_data_t **ghost_data)
 {
@@ -113,7 +112,7 @@ void charm_model_ns_timestep_single(charm_forest_t *p4est, charm_real_t *dt,
  * and logs every LOG_OUTPUT_STEP steps */
 void charm_timesteps(charm_forest_t *p4est)
 {
-    charm_ctx_t *c = ctx_of(p4est);
+    charm_ctx_t *c = charm_ctx_of(p4est);
     charm_real_t dt;
     if (!c) return;
     dt = c->cfg.model == CHARM_MODEL_EULER ? charm_model_euler_get_dt(p4est) : charm_model_ns_get_dt(p4est);
@@ -133,13 +132,13 @@ void charm_timesteps(charm_forest_t *p4est)
 /* src/limiters/charm_limiter.c:9-16 */
 void charm_limiter(charm_forest_t *p4est, charm_ghost_t *ghost, charm_data_t *ghost_data)
 {
-    charm_ctx_t *c = ctx_of(p4est);
+    charm_ctx_t *c = charm_ctx_of(p4est);
     if (c && c->cfg.limiter) charm_limiter_bj(p4est, ghost, ghost_data);
 }
 
 void charm_limiter_bj(charm_forest_t *p4est, charm_ghost_t *ghost, charm_data_t *ghost_data)
 {
-    charm_ctx_t *c = ctx_of(p4est);
+    charm_ctx_t *c = charm_ctx_of(p4est);
     int rc;
     (void) ghost; (void) ghost_data;
     if (!c) return;
@@ -153,7 +152,7 @@ static void flux_point(charm_forest_t *p4est, int kind, charm_prim_t prim[2], ch
                        charm_real_t *qv, charm_real_t *qw, charm_real_t *qe, charm_real_t qc[],
                        charm_real_t n[3])
 {
-    charm_ctx_t *c = ctx_of(p4est);
+    charm_ctx_t *c = charm_ctx_of(p4est);
     double row[2][5 + CHARM_MAX_COMP], q[4 + CHARM_MAX_COMP];
     int s, i, nc;
     if (!c) return;

@@ -87,6 +87,11 @@ typedef struct {
     int    log_period;         /* LOG_OUTPUT_STEP (0: no log) */
     int    chunk_elems;        /* elements per workgroup chunk (0: default) */
     int    chunk_faces;        /* max faces per chunk (0: default) */
+    /* adaptation (src/charm_init_yaml.cpp:383-387) */
+    double max_err;            /* MAX_ERROR */
+    int    min_level;          /* MIN_LEVEL */
+    int    max_level;          /* MAX_LEVEL */
+    int    refine_period;      /* REFINE_PERIOD (0: no adaptation) */
 } charm_config_t;
 
 /* ------------------------------------------------------------ forest */
@@ -176,13 +181,32 @@ int charm_gpu_synchronize(charm_gpu_ctx_t *ctx);
 /* device time of the last charm_gpu_run, split per kernel family (ms) */
 int charm_gpu_last_timing(charm_gpu_ctx_t *ctx, double *stage_ms, double *limiter_ms);
 
+/* adaptation, device side.  amr_indicator: per leaf the FP64 interpolation
+ * error estimate err2 = |grad rho|^2 / 12 V^(5/3) of the Green-Gauss density
+ * gradient and the cell-mean density rho_c (src/charm_amr.c:12-30,350-599).
+ * remesh: uploads the mesh of the adapted forest and projects the state
+ * onto it in FP64 (charm_replace_quads, src/charm_amr.c:206-307); `map` is
+ * the old->new map of charm_forest_adapt and is checked against the leaves
+ * of the previous mesh before any kernel runs. */
+int charm_gpu_amr_indicator(charm_gpu_ctx_t *ctx, double *err2, double *rho_c, int64_t n);
+int charm_gpu_remesh(charm_gpu_ctx_t *ctx, const charm_mesh_t *mesh, const int64_t *map, int64_t n_map);
+
+/* refine (+1) / coarsen (-1 on all eight members of a family) / keep (0)
+ * decisions from the estimator, with the criteria of
+ * charm_refine_err_estimate / charm_coarsen_err_estimate
+ * (src/charm_amr.c:32-48,90-142) and p4est_refine_ext's max_level cap.
+ * leaves [n][4] in Morton order (charm_forest_get_leaves). */
+int charm_amr_flags(int64_t n, const int32_t *leaves, const double *err2, const double *rho_c,
+                    double max_err, int min_level, int max_level, int8_t *flags);
+
 /* pointwise flux on the device for a batch of point pairs: prim rows are
  * (r, u, v, w, p, c[0..n_comp-1]); normals [n][3]; out [n][4 + n_comp] */
 int charm_gpu_flux_batch(charm_gpu_ctx_t *ctx, int flux, int64_t n, const double *prim_l,
                          const double *prim_r, const double *normals, double *out);
 
 /* ------------------------------------------------ reference entry points
- * (include/charm_models.h:15-23, charm_limiter.h:13-15, charm_fluxes.h:19-32)
+ * (include/charm_models.h:15-23, charm_limiter.h:13-15, charm_fluxes.h:19-32,
+ * charm_amr.h:9-10)
  * The forest's user pointer is a charm_ctx_t built by charm_ctx_create. */
 typedef struct charm_ctx charm_ctx_t;
 typedef struct charm_ghost charm_ghost_t;     /* p4est_ghost_t: single rank, unused */
@@ -208,6 +232,11 @@ charm_real_t charm_model_ns_get_dt(charm_forest_t *p4est);
 /* runs steps until t >= TMAX like src/charm_timesteps.c:23-51 */
 void charm_timesteps(charm_forest_t *p4est);
 void charm_limiter(charm_forest_t *p4est, charm_ghost_t *ghost, charm_data_t *ghost_data);
+/* one adaptation of the forest and the state: estimator, flags, refine /
+ * coarsen / 2:1 face balance, re-flattening and FP64 projection
+ * (include/charm_amr.h:9-10, src/charm_amr.c:575-610).  The time-step entry
+ * points call it when REFINE_PERIOD divides a nonzero step number. */
+void charm_adapt(charm_forest_t *p4est, charm_ghost_t *ghost, charm_data_t *ghost_data);
 void charm_limiter_bj(charm_forest_t *p4est, charm_ghost_t *ghost, charm_data_t *ghost_data);
 void charm_calc_flux_lf(charm_forest_t *p4est, charm_prim_t prim[2], charm_real_t *qu,
                         charm_real_t *qv, charm_real_t *qw, charm_real_t *qe,

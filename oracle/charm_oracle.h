@@ -121,6 +121,15 @@ void or_geom(const or_sim_t *s, int64_t leaf, double *out);
 void   or_conservation_sums(const or_sim_t *s, double *out);
 double or_cfl_dt(const or_sim_t *s, int mode);
 
+/* adaptation: estimator (err2, centre density per leaf), refine (+1) /
+ * coarsen (-1) flags, and the L2 projection onto a new leaf set given the
+ * new->old map (>= 0 source leaf, -1 - first child of a coarsened family) */
+void or_amr_indicator(or_sim_t *s, double *err2, double *rho_c);
+void or_amr_flags(const or_sim_t *s, const double *err2, const double *rho_c, double max_err,
+                  int min_level, int max_level, int8_t *flags);
+void or_amr_project(int nv, int64_t n_old, const int32_t *old_leaves, const double *u_old,
+                    int64_t n_new, const int32_t *new_leaves, const int64_t *map, double *u_new);
+
 /* pointwise numerics (batched for tests) */
 void or_cons_to_prim(const or_config_t *cfg, const or_cons_t *c, or_prim_t *p);
 void or_eos(const or_config_t *cfg, or_prim_t *p, int flag);

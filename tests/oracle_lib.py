@@ -93,6 +93,9 @@ def lib():
         L.or_bnd.argtypes = [C.POINTER(Config), C.c_int, C.POINTER(Prim), C.POINTER(Prim), dp]
         L.or_matr_inv.argtypes = [dp, dp]
         L.or_rim.argtypes = [dp] * 6 + [C.c_double] * 12
+        L.or_amr_indicator.argtypes = [vp, dp, dp]
+        L.or_amr_flags.argtypes = [vp, dp, dp, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int8)]
+        L.or_amr_project.argtypes = [C.c_int, i64, ip, dp, i64, ip, C.POINTER(C.c_int64), dp]
         L.or_gas_r.restype = C.c_double
         for fn in ("or_prim_size", "or_cons_size", "or_config_size"):
             getattr(L, fn).restype = C.c_size_t
@@ -204,6 +207,18 @@ class OracleSim:
             o += size
         return res
 
+    def amr_indicator(self):
+        err2, rho = np.zeros(self.n), np.zeros(self.n)
+        lib().or_amr_indicator(self.h, _dptr(err2), _dptr(rho))
+        self.check()
+        return err2, rho
+
+    def amr_flags(self, err2, rho, max_err, min_level, max_level):
+        fl = np.zeros(self.n, dtype=np.int8)
+        lib().or_amr_flags(self.h, _dptr(np.ascontiguousarray(err2)), _dptr(np.ascontiguousarray(rho)),
+                           max_err, min_level, max_level, fl.ctypes.data_as(C.POINTER(C.c_int8)))
+        return fl
+
     def conservation_sums(self):
         out = np.zeros(self.nv)
         lib().or_conservation_sums(self.h, _dptr(out))
@@ -260,3 +275,15 @@ def rim(rb, pb, ub, vb, wb, re, pe, ue, ve, we, gb, ge=None):
     outs = [C.c_double() for _ in range(6)]
     lib().or_rim(*[C.byref(o) for o in outs], rb, pb, ub, vb, wb, re, pe, ue, ve, we, gb, ge)
     return [o.value for o in outs]   # RI, EI, PI, UI, VI, WI
+
+
+def amr_project(nv, old_leaves, u_old, new_leaves, mp):
+    ol = np.ascontiguousarray(old_leaves, dtype=np.int32)
+    nl = np.ascontiguousarray(new_leaves, dtype=np.int32)
+    uo = np.ascontiguousarray(u_old, dtype=np.float64)
+    mp = np.ascontiguousarray(mp, dtype=np.int64)
+    out = np.zeros((len(nl), nv, 4))
+    lib().or_amr_project(nv, len(ol), ol.ctypes.data_as(C.POINTER(C.c_int32)), _dptr(uo), len(nl),
+                         nl.ctypes.data_as(C.POINTER(C.c_int32)), mp.ctypes.data_as(C.POINTER(C.c_int64)),
+                         _dptr(out))
+    return out

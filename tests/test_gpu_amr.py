@@ -1,0 +1,175 @@
+"""GPU parity of the adaptation: the device estimator, the FP64 projection
+onto the adapted leaves and whole runs with REFINE_PERIOD, each against the
+oracle (FP64 device path to 1e-12 relative, as the residual)."""
+import numpy as np
+import pytest
+
+import charm_3d_amd as CA
+import mesh_util as MU
+import oracle_lib as OL
+from test_gpu_parity import device, oracle_cfg, random_state
+
+pytestmark = pytest.mark.gpu
+
+
+def rand_flags(lv, rng, pref=0.15, pco=0.5):
+    """Random refine flags and whole-family coarsen flags (Morton order)."""
+    n = len(lv)
+    fl = np.zeros(n, np.int8)
+    fl[rng.random(n) < pref] = 1
+    i = 0
+    while i + 8 <= n:
+        q, lev = lv[i], lv[i, 3]
+        if lev >= 1:
+            plen = MU.ROOT >> (lev - 1)
+            if (q[0] % plen == 0 and q[1] % plen == 0 and q[2] % plen == 0
+                    and np.all(lv[i:i + 8, 3] == lev) and np.all(fl[i:i + 8] == 0)):
+                if rng.random() < pco:
+                    fl[i:i + 8] = -1
+                i += 8
+                continue
+        i += 1
+    return fl
+
+
+MESHES = {
+    "adapted_outlet": dict(base=2, top=4, seed=11, bnd=("OUTLET",) * 6, params=None, per=(False,) * 3),
+    "adapted_walls_inlet": dict(base=2, top=4, seed=12,
+                                bnd=("INLET", "OUTLET", "WALL_SLIP", "WALL_NO_SLIP", "WALL_SLIP", "OUTLET"),
+                                params=[[0.3, 0.1, -0.2, 1.1, 0.9], None, None, [1.2], None, None],
+                                per=(False,) * 3),
+    "adapted_periodic": dict(base=2, top=4, seed=13, bnd=("PERIODIC",) * 6, params=None, per=(True,) * 3),
+}
+
+
+@pytest.mark.parametrize("name", sorted(MESHES))
+@pytest.mark.parametrize("nc", [1, 2])
+def test_indicator_matches_oracle(name, nc):
+    m = MESHES[name]
+    lv = MU.random_adapted(m["base"], m["top"], seed=m["seed"], frac=0.3, periodic=m["per"])
+    rng = np.random.default_rng(m["seed"] + nc)
+    u0 = random_state(lv, rng, amp=0.2, nc=nc)
+    kw = dict(comp_m=(CA.GAS_R,) * nc, comp_cp=(3.5,) * nc) if nc == 1 else \
+        dict(comp_m=(0.029, 0.004), comp_cp=(1.4 * CA.GAS_R / (0.4 * 0.029), 1.67 * CA.GAS_R / (0.67 * 0.004)),
+             eos="MIX")
+    cfg = CA.make_config(precision=CA.FP64, bnd=m["bnd"], bnd_param=m["params"], **kw)
+    ctx = device(cfg, lv, m["per"], u0)
+    err2, rho_c = ctx.amr_indicator()
+    sim = OL.OracleSim(oracle_cfg(cfg), lv)
+    sim.set_state(u0)
+    e_ref, r_ref = sim.amr_indicator()
+    scale = MU.centers(lv)[1] ** 5                      # V^(5/3): compare |grad|^2 / 12
+    g, g_ref = err2 / scale, e_ref / scale
+    assert np.abs(g - g_ref).max() <= 1e-12 * g_ref.max(), np.abs(g - g_ref).max() / g_ref.max()
+    np.testing.assert_allclose(rho_c, r_ref, rtol=1e-14)
+    # a threshold between two data values (err2 > me^2 V on about half the
+    # leaves): the decision is then not a tie broken by rounding
+    vals = np.sort(e_ref / scale ** 0.6)
+    me = float(np.sqrt(0.5 * (vals[len(vals) // 2] + vals[len(vals) // 2 + 1])))
+    for lo, hi in ((0, 5), (3, 4)):
+        fl = CA.amr_flags(lv, err2, rho_c, me, lo, hi)
+        assert np.array_equal(fl, sim.amr_flags(e_ref, r_ref, me, lo, hi))
+    # FP32 state storage: the estimator still runs in FP64 arithmetic
+    cfg32 = CA.make_config(precision=CA.FP32, bnd=m["bnd"], bnd_param=m["params"], **kw)
+    e32, _ = device(cfg32, lv, m["per"], u0).amr_indicator()
+    assert np.abs(e32 / scale - g_ref).max() <= 1e-5 * g_ref.max()
+
+
+@pytest.mark.parametrize("seed", [0, 2, 7])
+@pytest.mark.parametrize("prec", [CA.FP64, CA.FP32])
+def test_remesh_projection_matches_oracle(seed, prec):
+    lv = MU.random_adapted(1, 4, seed=seed, frac=0.35)
+    rng = np.random.default_rng(100 + seed)
+    u0 = random_state(lv, rng, amp=0.2)
+    cfg = CA.make_config(precision=prec)
+    ctx = device(cfg, lv, (False,) * 3, u0)
+    u_old = ctx.get_state()                           # as stored (FP32 rounding included)
+    f = CA.Forest.from_leaves(lv)
+    mp = f.adapt(rand_flags(lv, rng), 5, 0)
+    new = f.leaves()
+    neg = mp < 0
+    assert (new[neg, 3] >= lv[-1 - mp[neg], 3]).any()   # coarsened parents refined again
+    ce, cf = CA.recommended_chunks(cfg)
+    ctx.remesh(f.flatten(ce, cf), mp)
+    got = ctx.get_state()
+    want = OL.amr_project(5, lv, u_old, new, mp)
+    tol = 1e-14 if prec == CA.FP64 else 2e-7
+    assert np.abs(got - want).max() <= tol * np.abs(want).max()
+    if prec == CA.FP64:
+        vol_o, vol_n = MU.centers(lv)[1] ** 3, MU.centers(new)[1] ** 3
+        np.testing.assert_allclose((vol_n[:, None] * got[:, :, 0]).sum(0),
+                                   (vol_o[:, None] * u_old[:, :, 0]).sum(0), rtol=1e-13)
+
+
+def test_remesh_rejects_bad_map():
+    lv = MU.uniform_leaves(2)
+    cfg = CA.make_config(precision=CA.FP64)
+    ctx = device(cfg, lv, (False,) * 3, random_state(lv, np.random.default_rng(0)))
+    f = CA.Forest.from_leaves(lv)
+    fl = np.zeros(len(lv), np.int8)
+    fl[5] = 1
+    mp = f.adapt(fl, 4, 0)
+    mesh = f.flatten(*CA.recommended_chunks(cfg))
+    bad = mp.copy()
+    bad[0] = len(lv) + 3
+    with pytest.raises(CA.CharmError, match="map"):
+        ctx.remesh(mesh, bad)
+    bad = mp.copy()
+    bad[1] = -1 - 60                                   # no family there
+    with pytest.raises(CA.CharmError, match="family"):
+        ctx.remesh(mesh, bad)
+    ctx.remesh(mesh, mp)                               # the context is still usable
+    assert ctx.get_state().shape == (len(mp), 5, 4)
+
+
+def bubble_case(level=3):
+    lv = MU.uniform_leaves(level)
+
+    def fn(x):
+        r = np.linalg.norm(x - np.array([0.45, 0.5, 0.55]), axis=1)
+        rho = 1.0 + 0.5 * np.tanh((0.22 - r) / 0.04)
+        out = np.zeros((len(x), 5))
+        out[:, 0] = 0.2 * rho
+        out[:, 3] = 1.0 / 0.4 + 0.5 * 0.04 * rho
+        out[:, 4] = rho
+        return out
+    return lv, MU.project_p1(lv, fn, 5)
+
+
+def test_adapt_run_matches_oracle():
+    """charm_timesteps with REFINE_PERIOD 2 (adapt before steps 2 and 4)
+    against the oracle driven through the same sequence."""
+    lv, u0 = bubble_case()
+    dt = 2e-3
+    cfg = CA.make_config(precision=CA.FP64, flux="HLLC", limiter=True, max_err=0.1,
+                         min_level=2, max_level=4, refine_period=2, dt=dt, tmax=5.5 * dt)
+    f = CA.Forest.from_leaves(lv)
+    s = CA.Solver(f, cfg)
+    s.set_state(u0)
+    s.timesteps()
+    got, leaves = s.get_state(), f.leaves()
+    s.close()
+
+    ocfg = oracle_cfg(cfg)
+    cur, u = lv, u0
+    fo = CA.Forest.from_leaves(lv)                   # the leaf set: product forest (CPU-tested)
+    sim = OL.OracleSim(ocfg, cur)
+    sim.set_state(u)
+    n_ref = n_coarse = 0
+    for step in range(6):
+        if step and step % 2 == 0:
+            e2, rc = sim.amr_indicator()
+            fl = sim.amr_flags(e2, rc, 0.1, 2, 4)
+            n_ref += int((fl == 1).sum())
+            n_coarse += int((fl == -1).sum())
+            mp = fo.adapt(fl, 4, 2)
+            new = fo.leaves()
+            u = OL.amr_project(5, cur, sim.get_state(), new, mp)
+            cur = new
+            sim = OL.OracleSim(ocfg, cur)
+            sim.set_state(u)
+        sim.run(dt, 1)
+    assert n_ref > 0 and n_coarse > 0
+    assert np.array_equal(leaves, cur)
+    want = sim.get_state()
+    assert np.abs(got - want).max() <= 1e-11 * np.abs(want).max()
