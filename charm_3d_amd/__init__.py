@@ -100,6 +100,7 @@ def lib():
         "charm_amr_flags": (C.c_int, [i64, i32p, dp, dp, C.c_double, C.c_int, C.c_int,
                                       C.POINTER(C.c_int8)]),
         "charm_adapt": (None, [vp, vp, vp]),
+        "charm_ctx_error": (C.c_int, [vp]),
         "charm_ctx_create": (C.c_int, [vp, C.POINTER(Config), C.POINTER(vp)]),
         "charm_ctx_destroy": (C.c_int, [vp]),
         "charm_ctx_gpu": (vp, [vp]),
@@ -393,10 +394,6 @@ class Solver:
     def __del__(self):
         self.close()
 
-    def _check(self):
-        msg = lib().charm_gpu_last_error()
-        return msg.decode() if msg else ""
-
     def set_state(self, u):
         u = np.ascontiguousarray(u, dtype=np.float64)
         assert u.shape == (self.forest.n, self.nv, 4), u.shape
@@ -405,6 +402,25 @@ class Solver:
     def get_state(self):
         out = np.zeros((self.forest.n, self.nv, 4))
         check(lib().charm_gpu_get_state(self.gpu, _dptr(out), len(out)))
+        return out
+
+    def run(self, dt, nsteps):
+        """nsteps device steps with a fixed dt (no adaptation)."""
+        check(lib().charm_gpu_run(self.gpu, dt, nsteps))
+
+    def last_ms(self):
+        a, b = C.c_double(), C.c_double()
+        check(lib().charm_gpu_last_timing(self.gpu, C.byref(a), C.byref(b)))
+        return a.value
+
+    def get_dt(self, mode=1):
+        dt = C.c_double()
+        check(lib().charm_gpu_get_dt(self.gpu, mode, C.byref(dt)))
+        return dt.value
+
+    def conservation_sums(self):
+        out = np.zeros(self.nv)
+        check(lib().charm_gpu_conservation_sums(self.gpu, _dptr(out)))
         return out
 
     def timestep(self):
@@ -416,10 +432,17 @@ class Solver:
         return dt.value
 
     def adapt(self):
+        """charm_adapt; raises if the adaptation failed."""
         lib().charm_adapt(self.forest.h, None, None)
+        err = lib().charm_ctx_error(self.h)
+        if err:
+            check(err)
 
     def timesteps(self):
         lib().charm_timesteps(self.forest.h)
+        err = lib().charm_ctx_error(self.h)
+        if err:
+            check(err)
 
 
 def device_count():

@@ -288,6 +288,13 @@ __device__ __forceinline__ void godunov_x(const Frame<R, NC> &Lf, const Frame<R,
     const R TB = (QB * R(0.5)), TE = (QE * R(0.5));
     const R eps = R(1.e-5);
     const R CB = dsqrt(dv(gb * PB, RB)), CE = dsqrt(dv(ge * PE, RE));
+    // The reference stops Newton when |DU - F1 - F2| <= 1e-5 (absolute, in
+    // velocity units).  FP64 keeps that test.  In FP32 the residual of
+    // states with speeds ~1e3 cannot get below a few ulp of the speed scale,
+    // so the test is max(1e-5, 8 ulp of (cB + cE + |uB| + |uE|)); Newton's
+    // quadratic convergence then ends in a few steps instead of the bound.
+    const R tol = sizeof(R) == 8 ? eps
+                                 : fmax(eps, R(8 * 1.1920929e-7) * (CB + CE + fabs(UB) + fabs(UE)));
     const R EB = dv(CB * CB, SB), EE = dv(CE * CE, SE);
     const R RCB = RB * CB, RCE = RE * CE;
     const R DU = UB - UE;
@@ -320,7 +327,7 @@ __device__ __forceinline__ void godunov_x(const Frame<R, NC> &Lf, const Frame<R,
                 FS2 = dv(ZFE, ge * P);
             }
             P = P + dv(DU - F1 - F2, FS1 + FS2);
-        } while ((fabs(DU - F1 - F2) > eps) && (++step < max_steps));
+        } while ((fabs(DU - F1 - F2) > tol) && (++step < max_steps));
         const R PPB = dv(P, PB), PPE = dv(P, PE);
         const R ZFB = CB * exp(log(PPB) * OB), ZFE = CE * exp(log(PPE) * OE);
         if (PB <= P) {

@@ -53,6 +53,15 @@ int charm_ctx_destroy(charm_ctx_t *c)
 
 charm_gpu_ctx_t *charm_ctx_gpu(charm_ctx_t *c) { return c ? c->gpu : NULL; }
 
+int charm_ctx_error(charm_ctx_t *c)
+{
+    int e;
+    if (!c) { charm_set_error("null context"); return CHARM_ERR_ARG; }
+    e = c->err;
+    c->err = CHARM_OK;
+    return e;
+}
+
 charm_ctx_t *charm_ctx_of(charm_forest_t *p4est)
 {
     charm_ctx_t *c = (charm_ctx_t *) charm_forest_user_pointer(p4est);

@@ -7,7 +7,7 @@ piecewise-constant regions (src/charm_init.c:46-68).
 """
 import numpy as np
 
-from . import GAS_R, ROOT_LEN, Forest, make_config
+from . import FP32, GAS_R, ROOT_LEN, Forest, GpuContext, make_config, recommended_chunks
 
 GAMMA = 1.4
 S3 = 1.0 / np.sqrt(3.0)
@@ -88,6 +88,82 @@ def c5_pulse(level=6, precision=1):
     cfg = make_config(precision=precision, model="NS", flux="HLLC", limiter=False,
                       bnd=("OUTLET",) * 6, comp_m=(GAS_R,), comp_cp=(3.5,), cfl=0.3)
     return cfg, leaves, project_p1(leaves, fn, 5), (False, False, False)
+
+
+# two-species bubble case: species A (gamma 1.4, M 0.029) with 16 bubbles of
+# species B (gamma 1.67, M 0.004); cp_i = gamma_i R / ((gamma_i - 1) M_i)
+C3_GAMMA = (1.4, 1.67)
+C3_M = (0.029, 0.004)
+C3_CP = tuple(g * GAS_R / ((g - 1.0) * m) for g, m in zip(C3_GAMMA, C3_M))
+
+
+def c3_bubbles(seed=3, n=16):
+    """Bubble centres U(0, 1)^3 and radii U(0.05, 0.15) (seeded)."""
+    rng = np.random.default_rng(seed)
+    return rng.uniform(0.0, 1.0, (n, 3)), rng.uniform(0.05, 0.15, n)
+
+
+def c3_state_fn(centres, radii, width, p=1.0e5, t=300.0, vel=(100.0, 50.0, 25.0)):
+    """Conserved variables (ru, rv, rw, re, rc_A, rc_B) at points: species-B mass
+    fraction 0.5 (1 + tanh((r_b - d_b) / width)), maximum over the bubbles,
+    uniform p, T and velocity; mixture EOS (cp = sum c_i cp_i,
+    M = 1 / sum c_i / M_i, cv = cp - R / M)."""
+    vel = np.asarray(vel, dtype=np.float64)
+
+    def fn(x):
+        cb = np.zeros(len(x))
+        for c, r in zip(centres, radii):
+            d = np.sqrt(((x - c) ** 2).sum(1))
+            cb = np.maximum(cb, 0.5 * (1.0 + np.tanh((r - d) / width)))
+        ca = 1.0 - cb
+        m = 1.0 / (ca / C3_M[0] + cb / C3_M[1])
+        rho = p * m / (GAS_R * t)
+        cv = ca * C3_CP[0] + cb * C3_CP[1] - GAS_R / m
+        out = np.empty((len(x), 6))
+        out[:, 0:3] = rho[:, None] * vel[None, :]
+        out[:, 3] = rho * (cv * t + 0.5 * (vel ** 2).sum())
+        out[:, 4] = rho * ca
+        out[:, 5] = rho * cb
+        return out
+    return fn
+
+
+def c3_config(precision=FP32, max_err=0.04, min_level=5, max_level=8, refine_period=20):
+    """Exact Riemann solver, mixture EOS, Barth-Jespersen, SSP-RK3, transmissive
+    boundaries, re-adaptation every `refine_period` steps."""
+    return make_config(precision=precision, model="NS", flux="GODUNOV", limiter=True, eos="MIX",
+                       comp_m=C3_M, comp_cp=C3_CP, bnd=("OUTLET",) * 6, cfl=0.3,
+                       max_err=max_err, min_level=min_level, max_level=max_level,
+                       refine_period=refine_period)
+
+
+def c3(precision=FP32, seed=3, min_level=5, max_level=8, max_err=0.04, width=None):
+    """Initial leaf set of the bubble case: from a uniform min_level forest,
+    refine (then 2:1 balance) every leaf whose device estimator exceeds the
+    adaptation threshold (err2 > MAX_ERROR^2 V, i.e. |grad rho| h >
+    sqrt(12) MAX_ERROR) on the projected initial state, one level per pass,
+    re-projecting the analytic state after each pass.  Needs a GPU.
+    Returns (cfg, forest, u0)."""
+    centres, radii = c3_bubbles(seed)
+    if width is None:
+        width = 3.0 / (1 << max_level)
+    fn = c3_state_fn(centres, radii, width)
+    cfg = c3_config(precision, max_err, min_level, max_level)
+    forest = Forest.uniform(min_level)
+    ce, cf = recommended_chunks(cfg)
+    for _ in range(max_level - min_level):
+        leaves = forest.leaves()
+        ctx = GpuContext(cfg)
+        ctx.upload_mesh(forest.flatten(ce, cf))
+        ctx.set_state(project_p1(leaves, fn, 6))
+        err2, _ = ctx.amr_indicator()
+        ctx.close()
+        vol = ((ROOT_LEN >> leaves[:, 3].astype(np.int64)) / ROOT_LEN) ** 3
+        flags = ((err2 > max_err ** 2 * vol) & (leaves[:, 3] < max_level)).astype(np.int8)
+        if not flags.any():
+            break
+        forest.adapt(flags, max_level, min_level)
+    return cfg, forest, project_p1(forest.leaves(), fn, 6)
 
 
 def dof_updates(n_elem, nv, stages, steps):

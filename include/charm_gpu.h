@@ -222,6 +222,10 @@ typedef struct {
 int  charm_ctx_create(charm_forest_t *f, const charm_config_t *cfg, charm_ctx_t **out);
 int  charm_ctx_destroy(charm_ctx_t *c);
 charm_gpu_ctx_t *charm_ctx_gpu(charm_ctx_t *c);
+/* status of the last failed reference-named call on this context (they
+ * return void like the reference's); reading it clears it.  The message is
+ * charm_gpu_last_error(). */
+int charm_ctx_error(charm_ctx_t *c);
 
 void         charm_model_euler_timestep_single(charm_forest_t *p4est, charm_real_t *dt,
                                                charm_ghost_t **ghost, charm_data_t **ghost_data);
