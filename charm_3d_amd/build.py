@@ -19,7 +19,7 @@ HIP_SOURCES = ["charm_kernels.hip", "charm_gpu.hip", "charm_amr.hip"]
 ARCH = os.environ.get("CHARM_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-C_FLAGS = ["-O2", "-fPIC", "-std=gnu99", "-Wall", "-Wno-unused-function", "-I", INC, "-I", SRC]
+C_FLAGS = ["-O2", "-fPIC", "-pthread", "-std=gnu99", "-Wall", "-Wno-unused-function", "-I", INC, "-I", SRC]
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17",
              # packed FP32 VALU forms are kept out of the generated code
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -49,7 +49,7 @@ def build(verbose=False):
             if verbose and (r.stdout or r.stderr):
                 print(r.stdout, r.stderr, file=sys.stderr)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + [o for _, o in jobs])
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-pthread", "-o", tmp] + [o for _, o in jobs])
     os.replace(tmp, LIB)
     return LIB
 

@@ -5,13 +5,18 @@
  * into the element / face / chunk arrays the device kernels consume.
  *
  * Leaves are kept as a Morton-sorted array of anchors; a point query is a
- * binary search on the anchors' Morton keys (a leaf at level l owns the key
- * interval [key, key + 8^(19-l))).
+ * search on the anchors' Morton keys (a leaf at level l owns the key
+ * interval [key, key + 8^(19-l))), galloping outward from a nearby leaf when
+ * one is known.  The per-leaf scans (balance, face listing) run on host
+ * threads over contiguous leaf ranges; their results do not depend on the
+ * thread count.
  */
+#include <pthread.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <unistd.h>
 #include "charm_internal.h"
 
 /* ------------------------------------------------------------ errors */
@@ -69,6 +74,86 @@ int64_t charm_forest_find(const charm_forest_t *f, int32_t x, int32_t y, int32_t
     if (lo < 0) return -1;
     if (k >= f->key[lo] + span(f->q[4*lo+3])) return -1;
     return lo;
+}
+
+/* leaf containing the point with Morton key k, searched outward from leaf
+ * `hint` (exponential steps, then bisection): O(log distance) */
+static int64_t find_near(const charm_forest_t *f, uint64_t k, int64_t hint)
+{
+    int64_t lo, hi, step = 1;
+    if (f->key[hint] <= k) {             /* key[lo] <= k < key[hi] (hi = n: none) */
+        lo = hint;
+        for (;;) {
+            hi = lo + step;
+            if (hi >= f->n) { hi = f->n; break; }
+            if (f->key[hi] > k) break;
+            lo = hi;
+            step <<= 1;
+        }
+    } else {
+        hi = hint;
+        for (;;) {
+            lo = hi - step;
+            if (lo < 0) { lo = -1; break; }
+            if (f->key[lo] <= k) break;
+            hi = lo;
+            step <<= 1;
+        }
+    }
+    while (hi - lo > 1) {
+        int64_t mid = lo + (hi - lo) / 2;
+        if (f->key[mid] <= k) lo = mid; else hi = mid;
+    }
+    if (lo < 0 || k >= f->key[lo] + span(f->q[4*lo+3])) return -1;
+    return lo;
+}
+
+/* ------------------------------------------------------------ host threads */
+
+/* CHARM_HOST_THREADS, else OMP_NUM_THREADS, else the online cores; [1, 64] */
+static int host_threads(void)
+{
+    const char *s = getenv("CHARM_HOST_THREADS");
+    long t = s ? atol(s) : 0;
+    if (t <= 0) { s = getenv("OMP_NUM_THREADS"); t = s ? atol(s) : 0; }
+    if (t <= 0) t = sysconf(_SC_NPROCESSORS_ONLN);
+    return t < 1 ? 1 : (t > 64 ? 64 : (int) t);
+}
+
+typedef void (*range_fn)(void *arg, int64_t lo, int64_t hi, int part);
+typedef struct { range_fn fn; void *arg; int64_t lo, hi; int part; } range_job_t;
+
+static void *range_run(void *p)
+{
+    range_job_t *j = (range_job_t *) p;
+    j->fn(j->arg, j->lo, j->hi, j->part);
+    return NULL;
+}
+
+/* number of contiguous parts [0, n) is split into (>= min_per_part each) */
+static int par_parts(int64_t n, int64_t min_per_part)
+{
+    int64_t t = host_threads();
+    if (n / min_per_part < t) t = n / min_per_part;
+    return t < 1 ? 1 : (int) t;
+}
+
+/* fn over the `parts` contiguous ranges of [0, n), part 0 on the caller */
+static void par_for(int64_t n, int parts, range_fn fn, void *arg)
+{
+    range_job_t job[64];
+    pthread_t th[64];
+    int ok[64], p;
+    for (p = 0; p < parts; p++) {
+        job[p].fn = fn; job[p].arg = arg; job[p].part = p;
+        job[p].lo = n * p / parts; job[p].hi = n * (p + 1) / parts;
+    }
+    for (p = 1; p < parts; p++) ok[p] = pthread_create(&th[p], NULL, range_run, &job[p]) == 0;
+    range_run(&job[0]);
+    for (p = 1; p < parts; p++) {
+        if (ok[p]) pthread_join(th[p], NULL);
+        else range_run(&job[p]);          /* no thread: run it here */
+    }
 }
 
 /* ------------------------------------------------------------ lifecycle */
@@ -249,35 +334,52 @@ static int forest_rebuild(charm_forest_t *f, const int8_t *act, int64_t **map_ou
 
 /* 2:1 face balance by ripple refinement: a leaf at level l forces the leaf
  * across each of its faces to level >= l - 1 */
+typedef struct { const charm_forest_t *f; int8_t *act; int8_t found[64]; } bal_arg_t;
+
+static void balance_scan(void *arg, int64_t lo, int64_t hi, int part)
+{
+    bal_arg_t *b = (bal_arg_t *) arg;
+    const charm_forest_t *f = b->f;
+    int64_t i;
+    for (i = lo; i < hi; i++) {
+        const int32_t *p = &f->q[4*i];
+        int32_t len = CHARM_ROOT_LEN >> p[3];
+        int a, s;
+        if (p[3] < 2) continue;
+        for (a = 0; a < 3; a++)
+            for (s = 0; s < 2; s++) {
+                int32_t x[3] = {p[0], p[1], p[2]};
+                int64_t nb;
+                x[a] += s ? len : -1;
+                if (x[a] < 0 || x[a] >= CHARM_ROOT_LEN) {
+                    if (!f->periodic[a]) continue;
+                    x[a] = (x[a] + CHARM_ROOT_LEN) % CHARM_ROOT_LEN;
+                }
+                nb = find_near(f, charm_morton(x[0], x[1], x[2]), i);
+                if (nb >= 0 && f->q[4*nb+3] < p[3] - 1) {
+                    __atomic_store_n(&b->act[nb], (int8_t) 1, __ATOMIC_RELAXED);
+                    b->found[part] = 1;
+                }
+            }
+    }
+}
+
 static int forest_balance_map(charm_forest_t *f, int64_t **map)
 {
     for (;;) {
-        int64_t i, nmark = 0;
-        int8_t *act = (int8_t *) calloc((size_t) f->n, 1);
-        if (!act) { charm_set_error("forest: out of memory"); return CHARM_ERR_MEM; }
-        for (i = 0; i < f->n; i++) {
-            const int32_t *p = &f->q[4*i];
-            int32_t len = CHARM_ROOT_LEN >> p[3];
-            int a, s;
-            if (p[3] < 2) continue;
-            for (a = 0; a < 3; a++)
-                for (s = 0; s < 2; s++) {
-                    int32_t x[3] = {p[0], p[1], p[2]};
-                    int64_t nb;
-                    x[a] += s ? len : -1;
-                    if (x[a] < 0 || x[a] >= CHARM_ROOT_LEN) {
-                        if (!f->periodic[a]) continue;
-                        x[a] = (x[a] + CHARM_ROOT_LEN) % CHARM_ROOT_LEN;
-                    }
-                    nb = charm_forest_find(f, x[0], x[1], x[2]);
-                    if (nb >= 0 && f->q[4*nb+3] < p[3] - 1 && !act[nb]) { act[nb] = 1; nmark++; }
-                }
-        }
-        if (nmark == 0) { free(act); return CHARM_OK; }
+        bal_arg_t b;
+        int p, parts = par_parts(f->n, 4096), any = 0;
+        memset(&b, 0, sizeof(b));
+        b.f = f;
+        b.act = (int8_t *) calloc((size_t) f->n, 1);
+        if (!b.act) { charm_set_error("forest: out of memory"); return CHARM_ERR_MEM; }
+        par_for(f->n, parts, balance_scan, &b);
+        for (p = 0; p < parts; p++) any |= b.found[p];
+        if (!any) { free(b.act); return CHARM_OK; }
         {
             int64_t *m2 = NULL, nm2, j;
-            int rc = forest_rebuild(f, act, map ? &m2 : NULL, &nm2);
-            free(act);
+            int rc = forest_rebuild(f, b.act, map ? &m2 : NULL, &nm2);
+            free(b.act);
             if (rc) return rc;
             if (map && *map) {           /* compose maps: new -> old original */
                 for (j = 0; j < nm2; j++) m2[j] = (*map)[m2[j]];
@@ -334,76 +436,112 @@ int charm_forest_adapt(charm_forest_t *f, const int8_t *flags, int max_level, in
 
 /* ------------------------------------------------------------ flattening */
 
-static int mesh_faces(const charm_forest_t *f, charm_mesh_t *m)
+/* faces owned by leaf e (its lower domain faces and every face on its upper
+ * side along each axis), at most 15, written to out[4 * k]; returns the
+ * count or -1 (no leaf across a face) / -2 (not 2:1 balanced) */
+static int leaf_faces(const charm_forest_t *f, int64_t e, int32_t *out)
 {
-    int64_t e, cap = f->n * 3 + 64, nf = 0;
-    int32_t *f4 = (int32_t *) malloc((size_t) cap * 4 * sizeof(int32_t));
-    if (!f4) goto oom;
-#define PUSH(E0, E1, INFO) do {                                              \
-        if (nf == cap) {                                                     \
-            int32_t *t_; cap *= 2;                                           \
-            t_ = (int32_t *) realloc(f4, (size_t) cap * 4 * sizeof(int32_t)); \
-            if (!t_) goto oom;                                               \
-            f4 = t_;                                                         \
-        }                                                                    \
-        f4[4*nf] = (E0); f4[4*nf+1] = (E1); f4[4*nf+2] = (INFO); f4[4*nf+3] = 0; nf++; \
-    } while (0)
-    for (e = 0; e < f->n; e++) {
-        const int32_t *p = &f->q[4*e];
-        int l = p[3], a;
-        int32_t len = CHARM_ROOT_LEN >> l;
-        for (a = 0; a < 3; a++) {
-            int t1 = (a == 0) ? 1 : 0, t2 = (a == 2) ? 1 : 2;
-            int32_t x[3] = {p[0], p[1], p[2]};
-            int64_t nb;
-            if (p[a] == 0 && !f->periodic[a])
-                PUSH((int32_t) e, -1 - 2*a, CHARM_FACE_INFO(2*a, CHARM_FACE_BND, 0));
-            x[a] += len;
-            if (x[a] == CHARM_ROOT_LEN) {
-                if (!f->periodic[a]) {
-                    PUSH((int32_t) e, -1 - (2*a+1), CHARM_FACE_INFO(2*a+1, CHARM_FACE_BND, 0));
-                    continue;
-                }
-                x[a] = 0;
+    const int32_t *p = &f->q[4*e];
+    int l = p[3], a, nf = 0;
+    int32_t len = CHARM_ROOT_LEN >> l;
+#define PUT(E0, E1, INFO) do { out[4*nf] = (E0); out[4*nf+1] = (E1); out[4*nf+2] = (INFO); out[4*nf+3] = 0; nf++; } while (0)
+    for (a = 0; a < 3; a++) {
+        int t1 = (a == 0) ? 1 : 0, t2 = (a == 2) ? 1 : 2;
+        int32_t x[3] = {p[0], p[1], p[2]};
+        int64_t nb;
+        const int32_t *r;
+        if (p[a] == 0 && !f->periodic[a])
+            PUT((int32_t) e, -1 - 2*a, CHARM_FACE_INFO(2*a, CHARM_FACE_BND, 0));
+        x[a] += len;
+        if (x[a] == CHARM_ROOT_LEN) {
+            if (!f->periodic[a]) {
+                PUT((int32_t) e, -1 - (2*a+1), CHARM_FACE_INFO(2*a+1, CHARM_FACE_BND, 0));
+                continue;
             }
-            nb = charm_forest_find(f, x[0], x[1], x[2]);
-            if (nb < 0) { charm_set_error("flatten: no leaf across face"); free(f4); return CHARM_ERR_MESH; }
-            {
-                const int32_t *r = &f->q[4*nb];
-                if (r[3] == l) {
-                    PUSH((int32_t) e, (int32_t) nb, CHARM_FACE_INFO(2*a+1, CHARM_FACE_CONF, 0));
-                } else if (r[3] == l - 1) {
-                    int sub = ((p[t1] - r[t1]) / len) | (((p[t2] - r[t2]) / len) << 1);
-                    PUSH((int32_t) e, (int32_t) nb, CHARM_FACE_INFO(2*a+1, CHARM_FACE_HANG0, sub));
-                } else if (r[3] == l + 1) {
-                    int j;
-                    for (j = 0; j < 4; j++) {
-                        int32_t y[3] = {x[0], x[1], x[2]};
-                        int64_t sm;
-                        y[t1] += (j & 1) * (len / 2);
-                        y[t2] += (j >> 1) * (len / 2);
-                        sm = charm_forest_find(f, y[0], y[1], y[2]);
-                        if (sm < 0 || f->q[4*sm+3] != l + 1) {
-                            charm_set_error("flatten: forest is not 2:1 face balanced");
-                            free(f4); return CHARM_ERR_MESH;
-                        }
-                        PUSH((int32_t) e, (int32_t) sm, CHARM_FACE_INFO(2*a+1, CHARM_FACE_HANG1, j));
-                    }
-                } else {
-                    charm_set_error("flatten: forest is not 2:1 face balanced (levels %d / %d)", l, r[3]);
-                    free(f4); return CHARM_ERR_MESH;
-                }
+            x[a] = 0;
+        }
+        nb = find_near(f, charm_morton(x[0], x[1], x[2]), e);
+        if (nb < 0) return -1;
+        r = &f->q[4*nb];
+        if (r[3] == l) {
+            PUT((int32_t) e, (int32_t) nb, CHARM_FACE_INFO(2*a+1, CHARM_FACE_CONF, 0));
+        } else if (r[3] == l - 1) {
+            int sub = ((p[t1] - r[t1]) / len) | (((p[t2] - r[t2]) / len) << 1);
+            PUT((int32_t) e, (int32_t) nb, CHARM_FACE_INFO(2*a+1, CHARM_FACE_HANG0, sub));
+        } else if (r[3] == l + 1) {
+            int j;
+            for (j = 0; j < 4; j++) {
+                int32_t y[3] = {x[0], x[1], x[2]};
+                int64_t sm;
+                y[t1] += (j & 1) * (len / 2);
+                y[t2] += (j >> 1) * (len / 2);
+                sm = find_near(f, charm_morton(y[0], y[1], y[2]), nb);
+                if (sm < 0 || f->q[4*sm+3] != l + 1) return -2;
+                PUT((int32_t) e, (int32_t) sm, CHARM_FACE_INFO(2*a+1, CHARM_FACE_HANG1, j));
             }
+        } else {
+            return -2;
         }
     }
-#undef PUSH
+#undef PUT
+    return nf;
+}
+
+typedef struct {
+    const charm_forest_t *f;
+    int32_t *buf[64];
+    int64_t cnt[64];
+    int err[64];
+} faces_arg_t;
+
+static void faces_scan(void *arg, int64_t lo, int64_t hi, int part)
+{
+    faces_arg_t *a = (faces_arg_t *) arg;
+    int64_t e, cap = (hi - lo) * 3 + 64, n = 0;
+    int32_t *buf = (int32_t *) malloc((size_t) cap * 4 * sizeof(int32_t));
+    a->buf[part] = NULL; a->cnt[part] = 0; a->err[part] = 0;
+    if (!buf) { a->err[part] = -3; return; }
+    for (e = lo; e < hi; e++) {
+        int k;
+        if (n + 15 > cap) {
+            int32_t *t;
+            cap *= 2;
+            t = (int32_t *) realloc(buf, (size_t) cap * 4 * sizeof(int32_t));
+            if (!t) { a->err[part] = -3; break; }
+            buf = t;
+        }
+        k = leaf_faces(a->f, e, &buf[4*n]);
+        if (k < 0) { a->err[part] = k; break; }
+        n += k;
+    }
+    a->buf[part] = buf;
+    a->cnt[part] = n;
+}
+
+/* every (sub)face once, in leaf order, side 0 = lower-coordinate leaf */
+static int mesh_faces(const charm_forest_t *f, charm_mesh_t *m)
+{
+    faces_arg_t a;
+    int parts = par_parts(f->n, 8192), p, err = 0;
+    int64_t nf = 0, o = 0;
+    memset(&a, 0, sizeof(a));
+    a.f = f;
+    par_for(f->n, parts, faces_scan, &a);
+    for (p = 0; p < parts; p++) { nf += a.cnt[p]; if (a.err[p] && !err) err = a.err[p]; }
+    if (!err) {
+        m->f4 = (int32_t *) malloc((size_t) (nf ? nf : 1) * 4 * sizeof(int32_t));
+        if (!m->f4) err = -3;
+    }
+    for (p = 0; p < parts; p++) {
+        if (!err) memcpy(&m->f4[4*o], a.buf[p], (size_t) a.cnt[p] * 4 * sizeof(int32_t));
+        o += a.cnt[p];
+        free(a.buf[p]);
+    }
+    if (err == -1) { charm_set_error("flatten: no leaf across face"); return CHARM_ERR_MESH; }
+    if (err == -2) { charm_set_error("flatten: forest is not 2:1 face balanced"); return CHARM_ERR_MESH; }
+    if (err) { charm_set_error("flatten: out of memory"); return CHARM_ERR_MEM; }
     m->nf = nf;
-    m->f4 = f4;
     return CHARM_OK;
-oom:
-    free(f4);
-    charm_set_error("flatten: out of memory");
-    return CHARM_ERR_MEM;
 }
 
 static int mesh_neighbors(charm_mesh_t *m)
