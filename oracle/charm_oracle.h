@@ -86,6 +86,7 @@ typedef struct {
     int    n_react;
     or_reaction_t react[4];
     double cfl;
+    int    chem_parity;        /* 1: the reference's chem_w[species] index (see or_ns.c) */
 } or_config_t;
 
 typedef struct or_sim or_sim_t;
@@ -107,6 +108,14 @@ void or_limiter(or_sim_t *s);
 void or_euler_step(or_sim_t *s, double dt);
 void or_ns_step(or_sim_t *s, double dt);
 void or_run(or_sim_t *s, double dt, int nsteps);
+
+/* Navier-Stokes pieces for tests: full stage residual (zero + viscous +
+ * convective), auxiliary tau[6][4] q[3][4] per leaf ([n][9][4]), diffusion
+ * coefficients [n][n_comp], chemistry applied to the state (chem_rhs [n]) */
+void or_ns_residual(or_sim_t *s, double *out);
+void or_ns_aux(or_sim_t *s, double *out);
+void or_ns_diff_coeffs(or_sim_t *s, double *out);
+void or_ns_chem_apply(or_sim_t *s, double dt, double *chem_rhs);
 
 /* face listing for flattening checks: per face
  * [nsides, face0, face1, hang0, hang1, q0[4], q1[4], bnd, wrap_axis] = 16 int32 */

@@ -75,6 +75,7 @@ struct or_sim {
     int64_t     nf;
     or_face_t  *f;
     int         nv;
+    double      dt;          /* ctx->dt: the chemistry's step */
     char        err[256];
 };
 
@@ -97,5 +98,9 @@ double or_bf_dx(const or_geom_t *g, int k);
 double or_bf_dy(const or_geom_t *g, int k);
 double or_bf_dz(const or_geom_t *g, int k);
 void   or_geom_calc(const int32_t q[4], or_geom_t *g);
+void   or_get_fields(const or_sim_t *s, const or_data_t *d, const double x[3], or_cons_t *c);
+void   or_get_fields_avg(const or_sim_t *s, const or_data_t *d, or_cons_t *c);
+void   or_face_normal(const or_data_t *d, int face, double n[3]);
+void   or_shift(const or_face_t *f, int side, int owner, const double x[3], double xs[3]);
 
 #endif

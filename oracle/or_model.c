@@ -245,7 +245,7 @@ void or_geom(const or_sim_t *s, int64_t leaf, double *out)
 
 /* charm_get_fields, src/charm_base_func.c:139-211 (loops bounded by the
  * real species count, as the reference's @todo intends) */
-static void or_get_fields(const or_sim_t *s, const or_data_t *d, const double x[3], or_cons_t *c)
+void or_get_fields(const or_sim_t *s, const or_data_t *d, const double x[3], or_cons_t *c)
 {
     int v, i;
     double r[OR_NV_MAX];
@@ -259,7 +259,7 @@ static void or_get_fields(const or_sim_t *s, const or_data_t *d, const double x[
 }
 
 /* charm_get_fields_avg, src/charm_base_func.c:214-240 */
-static void or_get_fields_avg(const or_sim_t *s, const or_data_t *d, or_cons_t *c)
+void or_get_fields_avg(const or_sim_t *s, const or_data_t *d, or_cons_t *c)
 {
     int igp, v, i;
     double r[OR_NV_MAX];
@@ -287,7 +287,7 @@ static void or_cons_get(const or_sim_t *s, const or_cons_t *c, double *u)
 
 /* outward normal of side 0's face: charm_face_get_normal plus the
  * orientation flip by (fc - c), src/models/euler/charm_model.c:332-344 */
-static void or_face_normal(const or_data_t *d, int face, double n[3])
+void or_face_normal(const or_data_t *d, int face, double n[3])
 {
     double l[3];
     int i;
@@ -296,12 +296,14 @@ static void or_face_normal(const or_data_t *d, int face, double n[3])
         for (i = 0; i < 3; i++) n[i] *= -1.0;
 }
 
-static void or_shift(const or_face_t *f, int side, int owner, const double x[3], double xs[3])
+void or_shift(const or_face_t *f, int side, int owner, const double x[3], double xs[3])
 {
     xs[0] = x[0]; xs[1] = x[1]; xs[2] = x[2];
     if (f->wrap_axis >= 0 && side != owner)
         xs[f->wrap_axis] += (side == 1) ? -1.0 : 1.0;
 }
+
+void or_ns_diff(or_sim_t *s);
 
 /* ----------------------------------------------------- convective terms */
 
@@ -441,6 +443,19 @@ static void or_conv(or_sim_t *s)
         if (s->f[i].nsides != 2) or_conv_bnd(s, &s->f[i]);
         else or_conv_inner(s, &s->f[i]);
     }
+}
+
+/* zero + viscous (when enabled) + convective pass of an NS stage:
+ * src/models/ns/charm_model.c:246-252 without the chemistry */
+void or_ns_residual(or_sim_t *s, double *out)
+{
+    int64_t e; int v, k;
+    or_zero(s);
+    or_ns_diff(s);
+    or_conv(s);
+    for (e = 0; e < s->n; e++)
+        for (v = 0; v < s->nv; v++)
+            for (k = 0; k < OR_BF; k++) out[(e*s->nv + v)*OR_BF + k] = s->d[e].integ[v][k];
 }
 
 void or_conv_residual(or_sim_t *s, double *out)
@@ -596,6 +611,7 @@ static void or_ns_stage(or_sim_t *s, double dt)
 void or_ns_step(or_sim_t *s, double dt)
 {
     int64_t e; int v, i;
+    s->dt = dt;                                   /* ctx->dt of the chemistry */
     for (e = 0; e < s->n; e++) memcpy(s->d[e].c_old, s->d[e].c, sizeof(s->d[e].c));
     or_ns_stage(s, dt);
     or_ns_stage(s, dt);

@@ -50,7 +50,7 @@ class Config(C.Structure):
                 ("bnd_type", C.c_int * 6), ("bnd_param", (C.c_double * 16) * 6),
                 ("grav", C.c_double * 3), ("use_visc", C.c_int), ("use_diff", C.c_int),
                 ("t_ref", C.c_double), ("n_react", C.c_int), ("react", Reaction * 4),
-                ("cfl", C.c_double)]
+                ("cfl", C.c_double), ("chem_parity", C.c_int)]
 
 
 def build():
@@ -96,6 +96,9 @@ def lib():
         L.or_amr_indicator.argtypes = [vp, dp, dp]
         L.or_amr_flags.argtypes = [vp, dp, dp, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int8)]
         L.or_amr_project.argtypes = [C.c_int, i64, ip, dp, i64, ip, C.POINTER(C.c_int64), dp]
+        for fn in ("or_ns_residual", "or_ns_aux", "or_ns_diff_coeffs"):
+            getattr(L, fn).argtypes = [vp, dp]
+        L.or_ns_chem_apply.argtypes = [vp, C.c_double, dp]
         L.or_gas_r.restype = C.c_double
         for fn in ("or_prim_size", "or_cons_size", "or_config_size"):
             getattr(L, fn).restype = C.c_size_t
@@ -112,7 +115,10 @@ def _dptr(a):
 
 def make_config(n_comp=1, comp_m=(GAS_R,), comp_cp=((3.5,),), eos="IDEAL", flux="LF",
                 limiter=False, model="NS", bnd=("OUTLET",) * 6, bnd_param=None,
-                grav=(0, 0, 0), cfl=0.3):
+                grav=(0, 0, 0), cfl=0.3, transport=None, use_visc=False, use_diff=False,
+                t_ref=298.15, reactions=(), chem_parity=False):
+    """transport: per species dict with ml0, kp0, t0, ts, sig, ek, h0, ml_type,
+    kp_type (Sutherland = 1); reactions: (left[3], right[3], a, e, n) tuples."""
     cfg = Config()
     cfg.n_comp = n_comp
     for i in range(n_comp):
@@ -135,6 +141,18 @@ def make_config(n_comp=1, comp_m=(GAS_R,), comp_cp=((3.5,),), eos="IDEAL", flux=
     for i in range(3):
         cfg.grav[i] = grav[i]
     cfg.cfl = cfl
+    if transport is not None:
+        for i, tr in enumerate(transport):
+            for k, v in tr.items():
+                setattr(cfg.comp[i], k, v)
+    cfg.use_visc, cfg.use_diff, cfg.t_ref = int(use_visc), int(use_diff), t_ref
+    cfg.n_react = len(reactions)
+    for j, (left, right, a, e, n) in enumerate(reactions):
+        for m in range(3):
+            cfg.react[j].left[m] = left[m]
+            cfg.react[j].right[m] = right[m]
+        cfg.react[j].a, cfg.react[j].e, cfg.react[j].n = a, e, n
+    cfg.chem_parity = int(chem_parity)
     return cfg
 
 
@@ -176,6 +194,28 @@ class OracleSim:
         out = np.zeros((self.n, self.nv, 4))
         lib().or_conv_residual(self.h, _dptr(out))
         self.check()
+        return out
+
+    def ns_residual(self):
+        out = np.zeros((self.n, self.nv, 4))
+        lib().or_ns_residual(self.h, _dptr(out))
+        self.check()
+        return out
+
+    def ns_aux(self):
+        """tau (xx yy zz xy xz yz) and q (x y z) coefficients: [n][9][4]"""
+        out = np.zeros((self.n, 9, 4))
+        lib().or_ns_aux(self.h, _dptr(out))
+        return out
+
+    def ns_diff_coeffs(self):
+        out = np.zeros((self.n, self.nv - 4))
+        lib().or_ns_diff_coeffs(self.h, _dptr(out))
+        return out
+
+    def ns_chem_apply(self, dt):
+        out = np.zeros(self.n)
+        lib().or_ns_chem_apply(self.h, dt, _dptr(out))
         return out
 
     def limiter(self):
