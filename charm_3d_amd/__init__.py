@@ -31,7 +31,13 @@ class CharmError(RuntimeError):
 class CompDesc(C.Structure):
     _fields_ = [("m", C.c_double), ("cp_type", C.c_int), ("ml_type", C.c_int),
                 ("n_cp", C.c_int), ("cp", C.c_double * 8), ("ml0", C.c_double),
-                ("kp0", C.c_double), ("t0", C.c_double), ("ts", C.c_double), ("h0", C.c_double)]
+                ("kp0", C.c_double), ("t0", C.c_double), ("ts", C.c_double), ("h0", C.c_double),
+                ("kp_type", C.c_int), ("sig", C.c_double), ("ek", C.c_double)]
+
+
+class ReactionDesc(C.Structure):
+    _fields_ = [("left", C.c_int * 3), ("right", C.c_int * 3), ("a", C.c_double),
+                ("e", C.c_double), ("n", C.c_double)]
 
 
 class Config(C.Structure):
@@ -42,7 +48,9 @@ class Config(C.Structure):
                 ("cfl", C.c_double), ("dt", C.c_double), ("tmax", C.c_double),
                 ("log_period", C.c_int), ("chunk_elems", C.c_int), ("chunk_faces", C.c_int),
                 ("max_err", C.c_double), ("min_level", C.c_int), ("max_level", C.c_int),
-                ("refine_period", C.c_int)]
+                ("refine_period", C.c_int), ("use_visc", C.c_int), ("use_diff", C.c_int),
+                ("t_ref", C.c_double), ("n_react", C.c_int), ("react", ReactionDesc * 4),
+                ("chem_parity", C.c_int)]
 
 
 _lib = None
@@ -96,6 +104,8 @@ def lib():
         "charm_gpu_last_timing": (C.c_int, [vp, dp, dp]),
         "charm_gpu_flux_batch": (C.c_int, [vp, C.c_int, i64, dp, dp, dp, dp]),
         "charm_gpu_amr_indicator": (C.c_int, [vp, dp, dp, i64]),
+        "charm_gpu_ns_aux": (C.c_int, [vp, dp, i64]),
+        "charm_gpu_ns_chem": (C.c_int, [vp, C.c_double, dp, i64]),
         "charm_gpu_remesh": (C.c_int, [vp, vp, C.POINTER(C.c_int64), i64]),
         "charm_amr_flags": (C.c_int, [i64, i32p, dp, dp, C.c_double, C.c_int, C.c_int,
                                       C.POINTER(C.c_int8)]),
@@ -142,8 +152,13 @@ def _periodic(per):
 def make_config(precision=FP32, model="NS", flux="HLLC", limiter=False, eos="IDEAL",
                 comp_m=(GAS_R,), comp_cp=(3.5,), bnd=("OUTLET",) * 6, bnd_param=None,
                 grav=(0.0, 0.0, 0.0), cfl=0.3, dt=0.0, tmax=0.0, chunk_elems=0, chunk_faces=0,
-                max_err=0.0, min_level=0, max_level=0, refine_period=0):
-    """charm_config_t from task.yaml-like keys (constant cp per species)."""
+                max_err=0.0, min_level=0, max_level=0, refine_period=0, transport=None,
+                use_visc=False, use_diff=False, t_ref=298.15, reactions=(), chem_parity=False):
+    """charm_config_t from task.yaml-like keys.  comp_cp entries are a constant
+    cp or a tuple of power-series coefficients (polynomial cp, which as in
+    the reference takes effect with Sutherland viscosity, ml_type 1);
+    transport: per species dict of charm_comp_desc_t fields (ml0, kp0, t0,
+    ts, h0, ml_type, kp_type, sig, ek); reactions: (left[3], right[3], a, e, n)."""
     cfg = Config()
     cfg.precision = precision
     cfg.model = MODEL[model]
@@ -153,8 +168,12 @@ def make_config(precision=FP32, model="NS", flux="HLLC", limiter=False, eos="IDE
     cfg.n_comp = len(comp_m)
     for i, (m, cp) in enumerate(zip(comp_m, comp_cp)):
         cfg.comp[i].m = m
-        cfg.comp[i].n_cp = 1
-        cfg.comp[i].cp[0] = cp
+        cps = tuple(cp) if isinstance(cp, (tuple, list)) else (cp,)
+        cfg.comp[i].n_cp = len(cps)
+        for j, v in enumerate(cps):
+            cfg.comp[i].cp[j] = v
+        cfg.comp[i].cp_type = 0 if len(cps) == 1 else 1
+        cfg.comp[i].ml_type = 0 if len(cps) == 1 else 1
     for i in range(6):
         cfg.bnd_type[i] = BND[bnd[i]]
         if bnd_param is not None and bnd_param[i] is not None:
@@ -166,6 +185,18 @@ def make_config(precision=FP32, model="NS", flux="HLLC", limiter=False, eos="IDE
     cfg.chunk_elems, cfg.chunk_faces = chunk_elems, chunk_faces
     cfg.max_err, cfg.min_level, cfg.max_level = max_err, min_level, max_level
     cfg.refine_period = refine_period
+    if transport is not None:
+        for i, tr in enumerate(transport):
+            for k, v in tr.items():
+                setattr(cfg.comp[i], k, v)
+    cfg.use_visc, cfg.use_diff, cfg.t_ref = int(use_visc), int(use_diff), t_ref
+    cfg.n_react = len(reactions)
+    for j, (left, right, a, e, n) in enumerate(reactions):
+        for m in range(3):
+            cfg.react[j].left[m] = left[m]
+            cfg.react[j].right[m] = right[m]
+        cfg.react[j].a, cfg.react[j].e, cfg.react[j].n = a, e, n
+    cfg.chem_parity = int(chem_parity)
     return cfg
 
 
@@ -356,6 +387,18 @@ class GpuContext:
         err2, rho_c = np.zeros(self.n), np.zeros(self.n)
         check(lib().charm_gpu_amr_indicator(self.h, _dptr(err2), _dptr(rho_c), self.n))
         return err2, rho_c
+
+    def ns_aux(self):
+        """BR1 auxiliary tau (xx yy zz xy xz yz) and q (x y z): [n][9][4]."""
+        out = np.zeros((self.n, 9, 4))
+        check(lib().charm_gpu_ns_aux(self.h, _dptr(out), self.n))
+        return out
+
+    def ns_chem(self, dt):
+        """Chemistry applied to the state; returns the energy source [n]."""
+        out = np.zeros(self.n)
+        check(lib().charm_gpu_ns_chem(self.h, dt, _dptr(out), self.n))
+        return out
 
     def remesh(self, mesh, amap):
         """Upload the adapted mesh and project the state with the old->new map."""

@@ -15,7 +15,9 @@ OBJ = os.path.join(ROOT, "_build", "charm3d")
 LIB = os.path.join(PKG, "libcharm3d.so")
 
 C_SOURCES = ["charm_forest.c", "charm_geom.c", "charm_model.c", "charm_amr.c"]
-HIP_SOURCES = ["charm_kernels.hip", "charm_gpu.hip", "charm_amr.hip"]
+HIP_SOURCES = ["charm_stage_f32_ph0.hip", "charm_stage_f32_ph1.hip", "charm_stage_f64_ph0.hip",
+               "charm_stage_f64_ph1.hip", "charm_kernels.hip", "charm_gpu.hip", "charm_amr.hip",
+               "charm_ns.hip"]
 ARCH = os.environ.get("CHARM_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -44,7 +46,7 @@ def build(verbose=False):
     for s in HIP_SOURCES:
         o = os.path.join(OBJ, s + ".o")
         jobs.append(([HIPCC] + HIP_FLAGS + ["-c", os.path.join(SRC, s), "-o", o], o))
-    with ThreadPoolExecutor(max_workers=4) as ex:
+    with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 4)) as ex:
         for r in ex.map(lambda j: _run(j[0]), jobs):
             if verbose and (r.stdout or r.stderr):
                 print(r.stdout, r.stderr, file=sys.stderr)

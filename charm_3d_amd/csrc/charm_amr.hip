@@ -40,7 +40,7 @@ __device__ __forceinline__ void rho_coeffs(const S *__restrict__ U, int64_t n, i
 // at the six face Gauss points.  One thread per leaf
 // gathers over its six face slots (no atomics); rho_c is the floored density
 // of the cell mean (the value src/charm_amr.c:116-127 compares).
-template <typename S, int NC>
+template <typename S, int NC, int PH>
 __global__ __launch_bounds__(256)
 void k_amr_indicator(AmrArgs A, DevParams P)
 {
@@ -89,8 +89,8 @@ void k_amr_indicator(AmrArgs A, DevParams P)
                 #pragma unroll
                 for (int v = 0; v < NV; v++) u[v] = c[v][0] + half * c[v][1 + a] + o1 * c[v][1 + t1] + o2 * c[v][1 + t2];
                 Prim<double, NC> pi, po;
-                cons_to_prim<double, NC>(P, u, pi);
-                bnd_state<double, NC>(P, -1 - nb, a, sg, pi, po, -1);
+                cons_to_prim<double, NC, PH>(P, u, pi);
+                bnd_state<double, NC, PH>(P, -1 - nb, a, sg, pi, po, -1);
                 sum += (area * (1.0 / 6.0)) * (0.5 * (pi.r + po.r));
             }
         } else {
@@ -189,24 +189,25 @@ void k_amr_project(ProjArgs A)
     }
 }
 
-template <typename S>
+template <typename S, int PH>
 static hipError_t launch_amr_indicator_r(int nc, const AmrArgs &A, const DevParams &P, hipStream_t st)
 {
     const dim3 grid((unsigned) ((A.n + 255) / 256)), block(256);
     switch (nc) {
-    case 1: hipLaunchKernelGGL((k_amr_indicator<S, 1>), grid, block, 0, st, A, P); break;
-    case 2: hipLaunchKernelGGL((k_amr_indicator<S, 2>), grid, block, 0, st, A, P); break;
-    case 3: hipLaunchKernelGGL((k_amr_indicator<S, 3>), grid, block, 0, st, A, P); break;
-    case 4: hipLaunchKernelGGL((k_amr_indicator<S, 4>), grid, block, 0, st, A, P); break;
+    case 1: hipLaunchKernelGGL((k_amr_indicator<S, 1, PH>), grid, block, 0, st, A, P); break;
+    case 2: hipLaunchKernelGGL((k_amr_indicator<S, 2, PH>), grid, block, 0, st, A, P); break;
+    case 3: hipLaunchKernelGGL((k_amr_indicator<S, 3, PH>), grid, block, 0, st, A, P); break;
+    case 4: hipLaunchKernelGGL((k_amr_indicator<S, 4, PH>), grid, block, 0, st, A, P); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_amr_indicator(int prec, int nc, const AmrArgs &A, const DevParams &P, hipStream_t st)
+hipError_t launch_amr_indicator(int prec, int nc, int ph, const AmrArgs &A, const DevParams &P, hipStream_t st)
 {
     if (A.n <= 0) return hipSuccess;
-    return prec ? launch_amr_indicator_r<float>(nc, A, P, st) : launch_amr_indicator_r<double>(nc, A, P, st);
+    if (ph) return prec ? launch_amr_indicator_r<float, 1>(nc, A, P, st) : launch_amr_indicator_r<double, 1>(nc, A, P, st);
+    return prec ? launch_amr_indicator_r<float, 0>(nc, A, P, st) : launch_amr_indicator_r<double, 0>(nc, A, P, st);
 }
 
 hipError_t launch_amr_project(int prec, const ProjArgs &A, hipStream_t st)

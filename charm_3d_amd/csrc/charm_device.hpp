@@ -35,6 +35,19 @@ struct DevParams {
     float  comp_m_f[kMaxComp], comp_cp_f[kMaxComp];
     float  bnd_param_f[6][5];
     float  grav_f[3];
+    // Navier-Stokes / real-gas extensions, read by the PH = 1 kernels only
+    int    newton_t;                   // comp 0 has a polynomial cp: T by FP64 Newton
+    int    cp_poly[kMaxComp];          // cp(T) is the power series
+    int    n_cp[kMaxComp];
+    double cp_c[kMaxComp][8];
+    int    ml_type[kMaxComp], kp_type[kMaxComp];
+    double ml0[kMaxComp], kp0[kMaxComp], t0[kMaxComp], ts[kMaxComp], h0[kMaxComp];
+    double sig[kMaxComp], ek[kMaxComp];
+    double t_ref;
+    int    use_visc, use_diff;
+    int    n_react, chem_parity;
+    int    r_left[4][3], r_right[4][3];
+    double r_a[4], r_e[4], r_n[4];
 };
 
 template <typename R> __device__ __forceinline__ R cst(double d, float f);
@@ -51,17 +64,94 @@ template <> __device__ __forceinline__ float dv<float>(float a, float b) { retur
 template <typename R> __device__ __forceinline__ R dsqrt(R x) { return sqrt(x); }
 template <> __device__ __forceinline__ float dsqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 
+// cp of species i at T, charm_comp_calc_cp (src/charm_globals.c:603-625):
+// the power series when cp_poly (cp_type polynomial with ml_type 1, as the
+// reference's test reads), else cp[0]
+__device__ __forceinline__ double comp_cp(const DevParams &P, int i, double t)
+{
+    if (!P.cp_poly[i]) return P.comp_cp[i];
+    double r = 0.0, tt = 1.0;
+    for (int k = 0; k < P.n_cp[i]; k++) { r += tt * P.cp_c[i][k]; tt *= t; }
+    return r;
+}
+
+// d cp / dT, charm_comp_calc_cp_dt (src/charm_globals.c:628-649)
+__device__ __forceinline__ double comp_cp_dt(const DevParams &P, int i, double t)
+{
+    double r = 0.0, tt = 1.0;
+    for (int k = 1; k < P.n_cp[i]; k++) { r += tt * P.cp_c[i][k] * k; tt *= t; }
+    return r;
+}
+
+// temperature of the polynomial-cp mixture by Newton from T = 1 K,
+// (cp(T) - R sum c_i / M_i) T = e, at most 100 steps, stop at dT^2 < 1e-12
+// (_calc_t, src/charm_eos.c:12-54); FP64 whatever the storage precision
+template <int NC>
+__device__ __forceinline__ double newton_t(const DevParams &P, double e, const double *c)
+{
+    double tt = 1.0, m_ = 0.0;
+    #pragma unroll
+    for (int i = 0; i < NC; i++) m_ += c[i] / P.comp_m[i];
+    const double rm = kGasR * m_;
+    for (int j = 0; j < 100; j++) {
+        double cp = 0.0, cp_dt = 0.0;
+        #pragma unroll
+        for (int i = 0; i < NC; i++) { cp += c[i] * comp_cp(P, i, tt); cp_dt += c[i] * comp_cp_dt(P, i, tt); }
+        const double ft = (cp - rm) * tt - e, ft_dt = (cp - rm) + cp_dt * tt;
+        const double tt1 = tt - ft / ft_dt;
+        const double d = tt1 - tt;
+        tt = tt1;
+        if (d * d < kEps) break;
+    }
+    return tt;
+}
+
 template <typename R, int NC>
 struct Prim {
     R r, u, v, w, e_tot, p, cz, gam, e, cv, m;
     R c[NC];
 };
 
-// EOS closure: gamma, cv, molar mass of the state (src/charm_eos.c:98-148
-// for a constant cp per species).
+// EOS closure at a given temperature (FP64): cp, M, cv, gamma of the ideal
+// gas (component 0) or the mixture, src/charm_eos.c:98-148
 template <typename R, int NC>
+__device__ __forceinline__ void eos_coeffs_at(const DevParams &P, Prim<R, NC> &p, double t)
+{
+    double cp, m;
+    if (P.eos == EOS_IDEAL) {
+        cp = comp_cp(P, 0, t);
+        m = P.comp_m[0];
+    } else {
+        double M_ = 0.0;
+        cp = 0.0;
+        #pragma unroll
+        for (int i = 0; i < NC; i++) {
+            M_ += (double) p.c[i] / P.comp_m[i];
+            cp += (double) p.c[i] * comp_cp(P, i, t);
+        }
+        m = 1.0 / M_;
+    }
+    const double cv = cp - kGasR / m;
+    p.m = (R) m; p.cv = (R) cv; p.gam = (R) (cp / cv);
+}
+
+// EOS closure: gamma, cv, molar mass of the state (src/charm_eos.c:98-148).
+// PH = 0: constant cp per species.  PH = 1: any cp; with a polynomial cp the
+// temperature comes from the FP64 Newton solve on e (flags 1 and 4).
+template <typename R, int NC, int PH = 0>
 __device__ __forceinline__ void eos_coeffs(const DevParams &P, Prim<R, NC> &p)
 {
+    if constexpr (PH == 1) {
+        double t = 0.0;
+        if (P.newton_t) {
+            double c[NC];
+            #pragma unroll
+            for (int i = 0; i < NC; i++) c[i] = (double) p.c[i];
+            t = newton_t<NC>(P, (double) p.e, c);
+        }
+        eos_coeffs_at<R, NC>(P, p, t);
+        return;
+    }
     if (P.eos == EOS_IDEAL || NC == 1) {
         p.gam = cst<R>(P.gam, P.gam_f); p.cv = cst<R>(P.cv, P.cv_f); p.m = cst<R>(P.m, P.m_f);
     } else {
@@ -98,7 +188,7 @@ __device__ __forceinline__ R gp_t2(int k)
 // charm_param_cons_to_prim + EOS flag 4, src/charm_globals.c:183-208 and
 // src/charm_eos.c:86-91: velocities and energy use the unfloored density,
 // pressure and sound speed the floored one.
-template <typename R, int NC>
+template <typename R, int NC, int PH = 0>
 __device__ __forceinline__ void cons_to_prim(const DevParams &P, const R *u, Prim<R, NC> &p)
 {
     R r = R(0);
@@ -115,7 +205,7 @@ __device__ __forceinline__ void cons_to_prim(const DevParams &P, const R *u, Pri
         c = c > R(1) ? R(1) : c;
         p.c[i] = c;
     }
-    eos_coeffs<R, NC>(P, p);
+    eos_coeffs<R, NC, PH>(P, p);
     if (r < R(kEps)) r = R(kEps);
     p.r = r;
     p.p = r * p.e * (p.gam - R(1));
@@ -127,7 +217,7 @@ __device__ __forceinline__ void cons_to_prim(const DevParams &P, const R *u, Pri
 // (T, p) and e_tot is recomputed.  n = s * e_axis (axis-aligned octree).
 // With frame_axis >= 0 the velocity components are stored in the rotated
 // order (frame_axis, t1, t2) and the inlet velocity is rotated alike.
-template <typename R, int NC>
+template <typename R, int NC, int PH = 0>
 __device__ __forceinline__ void bnd_state(const DevParams &P, int face, int axis, R s,
                                           const Prim<R, NC> &in, Prim<R, NC> &out, int frame_axis)
 {
@@ -167,7 +257,8 @@ __device__ __forceinline__ void bnd_state(const DevParams &P, int face, int axis
         t = cst<R>(P.bnd_param[face][0], P.bnd_param_f[face][0]);
     }
     out.u = vel[0]; out.v = vel[1]; out.w = vel[2];
-    eos_coeffs<R, NC>(P, out);
+    if constexpr (PH == 1) eos_coeffs_at<R, NC>(P, out, (double) t);   // flag 3: T given
+    else eos_coeffs<R, NC>(P, out);
     if (pr < R(kEps)) pr = R(kEps);
     out.p = pr;
     out.r = dv(pr * out.m, t * R(kGasR));

@@ -20,6 +20,7 @@ extern "C" const char *charm_host_last_error(void);
 struct charm_gpu_ctx {
     charm_config_t cfg;
     int prec, nc, nv;
+    int ph;                   // 1: Navier-Stokes / real-gas kernels (PH = 1)
     size_t esz;
     DevParams P;
     hipStream_t st;
@@ -35,6 +36,11 @@ struct charm_gpu_ctx {
     uint16_t *d_slots;
     int4 *d_q;                // leaves (x, y, z, level): adaptation
     int32_t *hq;              // host copy of the leaves of the uploaded mesh
+    // Navier-Stokes buffers (allocated when ph)
+    void *d_auxf;             // [36][n] tau, q coefficients
+    void *d_vi;               // [16][n] viscous integrals
+    double *d_dcoef;          // [nc][n] diffusion coefficients
+    double *d_chem;           // [n] chemistry energy source
     // state
     void *B[3];
     double *d_stage64;        // FP64 staging for host transfers
@@ -65,7 +71,8 @@ static void free_mesh(charm_gpu_ctx_t *c)
     void **p[] = {(void **) &c->d_level, (void **) &c->d_cf4, (void **) &c->d_aux,
                   (void **) &c->d_ch_ebeg, (void **) &c->d_ch_fbeg, (void **) &c->d_slot_ptr,
                   (void **) &c->d_nbr, (void **) &c->d_slots, &c->B[0], &c->B[1], &c->B[2],
-                  (void **) &c->d_stage64, (void **) &c->d_partial, (void **) &c->d_q};
+                  (void **) &c->d_stage64, (void **) &c->d_partial, (void **) &c->d_q,
+                  &c->d_auxf, &c->d_vi, (void **) &c->d_dcoef, (void **) &c->d_chem};
     for (size_t i = 0; i < sizeof(p) / sizeof(p[0]); i++)
         if (*p[i]) { (void) hipFree(*p[i]); *p[i] = NULL; }
     free(c->hq);
@@ -85,9 +92,18 @@ extern "C" int charm_gpu_create(const charm_config_t *cfg, charm_gpu_ctx_t **out
         return CHARM_ERR_ARG;
     }
     for (int i = 0; i < cfg->n_comp; i++)
-        if (cfg->comp[i].cp_type != 0) {
-            charm_set_error("gpu_create: polynomial cp needs the FP64 EOS Newton solve, not built");
-            return CHARM_ERR_UNSUPPORTED;
+        if (cfg->comp[i].n_cp < 1 || cfg->comp[i].n_cp > 8 || !(cfg->comp[i].m > 0)) {
+            charm_set_error("gpu_create: species %d needs 1..8 cp coefficients and a positive molar mass", i);
+            return CHARM_ERR_ARG;
+        }
+    if (cfg->n_react < 0 || cfg->n_react > 4) { charm_set_error("gpu_create: %d reactions (at most 4)", cfg->n_react); return CHARM_ERR_ARG; }
+    for (int r = 0; r < cfg->n_react; r++)
+        for (int m = 0; m < 3; m++) {
+            const int l = cfg->react[r].left[m], q = cfg->react[r].right[m];
+            if (l < -1 || l >= cfg->n_comp || q < -1 || q >= cfg->n_comp) {
+                charm_set_error("gpu_create: reaction %d names species outside [0, %d)", r, cfg->n_comp);
+                return CHARM_ERR_ARG;
+            }
         }
     for (int i = 0; i < 6; i++) {
         int t = cfg->bnd_type[i];
@@ -124,6 +140,31 @@ extern "C" int charm_gpu_create(const charm_config_t *cfg, charm_gpu_ctx_t **out
     }
     for (int d = 0; d < 3; d++) P.grav[d] = cfg->grav[d];
     P.cfl = cfg->cfl;
+    // Navier-Stokes / real-gas parameters
+    P.newton_t = cfg->comp[0].cp_type != 0;                  // src/charm_eos.c:104,127
+    int any_poly = 0;
+    for (int i = 0; i < c->nc; i++) {
+        const charm_comp_desc_t &q = cfg->comp[i];
+        // charm_comp_calc_cp reads the power series when cp_type is not
+        // constant and ml_type is 1 (src/charm_globals.c:609-613)
+        P.cp_poly[i] = q.cp_type != 0 && q.ml_type == 1;
+        any_poly |= P.cp_poly[i];
+        P.n_cp[i] = q.n_cp;
+        for (int k = 0; k < 8; k++) P.cp_c[i][k] = k < q.n_cp ? q.cp[k] : 0.0;
+        P.ml_type[i] = q.ml_type; P.kp_type[i] = q.kp_type;
+        P.ml0[i] = q.ml0; P.kp0[i] = q.kp0; P.t0[i] = q.t0; P.ts[i] = q.ts; P.h0[i] = q.h0;
+        P.sig[i] = q.sig; P.ek[i] = q.ek;
+    }
+    P.t_ref = cfg->t_ref;
+    P.use_visc = cfg->use_visc != 0;
+    P.use_diff = cfg->use_diff != 0;
+    P.n_react = cfg->n_react;
+    P.chem_parity = cfg->chem_parity != 0;
+    for (int r = 0; r < cfg->n_react; r++) {
+        for (int m = 0; m < 3; m++) { P.r_left[r][m] = cfg->react[r].left[m]; P.r_right[r][m] = cfg->react[r].right[m]; }
+        P.r_a[r] = cfg->react[r].a; P.r_e[r] = cfg->react[r].e; P.r_n[r] = cfg->react[r].n;
+    }
+    c->ph = (P.newton_t || any_poly || P.use_visc || P.n_react > 0) ? 1 : 0;
     {
         // FP64 mass-matrix inverse (level 0); the octree's matrices are
         // diagonal, which is checked here
@@ -212,9 +253,17 @@ extern "C" int charm_gpu_upload_mesh(charm_gpu_ctx_t *c, const charm_mesh_t *m)
     DMALLOC(c->d_nbr, (size_t) m->n * 6 * sizeof(int));
     DMALLOC(c->d_aux, (size_t) m->naux * sizeof(int4));
     for (int i = 0; i < 3; i++) DMALLOC(c->B[i], ssz);
-    DMALLOC(c->d_stage64, (size_t) m->n * 4 * c->nv * sizeof(double));
+    // FP64 staging: the state, or the 36 auxiliary coefficients of the
+    // Navier-Stokes test hook, whichever is larger
+    DMALLOC(c->d_stage64, (size_t) m->n * (4 * c->nv > 36 || !c->ph ? 4 * c->nv : 36) * sizeof(double));
     DMALLOC(c->d_partial, (size_t) ((m->n + 255) / 256 + 1) * (c->nv + 1) * sizeof(double));
     DMALLOC(c->d_q, (size_t) m->n * sizeof(int4));
+    if (c->ph) {
+        DMALLOC(c->d_auxf, (size_t) m->n * 36 * c->esz);
+        DMALLOC(c->d_vi, (size_t) m->n * 16 * c->esz);
+        DMALLOC(c->d_dcoef, (size_t) m->n * c->nc * sizeof(double));
+        DMALLOC(c->d_chem, (size_t) m->n * sizeof(double));
+    }
     c->hq = (int32_t *) malloc((size_t) (m->n ? m->n : 1) * 4 * sizeof(int32_t));
     if (!c->hq) { charm_set_error("upload_mesh: out of memory"); free_mesh(c); return CHARM_ERR_MEM; }
     memcpy(c->hq, m->q, (size_t) m->n * 4 * sizeof(int32_t));
@@ -229,6 +278,10 @@ extern "C" int charm_gpu_upload_mesh(charm_gpu_ctx_t *c, const charm_mesh_t *m)
     if (m->naux)
         HIPCHK(hipMemcpyAsync(c->d_aux, m->aux, (size_t) m->naux * sizeof(int4), hipMemcpyHostToDevice, c->st));
     for (int i = 0; i < 3; i++) HIPCHK(hipMemsetAsync(c->B[i], 0, ssz, c->st));
+    if (c->ph) {
+        HIPCHK(hipMemsetAsync(c->d_dcoef, 0, (size_t) m->n * c->nc * sizeof(double), c->st));
+        HIPCHK(hipMemsetAsync(c->d_chem, 0, (size_t) m->n * sizeof(double), c->st));
+    }
     HIPCHK(hipStreamSynchronize(c->st));
     c->n = m->n; c->nch = m->nch; c->ncf = m->ncf; c->nslot = m->nslot; c->naux = m->naux;
     c->max_faces = m->chunk_faces;
@@ -283,6 +336,7 @@ static StageArgs stage_args(charm_gpu_ctx_t *c, const void *U, const void *U0, v
     A.n_elem = c->n; A.n_chunks = (int) c->nch; A.max_faces = c->max_faces; A.chunk_elems = c->chunk_elems;
     A.n_xcd = c->n_xcd;
     A.mode = mode; A.combine = combine; A.dt = dt;
+    A.visc = (c->ph && c->cfg.use_visc) ? c->d_vi : NULL;
     return A;
 }
 
@@ -307,12 +361,54 @@ static int check_flag(charm_gpu_ctx_t *c)
     return CHARM_OK;
 }
 
+static NsArgs ns_args(charm_gpu_ctx_t *c, const void *U, void *out, double dt)
+{
+    NsArgs A;
+    A.U = U; A.out = out; A.level = c->d_level; A.nbr = c->d_nbr; A.aux = c->d_aux;
+    A.chem_rhs = c->d_chem;
+    A.dcoef = (c->cfg.use_diff && c->nc >= 2) ? c->d_dcoef : NULL;
+    A.aux_in = c->d_auxf; A.aux_out = c->d_auxf; A.visc_out = c->d_vi;
+    A.n = c->n; A.dt = dt;
+    return A;
+}
+
+// The Navier-Stokes pieces in front of a stage (src/models/ns/charm_model.c:
+// 226-238): chemistry on `in` into `chem_out` (may be `in`; the energy source
+// goes to d_chem), then with viscosity the diffusion coefficients, the
+// auxiliary tau / q and the viscous integrals the stage adds.  Returns the
+// state the stage reads.
+static hipError_t ns_prepare(charm_gpu_ctx_t *c, const void *in, void *chem_out, double dt, const void **stage_in)
+{
+    hipError_t e = hipSuccess;
+    *stage_in = in;
+    if (c->cfg.model != CHARM_MODEL_NS || !c->ph) return hipSuccess;
+    if (c->cfg.n_react > 0) {
+        NsArgs A = ns_args(c, in, chem_out, dt);
+        if ((e = launch_ns(c->prec, 0, c->nc, A, c->P, c->st)) != hipSuccess) return e;
+        *stage_in = chem_out;
+    }
+    if (c->cfg.use_visc) {
+        NsArgs A = ns_args(c, *stage_in, NULL, dt);
+        if (A.dcoef && (e = launch_ns(c->prec, 1, c->nc, A, c->P, c->st)) != hipSuccess) return e;
+        if ((e = launch_ns(c->prec, 2, c->nc, A, c->P, c->st)) != hipSuccess) return e;
+        if ((e = launch_ns(c->prec, 3, c->nc, A, c->P, c->st)) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 extern "C" int charm_gpu_residual(charm_gpu_ctx_t *c, double *out, int64_t n)
 {
     int rc = need_mesh(c, n);
     if (rc) return rc;
+    if (c->ph && c->cfg.model == CHARM_MODEL_NS && c->cfg.use_visc) {
+        // viscous passes on the current state, without the chemistry
+        NsArgs N = ns_args(c, c->B[0], NULL, 0.0);
+        if (N.dcoef) HIPCHK(launch_ns(c->prec, 1, c->nc, N, c->P, c->st));
+        HIPCHK(launch_ns(c->prec, 2, c->nc, N, c->P, c->st));
+        HIPCHK(launch_ns(c->prec, 3, c->nc, N, c->P, c->st));
+    }
     StageArgs A = stage_args(c, c->B[0], c->B[0], c->B[1], 0, 0, 0.0);
-    HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, A, c->P, c->st));
+    HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st));
     rc = check_flag(c);
     if (rc) return rc;
     return copy_out(c, c->B[1], out, n);
@@ -323,7 +419,7 @@ extern "C" int charm_gpu_limiter_bj(charm_gpu_ctx_t *c)
     int rc = need_mesh(c, -1);
     if (rc) return rc;
     LimArgs A = lim_args(c, c->B[0], c->B[0], c->B[1], 1, 0);
-    HIPCHK(launch_limiter(c->prec, c->nc, A, c->P, c->st));
+    HIPCHK(launch_limiter(c->prec, c->nc, c->ph, A, c->P, c->st));
     void *t = c->B[0]; c->B[0] = c->B[1]; c->B[1] = t;
     HIPCHK(hipStreamSynchronize(c->st));
     return CHARM_OK;
@@ -337,37 +433,46 @@ static int step_once(charm_gpu_ctx_t *c, double dt)
     if (c->cfg.model == CHARM_MODEL_EULER) {
         // forward Euler + limiter, src/models/euler/charm_model.c:519-535
         StageArgs A = stage_args(c, B0, B0, B1, 1, 0, dt);
-        HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, A, c->P, c->st));
+        HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st));
         if (lim) {
             LimArgs L = lim_args(c, B1, B0, B0, 1, 0);
-            HIPCHK(launch_limiter(c->prec, c->nc, L, c->P, c->st));
+            HIPCHK(launch_limiter(c->prec, c->nc, c->ph, L, c->P, c->st));
         } else {
             c->B[0] = B1; c->B[1] = B0;
         }
         return CHARM_OK;
     }
-    // SSP-RK3 with the limiter inside each stage and the combines after it,
-    // src/models/ns/charm_model.c:244-266
+    // SSP-RK3 with the chemistry and viscous passes in front of each stage,
+    // the limiter inside it and the combines after it,
+    // src/models/ns/charm_model.c:226-266.  The first stage's chemistry
+    // writes a copy (B2): u^n in B0 stays the combine's base.
+    const void *X;
     if (lim) {
-        StageArgs A = stage_args(c, B0, B0, B1, 1, 0, dt);
-        HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, A, c->P, c->st));
+        HIPCHK(ns_prepare(c, B0, B2, dt, &X));
+        StageArgs A = stage_args(c, X, B0, B1, 1, 0, dt);
+        HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st));
         LimArgs L = lim_args(c, B1, B0, B2, 1, 0);
-        HIPCHK(launch_limiter(c->prec, c->nc, L, c->P, c->st));
-        A = stage_args(c, B2, B0, B1, 1, 0, dt);
-        HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, A, c->P, c->st));
+        HIPCHK(launch_limiter(c->prec, c->nc, c->ph, L, c->P, c->st));
+        HIPCHK(ns_prepare(c, B2, B2, dt, &X));
+        A = stage_args(c, X, B0, B1, 1, 0, dt);
+        HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st));
         L = lim_args(c, B1, B0, B2, 1, 1);
-        HIPCHK(launch_limiter(c->prec, c->nc, L, c->P, c->st));
-        A = stage_args(c, B2, B0, B1, 1, 0, dt);
-        HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, A, c->P, c->st));
+        HIPCHK(launch_limiter(c->prec, c->nc, c->ph, L, c->P, c->st));
+        HIPCHK(ns_prepare(c, B2, B2, dt, &X));
+        A = stage_args(c, X, B0, B1, 1, 0, dt);
+        HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st));
         L = lim_args(c, B1, B0, B0, 1, 2);
-        HIPCHK(launch_limiter(c->prec, c->nc, L, c->P, c->st));
+        HIPCHK(launch_limiter(c->prec, c->nc, c->ph, L, c->P, c->st));
     } else {
-        StageArgs A = stage_args(c, B0, B0, B1, 1, 0, dt);
-        HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, A, c->P, c->st));
-        A = stage_args(c, B1, B0, B2, 1, 1, dt);
-        HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, A, c->P, c->st));
-        A = stage_args(c, B2, B0, B0, 1, 2, dt);
-        HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, A, c->P, c->st));
+        HIPCHK(ns_prepare(c, B0, B2, dt, &X));
+        StageArgs A = stage_args(c, X, B0, B1, 1, 0, dt);
+        HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st));
+        HIPCHK(ns_prepare(c, B1, B1, dt, &X));
+        A = stage_args(c, X, B0, B2, 1, 1, dt);
+        HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st));
+        HIPCHK(ns_prepare(c, B2, B2, dt, &X));
+        A = stage_args(c, X, B0, B0, 1, 2, dt);
+        HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st));
     }
     return CHARM_OK;
 }
@@ -439,14 +544,14 @@ extern "C" int charm_gpu_time_kernels(charm_gpu_ctx_t *c, double dt, int reps, d
     float ms = 0.f;
     StageArgs A = stage_args(c, c->B[0], c->B[0], c->B[1], 1, 0, dt);
     HIPCHK(hipEventRecord(c->ev0, c->st));
-    for (int r = 0; r < reps; r++) HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, A, c->P, c->st));
+    for (int r = 0; r < reps; r++) HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st));
     HIPCHK(hipEventRecord(c->ev1, c->st));
     HIPCHK(hipEventSynchronize(c->ev1));
     HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     if (stage_ms) *stage_ms = ms / reps;
     LimArgs L = lim_args(c, c->B[1], c->B[0], c->B[2], 1, 1);
     HIPCHK(hipEventRecord(c->ev0, c->st));
-    for (int r = 0; r < reps; r++) HIPCHK(launch_limiter(c->prec, c->nc, L, c->P, c->st));
+    for (int r = 0; r < reps; r++) HIPCHK(launch_limiter(c->prec, c->nc, c->ph, L, c->P, c->st));
     HIPCHK(hipEventRecord(c->ev1, c->st));
     HIPCHK(hipEventSynchronize(c->ev1));
     HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
@@ -467,7 +572,7 @@ static int reduce(charm_gpu_ctx_t *c, int mode, double *host)
     RedArgs A;
     A.U = c->B[0]; A.level = c->d_level; A.partial = c->d_partial; A.result = c->d_result;
     A.n = c->n; A.mode = mode;
-    HIPCHK(launch_reduce(c->prec, c->nc, A, c->P, c->st));
+    HIPCHK(launch_reduce(c->prec, c->nc, c->ph, A, c->P, c->st));
     HIPCHK(hipMemcpyAsync(host, c->d_result, (size_t) (c->nv + 1) * sizeof(double), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     return CHARM_OK;
@@ -551,7 +656,7 @@ extern "C" int charm_gpu_amr_indicator(charm_gpu_ctx_t *c, double *err2, double 
     AmrArgs A;
     A.U = c->B[0]; A.level = c->d_level; A.nbr = c->d_nbr; A.aux = c->d_aux;
     A.err2 = c->d_stage64; A.rho_c = c->d_stage64 + n; A.n = n;
-    HIPCHK(launch_amr_indicator(c->prec, c->nc, A, c->P, c->st));
+    HIPCHK(launch_amr_indicator(c->prec, c->nc, c->ph, A, c->P, c->st));
     HIPCHK(hipMemcpyAsync(err2, A.err2, (size_t) n * sizeof(double), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipMemcpyAsync(rho_c, A.rho_c, (size_t) n * sizeof(double), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
@@ -640,5 +745,34 @@ extern "C" int charm_gpu_remesh(charm_gpu_ctx_t *c, const charm_mesh_t *m, const
     (void) hipFree(q_old);
     if (rc) return rc;
     if (e != hipSuccess) { charm_set_error("remesh: %s", hipGetErrorString(e)); return CHARM_ERR_HIP; }
+    return CHARM_OK;
+}
+
+// ------------------------------------------------ Navier-Stokes test hooks
+
+extern "C" int charm_gpu_ns_aux(charm_gpu_ctx_t *c, double *aux, int64_t n)
+{
+    int rc = need_mesh(c, n);
+    if (rc) return rc;
+    if (!c->ph || !aux) { charm_set_error("ns_aux: no Navier-Stokes terms in this configuration"); return CHARM_ERR_ARG; }
+    NsArgs N = ns_args(c, c->B[0], NULL, 0.0);
+    if (N.dcoef) HIPCHK(launch_ns(c->prec, 1, c->nc, N, c->P, c->st));
+    HIPCHK(launch_ns(c->prec, 2, c->nc, N, c->P, c->st));
+    // [36][n] -> [n][9][4] FP64
+    HIPCHK(launch_convert(c->prec, 1, c->d_auxf, c->d_stage64, n, 36, c->st));
+    HIPCHK(hipMemcpyAsync(aux, c->d_stage64, (size_t) n * 36 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return CHARM_OK;
+}
+
+extern "C" int charm_gpu_ns_chem(charm_gpu_ctx_t *c, double dt, double *chem_rhs, int64_t n)
+{
+    int rc = need_mesh(c, n);
+    if (rc) return rc;
+    if (!c->ph || c->cfg.n_react <= 0) { charm_set_error("ns_chem: no reactions in this configuration"); return CHARM_ERR_ARG; }
+    NsArgs N = ns_args(c, c->B[0], c->B[0], dt);
+    HIPCHK(launch_ns(c->prec, 0, c->nc, N, c->P, c->st));
+    if (chem_rhs) HIPCHK(hipMemcpyAsync(chem_rhs, c->d_chem, (size_t) n * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
     return CHARM_OK;
 }

@@ -22,6 +22,7 @@ struct StageArgs {
     int mode;                 // 0: write int_*; 1: c - dt A^-1 int
     int combine;              // 0: none, 1: w/4 + 3 u0/4, 2: 2w/3 + u0/3
     double dt;
+    const void *visc;         // [16][n] viscous integrals added before A^-1 (or null)
 };
 
 struct LimArgs {
@@ -71,13 +72,35 @@ struct ProjArgs {
     int nvk;                     // 4 V coefficients per leaf
 };
 
-hipError_t launch_stage(int prec, int nc, int flux, const StageArgs &A, const DevParams &P, hipStream_t st);
+// Navier-Stokes kernels (one thread per leaf)
+struct NsArgs {
+    const void *U;            // state
+    void *out;                // chemistry: state with the reacted species means
+    const uint8_t *level;
+    const int *nbr;
+    const int4 *aux;
+    double *chem_rhs;         // [n] energy source of the chemistry (or null)
+    double *dcoef;            // [n_comp][n] diffusion coefficients (or null)
+    const void *aux_in;       // [36][n] tau, q coefficients (viscous integrals)
+    void *aux_out;            // [36][n] (gradient pass)
+    void *visc_out;           // [16][n] viscous integrals of ru rv rw re
+    int64_t n;
+    double dt;
+};
+
+hipError_t launch_stage_f32_ph0(int nc, int flux, const StageArgs &A, const DevParams &P, hipStream_t st);
+hipError_t launch_stage_f32_ph1(int nc, int flux, const StageArgs &A, const DevParams &P, hipStream_t st);
+hipError_t launch_stage_f64_ph0(int nc, int flux, const StageArgs &A, const DevParams &P, hipStream_t st);
+hipError_t launch_stage_f64_ph1(int nc, int flux, const StageArgs &A, const DevParams &P, hipStream_t st);
+hipError_t launch_stage(int prec, int nc, int flux, int ph, const StageArgs &A, const DevParams &P, hipStream_t st);
 size_t stage_smem_bytes(int prec, int nc, int max_faces);
-hipError_t launch_limiter(int prec, int nc, const LimArgs &A, const DevParams &P, hipStream_t st);
-hipError_t launch_reduce(int prec, int nc, const RedArgs &A, const DevParams &P, hipStream_t st);
+hipError_t launch_limiter(int prec, int nc, int ph, const LimArgs &A, const DevParams &P, hipStream_t st);
+hipError_t launch_reduce(int prec, int nc, int ph, const RedArgs &A, const DevParams &P, hipStream_t st);
 hipError_t launch_flux_points(int prec, int nc, int flux, const FluxArgs &A, const DevParams &P, hipStream_t st);
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st);
-hipError_t launch_amr_indicator(int prec, int nc, const AmrArgs &A, const DevParams &P, hipStream_t st);
+// what: 0 chemistry, 1 diffusion coefficients, 2 tau / q, 3 viscous integrals
+hipError_t launch_ns(int prec, int what, int nc, const NsArgs &A, const DevParams &P, hipStream_t st);
+hipError_t launch_amr_indicator(int prec, int nc, int ph, const AmrArgs &A, const DevParams &P, hipStream_t st);
 hipError_t launch_amr_project(int prec, const ProjArgs &A, hipStream_t st);
 hipError_t launch_convert(int prec, int to_f64, const void *src, void *dst, int64_t n, int nvk, hipStream_t st);
 

@@ -1,349 +1,14 @@
-// charm_kernels.hip -- gfx950 kernels of the DG residual / time-step path.
+// charm_kernels.hip -- gfx950 kernels of the DG residual / time-step path
+// (limiter, reductions, pointwise fluxes, conversions; the stage kernel is
+// in charm_stage.hpp, instantiated by charm_stage_*.hip).
 //
-// State layout (structure of arrays): U[(4 v + k) n + e] for element e of n,
-// variable v = ru rv rw re rc[..], Taylor coefficient k (1, (x-xc)/h,
-// (y-yc)/h, (z-zc)/h): a wave's per-element accesses are unit-stride.
-//
-// One RK stage = k_stage over workgroup chunks (<= 128 Morton-consecutive
-// elements each):
-//   phase 1  one thread per (sub)face touching the chunk: both traces at the
-//            six face Gauss points, the Riemann flux, and the three flux
-//            moments per variable (sum q wJ, sum q phi_t1 wJ, sum q phi_t2 wJ in
-//            the small side's face frame) written to LDS;
-//   phase 2  one thread per element: volume integral at the 8 Gauss points,
-//            gather of its face moments from LDS (no atomics: a fixed slot
-//            list per element), A^-1 (diagonal on the cube octree) and the
-//            SSP-RK update / combine, one write of the new state.
-// Faces shared by two chunks are evaluated by both (each keeps its side).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "charm_device.hpp"
 #include "charm_kernels.h"
+#include "charm_stage.hpp"
 
 namespace charm {
-
-// Workgroups are dispatched round-robin over the XCDs (each with its own
-// L2): workgroup b runs on XCD b % n_xcd.  Renumber so that every XCD walks a
-// contiguous range of Morton-ordered chunks / element blocks and neighbours
-// share its L2.  The grid is n_xcd * ceil(n / n_xcd) workgroups.
-__device__ __forceinline__ int xcd_block(int n_xcd)
-{
-    const int per = (int) gridDim.x / n_xcd;
-    return ((int) blockIdx.x % n_xcd) * per + (int) blockIdx.x / n_xcd;
-}
-
-// SSP-RK combines rk_1 (c = c/4 + 3 c_old/4) and rk_2 (c = 2c/3 + c_old/3)
-// in the reference's operation order, src/models/ns/charm_model.c:136-184
-template <typename R>
-__device__ __forceinline__ R rk_combine(int mode, R w, R u0)
-{
-    if (mode == 1) { R c = w * R(0.25); return c + R(0.75) * u0; }
-    R c = w * R(2);
-    c /= R(3);
-    return c + u0 / R(3);
-}
-
-// per-(v, k) array bases are wave-uniform (scalar registers); the element
-// index is the 32-bit per-lane offset
-template <typename R, int NV>
-__device__ __forceinline__ void load_elem(const R *__restrict__ U, int64_t n, int e, R *u)
-{
-    #pragma unroll
-    for (int i = 0; i < 4 * NV; i++) u[i] = (U + i * n)[e];
-}
-
-template <typename R, int NV>
-__device__ __forceinline__ void trace(const R *u, R px, R py, R pz, R *t)
-{
-    #pragma unroll
-    for (int v = 0; v < NV; v++)
-        t[v] = u[4*v] + px * u[4*v+1] + py * u[4*v+2] + pz * u[4*v+3];
-}
-
-// spread (normal, t1, t2) components onto (x, y, z)
-template <typename R>
-__device__ __forceinline__ void to_xyz(int a, R n, R t1, R t2, R &x, R &y, R &z)
-{
-    x = (a == 0) ? n : t1;
-    y = (a == 1) ? n : ((a == 0) ? t1 : t2);
-    z = (a == 2) ? n : t2;
-}
-
-// Moments of one face normal to axis AX from the two element states u0, u1
-// (u1 unused on a boundary face):
-// out[3][NV] = wJ * (sum q, sum q phi_t1, sum q phi_t2) over the six points,
-// phi_t the tangential Taylor coordinates of the small (or side-0) face.
-template <typename R, int NC, int FLUX, int AX, bool BND>
-__device__ __forceinline__ bool face_core(const DevParams &P, const R *u0, const R *u1, int info,
-                                          int lown, R *out, int bnd_axis)
-{
-    constexpr int NV = 4 + NC;
-    constexpr int T1 = (AX == 0) ? 1 : 0, T2 = (AX == 2) ? 1 : 2;
-    const int dir0 = info & 7, kind = (info >> 3) & 3, sub = (info >> 5) & 3;
-    const R s = (dir0 & 1) ? R(1) : R(-1);              // outward normal of side 0
-    const R h = ldexp(R(1), -lown);
-    const R wj = h * h * R(1.0 / 6.0);
-    // normal-basis values on the face: side 0 at +-1/2 by its face parity
-    const R pa0 = R(0.5) * s, pa1 = -pa0;
-    // the big side of a hanging face sees the small face's points at
-    // delta + offset / 2 in its own units
-    const R d1 = (sub & 1) ? R(0.25) : R(-0.25);
-    const R d2 = (sub & 2) ? R(0.25) : R(-0.25);
-    const bool big0 = (kind == FACE_HANG1), big1 = (kind == FACE_HANG0);
-    const R m0 = big0 ? R(0.5) : R(1), m1 = big1 ? R(0.5) : R(1);
-    const R o01 = big0 ? d1 : R(0), o02 = big0 ? d2 : R(0);
-    const R o11 = big1 ? d1 : R(0), o12 = big1 ? d2 : R(0);
-    R S0[NV], S1[NV], S2[NV];
-    bool ok = true;
-    #pragma unroll
-    for (int k = 0; k < 6; k++) {
-        const R g1 = gp_t1<R>(k), g2 = gp_t2<R>(k);
-        R ph[3], t[NV];
-        ph[AX] = pa0; ph[T1] = o01 + m0 * g1; ph[T2] = o02 + m0 * g2;
-        trace<R, NV>(u0, ph[0], ph[1], ph[2], t);
-        Prim<R, NC> pl, pr;
-        cons_to_prim<R, NC>(P, t, pl);
-        if (BND) {
-            bnd_state<R, NC>(P, dir0, AX, s, pl, pr, bnd_axis);
-        } else {
-            ph[AX] = pa1; ph[T1] = o11 + m1 * g1; ph[T2] = o12 + m1 * g2;
-            trace<R, NV>(u1, ph[0], ph[1], ph[2], t);
-            cons_to_prim<R, NC>(P, t, pr);
-        }
-        // face frame (s e_AX, e_T1, e_T2)
-        Frame<R, NC> fl, fr;
-        {
-            const R vl[3] = {pl.u, pl.v, pl.w}, vr[3] = {pr.u, pr.v, pr.w};
-            fl.r = pl.r; fl.e_tot = pl.e_tot; fl.p = pl.p; fl.cz = pl.cz; fl.gam = pl.gam;
-            fr.r = pr.r; fr.e_tot = pr.e_tot; fr.p = pr.p; fr.cz = pr.cz; fr.gam = pr.gam;
-            fl.un = s * vl[AX]; fr.un = s * vr[AX];
-            fl.ut1 = vl[T1]; fr.ut1 = vr[T1];
-            fl.ut2 = vl[T2]; fr.ut2 = vr[T2];
-            #pragma unroll
-            for (int i = 0; i < NC; i++) { fl.c[i] = pl.c[i]; fr.c[i] = pr.c[i]; }
-        }
-        R q[NV], qg[NV];
-        ok &= flux_x<R, NC, FLUX>(fl, fr, q);
-        qg[AX] = s * q[0]; qg[T1] = q[1]; qg[T2] = q[2];
-        #pragma unroll
-        for (int v = 3; v < NV; v++) qg[v] = q[v];
-        #pragma unroll
-        for (int v = 0; v < NV; v++) {
-            if (k == 0) { S0[v] = qg[v]; S1[v] = qg[v] * g1; S2[v] = qg[v] * g2; }
-            else { S0[v] += qg[v]; S1[v] += qg[v] * g1; S2[v] += qg[v] * g2; }
-        }
-    }
-    #pragma unroll
-    for (int v = 0; v < NV; v++) {
-        out[v] = S0[v] * wj;
-        out[NV + v] = S1[v] * wj;
-        out[2 * NV + v] = S2[v] * wj;
-    }
-    return ok;
-}
-
-template <typename R, int NC, int FLUX, int AX>
-__device__ __forceinline__ bool face_moments(const DevParams &P, const R *__restrict__ U, int64_t n,
-                                             const uint8_t *__restrict__ level, int4 rec, R *out)
-{
-    constexpr int NV = 4 + NC;
-    const int kind = (rec.z >> 3) & 3;
-    const int lev0 = level[rec.x];
-    R u0[4 * NV], u1[4 * NV];
-    load_elem<R, NV>(U, n, rec.x, u0);
-    load_elem<R, NV>(U, n, rec.y, u1);
-    return face_core<R, NC, FLUX, AX, false>(P, u0, u1, rec.z, kind == FACE_HANG1 ? lev0 + 1 : lev0, out, AX);
-}
-
-// Boundary faces: the element state is rotated into the frame (a, t1, t2)
-// of its face (Taylor slopes and momentum components permuted), evaluated
-// with the x-normal code, and the momentum moments are rotated back; one
-// code path serves the three axes.
-template <typename R, int NC, int FLUX>
-__device__ __forceinline__ bool face_moments_bnd(const DevParams &P, const R *__restrict__ U, int64_t n,
-                                                 const uint8_t *__restrict__ level, int4 rec, R *out)
-{
-    constexpr int NV = 4 + NC;
-    const int ax = (rec.z & 7) >> 1;
-    R u[4 * NV], w[4 * NV];
-    load_elem<R, NV>(U, n, rec.x, u);
-    #pragma unroll
-    for (int v = 0; v < NV; v++) {
-        // source variable: momentum components follow the axis permutation
-        R c[4];
-        #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            if (v < 3) {
-                const R x0 = u[4 * 0 + k], x1 = u[4 * 1 + k], x2 = u[4 * 2 + k];
-                // frame var v -> global var: v=0 -> ax, v=1 -> t1, v=2 -> t2
-                const int gv = (v == 0) ? ax : ((v == 1) ? (ax == 0 ? 1 : 0) : (ax == 2 ? 1 : 2));
-                c[k] = (gv == 0) ? x0 : ((gv == 1) ? x1 : x2);
-            } else {
-                c[k] = u[4 * v + k];
-            }
-        }
-        const R ca = (ax == 0) ? c[1] : ((ax == 1) ? c[2] : c[3]);
-        const R ct1 = (ax == 0) ? c[2] : c[1];
-        const R ct2 = (ax == 2) ? c[2] : c[3];
-        w[4 * v] = c[0]; w[4 * v + 1] = ca; w[4 * v + 2] = ct1; w[4 * v + 3] = ct2;
-    }
-    R o[3 * NV];
-    const bool ok = face_core<R, NC, FLUX, 0, true>(P, w, w, rec.z, level[rec.x], o, ax);
-    #pragma unroll
-    for (int m = 0; m < 3; m++) {
-        const R f0 = o[m * NV + 0], f1 = o[m * NV + 1], f2 = o[m * NV + 2];
-        // global var g takes frame var: g == ax -> 0, g == t1 -> 1, g == t2 -> 2
-        out[m * NV + 0] = (ax == 0) ? f0 : f1;
-        out[m * NV + 1] = (ax == 1) ? f0 : ((ax == 0) ? f1 : f2);
-        out[m * NV + 2] = (ax == 2) ? f0 : f2;
-        #pragma unroll
-        for (int v = 3; v < NV; v++) out[m * NV + v] = o[m * NV + v];
-    }
-    return ok;
-}
-
-// volume integral, src/models/euler/charm_model.c:66-132: with the Taylor
-// basis on a cube, grad phi_k = e_k / h and phi_k(gp) = +-1/(2 sqrt 3), so
-// int[v][1+d] -= (wJ/h) sum_gp F_d,v and the gravity term adds
-// -g_v sum_gp rho phi_k wJ.
-template <typename R, int NC>
-__device__ __forceinline__ void volume_integral(const DevParams &P, const R *u, R h, R *res)
-{
-    constexpr int NV = 4 + NC;
-    const R ph = R(0.28867513459481288225);           // 1/(2 sqrt 3)
-    const R hinv = rcp(h);
-    const R wj = h * h * h * R(0.125);
-    R SF[3][NV];
-    R G[4] = {R(0), R(0), R(0), R(0)};
-    #pragma unroll
-    for (int d = 0; d < 3; d++)
-        #pragma unroll
-        for (int v = 0; v < NV; v++) SF[d][v] = R(0);
-    #pragma unroll
-    for (int g = 0; g < 8; g++) {
-        const R px = (g & 1) ? ph : -ph, py = (g & 2) ? ph : -ph, pz = (g & 4) ? ph : -ph;
-        R t[NV];
-        trace<R, NV>(u, px, py, pz, t);
-        Prim<R, NC> p;
-        cons_to_prim<R, NC>(P, t, p);
-        const R m[3] = {t[0], t[1], t[2]};
-        const R vel[3] = {p.u, p.v, p.w};
-        #pragma unroll
-        for (int d = 0; d < 3; d++) {
-            SF[d][0] += m[d] * p.u + (d == 0 ? p.p : R(0));
-            SF[d][1] += m[d] * p.v + (d == 1 ? p.p : R(0));
-            SF[d][2] += m[d] * p.w + (d == 2 ? p.p : R(0));
-            SF[d][3] += m[d] * p.e_tot + p.p * vel[d];
-            #pragma unroll
-            for (int i = 0; i < NC; i++) SF[d][4 + i] += m[d] * p.c[i];
-        }
-        G[0] += p.r; G[1] += p.r * px; G[2] += p.r * py; G[3] += p.r * pz;
-    }
-    const R f = wj * hinv;
-    #pragma unroll
-    for (int v = 0; v < NV; v++) {
-        res[4*v+0] = R(0);
-        res[4*v+1] = -SF[0][v] * f;
-        res[4*v+2] = -SF[1][v] * f;
-        res[4*v+3] = -SF[2][v] * f;
-    }
-    #pragma unroll
-    for (int v = 0; v < 3; v++) {
-        const R gv = cst<R>(P.grav[v], P.grav_f[v]);
-        #pragma unroll
-        for (int k = 0; k < 4; k++) res[4*v+k] -= gv * G[k] * wj;
-    }
-}
-
-template <typename R, int NC, int FLUX, int BS>
-__global__ __launch_bounds__(BS)
-void k_stage(StageArgs A, DevParams P)
-{
-    constexpr int NV = 4 + NC;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    R *mom = reinterpret_cast<R *>(smem_raw);                       // [maxf][3][NV]
-    int *finfo = reinterpret_cast<int *>(smem_raw + (size_t) A.max_faces * 3 * NV * sizeof(R));
-    const R *__restrict__ U = reinterpret_cast<const R *>(A.U);
-    const int c = xcd_block(A.n_xcd);
-    if (c >= A.n_chunks) return;                 // whole workgroup exits together
-    const int fb = A.ch_fbeg[c], fe = A.ch_fbeg[c + 1];
-    const int eb = A.ch_ebeg[c], ee = A.ch_ebeg[c + 1];
-    bool ok = true;
-    for (int f = fb + (int) threadIdx.x; f < fe; f += blockDim.x) {
-        const int4 rec = A.cf4[f];
-        R *dst = mom + (size_t) (f - fb) * 3 * NV;
-        // faces of a chunk are sorted by (axis, kind): waves are mostly
-        // uniform in the branch taken here
-        const int ax = (rec.z & 7) >> 1;
-        if (((rec.z >> 3) & 3) == FACE_BND) {
-            ok &= face_moments_bnd<R, NC, FLUX>(P, U, A.n_elem, A.level, rec, dst);
-        } else if (ax == 0) {
-            ok &= face_moments<R, NC, FLUX, 0>(P, U, A.n_elem, A.level, rec, dst);
-        } else if (ax == 1) {
-            ok &= face_moments<R, NC, FLUX, 1>(P, U, A.n_elem, A.level, rec, dst);
-        } else {
-            ok &= face_moments<R, NC, FLUX, 2>(P, U, A.n_elem, A.level, rec, dst);
-        }
-        finfo[f - fb] = rec.z;
-    }
-    __syncthreads();
-    const int e = eb + (int) threadIdx.x;
-    if (e < ee) {
-        R u[4 * NV], res[4 * NV];
-        load_elem<R, NV>(U, A.n_elem, e, u);
-        const int lev = A.level[e];
-        const R h = ldexp(R(1), -lev);
-        volume_integral<R, NC>(P, u, h, res);
-        const int s_end = A.slot_ptr[e + 1];
-        for (int si = A.slot_ptr[e]; si < s_end; si++) {
-            const int sl = A.slots[si];
-            const int fl = sl >> 1, side = sl & 1;
-            const int info = finfo[fl];
-            const int dir0 = info & 7, kind = (info >> 3) & 3, sub = (info >> 5) & 3;
-            const int a = dir0 >> 1;
-            const R sg = side ? R(-1) : R(1);
-            const R pa = ((dir0 & 1) ^ side) ? R(0.5) : R(-0.5);
-            const bool big = (kind == FACE_HANG0 && side == 1) || (kind == FACE_HANG1 && side == 0);
-            const R mu = big ? R(0.5) : R(1);
-            const R d1 = big ? ((sub & 1) ? R(0.25) : R(-0.25)) : R(0);
-            const R d2 = big ? ((sub & 2) ? R(0.25) : R(-0.25)) : R(0);
-            const R *m = mom + (size_t) fl * 3 * NV;
-            #pragma unroll
-            for (int v = 0; v < NV; v++) {
-                const R M0 = sg * m[v], M1 = sg * m[NV + v], M2 = sg * m[2 * NV + v];
-                R cx, cy, cz;
-                to_xyz<R>(a, pa * M0, mu * M1 + d1 * M0, mu * M2 + d2 * M0, cx, cy, cz);
-                res[4*v+0] += M0;
-                res[4*v+1] += cx;
-                res[4*v+2] += cy;
-                res[4*v+3] += cz;
-            }
-        }
-        R *out = reinterpret_cast<R *>(A.out);
-        const int64_t n = A.n_elem;
-        if (A.mode == 0) {
-            #pragma unroll
-            for (int i = 0; i < 4 * NV; i++) (out + i * n)[e] = res[i];
-        } else {
-            // c -= dt * A^-1 int (src/models/euler/charm_model.c:402-433); A^-1
-            // of a level-l cube is the FP64 level-0 inverse times 8^l (exact
-            // power-of-two scaling), diagonal on the cube octree; then the
-            // SSP-RK combine when no limiter sits in between
-            const R sc = ldexp(R(1), 3 * lev);
-            const R dt = R(A.dt);
-            const R *__restrict__ U0 = reinterpret_cast<const R *>(A.U0);
-            #pragma unroll
-            for (int v = 0; v < NV; v++)
-                #pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const R ai = cst<R>(P.ainv0[k], P.ainv0_f[k]) * sc;
-                    const R w = u[4*v+k] - dt * (ai * res[4*v+k]);
-                    (out + (4*v+k) * n)[e] = A.combine ? rk_combine<R>(A.combine, w, (U0 + (4*v+k) * n)[e]) : w;
-                }
-        }
-    }
-    if (!ok) A.err[0] = 1;
-}
 
 // Barth-Jespersen limiter, src/limiters/charm_limiter_bj.c:11-332, fused
 // with the SSP-RK combine that follows it (src/models/ns/charm_model.c:
@@ -352,7 +17,7 @@ void k_stage(StageArgs A, DevParams P)
 // average of its four small neighbours, a boundary face the mean pushed
 // through the boundary condition.  Only coefficient 0 of neighbours is read,
 // so the slopes of W may be limited while other threads read means.
-template <typename R, int NC>
+template <typename R, int NC, int PH>
 __global__ __launch_bounds__(256)
 void k_limiter(LimArgs A, DevParams P)
 {
@@ -379,9 +44,9 @@ void k_limiter(LimArgs A, DevParams P)
                 #pragma unroll
                 for (int v = 0; v < NV; v++) t[v] = u[4*v];
                 Prim<R, NC> pi, po;
-                cons_to_prim<R, NC>(P, t, pi);
+                cons_to_prim<R, NC, PH>(P, t, pi);
                 const int face = -1 - nb, a = face >> 1;
-                bnd_state<R, NC>(P, face, a, (face & 1) ? R(1) : R(-1), pi, po, -1);
+                bnd_state<R, NC, PH>(P, face, a, (face & 1) ? R(1) : R(-1), pi, po, -1);
                 m[0] = po.r * po.u; m[1] = po.r * po.v; m[2] = po.r * po.w;
                 m[3] = po.r * (po.e + R(0.5) * (po.u * po.u + po.v * po.v + po.w * po.w));
                 #pragma unroll
@@ -435,7 +100,7 @@ void k_limiter(LimArgs A, DevParams P)
 }
 
 // per-element CFL step and FP64 conservation sums, reduced per block
-template <typename R, int NC>
+template <typename R, int NC, int PH>
 __global__ __launch_bounds__(256)
 void k_reduce(RedArgs A, DevParams P)
 {
@@ -456,7 +121,7 @@ void k_reduce(RedArgs A, DevParams P)
         #pragma unroll
         for (int v = 0; v < NV; v++) t[v] = u[4 * v * A.n];
         Prim<R, NC> p;
-        cons_to_prim<R, NC>(P, t, p);
+        cons_to_prim<R, NC, PH>(P, t, p);
         const double len = A.mode == 0 ? vol : h;
         const double spd = sqrt((double) (p.u * p.u + p.v * p.v + p.w * p.w)) + (double) p.cz;
         loc[NV] = P.cfl * len / spd;
@@ -587,46 +252,10 @@ hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st)
 }
 
 // ---------------------------------------------------------------- launchers
-template <typename R, int NC, int BS>
-static hipError_t launch_stage_bs(int flux, const StageArgs &A, const DevParams &P, hipStream_t st)
+hipError_t launch_stage(int prec, int nc, int flux, int ph, const StageArgs &A, const DevParams &P, hipStream_t st)
 {
-    constexpr int NV = 4 + NC;
-    const size_t sh = (size_t) A.max_faces * (3 * NV * sizeof(R) + sizeof(int));
-    const dim3 grid((unsigned) (((A.n_chunks + A.n_xcd - 1) / A.n_xcd) * A.n_xcd)), block(BS);
-    switch (flux) {
-    case FLUX_LF: hipLaunchKernelGGL((k_stage<R, NC, FLUX_LF, BS>), grid, block, sh, st, A, P); break;
-    case FLUX_HLLC: hipLaunchKernelGGL((k_stage<R, NC, FLUX_HLLC, BS>), grid, block, sh, st, A, P); break;
-    case FLUX_GODUNOV: hipLaunchKernelGGL((k_stage<R, NC, FLUX_GODUNOV, BS>), grid, block, sh, st, A, P); break;
-    case FLUX_CD: hipLaunchKernelGGL((k_stage<R, NC, FLUX_CD, BS>), grid, block, sh, st, A, P); break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
-
-// one thread per chunk element: 128- or 256-element chunks
-template <typename R, int NC>
-static hipError_t launch_stage_nc(int flux, const StageArgs &A, const DevParams &P, hipStream_t st)
-{
-    if (A.chunk_elems <= 128) return launch_stage_bs<R, NC, 128>(flux, A, P, st);
-    if (A.chunk_elems <= 256) return launch_stage_bs<R, NC, 256>(flux, A, P, st);
-    return hipErrorInvalidValue;
-}
-
-template <typename R>
-static hipError_t launch_stage_r(int nc, int flux, const StageArgs &A, const DevParams &P, hipStream_t st)
-{
-    switch (nc) {
-    case 1: return launch_stage_nc<R, 1>(flux, A, P, st);
-    case 2: return launch_stage_nc<R, 2>(flux, A, P, st);
-    case 3: return launch_stage_nc<R, 3>(flux, A, P, st);
-    case 4: return launch_stage_nc<R, 4>(flux, A, P, st);
-    }
-    return hipErrorInvalidValue;
-}
-
-hipError_t launch_stage(int prec, int nc, int flux, const StageArgs &A, const DevParams &P, hipStream_t st)
-{
-    return prec ? launch_stage_r<float>(nc, flux, A, P, st) : launch_stage_r<double>(nc, flux, A, P, st);
+    if (prec) return ph ? launch_stage_f32_ph1(nc, flux, A, P, st) : launch_stage_f32_ph0(nc, flux, A, P, st);
+    return ph ? launch_stage_f64_ph1(nc, flux, A, P, st) : launch_stage_f64_ph0(nc, flux, A, P, st);
 }
 
 size_t stage_smem_bytes(int prec, int nc, int max_faces)
@@ -634,44 +263,46 @@ size_t stage_smem_bytes(int prec, int nc, int max_faces)
     return (size_t) max_faces * (3 * (4 + nc) * (prec ? 4 : 8) + sizeof(int));
 }
 
-template <typename R>
+template <typename R, int PH>
 static hipError_t launch_limiter_r(int nc, const LimArgs &A, const DevParams &P, hipStream_t st)
 {
     const int64_t nb = (A.n + 255) / 256;
     const dim3 grid((unsigned) (((nb + A.n_xcd - 1) / A.n_xcd) * A.n_xcd)), block(256);
     switch (nc) {
-    case 1: hipLaunchKernelGGL((k_limiter<R, 1>), grid, block, 0, st, A, P); break;
-    case 2: hipLaunchKernelGGL((k_limiter<R, 2>), grid, block, 0, st, A, P); break;
-    case 3: hipLaunchKernelGGL((k_limiter<R, 3>), grid, block, 0, st, A, P); break;
-    case 4: hipLaunchKernelGGL((k_limiter<R, 4>), grid, block, 0, st, A, P); break;
+    case 1: hipLaunchKernelGGL((k_limiter<R, 1, PH>), grid, block, 0, st, A, P); break;
+    case 2: hipLaunchKernelGGL((k_limiter<R, 2, PH>), grid, block, 0, st, A, P); break;
+    case 3: hipLaunchKernelGGL((k_limiter<R, 3, PH>), grid, block, 0, st, A, P); break;
+    case 4: hipLaunchKernelGGL((k_limiter<R, 4, PH>), grid, block, 0, st, A, P); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_limiter(int prec, int nc, const LimArgs &A, const DevParams &P, hipStream_t st)
+hipError_t launch_limiter(int prec, int nc, int ph, const LimArgs &A, const DevParams &P, hipStream_t st)
 {
-    return prec ? launch_limiter_r<float>(nc, A, P, st) : launch_limiter_r<double>(nc, A, P, st);
+    if (ph) return prec ? launch_limiter_r<float, 1>(nc, A, P, st) : launch_limiter_r<double, 1>(nc, A, P, st);
+    return prec ? launch_limiter_r<float, 0>(nc, A, P, st) : launch_limiter_r<double, 0>(nc, A, P, st);
 }
 
-template <typename R>
+template <typename R, int PH>
 static hipError_t launch_reduce_r(int nc, const RedArgs &A, const DevParams &P, hipStream_t st)
 {
     const dim3 grid((unsigned) ((A.n + 255) / 256)), block(256);
     switch (nc) {
-    case 1: hipLaunchKernelGGL((k_reduce<R, 1>), grid, block, 0, st, A, P); break;
-    case 2: hipLaunchKernelGGL((k_reduce<R, 2>), grid, block, 0, st, A, P); break;
-    case 3: hipLaunchKernelGGL((k_reduce<R, 3>), grid, block, 0, st, A, P); break;
-    case 4: hipLaunchKernelGGL((k_reduce<R, 4>), grid, block, 0, st, A, P); break;
+    case 1: hipLaunchKernelGGL((k_reduce<R, 1, PH>), grid, block, 0, st, A, P); break;
+    case 2: hipLaunchKernelGGL((k_reduce<R, 2, PH>), grid, block, 0, st, A, P); break;
+    case 3: hipLaunchKernelGGL((k_reduce<R, 3, PH>), grid, block, 0, st, A, P); break;
+    case 4: hipLaunchKernelGGL((k_reduce<R, 4, PH>), grid, block, 0, st, A, P); break;
     default: return hipErrorInvalidValue;
     }
     hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, st, A.partial, (int) grid.x, 4 + nc, A.result);
     return hipGetLastError();
 }
 
-hipError_t launch_reduce(int prec, int nc, const RedArgs &A, const DevParams &P, hipStream_t st)
+hipError_t launch_reduce(int prec, int nc, int ph, const RedArgs &A, const DevParams &P, hipStream_t st)
 {
-    return prec ? launch_reduce_r<float>(nc, A, P, st) : launch_reduce_r<double>(nc, A, P, st);
+    if (ph) return prec ? launch_reduce_r<float, 1>(nc, A, P, st) : launch_reduce_r<double, 1>(nc, A, P, st);
+    return prec ? launch_reduce_r<float, 0>(nc, A, P, st) : launch_reduce_r<double, 0>(nc, A, P, st);
 }
 
 template <typename R, int NC>

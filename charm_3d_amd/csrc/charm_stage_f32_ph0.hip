@@ -1,0 +1,11 @@
+// charm_stage_f32_ph0.hip -- stage kernels, float arithmetic, constant-cp Euler (PH = 0) variant
+#include "charm_stage.hpp"
+
+namespace charm {
+
+hipError_t launch_stage_f32_ph0(int nc, int flux, const StageArgs &A, const DevParams &P, hipStream_t st)
+{
+    return launch_stage_t<float, 0>(nc, flux, A, P, st);
+}
+
+} // namespace charm

@@ -1,0 +1,11 @@
+// charm_stage_f64_ph1.hip -- stage kernels, double arithmetic, Navier-Stokes / real-gas (PH = 1) variant
+#include "charm_stage.hpp"
+
+namespace charm {
+
+hipError_t launch_stage_f64_ph1(int nc, int flux, const StageArgs &A, const DevParams &P, hipStream_t st)
+{
+    return launch_stage_t<double, 1>(nc, flux, A, P, st);
+}
+
+} // namespace charm

@@ -67,7 +67,16 @@ typedef struct {
     int    n_cp;
     double cp[8];              /* cp = sum cp[i] T^i */
     double ml0, kp0, t0, ts, h0;
+    int    kp_type;            /* 0 const, 1 Sutherland */
+    double sig, ek;            /* Lennard-Jones sigma (Angstrom), eps / k (K): diffusion */
 } charm_comp_desc_t;
+
+/* one reaction (include/charm_globals.h:102-110): species indices (-1:
+ * none) and k = a T^n exp(-e / (R T)) */
+typedef struct {
+    int    left[3], right[3];
+    double a, e, n;
+} charm_reaction_desc_t;
 
 /* the task.yaml keys this path consumes (src/charm_init_yaml.cpp:333-426) */
 typedef struct {
@@ -92,6 +101,14 @@ typedef struct {
     int    min_level;          /* MIN_LEVEL */
     int    max_level;          /* MAX_LEVEL */
     int    refine_period;      /* REFINE_PERIOD (0: no adaptation) */
+    /* Navier-Stokes model (src/charm_init_yaml.cpp:394-426) */
+    int    use_visc;           /* VISCOSITY */
+    int    use_diff;           /* DIFFUSION */
+    double t_ref;              /* T_REF of the enthalpy */
+    int    n_react;
+    charm_reaction_desc_t react[4];
+    int    chem_parity;        /* 1: the reference's chem_w[species] index quirk
+                                  (src/models/ns/charm_model_ns_chem.c:97) */
 } charm_config_t;
 
 /* ------------------------------------------------------------ forest */
@@ -190,6 +207,14 @@ int charm_gpu_last_timing(charm_gpu_ctx_t *ctx, double *stage_ms, double *limite
  * of the previous mesh before any kernel runs. */
 int charm_gpu_amr_indicator(charm_gpu_ctx_t *ctx, double *err2, double *rho_c, int64_t n);
 int charm_gpu_remesh(charm_gpu_ctx_t *ctx, const charm_mesh_t *mesh, const int64_t *map, int64_t n_map);
+
+/* Navier-Stokes pieces on the current state, for checks: ns_aux = the BR1
+ * auxiliary stress and heat flux, FP64 [n][9][4] (tau xx yy zz xy xz yz,
+ * q x y z; src/models/ns/charm_model_ns_diff_grad.c:485-507); ns_chem = the
+ * chemistry applied to the state with step dt and its energy source [n]
+ * (src/models/ns/charm_model_ns_chem.c:111-202). */
+int charm_gpu_ns_aux(charm_gpu_ctx_t *ctx, double *aux, int64_t n);
+int charm_gpu_ns_chem(charm_gpu_ctx_t *ctx, double dt, double *chem_rhs, int64_t n);
 
 /* refine (+1) / coarsen (-1 on all eight members of a family) / keep (0)
  * decisions from the estimator, with the criteria of

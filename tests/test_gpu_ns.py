@@ -1,0 +1,186 @@
+"""GPU parity of the Navier-Stokes terms against the oracle: the BR1
+auxiliary stress / heat flux, the full stage residual with the viscous
+integrals, the chemistry, the polynomial-cp EOS (FP64 Newton temperature)
+and whole SSP-RK3 steps with all of them; FP64 device path to 1e-12
+relative per variable (1e-11 after several steps), FP32 bulk path against
+the FP64 device path with the tolerance written in the test."""
+import numpy as np
+import pytest
+
+import charm_3d_amd as CA
+import mesh_util as MU
+import oracle_lib as OL
+from test_gpu_parity import device, oracle_cfg
+
+pytestmark = pytest.mark.gpu
+
+R = CA.GAS_R
+
+
+def var_err(a, b):
+    """max over variables of max|a - b| / max|b| (per variable)"""
+    out = 0.0
+    for v in range(a.shape[1]):
+        scale = max(np.abs(b[:, v]).max(), 1e-300)
+        out = max(out, np.abs(a[:, v] - b[:, v]).max() / scale)
+    return out
+
+
+# --- configurations -------------------------------------------------------
+
+def cfg_const(prec=CA.FP64, bnd=("OUTLET",) * 6, params=None, **kw):
+    """one species, nondimensional (R / M = 1, cp 3.5), constant mu and k"""
+    tr = [dict(ml0=0.05, kp0=0.1, ml_type=0, kp_type=0)]
+    return CA.make_config(precision=prec, model="NS", flux="HLLC", bnd=bnd, bnd_param=params,
+                          comp_m=(R,), comp_cp=(3.5,), transport=tr, use_visc=True, **kw)
+
+
+def cfg_poly(prec=CA.FP64, bnd=("OUTLET",) * 6, params=None, **kw):
+    """one species with cp(T) = 3.3 + 0.2 T + 0.05 T^2 (Newton T), Sutherland
+    mu and k (the polynomial cp takes effect with Sutherland viscosity)"""
+    tr = [dict(ml0=0.05, kp0=0.1, t0=1.0, ts=0.4, ml_type=1, kp_type=1)]
+    return CA.make_config(precision=prec, model="NS", flux="HLLC", bnd=bnd, bnd_param=params,
+                          comp_m=(R,), comp_cp=((3.3, 0.2, 0.05),), transport=tr,
+                          use_visc=True, **kw)
+
+
+MS = (0.028, 0.032)
+CP2 = ((950.0, 0.2), (880.0, 0.25))
+
+
+def cfg_mix(prec=CA.FP64, bnd=("OUTLET",) * 6, params=None, reactions=True, parity=False, **kw):
+    """two species in SI units, polynomial cp, Sutherland transport,
+    multicomponent diffusion, one reaction A -> B"""
+    tr = [dict(ml0=1.7e-5, kp0=0.024, t0=273.0, ts=110.0, ml_type=1, kp_type=1, sig=3.7, ek=80.0,
+               h0=0.0),
+          dict(ml0=1.9e-5, kp0=0.025, t0=273.0, ts=120.0, ml_type=1, kp_type=1, sig=3.4, ek=110.0,
+               h0=-2.0e5)]
+    rx = [((0, -1, -1), (1, -1, -1), 3.0e3, 2000.0 * R, 0.0)] if reactions else []
+    return CA.make_config(precision=prec, model="NS", flux="HLLC", eos="MIX", bnd=bnd, bnd_param=params,
+                          comp_m=MS, comp_cp=CP2, transport=tr, use_visc=True, use_diff=True,
+                          t_ref=298.15, reactions=rx, chem_parity=parity, **kw)
+
+
+def state(leaves, rng, cfg, t0, p0, vel, amp=0.05):
+    """random cell states around (t0, p0, vel) with small random slopes;
+    e from (cp(T) - R / M) T = e as the reference's EOS"""
+    n, nc = len(leaves), cfg.n_comp
+    ms = np.array([cfg.comp[i].m for i in range(nc)])
+    t = t0 * (1 + amp * rng.uniform(-1, 1, n))
+    p = p0 * (1 + amp * rng.uniform(-1, 1, n))
+    y = rng.dirichlet(np.ones(nc) * 4, size=n) if nc > 1 else np.ones((n, 1))
+    m = 1.0 / (y / ms).sum(1)
+    cp = np.zeros(n)
+    for i in range(nc):
+        q = cfg.comp[i]
+        ci = np.full(n, q.cp[0]) if not (q.cp_type and q.ml_type == 1) else \
+            sum(q.cp[k] * t ** k for k in range(q.n_cp))
+        cp += y[:, i] * ci
+    e = (cp - R / m) * t
+    rho = p * m / (R * t)
+    u = np.zeros((n, 4 + nc, 4))
+    v = np.asarray(vel) * (1 + amp * rng.normal(size=(n, 3)))
+    u[:, 0:3, 0] = rho[:, None] * v
+    u[:, 3, 0] = rho * (e + 0.5 * (v ** 2).sum(1))
+    u[:, 4:, 0] = rho[:, None] * y
+    u[:, :, 1:] = 0.02 * amp * rng.normal(size=(n, 4 + nc, 3)) * np.abs(u[:, :, :1])
+    return u
+
+
+WALLS = ("INLET", "OUTLET", "WALL_NO_SLIP", "WALL_SLIP", "WALL_NO_SLIP", "OUTLET")
+
+
+def cases():
+    nd_params = [[0.3, 0.1, 0.0, 1.1, 1.0], None, [1.05], None, [0.95], None]
+    si_params = [[20.0, 5.0, 0.0, 320.0, 1.0e5], None, [310.0], None, [290.0], None]
+    return {
+        "const_outlet": (cfg_const, {}, (1.0, 1.0, (0.3, 0.2, 0.1))),
+        "const_walls": (cfg_const, dict(bnd=WALLS, params=nd_params), (1.0, 1.0, (0.3, 0.2, 0.1))),
+        "poly_walls": (cfg_poly, dict(bnd=WALLS, params=nd_params), (1.0, 1.0, (0.3, 0.2, 0.1))),
+        "mix_walls": (cfg_mix, dict(bnd=WALLS, params=si_params), (300.0, 1.0e5, (20.0, 10.0, 5.0))),
+        "mix_periodic": (cfg_mix, dict(bnd=("PERIODIC",) * 6), (300.0, 1.0e5, (20.0, 10.0, 5.0))),
+    }
+
+
+def setup(name, prec=CA.FP64, seed=0):
+    mk, kw, (t0, p0, vel) = cases()[name]
+    per = (True,) * 3 if name.endswith("periodic") else (False,) * 3
+    cfg = mk(prec=prec, **kw)
+    lv = MU.random_adapted(2, 3, seed=21 + seed, frac=0.3, periodic=per)
+    u0 = state(lv, np.random.default_rng(seed), cfg, t0, p0, vel)
+    return cfg, lv, per, u0
+
+
+@pytest.mark.parametrize("name", sorted(cases()))
+def test_aux_matches_oracle(name):
+    cfg, lv, per, u0 = setup(name)
+    got = device(cfg, lv, per, u0).ns_aux()
+    sim = OL.OracleSim(oracle_cfg(cfg), lv)
+    sim.set_state(u0)
+    want = sim.ns_aux()
+    for grp in (slice(0, 6), slice(6, 9)):
+        scale = np.abs(want[:, grp]).max()
+        assert np.abs(got[:, grp] - want[:, grp]).max() <= 1e-12 * scale, name
+
+
+@pytest.mark.parametrize("name", sorted(cases()))
+def test_ns_residual_matches_oracle(name):
+    cfg, lv, per, u0 = setup(name)
+    got = device(cfg, lv, per, u0).residual()
+    sim = OL.OracleSim(oracle_cfg(cfg), lv)
+    sim.set_state(u0)
+    want = sim.ns_residual()
+    assert var_err(got, want) <= 1e-12, var_err(got, want)
+
+
+@pytest.mark.parametrize("parity", [False, True])
+def test_chemistry_matches_oracle(parity):
+    cfg, lv, per, u0 = setup("mix_walls")
+    cfg = cfg_mix(bnd=WALLS, params=cases()["mix_walls"][1]["params"], parity=parity)
+    ctx = device(cfg, lv, per, u0)
+    rhs = ctx.ns_chem(2e-5)
+    sim = OL.OracleSim(oracle_cfg(cfg), lv)
+    sim.set_state(u0)
+    rhs_ref = sim.ns_chem_apply(2e-5)
+    assert np.abs(rhs - rhs_ref).max() <= 1e-12 * np.abs(rhs_ref).max()
+    assert var_err(ctx.get_state(), sim.get_state()) <= 1e-13
+
+
+def test_poly_eos_euler_residual_matches_oracle():
+    """polynomial cp without viscosity: the convective terms through the
+    Newton temperature"""
+    cfg, lv, per, u0 = setup("poly_walls")
+    cfg.use_visc = 0
+    got = device(cfg, lv, per, u0).residual()
+    sim = OL.OracleSim(oracle_cfg(cfg), lv)
+    sim.set_state(u0)
+    want = sim.residual()
+    assert var_err(got, want) <= 1e-12, var_err(got, want)
+
+
+@pytest.mark.parametrize("name,lim", [("const_walls", False), ("poly_walls", True), ("mix_walls", True)])
+def test_ns_steps_match_oracle(name, lim):
+    cfg, lv, per, u0 = setup(name, seed=3)
+    cfg.limiter = int(lim)
+    dt = 1e-3 if not name.startswith("mix") else 2e-5
+    ctx = device(cfg, lv, per, u0)
+    ctx.run(dt, 4)
+    sim = OL.OracleSim(oracle_cfg(cfg), lv)
+    sim.set_state(u0)
+    sim.run(dt, 4)
+    err = var_err(ctx.get_state(), sim.get_state())
+    assert err <= 1e-11, err
+
+
+def test_fp32_ns_close_to_fp64():
+    cfg, lv, per, u0 = setup("mix_walls", seed=4)
+    dt = 2e-5
+    out = {}
+    for prec in (CA.FP64, CA.FP32):
+        c = cfg_mix(prec=prec, bnd=WALLS, params=cases()["mix_walls"][1]["params"])
+        c.limiter = 1
+        ctx = device(c, lv, per, u0)
+        ctx.run(dt, 4)
+        out[prec] = ctx.get_state()
+    err = np.sqrt(((out[CA.FP32] - out[CA.FP64]) ** 2).sum() / (out[CA.FP64] ** 2).sum())
+    assert err < 2e-6, err

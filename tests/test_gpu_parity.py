@@ -18,16 +18,24 @@ REL64 = 1e-12
 
 
 def oracle_cfg(cfg):
+    """The oracle's config for a device config (every field this path reads)."""
     kinds = {v: k for k, v in CA.BND.items()}
     bnd = tuple(kinds[cfg.bnd_type[i]] for i in range(6))
     params = [[cfg.bnd_param[i][j] for j in range(6)] for i in range(6)]
-    return OL.make_config(n_comp=cfg.n_comp, comp_m=[cfg.comp[i].m for i in range(cfg.n_comp)],
-                          comp_cp=[(cfg.comp[i].cp[0],) for i in range(cfg.n_comp)],
+    comps = [cfg.comp[i] for i in range(cfg.n_comp)]
+    tr = [dict(ml0=q.ml0, kp0=q.kp0, t0=q.t0, ts=q.ts, h0=q.h0, sig=q.sig, ek=q.ek,
+               ml_type=q.ml_type, kp_type=q.kp_type, cp_type=q.cp_type) for q in comps]
+    rx = [(tuple(cfg.react[j].left), tuple(cfg.react[j].right), cfg.react[j].a, cfg.react[j].e,
+           cfg.react[j].n) for j in range(cfg.n_react)]
+    return OL.make_config(n_comp=cfg.n_comp, comp_m=[q.m for q in comps],
+                          comp_cp=[tuple(q.cp[k] for k in range(q.n_cp)) for q in comps],
                           eos="IDEAL" if cfg.eos == 0 else "MIX",
                           flux={v: k for k, v in CA.FLUX.items()}[cfg.flux],
                           limiter=bool(cfg.limiter), model="NS" if cfg.model == 1 else "EULER",
                           bnd=bnd, bnd_param=params, grav=[cfg.grav[i] for i in range(3)],
-                          cfl=cfg.cfl)
+                          cfl=cfg.cfl, transport=tr, use_visc=bool(cfg.use_visc),
+                          use_diff=bool(cfg.use_diff), t_ref=cfg.t_ref, reactions=rx,
+                          chem_parity=bool(cfg.chem_parity))
 
 
 def device(cfg, leaves, periodic, u0):
