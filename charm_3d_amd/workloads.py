@@ -166,6 +166,68 @@ def c3(precision=FP32, seed=3, min_level=5, max_level=8, max_err=0.04, width=Non
     return cfg, forest, project_p1(forest.leaves(), fn, 6)
 
 
+# reacting hot-spot case: H2, O2, H2O, N2 with the NASA 7-coefficient
+# polynomials (200-1000 K fits, GRI-Mech 3.0) as the cp power series
+# cp_i(T) = (R / M_i) (a1 + a2 T + a3 T^2 + a4 T^3 + a5 T^4), Sutherland
+# viscosity and conductivity with KP0 = ML0 cp(T0) / 0.72 (Prandtl 0.72),
+# Lennard-Jones diffusion parameters, and one reaction H2 + O2 -> H2O with
+# k = 1e9 exp(-15000 / T)
+C4_SPECIES = ("H2", "O2", "H2O", "N2")
+C4_M = (0.002016, 0.031999, 0.018015, 0.028014)
+C4_NASA = ((2.34433112, 7.98052075e-03, -1.94781510e-05, 2.01572094e-08, -7.37611761e-12),
+           (3.78245636, -2.99673416e-03, 9.84730201e-06, -9.68129509e-09, 3.24372837e-12),
+           (4.19864056, -2.03643410e-03, 6.52040211e-06, -5.48797062e-09, 1.77197817e-12),
+           (3.29867700, 1.40824040e-03, -3.96322200e-06, 5.64151500e-09, -2.44485400e-12))
+C4_H0 = (0.0, 0.0, -241826.0 / 0.018015, 0.0)            # formation enthalpy at 298.15 K, J/kg
+C4_ML = ((8.41e-6, 72.0), (1.919e-5, 127.0), (1.12e-5, 1064.0), (1.663e-5, 111.0))   # mu(273 K), S
+C4_LJ = ((2.92, 38.0), (3.458, 107.4), (2.605, 572.4), (3.621, 97.53))               # sigma A, eps/k K
+C4_Y = (0.02, 0.22, 0.0, 0.76)
+
+
+def c4_config(precision=FP32, limiter=False):
+    cps = tuple(tuple(a * GAS_R / m for a in nasa) for nasa, m in zip(C4_NASA, C4_M))
+    t0 = 273.0
+    tr = []
+    for i in range(4):
+        cp0 = sum(c * t0 ** k for k, c in enumerate(cps[i]))
+        ml0, ts = C4_ML[i]
+        tr.append(dict(ml0=ml0, kp0=ml0 * cp0 / 0.72, t0=t0, ts=ts, ml_type=1, kp_type=1,
+                       sig=C4_LJ[i][0], ek=C4_LJ[i][1], h0=C4_H0[i]))
+    rx = [((0, 1, -1), (2, -1, -1), 1.0e9, 15000.0 * GAS_R, 0.0)]
+    return make_config(precision=precision, model="NS", flux="HLLC", limiter=limiter, eos="MIX",
+                       comp_m=C4_M, comp_cp=cps, bnd=("OUTLET",) * 6, cfl=0.3, transport=tr,
+                       use_visc=True, use_diff=True, t_ref=298.15, reactions=rx)
+
+
+def c4_state_fn(cfg, r0=0.1, p=101325.0):
+    """T = 300 + 1200 exp(-r^2 / r0^2) about the centre, p = 1 atm, at rest,
+    premixed mass fractions; e = (cp(T) - R / M) T as the mixture EOS"""
+    y = np.asarray(C4_Y)
+    ms = np.asarray(C4_M)
+    m = 1.0 / (y / ms).sum()
+    cps = [[cfg.comp[i].cp[k] for k in range(cfg.comp[i].n_cp)] for i in range(4)]
+
+    def fn(x):
+        r2 = ((x - 0.5) ** 2).sum(1)
+        t = 300.0 + 1200.0 * np.exp(-r2 / r0 ** 2)
+        cp = sum(y[i] * sum(c * t ** k for k, c in enumerate(cps[i])) for i in range(4))
+        e = (cp - GAS_R / m) * t
+        rho = p * m / (GAS_R * t)
+        out = np.zeros((len(x), 8))
+        out[:, 3] = rho * e
+        out[:, 4:] = rho[:, None] * y[None, :]
+        return out
+    return fn
+
+
+def c4(level=6, precision=FP32):
+    """Reacting hot spot: level-6 uniform cube (262,144 leaves), four
+    species, HLLC + BR1 viscous terms + diffusion + chemistry, SSP-RK3."""
+    leaves = Forest.uniform(level).leaves()
+    cfg = c4_config(precision)
+    return cfg, leaves, project_p1(leaves, c4_state_fn(cfg), 8), (False, False, False)
+
+
 def dof_updates(n_elem, nv, stages, steps):
     """one DOF advanced through one RK stage: E * 4 * V * stages * steps"""
     return n_elem * 4 * nv * stages * steps

@@ -184,3 +184,18 @@ def test_fp32_ns_close_to_fp64():
         out[prec] = ctx.get_state()
     err = np.sqrt(((out[CA.FP32] - out[CA.FP64]) ** 2).sum() / (out[CA.FP64] ** 2).sum())
     assert err < 2e-6, err
+
+
+def test_reacting_hot_spot_small_matches_oracle():
+    """The four-species reacting hot spot (NASA polynomials, Sutherland
+    transport, diffusion, H2 + O2 -> H2O) at level 3, two steps."""
+    import charm_3d_amd.workloads as W
+    cfg, lv, u0, per = W.c4(3, CA.FP64)
+    sim = OL.OracleSim(oracle_cfg(cfg), lv)
+    sim.set_state(u0)
+    dt = sim.cfl_dt(1)
+    sim.run(dt, 2)
+    ctx = device(cfg, lv, per, u0)
+    ctx.run(dt, 2)
+    err = var_err(ctx.get_state(), sim.get_state())
+    assert err <= 1e-11, err
