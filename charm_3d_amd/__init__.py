@@ -105,6 +105,9 @@ def lib():
         "charm_gpu_flux_batch": (C.c_int, [vp, C.c_int, i64, dp, dp, dp, dp]),
         "charm_gpu_amr_indicator": (C.c_int, [vp, dp, dp, i64]),
         "charm_gpu_ns_aux": (C.c_int, [vp, dp, i64]),
+        "charm_gpu_objective": (C.c_int, [vp, C.c_double, C.c_int, dp, dp, dp]),
+        "charm_gpu_adjoint": (C.c_int, [vp, C.c_double, C.c_int, C.c_int, dp, dp, C.c_int, dp, dp, dp,
+                                        dp, dp]),
         "charm_gpu_ns_chem": (C.c_int, [vp, C.c_double, dp, i64]),
         "charm_gpu_remesh": (C.c_int, [vp, vp, C.POINTER(C.c_int64), i64]),
         "charm_amr_flags": (C.c_int, [i64, i32p, dp, dp, C.c_double, C.c_int, C.c_int,
@@ -399,6 +402,23 @@ class GpuContext:
         out = np.zeros(self.n)
         check(lib().charm_gpu_ns_chem(self.h, dt, _dptr(out), self.n))
         return out
+
+    def objective(self, dt, nsteps, lo, hi):
+        """J = sum over the steps of dt * int_box p (forward pass)."""
+        lo_, hi_ = np.asarray(lo, dtype=np.float64), np.asarray(hi, dtype=np.float64)
+        j = C.c_double()
+        check(lib().charm_gpu_objective(self.h, dt, nsteps, _dptr(lo_), _dptr(hi_), C.byref(j)))
+        return j.value
+
+    def adjoint(self, dt, nsteps, lo, hi, ckpt=20, theta_face=-1):
+        """(J, dJ/du0 [n][V][4], dJ/d(inlet u, T, p), forward ms, adjoint ms)"""
+        lo_, hi_ = np.asarray(lo, dtype=np.float64), np.asarray(hi, dtype=np.float64)
+        j, fm, am = C.c_double(), C.c_double(), C.c_double()
+        g = np.zeros((self.n, self.nv, 4))
+        th = np.zeros(3)
+        check(lib().charm_gpu_adjoint(self.h, dt, nsteps, ckpt, _dptr(lo_), _dptr(hi_), theta_face,
+                                      C.byref(j), _dptr(g), _dptr(th), C.byref(fm), C.byref(am)))
+        return j.value, g, th, fm.value, am.value
 
     def remesh(self, mesh, amap):
         """Upload the adapted mesh and project the state with the old->new map."""

@@ -1,0 +1,359 @@
+// charm_adj.hip -- gfx950 kernels of the discrete adjoint of the SSP-RK3
+// time step (Euler terms, constant cp, no limiter): the transposed
+// linearisation of one stage S(U) = U - dt A^-1 int(U), the gradient of the
+// objective Q(U) = sum over the leaves of a box of int p, and Q itself.
+//
+// A stage adjoint is element-centric: one thread per leaf forms
+//   T = sb + (d int / d U)^T W,  W = -dt A^-1 sb,  sb = c_in X,
+// from the volume term and every point of its six face slots.  At each point
+// the pointwise numerics (cons_to_prim, boundary state, HLLC / LF) run on
+// forward-mode dual numbers whose tangents are this leaf's point values;
+// the contraction s = sum_v q_v G_v with the adjoint weights G of both sides
+// then carries d s / d u(point) in its tangents.  Every interior point is
+// evaluated from both sides, each side keeping its own derivative (no
+// atomics).  The inlet parameters (velocity x, T, p of one boundary face)
+// get their own dual pass.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "charm_device.hpp"
+#include "charm_dual.hpp"
+#include "charm_elemfaces.hpp"
+#include "charm_kernels.h"
+
+namespace charm {
+
+namespace {
+
+// adjoint weights of leaf e: W[4v + k] = -dt ainv_k(level) c_in X[4v + k]
+template <typename R, int NV>
+__device__ __forceinline__ void adj_weights(const AdjArgs &A, const DevParams &P, const R *__restrict__ X,
+                                            int e, R *w)
+{
+    const int lev = A.level[e];
+    const R sc = ldexp(R(1), 3 * lev);
+    #pragma unroll
+    for (int v = 0; v < NV; v++)
+        #pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const R ai = cst<R>(P.ainv0[k], P.ainv0_f[k]) * sc;
+            w[4 * v + k] = R(-A.dt) * ai * (R(A.c_in) * (X + (4 * v + k) * A.n)[e]);
+        }
+}
+
+template <typename R, int NV>
+__device__ __forceinline__ R wdot(const R *w, int v, const R ph[3])
+{
+    return w[4 * v] + ph[0] * w[4 * v + 1] + ph[1] * w[4 * v + 2] + ph[2] * w[4 * v + 3];
+}
+
+// flux in the frame of n = sg e_a (global-order momentum components out)
+template <typename D, int NC, int FLUX>
+__device__ __forceinline__ void flux_global(const Prim<D, NC> &pl, const Prim<D, NC> &pr, int a, D sg, D *qg)
+{
+    constexpr int NV = 4 + NC;
+    const int t1 = (a == 0) ? 1 : 0, t2 = (a == 2) ? 1 : 2;
+    Frame<D, NC> fl, fr;
+    const D vl[3] = {pl.u, pl.v, pl.w}, vr[3] = {pr.u, pr.v, pr.w};
+    fl.r = pl.r; fl.e_tot = pl.e_tot; fl.p = pl.p; fl.cz = pl.cz; fl.gam = pl.gam;
+    fr.r = pr.r; fr.e_tot = pr.e_tot; fr.p = pr.p; fr.cz = pr.cz; fr.gam = pr.gam;
+    fl.un = sg * vl[a]; fr.un = sg * vr[a];
+    fl.ut1 = vl[t1]; fr.ut1 = vr[t1];
+    fl.ut2 = vl[t2]; fr.ut2 = vr[t2];
+    #pragma unroll
+    for (int i = 0; i < NC; i++) { fl.c[i] = pl.c[i]; fr.c[i] = pr.c[i]; }
+    D q[NV];
+    flux_x<D, NC, FLUX>(fl, fr, q);
+    qg[a] = sg * q[0]; qg[t1] = q[1]; qg[t2] = q[2];
+    #pragma unroll
+    for (int v = 3; v < NV; v++) qg[v] = q[v];
+}
+
+// inlet exterior state with the parameters as duals (u = theta0, T = theta1,
+// p = theta2; v, w fixed), as bnd_state for BOUND_INLET (src/charm_bnd_cond.c:
+// 36-47) followed by EOS flag 3
+template <typename R, int NC>
+__device__ __forceinline__ void inlet_dual(const DevParams &P, int face, const Prim<Dual<R, 3>, NC> &in,
+                                           Prim<Dual<R, 3>, NC> &out)
+{
+    using D = Dual<R, 3>;
+    out = in;
+    out.u = D::var(cst<R>(P.bnd_param[face][0], P.bnd_param_f[face][0]), 0);
+    out.v = D(cst<R>(P.bnd_param[face][1], P.bnd_param_f[face][1]));
+    out.w = D(cst<R>(P.bnd_param[face][2], P.bnd_param_f[face][2]));
+    const D t = D::var(cst<R>(P.bnd_param[face][3], P.bnd_param_f[face][3]), 1);
+    D pr = D::var(cst<R>(P.bnd_param[face][4], P.bnd_param_f[face][4]), 2);
+    eos_coeffs<D, NC>(P, out);
+    if (pr < D(kEps)) pr = D(kEps);
+    out.p = pr;
+    out.r = dv(pr * out.m, t * D(kGasR));
+    out.cz = dsqrt(dv(out.gam * pr, out.r));
+    out.e = dv(pr, out.r * (out.gam - D(1)));
+    out.e_tot = out.e + D(0.5) * (out.u * out.u + out.v * out.v + out.w * out.w);
+}
+
+} // namespace
+
+template <typename R, int NC, int FLUX>
+__global__ __launch_bounds__(128)
+void k_adj_stage(AdjArgs A, DevParams P)
+{
+    constexpr int NV = 4 + NC;
+    using D = Dual<R, NV>;
+    const int64_t n = A.n;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const R *__restrict__ U = reinterpret_cast<const R *>(A.U);
+    const R *__restrict__ X = reinterpret_cast<const R *>(A.X);
+    const int lev = A.level[e];
+    const R h = ldexp(R(1), -lev), hinv = ldexp(R(1), lev);
+    R u[4 * NV], w[4 * NV], ub[4 * NV];
+    load_coeffs<R, NV>(U, n, e, u);
+    adj_weights<R, NV>(A, P, X, e, w);
+    #pragma unroll
+    for (int i = 0; i < 4 * NV; i++) ub[i] = R(0);
+
+    // volume: int[v][1+d] -= (wJ / h) sum_gp F_d,v ; gravity int[v<3][k] -= g_v rho phi_k wJ
+    const R gph = R(0.28867513459481288225);
+    const R wj = h * h * h * R(0.125);
+    for (int g = 0; g < 8; g++) {
+        const R ph[3] = {(g & 1) ? gph : -gph, (g & 2) ? gph : -gph, (g & 4) ? gph : -gph};
+        D t[NV];
+        #pragma unroll
+        for (int v = 0; v < NV; v++) t[v] = D::var(at<R>(&u[4 * v], ph[0], ph[1], ph[2]), v);
+        Prim<D, NC> p;
+        cons_to_prim<D, NC>(P, t, p);
+        const D vel[3] = {p.u, p.v, p.w};
+        D s(0.0);
+        #pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const D md = t[d];
+            const R c0 = -wj * hinv * w[4 * 0 + 1 + d], c1 = -wj * hinv * w[4 * 1 + 1 + d];
+            const R c2 = -wj * hinv * w[4 * 2 + 1 + d], c3 = -wj * hinv * w[4 * 3 + 1 + d];
+            s += D(c0) * (md * p.u + (d == 0 ? p.p : D(0.0)));
+            s += D(c1) * (md * p.v + (d == 1 ? p.p : D(0.0)));
+            s += D(c2) * (md * p.w + (d == 2 ? p.p : D(0.0)));
+            s += D(c3) * (md * p.e_tot + p.p * vel[d]);
+            #pragma unroll
+            for (int i = 0; i < NC; i++) s += D(-wj * hinv * w[4 * (4 + i) + 1 + d]) * (md * p.c[i]);
+        }
+        #pragma unroll
+        for (int v = 0; v < 3; v++) {
+            const R gv = cst<R>(P.grav[v], P.grav_f[v]);
+            s += D(-gv * wj * wdot<R, NV>(w, v, ph)) * p.r;
+        }
+        const R phi[4] = {R(1), ph[0], ph[1], ph[2]};
+        #pragma unroll
+        for (int v = 0; v < NV; v++)
+            #pragma unroll
+            for (int k = 0; k < 4; k++) ub[4 * v + k] += phi[k] * s.d[v];
+    }
+
+    double th[3] = {0.0, 0.0, 0.0};
+    face_points<R>(A.nbr, A.aux, A.level, e, h,
+        [&](int d, int nb, const R ps[3], const R pn[3], R wjf, R sign) {
+            R un[4 * NV], wn[4 * NV];
+            load_coeffs<R, NV>(U, n, nb, un);
+            adj_weights<R, NV>(A, P, X, nb, wn);
+            D ts[NV], tn[NV];
+            #pragma unroll
+            for (int v = 0; v < NV; v++) {
+                ts[v] = D::var(at<R>(&u[4 * v], ps[0], ps[1], ps[2]), v);
+                tn[v] = D(at<R>(&un[4 * v], pn[0], pn[1], pn[2]));
+            }
+            Prim<D, NC> psf, pnb;
+            cons_to_prim<D, NC>(P, ts, psf);
+            cons_to_prim<D, NC>(P, tn, pnb);
+            D qg[NV];
+            // q = flux(side 0, side 1) with n = +e_a; e is side 0 when sign > 0
+            if (sign > R(0)) flux_global<D, NC, FLUX>(psf, pnb, d >> 1, D(1.0), qg);
+            else flux_global<D, NC, FLUX>(pnb, psf, d >> 1, D(1.0), qg);
+            D s(0.0);
+            #pragma unroll
+            for (int v = 0; v < NV; v++)
+                s += qg[v] * D(sign * wjf * (wdot<R, NV>(w, v, ps) - wdot<R, NV>(wn, v, pn)));
+            const R phi[4] = {R(1), ps[0], ps[1], ps[2]};
+            #pragma unroll
+            for (int v = 0; v < NV; v++)
+                #pragma unroll
+                for (int k = 0; k < 4; k++) ub[4 * v + k] += phi[k] * s.d[v];
+        },
+        [&](int d, int face, int, const R ps[3], R wjf) {
+            const int a = d >> 1;
+            const R sg = (d & 1) ? R(1) : R(-1);
+            {
+                D ts[NV];
+                #pragma unroll
+                for (int v = 0; v < NV; v++) ts[v] = D::var(at<R>(&u[4 * v], ps[0], ps[1], ps[2]), v);
+                Prim<D, NC> pi, po;
+                cons_to_prim<D, NC>(P, ts, pi);
+                bnd_state<D, NC>(P, face, a, D(sg), pi, po, -1);
+                D qg[NV];
+                flux_global<D, NC, FLUX>(pi, po, a, D(sg), qg);
+                D s(0.0);
+                #pragma unroll
+                for (int v = 0; v < NV; v++) s += qg[v] * D(wjf * wdot<R, NV>(w, v, ps));
+                const R phi[4] = {R(1), ps[0], ps[1], ps[2]};
+                #pragma unroll
+                for (int v = 0; v < NV; v++)
+                    #pragma unroll
+                    for (int k = 0; k < 4; k++) ub[4 * v + k] += phi[k] * s.d[v];
+            }
+            if (face == A.theta_face && A.theta_bar && P.bnd_type[face] == BND_INLET) {
+                using T3 = Dual<R, 3>;
+                T3 ts[NV];
+                #pragma unroll
+                for (int v = 0; v < NV; v++) ts[v] = T3(at<R>(&u[4 * v], ps[0], ps[1], ps[2]));
+                Prim<T3, NC> pi, po;
+                cons_to_prim<T3, NC>(P, ts, pi);
+                inlet_dual<R, NC>(P, face, pi, po);
+                T3 qg[NV];
+                flux_global<T3, NC, FLUX>(pi, po, a, T3(sg), qg);
+                T3 s(0.0);
+                #pragma unroll
+                for (int v = 0; v < NV; v++) s += qg[v] * T3(wjf * wdot<R, NV>(w, v, ps));
+                th[0] += (double) s.d[0]; th[1] += (double) s.d[1]; th[2] += (double) s.d[2];
+            }
+        });
+
+    R *__restrict__ out = reinterpret_cast<R *>(A.OUT);
+    const R *__restrict__ acc = reinterpret_cast<const R *>(A.ACC);
+    #pragma unroll
+    for (int i = 0; i < 4 * NV; i++) {
+        const R sb = R(A.c_in) * (X + i * n)[e];
+        R o = R(A.c_out) * (sb + ub[i]);
+        if (acc) o += R(A.c_acc) * (acc + i * n)[e];
+        (out + i * n)[e] = o;
+    }
+    if (A.theta_bar) {
+        A.theta_bar[3 * e + 0] += th[0];
+        A.theta_bar[3 * e + 1] += th[1];
+        A.theta_bar[3 * e + 2] += th[2];
+    }
+}
+
+// objective Q(U) = sum over leaves with centre in the box of sum_gp wJ p:
+// per-block FP64 partial sums (mode 0), or lambda += coef dQ/dU (mode 1)
+template <typename R, int NC>
+__global__ __launch_bounds__(256)
+void k_adj_objective(ObjArgs A, DevParams P)
+{
+    constexpr int NV = 4 + NC;
+    using D = Dual<R, NV>;
+    __shared__ double red[256];
+    const int64_t n = A.n;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    double val = 0.0;
+    if (e < n) {
+        const int4 q = A.q[e];
+        const int len = (1 << 19) >> q.w;
+        const double inv = 1.0 / (double) (1 << 19);
+        const double c[3] = {(q.x + 0.5 * len) * inv, (q.y + 0.5 * len) * inv, (q.z + 0.5 * len) * inv};
+        bool in = true;
+        #pragma unroll
+        for (int d = 0; d < 3; d++) in = in && c[d] >= A.lo[d] && c[d] <= A.hi[d];
+        if (in) {
+            const R *__restrict__ U = reinterpret_cast<const R *>(A.U);
+            R u[4 * NV], g[4 * NV];
+            load_coeffs<R, NV>(U, n, e, u);
+            #pragma unroll
+            for (int i = 0; i < 4 * NV; i++) g[i] = R(0);
+            const R h = ldexp(R(1), -q.w), wj = h * h * h * R(0.125);
+            const R gph = R(0.28867513459481288225);
+            for (int gp = 0; gp < 8; gp++) {
+                const R ph[3] = {(gp & 1) ? gph : -gph, (gp & 2) ? gph : -gph, (gp & 4) ? gph : -gph};
+                D t[NV];
+                #pragma unroll
+                for (int v = 0; v < NV; v++) t[v] = D::var(at<R>(&u[4 * v], ph[0], ph[1], ph[2]), v);
+                Prim<D, NC> p;
+                cons_to_prim<D, NC>(P, t, p);
+                val += (double) (wj * p.p.v);
+                const R phi[4] = {R(1), ph[0], ph[1], ph[2]};
+                #pragma unroll
+                for (int v = 0; v < NV; v++)
+                    #pragma unroll
+                    for (int k = 0; k < 4; k++) g[4 * v + k] += wj * phi[k] * p.p.d[v];
+            }
+            if (A.mode == 1) {
+                R *__restrict__ L = reinterpret_cast<R *>(A.lambda);
+                #pragma unroll
+                for (int i = 0; i < 4 * NV; i++) (L + i * n)[e] += R(A.coef) * g[i];
+            }
+        }
+    }
+    if (A.mode == 0) {
+        red[threadIdx.x] = val;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if ((int) threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) A.partial[blockIdx.x] = red[0];
+    }
+}
+
+// y = a x + b y on n * nvk values
+template <typename R>
+__global__ __launch_bounds__(256)
+void k_adj_axpby(const R *__restrict__ x, R *__restrict__ y, R a, R b, int64_t m)
+{
+    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) y[i] = a * x[i] + b * y[i];
+}
+
+template <typename R, int NC>
+static hipError_t launch_adj_nc(int flux, const AdjArgs &A, const DevParams &P, hipStream_t st)
+{
+    const dim3 grid((unsigned) ((A.n + 127) / 128)), block(128);
+    switch (flux) {
+    case FLUX_LF: hipLaunchKernelGGL((k_adj_stage<R, NC, FLUX_LF>), grid, block, 0, st, A, P); break;
+    case FLUX_HLLC: hipLaunchKernelGGL((k_adj_stage<R, NC, FLUX_HLLC>), grid, block, 0, st, A, P); break;
+    case FLUX_CD: hipLaunchKernelGGL((k_adj_stage<R, NC, FLUX_CD>), grid, block, 0, st, A, P); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <typename R>
+static hipError_t launch_adj_r(int nc, int flux, const AdjArgs &A, const DevParams &P, hipStream_t st)
+{
+    switch (nc) {
+    case 1: return launch_adj_nc<R, 1>(flux, A, P, st);
+    case 2: return launch_adj_nc<R, 2>(flux, A, P, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_adj_stage(int prec, int nc, int flux, const AdjArgs &A, const DevParams &P, hipStream_t st)
+{
+    if (A.n <= 0) return hipSuccess;
+    return prec ? launch_adj_r<float>(nc, flux, A, P, st) : launch_adj_r<double>(nc, flux, A, P, st);
+}
+
+template <typename R>
+static hipError_t launch_obj_r(int nc, const ObjArgs &A, const DevParams &P, hipStream_t st)
+{
+    const dim3 grid((unsigned) ((A.n + 255) / 256)), block(256);
+    switch (nc) {
+    case 1: hipLaunchKernelGGL((k_adj_objective<R, 1>), grid, block, 0, st, A, P); break;
+    case 2: hipLaunchKernelGGL((k_adj_objective<R, 2>), grid, block, 0, st, A, P); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_objective(int prec, int nc, const ObjArgs &A, const DevParams &P, hipStream_t st)
+{
+    if (A.n <= 0) return hipSuccess;
+    return prec ? launch_obj_r<float>(nc, A, P, st) : launch_obj_r<double>(nc, A, P, st);
+}
+
+hipError_t launch_adj_axpby(int prec, const void *x, void *y, double a, double b, int64_t m, hipStream_t st)
+{
+    if (m <= 0) return hipSuccess;
+    const dim3 grid((unsigned) ((m + 255) / 256)), block(256);
+    if (prec) hipLaunchKernelGGL(k_adj_axpby<float>, grid, block, 0, st, (const float *) x, (float *) y, (float) a, (float) b, m);
+    else hipLaunchKernelGGL(k_adj_axpby<double>, grid, block, 0, st, (const double *) x, (double *) y, a, b, m);
+    return hipGetLastError();
+}
+
+} // namespace charm

@@ -50,7 +50,7 @@ struct DevParams {
     double r_a[4], r_e[4], r_n[4];
 };
 
-template <typename R> __device__ __forceinline__ R cst(double d, float f);
+template <typename R> __device__ __forceinline__ R cst(double d, float) { return R(d); }
 template <> __device__ __forceinline__ double cst<double>(double d, float) { return d; }
 template <> __device__ __forceinline__ float cst<float>(double, float f) { return f; }
 
@@ -482,13 +482,15 @@ __device__ __forceinline__ void godunov_x(const Frame<R, NC> &Lf, const Frame<R,
 template <typename R, int NC, int FLUX>
 __device__ __forceinline__ bool flux_x(const Frame<R, NC> &L, const Frame<R, NC> &Rt, R *q)
 {
-    if (FLUX == FLUX_HLLC) return hllc_x<R, NC>(L, Rt, q);
-    if (FLUX == FLUX_GODUNOV) {
+    if constexpr (FLUX == FLUX_HLLC) {
+        return hllc_x<R, NC>(L, Rt, q);
+    } else if constexpr (FLUX == FLUX_GODUNOV) {
         godunov_x<R, NC>(L, Rt, q, sizeof(R) == 8 ? 5000 : 64);
         return true;
+    } else {
+        lf_x<R, NC>(L, Rt, q, FLUX == FLUX_CD);
+        return true;
     }
-    lf_x<R, NC>(L, Rt, q, FLUX == FLUX_CD);
-    return true;
 }
 
 } // namespace charm

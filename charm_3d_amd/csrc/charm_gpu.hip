@@ -776,3 +776,186 @@ extern "C" int charm_gpu_ns_chem(charm_gpu_ctx_t *c, double dt, double *chem_rhs
     HIPCHK(hipStreamSynchronize(c->st));
     return CHARM_OK;
 }
+
+// ------------------------------------------------------------ adjoint
+//
+// Discrete adjoint of nsteps SSP-RK3 steps (Euler terms, no limiter) for
+// J = sum_{n=1..N} dt Q(u^n), Q(u) = sum over the leaves whose centre lies in
+// the box [lo, hi] of int p.  The forward pass stores u^n every `ckpt` steps;
+// the reverse pass recomputes each segment's stage states and runs the
+// transposed stages:
+//   l += dt dQ/du(u^{n+1});  ub2 = S^T_{u2}(2/3 l);  a = l/3 + 3/4 ub2;
+//   ub1 = S^T_{u1}(1/4 ub2);  l = S^T_{u0}(ub1) + a,
+// S^T_U(y) = y + (d int / d U)^T (-dt A^-1 y).  The inlet parameters
+// (velocity x, T, p of face theta_face) accumulate their gradient per leaf.
+
+static int adj_check(charm_gpu_ctx_t *c, int nsteps, int ckpt, int theta_face)
+{
+    int rc = need_mesh(c, -1);
+    if (rc) return rc;
+    if (c->cfg.model != CHARM_MODEL_NS || c->cfg.limiter || c->ph || c->nc > 2 ||
+        c->cfg.flux == CHARM_FLUX_GODUNOV) {
+        charm_set_error("adjoint: needs the SSP-RK3 model without limiter, constant cp, 1-2 species and "
+                        "the LF, HLLC or CD flux");
+        return CHARM_ERR_UNSUPPORTED;
+    }
+    if (nsteps < 1 || ckpt < 1 || theta_face < -1 || theta_face > 5 ||
+        (theta_face >= 0 && c->cfg.bnd_type[theta_face] != CHARM_BND_INLET)) {
+        charm_set_error("adjoint: bad step count, checkpoint period or inlet face");
+        return CHARM_ERR_ARG;
+    }
+    return CHARM_OK;
+}
+
+static ObjArgs obj_args(charm_gpu_ctx_t *c, const void *U, const double lo[3], const double hi[3])
+{
+    ObjArgs O;
+    O.U = U; O.lambda = NULL; O.q = c->d_q; O.coef = 0.0; O.partial = NULL; O.mode = 0; O.n = c->n;
+    for (int d = 0; d < 3; d++) { O.lo[d] = lo[d]; O.hi[d] = hi[d]; }
+    return O;
+}
+
+// forward pass: nsteps steps from B[0]; per-step objective partials into
+// part[nsteps][nblk]; optional checkpoints every ckpt steps
+static int adj_forward(charm_gpu_ctx_t *c, double dt, int nsteps, int ckpt, const double lo[3],
+                       const double hi[3], char *cp, double *part, int64_t nblk, double *J)
+{
+    const size_t ssz = (size_t) c->n * 4 * c->nv * c->esz;
+    for (int s = 0; s < nsteps; s++) {
+        if (cp && s % ckpt == 0)
+            HIPCHK(hipMemcpyAsync(cp + (size_t) (s / ckpt) * ssz, c->B[0], ssz, hipMemcpyDeviceToDevice, c->st));
+        int rc = step_once(c, dt);
+        if (rc) return rc;
+        ObjArgs O = obj_args(c, c->B[0], lo, hi);
+        O.partial = part + (size_t) s * nblk;
+        HIPCHK(launch_adj_objective(c->prec, c->nc, O, c->P, c->st));
+    }
+    double *h = (double *) malloc((size_t) nsteps * nblk * sizeof(double));
+    if (!h) { charm_set_error("adjoint: out of memory"); return CHARM_ERR_MEM; }
+    hipError_t e = hipMemcpyAsync(h, part, (size_t) nsteps * nblk * sizeof(double), hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    if (e != hipSuccess) { free(h); charm_set_error("adjoint: %s", hipGetErrorString(e)); return CHARM_ERR_HIP; }
+    double j = 0.0;
+    for (int s = 0; s < nsteps; s++) {
+        double q = 0.0;
+        for (int64_t b = 0; b < nblk; b++) q += h[(size_t) s * nblk + b];
+        j += dt * q;
+    }
+    free(h);
+    *J = j;
+    return check_flag(c);
+}
+
+extern "C" int charm_gpu_objective(charm_gpu_ctx_t *c, double dt, int nsteps, const double lo[3],
+                                   const double hi[3], double *J)
+{
+    int rc = adj_check(c, nsteps, 1, -1);
+    if (rc) return rc;
+    if (!lo || !hi || !J) { charm_set_error("objective: null argument"); return CHARM_ERR_ARG; }
+    const int64_t nblk = (c->n + 255) / 256;
+    double *part = NULL;
+    HIPCHK(hipMalloc((void **) &part, (size_t) nsteps * nblk * sizeof(double)));
+    rc = adj_forward(c, dt, nsteps, 1, lo, hi, NULL, part, nblk, J);
+    (void) hipFree(part);
+    return rc;
+}
+
+extern "C" int charm_gpu_adjoint(charm_gpu_ctx_t *c, double dt, int nsteps, int ckpt, const double lo[3],
+                                 const double hi[3], int theta_face, double *J, double *grad_u0,
+                                 double *grad_theta, double *fwd_ms, double *adj_ms)
+{
+    int rc = adj_check(c, nsteps, ckpt, theta_face);
+    if (rc) return rc;
+    if (!lo || !hi || !J) { charm_set_error("adjoint: null argument"); return CHARM_ERR_ARG; }
+    const int64_t n = c->n, nblk = (n + 255) / 256;
+    const size_t ssz = (size_t) n * 4 * c->nv * c->esz;
+    const int64_t m = n * 4 * c->nv;
+    const int ncp = (nsteps + ckpt - 1) / ckpt;
+    char *cp = NULL, *seg = NULL, *la = NULL, *lb = NULL, *lc = NULL, *uend = NULL;
+    double *part = NULL, *theta = NULL;
+    hipError_t e = hipMalloc((void **) &cp, (size_t) ncp * ssz);
+    if (e == hipSuccess) e = hipMalloc((void **) &seg, (size_t) 3 * ckpt * ssz);
+    if (e == hipSuccess) e = hipMalloc((void **) &la, ssz);
+    if (e == hipSuccess) e = hipMalloc((void **) &lb, ssz);
+    if (e == hipSuccess) e = hipMalloc((void **) &lc, ssz);
+    if (e == hipSuccess) e = hipMalloc((void **) &uend, ssz);
+    if (e == hipSuccess) e = hipMalloc((void **) &part, (size_t) nsteps * nblk * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void **) &theta, (size_t) n * 3 * sizeof(double));
+    float ms_f = 0.f, ms_a = 0.f;
+    if (e != hipSuccess) {
+        charm_set_error("adjoint: device memory: %s", hipGetErrorString(e));
+        rc = CHARM_ERR_MEM;
+        goto done;
+    }
+#define ADJ_CHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { charm_set_error("adjoint: %s", hipGetErrorString(e_)); rc = CHARM_ERR_HIP; goto done; } } while (0)
+    ADJ_CHK(hipEventRecord(c->ev0, c->st));
+    if ((rc = adj_forward(c, dt, nsteps, ckpt, lo, hi, cp, part, nblk, J))) goto done;
+    ADJ_CHK(hipEventRecord(c->ev1, c->st));
+    ADJ_CHK(hipEventSynchronize(c->ev1));
+    ADJ_CHK(hipEventElapsedTime(&ms_f, c->ev0, c->ev1));
+
+    ADJ_CHK(hipEventRecord(c->ev0, c->st));
+    ADJ_CHK(hipMemsetAsync(la, 0, ssz, c->st));
+    ADJ_CHK(hipMemsetAsync(theta, 0, (size_t) n * 3 * sizeof(double), c->st));
+    for (int sg = ncp - 1; sg >= 0; sg--) {
+        const int start = sg * ckpt, end = (start + ckpt < nsteps) ? start + ckpt : nsteps, L = end - start;
+        char *S0 = seg, *S1 = seg + (size_t) ckpt * ssz, *S2 = seg + (size_t) 2 * ckpt * ssz;
+        ADJ_CHK(hipMemcpyAsync(S0, cp + (size_t) sg * ssz, ssz, hipMemcpyDeviceToDevice, c->st));
+        for (int j = 0; j < L; j++) {
+            void *u0 = S0 + (size_t) j * ssz, *u1 = S1 + (size_t) j * ssz, *u2 = S2 + (size_t) j * ssz;
+            void *un = (j + 1 < L) ? (void *) (S0 + (size_t) (j + 1) * ssz) : (void *) uend;
+            StageArgs A = stage_args(c, u0, u0, u1, 1, 0, dt);
+            ADJ_CHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st));
+            A = stage_args(c, u1, u0, u2, 1, 1, dt);
+            ADJ_CHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st));
+            A = stage_args(c, u2, u0, un, 1, 2, dt);
+            ADJ_CHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st));
+        }
+        for (int j = L - 1; j >= 0; j--) {
+            void *u0 = S0 + (size_t) j * ssz, *u1 = S1 + (size_t) j * ssz, *u2 = S2 + (size_t) j * ssz;
+            void *un = (j + 1 < L) ? (void *) (S0 + (size_t) (j + 1) * ssz) : (void *) uend;
+            ObjArgs O = obj_args(c, un, lo, hi);
+            O.mode = 1; O.lambda = la; O.coef = dt;
+            ADJ_CHK(launch_adj_objective(c->prec, c->nc, O, c->P, c->st));
+            AdjArgs A;
+            A.level = c->d_level; A.nbr = c->d_nbr; A.aux = c->d_aux; A.n = n; A.dt = dt;
+            A.theta_bar = theta_face >= 0 ? theta : NULL; A.theta_face = theta_face;
+            A.U = u2; A.X = la; A.ACC = NULL; A.OUT = lb; A.c_in = 2.0 / 3.0; A.c_out = 1.0; A.c_acc = 0.0;
+            ADJ_CHK(launch_adj_stage(c->prec, c->nc, c->cfg.flux, A, c->P, c->st));
+            ADJ_CHK(launch_adj_axpby(c->prec, lb, la, 0.75, 1.0 / 3.0, m, c->st));
+            A.U = u1; A.X = lb; A.OUT = lc; A.c_in = 0.25;
+            ADJ_CHK(launch_adj_stage(c->prec, c->nc, c->cfg.flux, A, c->P, c->st));
+            A.U = u0; A.X = lc; A.ACC = la; A.OUT = la; A.c_in = 1.0; A.c_acc = 1.0;
+            ADJ_CHK(launch_adj_stage(c->prec, c->nc, c->cfg.flux, A, c->P, c->st));
+        }
+    }
+    ADJ_CHK(hipEventRecord(c->ev1, c->st));
+    ADJ_CHK(hipEventSynchronize(c->ev1));
+    ADJ_CHK(hipEventElapsedTime(&ms_a, c->ev0, c->ev1));
+    if (grad_u0) {
+        ADJ_CHK(launch_convert(c->prec, 1, la, c->d_stage64, n, 4 * c->nv, c->st));
+        ADJ_CHK(hipMemcpyAsync(grad_u0, c->d_stage64, (size_t) m * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    }
+    if (grad_theta) {
+        double *h = (double *) malloc((size_t) n * 3 * sizeof(double));
+        if (!h) { charm_set_error("adjoint: out of memory"); rc = CHARM_ERR_MEM; goto done; }
+        e = hipMemcpyAsync(h, theta, (size_t) n * 3 * sizeof(double), hipMemcpyDeviceToHost, c->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+        grad_theta[0] = grad_theta[1] = grad_theta[2] = 0.0;
+        for (int64_t i = 0; i < n; i++)
+            for (int k = 0; k < 3; k++) grad_theta[k] += h[3 * i + k];
+        free(h);
+        ADJ_CHK(e);
+    }
+    ADJ_CHK(hipStreamSynchronize(c->st));
+    if (fwd_ms) *fwd_ms = ms_f;
+    if (adj_ms) *adj_ms = ms_a;
+    rc = check_flag(c);
+#undef ADJ_CHK
+done:
+    (void) hipStreamSynchronize(c->st);
+    void *bufs[] = {cp, seg, la, lb, lc, uend, part, theta};
+    for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); i++)
+        if (bufs[i]) (void) hipFree(bufs[i]);
+    return rc;
+}

@@ -88,6 +88,37 @@ struct NsArgs {
     double dt;
 };
 
+// adjoint of one SSP-RK stage (one thread per leaf):
+// OUT = c_out (sb + (d int / d U)^T W) + c_acc ACC,  sb = c_in X,  W = -dt A^-1 sb
+struct AdjArgs {
+    const void *U, *X, *ACC;
+    void *OUT;
+    double c_in, c_out, c_acc, dt;
+    const uint8_t *level;
+    const int *nbr;
+    const int4 *aux;
+    double *theta_bar;        // [n][3] += d/d(inlet u, T, p) (or null)
+    int theta_face;           // the inlet face of the parameters (-1: none)
+    int64_t n;
+};
+
+// objective over a box of leaves: mode 0 per-block partial sums of
+// sum_gp wJ p, mode 1 lambda += coef dQ/dU
+struct ObjArgs {
+    const void *U;
+    void *lambda;
+    const int4 *q;
+    double lo[3], hi[3];
+    double coef;
+    double *partial;
+    int mode;
+    int64_t n;
+};
+
+hipError_t launch_adj_stage(int prec, int nc, int flux, const AdjArgs &A, const DevParams &P, hipStream_t st);
+hipError_t launch_adj_objective(int prec, int nc, const ObjArgs &A, const DevParams &P, hipStream_t st);
+hipError_t launch_adj_axpby(int prec, const void *x, void *y, double a, double b, int64_t m, hipStream_t st);
+
 hipError_t launch_stage_f32_ph0(int nc, int flux, const StageArgs &A, const DevParams &P, hipStream_t st);
 hipError_t launch_stage_f32_ph1(int nc, int flux, const StageArgs &A, const DevParams &P, hipStream_t st);
 hipError_t launch_stage_f64_ph0(int nc, int flux, const StageArgs &A, const DevParams &P, hipStream_t st);

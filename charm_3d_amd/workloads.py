@@ -76,17 +76,27 @@ def c2(level=7, precision=1, seed=2):
     return cfg, leaves, u, (False, False, False)
 
 
-def c5_pulse(level=6, precision=1):
-    """Gaussian density pulse rho = 1 + 0.1 exp(-|x - x0|^2 / 0.1^2), x0 = 0.3."""
-    leaves = Forest.uniform(level).leaves()
+C5_BOX = ((0.6, 0.6, 0.6), (0.8, 0.8, 0.8))
+
+
+def c5_pulse(level=6, precision=1, inlet=(0.5, 1.0, 1.0), leaves=None):
+    """Gaussian density pulse rho = 1 + 0.1 exp(-|x - x0|^2 / 0.1^2),
+    x0 = (0.3, 0.3, 0.3), p = 1, at rest; Euler terms, HLLC, no limiter,
+    SSP-RK3; inlet on the -x face with (u, T, p) = inlet (v = w = 0; the
+    adjoint's parameters), transmissive elsewhere.  Nondimensional gas:
+    R / M = 1, cp 3.5."""
+    if leaves is None:
+        leaves = Forest.uniform(level).leaves()
 
     def fn(x):
         r2 = ((x - 0.3) ** 2).sum(1)
         rho = 1 + 0.1 * np.exp(-r2 / 0.01)
         return prim_to_cons(rho, np.zeros((len(x), 3)), np.ones(len(x)))
 
+    u, t, p = inlet
     cfg = make_config(precision=precision, model="NS", flux="HLLC", limiter=False,
-                      bnd=("OUTLET",) * 6, comp_m=(GAS_R,), comp_cp=(3.5,), cfl=0.3)
+                      bnd=("INLET",) + ("OUTLET",) * 5, bnd_param=[(u, 0.0, 0.0, t, p)] + [None] * 5,
+                      comp_m=(GAS_R,), comp_cp=(3.5,), cfl=0.3)
     return cfg, leaves, project_p1(leaves, fn, 5), (False, False, False)
 
 
@@ -166,18 +176,42 @@ def c3(precision=FP32, seed=3, min_level=5, max_level=8, max_err=0.04, width=Non
     return cfg, forest, project_p1(forest.leaves(), fn, 6)
 
 
-# reacting hot-spot case: H2, O2, H2O, N2 with the NASA 7-coefficient
-# polynomials (200-1000 K fits, GRI-Mech 3.0) as the cp power series
-# cp_i(T) = (R / M_i) (a1 + a2 T + a3 T^2 + a4 T^3 + a5 T^4), Sutherland
-# viscosity and conductivity with KP0 = ML0 cp(T0) / 0.72 (Prandtl 0.72),
-# Lennard-Jones diffusion parameters, and one reaction H2 + O2 -> H2O with
-# k = 1e9 exp(-15000 / T)
+# reacting hot-spot case: H2, O2, H2O, N2.  cp_i(T) is a power series
+# (R / M_i)(a1 + a2 T + a3 T^2 + a4 T^3 + a5 T^4): the least-squares quartic
+# through the two-range NASA 7-coefficient polynomials (GRI-Mech 3.0 thermo
+# data, 200-1000 K and 1000-3500 K) sampled over 250-1700 K.  A single range
+# does not serve a 300-1500 K hot spot: the 200-1000 K fit of H2 extrapolates
+# to cp / R = 1.18 at 1500 K (3.88 from the high range), which makes the
+# mixture energy (cp(T) - R / M) T non-monotonic, and the reference's Newton
+# temperature then lands on a spurious root.  Sutherland viscosity and
+# conductivity with KP0 = ML0 cp(T0) / 0.72, Lennard-Jones diffusion
+# parameters, and one reaction H2 + O2 -> H2O with k = 1e9 exp(-15000 / T).
 C4_SPECIES = ("H2", "O2", "H2O", "N2")
 C4_M = (0.002016, 0.031999, 0.018015, 0.028014)
-C4_NASA = ((2.34433112, 7.98052075e-03, -1.94781510e-05, 2.01572094e-08, -7.37611761e-12),
-           (3.78245636, -2.99673416e-03, 9.84730201e-06, -9.68129509e-09, 3.24372837e-12),
-           (4.19864056, -2.03643410e-03, 6.52040211e-06, -5.48797062e-09, 1.77197817e-12),
-           (3.29867700, 1.40824040e-03, -3.96322200e-06, 5.64151500e-09, -2.44485400e-12))
+C4_NASA_LOW = ((2.34433112, 7.98052075e-03, -1.94781510e-05, 2.01572094e-08, -7.37611761e-12),
+               (3.78245636, -2.99673416e-03, 9.84730201e-06, -9.68129509e-09, 3.24372837e-12),
+               (4.19864056, -2.03643410e-03, 6.52040211e-06, -5.48797062e-09, 1.77197817e-12),
+               (3.29867700, 1.40824040e-03, -3.96322200e-06, 5.64151500e-09, -2.44485400e-12))
+C4_NASA_HIGH = ((3.33727920, -4.94024731e-05, 4.99456778e-07, -1.79566394e-10, 2.00255376e-14),
+                (3.28253784, 1.48308754e-03, -7.57966669e-07, 2.09470555e-10, -2.16717794e-14),
+                (3.03399249, 2.17691804e-03, -1.64072518e-07, -9.70419870e-11, 1.68200992e-14),
+                (2.92664000, 1.48797680e-03, -5.68476000e-07, 1.00970380e-10, -6.75335100e-15))
+
+
+def c4_cp_over_r():
+    """quartic cp / R per species (T in K), fitted to both NASA ranges"""
+    t = np.linspace(250.0, 1700.0, 400)
+    x = t / 1000.0
+    a = np.vstack([x ** k for k in range(5)]).T
+    out = []
+    for lo, hi in zip(C4_NASA_LOW, C4_NASA_HIGH):
+        y = np.where(t <= 1000.0, sum(c * t ** k for k, c in enumerate(lo)),
+                     sum(c * t ** k for k, c in enumerate(hi)))
+        coef = np.linalg.lstsq(a, y, rcond=None)[0]
+        out.append(tuple(float(coef[k]) / 1000.0 ** k for k in range(5)))
+    return tuple(out)
+
+
 C4_H0 = (0.0, 0.0, -241826.0 / 0.018015, 0.0)            # formation enthalpy at 298.15 K, J/kg
 C4_ML = ((8.41e-6, 72.0), (1.919e-5, 127.0), (1.12e-5, 1064.0), (1.663e-5, 111.0))   # mu(273 K), S
 C4_LJ = ((2.92, 38.0), (3.458, 107.4), (2.605, 572.4), (3.621, 97.53))               # sigma A, eps/k K
@@ -185,7 +219,7 @@ C4_Y = (0.02, 0.22, 0.0, 0.76)
 
 
 def c4_config(precision=FP32, limiter=False):
-    cps = tuple(tuple(a * GAS_R / m for a in nasa) for nasa, m in zip(C4_NASA, C4_M))
+    cps = tuple(tuple(a * GAS_R / m for a in fit) for fit, m in zip(c4_cp_over_r(), C4_M))
     t0 = 273.0
     tr = []
     for i in range(4):

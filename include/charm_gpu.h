@@ -216,6 +216,22 @@ int charm_gpu_remesh(charm_gpu_ctx_t *ctx, const charm_mesh_t *mesh, const int64
 int charm_gpu_ns_aux(charm_gpu_ctx_t *ctx, double *aux, int64_t n);
 int charm_gpu_ns_chem(charm_gpu_ctx_t *ctx, double dt, double *chem_rhs, int64_t n);
 
+/* Discrete adjoint (no counterpart in the reference): nsteps SSP-RK3 steps
+ * (Euler terms, no limiter, constant cp, 1-2 species, LF / HLLC / CD) from
+ * the current state with a fixed dt; J = sum_{n=1..N} dt Q(u^n), Q(u) = int
+ * of the pressure over the leaves whose centre lies in [lo, hi].
+ * charm_gpu_objective runs the forward pass and returns J.
+ * charm_gpu_adjoint also returns dJ/du^0 (FP64 [n][V][4], may be NULL) and
+ * dJ/d(inlet u, T, p) of boundary face theta_face (-1: none) by reverse
+ * passes over checkpoints stored every ckpt steps (each segment's stage
+ * states are recomputed), and the device times of the forward and the
+ * reverse pass.  Both leave the state at u^N. */
+int charm_gpu_objective(charm_gpu_ctx_t *ctx, double dt, int nsteps, const double lo[3],
+                        const double hi[3], double *J);
+int charm_gpu_adjoint(charm_gpu_ctx_t *ctx, double dt, int nsteps, int ckpt, const double lo[3],
+                      const double hi[3], int theta_face, double *J, double *grad_u0,
+                      double *grad_theta, double *fwd_ms, double *adj_ms);
+
 /* refine (+1) / coarsen (-1 on all eight members of a family) / keep (0)
  * decisions from the estimator, with the criteria of
  * charm_refine_err_estimate / charm_coarsen_err_estimate
