@@ -6,7 +6,10 @@ transmissive boundaries, dt = 0.3 h / max(|u| + c) fixed at t = 0).  Both the
 FP32-bulk and the FP64 plain paths run K steps from the same initial state;
 value = DOF-updates/s of the FP32 path (E * 4 * V * 3 stages * K / wall).
 
-    python bench.py [--steps K] [--warmup W] [--level L] [--no-fp64] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--level L] [--no-fp64] [--no-cpu] [--no-extra]
+
+The JSON line also carries the other configurations at full size (adaptive
+bubbles, reacting hot spot, adjoint of the pulse case) under "workloads".
 """
 import argparse
 import json
@@ -87,6 +90,69 @@ def cpu_baseline(W, budget_s):
                       "HLLC + BJ, SSP-RK3, %d steps in %.1f s on one host core" % (steps, t)}
 
 
+def extra_workloads(CA, W):
+    """The other configurations at their full sizes (FP32), each a few
+    seconds: the adaptive bubble case, the reacting hot spot, and the
+    adjoint of the pulse case (adjoint / forward wall-time ratio)."""
+    out = {}
+    # bubbles: levels 5-8 (about 1M leaves), exact Riemann, two species,
+    # re-adaptation every 20 steps
+    t0 = time.perf_counter()
+    cfg, forest, u0 = W.c3(precision=CA.FP32)
+    t_init = time.perf_counter() - t0
+    s = CA.Solver(forest, cfg)
+    s.set_state(u0)
+    dt = s.get_dt(1)
+    n0 = forest.n
+    m0 = s.conservation_sums()
+    gpu_ms, adapt_s, dofs = 0.0, 0.0, 0
+    for blk in range(10):
+        if blk:
+            t = time.perf_counter()
+            s.adapt()
+            adapt_s += time.perf_counter() - t
+        dofs += W.dof_updates(forest.n, 6, 3, 20)
+        s.run(dt, 20)
+        gpu_ms += s.last_ms()
+    m1 = s.conservation_sums()
+    out["bubbles_adaptive"] = {
+        "leaves_initial": n0, "leaves_final": forest.n, "steps": 200, "adapt_every": 20,
+        "dof_updates_per_s_device": dofs / (gpu_ms * 1e-3),
+        "dof_updates_per_s_with_adaptation": dofs / (gpu_ms * 1e-3 + adapt_s),
+        "ms_per_step": gpu_ms / 200, "host_s_per_adaptation": adapt_s / 9, "init_s": t_init,
+        "species_mass_change": [float((m1[4 + i] - m0[4 + i]) / m0[4 + i]) for i in range(2)]}
+    s.close()
+    # reacting hot spot: level 6, four species, BR1 viscous terms, diffusion,
+    # chemistry, 100 steps
+    cfg, leaves, u0, per = W.c4(6, CA.FP32)
+    ctx = CA.GpuContext(cfg)
+    ce, cf = CA.recommended_chunks(cfg)
+    ctx.upload_mesh(CA.Forest.uniform(6).flatten(ce, cf))
+    ctx.set_state(u0)
+    dt = ctx.get_dt(1)
+    ctx.run(dt, 2)
+    ctx.set_state(u0)
+    ctx.run(dt, 100)
+    ms = ctx.last_ms()
+    out["reacting_hot_spot"] = {"leaves": len(leaves), "steps": 100, "ms_per_step": ms / 100,
+                                "dof_updates_per_s": W.dof_updates(len(leaves), 8, 3, 100) / (ms * 1e-3)}
+    ctx.close()
+    # adjoint of the pulse case: level 6, 400 steps, checkpoints every 20
+    cfg, leaves, u0, per = W.c5_pulse(6, CA.FP32)
+    ctx = CA.GpuContext(cfg)
+    ce, cf = CA.recommended_chunks(cfg)
+    ctx.upload_mesh(CA.Forest.uniform(6).flatten(ce, cf))
+    ctx.set_state(u0)
+    dt = ctx.get_dt(1)
+    lo, hi = W.C5_BOX
+    J, g, th, fwd_ms, adj_ms = ctx.adjoint(dt, 400, lo, hi, ckpt=20, theta_face=0)
+    out["pulse_adjoint"] = {"leaves": len(leaves), "steps": 400, "checkpoint_every": 20, "J": J,
+                            "dJ_dinlet_u_T_p": th.tolist(), "forward_ms": fwd_ms, "adjoint_ms": adj_ms,
+                            "adjoint_over_forward": adj_ms / fwd_ms}
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +161,7 @@ def main():
     ap.add_argument("--level", type=int, default=7)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--kernel-reps", type=int, default=20)
     args = ap.parse_args()
@@ -165,6 +232,11 @@ def main():
                                  "l2_rel_means": float(np.linalg.norm(d[:, :, 0]) / np.linalg.norm(r64["state"][:, :, 0]))}
         out["speedup_fp32_over_fp64"] = r64["wall"] / r32["wall"]
         del r64
+
+    if not args.no_extra:
+        print("[bench] bubble, hot-spot and adjoint cases", file=sys.stderr, flush=True)
+        out["workloads"] = extra_workloads(CA, W)
+        out["adjoint_over_forward"] = out["workloads"]["pulse_adjoint"]["adjoint_over_forward"]
 
     if not args.no_cpu:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
