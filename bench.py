@@ -93,8 +93,20 @@ def cpu_baseline(W, budget_s):
 def extra_workloads(CA, W):
     """The other configurations at their full sizes (FP32), each a few
     seconds: the adaptive bubble case, the reacting hot spot, and the
-    adjoint of the pulse case (adjoint / forward wall-time ratio)."""
+    adjoint of the pulse case (adjoint / forward wall-time ratio).  A case
+    that fails reports its error instead of a measurement."""
     out = {}
+    for name, fn in (("bubbles_adaptive", bench_bubbles), ("reacting_hot_spot", bench_hot_spot),
+                     ("pulse_adjoint", bench_adjoint)):
+        print("[bench]   %s" % name, file=sys.stderr, flush=True)
+        try:
+            out[name] = fn(CA, W)
+        except CA.CharmError as e:
+            out[name] = {"error": str(e)}
+    return out
+
+
+def bench_bubbles(CA, W):
     # bubbles: levels 5-8 (about 1M leaves), exact Riemann, two species,
     # re-adaptation every 20 steps
     t0 = time.perf_counter()
@@ -115,13 +127,17 @@ def extra_workloads(CA, W):
         s.run(dt, 20)
         gpu_ms += s.last_ms()
     m1 = s.conservation_sums()
-    out["bubbles_adaptive"] = {
+    out = {
         "leaves_initial": n0, "leaves_final": forest.n, "steps": 200, "adapt_every": 20,
         "dof_updates_per_s_device": dofs / (gpu_ms * 1e-3),
         "dof_updates_per_s_with_adaptation": dofs / (gpu_ms * 1e-3 + adapt_s),
         "ms_per_step": gpu_ms / 200, "host_s_per_adaptation": adapt_s / 9, "init_s": t_init,
         "species_mass_change": [float((m1[4 + i] - m0[4 + i]) / m0[4 + i]) for i in range(2)]}
     s.close()
+    return out
+
+
+def bench_hot_spot(CA, W):
     # reacting hot spot: level 6, four species, BR1 viscous terms, diffusion,
     # chemistry, 100 steps
     cfg, leaves, u0, per = W.c4(6, CA.FP32)
@@ -134,9 +150,12 @@ def extra_workloads(CA, W):
     ctx.set_state(u0)
     ctx.run(dt, 100)
     ms = ctx.last_ms()
-    out["reacting_hot_spot"] = {"leaves": len(leaves), "steps": 100, "ms_per_step": ms / 100,
-                                "dof_updates_per_s": W.dof_updates(len(leaves), 8, 3, 100) / (ms * 1e-3)}
     ctx.close()
+    return {"leaves": len(leaves), "steps": 100, "ms_per_step": ms / 100,
+            "dof_updates_per_s": W.dof_updates(len(leaves), 8, 3, 100) / (ms * 1e-3)}
+
+
+def bench_adjoint(CA, W):
     # adjoint of the pulse case: level 6, 400 steps, checkpoints every 20
     cfg, leaves, u0, per = W.c5_pulse(6, CA.FP32)
     ctx = CA.GpuContext(cfg)
@@ -146,11 +165,10 @@ def extra_workloads(CA, W):
     dt = ctx.get_dt(1)
     lo, hi = W.C5_BOX
     J, g, th, fwd_ms, adj_ms = ctx.adjoint(dt, 400, lo, hi, ckpt=20, theta_face=0)
-    out["pulse_adjoint"] = {"leaves": len(leaves), "steps": 400, "checkpoint_every": 20, "J": J,
-                            "dJ_dinlet_u_T_p": th.tolist(), "forward_ms": fwd_ms, "adjoint_ms": adj_ms,
-                            "adjoint_over_forward": adj_ms / fwd_ms}
     ctx.close()
-    return out
+    return {"leaves": len(leaves), "steps": 400, "checkpoint_every": 20, "J": J,
+            "dJ_dinlet_u_T_p": th.tolist(), "forward_ms": fwd_ms, "adjoint_ms": adj_ms,
+            "adjoint_over_forward": adj_ms / fwd_ms}
 
 
 def main():
@@ -236,7 +254,7 @@ def main():
     if not args.no_extra:
         print("[bench] bubble, hot-spot and adjoint cases", file=sys.stderr, flush=True)
         out["workloads"] = extra_workloads(CA, W)
-        out["adjoint_over_forward"] = out["workloads"]["pulse_adjoint"]["adjoint_over_forward"]
+        out["adjoint_over_forward"] = out["workloads"]["pulse_adjoint"].get("adjoint_over_forward")
 
     if not args.no_cpu:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
