@@ -84,7 +84,8 @@ def c5_pulse(level=6, precision=1, inlet=(0.5, 1.0, 1.0), leaves=None):
     x0 = (0.3, 0.3, 0.3), p = 1, at rest; Euler terms, HLLC, no limiter,
     SSP-RK3; inlet on the -x face with (u, T, p) = inlet (v = w = 0; the
     adjoint's parameters), transmissive elsewhere.  Nondimensional gas:
-    R / M = 1, cp 3.5."""
+    R / M = 1, cp 3.5.  Without a limiter the P1 scheme needs CFL 0.1
+    (dt = 0.1 h / max(|u| + c)); at 0.3 it diverges within 50 steps."""
     if leaves is None:
         leaves = Forest.uniform(level).leaves()
 
@@ -96,7 +97,7 @@ def c5_pulse(level=6, precision=1, inlet=(0.5, 1.0, 1.0), leaves=None):
     u, t, p = inlet
     cfg = make_config(precision=precision, model="NS", flux="HLLC", limiter=False,
                       bnd=("INLET",) + ("OUTLET",) * 5, bnd_param=[(u, 0.0, 0.0, t, p)] + [None] * 5,
-                      comp_m=(GAS_R,), comp_cp=(3.5,), cfl=0.3)
+                      comp_m=(GAS_R,), comp_cp=(3.5,), cfl=0.1)
     return cfg, leaves, project_p1(leaves, fn, 5), (False, False, False)
 
 
