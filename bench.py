@@ -238,6 +238,9 @@ def main():
     mass0 = r32["m0"][4]
     out["fp32"] = {"mass_drift": abs(r32["m1"][4] - mass0) / abs(mass0)}
 
+    print("[bench] mixed path (FP64 state, FP32 residual)", file=sys.stderr, flush=True)
+    rmx = run_path(CA, W, args.level, CA.FP32_MIX, args.steps, args.warmup, u0=r32["u0"])
+
     if not args.no_fp64:
         print("[bench] FP64 plain path", file=sys.stderr, flush=True)
         r64 = run_path(CA, W, args.level, CA.FP64, args.steps, min(args.warmup, 2), u0=r32["u0"])
@@ -249,6 +252,11 @@ def main():
         out["errors_vs_fp64"] = {"l2_rel": l2, "linf_rel": linf,
                                  "l2_rel_means": float(np.linalg.norm(d[:, :, 0]) / np.linalg.norm(r64["state"][:, :, 0]))}
         out["speedup_fp32_over_fp64"] = r64["wall"] / r32["wall"]
+        dm = rmx["state"] - r64["state"]
+        out["fp32_mix"] = {"value": dofs / rmx["wall"], "ms_per_step": 1e3 * rmx["wall"] / args.steps,
+                           "mass_drift": abs(rmx["m1"][4] - rmx["m0"][4]) / abs(rmx["m0"][4]),
+                           "errors_vs_fp64": {"l2_rel": float(np.linalg.norm(dm) / np.linalg.norm(r64["state"])),
+                                              "linf_rel": float(np.abs(dm).max() / np.abs(r64["state"]).max())}}
         del r64
 
     if not args.no_extra:

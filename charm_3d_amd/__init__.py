@@ -14,7 +14,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libcharm3d.so")
 
 MAX_COMP = 8
-FP64, FP32 = 0, 1
+FP64, FP32, FP32_MIX = 0, 1, 2
 FLUX = {"LF": 0, "HLLC": 1, "GODUNOV": 2, "CD": 3}
 BND = {"INLET": 0, "OUTLET": 1, "WALL_SLIP": 2, "WALL_NO_SLIP": 3, "MASS_FLOW": 4,
        "PERIODIC": 100}

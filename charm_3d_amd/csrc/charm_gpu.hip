@@ -20,6 +20,7 @@ extern "C" const char *charm_host_last_error(void);
 struct charm_gpu_ctx {
     charm_config_t cfg;
     int prec, nc, nv;
+    int sprec;                // storage: 1 FP32 state, 0 FP64 state (FP64 and mixed modes)
     int ph;                   // 1: Navier-Stokes / real-gas kernels (PH = 1)
     size_t esz;
     DevParams P;
@@ -87,7 +88,8 @@ extern "C" int charm_gpu_create(const charm_config_t *cfg, charm_gpu_ctx_t **out
         charm_set_error("gpu_create: n_comp %d not in [1, 4]", cfg->n_comp);
         return CHARM_ERR_UNSUPPORTED;
     }
-    if (cfg->flux < 0 || cfg->flux > 3 || (cfg->precision != CHARM_FP64 && cfg->precision != CHARM_FP32)) {
+    if (cfg->flux < 0 || cfg->flux > 3 ||
+        (cfg->precision != CHARM_FP64 && cfg->precision != CHARM_FP32 && cfg->precision != CHARM_FP32_MIX)) {
         charm_set_error("gpu_create: bad flux %d or precision %d", cfg->flux, cfg->precision);
         return CHARM_ERR_ARG;
     }
@@ -117,9 +119,10 @@ extern "C" int charm_gpu_create(const charm_config_t *cfg, charm_gpu_ctx_t **out
     if (!c) { charm_set_error("gpu_create: out of memory"); return CHARM_ERR_MEM; }
     c->cfg = *cfg;
     c->prec = cfg->precision;
+    c->sprec = cfg->precision == CHARM_FP32;
     c->nc = cfg->n_comp;
     c->nv = 4 + cfg->n_comp;
-    c->esz = c->prec ? sizeof(float) : sizeof(double);
+    c->esz = c->sprec ? sizeof(float) : sizeof(double);
     // MI355X: 8 XCDs; only the workgroup-to-L2 placement depends on it
     c->n_xcd = 8;
     if (getenv("CHARM_NUM_XCD")) c->n_xcd = atoi(getenv("CHARM_NUM_XCD"));
@@ -165,15 +168,21 @@ extern "C" int charm_gpu_create(const charm_config_t *cfg, charm_gpu_ctx_t **out
         P.r_a[r] = cfg->react[r].a; P.r_e[r] = cfg->react[r].e; P.r_n[r] = cfg->react[r].n;
     }
     c->ph = (P.newton_t || any_poly || P.use_visc || P.n_react > 0) ? 1 : 0;
+    if (c->ph && c->prec == CHARM_FP32_MIX) {
+        free(c);
+        charm_set_error("gpu_create: the mixed precision mode covers the constant-cp Euler terms only");
+        return CHARM_ERR_UNSUPPORTED;
+    }
     {
         // FP64 mass-matrix inverse (level 0); the octree's matrices are
         // diagonal, which is checked here
         double a[4][4], ai[4][4];
-        if (charm_level_mass_inverse(0, a, ai)) return CHARM_ERR_ARG;
+        if (charm_level_mass_inverse(0, a, ai)) { free(c); return CHARM_ERR_ARG; }
         for (int i = 0; i < 4; i++)
             for (int j = 0; j < 4; j++)
                 if (i != j && fabs(ai[i][j]) > 1e-14 * fabs(ai[i][i])) {
                     charm_set_error("gpu_create: mass matrix inverse is not diagonal");
+                    free(c);
                     return CHARM_ERR_ARG;
                 }
         for (int k = 0; k < 4; k++) { P.ainv0[k] = ai[k][k]; P.ainv0_f[k] = (float) ai[k][k]; }
@@ -304,7 +313,7 @@ extern "C" int charm_gpu_set_state(charm_gpu_ctx_t *c, const double *u, int64_t 
     if (!u) { charm_set_error("set_state: null state"); return CHARM_ERR_ARG; }
     const int64_t cnt = n * 4 * c->nv;
     HIPCHK(hipMemcpyAsync(c->d_stage64, u, (size_t) cnt * sizeof(double), hipMemcpyHostToDevice, c->st));
-    HIPCHK(launch_convert(c->prec, 0, c->d_stage64, c->B[0], n, 4 * c->nv, c->st));
+    HIPCHK(launch_convert(c->sprec, 0, c->d_stage64, c->B[0], n, 4 * c->nv, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     return CHARM_OK;
 }
@@ -312,7 +321,7 @@ extern "C" int charm_gpu_set_state(charm_gpu_ctx_t *c, const double *u, int64_t 
 static int copy_out(charm_gpu_ctx_t *c, const void *src, double *u, int64_t n)
 {
     const int64_t cnt = n * 4 * c->nv;
-    HIPCHK(launch_convert(c->prec, 1, src, c->d_stage64, n, 4 * c->nv, c->st));
+    HIPCHK(launch_convert(c->sprec, 1, src, c->d_stage64, n, 4 * c->nv, c->st));
     HIPCHK(hipMemcpyAsync(u, c->d_stage64, (size_t) cnt * sizeof(double), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     return CHARM_OK;
@@ -384,14 +393,14 @@ static hipError_t ns_prepare(charm_gpu_ctx_t *c, const void *in, void *chem_out,
     if (c->cfg.model != CHARM_MODEL_NS || !c->ph) return hipSuccess;
     if (c->cfg.n_react > 0) {
         NsArgs A = ns_args(c, in, chem_out, dt);
-        if ((e = launch_ns(c->prec, 0, c->nc, A, c->P, c->st)) != hipSuccess) return e;
+        if ((e = launch_ns(c->sprec, 0, c->nc, A, c->P, c->st)) != hipSuccess) return e;
         *stage_in = chem_out;
     }
     if (c->cfg.use_visc) {
         NsArgs A = ns_args(c, *stage_in, NULL, dt);
-        if (A.dcoef && (e = launch_ns(c->prec, 1, c->nc, A, c->P, c->st)) != hipSuccess) return e;
-        if ((e = launch_ns(c->prec, 2, c->nc, A, c->P, c->st)) != hipSuccess) return e;
-        if ((e = launch_ns(c->prec, 3, c->nc, A, c->P, c->st)) != hipSuccess) return e;
+        if (A.dcoef && (e = launch_ns(c->sprec, 1, c->nc, A, c->P, c->st)) != hipSuccess) return e;
+        if ((e = launch_ns(c->sprec, 2, c->nc, A, c->P, c->st)) != hipSuccess) return e;
+        if ((e = launch_ns(c->sprec, 3, c->nc, A, c->P, c->st)) != hipSuccess) return e;
     }
     return hipSuccess;
 }
@@ -403,9 +412,9 @@ extern "C" int charm_gpu_residual(charm_gpu_ctx_t *c, double *out, int64_t n)
     if (c->ph && c->cfg.model == CHARM_MODEL_NS && c->cfg.use_visc) {
         // viscous passes on the current state, without the chemistry
         NsArgs N = ns_args(c, c->B[0], NULL, 0.0);
-        if (N.dcoef) HIPCHK(launch_ns(c->prec, 1, c->nc, N, c->P, c->st));
-        HIPCHK(launch_ns(c->prec, 2, c->nc, N, c->P, c->st));
-        HIPCHK(launch_ns(c->prec, 3, c->nc, N, c->P, c->st));
+        if (N.dcoef) HIPCHK(launch_ns(c->sprec, 1, c->nc, N, c->P, c->st));
+        HIPCHK(launch_ns(c->sprec, 2, c->nc, N, c->P, c->st));
+        HIPCHK(launch_ns(c->sprec, 3, c->nc, N, c->P, c->st));
     }
     StageArgs A = stage_args(c, c->B[0], c->B[0], c->B[1], 0, 0, 0.0);
     HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st));
@@ -419,7 +428,7 @@ extern "C" int charm_gpu_limiter_bj(charm_gpu_ctx_t *c)
     int rc = need_mesh(c, -1);
     if (rc) return rc;
     LimArgs A = lim_args(c, c->B[0], c->B[0], c->B[1], 1, 0);
-    HIPCHK(launch_limiter(c->prec, c->nc, c->ph, A, c->P, c->st));
+    HIPCHK(launch_limiter(c->sprec, c->nc, c->ph, A, c->P, c->st));
     void *t = c->B[0]; c->B[0] = c->B[1]; c->B[1] = t;
     HIPCHK(hipStreamSynchronize(c->st));
     return CHARM_OK;
@@ -436,7 +445,7 @@ static int step_once(charm_gpu_ctx_t *c, double dt)
         HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st));
         if (lim) {
             LimArgs L = lim_args(c, B1, B0, B0, 1, 0);
-            HIPCHK(launch_limiter(c->prec, c->nc, c->ph, L, c->P, c->st));
+            HIPCHK(launch_limiter(c->sprec, c->nc, c->ph, L, c->P, c->st));
         } else {
             c->B[0] = B1; c->B[1] = B0;
         }
@@ -452,17 +461,17 @@ static int step_once(charm_gpu_ctx_t *c, double dt)
         StageArgs A = stage_args(c, X, B0, B1, 1, 0, dt);
         HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st));
         LimArgs L = lim_args(c, B1, B0, B2, 1, 0);
-        HIPCHK(launch_limiter(c->prec, c->nc, c->ph, L, c->P, c->st));
+        HIPCHK(launch_limiter(c->sprec, c->nc, c->ph, L, c->P, c->st));
         HIPCHK(ns_prepare(c, B2, B2, dt, &X));
         A = stage_args(c, X, B0, B1, 1, 0, dt);
         HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st));
         L = lim_args(c, B1, B0, B2, 1, 1);
-        HIPCHK(launch_limiter(c->prec, c->nc, c->ph, L, c->P, c->st));
+        HIPCHK(launch_limiter(c->sprec, c->nc, c->ph, L, c->P, c->st));
         HIPCHK(ns_prepare(c, B2, B2, dt, &X));
         A = stage_args(c, X, B0, B1, 1, 0, dt);
         HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st));
         L = lim_args(c, B1, B0, B0, 1, 2);
-        HIPCHK(launch_limiter(c->prec, c->nc, c->ph, L, c->P, c->st));
+        HIPCHK(launch_limiter(c->sprec, c->nc, c->ph, L, c->P, c->st));
     } else {
         HIPCHK(ns_prepare(c, B0, B2, dt, &X));
         StageArgs A = stage_args(c, X, B0, B1, 1, 0, dt);
@@ -551,7 +560,7 @@ extern "C" int charm_gpu_time_kernels(charm_gpu_ctx_t *c, double dt, int reps, d
     if (stage_ms) *stage_ms = ms / reps;
     LimArgs L = lim_args(c, c->B[1], c->B[0], c->B[2], 1, 1);
     HIPCHK(hipEventRecord(c->ev0, c->st));
-    for (int r = 0; r < reps; r++) HIPCHK(launch_limiter(c->prec, c->nc, c->ph, L, c->P, c->st));
+    for (int r = 0; r < reps; r++) HIPCHK(launch_limiter(c->sprec, c->nc, c->ph, L, c->P, c->st));
     HIPCHK(hipEventRecord(c->ev1, c->st));
     HIPCHK(hipEventSynchronize(c->ev1));
     HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
@@ -572,7 +581,7 @@ static int reduce(charm_gpu_ctx_t *c, int mode, double *host)
     RedArgs A;
     A.U = c->B[0]; A.level = c->d_level; A.partial = c->d_partial; A.result = c->d_result;
     A.n = c->n; A.mode = mode;
-    HIPCHK(launch_reduce(c->prec, c->nc, c->ph, A, c->P, c->st));
+    HIPCHK(launch_reduce(c->sprec, c->nc, c->ph, A, c->P, c->st));
     HIPCHK(hipMemcpyAsync(host, c->d_result, (size_t) (c->nv + 1) * sizeof(double), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     return CHARM_OK;
@@ -637,7 +646,7 @@ extern "C" int charm_gpu_flux_batch(charm_gpu_ctx_t *c, int flux, int64_t n, con
     if (e == hipSuccess) {
         FluxArgs A;
         A.prim_l = dl; A.prim_r = dr; A.normals = dn; A.out = dq; A.n = n;
-        e = launch_flux_points(c->prec, c->nc, flux, A, c->P, c->st);
+        e = launch_flux_points(c->sprec, c->nc, flux, A, c->P, c->st);
     }
     if (e == hipSuccess) e = hipMemcpyAsync(out, dq, no, hipMemcpyDeviceToHost, c->st);
     if (e == hipSuccess) e = hipStreamSynchronize(c->st);
@@ -656,7 +665,7 @@ extern "C" int charm_gpu_amr_indicator(charm_gpu_ctx_t *c, double *err2, double 
     AmrArgs A;
     A.U = c->B[0]; A.level = c->d_level; A.nbr = c->d_nbr; A.aux = c->d_aux;
     A.err2 = c->d_stage64; A.rho_c = c->d_stage64 + n; A.n = n;
-    HIPCHK(launch_amr_indicator(c->prec, c->nc, c->ph, A, c->P, c->st));
+    HIPCHK(launch_amr_indicator(c->sprec, c->nc, c->ph, A, c->P, c->st));
     HIPCHK(hipMemcpyAsync(err2, A.err2, (size_t) n * sizeof(double), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipMemcpyAsync(rho_c, A.rho_c, (size_t) n * sizeof(double), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
@@ -735,7 +744,7 @@ extern "C" int charm_gpu_remesh(charm_gpu_ctx_t *c, const charm_mesh_t *m, const
             ProjArgs A;
             A.Uold = u_old; A.Unew = c->B[0]; A.old_q = q_old; A.new_q = c->d_q; A.map = d_map;
             A.n_old = n_old; A.n_new = m->n; A.nvk = 4 * c->nv;
-            e = launch_amr_project(c->prec, A, c->st);
+            e = launch_amr_project(c->sprec, A, c->st);
         }
         if (e == hipSuccess) e = hipStreamSynchronize(c->st);
     }
@@ -756,10 +765,10 @@ extern "C" int charm_gpu_ns_aux(charm_gpu_ctx_t *c, double *aux, int64_t n)
     if (rc) return rc;
     if (!c->ph || !aux) { charm_set_error("ns_aux: no Navier-Stokes terms in this configuration"); return CHARM_ERR_ARG; }
     NsArgs N = ns_args(c, c->B[0], NULL, 0.0);
-    if (N.dcoef) HIPCHK(launch_ns(c->prec, 1, c->nc, N, c->P, c->st));
-    HIPCHK(launch_ns(c->prec, 2, c->nc, N, c->P, c->st));
+    if (N.dcoef) HIPCHK(launch_ns(c->sprec, 1, c->nc, N, c->P, c->st));
+    HIPCHK(launch_ns(c->sprec, 2, c->nc, N, c->P, c->st));
     // [36][n] -> [n][9][4] FP64
-    HIPCHK(launch_convert(c->prec, 1, c->d_auxf, c->d_stage64, n, 36, c->st));
+    HIPCHK(launch_convert(c->sprec, 1, c->d_auxf, c->d_stage64, n, 36, c->st));
     HIPCHK(hipMemcpyAsync(aux, c->d_stage64, (size_t) n * 36 * sizeof(double), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     return CHARM_OK;
@@ -771,7 +780,7 @@ extern "C" int charm_gpu_ns_chem(charm_gpu_ctx_t *c, double dt, double *chem_rhs
     if (rc) return rc;
     if (!c->ph || c->cfg.n_react <= 0) { charm_set_error("ns_chem: no reactions in this configuration"); return CHARM_ERR_ARG; }
     NsArgs N = ns_args(c, c->B[0], c->B[0], dt);
-    HIPCHK(launch_ns(c->prec, 0, c->nc, N, c->P, c->st));
+    HIPCHK(launch_ns(c->sprec, 0, c->nc, N, c->P, c->st));
     if (chem_rhs) HIPCHK(hipMemcpyAsync(chem_rhs, c->d_chem, (size_t) n * sizeof(double), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     return CHARM_OK;
@@ -793,7 +802,7 @@ static int adj_check(charm_gpu_ctx_t *c, int nsteps, int ckpt, int theta_face)
 {
     int rc = need_mesh(c, -1);
     if (rc) return rc;
-    if (c->cfg.model != CHARM_MODEL_NS || c->cfg.limiter || c->ph || c->nc > 2 ||
+    if (c->cfg.model != CHARM_MODEL_NS || c->cfg.limiter || c->ph || c->nc > 2 || c->prec == CHARM_FP32_MIX ||
         c->cfg.flux == CHARM_FLUX_GODUNOV) {
         charm_set_error("adjoint: needs the SSP-RK3 model without limiter, constant cp, 1-2 species and "
                         "the LF, HLLC or CD flux");
@@ -828,7 +837,7 @@ static int adj_forward(charm_gpu_ctx_t *c, double dt, int nsteps, int ckpt, cons
         if (rc) return rc;
         ObjArgs O = obj_args(c, c->B[0], lo, hi);
         O.partial = part + (size_t) s * nblk;
-        HIPCHK(launch_adj_objective(c->prec, c->nc, O, c->P, c->st));
+        HIPCHK(launch_adj_objective(c->sprec, c->nc, O, c->P, c->st));
     }
     double *h = (double *) malloc((size_t) nsteps * nblk * sizeof(double));
     if (!h) { charm_set_error("adjoint: out of memory"); return CHARM_ERR_MEM; }
@@ -916,24 +925,24 @@ extern "C" int charm_gpu_adjoint(charm_gpu_ctx_t *c, double dt, int nsteps, int 
             void *un = (j + 1 < L) ? (void *) (S0 + (size_t) (j + 1) * ssz) : (void *) uend;
             ObjArgs O = obj_args(c, un, lo, hi);
             O.mode = 1; O.lambda = la; O.coef = dt;
-            ADJ_CHK(launch_adj_objective(c->prec, c->nc, O, c->P, c->st));
+            ADJ_CHK(launch_adj_objective(c->sprec, c->nc, O, c->P, c->st));
             AdjArgs A;
             A.level = c->d_level; A.nbr = c->d_nbr; A.aux = c->d_aux; A.n = n; A.dt = dt;
             A.theta_bar = theta_face >= 0 ? theta : NULL; A.theta_face = theta_face;
             A.U = u2; A.X = la; A.ACC = NULL; A.OUT = lb; A.c_in = 2.0 / 3.0; A.c_out = 1.0; A.c_acc = 0.0;
-            ADJ_CHK(launch_adj_stage(c->prec, c->nc, c->cfg.flux, A, c->P, c->st));
-            ADJ_CHK(launch_adj_axpby(c->prec, lb, la, 0.75, 1.0 / 3.0, m, c->st));
+            ADJ_CHK(launch_adj_stage(c->sprec, c->nc, c->cfg.flux, A, c->P, c->st));
+            ADJ_CHK(launch_adj_axpby(c->sprec, lb, la, 0.75, 1.0 / 3.0, m, c->st));
             A.U = u1; A.X = lb; A.OUT = lc; A.c_in = 0.25;
-            ADJ_CHK(launch_adj_stage(c->prec, c->nc, c->cfg.flux, A, c->P, c->st));
+            ADJ_CHK(launch_adj_stage(c->sprec, c->nc, c->cfg.flux, A, c->P, c->st));
             A.U = u0; A.X = lc; A.ACC = la; A.OUT = la; A.c_in = 1.0; A.c_acc = 1.0;
-            ADJ_CHK(launch_adj_stage(c->prec, c->nc, c->cfg.flux, A, c->P, c->st));
+            ADJ_CHK(launch_adj_stage(c->sprec, c->nc, c->cfg.flux, A, c->P, c->st));
         }
     }
     ADJ_CHK(hipEventRecord(c->ev1, c->st));
     ADJ_CHK(hipEventSynchronize(c->ev1));
     ADJ_CHK(hipEventElapsedTime(&ms_a, c->ev0, c->ev1));
     if (grad_u0) {
-        ADJ_CHK(launch_convert(c->prec, 1, la, c->d_stage64, n, 4 * c->nv, c->st));
+        ADJ_CHK(launch_convert(c->sprec, 1, la, c->d_stage64, n, 4 * c->nv, c->st));
         ADJ_CHK(hipMemcpyAsync(grad_u0, c->d_stage64, (size_t) m * sizeof(double), hipMemcpyDeviceToHost, c->st));
     }
     if (grad_theta) {

@@ -254,6 +254,7 @@ hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st)
 // ---------------------------------------------------------------- launchers
 hipError_t launch_stage(int prec, int nc, int flux, int ph, const StageArgs &A, const DevParams &P, hipStream_t st)
 {
+    if (prec == 2) return ph ? hipErrorInvalidValue : launch_stage_mix_ph0(nc, flux, A, P, st);
     if (prec) return ph ? launch_stage_f32_ph1(nc, flux, A, P, st) : launch_stage_f32_ph0(nc, flux, A, P, st);
     return ph ? launch_stage_f64_ph1(nc, flux, A, P, st) : launch_stage_f64_ph0(nc, flux, A, P, st);
 }

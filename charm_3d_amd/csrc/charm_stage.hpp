@@ -20,6 +20,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 #include "charm_device.hpp"
 #include "charm_kernels.h"
 
@@ -53,6 +54,14 @@ __device__ __forceinline__ void load_elem(const R *__restrict__ U, int64_t n, in
 {
     #pragma unroll
     for (int i = 0; i < 4 * NV; i++) u[i] = (U + i * n)[e];
+}
+
+// load a state stored in S into the arithmetic type R
+template <typename S, typename R, int NV>
+__device__ __forceinline__ void load_conv(const S *__restrict__ U, int64_t n, int e, R *u)
+{
+    #pragma unroll
+    for (int i = 0; i < 4 * NV; i++) u[i] = (R) (U + i * n)[e];
 }
 
 template <typename R, int NV>
@@ -145,16 +154,16 @@ __device__ __forceinline__ bool face_core(const DevParams &P, const R *u0, const
     return ok;
 }
 
-template <typename R, int NC, int FLUX, int AX, int PH>
-__device__ __forceinline__ bool face_moments(const DevParams &P, const R *__restrict__ U, int64_t n,
+template <typename S, typename R, int NC, int FLUX, int AX, int PH>
+__device__ __forceinline__ bool face_moments(const DevParams &P, const S *__restrict__ U, int64_t n,
                                              const uint8_t *__restrict__ level, int4 rec, R *out)
 {
     constexpr int NV = 4 + NC;
     const int kind = (rec.z >> 3) & 3;
     const int lev0 = level[rec.x];
     R u0[4 * NV], u1[4 * NV];
-    load_elem<R, NV>(U, n, rec.x, u0);
-    load_elem<R, NV>(U, n, rec.y, u1);
+    load_conv<S, R, NV>(U, n, rec.x, u0);
+    load_conv<S, R, NV>(U, n, rec.y, u1);
     return face_core<R, NC, FLUX, AX, false, PH>(P, u0, u1, rec.z, kind == FACE_HANG1 ? lev0 + 1 : lev0, out, AX);
 }
 
@@ -162,14 +171,14 @@ __device__ __forceinline__ bool face_moments(const DevParams &P, const R *__rest
 // of its face (Taylor slopes and momentum components permuted), evaluated
 // with the x-normal code, and the momentum moments are rotated back; one
 // code path serves the three axes.
-template <typename R, int NC, int FLUX, int PH>
-__device__ __forceinline__ bool face_moments_bnd(const DevParams &P, const R *__restrict__ U, int64_t n,
+template <typename S, typename R, int NC, int FLUX, int PH>
+__device__ __forceinline__ bool face_moments_bnd(const DevParams &P, const S *__restrict__ U, int64_t n,
                                                  const uint8_t *__restrict__ level, int4 rec, R *out)
 {
     constexpr int NV = 4 + NC;
     const int ax = (rec.z & 7) >> 1;
     R u[4 * NV], w[4 * NV];
-    load_elem<R, NV>(U, n, rec.x, u);
+    load_conv<S, R, NV>(U, n, rec.x, u);
     #pragma unroll
     for (int v = 0; v < NV; v++) {
         // source variable: momentum components follow the axis permutation
@@ -258,9 +267,12 @@ __device__ __forceinline__ void volume_integral(const DevParams &P, const R *u, 
     }
 }
 
-// PH = 0: constant-cp Euler terms; PH = 1: any cp (FP64 Newton T) and the
-// viscous integrals of the Navier-Stokes model added before A^-1
-template <typename R, int NC, int FLUX, int BS, int PH>
+// S: storage type of the state; R: arithmetic type of the residual (S = R,
+// or the mixed mode S = double, R = float: FP32 fluxes and integrals, FP64
+// state, update and combine).  PH = 0: constant-cp Euler terms; PH = 1: any
+// cp (FP64 Newton T) and the viscous integrals of the Navier-Stokes model
+// added before A^-1.
+template <typename S, typename R, int NC, int FLUX, int BS, int PH>
 __global__ __launch_bounds__(BS)
 void k_stage(StageArgs A, DevParams P)
 {
@@ -268,7 +280,7 @@ void k_stage(StageArgs A, DevParams P)
     extern __shared__ __align__(16) unsigned char smem_raw[];
     R *mom = reinterpret_cast<R *>(smem_raw);                       // [maxf][3][NV]
     int *finfo = reinterpret_cast<int *>(smem_raw + (size_t) A.max_faces * 3 * NV * sizeof(R));
-    const R *__restrict__ U = reinterpret_cast<const R *>(A.U);
+    const S *__restrict__ U = reinterpret_cast<const S *>(A.U);
     const int c = xcd_block(A.n_xcd);
     if (c >= A.n_chunks) return;                 // whole workgroup exits together
     const int fb = A.ch_fbeg[c], fe = A.ch_fbeg[c + 1];
@@ -281,13 +293,13 @@ void k_stage(StageArgs A, DevParams P)
         // uniform in the branch taken here
         const int ax = (rec.z & 7) >> 1;
         if (((rec.z >> 3) & 3) == FACE_BND) {
-            ok &= face_moments_bnd<R, NC, FLUX, PH>(P, U, A.n_elem, A.level, rec, dst);
+            ok &= face_moments_bnd<S, R, NC, FLUX, PH>(P, U, A.n_elem, A.level, rec, dst);
         } else if (ax == 0) {
-            ok &= face_moments<R, NC, FLUX, 0, PH>(P, U, A.n_elem, A.level, rec, dst);
+            ok &= face_moments<S, R, NC, FLUX, 0, PH>(P, U, A.n_elem, A.level, rec, dst);
         } else if (ax == 1) {
-            ok &= face_moments<R, NC, FLUX, 1, PH>(P, U, A.n_elem, A.level, rec, dst);
+            ok &= face_moments<S, R, NC, FLUX, 1, PH>(P, U, A.n_elem, A.level, rec, dst);
         } else {
-            ok &= face_moments<R, NC, FLUX, 2, PH>(P, U, A.n_elem, A.level, rec, dst);
+            ok &= face_moments<S, R, NC, FLUX, 2, PH>(P, U, A.n_elem, A.level, rec, dst);
         }
         finfo[f - fb] = rec.z;
     }
@@ -295,15 +307,15 @@ void k_stage(StageArgs A, DevParams P)
     const int e = eb + (int) threadIdx.x;
     if (e < ee) {
         R u[4 * NV], res[4 * NV];
-        load_elem<R, NV>(U, A.n_elem, e, u);
+        load_conv<S, R, NV>(U, A.n_elem, e, u);
         const int lev = A.level[e];
         const R h = ldexp(R(1), -lev);
         volume_integral<R, NC, PH>(P, u, h, res);
         if constexpr (PH == 1) {
             if (A.visc) {
-                const R *__restrict__ VI = reinterpret_cast<const R *>(A.visc);
+                const S *__restrict__ VI = reinterpret_cast<const S *>(A.visc);
                 #pragma unroll
-                for (int i = 0; i < 16; i++) res[i] += (VI + i * A.n_elem)[e];
+                for (int i = 0; i < 16; i++) res[i] += (R) (VI + i * A.n_elem)[e];
             }
         }
         const int s_end = A.slot_ptr[e + 1];
@@ -331,43 +343,46 @@ void k_stage(StageArgs A, DevParams P)
                 res[4*v+3] += cz;
             }
         }
-        R *out = reinterpret_cast<R *>(A.out);
+        S *out = reinterpret_cast<S *>(A.out);
         const int64_t n = A.n_elem;
         if (A.mode == 0) {
             #pragma unroll
-            for (int i = 0; i < 4 * NV; i++) (out + i * n)[e] = res[i];
+            for (int i = 0; i < 4 * NV; i++) (out + i * n)[e] = (S) res[i];
         } else {
             // c -= dt * A^-1 int (src/models/euler/charm_model.c:402-433); A^-1
             // of a level-l cube is the FP64 level-0 inverse times 8^l (exact
             // power-of-two scaling), diagonal on the cube octree; then the
             // SSP-RK combine when no limiter sits in between
-            const R sc = ldexp(R(1), 3 * lev);
-            const R dt = R(A.dt);
-            const R *__restrict__ U0 = reinterpret_cast<const R *>(A.U0);
+            // (mixed mode: the stored FP64 coefficient is re-read, the FP32
+            // integral widened, and the update and combine run in FP64)
+            const S sc = ldexp(S(1), 3 * lev);
+            const S dt = S(A.dt);
+            const S *__restrict__ U0 = reinterpret_cast<const S *>(A.U0);
             #pragma unroll
             for (int v = 0; v < NV; v++)
                 #pragma unroll
                 for (int k = 0; k < 4; k++) {
-                    const R ai = cst<R>(P.ainv0[k], P.ainv0_f[k]) * sc;
-                    const R w = u[4*v+k] - dt * (ai * res[4*v+k]);
-                    (out + (4*v+k) * n)[e] = A.combine ? rk_combine<R>(A.combine, w, (U0 + (4*v+k) * n)[e]) : w;
+                    const S ai = cst<S>(P.ainv0[k], P.ainv0_f[k]) * sc;
+                    const S uk = std::is_same<S, R>::value ? (S) u[4*v+k] : (U + (4*v+k) * n)[e];
+                    const S w = uk - dt * (ai * (S) res[4*v+k]);
+                    (out + (4*v+k) * n)[e] = A.combine ? rk_combine<S>(A.combine, w, (U0 + (4*v+k) * n)[e]) : w;
                 }
         }
     }
     if (!ok) A.err[0] = 1;
 }
 
-template <typename R, int NC, int BS, int PH>
+template <typename S, typename R, int NC, int BS, int PH>
 static hipError_t launch_stage_bs(int flux, const StageArgs &A, const DevParams &P, hipStream_t st)
 {
     constexpr int NV = 4 + NC;
     const size_t sh = (size_t) A.max_faces * (3 * NV * sizeof(R) + sizeof(int));
     const dim3 grid((unsigned) (((A.n_chunks + A.n_xcd - 1) / A.n_xcd) * A.n_xcd)), block(BS);
     switch (flux) {
-    case FLUX_LF: hipLaunchKernelGGL((k_stage<R, NC, FLUX_LF, BS, PH>), grid, block, sh, st, A, P); break;
-    case FLUX_HLLC: hipLaunchKernelGGL((k_stage<R, NC, FLUX_HLLC, BS, PH>), grid, block, sh, st, A, P); break;
-    case FLUX_GODUNOV: hipLaunchKernelGGL((k_stage<R, NC, FLUX_GODUNOV, BS, PH>), grid, block, sh, st, A, P); break;
-    case FLUX_CD: hipLaunchKernelGGL((k_stage<R, NC, FLUX_CD, BS, PH>), grid, block, sh, st, A, P); break;
+    case FLUX_LF: hipLaunchKernelGGL((k_stage<S, R, NC, FLUX_LF, BS, PH>), grid, block, sh, st, A, P); break;
+    case FLUX_HLLC: hipLaunchKernelGGL((k_stage<S, R, NC, FLUX_HLLC, BS, PH>), grid, block, sh, st, A, P); break;
+    case FLUX_GODUNOV: hipLaunchKernelGGL((k_stage<S, R, NC, FLUX_GODUNOV, BS, PH>), grid, block, sh, st, A, P); break;
+    case FLUX_CD: hipLaunchKernelGGL((k_stage<S, R, NC, FLUX_CD, BS, PH>), grid, block, sh, st, A, P); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -375,25 +390,25 @@ static hipError_t launch_stage_bs(int flux, const StageArgs &A, const DevParams 
 
 // one thread per chunk element: 128- or 256-element chunks (the
 // Navier-Stokes variant always runs 256-thread workgroups)
-template <typename R, int PH>
+template <typename S, typename R, int PH>
 hipError_t launch_stage_t(int nc, int flux, const StageArgs &A, const DevParams &P, hipStream_t st)
 {
     if (A.chunk_elems > 256) return hipErrorInvalidValue;
     if constexpr (PH == 1) {
         switch (nc) {
-        case 1: return launch_stage_bs<R, 1, 256, PH>(flux, A, P, st);
-        case 2: return launch_stage_bs<R, 2, 256, PH>(flux, A, P, st);
-        case 3: return launch_stage_bs<R, 3, 256, PH>(flux, A, P, st);
-        case 4: return launch_stage_bs<R, 4, 256, PH>(flux, A, P, st);
+        case 1: return launch_stage_bs<S, R, 1, 256, PH>(flux, A, P, st);
+        case 2: return launch_stage_bs<S, R, 2, 256, PH>(flux, A, P, st);
+        case 3: return launch_stage_bs<S, R, 3, 256, PH>(flux, A, P, st);
+        case 4: return launch_stage_bs<S, R, 4, 256, PH>(flux, A, P, st);
         }
         return hipErrorInvalidValue;
     } else {
         const bool wide = A.chunk_elems > 128;
         switch (nc) {
-        case 1: return wide ? launch_stage_bs<R, 1, 256, PH>(flux, A, P, st) : launch_stage_bs<R, 1, 128, PH>(flux, A, P, st);
-        case 2: return wide ? launch_stage_bs<R, 2, 256, PH>(flux, A, P, st) : launch_stage_bs<R, 2, 128, PH>(flux, A, P, st);
-        case 3: return wide ? launch_stage_bs<R, 3, 256, PH>(flux, A, P, st) : launch_stage_bs<R, 3, 128, PH>(flux, A, P, st);
-        case 4: return wide ? launch_stage_bs<R, 4, 256, PH>(flux, A, P, st) : launch_stage_bs<R, 4, 128, PH>(flux, A, P, st);
+        case 1: return wide ? launch_stage_bs<S, R, 1, 256, PH>(flux, A, P, st) : launch_stage_bs<S, R, 1, 128, PH>(flux, A, P, st);
+        case 2: return wide ? launch_stage_bs<S, R, 2, 256, PH>(flux, A, P, st) : launch_stage_bs<S, R, 2, 128, PH>(flux, A, P, st);
+        case 3: return wide ? launch_stage_bs<S, R, 3, 256, PH>(flux, A, P, st) : launch_stage_bs<S, R, 3, 128, PH>(flux, A, P, st);
+        case 4: return wide ? launch_stage_bs<S, R, 4, 256, PH>(flux, A, P, st) : launch_stage_bs<S, R, 4, 128, PH>(flux, A, P, st);
         }
         return hipErrorInvalidValue;
     }

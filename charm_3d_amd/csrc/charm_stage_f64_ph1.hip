@@ -5,7 +5,7 @@ namespace charm {
 
 hipError_t launch_stage_f64_ph1(int nc, int flux, const StageArgs &A, const DevParams &P, hipStream_t st)
 {
-    return launch_stage_t<double, 1>(nc, flux, A, P, st);
+    return launch_stage_t<double, double, 1>(nc, flux, A, P, st);
 }
 
 } // namespace charm

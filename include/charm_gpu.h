@@ -44,8 +44,10 @@ enum {
     CHARM_ERR_UNSUPPORTED = -6
 };
 
-/* precision of the bulk arithmetic */
-enum { CHARM_FP64 = 0, CHARM_FP32 = 1 };
+/* precision of the bulk arithmetic: FP64 throughout, FP32 throughout, or
+ * mixed (FP64 state, update, RK combine and limiter; FP32 fluxes and
+ * integrals; constant-cp Euler terms) */
+enum { CHARM_FP64 = 0, CHARM_FP32 = 1, CHARM_FP32_MIX = 2 };
 /* FLUX_TYPE (src/charm_init_yaml.cpp:352-379) */
 enum { CHARM_FLUX_LF = 0, CHARM_FLUX_HLLC = 1, CHARM_FLUX_GODUNOV = 2, CHARM_FLUX_CD = 3 };
 /* boundary types (include/charm_globals.h:432-439) plus periodic links */

@@ -226,3 +226,37 @@ def test_reference_entry_points_drive_the_device():
     ref = device(cfg, leaves, periodic, u0)
     ref.run(dt, 3)
     assert np.array_equal(got, ref.get_state())
+
+
+def test_mixed_precision_residual_is_the_fp32_residual():
+    """FP64 state, FP32 residual arithmetic: the residual equals the FP32
+    path's on the same (FP32-rounded) state."""
+    rng = np.random.default_rng(9)
+    leaves = MU.random_adapted(2, 4, seed=9, frac=0.3)
+    u0 = random_state(leaves, rng)
+    r32 = device(CA.make_config(precision=CA.FP32), leaves, (False,) * 3, u0).residual()
+    rmx = device(CA.make_config(precision=CA.FP32_MIX), leaves, (False,) * 3, u0).residual()
+    assert np.array_equal(r32, rmx)
+
+
+@pytest.mark.parametrize("case", ["riemann_box", "sine_wave"])
+def test_reduced_precision_steps_close_to_fp64(case):
+    """FP32 and mixed (FP64 state / update / limiter, FP32 residual) paths
+    against the FP64 path: the Riemann box (level 4, 20 steps, HLLC + BJ)
+    and the smooth advected sine wave (level 5, 50 steps, LF, no limiter),
+    relative L2 over all coefficients."""
+    mk = (lambda p: W.c2(level=4, precision=p)) if case == "riemann_box" else (lambda p: W.c1(level=5, precision=p))
+    steps = 20 if case == "riemann_box" else 50
+    cfg, leaves, u0, periodic = mk(CA.FP64)
+    out = {}
+    for prec in (CA.FP64, CA.FP32_MIX, CA.FP32):
+        c, _, _, _ = mk(prec)
+        ctx = device(c, leaves, periodic, u0)
+        dt = ctx.get_dt(1)
+        ctx.run(dt, steps)
+        out[prec] = ctx.get_state()
+    ref = out[CA.FP64]
+    err = {p: np.linalg.norm(out[p] - ref) / np.linalg.norm(ref) for p in (CA.FP32_MIX, CA.FP32)}
+    print("%s L2 vs FP64: mixed %.2e, fp32 %.2e" % (case, err[CA.FP32_MIX], err[CA.FP32]))
+    assert err[CA.FP32_MIX] <= err[CA.FP32]
+    assert err[CA.FP32_MIX] < (1e-5 if case == "riemann_box" else 1e-6)
