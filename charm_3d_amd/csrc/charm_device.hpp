@@ -63,6 +63,14 @@ template <typename R> __device__ __forceinline__ R dv(R a, R b) { return a / b; 
 template <> __device__ __forceinline__ float dv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 template <typename R> __device__ __forceinline__ R dsqrt(R x) { return sqrt(x); }
 template <> __device__ __forceinline__ float dsqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
+// clamp to [0, 1] (mass fractions); x < 0 -> 0, x > 1 -> 1 as two compares,
+// one median-of-three on the FP32 path
+template <typename R> __device__ __forceinline__ R clamp01(R x)
+{
+    x = x < R(0) ? R(0) : x;
+    return x > R(1) ? R(1) : x;
+}
+template <> __device__ __forceinline__ float clamp01<float>(float x) { return __builtin_amdgcn_fmed3f(x, 0.f, 1.f); }
 
 // cp of species i at T, charm_comp_calc_cp (src/charm_globals.c:603-625):
 // the power series when cp_poly (cp_type polynomial with ml_type 1, as the
@@ -191,25 +199,22 @@ __device__ __forceinline__ R gp_t2(int k)
 template <typename R, int NC, int PH = 0>
 __device__ __forceinline__ void cons_to_prim(const DevParams &P, const R *u, Prim<R, NC> &p)
 {
-    R r = R(0);
+    R r = u[4];
     #pragma unroll
-    for (int i = 0; i < NC; i++) r += u[4 + i];
+    for (int i = 1; i < NC; i++) r += u[4 + i];
     const R ri = rcp(r);
     p.u = u[0] * ri; p.v = u[1] * ri; p.w = u[2] * ri;
     p.e_tot = u[3] * ri;
     p.e = p.e_tot - R(0.5) * (p.u * p.u + p.v * p.v + p.w * p.w);
     #pragma unroll
-    for (int i = 0; i < NC; i++) {
-        R c = u[4 + i] * ri;
-        c = c < R(0) ? R(0) : c;
-        c = c > R(1) ? R(1) : c;
-        p.c[i] = c;
-    }
+    for (int i = 0; i < NC; i++) p.c[i] = clamp01(u[4 + i] * ri);
     eos_coeffs<R, NC, PH>(P, p);
     if (r < R(kEps)) r = R(kEps);
     p.r = r;
-    p.p = r * p.e * (p.gam - R(1));
-    p.cz = dsqrt(p.gam * p.p * rcp(r));
+    const R gm1 = p.gam - R(1);
+    p.p = r * p.e * gm1;
+    // c^2 = gamma p / r = gamma (gamma - 1) e with the same (floored) r
+    p.cz = dsqrt(p.gam * gm1 * p.e);
 }
 
 // Exterior state of a boundary face, src/charm_bnd_cond.c:21-93: the BC

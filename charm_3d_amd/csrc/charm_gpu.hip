@@ -512,16 +512,60 @@ extern "C" int charm_gpu_run(charm_gpu_ctx_t *c, double dt, int nsteps)
     return charm_gpu_step(c, c->cfg.model, dt, nsteps);
 }
 
+// physics variant of a configuration: 1 when any cp is not constant, or
+// viscosity or reactions are on (the Navier-Stokes stage kernels)
+static int config_physics(const charm_config_t *cfg)
+{
+    int ph = cfg->comp[0].cp_type != 0 || cfg->use_visc != 0 || cfg->n_react > 0;
+    for (int i = 0; i < cfg->n_comp; i++) ph |= cfg->comp[i].cp_type != 0 && cfg->comp[i].ml_type == 1;
+    return ph ? 1 : 0;
+}
+
+// Workgroups per CU that the stage kernel's registers allow (4 SIMDs of 512
+// VGPRs per lane, at most 8 waves each), or 0 when it cannot be queried
+static int stage_groups_per_cu(const charm_config_t *cfg, int bs)
+{
+    hipFuncAttributes at;
+    const void *fn = stage_kernel(cfg->precision, cfg->n_comp, cfg->flux, config_physics(cfg), bs);
+    if (!fn || hipFuncGetAttributes(&at, fn) != hipSuccess || at.numRegs <= 0) {
+        (void) hipGetLastError();
+        return 0;
+    }
+    int waves = 512 / (((at.numRegs + 7) / 8) * 8);
+    if (waves > 8) waves = 8;
+    return waves * 4 / (bs / 64);
+}
+
 extern "C" int charm_gpu_recommended_chunks(const charm_config_t *cfg, int *ce, int *cf)
 {
     if (!cfg || !ce || !cf || cfg->n_comp < 1 || cfg->n_comp > 4) {
         charm_set_error("recommended_chunks: bad arguments");
         return CHARM_ERR_ARG;
     }
-    // about three faces per element on a uniform octree; keep the LDS of a
-    // workgroup near 48 KB so that three workgroups share a CU
-    int faces = 768;
-    while (faces > 96 && stage_smem_bytes(cfg->precision, cfg->n_comp, faces) > 49152) faces -= faces / 4;
+    // Chunk LDS sized to the occupancy the kernel's registers allow: the
+    // CU's LDS divided among that many workgroups (larger chunks evaluate
+    // fewer faces twice; more workgroups than the registers allow gain
+    // nothing).  Without a device to query: 40 KB, four workgroups of 160 KB.
+    size_t lds_cu = 160 * 1024;
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+        prop.maxSharedMemoryPerMultiProcessor > 0)
+        lds_cu = prop.maxSharedMemoryPerMultiProcessor;
+    else
+        (void) hipGetLastError();
+    const size_t per_face = stage_smem_bytes(cfg->precision, cfg->n_comp, 1);
+    int faces = 0;
+    for (int bs = 256; bs >= 128; bs -= 128) {
+        const int groups = stage_groups_per_cu(cfg, bs);
+        size_t budget = groups > 0 ? lds_cu / groups : 40960;
+        if (budget > 65536) budget = 65536;
+        faces = (int) (budget / per_face);
+        if (faces > 1024) faces = 1024;
+        // 128-thread workgroups serve chunks of at most 128 elements
+        if (bs == 128 || faces / 3 > 128 || config_physics(cfg)) break;
+    }
+    if (faces < 96) faces = 96;
     *cf = faces;
     *ce = faces / 3 > 256 ? 256 : faces / 3;
     return CHARM_OK;

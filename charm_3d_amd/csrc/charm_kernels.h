@@ -124,9 +124,17 @@ hipError_t launch_stage_f32_ph1(int nc, int flux, const StageArgs &A, const DevP
 hipError_t launch_stage_f64_ph0(int nc, int flux, const StageArgs &A, const DevParams &P, hipStream_t st);
 hipError_t launch_stage_f64_ph1(int nc, int flux, const StageArgs &A, const DevParams &P, hipStream_t st);
 hipError_t launch_stage_mix_ph0(int nc, int flux, const StageArgs &A, const DevParams &P, hipStream_t st);
+const void *stage_fn_f32_ph0(int nc, int flux, int bs);
+const void *stage_fn_f32_ph1(int nc, int flux, int bs);
+const void *stage_fn_f64_ph0(int nc, int flux, int bs);
+const void *stage_fn_f64_ph1(int nc, int flux, int bs);
+const void *stage_fn_mix_ph0(int nc, int flux, int bs);
 // prec: 0 FP64, 1 FP32, 2 mixed (FP64 state, FP32 residual)
 hipError_t launch_stage(int prec, int nc, int flux, int ph, const StageArgs &A, const DevParams &P, hipStream_t st);
 size_t stage_smem_bytes(int prec, int nc, int max_faces);
+// host address of the stage kernel a launch with these parameters uses
+// (bs = the workgroup size: 128 or 256)
+const void *stage_kernel(int prec, int nc, int flux, int ph, int bs);
 hipError_t launch_limiter(int prec, int nc, int ph, const LimArgs &A, const DevParams &P, hipStream_t st);
 hipError_t launch_reduce(int prec, int nc, int ph, const RedArgs &A, const DevParams &P, hipStream_t st);
 hipError_t launch_flux_points(int prec, int nc, int flux, const FluxArgs &A, const DevParams &P, hipStream_t st);

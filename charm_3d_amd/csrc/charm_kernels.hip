@@ -26,73 +26,96 @@ void k_limiter(LimArgs A, DevParams P)
     if (e >= A.n) return;
     const R *__restrict__ W = reinterpret_cast<const R *>(A.W);
     const int64_t n = A.n;
-    R u[4 * NV];
+    R u[4 * NV], u_old[4 * NV];
     load_elem<R, NV>(W, n, e, u);
+    // the RK combine's old state is fetched up front with the rest
+    if (A.combine) load_elem<R, NV>(reinterpret_cast<const R *>(A.U0), n, e, u_old);
     R umin[NV], umax[NV];
     #pragma unroll
     for (int v = 0; v < NV; v++) { umin[v] = u[4*v]; umax[v] = u[4*v]; }
     if (A.limit) {
+        // all six neighbour indices, then all six neighbour means, are put
+        // in flight together; the rare boundary / coarse-side neighbours
+        // (negative codes) are replaced afterwards
+        int nb[6];
+        {
+            // 24 bytes per element: three 8-byte loads (the array is
+            // 256-byte aligned)
+            const int2 *q = reinterpret_cast<const int2 *>(A.nbr + 6 * (int64_t) e);
+            const int2 a = q[0], b = q[1], c = q[2];
+            nb[0] = a.x; nb[1] = a.y; nb[2] = b.x; nb[3] = b.y; nb[4] = c.x; nb[5] = c.y;
+        }
+        R m[6][NV];
+        #pragma unroll
         for (int d = 0; d < 6; d++) {
-            const int nb = A.nbr[6 * e + d];
-            R m[NV];
-            if (nb >= 0) {
-                #pragma unroll
-                for (int v = 0; v < NV; v++) m[v] = (W + 4 * v * n)[nb];
-            } else if (nb > -8) {
+            const int j = nb[d] >= 0 ? nb[d] : e;
+            #pragma unroll
+            for (int v = 0; v < NV; v++) m[d][v] = (W + 4 * v * n)[j];
+        }
+        #pragma unroll
+        for (int d = 0; d < 6; d++) {
+            if (nb[d] >= 0) continue;
+            if (nb[d] > -8) {
                 // boundary: mean -> prim -> BC -> cons (charm_limiter_bj.c:79-91)
                 R t[NV];
                 #pragma unroll
                 for (int v = 0; v < NV; v++) t[v] = u[4*v];
                 Prim<R, NC> pi, po;
                 cons_to_prim<R, NC, PH>(P, t, pi);
-                const int face = -1 - nb, a = face >> 1;
+                const int face = -1 - nb[d], a = face >> 1;
                 bnd_state<R, NC, PH>(P, face, a, (face & 1) ? R(1) : R(-1), pi, po, -1);
-                m[0] = po.r * po.u; m[1] = po.r * po.v; m[2] = po.r * po.w;
-                m[3] = po.r * (po.e + R(0.5) * (po.u * po.u + po.v * po.v + po.w * po.w));
+                m[d][0] = po.r * po.u; m[d][1] = po.r * po.v; m[d][2] = po.r * po.w;
+                m[d][3] = po.r * (po.e + R(0.5) * (po.u * po.u + po.v * po.v + po.w * po.w));
                 #pragma unroll
-                for (int i = 0; i < NC; i++) m[4 + i] = po.r * po.c[i];
+                for (int i = 0; i < NC; i++) m[d][4 + i] = po.r * po.c[i];
             } else {
-                const int4 g = A.aux[-8 - nb];
+                const int4 g = A.aux[-8 - nb[d]];
                 const int q[4] = {g.x, g.y, g.z, g.w};
                 #pragma unroll
-                for (int v = 0; v < NV; v++) m[v] = R(0);
+                for (int v = 0; v < NV; v++) m[d][v] = R(0);
                 for (int j = 0; j < 4; j++)
                     #pragma unroll
-                    for (int v = 0; v < NV; v++) m[v] += (W + 4 * v * n)[q[j]];
+                    for (int v = 0; v < NV; v++) m[d][v] += (W + 4 * v * n)[q[j]];
                 #pragma unroll
-                for (int v = 0; v < NV; v++) m[v] *= R(0.25);
-            }
-            #pragma unroll
-            for (int v = 0; v < NV; v++) {
-                umin[v] = m[v] < umin[v] ? m[v] : umin[v];
-                umax[v] = m[v] > umax[v] ? m[v] : umax[v];
+                for (int v = 0; v < NV; v++) m[d][v] *= R(0.25);
             }
         }
         #pragma unroll
+        for (int d = 0; d < 6; d++)
+            #pragma unroll
+            for (int v = 0; v < NV; v++) {
+                umin[v] = m[d][v] < umin[v] ? m[d][v] : umin[v];
+                umax[v] = m[d][v] > umax[v] ? m[d][v] : umax[v];
+            }
+        // psi = min over the 8 vertices of min(1, (umax - u0) / df) where
+        // df > eps and min(1, (umin - u0) / df) where df < -eps.  The
+        // numerators have fixed signs (umin <= u0 <= umax), so the minimum
+        // over vertices is the quotient by the largest |df| on each side:
+        // two divisions per variable instead of eight, the same value
+        // (division is monotone in the divisor).
+        #pragma unroll
         for (int v = 0; v < NV; v++) {
             const R u0 = u[4*v];
-            R psi = R(1);
+            R dpos = R(0), dneg = R(0);
             #pragma unroll
             for (int vt = 0; vt < 8; vt++) {
                 const R f = u0 + ((vt & 1) ? R(0.5) : R(-0.5)) * u[4*v+1]
                                + ((vt & 2) ? R(0.5) : R(-0.5)) * u[4*v+2]
                                + ((vt & 4) ? R(0.5) : R(-0.5)) * u[4*v+3];
                 const R df = f - u0;
-                R pt = R(1);
-                if (fabs(df) > R(kEps)) {
-                    if (df > R(0)) pt = fmin(R(1), dv(umax[v] - u0, df));
-                    else if (df < R(0)) pt = fmin(R(1), dv(umin[v] - u0, df));
-                }
-                psi = pt < psi ? pt : psi;
+                dpos = (df > R(kEps) && df > dpos) ? df : dpos;
+                dneg = (-df > R(kEps) && -df > dneg) ? -df : dneg;
             }
+            R psi = R(1);
+            if (dpos > R(0)) psi = fmin(psi, dv(umax[v] - u0, dpos));
+            if (dneg > R(0)) psi = fmin(psi, dv(u0 - umin[v], dneg));
             u[4*v+1] *= psi; u[4*v+2] *= psi; u[4*v+3] *= psi;
         }
     }
     R *out = reinterpret_cast<R *>(A.out);
     if (A.combine) {
-        const R *__restrict__ U0 = reinterpret_cast<const R *>(A.U0);
         #pragma unroll
-        for (int i = 0; i < 4 * NV; i++) (out + i * n)[e] = rk_combine<R>(A.combine, u[i], (U0 + i * n)[e]);
+        for (int i = 0; i < 4 * NV; i++) (out + i * n)[e] = rk_combine<R>(A.combine, u[i], u_old[i]);
     } else {
         #pragma unroll
         for (int i = 0; i < 4 * NV; i++) (out + i * n)[e] = u[i];
@@ -257,6 +280,13 @@ hipError_t launch_stage(int prec, int nc, int flux, int ph, const StageArgs &A, 
     if (prec == 2) return ph ? hipErrorInvalidValue : launch_stage_mix_ph0(nc, flux, A, P, st);
     if (prec) return ph ? launch_stage_f32_ph1(nc, flux, A, P, st) : launch_stage_f32_ph0(nc, flux, A, P, st);
     return ph ? launch_stage_f64_ph1(nc, flux, A, P, st) : launch_stage_f64_ph0(nc, flux, A, P, st);
+}
+
+const void *stage_kernel(int prec, int nc, int flux, int ph, int bs)
+{
+    if (prec == 2) return ph ? nullptr : stage_fn_mix_ph0(nc, flux, bs);
+    if (prec) return ph ? stage_fn_f32_ph1(nc, flux, bs) : stage_fn_f32_ph0(nc, flux, bs);
+    return ph ? stage_fn_f64_ph1(nc, flux, bs) : stage_fn_f64_ph0(nc, flux, bs);
 }
 
 size_t stage_smem_bytes(int prec, int nc, int max_faces)

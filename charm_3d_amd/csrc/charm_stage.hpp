@@ -106,20 +106,29 @@ __device__ __forceinline__ bool face_core(const DevParams &P, const R *u0, const
     const R o01 = big0 ? d1 : R(0), o02 = big0 ? d2 : R(0);
     const R o11 = big1 ? d1 : R(0), o12 = big1 ? d2 : R(0);
     R S0[NV], S1[NV], S2[NV];
+    // the normal-direction part of each trace is the same at all six points
+    R c0[NV], c1[NV];
+    #pragma unroll
+    for (int v = 0; v < NV; v++) {
+        c0[v] = u0[4*v] + pa0 * u0[4*v+1+AX];
+        c1[v] = BND ? R(0) : u1[4*v] + pa1 * u1[4*v+1+AX];
+    }
     bool ok = true;
     #pragma unroll
     for (int k = 0; k < 6; k++) {
         const R g1 = gp_t1<R>(k), g2 = gp_t2<R>(k);
-        R ph[3], t[NV];
-        ph[AX] = pa0; ph[T1] = o01 + m0 * g1; ph[T2] = o02 + m0 * g2;
-        trace<R, NV>(u0, ph[0], ph[1], ph[2], t);
+        R t[NV];
+        #pragma unroll
+        for (int v = 0; v < NV; v++)
+            t[v] = c0[v] + (o01 + m0 * g1) * u0[4*v+1+T1] + (o02 + m0 * g2) * u0[4*v+1+T2];
         Prim<R, NC> pl, pr;
         cons_to_prim<R, NC, PH>(P, t, pl);
         if (BND) {
             bnd_state<R, NC, PH>(P, dir0, AX, s, pl, pr, bnd_axis);
         } else {
-            ph[AX] = pa1; ph[T1] = o11 + m1 * g1; ph[T2] = o12 + m1 * g2;
-            trace<R, NV>(u1, ph[0], ph[1], ph[2], t);
+            #pragma unroll
+            for (int v = 0; v < NV; v++)
+                t[v] = c1[v] + (o11 + m1 * g1) * u1[4*v+1+T1] + (o12 + m1 * g2) * u1[4*v+1+T2];
             cons_to_prim<R, NC, PH>(P, t, pr);
         }
         // face frame (s e_AX, e_T1, e_T2)
@@ -154,31 +163,26 @@ __device__ __forceinline__ bool face_core(const DevParams &P, const R *u0, const
     return ok;
 }
 
-template <typename S, typename R, int NC, int FLUX, int AX, int PH>
-__device__ __forceinline__ bool face_moments(const DevParams &P, const S *__restrict__ U, int64_t n,
-                                             const uint8_t *__restrict__ level, int4 rec, R *out)
+// interior face from the two preloaded element states; the small side's
+// level sets the face size
+template <typename R, int NC, int FLUX, int AX, int PH>
+__device__ __forceinline__ bool face_moments(const DevParams &P, const R *u0, const R *u1, int info,
+                                             int lev0, R *out)
 {
-    constexpr int NV = 4 + NC;
-    const int kind = (rec.z >> 3) & 3;
-    const int lev0 = level[rec.x];
-    R u0[4 * NV], u1[4 * NV];
-    load_conv<S, R, NV>(U, n, rec.x, u0);
-    load_conv<S, R, NV>(U, n, rec.y, u1);
-    return face_core<R, NC, FLUX, AX, false, PH>(P, u0, u1, rec.z, kind == FACE_HANG1 ? lev0 + 1 : lev0, out, AX);
+    const int kind = (info >> 3) & 3;
+    return face_core<R, NC, FLUX, AX, false, PH>(P, u0, u1, info, kind == FACE_HANG1 ? lev0 + 1 : lev0, out, AX);
 }
 
 // Boundary faces: the element state is rotated into the frame (a, t1, t2)
 // of its face (Taylor slopes and momentum components permuted), evaluated
 // with the x-normal code, and the momentum moments are rotated back; one
 // code path serves the three axes.
-template <typename S, typename R, int NC, int FLUX, int PH>
-__device__ __forceinline__ bool face_moments_bnd(const DevParams &P, const S *__restrict__ U, int64_t n,
-                                                 const uint8_t *__restrict__ level, int4 rec, R *out)
+template <typename R, int NC, int FLUX, int PH>
+__device__ __forceinline__ bool face_moments_bnd(const DevParams &P, const R *u, int info, int lev, R *out)
 {
     constexpr int NV = 4 + NC;
-    const int ax = (rec.z & 7) >> 1;
-    R u[4 * NV], w[4 * NV];
-    load_conv<S, R, NV>(U, n, rec.x, u);
+    const int ax = (info & 7) >> 1;
+    R w[4 * NV];
     #pragma unroll
     for (int v = 0; v < NV; v++) {
         // source variable: momentum components follow the axis permutation
@@ -200,7 +204,7 @@ __device__ __forceinline__ bool face_moments_bnd(const DevParams &P, const S *__
         w[4 * v] = c[0]; w[4 * v + 1] = ca; w[4 * v + 2] = ct1; w[4 * v + 3] = ct2;
     }
     R o[3 * NV];
-    const bool ok = face_core<R, NC, FLUX, 0, true, PH>(P, w, w, rec.z, level[rec.x], o, ax);
+    const bool ok = face_core<R, NC, FLUX, 0, true, PH>(P, w, w, info, lev, o, ax);
     #pragma unroll
     for (int m = 0; m < 3; m++) {
         const R f0 = o[m * NV + 0], f1 = o[m * NV + 1], f2 = o[m * NV + 2];
@@ -285,21 +289,28 @@ void k_stage(StageArgs A, DevParams P)
     if (c >= A.n_chunks) return;                 // whole workgroup exits together
     const int fb = A.ch_fbeg[c], fe = A.ch_fbeg[c + 1];
     const int eb = A.ch_ebeg[c], ee = A.ch_ebeg[c + 1];
+    const int64_t n = A.n_elem;
     bool ok = true;
-    for (int f = fb + (int) threadIdx.x; f < fe; f += blockDim.x) {
+    for (int f = fb + (int) threadIdx.x; f < fe; f += BS) {
         const int4 rec = A.cf4[f];
         R *dst = mom + (size_t) (f - fb) * 3 * NV;
+        R ua[4 * NV];
+        load_conv<S, R, NV>(U, n, rec.x, ua);
+        const int lev0 = A.level[rec.x];
         // faces of a chunk are sorted by (axis, kind): waves are mostly
         // uniform in the branch taken here
         const int ax = (rec.z & 7) >> 1;
         if (((rec.z >> 3) & 3) == FACE_BND) {
-            ok &= face_moments_bnd<S, R, NC, FLUX, PH>(P, U, A.n_elem, A.level, rec, dst);
-        } else if (ax == 0) {
-            ok &= face_moments<S, R, NC, FLUX, 0, PH>(P, U, A.n_elem, A.level, rec, dst);
-        } else if (ax == 1) {
-            ok &= face_moments<S, R, NC, FLUX, 1, PH>(P, U, A.n_elem, A.level, rec, dst);
+            ok &= face_moments_bnd<R, NC, FLUX, PH>(P, ua, rec.z, lev0, dst);
         } else {
-            ok &= face_moments<S, R, NC, FLUX, 2, PH>(P, U, A.n_elem, A.level, rec, dst);
+            R ub[4 * NV];
+            load_conv<S, R, NV>(U, n, rec.y, ub);
+            if (ax == 0)
+                ok &= face_moments<R, NC, FLUX, 0, PH>(P, ua, ub, rec.z, lev0, dst);
+            else if (ax == 1)
+                ok &= face_moments<R, NC, FLUX, 1, PH>(P, ua, ub, rec.z, lev0, dst);
+            else
+                ok &= face_moments<R, NC, FLUX, 2, PH>(P, ua, ub, rec.z, lev0, dst);
         }
         finfo[f - fb] = rec.z;
     }
@@ -307,7 +318,7 @@ void k_stage(StageArgs A, DevParams P)
     const int e = eb + (int) threadIdx.x;
     if (e < ee) {
         R u[4 * NV], res[4 * NV];
-        load_conv<S, R, NV>(U, A.n_elem, e, u);
+        load_conv<S, R, NV>(U, n, e, u);
         const int lev = A.level[e];
         const R h = ldexp(R(1), -lev);
         volume_integral<R, NC, PH>(P, u, h, res);
@@ -344,7 +355,6 @@ void k_stage(StageArgs A, DevParams P)
             }
         }
         S *out = reinterpret_cast<S *>(A.out);
-        const int64_t n = A.n_elem;
         if (A.mode == 0) {
             #pragma unroll
             for (int i = 0; i < 4 * NV; i++) (out + i * n)[e] = (S) res[i];
@@ -386,6 +396,32 @@ static hipError_t launch_stage_bs(int flux, const StageArgs &A, const DevParams 
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+// host address of one stage kernel instance (for its attributes)
+template <typename S, typename R, int NC, int BS, int PH>
+static const void *stage_fn_bs(int flux)
+{
+    switch (flux) {
+    case FLUX_LF: return reinterpret_cast<const void *>(&k_stage<S, R, NC, FLUX_LF, BS, PH>);
+    case FLUX_HLLC: return reinterpret_cast<const void *>(&k_stage<S, R, NC, FLUX_HLLC, BS, PH>);
+    case FLUX_GODUNOV: return reinterpret_cast<const void *>(&k_stage<S, R, NC, FLUX_GODUNOV, BS, PH>);
+    case FLUX_CD: return reinterpret_cast<const void *>(&k_stage<S, R, NC, FLUX_CD, BS, PH>);
+    }
+    return nullptr;
+}
+
+template <typename S, typename R, int PH>
+const void *stage_fn_t(int nc, int flux, int bs)
+{
+    const bool wide = PH == 1 || bs > 128;
+    switch (nc) {
+    case 1: return wide ? stage_fn_bs<S, R, 1, 256, PH>(flux) : stage_fn_bs<S, R, 1, 128, PH>(flux);
+    case 2: return wide ? stage_fn_bs<S, R, 2, 256, PH>(flux) : stage_fn_bs<S, R, 2, 128, PH>(flux);
+    case 3: return wide ? stage_fn_bs<S, R, 3, 256, PH>(flux) : stage_fn_bs<S, R, 3, 128, PH>(flux);
+    case 4: return wide ? stage_fn_bs<S, R, 4, 256, PH>(flux) : stage_fn_bs<S, R, 4, 128, PH>(flux);
+    }
+    return nullptr;
 }
 
 // one thread per chunk element: 128- or 256-element chunks (the

@@ -8,4 +8,9 @@ hipError_t launch_stage_f32_ph0(int nc, int flux, const StageArgs &A, const DevP
     return launch_stage_t<float, float, 0>(nc, flux, A, P, st);
 }
 
+const void *stage_fn_f32_ph0(int nc, int flux, int bs)
+{
+    return stage_fn_t<float, float, 0>(nc, flux, bs);
+}
+
 } // namespace charm

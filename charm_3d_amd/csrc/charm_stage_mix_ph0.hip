@@ -9,4 +9,9 @@ hipError_t launch_stage_mix_ph0(int nc, int flux, const StageArgs &A, const DevP
     return launch_stage_t<double, float, 0>(nc, flux, A, P, st);
 }
 
+const void *stage_fn_mix_ph0(int nc, int flux, int bs)
+{
+    return stage_fn_t<double, float, 0>(nc, flux, bs);
+}
+
 } // namespace charm
