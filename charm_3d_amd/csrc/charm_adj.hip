@@ -18,6 +18,7 @@
 #include "charm_device.hpp"
 #include "charm_dual.hpp"
 #include "charm_elemfaces.hpp"
+#include "charm_vjp.hpp"
 #include "charm_kernels.h"
 
 namespace charm {
@@ -93,12 +94,15 @@ __device__ __forceinline__ void inlet_dual(const DevParams &P, int face, const P
 
 } // namespace
 
+// Interior part of one stage adjoint: the volume term and every interior
+// face point of leaf e, each through the hand-written reverse sweep (the
+// primal at the point, then the flux and cons_to_prim VJPs for e's side).
+// Boundary faces are added by k_adj_bnd.
 template <typename R, int NC, int FLUX>
 __global__ __launch_bounds__(128)
 void k_adj_stage(AdjArgs A, DevParams P)
 {
     constexpr int NV = 4 + NC;
-    using D = Dual<R, NV>;
     const int64_t n = A.n;
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
@@ -112,74 +116,187 @@ void k_adj_stage(AdjArgs A, DevParams P)
     #pragma unroll
     for (int i = 0; i < 4 * NV; i++) ub[i] = R(0);
 
-    // volume: int[v][1+d] -= (wJ / h) sum_gp F_d,v ; gravity int[v<3][k] -= g_v rho phi_k wJ
+    // volume: s = sum_d sum_v cw_dv F_d,v(t) - sum_{v<3} g_v wJ w(ph) rho,
+    // cw_dv = -(wJ / h) w[v][1 + d]
     const R gph = R(0.28867513459481288225);
     const R wj = h * h * h * R(0.125);
+    const R f = -wj * hinv;
+    #pragma unroll 1
     for (int g = 0; g < 8; g++) {
         const R ph[3] = {(g & 1) ? gph : -gph, (g & 2) ? gph : -gph, (g & 4) ? gph : -gph};
-        D t[NV];
+        R t[NV], tb[NV];
         #pragma unroll
-        for (int v = 0; v < NV; v++) t[v] = D::var(at<R>(&u[4 * v], ph[0], ph[1], ph[2]), v);
-        Prim<D, NC> p;
-        cons_to_prim<D, NC>(P, t, p);
-        const D vel[3] = {p.u, p.v, p.w};
-        D s(0.0);
+        for (int v = 0; v < NV; v++) t[v] = at<R>(&u[4 * v], ph[0], ph[1], ph[2]);
+        Prim<R, NC> p;
+        cons_to_prim<R, NC>(P, t, p);
+        PrimBar<R, NC> pb;
+        pb.r = pb.u = pb.v = pb.w = pb.e_tot = pb.p = pb.cz = pb.gam = R(0);
+        #pragma unroll
+        for (int i = 0; i < NC; i++) pb.c[i] = R(0);
+        const R vel[3] = {p.u, p.v, p.w};
+        R velb[3] = {R(0), R(0), R(0)};
         #pragma unroll
         for (int d = 0; d < 3; d++) {
-            const D md = t[d];
-            const R c0 = -wj * hinv * w[4 * 0 + 1 + d], c1 = -wj * hinv * w[4 * 1 + 1 + d];
-            const R c2 = -wj * hinv * w[4 * 2 + 1 + d], c3 = -wj * hinv * w[4 * 3 + 1 + d];
-            s += D(c0) * (md * p.u + (d == 0 ? p.p : D(0.0)));
-            s += D(c1) * (md * p.v + (d == 1 ? p.p : D(0.0)));
-            s += D(c2) * (md * p.w + (d == 2 ? p.p : D(0.0)));
-            s += D(c3) * (md * p.e_tot + p.p * vel[d]);
+            R cw[NV];
             #pragma unroll
-            for (int i = 0; i < NC; i++) s += D(-wj * hinv * w[4 * (4 + i) + 1 + d]) * (md * p.c[i]);
+            for (int v = 0; v < NV; v++) cw[v] = f * w[4 * v + 1 + d];
+            const R md = t[d];
+            R td = cw[0] * p.u + cw[1] * p.v + cw[2] * p.w + cw[3] * p.e_tot;
+            #pragma unroll
+            for (int i = 0; i < NC; i++) { td += cw[4 + i] * p.c[i]; pb.c[i] += cw[4 + i] * md; }
+            tb[d] = td;
+            pb.u += cw[0] * md; pb.v += cw[1] * md; pb.w += cw[2] * md; pb.e_tot += cw[3] * md;
+            pb.p += cw[d] + cw[3] * vel[d];
+            velb[d] += cw[3] * p.p;
         }
+        pb.u += velb[0]; pb.v += velb[1]; pb.w += velb[2];
         #pragma unroll
-        for (int v = 0; v < 3; v++) {
-            const R gv = cst<R>(P.grav[v], P.grav_f[v]);
-            s += D(-gv * wj * wdot<R, NV>(w, v, ph)) * p.r;
-        }
+        for (int v = 0; v < 3; v++) pb.r -= cst<R>(P.grav[v], P.grav_f[v]) * wj * wdot<R, NV>(w, v, ph);
+        tb[3] = R(0);
+        #pragma unroll
+        for (int v = 4; v < NV; v++) tb[v] = R(0);
+        R tp[NV];
+        cons_to_prim_vjp<R, NC>(P, t, p, pb, tp);
         const R phi[4] = {R(1), ph[0], ph[1], ph[2]};
         #pragma unroll
         for (int v = 0; v < NV; v++)
             #pragma unroll
-            for (int k = 0; k < 4; k++) ub[4 * v + k] += phi[k] * s.d[v];
+            for (int k = 0; k < 4; k++) ub[4 * v + k] += phi[k] * (tb[v] + tp[v]);
     }
 
-    double th[3] = {0.0, 0.0, 0.0};
-    face_points<R>(A.nbr, A.aux, A.level, e, h,
-        [&](int d, int nb, const R ps[3], const R pn[3], R wjf, R sign) {
+    // interior face points: the neighbour's state and weights are loaded
+    // once per neighbour; e's trace gradient at each point from one
+    // reverse sweep through the flux
+    auto point = [&](const R *un, const R *wn, int a, const R ps[3], const R pn[3], R wjf, R sign) {
+        R ts[NV], tn[NV];
+        #pragma unroll
+        for (int v = 0; v < NV; v++) {
+            ts[v] = at<R>(&u[4 * v], ps[0], ps[1], ps[2]);
+            tn[v] = at<R>(&un[4 * v], pn[0], pn[1], pn[2]);
+        }
+        Prim<R, NC> pS, pN;
+        cons_to_prim<R, NC>(P, ts, pS);
+        cons_to_prim<R, NC>(P, tn, pN);
+        Frame<R, NC> fS, fN;
+        frame_of<R, NC>(pS, a, fS);
+        frame_of<R, NC>(pN, a, fN);
+        R G[NV];
+        #pragma unroll
+        for (int v = 0; v < NV; v++) G[v] = sign * wjf * (wdot<R, NV>(w, v, ps) - wdot<R, NV>(wn, v, pn));
+        const int t1 = (a == 0) ? 1 : 0, t2 = (a == 2) ? 1 : 2;
+        R qb[NV];
+        qb[0] = G[a]; qb[1] = G[t1]; qb[2] = G[t2];
+        #pragma unroll
+        for (int v = 3; v < NV; v++) qb[v] = G[v];
+        FrameBar<R, NC> b0, b1;
+        zero_bar<R, NC>(b0);
+        zero_bar<R, NC>(b1);
+        // q = flux(side 0, side 1) with n = +e_a; e is side 0 when sign > 0
+        if (sign > R(0)) flux_vjp<R, NC, FLUX>(fS, fN, qb, b0, b1);
+        else flux_vjp<R, NC, FLUX>(fN, fS, qb, b0, b1);
+        PrimBar<R, NC> pb;
+        prim_bar_of<R, NC>(sign > R(0) ? b0 : b1, a, pb);
+        R tb[NV];
+        cons_to_prim_vjp<R, NC>(P, ts, pS, pb, tb);
+        const R phi[4] = {R(1), ps[0], ps[1], ps[2]};
+        #pragma unroll
+        for (int v = 0; v < NV; v++)
+            #pragma unroll
+            for (int k = 0; k < 4; k++) ub[4 * v + k] += phi[k] * tb[v];
+    };
+    #pragma unroll 1
+    for (int d = 0; d < 6; d++) {
+        const int nb = A.nbr[6 * e + d];
+        const R sign = (d & 1) ? R(1) : R(-1);
+        const int a = d >> 1;
+        if (nb >= 0) {
+            R o1 = R(0), o2 = R(0), m = R(1);
+            if (A.level[nb] != lev) {
+                // e is sub-face j of the coarser neighbour's face
+                const int4 grp = A.aux[-kNbrAuxBase - A.nbr[6 * nb + (d ^ 1)]];
+                const int j = (grp.x == e) ? 0 : ((grp.y == e) ? 1 : ((grp.z == e) ? 2 : 3));
+                o1 = (j & 1) ? R(0.25) : R(-0.25);
+                o2 = (j & 2) ? R(0.25) : R(-0.25);
+                m = R(0.5);
+            }
             R un[4 * NV], wn[4 * NV];
             load_coeffs<R, NV>(U, n, nb, un);
             adj_weights<R, NV>(A, P, X, nb, wn);
-            D ts[NV], tn[NV];
-            #pragma unroll
-            for (int v = 0; v < NV; v++) {
-                ts[v] = D::var(at<R>(&u[4 * v], ps[0], ps[1], ps[2]), v);
-                tn[v] = D(at<R>(&un[4 * v], pn[0], pn[1], pn[2]));
+            const R wjf = h * h * R(1.0 / 6.0);
+            #pragma unroll 1
+            for (int k = 0; k < 6; k++) {
+                R ps[3], pn[3];
+                face_point<R>(d, k, R(0), R(0), R(1), ps);
+                face_point<R>(d ^ 1, k, o1, o2, m, pn);
+                point(un, wn, a, ps, pn, wjf, sign);
             }
-            Prim<D, NC> psf, pnb;
-            cons_to_prim<D, NC>(P, ts, psf);
-            cons_to_prim<D, NC>(P, tn, pnb);
-            D qg[NV];
-            // q = flux(side 0, side 1) with n = +e_a; e is side 0 when sign > 0
-            if (sign > R(0)) flux_global<D, NC, FLUX>(psf, pnb, d >> 1, D(1.0), qg);
-            else flux_global<D, NC, FLUX>(pnb, psf, d >> 1, D(1.0), qg);
-            D s(0.0);
-            #pragma unroll
-            for (int v = 0; v < NV; v++)
-                s += qg[v] * D(sign * wjf * (wdot<R, NV>(w, v, ps) - wdot<R, NV>(wn, v, pn)));
-            const R phi[4] = {R(1), ps[0], ps[1], ps[2]};
-            #pragma unroll
-            for (int v = 0; v < NV; v++)
-                #pragma unroll
-                for (int k = 0; k < 4; k++) ub[4 * v + k] += phi[k] * s.d[v];
-        },
-        [&](int d, int face, int, const R ps[3], R wjf) {
-            const int a = d >> 1;
-            const R sg = (d & 1) ? R(1) : R(-1);
+        } else if (nb <= -kNbrAuxBase) {
+            const int4 grp = A.aux[-kNbrAuxBase - nb];
+            const int q4[4] = {grp.x, grp.y, grp.z, grp.w};
+            const R wjf = R(0.25) * h * h * R(1.0 / 6.0);
+            #pragma unroll 1
+            for (int j = 0; j < 4; j++) {
+                R un[4 * NV], wn[4 * NV];
+                load_coeffs<R, NV>(U, n, q4[j], un);
+                adj_weights<R, NV>(A, P, X, q4[j], wn);
+                const R o1 = (j & 1) ? R(0.25) : R(-0.25), o2 = (j & 2) ? R(0.25) : R(-0.25);
+                #pragma unroll 1
+                for (int k = 0; k < 6; k++) {
+                    R ps[3], pn[3];
+                    face_point<R>(d, k, o1, o2, R(0.5), ps);
+                    face_point<R>(d ^ 1, k, R(0), R(0), R(1), pn);
+                    point(un, wn, a, ps, pn, wjf, sign);
+                }
+            }
+        }
+    }
+
+    R *__restrict__ out = reinterpret_cast<R *>(A.OUT);
+    const R *__restrict__ acc = reinterpret_cast<const R *>(A.ACC);
+    #pragma unroll
+    for (int i = 0; i < 4 * NV; i++) {
+        const R sb = R(A.c_in) * (X + i * n)[e];
+        R o = R(A.c_out) * (sb + ub[i]);
+        if (acc) o += R(A.c_acc) * (acc + i * n)[e];
+        (out + i * n)[e] = o;
+    }
+}
+
+// Boundary faces of the listed leaves (after k_adj_stage on the same
+// output): the boundary state functions run on forward-mode dual numbers
+// (tangents = the leaf's trace), OUT += c_out (d int / d U)^T W over the
+// boundary points; the inlet parameters get their own dual pass.
+template <typename R, int NC, int FLUX>
+__global__ __launch_bounds__(64)
+void k_adj_bnd(AdjArgs A, DevParams P)
+{
+    constexpr int NV = 4 + NC;
+    using D = Dual<R, NV>;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n_bnd) return;
+    const int e = A.bnd_elems[i];
+    const int64_t n = A.n;
+    const R *__restrict__ U = reinterpret_cast<const R *>(A.U);
+    const R *__restrict__ X = reinterpret_cast<const R *>(A.X);
+    const int lev = A.level[e];
+    const R h = ldexp(R(1), -lev);
+    R u[4 * NV], w[4 * NV], ub[4 * NV];
+    load_coeffs<R, NV>(U, n, e, u);
+    adj_weights<R, NV>(A, P, X, e, w);
+    #pragma unroll
+    for (int j = 0; j < 4 * NV; j++) ub[j] = R(0);
+    double th[3] = {0.0, 0.0, 0.0};
+    const R wjf = h * h * R(1.0 / 6.0);
+    #pragma unroll 1
+    for (int d = 0; d < 6; d++) {
+        const int nb = A.nbr[6 * e + d];
+        if (nb >= 0 || nb <= -kNbrAuxBase) continue;
+        const int face = -1 - nb, a = d >> 1;
+        const R sg = (d & 1) ? R(1) : R(-1);
+        #pragma unroll 1
+        for (int k = 0; k < 6; k++) {
+            R ps[3];
+            face_point<R>(d, k, R(0), R(0), R(1), ps);
             {
                 D ts[NV];
                 #pragma unroll
@@ -196,7 +313,7 @@ void k_adj_stage(AdjArgs A, DevParams P)
                 #pragma unroll
                 for (int v = 0; v < NV; v++)
                     #pragma unroll
-                    for (int k = 0; k < 4; k++) ub[4 * v + k] += phi[k] * s.d[v];
+                    for (int q = 0; q < 4; q++) ub[4 * v + q] += phi[q] * s.d[v];
             }
             if (face == A.theta_face && A.theta_bar && P.bnd_type[face] == BND_INLET) {
                 using T3 = Dual<R, 3>;
@@ -213,17 +330,11 @@ void k_adj_stage(AdjArgs A, DevParams P)
                 for (int v = 0; v < NV; v++) s += qg[v] * T3(wjf * wdot<R, NV>(w, v, ps));
                 th[0] += (double) s.d[0]; th[1] += (double) s.d[1]; th[2] += (double) s.d[2];
             }
-        });
-
-    R *__restrict__ out = reinterpret_cast<R *>(A.OUT);
-    const R *__restrict__ acc = reinterpret_cast<const R *>(A.ACC);
-    #pragma unroll
-    for (int i = 0; i < 4 * NV; i++) {
-        const R sb = R(A.c_in) * (X + i * n)[e];
-        R o = R(A.c_out) * (sb + ub[i]);
-        if (acc) o += R(A.c_acc) * (acc + i * n)[e];
-        (out + i * n)[e] = o;
+        }
     }
+    R *__restrict__ out = reinterpret_cast<R *>(A.OUT);
+    #pragma unroll
+    for (int j = 0; j < 4 * NV; j++) (out + j * n)[e] += R(A.c_out) * ub[j];
     if (A.theta_bar) {
         A.theta_bar[3 * e + 0] += th[0];
         A.theta_bar[3 * e + 1] += th[1];
@@ -304,11 +415,19 @@ template <typename R, int NC>
 static hipError_t launch_adj_nc(int flux, const AdjArgs &A, const DevParams &P, hipStream_t st)
 {
     const dim3 grid((unsigned) ((A.n + 127) / 128)), block(128);
+    const dim3 gb((unsigned) ((A.n_bnd + 63) / 64)), bb(64);
     switch (flux) {
     case FLUX_LF: hipLaunchKernelGGL((k_adj_stage<R, NC, FLUX_LF>), grid, block, 0, st, A, P); break;
     case FLUX_HLLC: hipLaunchKernelGGL((k_adj_stage<R, NC, FLUX_HLLC>), grid, block, 0, st, A, P); break;
     case FLUX_CD: hipLaunchKernelGGL((k_adj_stage<R, NC, FLUX_CD>), grid, block, 0, st, A, P); break;
     default: return hipErrorInvalidValue;
+    }
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess || A.n_bnd <= 0) return err;
+    switch (flux) {
+    case FLUX_LF: hipLaunchKernelGGL((k_adj_bnd<R, NC, FLUX_LF>), gb, bb, 0, st, A, P); break;
+    case FLUX_HLLC: hipLaunchKernelGGL((k_adj_bnd<R, NC, FLUX_HLLC>), gb, bb, 0, st, A, P); break;
+    default: hipLaunchKernelGGL((k_adj_bnd<R, NC, FLUX_CD>), gb, bb, 0, st, A, P); break;
     }
     return hipGetLastError();
 }

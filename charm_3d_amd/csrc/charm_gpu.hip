@@ -913,6 +913,42 @@ extern "C" int charm_gpu_objective(charm_gpu_ctx_t *c, double dt, int nsteps, co
     return rc;
 }
 
+// device list of the leaves with a boundary face (from the neighbour table:
+// codes -1..-7 are boundary faces)
+static int adj_boundary_leaves(charm_gpu_ctx_t *c, int **out, int *count)
+{
+    const int64_t n = c->n;
+    int *nbr = (int *) malloc((size_t) n * 6 * sizeof(int));
+    int *list = (int *) malloc((size_t) n * sizeof(int));
+    if (!nbr || !list) {
+        free(nbr); free(list);
+        charm_set_error("adjoint: out of memory");
+        return CHARM_ERR_MEM;
+    }
+    hipError_t e = hipMemcpyAsync(nbr, c->d_nbr, (size_t) n * 6 * sizeof(int), hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    int m = 0;
+    if (e == hipSuccess) {
+        for (int64_t i = 0; i < n; i++) {
+            bool b = false;
+            for (int d = 0; d < 6; d++) b |= nbr[6 * i + d] < 0 && nbr[6 * i + d] > -8;
+            if (b) list[m++] = (int) i;
+        }
+    }
+    *out = NULL;
+    if (e == hipSuccess && m > 0) e = hipMalloc((void **) out, (size_t) m * sizeof(int));
+    if (e == hipSuccess && m > 0) e = hipMemcpy(*out, list, (size_t) m * sizeof(int), hipMemcpyHostToDevice);
+    free(nbr); free(list);
+    if (e != hipSuccess) {
+        if (*out) (void) hipFree(*out);
+        *out = NULL;
+        charm_set_error("adjoint: %s", hipGetErrorString(e));
+        return CHARM_ERR_HIP;
+    }
+    *count = m;
+    return CHARM_OK;
+}
+
 extern "C" int charm_gpu_adjoint(charm_gpu_ctx_t *c, double dt, int nsteps, int ckpt, const double lo[3],
                                  const double hi[3], int theta_face, double *J, double *grad_u0,
                                  double *grad_theta, double *fwd_ms, double *adj_ms)
@@ -926,6 +962,8 @@ extern "C" int charm_gpu_adjoint(charm_gpu_ctx_t *c, double dt, int nsteps, int 
     const int ncp = (nsteps + ckpt - 1) / ckpt;
     char *cp = NULL, *seg = NULL, *la = NULL, *lb = NULL, *lc = NULL, *uend = NULL;
     double *part = NULL, *theta = NULL;
+    int *bnd = NULL, n_bnd = 0;
+    if ((rc = adj_boundary_leaves(c, &bnd, &n_bnd))) return rc;
     hipError_t e = hipMalloc((void **) &cp, (size_t) ncp * ssz);
     if (e == hipSuccess) e = hipMalloc((void **) &seg, (size_t) 3 * ckpt * ssz);
     if (e == hipSuccess) e = hipMalloc((void **) &la, ssz);
@@ -973,6 +1011,7 @@ extern "C" int charm_gpu_adjoint(charm_gpu_ctx_t *c, double dt, int nsteps, int 
             AdjArgs A;
             A.level = c->d_level; A.nbr = c->d_nbr; A.aux = c->d_aux; A.n = n; A.dt = dt;
             A.theta_bar = theta_face >= 0 ? theta : NULL; A.theta_face = theta_face;
+            A.bnd_elems = bnd; A.n_bnd = n_bnd;
             A.U = u2; A.X = la; A.ACC = NULL; A.OUT = lb; A.c_in = 2.0 / 3.0; A.c_out = 1.0; A.c_acc = 0.0;
             ADJ_CHK(launch_adj_stage(c->sprec, c->nc, c->cfg.flux, A, c->P, c->st));
             ADJ_CHK(launch_adj_axpby(c->sprec, lb, la, 0.75, 1.0 / 3.0, m, c->st));
@@ -1007,7 +1046,7 @@ extern "C" int charm_gpu_adjoint(charm_gpu_ctx_t *c, double dt, int nsteps, int 
 #undef ADJ_CHK
 done:
     (void) hipStreamSynchronize(c->st);
-    void *bufs[] = {cp, seg, la, lb, lc, uend, part, theta};
+    void *bufs[] = {cp, seg, la, lb, lc, uend, part, theta, bnd};
     for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); i++)
         if (bufs[i]) (void) hipFree(bufs[i]);
     return rc;

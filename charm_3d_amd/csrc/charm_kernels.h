@@ -100,6 +100,8 @@ struct AdjArgs {
     double *theta_bar;        // [n][3] += d/d(inlet u, T, p) (or null)
     int theta_face;           // the inlet face of the parameters (-1: none)
     int64_t n;
+    const int *bnd_elems;     // leaves with at least one boundary face
+    int n_bnd;
 };
 
 // objective over a box of leaves: mode 0 per-block partial sums of

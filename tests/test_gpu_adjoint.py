@@ -80,3 +80,73 @@ def test_fp32_adjoint_close_to_fp64():
     g64, g32 = res[CA.FP64][1], res[CA.FP32][1]
     assert np.sqrt(((g32 - g64) ** 2).sum() / (g64 ** 2).sum()) < 1e-4
     np.testing.assert_allclose(res[CA.FP32][2], res[CA.FP64][2], rtol=1e-4)
+
+
+def _mixture_cons(rho, vel, p, y, ms, cps):
+    """conservative state of a constant-cp mixture (cp = sum y cp_i,
+    M = 1 / sum y / M_i, e = cv T, p = rho R T / M)"""
+    cp = sum(y[:, i] * cps[i] for i in range(len(ms)))
+    rm = sum(y[:, i] / ms[i] for i in range(len(ms))) * CA.GAS_R      # R / M
+    cv = cp - rm
+    e = cv * p / (rho * rm)
+    out = np.zeros((len(rho), 4 + len(ms)))
+    out[:, 0:3] = rho[:, None] * vel
+    out[:, 3] = rho * (e + 0.5 * (vel ** 2).sum(1))
+    out[:, 4:] = rho[:, None] * y
+    return out
+
+
+def _branch_case(name):
+    """configurations that drive the stage VJPs through their branches:
+    the LF and central fluxes, HLLC in supersonic flow (outer wave-speed
+    regions), a strong pressure jump (shock branches of the wave-speed
+    estimate) and a two-species mixture (gamma depending on the state)"""
+    lv = MU.random_adapted(2, 3, seed=8, frac=0.3)
+    flux, vel0, jump, nc = {"lf": ("LF", 0.2, False, 1), "central": ("CD", 0.2, False, 1),
+                            "supersonic": ("HLLC", 2.0, False, 1), "jump": ("HLLC", 0.1, True, 1),
+                            "mixture": ("HLLC", 0.3, False, 2)}[name]
+    ms = (CA.GAS_R, 2.0 * CA.GAS_R)[:nc]
+    cps = (3.5, 1.67 / 0.67 * 0.5)[:nc]
+
+    def fn(x):
+        r2 = ((x - 0.4) ** 2).sum(1)
+        rho = 1 + 0.1 * np.exp(-r2 / 0.02)
+        p = np.ones(len(x))
+        if jump:
+            s = 0.5 * (1 + np.tanh((x[:, 0] - 0.5) / 0.05))
+            rho = rho * (1 - 0.6 * s)
+            p = 1 - 0.8 * s
+        vel = np.column_stack([vel0 + 0.05 * np.sin(2 * np.pi * x[:, 1]), 0.1 * np.cos(2 * np.pi * x[:, 0]),
+                               -0.05 * np.ones(len(x))])
+        y = np.column_stack([0.7 + 0.2 * np.sin(2 * np.pi * x[:, 2]), 0.3 - 0.2 * np.sin(2 * np.pi * x[:, 2])])[:, :nc]
+        y = y / y.sum(1, keepdims=True)
+        return _mixture_cons(rho, vel, p, y, ms, cps)
+
+    t_in = 1.0 / ((CA.GAS_R / ms[0]) if nc == 1 else 1.0)
+    cfg = CA.make_config(precision=CA.FP64, model="NS", flux=flux, limiter=False,
+                         eos="IDEAL" if nc == 1 else "MIX",
+                         bnd=("INLET",) + ("OUTLET",) * 5,
+                         bnd_param=[(vel0, 0.0, 0.0, t_in, 1.0)] + [None] * 5,
+                         comp_m=ms, comp_cp=cps, cfl=0.1)
+    return cfg, lv, W.project_p1(lv, fn, 4 + nc)
+
+
+@pytest.mark.parametrize("name", ["lf", "central", "supersonic", "jump", "mixture"])
+def test_stage_vjp_branches_match_directional_differences(name):
+    cfg, lv, u0 = _branch_case(name)
+    per = (False,) * 3
+    ctx = device(cfg, lv, per, u0)
+    dt = 0.5 * ctx.get_dt(1)
+    steps = 8
+    _, g, _, _, _ = ctx.adjoint(dt, steps, LO, HI, ckpt=3)
+    assert np.isfinite(g).all()
+    rng = np.random.default_rng(11)
+    for _ in range(2):
+        v = rng.normal(size=u0.shape) * np.abs(u0).max(axis=0, keepdims=True)
+        eps = 1e-7
+        jp = device(cfg, lv, per, u0 + eps * v).objective(dt, steps, LO, HI)
+        jm = device(cfg, lv, per, u0 - eps * v).objective(dt, steps, LO, HI)
+        fd = (jp - jm) / (2 * eps)
+        ad = float((g * v).sum())
+        print("%s: adjoint %.12e, central difference %.12e" % (name, ad, fd))
+        assert abs(ad - fd) <= 1e-5 * abs(fd), (name, ad, fd)
