@@ -92,32 +92,13 @@ __device__ __forceinline__ void inlet_dual(const DevParams &P, int face, const P
     out.e_tot = out.e + D(0.5) * (out.u * out.u + out.v * out.v + out.w * out.w);
 }
 
-} // namespace
-
-// Interior part of one stage adjoint: the volume term and every interior
-// face point of leaf e, each through the hand-written reverse sweep (the
-// primal at the point, then the flux and cons_to_prim VJPs for e's side).
-// Boundary faces are added by k_adj_bnd.
-template <typename R, int NC, int FLUX>
-__global__ __launch_bounds__(128)
-void k_adj_stage(AdjArgs A, DevParams P)
+// Volume term of a stage adjoint at leaf e (coefficients u, adjoint weights
+// w): ub += (d / d u) of sum_d sum_v cw_dv F_d,v(t) - sum_{v<3} g_v wJ w(ph) rho
+// over the 8 Gauss points, cw_dv = -(wJ / h) w[v][1 + d]
+template <typename R, int NC>
+__device__ __forceinline__ void adj_volume(const DevParams &P, const R *u, const R *w, R h, R hinv, R *ub)
 {
     constexpr int NV = 4 + NC;
-    const int64_t n = A.n;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    const R *__restrict__ U = reinterpret_cast<const R *>(A.U);
-    const R *__restrict__ X = reinterpret_cast<const R *>(A.X);
-    const int lev = A.level[e];
-    const R h = ldexp(R(1), -lev), hinv = ldexp(R(1), lev);
-    R u[4 * NV], w[4 * NV], ub[4 * NV];
-    load_coeffs<R, NV>(U, n, e, u);
-    adj_weights<R, NV>(A, P, X, e, w);
-    #pragma unroll
-    for (int i = 0; i < 4 * NV; i++) ub[i] = R(0);
-
-    // volume: s = sum_d sum_v cw_dv F_d,v(t) - sum_{v<3} g_v wJ w(ph) rho,
-    // cw_dv = -(wJ / h) w[v][1 + d]
     const R gph = R(0.28867513459481288225);
     const R wj = h * h * h * R(0.125);
     const R f = -wj * hinv;
@@ -163,6 +144,34 @@ void k_adj_stage(AdjArgs A, DevParams P)
             #pragma unroll
             for (int k = 0; k < 4; k++) ub[4 * v + k] += phi[k] * (tb[v] + tp[v]);
     }
+
+}
+
+} // namespace
+
+// Interior part of one stage adjoint: the volume term and every interior
+// face point of leaf e, each through the hand-written reverse sweep (the
+// primal at the point, then the flux and cons_to_prim VJPs for e's side).
+// Boundary faces are added by k_adj_bnd_pts / k_adj_bnd_sum.
+template <typename R, int NC, int FLUX>
+__global__ __launch_bounds__(128)
+void k_adj_stage(AdjArgs A, DevParams P)
+{
+    constexpr int NV = 4 + NC;
+    const int64_t n = A.n;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const R *__restrict__ U = reinterpret_cast<const R *>(A.U);
+    const R *__restrict__ X = reinterpret_cast<const R *>(A.X);
+    const int lev = A.level[e];
+    const R h = ldexp(R(1), -lev), hinv = ldexp(R(1), lev);
+    R u[4 * NV], w[4 * NV], ub[4 * NV];
+    load_coeffs<R, NV>(U, n, e, u);
+    adj_weights<R, NV>(A, P, X, e, w);
+    #pragma unroll
+    for (int i = 0; i < 4 * NV; i++) ub[i] = R(0);
+
+    adj_volume<R, NC>(P, u, w, h, hinv, ub);
 
     // interior face points: the neighbour's state and weights are loaded
     // once per neighbour; e's trace gradient at each point from one
@@ -262,84 +271,285 @@ void k_adj_stage(AdjArgs A, DevParams P)
     }
 }
 
-// Boundary faces of the listed leaves (after k_adj_stage on the same
-// output): the boundary state functions run on forward-mode dual numbers
-// (tangents = the leaf's trace), OUT += c_out (d int / d U)^T W over the
-// boundary points; the inlet parameters get their own dual pass.
+// One interior face of a chunk, both sides at once: at each of the six
+// points one primal evaluation and one reverse sweep of
+//   s = wJ sum_v q_v(t0, t1) (w0_v(p0) - w1_v(p1))
+// (q the flux along side 0's outward normal, global component order; w_s
+// the adjoint weights of side s at its point) give d s / d t0 and d s / d t1;
+// out[side][m][v] = sum over the points of (d s / d t_side)_v g_m,
+// g = (1, t1 coordinate, t2 coordinate) of the small face's point.
 template <typename R, int NC, int FLUX>
-__global__ __launch_bounds__(64)
-void k_adj_bnd(AdjArgs A, DevParams P)
+__device__ __forceinline__ void adj_face(const AdjArgs &A, const DevParams &P, const R *__restrict__ U,
+                                         const R *__restrict__ X, int4 rec, R *out)
 {
     constexpr int NV = 4 + NC;
-    using D = Dual<R, NV>;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= A.n_bnd) return;
-    const int e = A.bnd_elems[i];
     const int64_t n = A.n;
+    const int info = rec.z;
+    const int dir0 = info & 7, kind = (info >> 3) & 3, sub = (info >> 5) & 3;
+    const int a = dir0 >> 1, t1 = (a == 0) ? 1 : 0, t2 = (a == 2) ? 1 : 2;
+    const R s = (dir0 & 1) ? R(1) : R(-1);
+    const int lown = A.level[rec.x] + (kind == FACE_HANG1 ? 1 : 0);
+    const R h = ldexp(R(1), -lown);
+    const R wj = h * h * R(1.0 / 6.0);
+    const R pa0 = R(0.5) * s, pa1 = -pa0;
+    const R d1 = (sub & 1) ? R(0.25) : R(-0.25), d2 = (sub & 2) ? R(0.25) : R(-0.25);
+    const bool big0 = kind == FACE_HANG1, big1 = kind == FACE_HANG0;
+    const R m0 = big0 ? R(0.5) : R(1), m1 = big1 ? R(0.5) : R(1);
+    const R o01 = big0 ? d1 : R(0), o02 = big0 ? d2 : R(0);
+    const R o11 = big1 ? d1 : R(0), o12 = big1 ? d2 : R(0);
+    R u0[4 * NV], u1[4 * NV], w0[4 * NV], w1[4 * NV];
+    load_coeffs<R, NV>(U, n, rec.x, u0);
+    load_coeffs<R, NV>(U, n, rec.y, u1);
+    adj_weights<R, NV>(A, P, X, rec.x, w0);
+    adj_weights<R, NV>(A, P, X, rec.y, w1);
+    R M0[3][NV], M1[3][NV];
+    #pragma unroll
+    for (int m = 0; m < 3; m++)
+        #pragma unroll
+        for (int v = 0; v < NV; v++) { M0[m][v] = R(0); M1[m][v] = R(0); }
+    #pragma unroll 1
+    for (int k = 0; k < 6; k++) {
+        const R g1 = gp_t1<R>(k), g2 = gp_t2<R>(k);
+        const R q01 = o01 + m0 * g1, q02 = o02 + m0 * g2;     // side 0 tangential coordinates
+        const R q11 = o11 + m1 * g1, q12 = o12 + m1 * g2;     // side 1
+        R tL[NV], tR[NV], G[NV];
+        #pragma unroll
+        for (int v = 0; v < NV; v++) {
+            tL[v] = u0[4*v] + pa0 * u0[4*v+1+a] + q01 * u0[4*v+1+t1] + q02 * u0[4*v+1+t2];
+            tR[v] = u1[4*v] + pa1 * u1[4*v+1+a] + q11 * u1[4*v+1+t1] + q12 * u1[4*v+1+t2];
+            const R wl = w0[4*v] + pa0 * w0[4*v+1+a] + q01 * w0[4*v+1+t1] + q02 * w0[4*v+1+t2];
+            const R wr = w1[4*v] + pa1 * w1[4*v+1+a] + q11 * w1[4*v+1+t1] + q12 * w1[4*v+1+t2];
+            G[v] = wj * (wl - wr);
+        }
+        Prim<R, NC> pL, pR;
+        cons_to_prim<R, NC>(P, tL, pL);
+        cons_to_prim<R, NC>(P, tR, pR);
+        Frame<R, NC> fL, fR;
+        frame_of<R, NC>(pL, a, fL);
+        frame_of<R, NC>(pR, a, fR);
+        fL.un *= s; fR.un *= s;
+        R qb[NV];
+        qb[0] = s * G[a]; qb[1] = G[t1]; qb[2] = G[t2];
+        #pragma unroll
+        for (int v = 3; v < NV; v++) qb[v] = G[v];
+        FrameBar<R, NC> b0, b1;
+        zero_bar<R, NC>(b0);
+        zero_bar<R, NC>(b1);
+        flux_vjp<R, NC, FLUX>(fL, fR, qb, b0, b1);
+        b0.un *= s; b1.un *= s;
+        PrimBar<R, NC> pb;
+        R tb[NV];
+        prim_bar_of<R, NC>(b0, a, pb);
+        cons_to_prim_vjp<R, NC>(P, tL, pL, pb, tb);
+        #pragma unroll
+        for (int v = 0; v < NV; v++) { M0[0][v] += tb[v]; M0[1][v] += tb[v] * g1; M0[2][v] += tb[v] * g2; }
+        prim_bar_of<R, NC>(b1, a, pb);
+        cons_to_prim_vjp<R, NC>(P, tR, pR, pb, tb);
+        #pragma unroll
+        for (int v = 0; v < NV; v++) { M1[0][v] += tb[v]; M1[1][v] += tb[v] * g1; M1[2][v] += tb[v] * g2; }
+    }
+    #pragma unroll
+    for (int m = 0; m < 3; m++)
+        #pragma unroll
+        for (int v = 0; v < NV; v++) {
+            out[m * NV + v] = M0[m][v];
+            out[(3 + m) * NV + v] = M1[m][v];
+        }
+}
+
+// Chunked face-centric stage adjoint (interior faces and volume; boundary
+// faces by k_adj_bnd_pts / k_adj_bnd_sum afterwards): the chunks of the stage kernel; phase 1
+// one thread per face of the chunk writes both sides' trace-gradient
+// moments to LDS, phase 2 one thread per element adds its volume term and
+// gathers its face slots (a fixed slot list, no atomics).  Each interior
+// face is evaluated once per chunk that holds one of its sides.
+template <typename R, int NC, int FLUX, int BS>
+__global__ __launch_bounds__(BS)
+void k_adj_chunk(AdjArgs A, DevParams P)
+{
+    constexpr int NV = 4 + NC;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    R *mom = reinterpret_cast<R *>(smem_raw);                           // [maxf][2][3][NV]
+    int *finfo = reinterpret_cast<int *>(smem_raw + (size_t) A.max_faces * 6 * NV * sizeof(R));
     const R *__restrict__ U = reinterpret_cast<const R *>(A.U);
     const R *__restrict__ X = reinterpret_cast<const R *>(A.X);
+    const int c = xcd_block(A.n_xcd);
+    if (c >= A.n_chunks) return;                 // whole workgroup exits together
+    const int fb = A.ch_fbeg[c], fe = A.ch_fbeg[c + 1];
+    const int eb = A.ch_ebeg[c], ee = A.ch_ebeg[c + 1];
+    for (int f = fb + (int) threadIdx.x; f < fe; f += BS) {
+        const int4 rec = A.cf4[f];
+        if (((rec.z >> 3) & 3) != FACE_BND) adj_face<R, NC, FLUX>(A, P, U, X, rec, mom + (size_t) (f - fb) * 6 * NV);
+        finfo[f - fb] = rec.z;
+    }
+    __syncthreads();
+    const int e = eb + (int) threadIdx.x;
+    if (e >= ee) return;
+    const int64_t n = A.n;
     const int lev = A.level[e];
-    const R h = ldexp(R(1), -lev);
+    const R h = ldexp(R(1), -lev), hinv = ldexp(R(1), lev);
     R u[4 * NV], w[4 * NV], ub[4 * NV];
     load_coeffs<R, NV>(U, n, e, u);
     adj_weights<R, NV>(A, P, X, e, w);
     #pragma unroll
-    for (int j = 0; j < 4 * NV; j++) ub[j] = R(0);
-    double th[3] = {0.0, 0.0, 0.0};
-    const R wjf = h * h * R(1.0 / 6.0);
-    #pragma unroll 1
-    for (int d = 0; d < 6; d++) {
-        const int nb = A.nbr[6 * e + d];
-        if (nb >= 0 || nb <= -kNbrAuxBase) continue;
-        const int face = -1 - nb, a = d >> 1;
-        const R sg = (d & 1) ? R(1) : R(-1);
-        #pragma unroll 1
-        for (int k = 0; k < 6; k++) {
-            R ps[3];
-            face_point<R>(d, k, R(0), R(0), R(1), ps);
-            {
-                D ts[NV];
-                #pragma unroll
-                for (int v = 0; v < NV; v++) ts[v] = D::var(at<R>(&u[4 * v], ps[0], ps[1], ps[2]), v);
-                Prim<D, NC> pi, po;
-                cons_to_prim<D, NC>(P, ts, pi);
-                bnd_state<D, NC>(P, face, a, D(sg), pi, po, -1);
-                D qg[NV];
-                flux_global<D, NC, FLUX>(pi, po, a, D(sg), qg);
-                D s(0.0);
-                #pragma unroll
-                for (int v = 0; v < NV; v++) s += qg[v] * D(wjf * wdot<R, NV>(w, v, ps));
-                const R phi[4] = {R(1), ps[0], ps[1], ps[2]};
-                #pragma unroll
-                for (int v = 0; v < NV; v++)
-                    #pragma unroll
-                    for (int q = 0; q < 4; q++) ub[4 * v + q] += phi[q] * s.d[v];
-            }
-            if (face == A.theta_face && A.theta_bar && P.bnd_type[face] == BND_INLET) {
-                using T3 = Dual<R, 3>;
-                T3 ts[NV];
-                #pragma unroll
-                for (int v = 0; v < NV; v++) ts[v] = T3(at<R>(&u[4 * v], ps[0], ps[1], ps[2]));
-                Prim<T3, NC> pi, po;
-                cons_to_prim<T3, NC>(P, ts, pi);
-                inlet_dual<R, NC>(P, face, pi, po);
-                T3 qg[NV];
-                flux_global<T3, NC, FLUX>(pi, po, a, T3(sg), qg);
-                T3 s(0.0);
-                #pragma unroll
-                for (int v = 0; v < NV; v++) s += qg[v] * T3(wjf * wdot<R, NV>(w, v, ps));
-                th[0] += (double) s.d[0]; th[1] += (double) s.d[1]; th[2] += (double) s.d[2];
-            }
+    for (int i = 0; i < 4 * NV; i++) ub[i] = R(0);
+    adj_volume<R, NC>(P, u, w, h, hinv, ub);
+    const int s_end = A.slot_ptr[e + 1];
+    for (int si = A.slot_ptr[e]; si < s_end; si++) {
+        const int sl = A.slots[si];
+        const int fl = sl >> 1, side = sl & 1;
+        const int info = finfo[fl];
+        const int dir0 = info & 7, kind = (info >> 3) & 3, sub = (info >> 5) & 3;
+        if (kind == FACE_BND) continue;
+        const int ax = dir0 >> 1;
+        const R pa = ((dir0 & 1) ^ side) ? R(0.5) : R(-0.5);
+        const bool big = (kind == FACE_HANG0 && side == 1) || (kind == FACE_HANG1 && side == 0);
+        const R mu = big ? R(0.5) : R(1);
+        const R d1 = big ? ((sub & 1) ? R(0.25) : R(-0.25)) : R(0);
+        const R d2 = big ? ((sub & 2) ? R(0.25) : R(-0.25)) : R(0);
+        const R *m = mom + (size_t) fl * 6 * NV + (size_t) side * 3 * NV;
+        const int t1 = (ax == 0) ? 1 : 0, t2 = (ax == 2) ? 1 : 2;
+        #pragma unroll
+        for (int v = 0; v < NV; v++) {
+            const R S0 = m[v], S1 = m[NV + v], S2 = m[2 * NV + v];
+            R c3[3];
+            c3[ax] = pa * S0;
+            c3[t1] = mu * S1 + d1 * S0;
+            c3[t2] = mu * S2 + d2 * S0;
+            ub[4*v+0] += S0;
+            ub[4*v+1] += c3[0];
+            ub[4*v+2] += c3[1];
+            ub[4*v+3] += c3[2];
         }
     }
     R *__restrict__ out = reinterpret_cast<R *>(A.OUT);
+    const R *__restrict__ acc = reinterpret_cast<const R *>(A.ACC);
     #pragma unroll
-    for (int j = 0; j < 4 * NV; j++) (out + j * n)[e] += R(A.c_out) * ub[j];
+    for (int i = 0; i < 4 * NV; i++) {
+        const R sb = R(A.c_in) * (X + i * n)[e];
+        R o = R(A.c_out) * (sb + ub[i]);
+        if (acc) o += R(A.c_acc) * (acc + i * n)[e];
+        (out + i * n)[e] = o;
+    }
+}
+
+// Boundary faces, one thread per (boundary face, point): the boundary state
+// functions run on forward-mode dual numbers (tangents = the leaf's trace);
+// the point's contribution phi_k (d s / d t) to the leaf's coefficient
+// gradient (and, on the inlet face of the parameters, d s / d theta) is
+// written to its own slot of a scratch array, summed per leaf in a fixed
+// order by k_adj_bnd_sum.
+template <typename R, int NC, int FLUX>
+__global__ __launch_bounds__(64)
+void k_adj_bnd_pts(AdjArgs A, DevParams P)
+{
+    constexpr int NV = 4 + NC;
+    using D = Dual<R, NV>;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 6 * A.n_bface) return;
+    const int f = i / 6, k = i - 6 * f;
+    const int code = A.bnd_faces[f];
+    const int e = code >> 3, d = code & 7;
+    const int face = -1 - A.nbr[6 * e + d], a = d >> 1;
+    const int64_t n = A.n;
+    const R *__restrict__ U = reinterpret_cast<const R *>(A.U);
+    const R *__restrict__ X = reinterpret_cast<const R *>(A.X);
+    const R h = ldexp(R(1), -(int) A.level[e]);
+    const R wjf = h * h * R(1.0 / 6.0);
+    const R sg = (d & 1) ? R(1) : R(-1);
+    R u[4 * NV], w[4 * NV];
+    load_coeffs<R, NV>(U, n, e, u);
+    adj_weights<R, NV>(A, P, X, e, w);
+    R ps[3];
+    face_point<R>(d, k, R(0), R(0), R(1), ps);
+    R *__restrict__ part = reinterpret_cast<R *>(A.bnd_part) + (size_t) i * 4 * NV;
+    {
+        D ts[NV];
+        #pragma unroll
+        for (int v = 0; v < NV; v++) ts[v] = D::var(at<R>(&u[4 * v], ps[0], ps[1], ps[2]), v);
+        Prim<D, NC> pi, po;
+        cons_to_prim<D, NC>(P, ts, pi);
+        bnd_state<D, NC>(P, face, a, D(sg), pi, po, -1);
+        D qg[NV];
+        flux_global<D, NC, FLUX>(pi, po, a, D(sg), qg);
+        D s(0.0);
+        #pragma unroll
+        for (int v = 0; v < NV; v++) s += qg[v] * D(wjf * wdot<R, NV>(w, v, ps));
+        const R phi[4] = {R(1), ps[0], ps[1], ps[2]};
+        #pragma unroll
+        for (int v = 0; v < NV; v++)
+            #pragma unroll
+            for (int q = 0; q < 4; q++) part[4 * v + q] = phi[q] * s.d[v];
+    }
+    if (A.theta_bar) {
+        double th[3] = {0.0, 0.0, 0.0};
+        if (face == A.theta_face && P.bnd_type[face] == BND_INLET) {
+            using T3 = Dual<R, 3>;
+            T3 ts[NV];
+            #pragma unroll
+            for (int v = 0; v < NV; v++) ts[v] = T3(at<R>(&u[4 * v], ps[0], ps[1], ps[2]));
+            Prim<T3, NC> pi, po;
+            cons_to_prim<T3, NC>(P, ts, pi);
+            inlet_dual<R, NC>(P, face, pi, po);
+            T3 qg[NV];
+            flux_global<T3, NC, FLUX>(pi, po, a, T3(sg), qg);
+            T3 s(0.0);
+            #pragma unroll
+            for (int v = 0; v < NV; v++) s += qg[v] * T3(wjf * wdot<R, NV>(w, v, ps));
+            th[0] = (double) s.d[0]; th[1] = (double) s.d[1]; th[2] = (double) s.d[2];
+        }
+        #pragma unroll
+        for (int j = 0; j < 3; j++) A.bnd_theta[3 * (size_t) i + j] = th[j];
+    }
+}
+
+// per boundary leaf: OUT += c_out * (sum over its boundary faces and their
+// points, in list order), theta_bar += the same sum of the parameter parts
+template <typename R, int NV>
+__global__ __launch_bounds__(128)
+void k_adj_bnd_sum(AdjArgs A)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n_bnd) return;
+    const int e = A.bnd_elems[i];
+    const int f0 = A.bnd_ptr[i], f1 = A.bnd_ptr[i + 1];
+    const R *__restrict__ part = reinterpret_cast<const R *>(A.bnd_part);
+    R acc[4 * NV];
+    #pragma unroll
+    for (int j = 0; j < 4 * NV; j++) acc[j] = R(0);
+    double th[3] = {0.0, 0.0, 0.0};
+    for (int p = 6 * f0; p < 6 * f1; p++) {
+        #pragma unroll
+        for (int j = 0; j < 4 * NV; j++) acc[j] += part[(size_t) p * 4 * NV + j];
+        if (A.theta_bar) {
+            #pragma unroll
+            for (int j = 0; j < 3; j++) th[j] += A.bnd_theta[3 * (size_t) p + j];
+        }
+    }
+    R *__restrict__ out = reinterpret_cast<R *>(A.OUT);
+    const int64_t n = A.n;
+    #pragma unroll
+    for (int j = 0; j < 4 * NV; j++) (out + j * n)[e] += R(A.c_out) * acc[j];
     if (A.theta_bar) {
         A.theta_bar[3 * e + 0] += th[0];
         A.theta_bar[3 * e + 1] += th[1];
         A.theta_bar[3 * e + 2] += th[2];
     }
+}
+
+template <typename R, int NC>
+static hipError_t launch_adj_bnd(int flux, const AdjArgs &A, const DevParams &P, hipStream_t st)
+{
+    if (A.n_bnd <= 0) return hipSuccess;
+    const dim3 gp((unsigned) ((6 * (int64_t) A.n_bface + 63) / 64)), bp(64);
+    switch (flux) {
+    case FLUX_LF: hipLaunchKernelGGL((k_adj_bnd_pts<R, NC, FLUX_LF>), gp, bp, 0, st, A, P); break;
+    case FLUX_HLLC: hipLaunchKernelGGL((k_adj_bnd_pts<R, NC, FLUX_HLLC>), gp, bp, 0, st, A, P); break;
+    default: hipLaunchKernelGGL((k_adj_bnd_pts<R, NC, FLUX_CD>), gp, bp, 0, st, A, P); break;
+    }
+    const dim3 gs((unsigned) ((A.n_bnd + 127) / 128)), bs(128);
+    hipLaunchKernelGGL((k_adj_bnd_sum<R, 4 + NC>), gs, bs, 0, st, A);
+    return hipGetLastError();
 }
 
 // objective Q(U) = sum over leaves with centre in the box of sum_gp wJ p:
@@ -411,11 +621,43 @@ void k_adj_axpby(const R *__restrict__ x, R *__restrict__ y, R a, R b, int64_t m
     if (i < m) y[i] = a * x[i] + b * y[i];
 }
 
+template <typename R, int NC, int FLUX, int BS>
+static hipError_t launch_adj_chunk_bs(const AdjArgs &A, const DevParams &P, hipStream_t st)
+{
+    const size_t sh = adj_chunk_smem_bytes(sizeof(R) == 4, NC, A.max_faces);
+    const void *fn = reinterpret_cast<const void *>(&k_adj_chunk<R, NC, FLUX, BS>);
+    hipError_t err = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int) sh);
+    if (err != hipSuccess) return err;
+    const dim3 grid((unsigned) (((A.n_chunks + A.n_xcd - 1) / A.n_xcd) * A.n_xcd)), block(BS);
+    hipLaunchKernelGGL((k_adj_chunk<R, NC, FLUX, BS>), grid, block, sh, st, A, P);
+    return hipGetLastError();
+}
+
+template <typename R, int NC, int FLUX>
+static hipError_t launch_adj_chunk(const AdjArgs &A, const DevParams &P, hipStream_t st)
+{
+    // FP32 only: the FP64 face phase does not fit the registers
+    if constexpr (sizeof(R) != 4) return hipErrorInvalidValue;
+    if (A.chunk_elems > 256) return hipErrorInvalidValue;
+    return A.chunk_elems > 128 ? launch_adj_chunk_bs<R, NC, FLUX, 256>(A, P, st)
+                               : launch_adj_chunk_bs<R, NC, FLUX, 128>(A, P, st);
+}
+
 template <typename R, int NC>
 static hipError_t launch_adj_nc(int flux, const AdjArgs &A, const DevParams &P, hipStream_t st)
 {
+    if (sizeof(R) == 4 && A.cf4) {
+        hipError_t err;
+        switch (flux) {
+        case FLUX_LF: err = launch_adj_chunk<R, NC, FLUX_LF>(A, P, st); break;
+        case FLUX_HLLC: err = launch_adj_chunk<R, NC, FLUX_HLLC>(A, P, st); break;
+        case FLUX_CD: err = launch_adj_chunk<R, NC, FLUX_CD>(A, P, st); break;
+        default: return hipErrorInvalidValue;
+        }
+        if (err != hipSuccess) return err;
+        return launch_adj_bnd<R, NC>(flux, A, P, st);
+    }
     const dim3 grid((unsigned) ((A.n + 127) / 128)), block(128);
-    const dim3 gb((unsigned) ((A.n_bnd + 63) / 64)), bb(64);
     switch (flux) {
     case FLUX_LF: hipLaunchKernelGGL((k_adj_stage<R, NC, FLUX_LF>), grid, block, 0, st, A, P); break;
     case FLUX_HLLC: hipLaunchKernelGGL((k_adj_stage<R, NC, FLUX_HLLC>), grid, block, 0, st, A, P); break;
@@ -423,13 +665,8 @@ static hipError_t launch_adj_nc(int flux, const AdjArgs &A, const DevParams &P, 
     default: return hipErrorInvalidValue;
     }
     hipError_t err = hipGetLastError();
-    if (err != hipSuccess || A.n_bnd <= 0) return err;
-    switch (flux) {
-    case FLUX_LF: hipLaunchKernelGGL((k_adj_bnd<R, NC, FLUX_LF>), gb, bb, 0, st, A, P); break;
-    case FLUX_HLLC: hipLaunchKernelGGL((k_adj_bnd<R, NC, FLUX_HLLC>), gb, bb, 0, st, A, P); break;
-    default: hipLaunchKernelGGL((k_adj_bnd<R, NC, FLUX_CD>), gb, bb, 0, st, A, P); break;
-    }
-    return hipGetLastError();
+    if (err != hipSuccess) return err;
+    return launch_adj_bnd<R, NC>(flux, A, P, st);
 }
 
 template <typename R>
@@ -440,6 +677,11 @@ static hipError_t launch_adj_r(int nc, int flux, const AdjArgs &A, const DevPara
     case 2: return launch_adj_nc<R, 2>(flux, A, P, st);
     }
     return hipErrorInvalidValue;
+}
+
+size_t adj_chunk_smem_bytes(int prec, int nc, int max_faces)
+{
+    return (size_t) max_faces * (6 * (4 + nc) * (prec ? 4 : 8) + sizeof(int));
 }
 
 hipError_t launch_adj_stage(int prec, int nc, int flux, const AdjArgs &A, const DevParams &P, hipStream_t st)

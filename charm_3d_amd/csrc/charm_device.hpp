@@ -54,6 +54,16 @@ template <typename R> __device__ __forceinline__ R cst(double d, float) { return
 template <> __device__ __forceinline__ double cst<double>(double d, float) { return d; }
 template <> __device__ __forceinline__ float cst<float>(double, float f) { return f; }
 
+// Workgroups are dispatched round-robin over the XCDs (each with its own
+// L2): workgroup b runs on XCD b % n_xcd.  Renumber so that every XCD walks a
+// contiguous range of Morton-ordered chunks / element blocks and neighbours
+// share its L2.  The grid is n_xcd * ceil(n / n_xcd) workgroups.
+__device__ __forceinline__ int xcd_block(int n_xcd)
+{
+    const int per = (int) gridDim.x / n_xcd;
+    return ((int) blockIdx.x % n_xcd) * per + (int) blockIdx.x / n_xcd;
+}
+
 // Reciprocal, quotient and square root.  The FP32 bulk path uses the
 // hardware v_rcp_f32 / v_sqrt_f32 (about 1 ulp; operands here are normal
 // numbers), the FP64 path IEEE division and sqrt.

@@ -913,39 +913,68 @@ extern "C" int charm_gpu_objective(charm_gpu_ctx_t *c, double dt, int nsteps, co
     return rc;
 }
 
-// device list of the leaves with a boundary face (from the neighbour table:
-// codes -1..-7 are boundary faces)
-static int adj_boundary_leaves(charm_gpu_ctx_t *c, int **out, int *count)
+// Boundary faces of the mesh for the adjoint's boundary kernels (from the
+// neighbour table: codes -1..-6 are boundary faces): the leaves with one,
+// their face ranges, the faces (leaf * 8 + direction, ordered by leaf), and
+// the per-point scratch of their contributions.
+struct AdjBoundary {
+    int *elems, *ptr, *faces;
+    void *part;
+    double *theta;
+    int n_elems, n_faces;
+};
+
+static void adj_boundary_free(AdjBoundary &b)
 {
+    void *bufs[] = {b.elems, b.ptr, b.faces, b.part, b.theta};
+    for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); i++)
+        if (bufs[i]) (void) hipFree(bufs[i]);
+    memset(&b, 0, sizeof(b));
+}
+
+static int adj_boundary(charm_gpu_ctx_t *c, AdjBoundary &b)
+{
+    memset(&b, 0, sizeof(b));
     const int64_t n = c->n;
     int *nbr = (int *) malloc((size_t) n * 6 * sizeof(int));
-    int *list = (int *) malloc((size_t) n * sizeof(int));
-    if (!nbr || !list) {
-        free(nbr); free(list);
+    int *elems = (int *) malloc((size_t) n * sizeof(int));
+    int *ptr = (int *) malloc(((size_t) n + 1) * sizeof(int));
+    int *faces = (int *) malloc((size_t) n * 6 * sizeof(int));
+    if (!nbr || !elems || !ptr || !faces) {
+        free(nbr); free(elems); free(ptr); free(faces);
         charm_set_error("adjoint: out of memory");
         return CHARM_ERR_MEM;
     }
     hipError_t e = hipMemcpyAsync(nbr, c->d_nbr, (size_t) n * 6 * sizeof(int), hipMemcpyDeviceToHost, c->st);
     if (e == hipSuccess) e = hipStreamSynchronize(c->st);
-    int m = 0;
+    int ne = 0, nf = 0;
     if (e == hipSuccess) {
+        ptr[0] = 0;
         for (int64_t i = 0; i < n; i++) {
-            bool b = false;
-            for (int d = 0; d < 6; d++) b |= nbr[6 * i + d] < 0 && nbr[6 * i + d] > -8;
-            if (b) list[m++] = (int) i;
+            const int f0 = nf;
+            for (int d = 0; d < 6; d++)
+                if (nbr[6 * i + d] < 0 && nbr[6 * i + d] > -8) faces[nf++] = (int) (i * 8 + d);
+            if (nf > f0) { elems[ne] = (int) i; ptr[++ne] = nf; }
         }
     }
-    *out = NULL;
-    if (e == hipSuccess && m > 0) e = hipMalloc((void **) out, (size_t) m * sizeof(int));
-    if (e == hipSuccess && m > 0) e = hipMemcpy(*out, list, (size_t) m * sizeof(int), hipMemcpyHostToDevice);
-    free(nbr); free(list);
+    if (e == hipSuccess && ne > 0) {
+        e = hipMalloc((void **) &b.elems, (size_t) ne * sizeof(int));
+        if (e == hipSuccess) e = hipMalloc((void **) &b.ptr, ((size_t) ne + 1) * sizeof(int));
+        if (e == hipSuccess) e = hipMalloc((void **) &b.faces, (size_t) nf * sizeof(int));
+        if (e == hipSuccess) e = hipMalloc(&b.part, (size_t) nf * 6 * 4 * c->nv * c->esz);
+        if (e == hipSuccess) e = hipMalloc((void **) &b.theta, (size_t) nf * 6 * 3 * sizeof(double));
+        if (e == hipSuccess) e = hipMemcpy(b.elems, elems, (size_t) ne * sizeof(int), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(b.ptr, ptr, ((size_t) ne + 1) * sizeof(int), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(b.faces, faces, (size_t) nf * sizeof(int), hipMemcpyHostToDevice);
+    }
+    free(nbr); free(elems); free(ptr); free(faces);
     if (e != hipSuccess) {
-        if (*out) (void) hipFree(*out);
-        *out = NULL;
+        adj_boundary_free(b);
         charm_set_error("adjoint: %s", hipGetErrorString(e));
         return CHARM_ERR_HIP;
     }
-    *count = m;
+    b.n_elems = ne;
+    b.n_faces = nf;
     return CHARM_OK;
 }
 
@@ -962,8 +991,15 @@ extern "C" int charm_gpu_adjoint(charm_gpu_ctx_t *c, double dt, int nsteps, int 
     const int ncp = (nsteps + ckpt - 1) / ckpt;
     char *cp = NULL, *seg = NULL, *la = NULL, *lb = NULL, *lc = NULL, *uend = NULL;
     double *part = NULL, *theta = NULL;
-    int *bnd = NULL, n_bnd = 0;
-    if ((rc = adj_boundary_leaves(c, &bnd, &n_bnd))) return rc;
+    AdjBoundary bnd;
+    if ((rc = adj_boundary(c, bnd))) return rc;
+    // FP32: the chunked face-centric stage adjoint when its LDS fits a
+    // workgroup (else, and in FP64, the element-centric one)
+    int dev = 0, lds_max = 0;
+    const bool chunked = c->sprec && hipGetDevice(&dev) == hipSuccess &&
+                         hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
+                         adj_chunk_smem_bytes(1, c->nc, c->max_faces) <= (size_t) lds_max;
+    (void) hipGetLastError();
     hipError_t e = hipMalloc((void **) &cp, (size_t) ncp * ssz);
     if (e == hipSuccess) e = hipMalloc((void **) &seg, (size_t) 3 * ckpt * ssz);
     if (e == hipSuccess) e = hipMalloc((void **) &la, ssz);
@@ -1011,7 +1047,12 @@ extern "C" int charm_gpu_adjoint(charm_gpu_ctx_t *c, double dt, int nsteps, int 
             AdjArgs A;
             A.level = c->d_level; A.nbr = c->d_nbr; A.aux = c->d_aux; A.n = n; A.dt = dt;
             A.theta_bar = theta_face >= 0 ? theta : NULL; A.theta_face = theta_face;
-            A.bnd_elems = bnd; A.n_bnd = n_bnd;
+            A.bnd_elems = bnd.elems; A.bnd_ptr = bnd.ptr; A.bnd_faces = bnd.faces;
+            A.n_bnd = bnd.n_elems; A.n_bface = bnd.n_faces; A.bnd_part = bnd.part; A.bnd_theta = bnd.theta;
+            A.cf4 = chunked ? c->d_cf4 : NULL;
+            A.ch_ebeg = c->d_ch_ebeg; A.ch_fbeg = c->d_ch_fbeg; A.slot_ptr = c->d_slot_ptr; A.slots = c->d_slots;
+            A.n_chunks = (int) c->nch; A.max_faces = c->max_faces; A.chunk_elems = c->chunk_elems;
+            A.n_xcd = c->n_xcd;
             A.U = u2; A.X = la; A.ACC = NULL; A.OUT = lb; A.c_in = 2.0 / 3.0; A.c_out = 1.0; A.c_acc = 0.0;
             ADJ_CHK(launch_adj_stage(c->sprec, c->nc, c->cfg.flux, A, c->P, c->st));
             ADJ_CHK(launch_adj_axpby(c->sprec, lb, la, 0.75, 1.0 / 3.0, m, c->st));
@@ -1046,8 +1087,9 @@ extern "C" int charm_gpu_adjoint(charm_gpu_ctx_t *c, double dt, int nsteps, int 
 #undef ADJ_CHK
 done:
     (void) hipStreamSynchronize(c->st);
-    void *bufs[] = {cp, seg, la, lb, lc, uend, part, theta, bnd};
+    void *bufs[] = {cp, seg, la, lb, lc, uend, part, theta};
     for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); i++)
         if (bufs[i]) (void) hipFree(bufs[i]);
+    adj_boundary_free(bnd);
     return rc;
 }

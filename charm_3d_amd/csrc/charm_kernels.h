@@ -101,7 +101,17 @@ struct AdjArgs {
     int theta_face;           // the inlet face of the parameters (-1: none)
     int64_t n;
     const int *bnd_elems;     // leaves with at least one boundary face
-    int n_bnd;
+    const int *bnd_ptr;       // [n_bnd + 1]: their ranges in bnd_faces
+    const int *bnd_faces;     // boundary faces, leaf * 8 + direction, by leaf
+    int n_bnd, n_bface;
+    void *bnd_part;           // [n_bface][6][4 V] point contributions (state type)
+    double *bnd_theta;        // [n_bface][6][3] parameter contributions
+    // chunked face-centric variant (chunks as for the stage kernel; null
+    // cf4: the element-centric kernel)
+    const int4 *cf4;
+    const int *ch_ebeg, *ch_fbeg, *slot_ptr;
+    const uint16_t *slots;
+    int n_chunks, max_faces, chunk_elems, n_xcd;
 };
 
 // objective over a box of leaves: mode 0 per-block partial sums of
@@ -118,6 +128,8 @@ struct ObjArgs {
 };
 
 hipError_t launch_adj_stage(int prec, int nc, int flux, const AdjArgs &A, const DevParams &P, hipStream_t st);
+// LDS bytes per workgroup of the chunked adjoint stage kernel
+size_t adj_chunk_smem_bytes(int prec, int nc, int max_faces);
 hipError_t launch_adj_objective(int prec, int nc, const ObjArgs &A, const DevParams &P, hipStream_t st);
 hipError_t launch_adj_axpby(int prec, const void *x, void *y, double a, double b, int64_t m, hipStream_t st);
 

@@ -26,16 +26,6 @@
 
 namespace charm {
 
-// Workgroups are dispatched round-robin over the XCDs (each with its own
-// L2): workgroup b runs on XCD b % n_xcd.  Renumber so that every XCD walks a
-// contiguous range of Morton-ordered chunks / element blocks and neighbours
-// share its L2.  The grid is n_xcd * ceil(n / n_xcd) workgroups.
-__device__ __forceinline__ int xcd_block(int n_xcd)
-{
-    const int per = (int) gridDim.x / n_xcd;
-    return ((int) blockIdx.x % n_xcd) * per + (int) blockIdx.x / n_xcd;
-}
-
 // SSP-RK combines rk_1 (c = c/4 + 3 c_old/4) and rk_2 (c = 2c/3 + c_old/3)
 // in the reference's operation order, src/models/ns/charm_model.c:136-184
 template <typename R>
