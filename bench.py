@@ -7,9 +7,13 @@ FP32-bulk and the FP64 plain paths run K steps from the same initial state;
 value = DOF-updates/s of the FP32 path (E * 4 * V * 3 stages * K / wall).
 
     python bench.py [--steps K] [--warmup W] [--level L] [--no-fp64] [--no-cpu] [--no-extra]
+                    [--no-traffic]
 
 The JSON line also carries the other configurations at full size (adaptive
-bubbles, reacting hot spot, adjoint of the pulse case) under "workloads".
+bubbles, reacting hot spot, adjoint of the pulse case) under "workloads",
+the FP32 errors of the smooth cases, the FP64 path's own sensitivity to the
+FP32 rounding of the initial state, and the stage kernel's HBM traffic per
+launch from two rocprofv3 counter passes (child processes).
 """
 import argparse
 import json
@@ -88,6 +92,100 @@ def cpu_baseline(W, budget_s):
             "sample": "FP64 oracle (C restatement of the reference numerics, AoS records, "
                       "callback face traversal), Riemann box at level 4 (4,096 hexes), "
                       "HLLC + BJ, SSP-RK3, %d steps in %.1f s on one host core" % (steps, t)}
+
+
+def rel_errors(a, ref):
+    d = a - ref
+    return {"l2_rel": float(np.linalg.norm(d) / np.linalg.norm(ref)),
+            "linf_rel": float(np.abs(d).max() / np.abs(ref).max()),
+            "l2_rel_means": float(np.linalg.norm(d[:, :, 0]) / np.linalg.norm(ref[:, :, 0]))}
+
+
+def smooth_errors(CA, W):
+    """FP32 bulk against the FP64 plain path on the two smooth
+    configurations: the advected sine wave (level 4, LF, periodic, 100
+    steps) and the Gaussian pulse (level 6, HLLC, inlet, 400 steps)."""
+    out = {}
+    cases = (("sine_wave_level4_lf_100_steps", lambda prec: W.c1(4, prec), 100),
+             ("gaussian_pulse_level6_hllc_400_steps", lambda prec: W.c5_pulse(6, prec), 400))
+    for name, mk, steps in cases:
+        res = {}
+        for prec in (CA.FP64, CA.FP32):
+            cfg, leaves, u0, per = mk(prec)
+            ctx = CA.GpuContext(cfg)
+            ce, cf = CA.recommended_chunks(cfg)
+            ctx.upload_mesh(CA.Forest.from_leaves(leaves, per).flatten(ce, cf))
+            ctx.set_state(u0)
+            dt = ctx.get_dt(1) if prec == CA.FP64 else res["dt"]
+            m0 = ctx.conservation_sums()
+            ctx.run(dt, steps)
+            m1 = ctx.conservation_sums()
+            res[prec] = ctx.get_state()
+            res["dt"] = dt
+            res["drift%d" % prec] = abs(m1[4] - m0[4]) / abs(m0[4])
+            ctx.close()
+        e = rel_errors(res[CA.FP32], res[CA.FP64])
+        e["mass_drift_fp32"] = res["drift%d" % CA.FP32]
+        e["mass_drift_fp64"] = res["drift%d" % CA.FP64]
+        out[name] = e
+    return out
+
+
+def traffic_probe(CA, W, level):
+    """Child of the traffic measurement: a few stage-kernel and copy-kernel
+    launches on the benchmark configuration, nothing else."""
+    cfg, leaves, u0, per = W.c2(level=level, precision=CA.FP32)
+    ctx = CA.GpuContext(cfg)
+    ce, cf = CA.recommended_chunks(cfg)
+    ctx.upload_mesh(CA.Forest.from_leaves(leaves, per).flatten(ce, cf))
+    ctx.set_state(u0)
+    dt = ctx.get_dt(mode=1)
+    ctx.time_kernels(dt, 3)
+    ctx.time_copy(3)
+    ctx.synchronize()
+
+
+def measure_traffic(level, copy_bytes):
+    """HBM bytes per stage-kernel launch from two rocprofv3 counter passes
+    (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each a child process
+    running traffic_probe; each counter is scaled by what it reads for the
+    copy kernel of known size (copy_bytes / 2 read, copy_bytes / 2 written).
+    Returns None when rocprofv3 is absent or a pass fails."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3")
+    if not prof:
+        return None
+    res = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+            d = os.path.join(tmp, ctr)
+            cmd = [prof, "--pmc", ctr, "--output-format", "csv", "-d", d, "-o", "p", "--",
+                   sys.executable, os.path.abspath(__file__), "--traffic-probe", "--level", str(level)]
+            print("[bench] counter pass %s" % ctr, file=sys.stderr, flush=True)
+            try:
+                r = subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=120)
+            except subprocess.TimeoutExpired:
+                return None
+            path = os.path.join(d, "p_counter_collection.csv")
+            if r.returncode != 0 or not os.path.exists(path):
+                return None
+            vals = {"stage": [], "copy": []}
+            for row in csv.DictReader(open(path)):
+                if row["Counter_Name"] != ctr:
+                    continue
+                k = "stage" if "k_stage" in row["Kernel_Name"] else ("copy" if "k_copy16" in row["Kernel_Name"] else None)
+                if k:
+                    vals[k].append(float(row["Counter_Value"]) * 1024.0)
+            if not vals["stage"] or not vals["copy"]:
+                return None
+            res[ctr] = (sum(vals["stage"]) / len(vals["stage"]), sum(vals["copy"]) / len(vals["copy"]))
+    half = copy_bytes / 2.0
+    f_read, f_write = half / res["FETCH_SIZE"][1], half / res["WRITE_SIZE"][1]
+    return {"read": res["FETCH_SIZE"][0] * f_read, "write": res["WRITE_SIZE"][0] * f_write,
+            "fetch_scale": f_read, "write_scale": f_write}
 
 
 def extra_workloads(CA, W):
@@ -182,6 +280,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--no-traffic", action="store_true")
+    ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -190,6 +290,10 @@ def main():
 
     import charm_3d_amd as CA
     import charm_3d_amd.workloads as W
+
+    if args.traffic_probe:
+        traffic_probe(CA, W, args.level)
+        return
 
     print("[bench] FP32 bulk path, level %d, %d steps" % (args.level, args.steps), file=sys.stderr, flush=True)
     r32 = run_path(CA, W, args.level, CA.FP32, args.steps, args.warmup)
@@ -252,6 +356,13 @@ def main():
         out["errors_vs_fp64"] = {"l2_rel": l2, "linf_rel": linf,
                                  "l2_rel_means": float(np.linalg.norm(d[:, :, 0]) / np.linalg.norm(r64["state"][:, :, 0]))}
         out["speedup_fp32_over_fp64"] = r64["wall"] / r32["wall"]
+        # conditioning of the case: the FP64 path from the initial state
+        # rounded to FP32 (the only difference), against the FP64 run
+        print("[bench] FP64 path from the FP32-rounded initial state", file=sys.stderr, flush=True)
+        u0r = r32["u0"].astype(np.float32).astype(np.float64)
+        r64r = run_path(CA, W, args.level, CA.FP64, args.steps, 0, u0=u0r)
+        out["fp64_from_fp32_rounded_input"] = rel_errors(r64r["state"], r64["state"])
+        del r64r
         dm = rmx["state"] - r64["state"]
         out["fp32_mix"] = {"value": dofs / rmx["wall"], "ms_per_step": 1e3 * rmx["wall"] / args.steps,
                            "mass_drift": abs(rmx["m1"][4] - rmx["m0"][4]) / abs(rmx["m0"][4]),
@@ -260,9 +371,17 @@ def main():
         del r64
 
     if not args.no_extra:
+        print("[bench] smooth cases, FP32 against FP64", file=sys.stderr, flush=True)
+        out["smooth_errors_vs_fp64"] = smooth_errors(CA, W)
         print("[bench] bubble, hot-spot and adjoint cases", file=sys.stderr, flush=True)
         out["workloads"] = extra_workloads(CA, W)
         out["adjoint_over_forward"] = out["workloads"]["pulse_adjoint"].get("adjoint_over_forward")
+
+    if not args.no_traffic:
+        tr = measure_traffic(args.level, copy_bytes)
+        if tr:
+            out["roofline"]["traffic"] = tr["read"] + tr["write"]
+            out["roofline"]["traffic_detail"] = tr
 
     if not args.no_cpu:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)

@@ -989,7 +989,9 @@ extern "C" int charm_gpu_adjoint(charm_gpu_ctx_t *c, double dt, int nsteps, int 
     const size_t ssz = (size_t) n * 4 * c->nv * c->esz;
     const int64_t m = n * 4 * c->nv;
     const int ncp = (nsteps + ckpt - 1) / ckpt;
-    char *cp = NULL, *seg = NULL, *la = NULL, *lb = NULL, *lc = NULL, *uend = NULL;
+    char *cp = NULL, *seg = NULL, *la = NULL, *lb = NULL, *lc = NULL, *segb[2] = {NULL, NULL};
+    hipStream_t st2 = NULL;
+    hipEvent_t ev_ready[2] = {NULL, NULL}, ev_done[2] = {NULL, NULL};
     double *part = NULL, *theta = NULL;
     AdjBoundary bnd;
     if ((rc = adj_boundary(c, bnd))) return rc;
@@ -1001,11 +1003,17 @@ extern "C" int charm_gpu_adjoint(charm_gpu_ctx_t *c, double dt, int nsteps, int 
                          adj_chunk_smem_bytes(1, c->nc, c->max_faces) <= (size_t) lds_max;
     (void) hipGetLastError();
     hipError_t e = hipMalloc((void **) &cp, (size_t) ncp * ssz);
-    if (e == hipSuccess) e = hipMalloc((void **) &seg, (size_t) 3 * ckpt * ssz);
+    if (e == hipSuccess) e = hipMalloc((void **) &seg, (size_t) 2 * (3 * ckpt + 1) * ssz);
+    if (e == hipSuccess) e = hipStreamCreate(&st2);
+    for (int i = 0; i < 2 && e == hipSuccess; i++) {
+        e = hipEventCreateWithFlags(&ev_ready[i], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ev_done[i], hipEventDisableTiming);
+    }
+    segb[0] = seg;
+    segb[1] = seg ? seg + (size_t) (3 * ckpt + 1) * ssz : NULL;
     if (e == hipSuccess) e = hipMalloc((void **) &la, ssz);
     if (e == hipSuccess) e = hipMalloc((void **) &lb, ssz);
     if (e == hipSuccess) e = hipMalloc((void **) &lc, ssz);
-    if (e == hipSuccess) e = hipMalloc((void **) &uend, ssz);
     if (e == hipSuccess) e = hipMalloc((void **) &part, (size_t) nsteps * nblk * sizeof(double));
     if (e == hipSuccess) e = hipMalloc((void **) &theta, (size_t) n * 3 * sizeof(double));
     float ms_f = 0.f, ms_a = 0.f;
@@ -1021,26 +1029,44 @@ extern "C" int charm_gpu_adjoint(charm_gpu_ctx_t *c, double dt, int nsteps, int 
     ADJ_CHK(hipEventSynchronize(c->ev1));
     ADJ_CHK(hipEventElapsedTime(&ms_f, c->ev0, c->ev1));
 
+    // Reverse sweep.  Segment sg's stage states are recomputed from its
+    // checkpoint into one of two segment buffers on a second stream while
+    // the adjoint of segment sg + 1 runs on the context stream from the
+    // other buffer (events order the reuse of a buffer); per segment buffer
+    // slot j of S0 holds u^(start+j) (slot 0 = the checkpoint, slot L the
+    // segment's end state), S1 / S2 the stage states.
     ADJ_CHK(hipEventRecord(c->ev0, c->st));
     ADJ_CHK(hipMemsetAsync(la, 0, ssz, c->st));
     ADJ_CHK(hipMemsetAsync(theta, 0, (size_t) n * 3 * sizeof(double), c->st));
-    for (int sg = ncp - 1; sg >= 0; sg--) {
-        const int start = sg * ckpt, end = (start + ckpt < nsteps) ? start + ckpt : nsteps, L = end - start;
-        char *S0 = seg, *S1 = seg + (size_t) ckpt * ssz, *S2 = seg + (size_t) 2 * ckpt * ssz;
-        ADJ_CHK(hipMemcpyAsync(S0, cp + (size_t) sg * ssz, ssz, hipMemcpyDeviceToDevice, c->st));
-        for (int j = 0; j < L; j++) {
-            void *u0 = S0 + (size_t) j * ssz, *u1 = S1 + (size_t) j * ssz, *u2 = S2 + (size_t) j * ssz;
-            void *un = (j + 1 < L) ? (void *) (S0 + (size_t) (j + 1) * ssz) : (void *) uend;
-            StageArgs A = stage_args(c, u0, u0, u1, 1, 0, dt);
-            ADJ_CHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st));
-            A = stage_args(c, u1, u0, u2, 1, 1, dt);
-            ADJ_CHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st));
-            A = stage_args(c, u2, u0, un, 1, 2, dt);
-            ADJ_CHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st));
+    ADJ_CHK(hipEventRecord(ev_done[0], c->st));
+    ADJ_CHK(hipEventRecord(ev_done[1], c->st));
+    for (int sg = ncp - 1; sg >= -1; sg--) {
+        // recompute segment sg - 1 ahead (sg = ncp - 1: segments ncp - 1 and ncp - 2)
+        for (int r = (sg == ncp - 1 ? sg : sg - 1); r >= sg - 1 && r >= 0; r--) {
+            const int start = r * ckpt, end = (start + ckpt < nsteps) ? start + ckpt : nsteps, L = end - start;
+            char *S0 = segb[r & 1], *S1 = S0 + (size_t) (ckpt + 1) * ssz, *S2 = S1 + (size_t) ckpt * ssz;
+            // the buffer's previous user (the adjoint of segment r + 2) is done
+            ADJ_CHK(hipStreamWaitEvent(st2, ev_done[r & 1], 0));
+            ADJ_CHK(hipMemcpyAsync(S0, cp + (size_t) r * ssz, ssz, hipMemcpyDeviceToDevice, st2));
+            for (int j = 0; j < L; j++) {
+                void *u0 = S0 + (size_t) j * ssz, *u1 = S1 + (size_t) j * ssz, *u2 = S2 + (size_t) j * ssz;
+                void *un = S0 + (size_t) (j + 1) * ssz;
+                StageArgs A = stage_args(c, u0, u0, u1, 1, 0, dt);
+                ADJ_CHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, st2));
+                A = stage_args(c, u1, u0, u2, 1, 1, dt);
+                ADJ_CHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, st2));
+                A = stage_args(c, u2, u0, un, 1, 2, dt);
+                ADJ_CHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, st2));
+            }
+            ADJ_CHK(hipEventRecord(ev_ready[r & 1], st2));
         }
+        if (sg < 0) break;
+        const int start = sg * ckpt, end = (start + ckpt < nsteps) ? start + ckpt : nsteps, L = end - start;
+        char *S0 = segb[sg & 1], *S1 = S0 + (size_t) (ckpt + 1) * ssz, *S2 = S1 + (size_t) ckpt * ssz;
+        ADJ_CHK(hipStreamWaitEvent(c->st, ev_ready[sg & 1], 0));
         for (int j = L - 1; j >= 0; j--) {
             void *u0 = S0 + (size_t) j * ssz, *u1 = S1 + (size_t) j * ssz, *u2 = S2 + (size_t) j * ssz;
-            void *un = (j + 1 < L) ? (void *) (S0 + (size_t) (j + 1) * ssz) : (void *) uend;
+            void *un = S0 + (size_t) (j + 1) * ssz;
             ObjArgs O = obj_args(c, un, lo, hi);
             O.mode = 1; O.lambda = la; O.coef = dt;
             ADJ_CHK(launch_adj_objective(c->sprec, c->nc, O, c->P, c->st));
@@ -1061,6 +1087,7 @@ extern "C" int charm_gpu_adjoint(charm_gpu_ctx_t *c, double dt, int nsteps, int 
             A.U = u0; A.X = lc; A.ACC = la; A.OUT = la; A.c_in = 1.0; A.c_acc = 1.0;
             ADJ_CHK(launch_adj_stage(c->sprec, c->nc, c->cfg.flux, A, c->P, c->st));
         }
+        ADJ_CHK(hipEventRecord(ev_done[sg & 1], c->st));
     }
     ADJ_CHK(hipEventRecord(c->ev1, c->st));
     ADJ_CHK(hipEventSynchronize(c->ev1));
@@ -1087,7 +1114,12 @@ extern "C" int charm_gpu_adjoint(charm_gpu_ctx_t *c, double dt, int nsteps, int 
 #undef ADJ_CHK
 done:
     (void) hipStreamSynchronize(c->st);
-    void *bufs[] = {cp, seg, la, lb, lc, uend, part, theta};
+    if (st2) { (void) hipStreamSynchronize(st2); (void) hipStreamDestroy(st2); }
+    for (int i = 0; i < 2; i++) {
+        if (ev_ready[i]) (void) hipEventDestroy(ev_ready[i]);
+        if (ev_done[i]) (void) hipEventDestroy(ev_done[i]);
+    }
+    void *bufs[] = {cp, seg, la, lb, lc, part, theta};
     for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); i++)
         if (bufs[i]) (void) hipFree(bufs[i]);
     adj_boundary_free(bnd);
