@@ -102,15 +102,16 @@ def rel_errors(a, ref):
 
 
 def smooth_errors(CA, W):
-    """FP32 bulk against the FP64 plain path on the two smooth
-    configurations: the advected sine wave (level 4, LF, periodic, 100
-    steps) and the Gaussian pulse (level 6, HLLC, inlet, 400 steps)."""
+    """FP32 bulk and FP32 mixed (FP64 state, FP32 residual) against the
+    FP64 plain path on the two smooth configurations: the advected sine
+    wave (level 4, LF, periodic, 100 steps) and the Gaussian pulse (level 6,
+    HLLC, inlet, 400 steps)."""
     out = {}
     cases = (("sine_wave_level4_lf_100_steps", lambda prec: W.c1(4, prec), 100),
              ("gaussian_pulse_level6_hllc_400_steps", lambda prec: W.c5_pulse(6, prec), 400))
     for name, mk, steps in cases:
         res = {}
-        for prec in (CA.FP64, CA.FP32):
+        for prec in (CA.FP64, CA.FP32, CA.FP32_MIX):
             cfg, leaves, u0, per = mk(prec)
             ctx = CA.GpuContext(cfg)
             ce, cf = CA.recommended_chunks(cfg)
@@ -127,6 +128,8 @@ def smooth_errors(CA, W):
         e = rel_errors(res[CA.FP32], res[CA.FP64])
         e["mass_drift_fp32"] = res["drift%d" % CA.FP32]
         e["mass_drift_fp64"] = res["drift%d" % CA.FP64]
+        e["fp32_mix"] = rel_errors(res[CA.FP32_MIX], res[CA.FP64])
+        e["fp32_mix"]["mass_drift"] = res["drift%d" % CA.FP32_MIX]
         out[name] = e
     return out
 

@@ -50,7 +50,8 @@ class Config(C.Structure):
                 ("max_err", C.c_double), ("min_level", C.c_int), ("max_level", C.c_int),
                 ("refine_period", C.c_int), ("use_visc", C.c_int), ("use_diff", C.c_int),
                 ("t_ref", C.c_double), ("n_react", C.c_int), ("react", ReactionDesc * 4),
-                ("chem_parity", C.c_int)]
+                ("chem_parity", C.c_int), ("write_period", C.c_int),
+                ("comp_name", (C.c_char * 32) * MAX_COMP), ("out_prefix", C.c_char * 256)]
 
 
 _lib = None
@@ -103,6 +104,8 @@ def lib():
         "charm_gpu_synchronize": (C.c_int, [vp]),
         "charm_gpu_last_timing": (C.c_int, [vp, dp, dp]),
         "charm_gpu_flux_batch": (C.c_int, [vp, C.c_int, i64, dp, dp, dp, dp]),
+        "charm_write_vtk": (C.c_int, [C.c_char_p, i64, i32p, dp, C.POINTER(Config)]),
+        "charm_write_solution": (None, [vp]),
         "charm_gpu_amr_indicator": (C.c_int, [vp, dp, dp, i64]),
         "charm_gpu_ns_aux": (C.c_int, [vp, dp, i64]),
         "charm_gpu_objective": (C.c_int, [vp, C.c_double, C.c_int, dp, dp, dp]),
@@ -156,7 +159,8 @@ def make_config(precision=FP32, model="NS", flux="HLLC", limiter=False, eos="IDE
                 comp_m=(GAS_R,), comp_cp=(3.5,), bnd=("OUTLET",) * 6, bnd_param=None,
                 grav=(0.0, 0.0, 0.0), cfl=0.3, dt=0.0, tmax=0.0, chunk_elems=0, chunk_faces=0,
                 max_err=0.0, min_level=0, max_level=0, refine_period=0, transport=None,
-                use_visc=False, use_diff=False, t_ref=298.15, reactions=(), chem_parity=False):
+                use_visc=False, use_diff=False, t_ref=298.15, reactions=(), chem_parity=False,
+                write_period=0, comp_names=(), out_prefix=""):
     """charm_config_t from task.yaml-like keys.  comp_cp entries are a constant
     cp or a tuple of power-series coefficients (polynomial cp, which as in
     the reference takes effect with Sutherland viscosity, ml_type 1);
@@ -200,7 +204,21 @@ def make_config(precision=FP32, model="NS", flux="HLLC", limiter=False, eos="IDE
             cfg.react[j].right[m] = right[m]
         cfg.react[j].a, cfg.react[j].e, cfg.react[j].n = a, e, n
     cfg.chem_parity = int(chem_parity)
+    cfg.write_period = write_period
+    for i, nm in enumerate(comp_names):
+        cfg.comp_name[i].value = nm.encode()[:31]
+    cfg.out_prefix = out_prefix.encode()[:255]
     return cfg
+
+
+def write_vtk(name, leaves, u, cfg):
+    """charm_write_vtk: <name>_0000.vtu + <name>.pvtu of leaves [n][4] and
+    the FP64 state u [n][V][4]."""
+    leaves = np.ascontiguousarray(leaves, dtype=np.int32)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    n = len(leaves)
+    check(lib().charm_write_vtk(name.encode(), n, leaves.ctypes.data_as(C.POINTER(C.c_int32)),
+                                u.ctypes.data_as(C.POINTER(C.c_double)), C.byref(cfg)))
 
 
 def level_mass_inverse(level):
@@ -466,6 +484,13 @@ class Solver:
         out = np.zeros((self.forest.n, self.nv, 4))
         check(lib().charm_gpu_get_state(self.gpu, _dptr(out), len(out)))
         return out
+
+    def write_solution(self):
+        """charm_write_solution: VTK files of the current state."""
+        lib().charm_write_solution(self.forest.h)
+        err = lib().charm_ctx_error(self.h)
+        if err:
+            check(err)
 
     def run(self, dt, nsteps):
         """nsteps device steps with a fixed dt (no adaptation)."""

@@ -96,12 +96,16 @@ static void timestep(charm_forest_t *p4est, int model, charm_real_t *dt)
         if (c->err) return;
     }
     *dt = model == CHARM_MODEL_EULER ? charm_model_euler_get_dt(p4est) : charm_model_ns_get_dt(p4est);
+    /* write out the solution before the step (src/models/euler/charm_model.c:507-510) */
+    if (c->cfg.write_period > 0 && c->timestep % c->cfg.write_period == 0) {
+        charm_write_solution(p4est);
+        if (c->err) return;
+    }
     rc = charm_gpu_step(c->gpu, model, *dt, 1);
     if (rc) c->err = rc;
 }
 
-/* src/models/euler/charm_model.c:454-540 (the partition and write hooks are
- * outside this path) */
+/* src/models/euler/charm_model.c:454-540 (single rank: no repartition) */
 void charm_model_euler_timestep_single(charm_forest_t *p4est, charm_real_t *dt,
                                        charm_ghost_t **ghost, charm_data_t **ghost_data)
 {

@@ -111,6 +111,10 @@ typedef struct {
     charm_reaction_desc_t react[4];
     int    chem_parity;        /* 1: the reference's chem_w[species] index quirk
                                   (src/models/ns/charm_model_ns_chem.c:97) */
+    /* output (src/charm_init_yaml.cpp:388, include/charm_globals.h:294) */
+    int    write_period;       /* FILE_OUTPUT_STEP: VTK files every that many steps (0: none) */
+    char   comp_name[CHARM_MAX_COMP][32];   /* components' names: cell data C_<name> */
+    char   out_prefix[256];    /* file prefix ("" : "charm_3d", CHARM_STRING) */
 } charm_config_t;
 
 /* ------------------------------------------------------------ forest */
@@ -247,6 +251,13 @@ int charm_amr_flags(int64_t n, const int32_t *leaves, const double *err2, const 
 int charm_gpu_flux_batch(charm_gpu_ctx_t *ctx, int flux, int64_t n, const double *prim_l,
                          const double *prim_r, const double *normals, double *out);
 
+/* VTK output (src/charm_output.c:10-126): <name>_0000.vtu (one
+ * UnstructuredGrid piece, raw appended data) and <name>.pvtu; points are the
+ * shared leaf corners, cells hexahedra with cell data treeid, level,
+ * mpirank and the cell-centre fields R P E E_TOT U V W T C_<species> of the
+ * state u (FP64 [n][V][4]) by cons_to_prim with EOS flag 4 */
+int charm_write_vtk(const char *name, int64_t n, const int32_t *leaves, const double *u,
+                    const charm_config_t *cfg);
 /* ------------------------------------------------ reference entry points
  * (include/charm_models.h:15-23, charm_limiter.h:13-15, charm_fluxes.h:19-32,
  * charm_amr.h:9-10)
@@ -285,6 +296,10 @@ void charm_limiter(charm_forest_t *p4est, charm_ghost_t *ghost, charm_data_t *gh
  * points call it when REFINE_PERIOD divides a nonzero step number. */
 void charm_adapt(charm_forest_t *p4est, charm_ghost_t *ghost, charm_data_t *ghost_data);
 void charm_limiter_bj(charm_forest_t *p4est, charm_ghost_t *ghost, charm_data_t *ghost_data);
+/* VTK files of the current state, <prefix>_<timestep %08d>{_0000.vtu,.pvtu}
+ * (src/charm_output.c:56-126; the time-step entry points call it when
+ * FILE_OUTPUT_STEP divides the step number, src/models/euler/charm_model.c:507-510) */
+void charm_write_solution(charm_forest_t *p4est);
 void charm_calc_flux_lf(charm_forest_t *p4est, charm_prim_t prim[2], charm_real_t *qu,
                         charm_real_t *qv, charm_real_t *qw, charm_real_t *qe,
                         charm_real_t qc[], charm_real_t n[3]);

@@ -269,3 +269,28 @@ def test_reduced_precision_steps_close_to_fp64(case):
     else:
         # smooth flow: the FP64 state removes most of the FP32 error
         assert err[CA.FP32_MIX] < 1e-6 and err[CA.FP32_MIX] < err[CA.FP32] / 3
+
+
+def test_write_solution_every_file_output_step(tmp_path):
+    """charm_timesteps with FILE_OUTPUT_STEP = 2 writes <prefix>_<step>
+    before steps 0, 2 and 4 (src/models/euler/charm_model.c:507-510); the
+    last file holds the state after 4 steps, as charm_write_vtk of that
+    state writes it."""
+    import filecmp
+    cfg, leaves, u0, periodic = W.c2(level=2, precision=CA.FP64)
+    dt = 1e-3
+    cfg.dt, cfg.tmax = dt, 4.5 * dt
+    cfg.write_period = 2
+    prefix = str(tmp_path / "run")
+    cfg.out_prefix = prefix.encode()
+    f = CA.Forest.from_leaves(leaves, periodic)
+    s = CA.Solver(f, cfg)
+    s.set_state(u0)
+    s.timesteps()                                   # steps 0..4
+    names = sorted(p.name for p in tmp_path.iterdir())
+    assert names == ["run_%08d%s" % (k, sfx) for k in (0, 2, 4) for sfx in (".pvtu", "_0000.vtu")]
+    ref = device(cfg, leaves, periodic, u0)
+    ref.run(dt, 4)
+    CA.write_vtk(str(tmp_path / "ref"), leaves, ref.get_state(), cfg)
+    assert filecmp.cmp(str(tmp_path / "run_00000004_0000.vtu"), str(tmp_path / "ref_0000.vtu"), shallow=False)
+    s.close()
