@@ -285,6 +285,8 @@ def main():
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--no-traffic", action="store_true")
     ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--only", choices=("bubbles", "hot_spot", "adjoint"),
+                    help="run one of the other configurations alone and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -296,6 +298,10 @@ def main():
 
     if args.traffic_probe:
         traffic_probe(CA, W, args.level)
+        return
+    if args.only:
+        fn = {"bubbles": bench_bubbles, "hot_spot": bench_hot_spot, "adjoint": bench_adjoint}[args.only]
+        print(json.dumps(fn(CA, W)))
         return
 
     print("[bench] FP32 bulk path, level %d, %d steps" % (args.level, args.steps), file=sys.stderr, flush=True)

@@ -14,7 +14,7 @@ INC = os.path.join(ROOT, "include")
 OBJ = os.path.join(ROOT, "_build", "charm3d")
 LIB = os.path.join(PKG, "libcharm3d.so")
 
-C_SOURCES = ["charm_forest.c", "charm_geom.c", "charm_model.c", "charm_amr.c", "charm_output.c"]
+C_SOURCES = ["charm_forest.c", "charm_geom.c", "charm_model.c", "charm_amr.c", "charm_output.c", "charm_yaml.c"]
 HIP_SOURCES = ["charm_stage_f32_ph0.hip", "charm_stage_f32_ph1.hip", "charm_stage_f64_ph0.hip",
                "charm_stage_f64_ph1.hip", "charm_stage_mix_ph0.hip", "charm_kernels.hip", "charm_gpu.hip", "charm_amr.hip",
                "charm_ns.hip", "charm_adj.hip"]

@@ -258,6 +258,38 @@ int charm_gpu_flux_batch(charm_gpu_ctx_t *ctx, int flux, int64_t n, const double
  * state u (FP64 [n][V][4]) by cons_to_prim with EOS flag 4 */
 int charm_write_vtk(const char *name, int64_t n, const int32_t *leaves, const double *u,
                     const charm_config_t *cfg);
+/* ------------------------------------------------------------ task.yaml */
+/* one initial-condition region (charm_reg_t, include/charm_globals.h) */
+typedef struct {
+    int    id, cell_type, mat_id;
+    char   name[64];
+    double v[3], t, p, grav[3];
+    double c[CHARM_MAX_COMP];          /* mass fractions by component index */
+} charm_region_desc_t;
+/* what charm_init_context_yaml reads (src/charm_init_yaml.cpp:333-426) */
+typedef struct {
+    charm_config_t cfg;                /* control, components, materials, model, reactions */
+    int    repartition_period;         /* REPARTITION_PERIOD (single rank: unused) */
+    char   bnd_name[6][64];            /* boundaries by cube side */
+    int    n_regions;
+    charm_region_desc_t region[16];
+    char   mesh_files_type[32];        /* mesh.files_type */
+    char   mesh_name[256];             /* mesh.name */
+    int    mesh_periodic[3];           /* unit_cube: mesh.periodic [x, y, z] */
+} charm_task_t;
+/* Parse a task.yaml (the reference's keys; yaml-cpp's role is taken by a
+ * parser of the YAML subset task files use).  Boundaries are bound to the
+ * unit cube's sides by name: LEFT -x, RIGHT +x, FRONT -y, BACK +y, BOTTOM -z,
+ * TOP +z.  This build's own keys: control.PRECISION (FP64, FP32, FP32_MIX;
+ * default FP64), mesh.files_type unit_cube with mesh.periodic.  Errors
+ * (with the reference's messages) are returned, never fatal. */
+int charm_task_load_yaml(const char *path, charm_task_t *task);
+/* the uniform forest at MIN_LEVEL (src/charm_3d.c:36-37); gmsh meshes are
+ * CHARM_ERR_UNSUPPORTED */
+int charm_task_forest(const charm_task_t *task, charm_forest_t **out);
+/* charm_init_initial_condition (src/charm_init.c:12-73): the region of cell
+ * type 0 through EOS flag 3 into coefficient 0, slopes zero; u FP64 [n][V][4] */
+int charm_task_initial_state(const charm_task_t *task, const charm_forest_t *f, double *u, int64_t n);
 /* ------------------------------------------------ reference entry points
  * (include/charm_models.h:15-23, charm_limiter.h:13-15, charm_fluxes.h:19-32,
  * charm_amr.h:9-10)
