@@ -54,6 +54,20 @@ class Config(C.Structure):
                 ("comp_name", (C.c_char * 32) * MAX_COMP), ("out_prefix", C.c_char * 256)]
 
 
+class RegionDesc(C.Structure):
+    _fields_ = [("id", C.c_int), ("cell_type", C.c_int), ("mat_id", C.c_int),
+                ("name", C.c_char * 64), ("v", C.c_double * 3), ("t", C.c_double),
+                ("p", C.c_double), ("grav", C.c_double * 3), ("c", C.c_double * MAX_COMP)]
+
+
+class Task(C.Structure):
+    """charm_task_t: what task.yaml holds."""
+    _fields_ = [("cfg", Config), ("repartition_period", C.c_int),
+                ("bnd_name", (C.c_char * 64) * 6), ("n_regions", C.c_int),
+                ("region", RegionDesc * 16), ("mesh_files_type", C.c_char * 32),
+                ("mesh_name", C.c_char * 256), ("mesh_periodic", C.c_int * 3)]
+
+
 _lib = None
 
 
@@ -106,6 +120,9 @@ def lib():
         "charm_gpu_flux_batch": (C.c_int, [vp, C.c_int, i64, dp, dp, dp, dp]),
         "charm_write_vtk": (C.c_int, [C.c_char_p, i64, i32p, dp, C.POINTER(Config)]),
         "charm_write_solution": (None, [vp]),
+        "charm_task_load_yaml": (C.c_int, [C.c_char_p, C.POINTER(Task)]),
+        "charm_task_forest": (C.c_int, [C.POINTER(Task), C.POINTER(vp)]),
+        "charm_task_initial_state": (C.c_int, [C.POINTER(Task), vp, dp, i64]),
         "charm_gpu_amr_indicator": (C.c_int, [vp, dp, dp, i64]),
         "charm_gpu_ns_aux": (C.c_int, [vp, dp, i64]),
         "charm_gpu_objective": (C.c_int, [vp, C.c_double, C.c_int, dp, dp, dp]),
@@ -209,6 +226,28 @@ def make_config(precision=FP32, model="NS", flux="HLLC", limiter=False, eos="IDE
         cfg.comp_name[i].value = nm.encode()[:31]
     cfg.out_prefix = out_prefix.encode()[:255]
     return cfg
+
+
+def load_task(path):
+    """charm_task_load_yaml: a Task from a task.yaml file."""
+    t = Task()
+    check(lib().charm_task_load_yaml(str(path).encode(), C.byref(t)))
+    return t
+
+
+def task_forest(task):
+    """charm_task_forest: the unit-cube forest at MIN_LEVEL."""
+    h = C.c_void_p()
+    check(lib().charm_task_forest(C.byref(task), C.byref(h)))
+    return Forest(h, tuple(task.mesh_periodic))
+
+
+def task_initial_state(task, forest):
+    """charm_task_initial_state: FP64 [n][V][4]."""
+    n = forest.n
+    u = np.zeros((n, 4 + task.cfg.n_comp, 4))
+    check(lib().charm_task_initial_state(C.byref(task), forest.h, _dptr(u), n))
+    return u
 
 
 def write_vtk(name, leaves, u, cfg):

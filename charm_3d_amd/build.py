@@ -13,6 +13,7 @@ SRC = os.path.join(PKG, "csrc")
 INC = os.path.join(ROOT, "include")
 OBJ = os.path.join(ROOT, "_build", "charm3d")
 LIB = os.path.join(PKG, "libcharm3d.so")
+EXE = os.path.join(PKG, "charm_3d")          # the charm_3d program (task.yaml driver)
 
 C_SOURCES = ["charm_forest.c", "charm_geom.c", "charm_model.c", "charm_amr.c", "charm_output.c", "charm_yaml.c"]
 HIP_SOURCES = ["charm_stage_f32_ph0.hip", "charm_stage_f32_ph1.hip", "charm_stage_f64_ph0.hip",
@@ -53,6 +54,11 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-pthread", "-o", tmp] + [o for _, o in jobs])
     os.replace(tmp, LIB)
+    # the program links the library from its own directory
+    main_o = os.path.join(OBJ, "charm_main.c.o")
+    _run(["gcc"] + C_FLAGS + ["-c", os.path.join(SRC, "charm_main.c"), "-o", main_o])
+    _run(["gcc", "-o", EXE + ".tmp", main_o, "-L", PKG, "-lcharm3d", "-Wl,-rpath,$ORIGIN"])
+    os.replace(EXE + ".tmp", EXE)
     return LIB
 
 

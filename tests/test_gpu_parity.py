@@ -4,6 +4,8 @@ FP64 device results must match the oracle to 1e-12 relative (the bar the
 FP64 path is held to); the FP32 bulk path is compared to the FP64 device
 path with the tolerances written in each test.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -294,3 +296,28 @@ def test_write_solution_every_file_output_step(tmp_path):
     CA.write_vtk(str(tmp_path / "ref"), leaves, ref.get_state(), cfg)
     assert filecmp.cmp(str(tmp_path / "run_00000004_0000.vtu"), str(tmp_path / "ref_0000.vtu"), shallow=False)
     s.close()
+
+
+def test_charm_3d_program_runs_a_task(tmp_path):
+    """the charm_3d program on a unit-cube task.yaml (level 2, TAU 1e-3,
+    TMAX 4.5e-3: five steps, files every 2): its last file equals the file
+    charm_write_vtk writes from the same task run through the API"""
+    import subprocess
+    import test_task_yaml as TY
+    text = (TY.EULER_TASK.replace("MIN_LEVEL: 3", "MIN_LEVEL: 2").replace("MAX_LEVEL: 5", "MAX_LEVEL: 2")
+            .replace("REFINE_PERIOD: 10", "REFINE_PERIOD: 0").replace("TAU: 1.0e-2", "TAU: 1.0e-6")
+            .replace("TMAX: 50 ", "TMAX: 4.5e-6 ").replace("FILE_OUTPUT_STEP: 100", "FILE_OUTPUT_STEP: 2")
+            .replace("LOG_OUTPUT_STEP: 10", "LOG_OUTPUT_STEP: 1"))
+    (tmp_path / "task.yaml").write_text(text)
+    exe = os.path.join(os.path.dirname(CA.__file__), "charm_3d")
+    r = subprocess.run([exe], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.count("STEP =") == 5
+    names = sorted(p.name for p in tmp_path.iterdir() if p.name.startswith("charm_3d_"))
+    assert names == ["charm_3d_%08d%s" % (k, s) for k in (0, 2, 4) for s in (".pvtu", "_0000.vtu")]
+    t = CA.load_task(str(tmp_path / "task.yaml"))
+    f = CA.task_forest(t)
+    ctx = device(t.cfg, f.leaves(), f.periodic, CA.task_initial_state(t, f))
+    ctx.run(t.cfg.dt, 4)
+    CA.write_vtk(str(tmp_path / "api"), f.leaves(), ctx.get_state(), t.cfg)
+    assert (tmp_path / "charm_3d_00000004_0000.vtu").read_bytes() == (tmp_path / "api_0000.vtu").read_bytes()
