@@ -63,6 +63,12 @@ struct charm_ctx {
 };
 
 charm_ctx_t *charm_ctx_of(charm_forest_t *f);
+/* host pointwise EOS pieces (charm_pointwise.c) */
+double charm_comp_cp(const charm_comp_desc_t *q, double t);
+double charm_comp_cp_dt(const charm_comp_desc_t *q, double t);
+double charm_calc_t(const charm_config_t *cfg, const double *c, double e);
+void   charm_eos_ideal_cfg(const charm_config_t *cfg, charm_prim_t *p, int flag);
+void   charm_eos_mix_cfg(const charm_config_t *cfg, charm_prim_t *p, int flag);
 uint64_t charm_morton(int32_t x, int32_t y, int32_t z);
 int64_t  charm_forest_find(const charm_forest_t *f, int32_t x, int32_t y, int32_t z);
 void     charm_set_error(const char *fmt, ...);

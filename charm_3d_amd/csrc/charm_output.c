@@ -20,79 +20,30 @@
 #include <string.h>
 #include "charm_internal.h"
 
-#define GAS_R 8.31446261815324          /* src/charm_eos.c:8 */
-
-/* cp of species i at T: the power series when cp_type is polynomial and
- * ml_type is 1 (charm_comp_calc_cp, src/charm_globals.c:603-625), else cp[0] */
-static double comp_cp(const charm_comp_desc_t *q, double t)
-{
-    double r = 0.0, tt = 1.0;
-    int k;
-    if (!(q->cp_type != 0 && q->ml_type == 1)) return q->cp[0];
-    for (k = 0; k < q->n_cp; k++) { r += tt * q->cp[k]; tt *= t; }
-    return r;
-}
-
-/* charm_comp_calc_cp_dt (src/charm_globals.c:628-649) */
-static double comp_cp_dt(const charm_comp_desc_t *q, double t)
-{
-    double r = 0.0, tt = 1.0;
-    int k;
-    for (k = 1; k < q->n_cp; k++) { r += tt * q->cp[k] * k; tt *= t; }
-    return r;
-}
-
-/* centre-state primitive fields: out = R P E E_TOT U V W T C_0..C_{nc-1} */
+/* centre-state primitive fields: out = R P E E_TOT U V W T C_0..C_{nc-1},
+ * charm_param_cons_to_prim of the coefficient-0 state */
 static void cell_fields(const charm_config_t *cfg, const double *u /* [V][4] */, double *out)
 {
     const int nc = cfg->n_comp;
-    double r = 0.0, c[CHARM_MAX_COMP], cp = 0.0, m, cv, gam, e, e_tot, vel[3], t, p;
+    charm_prim_t p;
     int i;
-    for (i = 0; i < nc; i++) r += u[4 * (4 + i)];
-    for (i = 0; i < 3; i++) vel[i] = u[4 * i] / r;
-    e_tot = u[4 * 3] / r;
-    e = e_tot - 0.5 * (vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2]);
+    memset(&p, 0, sizeof(p));
+    for (i = 0; i < nc; i++) p.r += u[4 * (4 + i)];
+    p.u = u[0] / p.r;
+    p.v = u[4] / p.r;
+    p.w = u[8] / p.r;
+    p.e_tot = u[12] / p.r;
+    p.e = p.e_tot - 0.5 * (p.u * p.u + p.v * p.v + p.w * p.w);
     for (i = 0; i < nc; i++) {
-        c[i] = u[4 * (4 + i)] / r;
-        if (c[i] < 0.0) c[i] = 0.0;
-        if (c[i] > 1.0) c[i] = 1.0;
+        p.c[i] = u[4 * (4 + i)] / p.r;
+        if (p.c[i] < 0.0) p.c[i] = 0.0;
+        if (p.c[i] > 1.0) p.c[i] = 1.0;
     }
-    /* temperature of a polynomial cp by Newton from 1 K (_calc_t,
-     * src/charm_eos.c:12-54), then cp, M, cv at that temperature */
-    t = 0.0;
-    if (cfg->comp[0].cp_type != 0) {
-        double tt = 1.0, m_ = 0.0, rm;
-        int j;
-        for (i = 0; i < nc; i++) m_ += c[i] / cfg->comp[i].m;
-        rm = GAS_R * m_;
-        for (j = 0; j < 100; j++) {
-            double a = 0.0, ad = 0.0, ft, fd, t1, d;
-            for (i = 0; i < nc; i++) { a += c[i] * comp_cp(&cfg->comp[i], tt); ad += c[i] * comp_cp_dt(&cfg->comp[i], tt); }
-            ft = (a - rm) * tt - e;
-            fd = (a - rm) + ad * tt;
-            t1 = tt - ft / fd;
-            d = t1 - tt;
-            tt = t1;
-            if (d * d < 1.e-12) break;
-        }
-        t = tt;
-    }
-    if (cfg->eos == CHARM_EOS_IDEAL || nc == 1) {
-        cp = comp_cp(&cfg->comp[0], t);
-        m = cfg->comp[0].m;
-    } else {
-        double M_ = 0.0;
-        for (i = 0; i < nc; i++) { M_ += c[i] / cfg->comp[i].m; cp += c[i] * comp_cp(&cfg->comp[i], t); }
-        m = 1.0 / M_;
-    }
-    cv = cp - GAS_R / m;
-    gam = cp / cv;
-    if (r < 1.e-12) r = 1.e-12;
-    p = r * e * (gam - 1.0);
-    out[0] = r; out[1] = p; out[2] = e; out[3] = e_tot;
-    out[4] = vel[0]; out[5] = vel[1]; out[6] = vel[2];
-    out[7] = e / cv;
-    for (i = 0; i < nc; i++) out[8 + i] = c[i];
+    if (cfg->eos == CHARM_EOS_IDEAL) charm_eos_ideal_cfg(cfg, &p, 4);
+    else charm_eos_mix_cfg(cfg, &p, 4);
+    out[0] = p.r; out[1] = p.p; out[2] = p.e; out[3] = p.e_tot;
+    out[4] = p.u; out[5] = p.v; out[6] = p.w; out[7] = p.t;
+    for (i = 0; i < nc; i++) out[8 + i] = p.c[i];
 }
 
 static int cmp_u64(const void *a, const void *b)

@@ -17,7 +17,6 @@
 #include <string.h>
 #include "charm_internal.h"
 
-#define GAS_R 8.31446261815324          /* src/charm_eos.c:8 */
 #define YAML_VERSION_MIN "0.1.1"        /* include/charm_version.h:9 */
 
 /* ---------------------------------------------------------------- nodes */
@@ -651,57 +650,45 @@ int charm_task_forest(const charm_task_t *task, charm_forest_t **out)
     return charm_forest_new_uniform(task->cfg.min_level, task->mesh_periodic, out);
 }
 
-/* ideal / mixture cp and molar mass at T (charm_mat_eos_ideal / _mix,
- * src/charm_eos.c:98-148; a polynomial cp when cp_type is polynomial and
- * ml_type 1, charm_comp_calc_cp) */
-static void eos_cp_m(const charm_config_t *cfg, const double *c, double t, double *cp, double *m)
-{
-    int i, k;
-    double M_ = 0.0;
-    *cp = 0.0;
-    for (i = 0; i < (cfg->eos == CHARM_EOS_IDEAL ? 1 : cfg->n_comp); i++) {
-        const charm_comp_desc_t *q = &cfg->comp[i];
-        double cpi = q->cp[0];
-        if (q->cp_type != 0 && q->ml_type == 1) {
-            double tt = 1.0;
-            cpi = 0.0;
-            for (k = 0; k < q->n_cp; k++) { cpi += tt * q->cp[k]; tt *= t; }
-        }
-        if (cfg->eos == CHARM_EOS_IDEAL) { *cp = cpi; M_ = 1.0 / q->m; }
-        else { *cp += c[i] * cpi; M_ += c[i] / q->m; }
-    }
-    *m = 1.0 / M_;
-}
-
 int charm_task_initial_state(const charm_task_t *task, const charm_forest_t *f, double *u, int64_t n)
 {
     const charm_config_t *cfg;
     const charm_region_desc_t *g;
-    double cp, m, cv, r, e, e_tot;
+    charm_prim_t p;
     int nv, i, k;
     int64_t q;
     if (!task || !f || !u) { charm_set_error("task_initial_state: null argument"); return CHARM_ERR_ARG; }
-    if (n != charm_forest_num_leaves(f)) { charm_set_error("task_initial_state: %lld leaves, state for %lld", (long long) charm_forest_num_leaves(f), (long long) n); return CHARM_ERR_ARG; }
+    if (n != charm_forest_num_leaves(f)) {
+        charm_set_error("task_initial_state: %lld leaves, state for %lld",
+                        (long long) charm_forest_num_leaves(f), (long long) n);
+        return CHARM_ERR_ARG;
+    }
     cfg = &task->cfg;
     nv = 4 + cfg->n_comp;
-    /* one tree: the region of cell type 0 (the first region without one) */
+    /* one tree: the region of cell type 0 (else the first region) */
     g = &task->region[0];
     for (i = 0; i < task->n_regions; i++)
         if (task->region[i].cell_type == 0) { g = &task->region[i]; break; }
-    /* EOS flag 3: (T, p) -> (r, e) (src/charm_eos.c:57-96), then
+    /* (T, p) -> (r, cz, e) by EOS flag 3 (src/charm_eos.c:57-96), then
      * charm_param_prim_to_cons (src/charm_globals.c:211-223) into
      * coefficient 0, the slopes zero (src/charm_init.c:47-66) */
-    eos_cp_m(cfg, g->c, g->t, &cp, &m);
-    cv = cp - GAS_R / m;
-    r = g->p * m / (g->t * GAS_R);
-    e = g->p / (r * (cp / cv - 1.0));
-    e_tot = e + 0.5 * (g->v[0] * g->v[0] + g->v[1] * g->v[1] + g->v[2] * g->v[2]);
+    memset(&p, 0, sizeof(p));
+    p.mat_id = g->mat_id;
+    p.p = g->p;
+    p.t = g->t;
+    p.u = g->v[0]; p.v = g->v[1]; p.w = g->v[2];
+    for (i = 0; i < cfg->n_comp; i++) p.c[i] = g->c[i];
+    if (cfg->eos == CHARM_EOS_IDEAL) charm_eos_ideal_cfg(cfg, &p, 3);
+    else charm_eos_mix_cfg(cfg, &p, 3);
+    p.e_tot = p.e + 0.5 * (p.u * p.u + p.v * p.v + p.w * p.w);
     for (q = 0; q < n; q++) {
         double *w = u + (size_t) q * nv * 4;
         for (i = 0; i < nv * 4; i++) w[i] = 0.0;
-        for (k = 0; k < 3; k++) w[4 * k] = r * g->v[k];
-        w[4 * 3] = r * e_tot;
-        for (i = 0; i < cfg->n_comp; i++) w[4 * (4 + i)] = r * g->c[i];
+        w[0] = p.r * p.u;
+        w[4] = p.r * p.v;
+        w[8] = p.r * p.w;
+        w[12] = p.r * (p.e + 0.5 * (p.u * p.u + p.v * p.v + p.w * p.w));
+        for (k = 0; k < cfg->n_comp; k++) w[4 * (4 + k)] = p.r * p.c[k];
     }
     return CHARM_OK;
 }

@@ -305,6 +305,12 @@ typedef struct {
     charm_real_t c[128];
 } charm_prim_t;
 
+/* the reference's conservative record (include/charm_globals.h:80-87) */
+typedef struct {
+    charm_real_t ru, rv, rw, re;
+    charm_real_t rc[128];
+    int          mat_id;
+} charm_cons_t;
 int  charm_ctx_create(charm_forest_t *f, const charm_config_t *cfg, charm_ctx_t **out);
 int  charm_ctx_destroy(charm_ctx_t *c);
 charm_gpu_ctx_t *charm_ctx_gpu(charm_ctx_t *c);
@@ -328,6 +334,37 @@ void charm_limiter(charm_forest_t *p4est, charm_ghost_t *ghost, charm_data_t *gh
  * points call it when REFINE_PERIOD divides a nonzero step number. */
 void charm_adapt(charm_forest_t *p4est, charm_ghost_t *ghost, charm_data_t *ghost_data);
 void charm_limiter_bj(charm_forest_t *p4est, charm_ghost_t *ghost, charm_data_t *ghost_data);
+/* Pointwise host functions with the reference's names and arguments, over
+ * the forest's configuration, FP64 (the device path does not call them; its
+ * kernels restate the same formulas):
+ *  EOS functions, eos_fn (include/charm_globals.h:122; src/charm_eos.c:
+ *  57-148): flag 0 p(r,e), 1 e(r,p), 2 r(T,p), 3 (T,p) -> (r,cz,e),
+ *  4 (r,e) -> (p,cz,T); a polynomial cp of component 0 takes T from the
+ *  Newton solve for flags 1 and 4 (include/charm_eos.h:14-18) */
+void charm_mat_eos_ideal(charm_forest_t *p4est, charm_prim_t *p, int flag);
+void charm_mat_eos_mix(charm_forest_t *p4est, charm_prim_t *p, int flag);
+/* boundary-condition functions, bnd_fn (include/charm_globals.h:237;
+ * include/charm_bnd_cond.h:14-33, src/charm_bnd_cond.c:36-146); the
+ * mass-flow one records CHARM_ERR_UNSUPPORTED's message and passes the
+ * state through */
+void charm_bnd_cond_fn_inlet(charm_prim_t *par_in, charm_prim_t *par_out, int8_t face, charm_real_t *param,
+                             charm_real_t n[3]);
+void charm_bnd_cond_fn_outlet(charm_prim_t *par_in, charm_prim_t *par_out, int8_t face, charm_real_t *param,
+                              charm_real_t n[3]);
+void charm_bnd_cond_fn_wall_slip(charm_prim_t *par_in, charm_prim_t *par_out, int8_t face, charm_real_t *param,
+                                 charm_real_t n[3]);
+void charm_bnd_cond_fn_wall_no_slip(charm_prim_t *par_in, charm_prim_t *par_out, int8_t face,
+                                    charm_real_t *param, charm_real_t n[3]);
+void charm_bnd_cond_fn_mass_flow(charm_prim_t *par_in, charm_prim_t *par_out, int8_t face,
+                                 charm_real_t *param, charm_real_t n[3]);
+/* charm_bnd_cond (src/charm_bnd_cond.c:21-33): the BC function of cube side
+ * `face`, then EOS flag 3 and e_tot */
+void charm_bnd_cond(charm_forest_t *p4est, int treeid, int8_t face, charm_prim_t *par_in,
+                    charm_prim_t *par_out, charm_real_t n[3]);
+/* src/charm_globals.c:183-240 */
+void charm_param_cons_to_prim(charm_forest_t *p4est, charm_prim_t *p, charm_cons_t *c);
+void charm_param_prim_to_cons(charm_forest_t *p4est, charm_cons_t *c, charm_prim_t *p);
+void charm_prim_cpy(charm_prim_t *dest, charm_prim_t *src);
 /* VTK files of the current state, <prefix>_<timestep %08d>{_0000.vtu,.pvtu}
  * (src/charm_output.c:56-126; the time-step entry points call it when
  * FILE_OUTPUT_STEP divides the step number, src/models/euler/charm_model.c:507-510) */

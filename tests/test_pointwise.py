@@ -1,0 +1,239 @@
+"""The reference's pointwise host entry points (charm_mat_eos_ideal / _mix,
+charm_bnd_cond_fn_*, charm_bnd_cond, charm_param_cons_to_prim /
+_prim_to_cons; src/charm_eos.c:12-148, src/charm_bnd_cond.c:21-146,
+src/charm_globals.c:183-240).  The boundary functions take no context and
+are checked on the CPU against their closed forms; the EOS, dispatcher and
+conversions read the forest's configuration through a charm_ctx_t (which
+holds a device context), so those tests are marked gpu and compare with the
+oracle's restatement (or_eos, or_bnd, or_cons_to_prim) to 1e-14."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import charm_3d_amd as CA
+import mesh_util as MU
+import oracle_lib as OL
+
+R = CA.GAS_R
+NCP = 128
+
+
+class ChPrim(C.Structure):
+    _fields_ = [(n, C.c_double) for n in
+                ("r", "u", "v", "w", "e", "e_tot", "p", "t", "cz", "gam", "cp", "cv", "m")] + \
+               [("mat_id", C.c_int), ("c", C.c_double * NCP)]
+
+
+class ChCons(C.Structure):
+    _fields_ = [("ru", C.c_double), ("rv", C.c_double), ("rw", C.c_double), ("re", C.c_double),
+                ("rc", C.c_double * NCP), ("mat_id", C.c_int)]
+
+
+FIELDS = ("r", "u", "v", "w", "e", "e_tot", "p", "t", "cz", "gam", "cp", "cv", "m")
+
+
+def bind():
+    L = CA.lib()
+    pp = C.POINTER(ChPrim)
+    dp = C.POINTER(C.c_double)
+    for nm in ("inlet", "outlet", "wall_slip", "wall_no_slip", "mass_flow"):
+        fn = getattr(L, "charm_bnd_cond_fn_" + nm)
+        fn.restype = None
+        fn.argtypes = [pp, pp, C.c_int8, dp, dp]
+    for nm in ("charm_mat_eos_ideal", "charm_mat_eos_mix"):
+        getattr(L, nm).restype = None
+        getattr(L, nm).argtypes = [C.c_void_p, pp, C.c_int]
+    L.charm_bnd_cond.restype = None
+    L.charm_bnd_cond.argtypes = [C.c_void_p, C.c_int, C.c_int8, pp, pp, dp]
+    L.charm_param_cons_to_prim.restype = None
+    L.charm_param_cons_to_prim.argtypes = [C.c_void_p, pp, C.POINTER(ChCons)]
+    L.charm_param_prim_to_cons.restype = None
+    L.charm_param_prim_to_cons.argtypes = [C.c_void_p, C.POINTER(ChCons), pp]
+    return L
+
+
+def rand_prim(rng, nc=1):
+    p = ChPrim()
+    p.r, p.u, p.v, p.w = 1 + rng.uniform(0, 1), *rng.normal(size=3)
+    p.p, p.t, p.e = 1 + rng.uniform(0, 1), 1 + rng.uniform(0, 1), 2 + rng.uniform(0, 1)
+    p.e_tot, p.cz, p.gam, p.cp, p.cv, p.m = rng.uniform(size=6)
+    p.mat_id = 3
+    y = rng.dirichlet(np.ones(nc)) if nc > 1 else [1.0]
+    for i in range(nc):
+        p.c[i] = y[i]
+    return p
+
+
+def as_dict(p, nc=1):
+    d = {k: getattr(p, k) for k in FIELDS}
+    d.update({"c%d" % i: p.c[i] for i in range(nc)})
+    d["mat_id"] = p.mat_id
+    return d
+
+
+def dptr(a):
+    return np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(C.POINTER(C.c_double))
+
+
+def test_boundary_functions_closed_forms():
+    L = bind()
+    rng = np.random.default_rng(5)
+    for _ in range(20):
+        pin = rand_prim(rng)
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        nn = np.ascontiguousarray(n)
+        par = np.ascontiguousarray(rng.normal(size=16))
+        v = np.array([pin.u, pin.v, pin.w])
+        base = as_dict(pin)
+        out = ChPrim()
+        # inlet: (u, v, w, T, p) = params 0..4, the rest copied
+        L.charm_bnd_cond_fn_inlet(C.byref(pin), C.byref(out), 1, dptr(par), dptr(nn))
+        exp = dict(base, u=par[0], v=par[1], w=par[2], t=par[3], p=par[4])
+        assert as_dict(out) == exp
+        # outlet: a copy
+        out = ChPrim()
+        L.charm_bnd_cond_fn_outlet(C.byref(pin), C.byref(out), 1, dptr(par), dptr(nn))
+        assert as_dict(out) == base
+        # slip wall: v - 2 (v . n) n, normal component reversed
+        out = ChPrim()
+        L.charm_bnd_cond_fn_wall_slip(C.byref(pin), C.byref(out), 1, dptr(par), dptr(nn))
+        vo = np.array([out.u, out.v, out.w])
+        np.testing.assert_allclose(vo, v - 2 * (v @ n) * n, rtol=0, atol=1e-15)
+        assert abs(vo @ n + v @ n) < 1e-14
+        assert {k: x for k, x in as_dict(out).items() if k not in "uvw"} == \
+               {k: x for k, x in base.items() if k not in "uvw"}
+        # no-slip wall: normal component removed, wall temperature param[0]
+        out = ChPrim()
+        L.charm_bnd_cond_fn_wall_no_slip(C.byref(pin), C.byref(out), 1, dptr(par), dptr(nn))
+        vo = np.array([out.u, out.v, out.w])
+        np.testing.assert_allclose(vo, v - (v @ n) * n, rtol=0, atol=1e-15)
+        assert abs(vo @ n) < 1e-14 and out.t == par[0]
+    # mass flow: unsupported, passes the state through and says so
+    out = ChPrim()
+    L.charm_bnd_cond_fn_mass_flow(C.byref(pin), C.byref(out), 1, dptr(par), dptr(nn))
+    assert as_dict(out) == base
+    assert b"MASS_FLOW" in CA.lib().charm_gpu_last_error()
+
+
+def configs():
+    tr = [dict(ml0=1.7e-5, kp0=0.024, t0=273.0, ts=110.0, ml_type=1, kp_type=1)] * 2
+    return {
+        "ideal_const": (CA.make_config(precision=CA.FP64, comp_m=(0.029,), comp_cp=(1005.0,)),
+                        OL.make_config(n_comp=1, comp_m=(0.029,), comp_cp=((1005.0,),)), 1),
+        "mix_const": (CA.make_config(precision=CA.FP64, eos="MIX", comp_m=(0.029, 0.004),
+                                     comp_cp=(1005.0, 5193.0)),
+                      OL.make_config(n_comp=2, comp_m=(0.029, 0.004), comp_cp=((1005.0,), (5193.0,)), eos="MIX"), 2),
+        "mix_poly": (CA.make_config(precision=CA.FP64, eos="MIX", comp_m=(0.028, 0.032),
+                                    comp_cp=((950.0, 0.2), (880.0, 0.25)), transport=tr),
+                     OL.make_config(n_comp=2, comp_m=(0.028, 0.032), comp_cp=((950.0, 0.2), (880.0, 0.25)),
+                                    eos="MIX", transport=tr), 2),
+        "ideal_poly": (CA.make_config(precision=CA.FP64, comp_m=(R,), comp_cp=((3.3, 0.2, 0.05),), transport=tr[:1]),
+                       OL.make_config(n_comp=1, comp_m=(R,), comp_cp=((3.3, 0.2, 0.05),), transport=tr[:1]), 1),
+    }
+
+
+def solver(cfg):
+    f = CA.Forest.uniform(1)
+    return f, CA.Solver(f, cfg)
+
+
+def to_or(p, nc):
+    q = OL.Prim()
+    for k in FIELDS:
+        setattr(q, k, getattr(p, k))
+    q.mat_id = p.mat_id
+    for i in range(nc):
+        q.c[i] = p.c[i]
+    return q
+
+
+def close(a, b, nc):
+    for k in FIELDS:
+        x, y = getattr(a, k), getattr(b, k)
+        assert abs(x - y) <= 1e-14 * max(1.0, abs(y)), (k, x, y)
+    for i in range(nc):
+        assert abs(a.c[i] - b.c[i]) <= 1e-15, i
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(configs()))
+def test_eos_flags_match_oracle(name):
+    cfg, ocfg, nc = configs()[name]
+    L = bind()
+    f, s = solver(cfg)
+    fn = L.charm_mat_eos_ideal if cfg.eos == CA.EOS["IDEAL"] else L.charm_mat_eos_mix
+    rng = np.random.default_rng(11)
+    for flag in range(5):
+        for _ in range(10):
+            p = rand_prim(rng, nc)
+            if cfg.comp[0].m > 1:                     # nondimensional gas: T, e of order 1
+                p.t, p.e, p.p = 1 + rng.uniform(), 2 + rng.uniform(), 1 + rng.uniform()
+            else:                                     # SI: T ~ 300 K, e ~ 2e5 J/kg, p ~ 1e5 Pa
+                p.t, p.e, p.p = 300 + 100 * rng.uniform(), 2e5 * (1 + rng.uniform()), 1e5 * (1 + rng.uniform())
+            q = to_or(p, nc)
+            fn(f.h, C.byref(p), flag)
+            OL.lib().or_eos(C.byref(ocfg), C.byref(q), flag)
+            close(p, q, nc)
+    s.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["ideal_const", "mix_const", "mix_poly"])
+def test_cons_to_prim_and_back(name):
+    cfg, ocfg, nc = configs()[name]
+    L = bind()
+    f, s = solver(cfg)
+    rng = np.random.default_rng(12)
+    for _ in range(20):
+        c = ChCons()
+        r = 1 + rng.uniform()
+        vel = rng.normal(size=3)
+        y = rng.dirichlet(np.ones(nc)) if nc > 1 else [1.0]
+        e = (1 + rng.uniform()) * (1.0 if cfg.comp[0].m > 1 else 2e5)
+        c.ru, c.rv, c.rw = r * vel
+        c.re = r * (e + 0.5 * vel @ vel)
+        for i in range(nc):
+            c.rc[i] = r * y[i]
+        p = ChPrim()
+        L.charm_param_cons_to_prim(f.h, C.byref(p), C.byref(c))
+        oc = OL.Cons()
+        oc.ru, oc.rv, oc.rw, oc.re = c.ru, c.rv, c.rw, c.re
+        for i in range(nc):
+            oc.rc[i] = c.rc[i]
+        q = OL.Prim()
+        OL.lib().or_cons_to_prim(C.byref(ocfg), C.byref(oc), C.byref(q))
+        close(p, q, nc)
+        back = ChCons()
+        L.charm_param_prim_to_cons(f.h, C.byref(back), C.byref(p))
+        for k in ("ru", "rv", "rw", "re"):
+            assert abs(getattr(back, k) - getattr(c, k)) <= 1e-13 * max(1.0, abs(getattr(c, k)))
+    s.close()
+
+
+@pytest.mark.gpu
+def test_bnd_cond_matches_oracle():
+    """charm_bnd_cond on every BC type of the sides: the BC function, EOS
+    flag 3 and e_tot, against the oracle's boundary state"""
+    bnd = ("INLET", "OUTLET", "WALL_SLIP", "WALL_NO_SLIP", "WALL_SLIP", "OUTLET")
+    params = [[0.3, 0.1, 0.0, 1.1, 1.0], None, None, [1.05], None, None]
+    cfg = CA.make_config(precision=CA.FP64, bnd=bnd, bnd_param=params)
+    ocfg = OL.make_config(bnd=bnd, bnd_param=params)
+    L = bind()
+    f, s = solver(cfg)
+    rng = np.random.default_rng(13)
+    for face in range(6):
+        n = np.zeros(3)
+        n[face // 2] = 1.0 if face & 1 else -1.0
+        pin = OL.prim_from(ocfg, 1 + rng.uniform(), *rng.normal(size=3), 1 + rng.uniform())
+        p = ChPrim()
+        for k in FIELDS:
+            setattr(p, k, getattr(pin, k))
+        p.c[0] = 1.0
+        out = ChPrim()
+        L.charm_bnd_cond(f.h, 0, face, C.byref(p), C.byref(out), dptr(n))
+        q = OL.Prim()
+        OL.lib().or_bnd(C.byref(ocfg), face, C.byref(pin), C.byref(q), dptr(n))
+        close(out, q, 1)
+    s.close()
