@@ -262,11 +262,13 @@ void k_adj_stage(AdjArgs A, DevParams P)
 
     R *__restrict__ out = reinterpret_cast<R *>(A.OUT);
     const R *__restrict__ acc = reinterpret_cast<const R *>(A.ACC);
+    const R *__restrict__ acc2 = reinterpret_cast<const R *>(A.ACC2);
     #pragma unroll
     for (int i = 0; i < 4 * NV; i++) {
         const R sb = R(A.c_in) * (X + i * n)[e];
         R o = R(A.c_out) * (sb + ub[i]);
         if (acc) o += R(A.c_acc) * (acc + i * n)[e];
+        if (acc2) o += R(A.c_acc2) * (acc2 + i * n)[e];
         (out + i * n)[e] = o;
     }
 }
@@ -357,8 +359,9 @@ __device__ __forceinline__ void adj_face(const AdjArgs &A, const DevParams &P, c
         }
 }
 
-// Chunked face-centric stage adjoint (interior faces and volume; boundary
-// faces by k_adj_bnd_pts / k_adj_bnd_sum afterwards): the chunks of the stage kernel; phase 1
+// Chunked face-centric stage adjoint (interior faces and volume; the
+// boundary faces' per-point parts come from k_adj_bnd_pts, launched first,
+// and phase 2 adds each leaf's parts): the chunks of the stage kernel; phase 1
 // one thread per face of the chunk writes both sides' trace-gradient
 // moments to LDS, phase 2 one thread per element adds its volume term and
 // gathers its face slots (a fixed slot list, no atomics).  Each interior
@@ -422,13 +425,33 @@ void k_adj_chunk(AdjArgs A, DevParams P)
             ub[4*v+3] += c3[2];
         }
     }
+    // boundary faces: the per-point parts of k_adj_bnd_pts, in list order
+    const int bi = A.bnd_of ? A.bnd_of[e] : -1;
+    if (bi >= 0) {
+        const R *__restrict__ part = reinterpret_cast<const R *>(A.bnd_part);
+        double th[3] = {0.0, 0.0, 0.0};
+        for (int p = 6 * A.bnd_ptr[bi]; p < 6 * A.bnd_ptr[bi + 1]; p++) {
+            #pragma unroll
+            for (int j = 0; j < 4 * NV; j++) ub[j] += part[(size_t) p * 4 * NV + j];
+            if (A.theta_bar) {
+                #pragma unroll
+                for (int j = 0; j < 3; j++) th[j] += A.bnd_theta[3 * (size_t) p + j];
+            }
+        }
+        if (A.theta_bar) {
+            #pragma unroll
+            for (int j = 0; j < 3; j++) A.theta_bar[3 * (size_t) e + j] += th[j];
+        }
+    }
     R *__restrict__ out = reinterpret_cast<R *>(A.OUT);
     const R *__restrict__ acc = reinterpret_cast<const R *>(A.ACC);
+    const R *__restrict__ acc2 = reinterpret_cast<const R *>(A.ACC2);
     #pragma unroll
     for (int i = 0; i < 4 * NV; i++) {
         const R sb = R(A.c_in) * (X + i * n)[e];
         R o = R(A.c_out) * (sb + ub[i]);
         if (acc) o += R(A.c_acc) * (acc + i * n)[e];
+        if (acc2) o += R(A.c_acc2) * (acc2 + i * n)[e];
         (out + i * n)[e] = o;
     }
 }
@@ -438,7 +461,8 @@ void k_adj_chunk(AdjArgs A, DevParams P)
 // the point's contribution phi_k (d s / d t) to the leaf's coefficient
 // gradient (and, on the inlet face of the parameters, d s / d theta) is
 // written to its own slot of a scratch array, summed per leaf in a fixed
-// order by k_adj_bnd_sum.
+// order by k_adj_bnd_sum (element-centric path) or by phase 2 of
+// k_adj_chunk.
 template <typename R, int NC, int FLUX>
 __global__ __launch_bounds__(64)
 void k_adj_bnd_pts(AdjArgs A, DevParams P)
@@ -538,7 +562,7 @@ void k_adj_bnd_sum(AdjArgs A)
 }
 
 template <typename R, int NC>
-static hipError_t launch_adj_bnd(int flux, const AdjArgs &A, const DevParams &P, hipStream_t st)
+static hipError_t launch_adj_bnd_pts(int flux, const AdjArgs &A, const DevParams &P, hipStream_t st)
 {
     if (A.n_bnd <= 0) return hipSuccess;
     const dim3 gp((unsigned) ((6 * (int64_t) A.n_bface + 63) / 64)), bp(64);
@@ -547,6 +571,17 @@ static hipError_t launch_adj_bnd(int flux, const AdjArgs &A, const DevParams &P,
     case FLUX_HLLC: hipLaunchKernelGGL((k_adj_bnd_pts<R, NC, FLUX_HLLC>), gp, bp, 0, st, A, P); break;
     default: hipLaunchKernelGGL((k_adj_bnd_pts<R, NC, FLUX_CD>), gp, bp, 0, st, A, P); break;
     }
+    return hipGetLastError();
+}
+
+// element-centric path: boundary points after the interior kernel, then the
+// per-leaf sum into OUT
+template <typename R, int NC>
+static hipError_t launch_adj_bnd(int flux, const AdjArgs &A, const DevParams &P, hipStream_t st)
+{
+    if (A.n_bnd <= 0) return hipSuccess;
+    hipError_t err = launch_adj_bnd_pts<R, NC>(flux, A, P, st);
+    if (err != hipSuccess) return err;
     const dim3 gs((unsigned) ((A.n_bnd + 127) / 128)), bs(128);
     hipLaunchKernelGGL((k_adj_bnd_sum<R, 4 + NC>), gs, bs, 0, st, A);
     return hipGetLastError();
@@ -612,15 +647,6 @@ void k_adj_objective(ObjArgs A, DevParams P)
     }
 }
 
-// y = a x + b y on n * nvk values
-template <typename R>
-__global__ __launch_bounds__(256)
-void k_adj_axpby(const R *__restrict__ x, R *__restrict__ y, R a, R b, int64_t m)
-{
-    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < m) y[i] = a * x[i] + b * y[i];
-}
-
 template <typename R, int NC, int FLUX, int BS>
 static hipError_t launch_adj_chunk_bs(const AdjArgs &A, const DevParams &P, hipStream_t st)
 {
@@ -647,15 +673,15 @@ template <typename R, int NC>
 static hipError_t launch_adj_nc(int flux, const AdjArgs &A, const DevParams &P, hipStream_t st)
 {
     if (sizeof(R) == 4 && A.cf4) {
-        hipError_t err;
+        hipError_t err = launch_adj_bnd_pts<R, NC>(flux, A, P, st);
+        if (err != hipSuccess) return err;
         switch (flux) {
         case FLUX_LF: err = launch_adj_chunk<R, NC, FLUX_LF>(A, P, st); break;
         case FLUX_HLLC: err = launch_adj_chunk<R, NC, FLUX_HLLC>(A, P, st); break;
         case FLUX_CD: err = launch_adj_chunk<R, NC, FLUX_CD>(A, P, st); break;
         default: return hipErrorInvalidValue;
         }
-        if (err != hipSuccess) return err;
-        return launch_adj_bnd<R, NC>(flux, A, P, st);
+        return err;
     }
     const dim3 grid((unsigned) ((A.n + 127) / 128)), block(128);
     switch (flux) {
@@ -708,13 +734,5 @@ hipError_t launch_adj_objective(int prec, int nc, const ObjArgs &A, const DevPar
     return prec ? launch_obj_r<float>(nc, A, P, st) : launch_obj_r<double>(nc, A, P, st);
 }
 
-hipError_t launch_adj_axpby(int prec, const void *x, void *y, double a, double b, int64_t m, hipStream_t st)
-{
-    if (m <= 0) return hipSuccess;
-    const dim3 grid((unsigned) ((m + 255) / 256)), block(256);
-    if (prec) hipLaunchKernelGGL(k_adj_axpby<float>, grid, block, 0, st, (const float *) x, (float *) y, (float) a, (float) b, m);
-    else hipLaunchKernelGGL(k_adj_axpby<double>, grid, block, 0, st, (const double *) x, (double *) y, a, b, m);
-    return hipGetLastError();
-}
 
 } // namespace charm

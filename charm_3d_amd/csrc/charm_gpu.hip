@@ -918,7 +918,7 @@ extern "C" int charm_gpu_objective(charm_gpu_ctx_t *c, double dt, int nsteps, co
 // their face ranges, the faces (leaf * 8 + direction, ordered by leaf), and
 // the per-point scratch of their contributions.
 struct AdjBoundary {
-    int *elems, *ptr, *faces;
+    int *elems, *ptr, *faces, *of;
     void *part;
     double *theta;
     int n_elems, n_faces;
@@ -926,7 +926,7 @@ struct AdjBoundary {
 
 static void adj_boundary_free(AdjBoundary &b)
 {
-    void *bufs[] = {b.elems, b.ptr, b.faces, b.part, b.theta};
+    void *bufs[] = {b.elems, b.ptr, b.faces, b.of, b.part, b.theta};
     for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); i++)
         if (bufs[i]) (void) hipFree(bufs[i]);
     memset(&b, 0, sizeof(b));
@@ -940,8 +940,9 @@ static int adj_boundary(charm_gpu_ctx_t *c, AdjBoundary &b)
     int *elems = (int *) malloc((size_t) n * sizeof(int));
     int *ptr = (int *) malloc(((size_t) n + 1) * sizeof(int));
     int *faces = (int *) malloc((size_t) n * 6 * sizeof(int));
-    if (!nbr || !elems || !ptr || !faces) {
-        free(nbr); free(elems); free(ptr); free(faces);
+    int *of = (int *) malloc((size_t) (n ? n : 1) * sizeof(int));
+    if (!nbr || !elems || !ptr || !faces || !of) {
+        free(nbr); free(elems); free(ptr); free(faces); free(of);
         charm_set_error("adjoint: out of memory");
         return CHARM_ERR_MEM;
     }
@@ -954,7 +955,8 @@ static int adj_boundary(charm_gpu_ctx_t *c, AdjBoundary &b)
             const int f0 = nf;
             for (int d = 0; d < 6; d++)
                 if (nbr[6 * i + d] < 0 && nbr[6 * i + d] > -8) faces[nf++] = (int) (i * 8 + d);
-            if (nf > f0) { elems[ne] = (int) i; ptr[++ne] = nf; }
+            of[i] = -1;
+            if (nf > f0) { of[i] = ne; elems[ne] = (int) i; ptr[++ne] = nf; }
         }
     }
     if (e == hipSuccess && ne > 0) {
@@ -966,8 +968,10 @@ static int adj_boundary(charm_gpu_ctx_t *c, AdjBoundary &b)
         if (e == hipSuccess) e = hipMemcpy(b.elems, elems, (size_t) ne * sizeof(int), hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(b.ptr, ptr, ((size_t) ne + 1) * sizeof(int), hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(b.faces, faces, (size_t) nf * sizeof(int), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMalloc((void **) &b.of, (size_t) n * sizeof(int));
+        if (e == hipSuccess) e = hipMemcpy(b.of, of, (size_t) n * sizeof(int), hipMemcpyHostToDevice);
     }
-    free(nbr); free(elems); free(ptr); free(faces);
+    free(nbr); free(elems); free(ptr); free(faces); free(of);
     if (e != hipSuccess) {
         adj_boundary_free(b);
         charm_set_error("adjoint: %s", hipGetErrorString(e));
@@ -1079,12 +1083,15 @@ extern "C" int charm_gpu_adjoint(charm_gpu_ctx_t *c, double dt, int nsteps, int 
             A.ch_ebeg = c->d_ch_ebeg; A.ch_fbeg = c->d_ch_fbeg; A.slot_ptr = c->d_slot_ptr; A.slots = c->d_slots;
             A.n_chunks = (int) c->nch; A.max_faces = c->max_faces; A.chunk_elems = c->chunk_elems;
             A.n_xcd = c->n_xcd;
-            A.U = u2; A.X = la; A.ACC = NULL; A.OUT = lb; A.c_in = 2.0 / 3.0; A.c_out = 1.0; A.c_acc = 0.0;
+            A.bnd_of = chunked ? bnd.of : NULL;
+            // l3 = S^T(2/3 l); l2 = S^T(1/4 l3); l = S^T(l2) + l / 3 + 3/4 l3
+            A.U = u2; A.X = la; A.ACC = NULL; A.ACC2 = NULL; A.OUT = lb;
+            A.c_in = 2.0 / 3.0; A.c_out = 1.0; A.c_acc = 0.0; A.c_acc2 = 0.0;
             ADJ_CHK(launch_adj_stage(c->sprec, c->nc, c->cfg.flux, A, c->P, c->st));
-            ADJ_CHK(launch_adj_axpby(c->sprec, lb, la, 0.75, 1.0 / 3.0, m, c->st));
             A.U = u1; A.X = lb; A.OUT = lc; A.c_in = 0.25;
             ADJ_CHK(launch_adj_stage(c->sprec, c->nc, c->cfg.flux, A, c->P, c->st));
-            A.U = u0; A.X = lc; A.ACC = la; A.OUT = la; A.c_in = 1.0; A.c_acc = 1.0;
+            A.U = u0; A.X = lc; A.ACC = la; A.ACC2 = lb; A.OUT = la; A.c_in = 1.0;
+            A.c_acc = 1.0 / 3.0; A.c_acc2 = 0.75;
             ADJ_CHK(launch_adj_stage(c->sprec, c->nc, c->cfg.flux, A, c->P, c->st));
         }
         ADJ_CHK(hipEventRecord(ev_done[sg & 1], c->st));

@@ -89,11 +89,12 @@ struct NsArgs {
 };
 
 // adjoint of one SSP-RK stage (one thread per leaf):
-// OUT = c_out (sb + (d int / d U)^T W) + c_acc ACC,  sb = c_in X,  W = -dt A^-1 sb
+// OUT = c_out (sb + (d int / d U)^T W) + c_acc ACC + c_acc2 ACC2,  sb = c_in X,
+// W = -dt A^-1 sb
 struct AdjArgs {
-    const void *U, *X, *ACC;
+    const void *U, *X, *ACC, *ACC2;
     void *OUT;
-    double c_in, c_out, c_acc, dt;
+    double c_in, c_out, c_acc, c_acc2, dt;
     const uint8_t *level;
     const int *nbr;
     const int4 *aux;
@@ -112,6 +113,9 @@ struct AdjArgs {
     const int *ch_ebeg, *ch_fbeg, *slot_ptr;
     const uint16_t *slots;
     int n_chunks, max_faces, chunk_elems, n_xcd;
+    // chunked variant: per leaf its index in bnd_elems (or -1); the chunk
+    // kernel then adds the boundary parts itself (k_adj_bnd_pts runs first)
+    const int *bnd_of;
 };
 
 // objective over a box of leaves: mode 0 per-block partial sums of
@@ -131,7 +135,6 @@ hipError_t launch_adj_stage(int prec, int nc, int flux, const AdjArgs &A, const 
 // LDS bytes per workgroup of the chunked adjoint stage kernel
 size_t adj_chunk_smem_bytes(int prec, int nc, int max_faces);
 hipError_t launch_adj_objective(int prec, int nc, const ObjArgs &A, const DevParams &P, hipStream_t st);
-hipError_t launch_adj_axpby(int prec, const void *x, void *y, double a, double b, int64_t m, hipStream_t st);
 
 hipError_t launch_stage_f32_ph0(int nc, int flux, const StageArgs &A, const DevParams &P, hipStream_t st);
 hipError_t launch_stage_f32_ph1(int nc, int flux, const StageArgs &A, const DevParams &P, hipStream_t st);
