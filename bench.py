@@ -70,9 +70,9 @@ def run_path(CA, W, level, precision, steps, warmup, u0=None):
                 state=state, m0=m0, m1=m1)
 
 
-def cpu_baseline(W, budget_s):
-    """The FP64 oracle (single thread) on the same workload at level 4
-    (4,096 hexes), steps until about budget_s of CPU time."""
+def cpu_probe(W, budget_s):
+    """One single-threaded FP64 oracle run on the level-4 Riemann box for
+    about budget_s; returns (DOF-updates, seconds, steps)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as OL
     import charm_3d_amd as CA
@@ -87,11 +87,43 @@ def cpu_baseline(W, budget_s):
         sim.run(dt, 1)
         steps += 1
         t = time.perf_counter() - t0
-    dofs = W.dof_updates(len(leaves), 5, 3, steps)
-    return {"value": dofs / t, "unit": "DOF-updates/s", "cores": 1, "kind": "port",
-            "sample": "FP64 oracle (C restatement of the reference numerics, AoS records, "
-                      "callback face traversal), Riemann box at level 4 (4,096 hexes), "
-                      "HLLC + BJ, SSP-RK3, %d steps in %.1f s on one host core" % (steps, t)}
+    return W.dof_updates(len(leaves), 5, 3, steps), t, steps
+
+
+def cpu_baseline(W, budget_s):
+    """The FP64 oracle (C restatement of the reference numerics, one thread
+    per run) on the same workload at level 4: one run alone, then one run
+    per host core at once (child processes that never touch the GPU).  The
+    aggregate of the concurrent runs is the upper bound of the reference's
+    MPI scaling on those cores (each rank would hold 1/P of the leaves)."""
+    import subprocess
+    dofs, t, steps = cpu_probe(W, budget_s)
+    single = {"value": dofs / t, "cores": 1,
+              "sample": "%d steps in %.1f s on one host core" % (steps, t)}
+    procs = int(os.environ.get("CHARM_CPU_PROCS", "0")) or min(16, os.cpu_count() or 1)
+    out = {"value": single["value"], "unit": "DOF-updates/s", "cores": 1, "kind": "port",
+           "sample": "FP64 oracle (C restatement of the reference numerics, AoS records, callback face "
+                     "traversal), Riemann box at level 4 (4,096 hexes), HLLC + BJ, SSP-RK3, " + single["sample"],
+           "single_core": single}
+    if procs > 1:
+        cmd = [sys.executable, os.path.abspath(__file__), "--cpu-probe", str(budget_s)]
+        ps = [subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True)
+              for _ in range(procs)]
+        res = []
+        for p in ps:
+            o, _ = p.communicate(timeout=budget_s * 10 + 120)
+            if p.returncode == 0:
+                res.append(json.loads(o.strip().splitlines()[-1]))
+        if len(res) == procs:
+            rate = sum(r["dofs"] / r["seconds"] for r in res)
+            out.update({"value": rate, "cores": procs,
+                        "sample": "FP64 oracle (C restatement of the reference numerics, AoS records, "
+                                  "callback face traversal), Riemann box at level 4 (4,096 hexes), HLLC + BJ, "
+                                  "SSP-RK3: %d single-threaded runs at once, one per host core, %d-%d steps in "
+                                  "%.1f s each; sum of their rates (the bound of MPI scaling on %d cores)"
+                                  % (procs, min(r["steps"] for r in res), max(r["steps"] for r in res),
+                                     budget_s, procs)})
+    return out
 
 
 def rel_errors(a, ref):
@@ -285,6 +317,7 @@ def main():
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--no-traffic", action="store_true")
     ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-probe", type=float, default=0.0, help=argparse.SUPPRESS)
     ap.add_argument("--only", choices=("bubbles", "hot_spot", "adjoint"),
                     help="run one of the other configurations alone and print its JSON")
     args = ap.parse_args()
@@ -293,8 +326,12 @@ def main():
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
-    import charm_3d_amd as CA
     import charm_3d_amd.workloads as W
+    if args.cpu_probe > 0:
+        dofs, t, steps = cpu_probe(W, args.cpu_probe)
+        print(json.dumps({"dofs": dofs, "seconds": t, "steps": steps}))
+        return
+    import charm_3d_amd as CA
 
     if args.traffic_probe:
         traffic_probe(CA, W, args.level)
