@@ -702,7 +702,13 @@ static int mesh_chunks(charm_mesh_t *m, int ch_e, int ch_f)
                     cur++;
                 }
                 slot_face[nslot] = fi;
-                slot_side[nslot] = (uint8_t) (m->f4[4*fi] == e ? 0 : 1);
+                /* a leaf that is its own periodic neighbour (a single leaf
+                 * across a periodic axis) holds both sides of the face: its
+                 * two incidences are adjacent, side 0 first */
+                if (m->f4[4*fi] == e && m->f4[4*fi+1] == e)
+                    slot_side[nslot] = (uint8_t) ((k > inc_ptr[e] && inc[k-1] == fi) ? 1 : 0);
+                else
+                    slot_side[nslot] = (uint8_t) (m->f4[4*fi] == e ? 0 : 1);
                 nslot++;
             }
         }

@@ -164,3 +164,9 @@ def test_level_mass_inverse_matches_oracle(level):
         leaves = MU.uniform_leaves(level)
         g = OL.OracleSim(OL.make_config(), leaves).geom(len(leaves) // 2)
         assert np.allclose(g["a_inv"], ai, rtol=1e-13, atol=1e-13 * ai.max())
+
+
+def test_chunk_faces_below_one_leaf_is_an_error():
+    """a chunk must hold every face of a leaf (six on a uniform mesh)"""
+    with pytest.raises(CA.CharmError, match="chunk_faces"):
+        CA.Forest.uniform(2).flatten(1, 5)
