@@ -185,6 +185,11 @@ def measure_traffic(level, copy_bytes):
     (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each a child process
     running traffic_probe; each counter is scaled by what it reads for the
     copy kernel of known size (copy_bytes / 2 read, copy_bytes / 2 written).
+    A third pass reads the stage kernel's VALU instruction counts and busy
+    cycles: valu_busy = the issue cycles of its wave64 VALU instructions (4
+    each on a 16-lane SIMD, over 4 SIMDs x 256 CUs) over the GPU's busy
+    cycles per XCD (GRBM_GUI_ACTIVE summed over the 8 XCDs); the
+    quarter-rate transcendentals are counted apart.
     Returns None when rocprofv3 is absent or a pass fails."""
     import csv
     import shutil
@@ -194,12 +199,13 @@ def measure_traffic(level, copy_bytes):
     if not prof:
         return None
     res = {}
+    passes = (("FETCH_SIZE",), ("WRITE_SIZE",), ("SQ_INSTS_VALU", "SQ_INSTS_VALU_TRANS_F32", "GRBM_GUI_ACTIVE"))
     with tempfile.TemporaryDirectory() as tmp:
-        for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
-            d = os.path.join(tmp, ctr)
-            cmd = [prof, "--pmc", ctr, "--output-format", "csv", "-d", d, "-o", "p", "--",
+        for i, ctrs in enumerate(passes):
+            d = os.path.join(tmp, "pass%d" % i)
+            cmd = [prof, "--pmc", *ctrs, "--output-format", "csv", "-d", d, "-o", "p", "--",
                    sys.executable, os.path.abspath(__file__), "--traffic-probe", "--level", str(level)]
-            print("[bench] counter pass %s" % ctr, file=sys.stderr, flush=True)
+            print("[bench] counter pass %s" % " ".join(ctrs), file=sys.stderr, flush=True)
             try:
                 r = subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=120)
             except subprocess.TimeoutExpired:
@@ -207,20 +213,27 @@ def measure_traffic(level, copy_bytes):
             path = os.path.join(d, "p_counter_collection.csv")
             if r.returncode != 0 or not os.path.exists(path):
                 return None
-            vals = {"stage": [], "copy": []}
+            vals = {}
             for row in csv.DictReader(open(path)):
-                if row["Counter_Name"] != ctr:
-                    continue
                 k = "stage" if "k_stage" in row["Kernel_Name"] else ("copy" if "k_copy16" in row["Kernel_Name"] else None)
-                if k:
-                    vals[k].append(float(row["Counter_Value"]) * 1024.0)
-            if not vals["stage"] or not vals["copy"]:
-                return None
-            res[ctr] = (sum(vals["stage"]) / len(vals["stage"]), sum(vals["copy"]) / len(vals["copy"]))
+                if k and row["Counter_Name"] in ctrs:
+                    vals.setdefault((k, row["Counter_Name"]), []).append(float(row["Counter_Value"]))
+            for c in ctrs:
+                for k in ("stage", "copy"):
+                    v = vals.get((k, c))
+                    if not v:
+                        return None
+                    res[(k, c)] = sum(v) / len(v)
     half = copy_bytes / 2.0
-    f_read, f_write = half / res["FETCH_SIZE"][1], half / res["WRITE_SIZE"][1]
-    return {"read": res["FETCH_SIZE"][0] * f_read, "write": res["WRITE_SIZE"][0] * f_write,
-            "fetch_scale": f_read, "write_scale": f_write}
+    f_read = half / (res[("copy", "FETCH_SIZE")] * 1024.0)
+    f_write = half / (res[("copy", "WRITE_SIZE")] * 1024.0)
+    simd_cycles = res[("stage", "SQ_INSTS_VALU")] * 4.0 / 1024.0
+    return {"read": res[("stage", "FETCH_SIZE")] * 1024.0 * f_read,
+            "write": res[("stage", "WRITE_SIZE")] * 1024.0 * f_write,
+            "fetch_scale": f_read, "write_scale": f_write,
+            "valu_insts": res[("stage", "SQ_INSTS_VALU")],
+            "valu_trans_insts": res[("stage", "SQ_INSTS_VALU_TRANS_F32")],
+            "valu_busy": simd_cycles / (res[("stage", "GRBM_GUI_ACTIVE")] / 8.0)}
 
 
 def extra_workloads(CA, W):
@@ -427,7 +440,9 @@ def main():
         tr = measure_traffic(args.level, copy_bytes)
         if tr:
             out["roofline"]["traffic"] = tr["read"] + tr["write"]
-            out["roofline"]["traffic_detail"] = tr
+            out["roofline"]["traffic_detail"] = {k: tr[k] for k in ("read", "write", "fetch_scale", "write_scale")}
+            # the bound the stage kernel actually sits at: its VALU issue rate
+            out["roofline"]["valu"] = {k: tr[k] for k in ("valu_insts", "valu_trans_insts", "valu_busy")}
 
     if not args.no_cpu:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
