@@ -98,6 +98,7 @@ def lib():
         "charm_mesh_destroy": (C.c_int, [vp]),
         "charm_mesh_num_faces": (i64, [vp]),
         "charm_mesh_num_chunks": (i64, [vp]),
+        "charm_mesh_num_chunk_faces": (i64, [vp]),
         "charm_mesh_get_faces": (C.c_int, [vp, i32p]),
         "charm_mesh_get_neighbors": (C.c_int, [vp, i32p]),
         "charm_gpu_create": (C.c_int, [C.POINTER(Config), C.POINTER(vp)]),
@@ -292,6 +293,10 @@ class Mesh:
     @property
     def num_chunks(self):
         return lib().charm_mesh_num_chunks(self.h)
+
+    @property
+    def num_chunk_faces(self):
+        return lib().charm_mesh_num_chunk_faces(self.h)
 
     def faces(self):
         out = np.zeros((self.num_faces, 4), dtype=np.int32)

@@ -770,6 +770,8 @@ int charm_mesh_destroy(charm_mesh_t *m)
 int64_t charm_mesh_num_faces(const charm_mesh_t *m) { return m ? m->nf : 0; }
 int64_t charm_mesh_num_chunks(const charm_mesh_t *m) { return m ? m->nch : 0; }
 
+int64_t charm_mesh_num_chunk_faces(const charm_mesh_t *m) { return m ? m->ncf : 0; }
+
 int charm_mesh_get_faces(const charm_mesh_t *m, int32_t *out)
 {
     if (!m || !out) { charm_set_error("mesh_get_faces: bad arguments"); return CHARM_ERR_ARG; }

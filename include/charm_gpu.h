@@ -147,6 +147,10 @@ int charm_forest_flatten(const charm_forest_t *f, int chunk_elems, int chunk_fac
 int charm_mesh_destroy(charm_mesh_t *m);
 int64_t charm_mesh_num_faces(const charm_mesh_t *m);
 int64_t charm_mesh_num_chunks(const charm_mesh_t *m);
+/* face records over all chunks: a face whose leaves lie in two chunks is
+ * held (and evaluated) by both; ratio to charm_mesh_num_faces = the stage
+ * kernel's face-work overhead */
+int64_t charm_mesh_num_chunk_faces(const charm_mesh_t *m);
 /* faces: [nf][4] = (e0, e1 (or -1-domain_face), info, 0) with info =
  * dir0 | kind << 3 | subface << 5; kind 0 conforming, 1 side 0 hanging
  * (small), 2 side 1 hanging (small), 3 boundary */
