@@ -52,7 +52,10 @@ def run_path(CA, W, level, precision, steps, warmup, u0=None):
     ce, cf = CA.recommended_chunks(cfg)
     if os.environ.get("CHARM_CHUNKS"):
         ce, cf = (int(x) for x in os.environ["CHARM_CHUNKS"].split(","))
-    ctx.upload_mesh(CA.Forest.from_leaves(leaves, periodic).flatten(ce, cf))
+    mesh = CA.Forest.from_leaves(leaves, periodic).flatten(ce, cf)
+    ctx.upload_mesh(mesh)
+    chunking = {"chunk_elems": ce, "chunk_faces": cf, "chunks": mesh.num_chunks, "faces": mesh.num_faces,
+                "face_copies_per_face": mesh.num_chunk_faces / mesh.num_faces}
     ctx.set_state(u0)
     dt = ctx.get_dt(mode=1)
     m0 = ctx.conservation_sums()
@@ -67,7 +70,7 @@ def run_path(CA, W, level, precision, steps, warmup, u0=None):
     state = ctx.get_state()
     m1 = ctx.conservation_sums()
     return dict(ctx=ctx, cfg=cfg, leaves=leaves, u0=u0, dt=dt, wall=wall, dev_ms=dev_ms,
-                state=state, m0=m0, m1=m1)
+                state=state, m0=m0, m1=m1, chunking=chunking)
 
 
 def cpu_probe(W, budget_s):
@@ -386,7 +389,8 @@ def main():
                                "+-1%% cell-mean perturbation, Euler P1 Taylor basis, uniform "
                                "octree level %d (%d hexes), HLLC flux + Barth-Jespersen limiter, "
                                "SSP-RK3, transmissive boundaries, dt = 0.3 h / max(|u|+c)" % (args.level, n),
-                   "elements": n, "vars": nv, "stages_per_step": 3, "dt": r32["dt"]},
+                   "elements": n, "vars": nv, "stages_per_step": 3, "dt": r32["dt"],
+                   "chunking": r32["chunking"]},
         "roofline": {"bound": "hbm", "kernel": "k_stage (fused face flux + volume + gather + update)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
