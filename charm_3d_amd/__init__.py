@@ -99,6 +99,7 @@ def lib():
         "charm_mesh_num_faces": (i64, [vp]),
         "charm_mesh_num_chunks": (i64, [vp]),
         "charm_mesh_num_chunk_faces": (i64, [vp]),
+        "charm_mesh_get_chunks": (C.c_int, [vp, i32p, i32p, i32p, i32p, C.POINTER(C.c_uint16)]),
         "charm_mesh_get_faces": (C.c_int, [vp, i32p]),
         "charm_mesh_get_neighbors": (C.c_int, [vp, i32p]),
         "charm_gpu_create": (C.c_int, [C.POINTER(Config), C.POINTER(vp)]),
@@ -297,6 +298,21 @@ class Mesh:
     @property
     def num_chunk_faces(self):
         return lib().charm_mesh_num_chunk_faces(self.h)
+
+    def chunks(self):
+        """(ch_ebeg, ch_fbeg, cf4, slot_ptr, slots) of charm_mesh_get_chunks"""
+        nch, ncf = self.num_chunks, self.num_chunk_faces
+        n = self.n
+        eb = np.zeros(nch + 1, dtype=np.int32)
+        fb = np.zeros(nch + 1, dtype=np.int32)
+        cf4 = np.zeros((ncf, 4), dtype=np.int32)
+        sp = np.zeros(n + 1, dtype=np.int32)
+        L = lib()
+        # slot count = slot_ptr[n]; read the pointers first through a full call
+        sl = np.zeros(24 * n + 1, dtype=np.uint16)
+        check(L.charm_mesh_get_chunks(self.h, _i32ptr(eb), _i32ptr(fb), _i32ptr(cf4), _i32ptr(sp),
+                                      sl.ctypes.data_as(C.POINTER(C.c_uint16))))
+        return eb, fb, cf4, sp, sl[:sp[-1]]
 
     def faces(self):
         out = np.zeros((self.num_faces, 4), dtype=np.int32)

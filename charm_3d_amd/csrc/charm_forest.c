@@ -586,41 +586,195 @@ static int mesh_neighbors(charm_mesh_t *m)
     return CHARM_OK;
 }
 
-/* Close a chunk: order its faces by (axis, kind) so that a wave of the
- * face phase mostly sees one face orientation, give each face its local
- * index and translate the chunk's element slots to local indices. */
-static void finish_chunk(charm_mesh_t *m, const int32_t *cf_face, int64_t fb, int64_t fe,
-                         int64_t eb, int64_t ee, const int32_t *slot_face, const uint8_t *slot_side,
-                         int32_t *loc, int32_t *sorted)
+/* ------------------------------------------------------------ chunks
+ *
+ * Workgroup chunks: Morton-consecutive leaves, at most ch_e leaves and ch_f
+ * distinct faces per chunk, grown greedily.  The leaves are cut into blocks
+ * of CHUNK_BLOCK leaves that are chunked independently on host threads (a
+ * chunk never spans two blocks), so the layout depends on the leaf count
+ * and the chunk limits only, not on the thread count.  Each chunk's faces
+ * are ordered by (axis, kind) so that a wave of the face phase mostly sees
+ * one face orientation; the slots of a leaf are its face incidences in
+ * incidence order, (local face << 1 | side). */
+#define CHUNK_BLOCK 32768
+
+typedef struct {
+    const charm_mesh_t *m;
+    const int32_t *inc_ptr, *inc;
+    int ch_e, ch_f;
+    int64_t n;
+    uint16_t *slots;            /* global, at incidence positions */
+    /* per block */
+    int32_t **ch_ebeg, **ch_nf, **cf;
+    int64_t *nch, *ncf;
+    int *err;
+} chunk_job_t;
+
+typedef struct {
+    int32_t *key, *val;
+    int cap;
+} face_map_t;
+
+static int map_find(const face_map_t *h, int32_t k)
 {
-    int64_t cnt[13] = {0}, i, e, s;
-    for (i = fb; i < fe; i++) {
-        int info = m->f4[4 * cf_face[i] + 2];
+    unsigned i = ((unsigned) k * 2654435761u) & (unsigned) (h->cap - 1);
+    while (h->key[i] != -1) {
+        if (h->key[i] == k) return h->val[i];
+        i = (i + 1) & (unsigned) (h->cap - 1);
+    }
+    return -1;
+}
+
+static void map_put(face_map_t *h, int32_t k, int32_t v)
+{
+    unsigned i = ((unsigned) k * 2654435761u) & (unsigned) (h->cap - 1);
+    while (h->key[i] != -1) i = (i + 1) & (unsigned) (h->cap - 1);
+    h->key[i] = k;
+    h->val[i] = v;
+}
+
+static void map_clear(face_map_t *h, const int32_t *keys, int64_t nk)
+{
+    int64_t j;
+    for (j = 0; j < nk; j++) {
+        unsigned i = ((unsigned) keys[j] * 2654435761u) & (unsigned) (h->cap - 1);
+        while (h->key[i] != keys[j]) i = (i + 1) & (unsigned) (h->cap - 1);
+        h->key[i] = -1;
+    }
+    /* re-insert displaced entries is unnecessary: the whole chunk is
+     * cleared, so the table is empty again */
+}
+
+/* the side of leaf e in face fi at incidence k: a leaf that is its own
+ * periodic neighbour holds both sides, its two incidences adjacent, side 0
+ * first */
+static int slot_side(const charm_mesh_t *m, const int32_t *inc_ptr, const int32_t *inc, int64_t e,
+                     int64_t k, int32_t fi)
+{
+    if (m->f4[4*fi] == e && m->f4[4*fi+1] == e) return (k > inc_ptr[e] && inc[k-1] == fi) ? 1 : 0;
+    return m->f4[4*fi] == e ? 0 : 1;
+}
+
+/* close the chunk [eb, ee) whose faces are cf[0..nf): sort them by (axis,
+ * kind), then write its leaves' slots with the sorted local indices */
+static void close_chunk(const chunk_job_t *J, face_map_t *h, int32_t *cf, int64_t nf, int64_t eb, int64_t ee,
+                        int32_t *tmp, int32_t *rank)
+{
+    const charm_mesh_t *m = J->m;
+    int64_t cnt[13] = {0}, i, e, k;
+    for (i = 0; i < nf; i++) {
+        const int info = m->f4[4 * cf[i] + 2];
         cnt[1 + ((info & 7) >> 1) * 4 + ((info >> 3) & 3)]++;
     }
     for (i = 1; i < 13; i++) cnt[i] += cnt[i - 1];
-    for (i = fb; i < fe; i++) {
-        int info = m->f4[4 * cf_face[i] + 2];
-        sorted[cnt[((info & 7) >> 1) * 4 + ((info >> 3) & 3)]++] = cf_face[i];
-    }
-    for (i = 0; i < fe - fb; i++) {
-        loc[sorted[i]] = (int32_t) i;
-        memcpy(&m->cf4[4 * (fb + i)], &m->f4[4 * sorted[i]], 4 * sizeof(int32_t));
+    for (i = 0; i < nf; i++) {
+        const int info = m->f4[4 * cf[i] + 2];
+        const int64_t to = cnt[((info & 7) >> 1) * 4 + ((info >> 3) & 3)]++;
+        tmp[to] = cf[i];
+        rank[i] = (int32_t) to;             /* provisional local index i -> sorted */
     }
     for (e = eb; e < ee; e++)
-        for (s = m->slot_ptr[e]; s < m->slot_ptr[e + 1]; s++)
-            m->slots[s] = (uint16_t) ((loc[slot_face[s]] << 1) | slot_side[s]);
+        for (k = J->inc_ptr[e]; k < J->inc_ptr[e + 1]; k++) {
+            const int32_t fi = J->inc[k];
+            J->slots[k] = (uint16_t) ((rank[map_find(h, fi)] << 1) | slot_side(m, J->inc_ptr, J->inc, e, k, fi));
+        }
+    map_clear(h, cf, nf);
+    memcpy(cf, tmp, (size_t) nf * sizeof(int32_t));
+}
+
+static void chunk_blocks(void *arg, int64_t lo, int64_t hi, int part)
+{
+    chunk_job_t *J = (chunk_job_t *) arg;
+    const int32_t *inc_ptr = J->inc_ptr, *inc = J->inc;
+    face_map_t h;
+    int32_t *tmp = NULL, *rank = NULL;
+    int64_t b;
+    (void) part;
+    h.cap = 1;
+    while (h.cap < 4 * (J->ch_f + 1)) h.cap <<= 1;
+    h.key = (int32_t *) malloc((size_t) h.cap * sizeof(int32_t));
+    h.val = (int32_t *) malloc((size_t) h.cap * sizeof(int32_t));
+    tmp = (int32_t *) malloc((size_t) (J->ch_f + 1) * sizeof(int32_t));
+    rank = (int32_t *) malloc((size_t) (J->ch_f + 1) * sizeof(int32_t));
+    if (!h.key || !h.val || !tmp || !rank) {
+        for (b = lo; b < hi; b++) J->err[b] = CHARM_ERR_MEM;
+        goto done;
+    }
+    memset(h.key, 0xff, (size_t) h.cap * sizeof(int32_t));
+    for (b = lo; b < hi; b++) {
+        const int64_t e0 = b * CHUNK_BLOCK, e1 = (e0 + CHUNK_BLOCK < J->n) ? e0 + CHUNK_BLOCK : J->n;
+        const int64_t nfb = inc_ptr[e1] - inc_ptr[e0];     /* incidences: bound on the face copies */
+        int32_t *cf = (int32_t *) malloc((size_t) (nfb + 1) * sizeof(int32_t));
+        int32_t *ceb = (int32_t *) malloc((size_t) (e1 - e0 + 1) * sizeof(int32_t));
+        int32_t *cnf = (int32_t *) malloc((size_t) (e1 - e0 + 1) * sizeof(int32_t));
+        int64_t e, k, eb = e0, fb = 0, cur = 0, nch = 0;
+        J->cf[b] = cf; J->ch_ebeg[b] = ceb; J->ch_nf[b] = cnf;
+        if (!cf || !ceb || !cnf) { J->err[b] = CHARM_ERR_MEM; continue; }
+        for (e = e0; e < e1; e++) {
+            int64_t nnew = 0;
+            for (k = inc_ptr[e]; k < inc_ptr[e + 1]; k++)
+                if (map_find(&h, inc[k]) < 0 && !(k > inc_ptr[e] && inc[k - 1] == inc[k])) nnew++;
+            if (e > eb && (e - eb >= J->ch_e || cur + nnew > J->ch_f)) {
+                close_chunk(J, &h, cf + fb, cur, eb, e, tmp, rank);
+                ceb[nch] = (int32_t) eb; cnf[nch] = (int32_t) cur; nch++;
+                fb += cur; eb = e; cur = 0;
+                nnew = 0;
+                for (k = inc_ptr[e]; k < inc_ptr[e + 1]; k++)
+                    if (!(k > inc_ptr[e] && inc[k - 1] == inc[k])) nnew++;
+            }
+            if (nnew > J->ch_f) { J->err[b] = CHARM_ERR_ARG; break; }
+            for (k = inc_ptr[e]; k < inc_ptr[e + 1]; k++)
+                if (map_find(&h, inc[k]) < 0) {
+                    map_put(&h, inc[k], (int32_t) cur);
+                    cf[fb + cur] = inc[k];
+                    cur++;
+                }
+        }
+        if (J->err[b]) { map_clear(&h, cf + fb, cur); continue; }
+        close_chunk(J, &h, cf + fb, cur, eb, e1, tmp, rank);
+        ceb[nch] = (int32_t) eb; cnf[nch] = (int32_t) cur; nch++;
+        J->nch[b] = nch;
+        J->ncf[b] = fb + cur;
+    }
+done:
+    free(h.key); free(h.val); free(tmp); free(rank);
+}
+
+typedef struct {
+    charm_mesh_t *m;
+    const chunk_job_t *J;
+    const int64_t *ch_off, *cf_off;
+} chunk_merge_t;
+
+static void chunk_merge(void *arg, int64_t lo, int64_t hi, int part)
+{
+    chunk_merge_t *M = (chunk_merge_t *) arg;
+    charm_mesh_t *m = M->m;
+    int64_t b, c, i;
+    (void) part;
+    for (b = lo; b < hi; b++) {
+        int64_t f = M->cf_off[b];
+        for (c = 0; c < M->J->nch[b]; c++) {
+            m->ch_ebeg[M->ch_off[b] + c] = M->J->ch_ebeg[b][c];
+            m->ch_fbeg[M->ch_off[b] + c] = (int32_t) f;
+            f += M->J->ch_nf[b][c];
+        }
+        for (i = 0; i < M->J->ncf[b]; i++)
+            memcpy(&m->cf4[4 * (M->cf_off[b] + i)], &m->f4[4 * M->J->cf[b][i]], 4 * sizeof(int32_t));
+    }
 }
 
 static int mesh_chunks(charm_mesh_t *m, int ch_e, int ch_f)
 {
-    int64_t n = m->n, nf = m->nf, i, e;
-    int32_t *inc_ptr = NULL, *inc = NULL, *stamp = NULL, *loc = NULL, *cf_face = NULL;
-    int32_t *slot_face = NULL, *sorted = NULL;
-    uint8_t *slot_side = NULL;
-    int64_t nch = 0, ncf = 0, nslot = 0, cap_ch, cap_cf;
+    const int64_t n = m->n, nf = m->nf, nblk = (n + CHUNK_BLOCK - 1) / CHUNK_BLOCK;
+    int32_t *inc_ptr = NULL, *inc = NULL, *fill = NULL;
+    int64_t i, e, b, *ch_off = NULL, *cf_off = NULL;
+    chunk_job_t J;
     int rc = CHARM_ERR_MEM;
+    memset(&J, 0, sizeof(J));
 
+    /* leaf -> incident faces, in face order (a face joining a leaf to
+     * itself appears twice, adjacent) */
     inc_ptr = (int32_t *) calloc((size_t) n + 1, sizeof(int32_t));
     if (!inc_ptr) goto done;
     for (i = 0; i < nf; i++) {
@@ -628,106 +782,61 @@ static int mesh_chunks(charm_mesh_t *m, int ch_e, int ch_f)
         if (m->f4[4*i+1] >= 0) inc_ptr[m->f4[4*i+1] + 1]++;
     }
     for (e = 0; e < n; e++) inc_ptr[e+1] += inc_ptr[e];
-    inc = (int32_t *) malloc((size_t) inc_ptr[n] * sizeof(int32_t));
-    stamp = (int32_t *) malloc((size_t) nf * sizeof(int32_t));
-    loc = (int32_t *) malloc((size_t) nf * sizeof(int32_t));
-    slot_face = (int32_t *) malloc((size_t) inc_ptr[n] * sizeof(int32_t));
-    slot_side = (uint8_t *) malloc((size_t) inc_ptr[n]);
-    sorted = (int32_t *) malloc((size_t) (ch_f + 1) * sizeof(int32_t));
-    if (!inc || !stamp || !loc || !slot_face || !slot_side || !sorted) goto done;
-    {
-        int32_t *fill = (int32_t *) malloc((size_t) n * sizeof(int32_t));
-        if (!fill) goto done;
-        memcpy(fill, inc_ptr, (size_t) n * sizeof(int32_t));
-        for (i = 0; i < nf; i++) {
-            inc[fill[m->f4[4*i]]++] = (int32_t) i;
-            if (m->f4[4*i+1] >= 0) inc[fill[m->f4[4*i+1]]++] = (int32_t) i;
-        }
-        free(fill);
+    inc = (int32_t *) malloc((size_t) (inc_ptr[n] + 1) * sizeof(int32_t));
+    fill = (int32_t *) malloc((size_t) (n + 1) * sizeof(int32_t));
+    if (!inc || !fill) goto done;
+    memcpy(fill, inc_ptr, (size_t) n * sizeof(int32_t));
+    for (i = 0; i < nf; i++) {
+        inc[fill[m->f4[4*i]]++] = (int32_t) i;
+        if (m->f4[4*i+1] >= 0) inc[fill[m->f4[4*i+1]]++] = (int32_t) i;
     }
-    for (i = 0; i < nf; i++) stamp[i] = -1;
 
-    cap_ch = n / (ch_e > 0 ? ch_e : 1) + 16;
-    cap_cf = nf * 2 + 64;
-    m->ch_ebeg = (int32_t *) malloc((size_t) (cap_ch + 1) * sizeof(int32_t));
-    m->ch_fbeg = (int32_t *) malloc((size_t) (cap_ch + 1) * sizeof(int32_t));
-    m->cf4 = (int32_t *) malloc((size_t) cap_cf * 4 * sizeof(int32_t));
-    cf_face = (int32_t *) malloc((size_t) cap_cf * sizeof(int32_t));
     m->slot_ptr = (int32_t *) malloc((size_t) (n + 1) * sizeof(int32_t));
-    m->slots = (uint16_t *) malloc((size_t) inc_ptr[n] * sizeof(uint16_t));
-    if (!m->ch_ebeg || !m->ch_fbeg || !m->cf4 || !cf_face || !m->slot_ptr || !m->slots) goto done;
-
-    {
-        int64_t ebeg = 0, cur = 0;
-        m->ch_ebeg[0] = 0; m->ch_fbeg[0] = 0;
-        for (e = 0; e < n; e++) {
-            int64_t nnew = 0, k;
-            for (k = inc_ptr[e]; k < inc_ptr[e+1]; k++) if (stamp[inc[k]] != nch) nnew++;
-            if (e > ebeg && (e - ebeg >= ch_e || cur + nnew > ch_f)) {
-                m->slot_ptr[e] = (int32_t) nslot;
-                finish_chunk(m, cf_face, m->ch_fbeg[nch], ncf, ebeg, e, slot_face, slot_side, loc, sorted);
-                nch++;
-                if (nch >= cap_ch) {
-                    int32_t *a, *b;
-                    cap_ch *= 2;
-                    a = (int32_t *) realloc(m->ch_ebeg, (size_t) (cap_ch + 1) * sizeof(int32_t));
-                    if (!a) goto done;
-                    m->ch_ebeg = a;
-                    b = (int32_t *) realloc(m->ch_fbeg, (size_t) (cap_ch + 1) * sizeof(int32_t));
-                    if (!b) goto done;
-                    m->ch_fbeg = b;
-                }
-                m->ch_ebeg[nch] = (int32_t) e;
-                m->ch_fbeg[nch] = (int32_t) ncf;
-                ebeg = e; cur = 0;
-                nnew = inc_ptr[e+1] - inc_ptr[e];
-            }
-            if (nnew > ch_f) { charm_set_error("flatten: chunk_faces %d below one element's faces", ch_f); rc = CHARM_ERR_ARG; goto done; }
-            m->slot_ptr[e] = (int32_t) nslot;
-            for (k = inc_ptr[e]; k < inc_ptr[e+1]; k++) {
-                int32_t fi = inc[k];
-                if (stamp[fi] != nch) {
-                    stamp[fi] = (int32_t) nch;
-                    if (ncf == cap_cf) {
-                        int32_t *t, *t2;
-                        cap_cf *= 2;
-                        t = (int32_t *) realloc(m->cf4, (size_t) cap_cf * 4 * sizeof(int32_t));
-                        if (!t) goto done;
-                        m->cf4 = t;
-                        t2 = (int32_t *) realloc(cf_face, (size_t) cap_cf * sizeof(int32_t));
-                        if (!t2) goto done;
-                        cf_face = t2;
-                    }
-                    cf_face[ncf++] = fi;
-                    cur++;
-                }
-                slot_face[nslot] = fi;
-                /* a leaf that is its own periodic neighbour (a single leaf
-                 * across a periodic axis) holds both sides of the face: its
-                 * two incidences are adjacent, side 0 first */
-                if (m->f4[4*fi] == e && m->f4[4*fi+1] == e)
-                    slot_side[nslot] = (uint8_t) ((k > inc_ptr[e] && inc[k-1] == fi) ? 1 : 0);
-                else
-                    slot_side[nslot] = (uint8_t) (m->f4[4*fi] == e ? 0 : 1);
-                nslot++;
-            }
+    m->slots = (uint16_t *) malloc((size_t) (inc_ptr[n] + 1) * sizeof(uint16_t));
+    J.ch_ebeg = (int32_t **) calloc((size_t) nblk + 1, sizeof(int32_t *));
+    J.ch_nf = (int32_t **) calloc((size_t) nblk + 1, sizeof(int32_t *));
+    J.cf = (int32_t **) calloc((size_t) nblk + 1, sizeof(int32_t *));
+    J.nch = (int64_t *) calloc((size_t) nblk + 1, sizeof(int64_t));
+    J.ncf = (int64_t *) calloc((size_t) nblk + 1, sizeof(int64_t));
+    J.err = (int *) calloc((size_t) nblk + 1, sizeof(int));
+    ch_off = (int64_t *) malloc((size_t) (nblk + 1) * sizeof(int64_t));
+    cf_off = (int64_t *) malloc((size_t) (nblk + 1) * sizeof(int64_t));
+    if (!m->slot_ptr || !m->slots || !J.ch_ebeg || !J.ch_nf || !J.cf || !J.nch || !J.ncf || !J.err ||
+        !ch_off || !cf_off)
+        goto done;
+    memcpy(m->slot_ptr, inc_ptr, (size_t) (n + 1) * sizeof(int32_t));
+    J.m = m; J.inc_ptr = inc_ptr; J.inc = inc; J.ch_e = ch_e; J.ch_f = ch_f; J.n = n; J.slots = m->slots;
+    par_for(nblk, par_parts(nblk, 1), chunk_blocks, &J);
+    for (b = 0; b < nblk; b++)
+        if (J.err[b]) {
+            rc = J.err[b];
+            if (rc == CHARM_ERR_ARG) charm_set_error("flatten: chunk_faces %d below one element's faces", ch_f);
+            goto done;
         }
-        m->slot_ptr[n] = (int32_t) nslot;
-        finish_chunk(m, cf_face, m->ch_fbeg[nch], ncf, ebeg, n, slot_face, slot_side, loc, sorted);
-        nch++;
-        m->ch_ebeg[nch] = (int32_t) n;
-        m->ch_fbeg[nch] = (int32_t) ncf;
+    ch_off[0] = cf_off[0] = 0;
+    for (b = 0; b < nblk; b++) { ch_off[b + 1] = ch_off[b] + J.nch[b]; cf_off[b + 1] = cf_off[b] + J.ncf[b]; }
+    m->nch = ch_off[nblk];
+    m->ncf = cf_off[nblk];
+    m->ch_ebeg = (int32_t *) malloc((size_t) (m->nch + 1) * sizeof(int32_t));
+    m->ch_fbeg = (int32_t *) malloc((size_t) (m->nch + 1) * sizeof(int32_t));
+    m->cf4 = (int32_t *) malloc((size_t) (m->ncf + 1) * 4 * sizeof(int32_t));
+    if (!m->ch_ebeg || !m->ch_fbeg || !m->cf4) goto done;
+    {
+        chunk_merge_t M;
+        M.m = m; M.J = &J; M.ch_off = ch_off; M.cf_off = cf_off;
+        par_for(nblk, par_parts(nblk, 1), chunk_merge, &M);
     }
-    m->nch = nch;
-    m->ncf = ncf;
-    m->nslot = nslot;
+    m->ch_ebeg[m->nch] = (int32_t) n;
+    m->ch_fbeg[m->nch] = (int32_t) m->ncf;
+    m->nslot = inc_ptr[n];
     m->chunk_elems = ch_e;
     m->chunk_faces = ch_f;
     rc = CHARM_OK;
 done:
     if (rc == CHARM_ERR_MEM) charm_set_error("flatten: out of memory");
-    free(inc_ptr); free(inc); free(stamp); free(loc); free(cf_face);
-    free(slot_face); free(slot_side); free(sorted);
+    for (b = 0; b < nblk && J.cf; b++) { free(J.cf[b]); free(J.ch_ebeg[b]); free(J.ch_nf[b]); }
+    free(J.ch_ebeg); free(J.ch_nf); free(J.cf); free(J.nch); free(J.ncf); free(J.err);
+    free(ch_off); free(cf_off); free(inc_ptr); free(inc); free(fill);
     return rc;
 }
 
@@ -771,6 +880,21 @@ int64_t charm_mesh_num_faces(const charm_mesh_t *m) { return m ? m->nf : 0; }
 int64_t charm_mesh_num_chunks(const charm_mesh_t *m) { return m ? m->nch : 0; }
 
 int64_t charm_mesh_num_chunk_faces(const charm_mesh_t *m) { return m ? m->ncf : 0; }
+
+int charm_mesh_get_chunks(const charm_mesh_t *m, int32_t *ch_ebeg, int32_t *ch_fbeg, int32_t *cf4,
+                          int32_t *slot_ptr, uint16_t *slots)
+{
+    if (!m || !ch_ebeg || !ch_fbeg || !cf4 || !slot_ptr || !slots) {
+        charm_set_error("mesh_get_chunks: bad arguments");
+        return CHARM_ERR_ARG;
+    }
+    memcpy(ch_ebeg, m->ch_ebeg, (size_t) (m->nch + 1) * sizeof(int32_t));
+    memcpy(ch_fbeg, m->ch_fbeg, (size_t) (m->nch + 1) * sizeof(int32_t));
+    memcpy(cf4, m->cf4, (size_t) m->ncf * 4 * sizeof(int32_t));
+    memcpy(slot_ptr, m->slot_ptr, (size_t) (m->n + 1) * sizeof(int32_t));
+    memcpy(slots, m->slots, (size_t) m->nslot * sizeof(uint16_t));
+    return CHARM_OK;
+}
 
 int charm_mesh_get_faces(const charm_mesh_t *m, int32_t *out)
 {

@@ -151,6 +151,12 @@ int64_t charm_mesh_num_chunks(const charm_mesh_t *m);
  * held (and evaluated) by both; ratio to charm_mesh_num_faces = the stage
  * kernel's face-work overhead */
 int64_t charm_mesh_num_chunk_faces(const charm_mesh_t *m);
+/* the workgroup chunks: ch_ebeg / ch_fbeg [nch + 1] leaf and face-record
+ * ranges, cf4 [ncf][4] the chunk-local face records (sorted by axis, then
+ * kind, within a chunk), slot_ptr [n + 1] and slots [slot_ptr[n]] the
+ * leaves' face incidences as (chunk-local face << 1 | side) */
+int charm_mesh_get_chunks(const charm_mesh_t *m, int32_t *ch_ebeg, int32_t *ch_fbeg, int32_t *cf4,
+                          int32_t *slot_ptr, uint16_t *slots);
 /* faces: [nf][4] = (e0, e1 (or -1-domain_face), info, 0) with info =
  * dir0 | kind << 3 | subface << 5; kind 0 conforming, 1 side 0 hanging
  * (small), 2 side 1 hanging (small), 3 boundary */

@@ -170,3 +170,73 @@ def test_chunk_faces_below_one_leaf_is_an_error():
     """a chunk must hold every face of a leaf (six on a uniform mesh)"""
     with pytest.raises(CA.CharmError, match="chunk_faces"):
         CA.Forest.uniform(2).flatten(1, 5)
+
+
+def check_chunks(f, ce, cf):
+    """every invariant the stage kernel relies on: chunks partition the
+    leaves in order with at most ce leaves and cf distinct faces, a chunk
+    holds exactly the faces incident to its leaves, sorted by (axis, kind),
+    and every leaf's slots decode to its own incidences (face, side)"""
+    m = f.flatten(ce, cf)
+    faces = m.faces()
+    eb, fb, cf4, sp, sl = m.chunks()
+    n = f.n
+    assert eb[0] == 0 and eb[-1] == n and (np.diff(eb) > 0).all() and (np.diff(eb) <= ce).all()
+    assert fb[0] == 0 and fb[-1] == len(cf4) and (np.diff(fb) <= cf).all()
+    rows = {tuple(r): i for i, r in enumerate(faces.tolist())}
+    assert len(rows) == len(faces)
+    inc = [[] for _ in range(n)]
+    for i, (e0, e1, _, _) in enumerate(faces.tolist()):
+        inc[e0].append((i, 0))
+        if e1 >= 0:
+            inc[e1].append((i, 1))
+    assert sp[-1] == sum(len(x) for x in inc)
+    for c in range(len(eb) - 1):
+        recs = cf4[fb[c]:fb[c + 1]]
+        ids = [rows[tuple(r)] for r in recs.tolist()]
+        assert len(set(ids)) == len(ids)
+        key = [((r[2] & 7) >> 1) * 4 + ((r[2] >> 3) & 3) for r in recs.tolist()]
+        assert key == sorted(key)
+        want = set()
+        for e in range(eb[c], eb[c + 1]):
+            want |= {i for i, _ in inc[e]}
+            got = [(ids[s >> 1], s & 1) for s in sl[sp[e]:sp[e + 1]].tolist()]
+            assert sorted(got) == sorted(inc[e]), (c, e)
+        assert set(ids) == want
+    return m
+
+
+@pytest.mark.parametrize("case", ["uniform4", "adapted", "adapted_periodic", "single_periodic"])
+def test_chunks_hold_their_leaves_faces(case):
+    if case == "uniform4":
+        f = CA.Forest.uniform(4)
+    elif case == "adapted":
+        f = CA.Forest.from_leaves(MU.random_adapted(2, 5, seed=31, frac=0.3))
+    elif case == "adapted_periodic":
+        per = (True, False, True)
+        f = CA.Forest.from_leaves(MU.random_adapted(2, 4, seed=37, frac=0.25, periodic=per), per)
+    else:
+        f = CA.Forest.uniform(0, (True, True, True))
+    for ce, cf in ((1, 24), (7, 40), (128, 512), (256, 1024)):
+        check_chunks(f, ce, cf)
+
+
+def test_chunk_layout_spans_blocks():
+    """a level-6 forest (262,144 leaves: eight independently chunked blocks
+    of 32,768) gives the same face-copy count whatever the thread count, and
+    no chunk crosses a block boundary"""
+    import subprocess
+    import sys
+    code = ("import sys; sys.path.insert(0, %r); import charm_3d_amd as CA; "
+            "m = CA.Forest.uniform(6).flatten(213, 640); eb = m.chunks()[0]; "
+            "print(m.num_chunks, m.num_chunk_faces, int(((eb[:-1] // 32768) != ((eb[1:] - 1) // 32768)).sum()))"
+            % os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    outs = set()
+    for th in ("1", "3", "8"):
+        r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300,
+                           env=dict(os.environ, CHARM_HOST_THREADS=th))
+        assert r.returncode == 0, r.stderr
+        outs.add(r.stdout.strip())
+    assert len(outs) == 1
+    nch, ncf, crossing = (int(x) for x in outs.pop().split())
+    assert crossing == 0 and ncf > CA.Forest.uniform(6).flatten(213, 640).num_faces
