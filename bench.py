@@ -140,14 +140,17 @@ def smooth_errors(CA, W):
     """FP32 bulk and FP32 mixed (FP64 state, FP32 residual) against the
     FP64 plain path on the two smooth configurations: the advected sine
     wave (level 4, LF, periodic, 100 steps) and the Gaussian pulse (level 6,
-    HLLC, inlet, 400 steps)."""
+    HLLC, inlet, 400 steps); and the FP64 path from the FP32-rounded initial
+    state, the case's own sensitivity to a 6e-8 change."""
     out = {}
     cases = (("sine_wave_level4_lf_100_steps", lambda prec: W.c1(4, prec), 100),
              ("gaussian_pulse_level6_hllc_400_steps", lambda prec: W.c5_pulse(6, prec), 400))
     for name, mk, steps in cases:
         res = {}
-        for prec in (CA.FP64, CA.FP32, CA.FP32_MIX):
-            cfg, leaves, u0, per = mk(prec)
+        for prec in (CA.FP64, CA.FP32, CA.FP32_MIX, "rounded"):
+            cfg, leaves, u0, per = mk(CA.FP64 if prec == "rounded" else prec)
+            if prec == "rounded":
+                u0 = u0.astype(np.float32).astype(np.float64)
             ctx = CA.GpuContext(cfg)
             ce, cf = CA.recommended_chunks(cfg)
             ctx.upload_mesh(CA.Forest.from_leaves(leaves, per).flatten(ce, cf))
@@ -158,13 +161,14 @@ def smooth_errors(CA, W):
             m1 = ctx.conservation_sums()
             res[prec] = ctx.get_state()
             res["dt"] = dt
-            res["drift%d" % prec] = abs(m1[4] - m0[4]) / abs(m0[4])
+            res["drift%s" % prec] = abs(m1[4] - m0[4]) / abs(m0[4])
             ctx.close()
         e = rel_errors(res[CA.FP32], res[CA.FP64])
-        e["mass_drift_fp32"] = res["drift%d" % CA.FP32]
-        e["mass_drift_fp64"] = res["drift%d" % CA.FP64]
+        e["mass_drift_fp32"] = res["drift%s" % CA.FP32]
+        e["mass_drift_fp64"] = res["drift%s" % CA.FP64]
         e["fp32_mix"] = rel_errors(res[CA.FP32_MIX], res[CA.FP64])
-        e["fp32_mix"]["mass_drift"] = res["drift%d" % CA.FP32_MIX]
+        e["fp32_mix"]["mass_drift"] = res["drift%s" % CA.FP32_MIX]
+        e["fp64_from_fp32_rounded_input"] = rel_errors(res["rounded"], res[CA.FP64])
         out[name] = e
     return out
 
