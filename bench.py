@@ -10,7 +10,8 @@ value = DOF-updates/s of the FP32 path (E * 4 * V * 3 stages * K / wall).
                     [--no-traffic]
 
 The JSON line also carries the other configurations at full size (adaptive
-bubbles, reacting hot spot, adjoint of the pulse case) under "workloads",
+bubbles, reacting hot spot, adjoint of the pulse case with its FP64 reference
+gradient checked against central differences) under "workloads",
 the FP32 errors of the smooth cases, the FP64 path's own sensitivity to the
 FP32 rounding of the initial state, and the stage kernel's HBM traffic per
 launch from two rocprofv3 counter passes (child processes).
@@ -319,9 +320,71 @@ def bench_adjoint(CA, W):
     lo, hi = W.C5_BOX
     J, g, th, fwd_ms, adj_ms = ctx.adjoint(dt, 400, lo, hi, ckpt=20, theta_face=0)
     ctx.close()
-    return {"leaves": len(leaves), "steps": 400, "checkpoint_every": 20, "J": J,
-            "dJ_dinlet_u_T_p": th.tolist(), "forward_ms": fwd_ms, "adjoint_ms": adj_ms,
-            "adjoint_over_forward": adj_ms / fwd_ms}
+    out = {"leaves": len(leaves), "steps": 400, "checkpoint_every": 20, "J": J,
+           "dJ_dinlet_u_T_p": th.tolist(), "forward_ms": fwd_ms, "adjoint_ms": adj_ms,
+           "adjoint_over_forward": adj_ms / fwd_ms}
+    out["fp64_reference"] = adjoint_fd_check(CA, W, dt, g, th)
+    return out
+
+
+def adjoint_fd_check(CA, W, dt, g32, th32, ndir=32, steps=(1e-6, 1e-7)):
+    """The FP64 adjoint of the same case, checked against central differences
+    of the FP64 forward objective: dJ/du0 along ndir random directions
+    (per-variable scaled), dJ/d(inlet u, T, p) per parameter, each at the
+    relative steps in `steps`; and the FP32 gradient against the FP64 one.
+    The objective has kinks (the switch of the HLLC wave-speed factor at
+    p* = p_k, min / max of the wave speeds), so a central difference carries an
+    error linear in h wherever a perturbation moves a face point across one;
+    the adjoint is the derivative of the branch taken."""
+    lo, hi = W.C5_BOX
+    inlet = np.array([0.5, 1.0, 1.0])
+
+    def device(u, theta):
+        cfg, leaves, u0, per = W.c5_pulse(6, CA.FP64, inlet=tuple(theta))
+        ctx = CA.GpuContext(cfg)
+        ce, cf = CA.recommended_chunks(cfg)
+        ctx.upload_mesh(CA.Forest.uniform(6).flatten(ce, cf))
+        ctx.set_state(u0 if u is None else u)
+        return ctx, u0
+
+    ctx, u0 = device(None, inlet)
+    J, g, th, fwd_ms, adj_ms = ctx.adjoint(dt, 400, lo, hi, ckpt=20, theta_face=0)
+    rng = np.random.default_rng(32)
+    scale = np.abs(u0).max(axis=0, keepdims=True)
+    dirs = [rng.normal(size=u0.shape) * scale for _ in range(ndir)]
+    ad = np.array([float((g * v).sum()) for v in dirs])
+    cd = {}
+    for h in steps:
+        fd = []
+        for v in dirs:
+            ctx.set_state(u0 + h * v)
+            jp = ctx.objective(dt, 400, lo, hi)
+            ctx.set_state(u0 - h * v)
+            jm = ctx.objective(dt, 400, lo, hi)
+            fd.append((jp - jm) / (2 * h))
+        fd = np.array(fd)
+        rel = np.abs(ad - fd) / np.abs(fd)
+        worst_th = 0.0
+        for k in range(3):
+            step = h * max(abs(inlet[k]), 1.0)
+            jj = []
+            for sgn in (1, -1):
+                t = inlet.copy()
+                t[k] += sgn * step
+                c, _ = device(u0, t)
+                jj.append(c.objective(dt, 400, lo, hi))
+                c.close()
+            fdt = (jj[0] - jj[1]) / (2 * step)
+            worst_th = max(worst_th, abs(th[k] - fdt) / abs(fdt))
+        cd["h=%g" % h] = {"directions": ndir, "worst_rel_diff_du0": float(rel.max()),
+                          "median_rel_diff_du0": float(np.median(rel)),
+                          "worst_diff_over_rms_du0": float(np.abs(ad - fd).max() / np.sqrt((ad ** 2).mean())),
+                          "worst_rel_diff_dinlet": worst_th}
+    ctx.close()
+    return {"J": J, "dJ_dinlet_u_T_p": th.tolist(), "forward_ms": fwd_ms, "adjoint_ms": adj_ms,
+            "central_differences": cd,
+            "fp32_gradient_vs_fp64": {"l2_rel_du0": float(np.linalg.norm(g32 - g) / np.linalg.norm(g)),
+                                      "max_rel_dinlet": float(np.abs((th32 - th) / th).max())}}
 
 
 def main():
