@@ -280,15 +280,17 @@ void k_adj_stage(AdjArgs A, DevParams P)
 // the adjoint weights of side s at its point) give d s / d t0 and d s / d t1;
 // out[side][m][v] = sum over the points of (d s / d t_side)_v g_m,
 // g = (1, t1 coordinate, t2 coordinate) of the small face's point.
-template <typename R, int NC, int FLUX>
+// The face's normal axis AX is a template argument, so that every
+// coefficient and velocity component is picked at compile time.
+template <typename R, int NC, int FLUX, int AX>
 __device__ __forceinline__ void adj_face(const AdjArgs &A, const DevParams &P, const R *__restrict__ U,
                                          const R *__restrict__ X, int4 rec, R *out)
 {
     constexpr int NV = 4 + NC;
+    constexpr int a = AX, t1 = (AX == 0) ? 1 : 0, t2 = (AX == 2) ? 1 : 2;
     const int64_t n = A.n;
     const int info = rec.z;
     const int dir0 = info & 7, kind = (info >> 3) & 3, sub = (info >> 5) & 3;
-    const int a = dir0 >> 1, t1 = (a == 0) ? 1 : 0, t2 = (a == 2) ? 1 : 2;
     const R s = (dir0 & 1) ? R(1) : R(-1);
     const int lown = A.level[rec.x] + (kind == FACE_HANG1 ? 1 : 0);
     const R h = ldexp(R(1), -lown);
@@ -382,7 +384,14 @@ void k_adj_chunk(AdjArgs A, DevParams P)
     const int eb = A.ch_ebeg[c], ee = A.ch_ebeg[c + 1];
     for (int f = fb + (int) threadIdx.x; f < fe; f += BS) {
         const int4 rec = A.cf4[f];
-        if (((rec.z >> 3) & 3) != FACE_BND) adj_face<R, NC, FLUX>(A, P, U, X, rec, mom + (size_t) (f - fb) * 6 * NV);
+        if (((rec.z >> 3) & 3) != FACE_BND) {
+            // faces of a chunk are sorted by axis, so the branch is mostly wave-uniform
+            R *dst = mom + (size_t) (f - fb) * 6 * NV;
+            const int ax = (rec.z & 7) >> 1;
+            if (ax == 0) adj_face<R, NC, FLUX, 0>(A, P, U, X, rec, dst);
+            else if (ax == 1) adj_face<R, NC, FLUX, 1>(A, P, U, X, rec, dst);
+            else adj_face<R, NC, FLUX, 2>(A, P, U, X, rec, dst);
+        }
         finfo[f - fb] = rec.z;
     }
     __syncthreads();
@@ -411,18 +420,15 @@ void k_adj_chunk(AdjArgs A, DevParams P)
         const R d1 = big ? ((sub & 1) ? R(0.25) : R(-0.25)) : R(0);
         const R d2 = big ? ((sub & 2) ? R(0.25) : R(-0.25)) : R(0);
         const R *m = mom + (size_t) fl * 6 * NV + (size_t) side * 3 * NV;
-        const int t1 = (ax == 0) ? 1 : 0, t2 = (ax == 2) ? 1 : 2;
         #pragma unroll
         for (int v = 0; v < NV; v++) {
             const R S0 = m[v], S1 = m[NV + v], S2 = m[2 * NV + v];
-            R c3[3];
-            c3[ax] = pa * S0;
-            c3[t1] = mu * S1 + d1 * S0;
-            c3[t2] = mu * S2 + d2 * S0;
+            // (normal, t1, t2) parts spread onto (x, y, z)
+            const R cn = pa * S0, c1 = mu * S1 + d1 * S0, c2 = mu * S2 + d2 * S0;
             ub[4*v+0] += S0;
-            ub[4*v+1] += c3[0];
-            ub[4*v+2] += c3[1];
-            ub[4*v+3] += c3[2];
+            ub[4*v+1] += (ax == 0) ? cn : c1;
+            ub[4*v+2] += (ax == 1) ? cn : ((ax == 0) ? c1 : c2);
+            ub[4*v+3] += (ax == 2) ? cn : c2;
         }
     }
     // boundary faces: the per-point parts of k_adj_bnd_pts, in list order
