@@ -309,7 +309,7 @@ def bench_hot_spot(CA, W):
             "dof_updates_per_s": W.dof_updates(len(leaves), 8, 3, 100) / (ms * 1e-3)}
 
 
-def bench_adjoint(CA, W):
+def bench_adjoint(CA, W, fd_check=True):
     # adjoint of the pulse case: level 6, 400 steps, checkpoints every 20
     cfg, leaves, u0, per = W.c5_pulse(6, CA.FP32)
     ctx = CA.GpuContext(cfg)
@@ -323,7 +323,8 @@ def bench_adjoint(CA, W):
     out = {"leaves": len(leaves), "steps": 400, "checkpoint_every": 20, "J": J,
            "dJ_dinlet_u_T_p": th.tolist(), "forward_ms": fwd_ms, "adjoint_ms": adj_ms,
            "adjoint_over_forward": adj_ms / fwd_ms}
-    out["fp64_reference"] = adjoint_fd_check(CA, W, dt, g, th)
+    if fd_check:
+        out["fp64_reference"] = adjoint_fd_check(CA, W, dt, g, th)
     return out
 
 
@@ -401,6 +402,8 @@ def main():
     ap.add_argument("--no-traffic", action="store_true")
     ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--cpu-probe", type=float, default=0.0, help=argparse.SUPPRESS)
+    ap.add_argument("--no-fd-check", action="store_true",
+                    help="with --only adjoint: the FP32 adjoint and its timing, without the FP64 reference gradient")
     ap.add_argument("--only", choices=("bubbles", "hot_spot", "adjoint"),
                     help="run one of the other configurations alone and print its JSON")
     args = ap.parse_args()
@@ -420,7 +423,8 @@ def main():
         traffic_probe(CA, W, args.level)
         return
     if args.only:
-        fn = {"bubbles": bench_bubbles, "hot_spot": bench_hot_spot, "adjoint": bench_adjoint}[args.only]
+        fn = {"bubbles": bench_bubbles, "hot_spot": bench_hot_spot,
+              "adjoint": lambda CA, W: bench_adjoint(CA, W, fd_check=not args.no_fd_check)}[args.only]
         print(json.dumps(fn(CA, W)))
         return
 
