@@ -52,19 +52,23 @@ template <typename D, int NC, int FLUX>
 __device__ __forceinline__ void flux_global(const Prim<D, NC> &pl, const Prim<D, NC> &pr, int a, D sg, D *qg)
 {
     constexpr int NV = 4 + NC;
-    const int t1 = (a == 0) ? 1 : 0, t2 = (a == 2) ? 1 : 2;
+    // the axis may differ between lanes: components are picked with selects,
+    // not by indexing arrays of duals (which go to private memory)
     Frame<D, NC> fl, fr;
-    const D vl[3] = {pl.u, pl.v, pl.w}, vr[3] = {pr.u, pr.v, pr.w};
     fl.r = pl.r; fl.e_tot = pl.e_tot; fl.p = pl.p; fl.cz = pl.cz; fl.gam = pl.gam;
     fr.r = pr.r; fr.e_tot = pr.e_tot; fr.p = pr.p; fr.cz = pr.cz; fr.gam = pr.gam;
-    fl.un = sg * vl[a]; fr.un = sg * vr[a];
-    fl.ut1 = vl[t1]; fr.ut1 = vr[t1];
-    fl.ut2 = vl[t2]; fr.ut2 = vr[t2];
+    fl.un = sg * sel(a == 0, pl.u, sel(a == 1, pl.v, pl.w));
+    fr.un = sg * sel(a == 0, pr.u, sel(a == 1, pr.v, pr.w));
+    fl.ut1 = sel(a == 0, pl.v, pl.u); fr.ut1 = sel(a == 0, pr.v, pr.u);     // t1 = 1 for a = 0, else 0
+    fl.ut2 = sel(a == 2, pl.v, pl.w); fr.ut2 = sel(a == 2, pr.v, pr.w);     // t2 = 1 for a = 2, else 2
     #pragma unroll
     for (int i = 0; i < NC; i++) { fl.c[i] = pl.c[i]; fr.c[i] = pr.c[i]; }
     D q[NV];
     flux_x<D, NC, FLUX>(fl, fr, q);
-    qg[a] = sg * q[0]; qg[t1] = q[1]; qg[t2] = q[2];
+    const D qn = sg * q[0];
+    qg[0] = sel(a == 0, qn, q[1]);
+    qg[1] = sel(a == 1, qn, sel(a == 0, q[1], q[2]));
+    qg[2] = sel(a == 2, qn, q[2]);
     #pragma unroll
     for (int v = 3; v < NV; v++) qg[v] = q[v];
 }

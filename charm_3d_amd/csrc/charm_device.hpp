@@ -50,6 +50,12 @@ struct DevParams {
     double r_a[4], r_e[4], r_n[4];
 };
 
+// c ? a : b on values (the dual-number overload selects component-wise, so
+// that a select of two structs never becomes a load through a selected
+// address in private memory)
+template <typename T>
+__device__ __forceinline__ T sel(bool c, const T &a, const T &b) { return c ? a : b; }
+
 template <typename R> __device__ __forceinline__ R cst(double d, float) { return R(d); }
 template <> __device__ __forceinline__ double cst<double>(double d, float) { return d; }
 template <> __device__ __forceinline__ float cst<float>(double, float f) { return f; }
@@ -247,28 +253,28 @@ __device__ __forceinline__ void bnd_state(const DevParams &P, int face, int axis
         for (int i = 0; i < 3; i++) pv[i] = cst<R>(P.bnd_param[face][i], P.bnd_param_f[face][i]);
         if (frame_axis >= 0) {
             const int a = frame_axis;
-            vel[0] = (a == 0) ? pv[0] : ((a == 1) ? pv[1] : pv[2]);
-            vel[1] = (a == 0) ? pv[1] : pv[0];
-            vel[2] = (a == 2) ? pv[1] : pv[2];
+            vel[0] = sel(a == 0, pv[0], sel(a == 1, pv[1], pv[2]));
+            vel[1] = sel(a == 0, pv[1], pv[0]);
+            vel[2] = sel(a == 2, pv[1], pv[2]);
         } else {
             vel[0] = pv[0]; vel[1] = pv[1]; vel[2] = pv[2];
         }
         t = cst<R>(P.bnd_param[face][3], P.bnd_param_f[face][3]);
         pr = cst<R>(P.bnd_param[face][4], P.bnd_param_f[face][4]);
     } else if (type == BND_WALL_SLIP) {
-        R va = (axis == 0) ? vel[0] : ((axis == 1) ? vel[1] : vel[2]);
+        R va = sel(axis == 0, vel[0], sel(axis == 1, vel[1], vel[2]));
         R vn = va * s;
         va = (va - s * vn) - s * vn;
-        vel[0] = (axis == 0) ? va : vel[0];
-        vel[1] = (axis == 1) ? va : vel[1];
-        vel[2] = (axis == 2) ? va : vel[2];
+        vel[0] = sel(axis == 0, va, vel[0]);
+        vel[1] = sel(axis == 1, va, vel[1]);
+        vel[2] = sel(axis == 2, va, vel[2]);
     } else if (type == BND_WALL_NO_SLIP) {
-        R va = (axis == 0) ? vel[0] : ((axis == 1) ? vel[1] : vel[2]);
+        R va = sel(axis == 0, vel[0], sel(axis == 1, vel[1], vel[2]));
         R vn = va * s;
         va = va - s * vn;
-        vel[0] = (axis == 0) ? va : vel[0];
-        vel[1] = (axis == 1) ? va : vel[1];
-        vel[2] = (axis == 2) ? va : vel[2];
+        vel[0] = sel(axis == 0, va, vel[0]);
+        vel[1] = sel(axis == 1, va, vel[1]);
+        vel[2] = sel(axis == 2, va, vel[2]);
         t = cst<R>(P.bnd_param[face][0], P.bnd_param_f[face][0]);
     }
     out.u = vel[0]; out.v = vel[1]; out.w = vel[2];

@@ -64,6 +64,18 @@ struct Dual {
     }
 };
 
+// component-wise select (see sel in charm_device.hpp)
+template <typename R, int N>
+__device__ __forceinline__ Dual<R, N> sel(bool c, const Dual<R, N> &a, const Dual<R, N> &b)
+{
+    Dual<R, N> o;
+    o.v = c ? a.v : b.v;
+    #pragma unroll
+    for (int i = 0; i < N; i++) o.d[i] = c ? a.d[i] : b.d[i];
+    return o;
+}
+
+
 #define CHARM_DUAL template <typename R, int N> __device__ __forceinline__
 
 CHARM_DUAL Dual<R, N> operator-(const Dual<R, N> &a)

@@ -31,10 +31,13 @@ __device__ __forceinline__ R at(const R *c, R px, R py, R pz)
 template <typename R>
 __device__ __forceinline__ void face_point(int d, int k, R o1, R o2, R scale, R ph[3])
 {
-    const int a = d >> 1, t1 = (a == 0) ? 1 : 0, t2 = (a == 2) ? 1 : 2;
-    ph[a] = (d & 1) ? R(0.5) : R(-0.5);
-    ph[t1] = o1 + scale * gp_t1<R>(k);
-    ph[t2] = o2 + scale * gp_t2<R>(k);
+    // d may differ between lanes: selects, not a store through ph[a]
+    const int a = d >> 1;
+    const R pn = (d & 1) ? R(0.5) : R(-0.5);
+    const R p1 = o1 + scale * gp_t1<R>(k), p2 = o2 + scale * gp_t2<R>(k);
+    ph[0] = (a == 0) ? pn : p1;                           // t1 = 0 unless a = 0
+    ph[1] = (a == 1) ? pn : ((a == 0) ? p1 : p2);
+    ph[2] = (a == 2) ? pn : p2;                           // t2 = 2 unless a = 2
 }
 
 // Enumerates the face points of leaf e: fn(d, nb, self_ph, nb_ph, wj, sign)
