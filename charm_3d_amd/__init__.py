@@ -68,6 +68,11 @@ class Task(C.Structure):
                 ("mesh_name", C.c_char * 256), ("mesh_periodic", C.c_int * 3)]
 
 
+class ConnPatch(C.Structure):
+    """charm_conn_patch_t: one $PhysicalNames entry."""
+    _fields_ = [("dim", C.c_int), ("id", C.c_int), ("name", C.c_char * 64)]
+
+
 _lib = None
 
 
@@ -123,6 +128,12 @@ def lib():
         "charm_write_vtk": (C.c_int, [C.c_char_p, i64, i32p, dp, C.POINTER(Config)]),
         "charm_write_solution": (None, [vp]),
         "charm_task_load_yaml": (C.c_int, [C.c_char_p, C.POINTER(Task)]),
+        "charm_conn_read_msh": (C.c_int, [C.c_char_p, C.POINTER(vp)]),
+        "charm_conn_destroy": (None, [vp]),
+        "charm_conn_info": (C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
+        "charm_conn_get": (C.c_int, [vp, dp, i32p, i32p, C.POINTER(C.c_int8), i32p, i32p]),
+        "charm_conn_patch": (C.c_int, [vp, C.c_int, C.POINTER(ConnPatch)]),
+        "charm_conn_bind_task": (C.c_int, [vp, C.POINTER(Task), i32p, i32p]),
         "charm_task_forest": (C.c_int, [C.POINTER(Task), C.POINTER(vp)]),
         "charm_task_initial_state": (C.c_int, [C.POINTER(Task), vp, dp, i64]),
         "charm_gpu_amr_indicator": (C.c_int, [vp, dp, dp, i64]),
@@ -250,6 +261,52 @@ def task_initial_state(task, forest):
     u = np.zeros((n, 4 + task.cfg.n_comp, 4))
     check(lib().charm_task_initial_state(C.byref(task), forest.h, _dptr(u), n))
     return u
+
+
+class Connectivity:
+    """charm_conn_read_msh: a Gmsh 2.x hexahedral mesh as multi-tree
+    connectivity (vertices, z-order tree corners, tree-to-tree face links with
+    p4est's face + 6 * orientation encoding, boundary patch and region ids)."""
+
+    def __init__(self, path):
+        self.h = C.c_void_p()
+        check(lib().charm_conn_read_msh(str(path).encode(), C.byref(self.h)))
+        nv, nt, npt = C.c_int64(), C.c_int64(), C.c_int()
+        check(lib().charm_conn_info(self.h, C.byref(nv), C.byref(nt), C.byref(npt)))
+        self.num_vertices, self.num_trees = nv.value, nt.value
+        self.vertices = np.zeros((nv.value, 3))
+        self.tree_to_vertex = np.zeros((nt.value, 8), dtype=np.int32)
+        self.tree_to_tree = np.zeros((nt.value, 6), dtype=np.int32)
+        self.tree_to_face = np.zeros((nt.value, 6), dtype=np.int8)
+        self.tree_to_patch = np.zeros((nt.value, 6), dtype=np.int32)
+        self.tree_region = np.zeros(nt.value, dtype=np.int32)
+        check(lib().charm_conn_get(self.h, _dptr(self.vertices), _i32ptr(self.tree_to_vertex),
+                                   _i32ptr(self.tree_to_tree),
+                                   self.tree_to_face.ctypes.data_as(C.POINTER(C.c_int8)),
+                                   _i32ptr(self.tree_to_patch), _i32ptr(self.tree_region)))
+        self.patches = []
+        for i in range(npt.value):
+            p = ConnPatch()
+            check(lib().charm_conn_patch(self.h, i, C.byref(p)))
+            self.patches.append((p.dim, p.id, p.name.decode()))
+
+    def bind_task(self, task):
+        """(tree_bnd [nt][6] boundary slot or -1, tree_reg [nt] region index)"""
+        tb = np.zeros((self.num_trees, 6), dtype=np.int32)
+        tr = np.zeros(self.num_trees, dtype=np.int32)
+        check(lib().charm_conn_bind_task(self.h, C.byref(task), _i32ptr(tb), _i32ptr(tr)))
+        return tb, tr
+
+    def close(self):
+        if self.h:
+            lib().charm_conn_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def write_vtk(name, leaves, u, cfg):

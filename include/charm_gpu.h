@@ -300,6 +300,36 @@ int charm_task_forest(const charm_task_t *task, charm_forest_t **out);
 /* charm_init_initial_condition (src/charm_init.c:12-73): the region of cell
  * type 0 through EOS flag 3 into coefficient 0, slopes zero; u FP64 [n][V][4] */
 int charm_task_initial_state(const charm_task_t *task, const charm_forest_t *f, double *u, int64_t n);
+/* ------------------------------------------------- Gmsh connectivity
+ * charm_conn_reader_msh (src/charm_connectivity.c:315-519) without p4est: a
+ * Gmsh 2.x ASCII mesh of hexahedra (type 5) and boundary quadrangles (type
+ * 3) as multi-tree connectivity.  Tree corners are in z-order; tree_to_face
+ * holds the neighbour's face + 6 * orientation as p4est's connectivity
+ * (orientation: the face corner of the higher face number that meets corner
+ * 0 of the lower one); a boundary face links to itself.  The kernels of this
+ * library run on the axis-aligned unit-cube octree: a connectivity is read,
+ * checked and bound to a task's boundaries and regions, not time-stepped. */
+typedef struct charm_conn charm_conn_t;
+typedef struct {
+    int  dim, id;                      /* $PhysicalNames: dimension, physical id */
+    char name[64];
+} charm_conn_patch_t;
+int  charm_conn_read_msh(const char *path, charm_conn_t **out);
+void charm_conn_destroy(charm_conn_t *c);
+int  charm_conn_info(const charm_conn_t *c, int64_t *num_vertices, int64_t *num_trees, int *num_patches);
+/* any output may be NULL: vertices [nv][3], tree_to_vertex [nt][8],
+ * tree_to_tree [nt][6], tree_to_face [nt][6], tree_to_patch [nt][6] (physical
+ * id of the boundary quadrangle on the face, -1 none), tree_region [nt]
+ * (physical id of the hexahedron) */
+int  charm_conn_get(const charm_conn_t *c, double *vertices, int32_t *tree_to_vertex, int32_t *tree_to_tree,
+                    int8_t *tree_to_face, int32_t *tree_to_patch, int32_t *tree_region);
+int  charm_conn_patch(const charm_conn_t *c, int i, charm_conn_patch_t *p);
+/* patches and regions by name against a task (charm_conn_bnd_find_by_id /
+ * charm_conn_reg_find_by_id, src/charm_connectivity.c:178-230):
+ * tree_bnd [nt][6] = the task's boundary slot (0..5) or -1 for faces without
+ * a patch, tree_reg [nt] = index into task->region; a name the task lacks is
+ * CHARM_ERR_MESH with the reference's message */
+int  charm_conn_bind_task(const charm_conn_t *c, const charm_task_t *task, int32_t *tree_bnd, int32_t *tree_reg);
 /* ------------------------------------------------ reference entry points
  * (include/charm_models.h:15-23, charm_limiter.h:13-15, charm_fluxes.h:19-32,
  * charm_amr.h:9-10)
