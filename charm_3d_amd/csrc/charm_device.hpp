@@ -176,18 +176,25 @@ __device__ __forceinline__ void eos_coeffs(const DevParams &P, Prim<R, NC> &p)
         eos_coeffs_at<R, NC>(P, p, t);
         return;
     }
-    if (P.eos == EOS_IDEAL || NC == 1) {
+    if constexpr (NC == 1) {
         p.gam = cst<R>(P.gam, P.gam_f); p.cv = cst<R>(P.cv, P.cv_f); p.m = cst<R>(P.m, P.m_f);
     } else {
+        // the mixture values, then a select on the (uniform) EOS type: with
+        // two branches writing the record the compiler kept both candidates
+        // in private memory
         R cp = R(0), M_ = R(0);
         #pragma unroll
         for (int i = 0; i < NC; i++) {
             M_ += dv(p.c[i], cst<R>(P.comp_m[i], P.comp_m_f[i]));
             cp += p.c[i] * cst<R>(P.comp_cp[i], P.comp_cp_f[i]);
         }
-        p.m = rcp(M_);
-        p.cv = cp - dv(R(kGasR), p.m);
-        p.gam = dv(cp, p.cv);
+        const R m = rcp(M_);
+        const R cv = cp - dv(R(kGasR), m);
+        const R gam = dv(cp, cv);
+        const bool ideal = P.eos == EOS_IDEAL;
+        p.gam = sel(ideal, cst<R>(P.gam, P.gam_f), gam);
+        p.cv = sel(ideal, cst<R>(P.cv, P.cv_f), cv);
+        p.m = sel(ideal, cst<R>(P.m, P.m_f), m);
     }
 }
 
@@ -314,6 +321,19 @@ __device__ __forceinline__ Frame<R, NC> to_frame(const Prim<R, NC> &p, R n0, R n
     return f;
 }
 
+// field-wise select of two face-frame states (a select of the two records'
+// addresses would send them to private memory)
+template <typename R, int NC>
+__device__ __forceinline__ Frame<R, NC> sel_frame(bool c, const Frame<R, NC> &a, const Frame<R, NC> &b)
+{
+    Frame<R, NC> o;
+    o.r = sel(c, a.r, b.r); o.un = sel(c, a.un, b.un); o.ut1 = sel(c, a.ut1, b.ut1); o.ut2 = sel(c, a.ut2, b.ut2);
+    o.e_tot = sel(c, a.e_tot, b.e_tot); o.p = sel(c, a.p, b.p); o.cz = sel(c, a.cz, b.cz); o.gam = sel(c, a.gam, b.gam);
+    #pragma unroll
+    for (int i = 0; i < NC; i++) o.c[i] = sel(c, a.c[i], b.c[i]);
+    return o;
+}
+
 // x-split HLLC, src/charm_fluxes.c:319-437; q = (qn, qt1, qt2, qe, qc[]).
 // Returns false where the reference aborts (:354-357).
 template <typename R, int NC>
@@ -336,7 +356,7 @@ __device__ __forceinline__ bool hllc_x(const Frame<R, NC> &L, const Frame<R, NC>
     if (s_star > sr) s_star = sr;
     const bool ok = (sl <= s_star) && (s_star <= sr);
     if (sl >= R(0) || sr <= R(0)) {
-        const Frame<R, NC> &K = (sl >= R(0)) ? L : Rt;
+        const Frame<R, NC> K = sel_frame<R, NC>(sl >= R(0), L, Rt);
         q[0] = K.r * K.un * K.un + K.p;
         q[1] = K.r * K.ut1 * K.un;
         q[2] = K.r * K.ut2 * K.un;
@@ -345,7 +365,7 @@ __device__ __forceinline__ bool hllc_x(const Frame<R, NC> &L, const Frame<R, NC>
         for (int i = 0; i < NC; i++) q[4 + i] = K.r * K.un * K.c[i];
     } else {
         const bool left = s_star >= R(0);
-        const Frame<R, NC> &K = left ? L : Rt;
+        const Frame<R, NC> K = sel_frame<R, NC>(left, L, Rt);
         const R sk = left ? sl : sr;
         // F_HLLC_U / _V / _E macros, src/charm_fluxes.c:319-321
         const R inv = rcp(sk - s_star);
