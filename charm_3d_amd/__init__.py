@@ -145,6 +145,8 @@ def lib():
         "charm_gpu_remesh": (C.c_int, [vp, vp, C.POINTER(C.c_int64), i64]),
         "charm_amr_flags": (C.c_int, [i64, i32p, dp, dp, C.c_double, C.c_int, C.c_int,
                                       C.POINTER(C.c_int8)]),
+        "charm_amr_adjoint_weights": (C.c_int, [i64, C.c_int, i32p, dp, dp]),
+        "charm_adapt_weighted": (C.c_int, [vp, dp, i64]),
         "charm_adapt": (None, [vp, vp, vp]),
         "charm_ctx_error": (C.c_int, [vp]),
         "charm_ctx_create": (C.c_int, [vp, C.POINTER(Config), C.POINTER(vp)]),
@@ -573,6 +575,15 @@ def amr_flags(leaves, err2, rho_c, max_err, min_level, max_level):
     return out
 
 
+def amr_adjoint_weights(leaves, grad):
+    """charm_amr_adjoint_weights: per-leaf refinement weights from dJ/du0."""
+    lv = np.ascontiguousarray(leaves, dtype=np.int32)
+    g = np.ascontiguousarray(grad, dtype=np.float64)
+    w = np.zeros(len(lv))
+    check(lib().charm_amr_adjoint_weights(len(lv), g.shape[1], _i32ptr(lv), _dptr(g), _dptr(w)))
+    return w
+
+
 class Solver:
     """charm_ctx_t over a forest: the reference-named entry points
     (charm_timesteps, charm_model_*_timestep_single, charm_adapt)."""
@@ -642,6 +653,21 @@ class Solver:
         err = lib().charm_ctx_error(self.h)
         if err:
             check(err)
+
+    def adapt_weighted(self, weight):
+        """charm_adapt_weighted: the estimator scaled by weight [n] per leaf."""
+        w = np.ascontiguousarray(weight, dtype=np.float64)
+        check(lib().charm_adapt_weighted(self.forest.h, _dptr(w), len(w)))
+
+    def adjoint(self, dt, nsteps, lo, hi, ckpt=20, theta_face=-1):
+        """charm_gpu_adjoint on this context's state: (J, dJ/du0, dJ/dtheta)"""
+        lo_, hi_ = np.asarray(lo, dtype=np.float64), np.asarray(hi, dtype=np.float64)
+        n, nv = self.forest.n, 4 + self.cfg.n_comp
+        j, fm, am = C.c_double(), C.c_double(), C.c_double()
+        g, th = np.zeros((n, nv, 4)), np.zeros(3)
+        check(lib().charm_gpu_adjoint(self.gpu, dt, nsteps, ckpt, _dptr(lo_), _dptr(hi_), theta_face,
+                                      C.byref(j), _dptr(g), _dptr(th), C.byref(fm), C.byref(am)))
+        return j.value, g, th
 
     def timesteps(self):
         lib().charm_timesteps(self.forest.h)

@@ -68,8 +68,32 @@ int charm_amr_flags(int64_t n, const int32_t *leaves, const double *err2, const 
     return CHARM_OK;
 }
 
-/* src/charm_amr.c:575-610 */
-static int adapt(charm_ctx_t *c)
+int charm_amr_adjoint_weights(int64_t n, int nv, const int32_t *leaves, const double *grad, double *w)
+{
+    double mean = 0.0;
+    int64_t i;
+    if (n < 0 || nv < 1 || (n > 0 && (!leaves || !grad || !w))) {
+        charm_set_error("amr_adjoint_weights: bad arguments");
+        return CHARM_ERR_ARG;
+    }
+    for (i = 0; i < n; i++) {
+        const double vol = ldexp(1.0, -3 * leaves[4*i + 3]);
+        double s = 0.0;
+        for (int v = 0; v < nv; v++) s += fabs(grad[((size_t) i * nv + v) * 4]);
+        w[i] = s / vol;
+        mean += w[i];
+    }
+    if (n > 0) mean /= (double) n;
+    if (!(mean > 0.0)) {
+        charm_set_error("amr_adjoint_weights: the gradient of the cell means is zero");
+        return CHARM_ERR_ARG;
+    }
+    for (i = 0; i < n; i++) w[i] /= mean;
+    return CHARM_OK;
+}
+
+/* src/charm_amr.c:575-610; weight (or NULL) scales the estimator per leaf */
+static int adapt(charm_ctx_t *c, const double *weight)
 {
     const int64_t n = charm_forest_num_leaves(c->forest);
     double *err2 = (double *) malloc((size_t) (n ? n : 1) * 2 * sizeof(double));
@@ -83,7 +107,9 @@ static int adapt(charm_ctx_t *c)
         charm_set_error("adapt: out of memory");
         return CHARM_ERR_MEM;
     }
-    if (!(rc = charm_gpu_amr_indicator(c->gpu, err2, err2 + n, n)) &&
+    if (!(rc = charm_gpu_amr_indicator(c->gpu, err2, err2 + n, n)) && weight)
+        for (int64_t i = 0; i < n; i++) err2[i] *= weight[i];
+    if (!rc &&
         !(rc = charm_forest_get_leaves(c->forest, leaves)) &&
         !(rc = charm_amr_flags(n, leaves, err2, err2 + n, c->cfg.max_err, c->cfg.min_level,
                                c->cfg.max_level, flags)) &&
@@ -101,6 +127,17 @@ void charm_adapt(charm_forest_t *p4est, charm_ghost_t *ghost, charm_data_t *ghos
     int rc;
     (void) ghost; (void) ghost_data;
     if (!c) return;
-    rc = adapt(c);
+    rc = adapt(c, NULL);
     if (rc) c->err = rc;
+}
+
+int charm_adapt_weighted(charm_forest_t *p4est, const double *weight, int64_t n)
+{
+    charm_ctx_t *c = charm_ctx_of(p4est);
+    if (!c || !weight || n != charm_forest_num_leaves(p4est)) {
+        charm_set_error("adapt_weighted: no context, no weights or %lld weights for %lld leaves",
+                        (long long) n, (long long) (p4est ? charm_forest_num_leaves(p4est) : 0));
+        return CHARM_ERR_ARG;
+    }
+    return adapt(c, weight);
 }

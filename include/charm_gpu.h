@@ -256,6 +256,17 @@ int charm_gpu_adjoint(charm_gpu_ctx_t *ctx, double dt, int nsteps, int ckpt, con
 int charm_amr_flags(int64_t n, const int32_t *leaves, const double *err2, const double *rho_c,
                     double max_err, int min_level, int max_level, int8_t *flags);
 
+/* Adjoint-weighted refinement (no counterpart in the reference, which has
+ * no adjoint; for drivers that refine toward an output functional).
+ * charm_amr_adjoint_weights: from dJ/du0 of charm_gpu_adjoint (grad
+ * [n][nv][4], FP64), w_i = lambda_i / mean(lambda) with lambda_i =
+ * sum_v |dJ/dU_{i,v,0}| / V_i, the sensitivity of J to leaf i's cell means
+ * per unit volume.  charm_adapt_weighted: charm_adapt with the estimator
+ * err2_i multiplied by weight_i in both the refine and the coarsen
+ * criteria (weight [n], n = the forest's leaves). */
+int charm_amr_adjoint_weights(int64_t n, int nv, const int32_t *leaves, const double *grad, double *w);
+int charm_adapt_weighted(charm_forest_t *p4est, const double *weight, int64_t n);
+
 /* pointwise flux on the device for a batch of point pairs: prim rows are
  * (r, u, v, w, p, c[0..n_comp-1]); normals [n][3]; out [n][4 + n_comp] */
 int charm_gpu_flux_batch(charm_gpu_ctx_t *ctx, int flux, int64_t n, const double *prim_l,
