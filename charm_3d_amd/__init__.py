@@ -139,6 +139,7 @@ def lib():
         "charm_gpu_amr_indicator": (C.c_int, [vp, dp, dp, i64]),
         "charm_gpu_ns_aux": (C.c_int, [vp, dp, i64]),
         "charm_gpu_objective": (C.c_int, [vp, C.c_double, C.c_int, dp, dp, dp]),
+        "charm_gpu_stage_vjp": (C.c_int, [vp, C.c_double, dp, dp, dp, i64]),
         "charm_gpu_adjoint": (C.c_int, [vp, C.c_double, C.c_int, C.c_int, dp, dp, C.c_int, dp, dp, dp,
                                         dp, dp]),
         "charm_gpu_ns_chem": (C.c_int, [vp, C.c_double, dp, i64]),
@@ -546,6 +547,14 @@ class GpuContext:
         j = C.c_double()
         check(lib().charm_gpu_objective(self.h, dt, nsteps, _dptr(lo_), _dptr(hi_), C.byref(j)))
         return j.value
+
+    def stage_vjp(self, dt, u, x):
+        """charm_gpu_stage_vjp: S_u^T x for S(u) = u - dt A^-1 int(u)."""
+        u_ = np.ascontiguousarray(u, dtype=np.float64)
+        x_ = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros_like(u_)
+        check(lib().charm_gpu_stage_vjp(self.h, dt, _dptr(u_), _dptr(x_), _dptr(out), self.n))
+        return out
 
     def adjoint(self, dt, nsteps, lo, hi, ckpt=20, theta_face=-1):
         """(J, dJ/du0 [n][V][4], dJ/d(inlet u, T, p), forward ms, adjoint ms)"""

@@ -982,6 +982,64 @@ static int adj_boundary(charm_gpu_ctx_t *c, AdjBoundary &b)
     return CHARM_OK;
 }
 
+// One forward-Euler stage S(u) = u - dt A^-1 int(u) (the Euler model's step
+// without limiter) and its vector-Jacobian product out = S_u^T x = x +
+// (d int / d u)^T (-dt A^-1 x): the stage kernel the adjoint sweeps with,
+// on host FP64 arrays [n][V][4].
+extern "C" int charm_gpu_stage_vjp(charm_gpu_ctx_t *c, double dt, const double *u, const double *x,
+                                   double *out, int64_t n)
+{
+    int rc = need_mesh(c, n);
+    if (rc) return rc;
+    if (!u || !x || !out || !(dt >= 0)) { charm_set_error("stage_vjp: bad argument"); return CHARM_ERR_ARG; }
+    if (c->cfg.limiter || c->ph || c->nc > 2 || c->prec == CHARM_FP32_MIX || c->cfg.flux == CHARM_FLUX_GODUNOV) {
+        charm_set_error("stage_vjp: needs no limiter, constant cp, 1-2 species and the LF, HLLC or CD flux");
+        return CHARM_ERR_UNSUPPORTED;
+    }
+    const size_t ssz = (size_t) n * 4 * c->nv * c->esz, dsz = (size_t) n * 4 * c->nv * sizeof(double);
+    AdjBoundary bnd;
+    if ((rc = adj_boundary(c, bnd))) return rc;
+    int dev = 0, lds_max = 0;
+    const bool chunked = c->sprec && hipGetDevice(&dev) == hipSuccess &&
+                         hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
+                         adj_chunk_smem_bytes(1, c->nc, c->max_faces) <= (size_t) lds_max;
+    (void) hipGetLastError();
+    char *dU = NULL, *dX = NULL, *dO = NULL;
+    hipError_t e = hipMalloc((void **) &dU, ssz);
+    if (e == hipSuccess) e = hipMalloc((void **) &dX, ssz);
+    if (e == hipSuccess) e = hipMalloc((void **) &dO, ssz);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->d_stage64, u, dsz, hipMemcpyHostToDevice, c->st);
+    if (e == hipSuccess) e = launch_convert(c->sprec, 0, c->d_stage64, dU, n, 4 * c->nv, c->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->d_stage64, x, dsz, hipMemcpyHostToDevice, c->st);
+    if (e == hipSuccess) e = launch_convert(c->sprec, 0, c->d_stage64, dX, n, 4 * c->nv, c->st);
+    if (e == hipSuccess) {
+        AdjArgs A;
+        memset(&A, 0, sizeof(A));
+        A.level = c->d_level; A.nbr = c->d_nbr; A.aux = c->d_aux; A.n = n; A.dt = dt;
+        A.theta_bar = NULL; A.theta_face = -1;
+        A.bnd_elems = bnd.elems; A.bnd_ptr = bnd.ptr; A.bnd_faces = bnd.faces;
+        A.n_bnd = bnd.n_elems; A.n_bface = bnd.n_faces; A.bnd_part = bnd.part; A.bnd_theta = bnd.theta;
+        A.cf4 = chunked ? c->d_cf4 : NULL;
+        A.ch_ebeg = c->d_ch_ebeg; A.ch_fbeg = c->d_ch_fbeg; A.slot_ptr = c->d_slot_ptr; A.slots = c->d_slots;
+        A.n_chunks = (int) c->nch; A.max_faces = c->max_faces; A.chunk_elems = c->chunk_elems;
+        A.n_xcd = c->n_xcd;
+        A.bnd_of = chunked ? bnd.of : NULL;
+        A.U = dU; A.X = dX; A.ACC = NULL; A.ACC2 = NULL; A.OUT = dO;
+        A.c_in = 1.0; A.c_out = 1.0; A.c_acc = 0.0; A.c_acc2 = 0.0;
+        e = launch_adj_stage(c->sprec, c->nc, c->cfg.flux, A, c->P, c->st);
+    }
+    if (e == hipSuccess) e = launch_convert(c->sprec, 1, dO, c->d_stage64, n, 4 * c->nv, c->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, c->d_stage64, dsz, hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    (void) hipStreamSynchronize(c->st);
+    void *bufs[] = {dU, dX, dO};
+    for (size_t i = 0; i < 3; i++)
+        if (bufs[i]) (void) hipFree(bufs[i]);
+    adj_boundary_free(bnd);
+    if (e != hipSuccess) { charm_set_error("stage_vjp: %s", hipGetErrorString(e)); return CHARM_ERR_HIP; }
+    return check_flag(c);
+}
+
 extern "C" int charm_gpu_adjoint(charm_gpu_ctx_t *c, double dt, int nsteps, int ckpt, const double lo[3],
                                  const double hi[3], int theta_face, double *J, double *grad_u0,
                                  double *grad_theta, double *fwd_ms, double *adj_ms)

@@ -244,6 +244,15 @@ int charm_gpu_ns_chem(charm_gpu_ctx_t *ctx, double dt, double *chem_rhs, int64_t
  * reverse pass.  Both leave the state at u^N. */
 int charm_gpu_objective(charm_gpu_ctx_t *ctx, double dt, int nsteps, const double lo[3],
                         const double hi[3], double *J);
+/* The stage VJP the adjoint sweeps with, on its own (no counterpart in the
+ * reference): for the forward-Euler stage S(u) = u - dt A^-1 int(u) (the
+ * Euler model's step, charm_gpu_step with CHARM_MODEL_EULER, no limiter),
+ * out = S_u^T x = x + (d int / d u)^T (-dt A^-1 x): the transposed volume,
+ * interior-face, hanging-face and boundary-face linearisations and the mass
+ * matrix solve.  u, x, out FP64 [n][V][4] on the host; the context's state
+ * is unchanged.  Same configurations as the adjoint. */
+int charm_gpu_stage_vjp(charm_gpu_ctx_t *ctx, double dt, const double *u, const double *x, double *out,
+                        int64_t n);
 int charm_gpu_adjoint(charm_gpu_ctx_t *ctx, double dt, int nsteps, int ckpt, const double lo[3],
                       const double hi[3], int theta_face, double *J, double *grad_u0,
                       double *grad_theta, double *fwd_ms, double *adj_ms);
