@@ -140,6 +140,7 @@ def lib():
         "charm_gpu_ns_aux": (C.c_int, [vp, dp, i64]),
         "charm_gpu_objective": (C.c_int, [vp, C.c_double, C.c_int, dp, dp, dp]),
         "charm_gpu_stage_vjp": (C.c_int, [vp, C.c_double, dp, dp, dp, i64]),
+        "charm_gpu_flux_vjp_batch": (C.c_int, [vp, C.c_int, i64, dp, dp, dp, dp, dp, dp]),
         "charm_gpu_adjoint": (C.c_int, [vp, C.c_double, C.c_int, C.c_int, dp, dp, C.c_int, dp, dp, dp,
                                         dp, dp]),
         "charm_gpu_ns_chem": (C.c_int, [vp, C.c_double, dp, i64]),
@@ -521,6 +522,17 @@ class GpuContext:
         check(lib().charm_gpu_flux_batch(self.h, FLUX[kind], n, _dptr(pl), _dptr(pr),
                                          _dptr(nn), _dptr(out)))
         return out
+
+    def flux_vjp_batch(self, kind, prim_l, prim_r, normals, qbar):
+        """charm_gpu_flux_vjp_batch: (d(qbar . q)/d prim_l, d(qbar . q)/d prim_r)"""
+        pl = np.ascontiguousarray(prim_l, dtype=np.float64)
+        pr = np.ascontiguousarray(prim_r, dtype=np.float64)
+        nn = np.ascontiguousarray(normals, dtype=np.float64)
+        qb = np.ascontiguousarray(qbar, dtype=np.float64)
+        bl, br = np.zeros_like(pl), np.zeros_like(pr)
+        check(lib().charm_gpu_flux_vjp_batch(self.h, FLUX[kind], len(pl), _dptr(pl), _dptr(pr), _dptr(nn),
+                                             _dptr(qb), _dptr(bl), _dptr(br)))
+        return bl, br
 
 
     def amr_indicator(self):

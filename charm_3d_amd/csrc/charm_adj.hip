@@ -715,6 +715,72 @@ static hipError_t launch_adj_r(int nc, int flux, const AdjArgs &A, const DevPara
     return hipErrorInvalidValue;
 }
 
+// Pointwise flux VJP: the computation of k_flux_points (prim rows -> EOS ->
+// face frame -> frame flux -> rotation back) on dual numbers carrying the
+// 2 (5 + NC) inputs of the point as tangents; bar = J^T qbar.
+template <int NC, int FLUX>
+__global__ void k_flux_points_vjp(FluxArgs A, DevParams P)
+{
+    constexpr int NV = 4 + NC, NI = 5 + NC;
+    using D = Dual<double, 2 * NI>;
+    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n) return;
+    Prim<D, NC> pp[2];
+    for (int s = 0; s < 2; s++) {
+        const double *row = (s ? A.prim_r : A.prim_l) + i * NI;
+        Prim<D, NC> &p = pp[s];
+        p.r = D::var(row[0], s * NI + 0); p.u = D::var(row[1], s * NI + 1); p.v = D::var(row[2], s * NI + 2);
+        p.w = D::var(row[3], s * NI + 3); p.p = D::var(row[4], s * NI + 4);
+        #pragma unroll
+        for (int c = 0; c < NC; c++) p.c[c] = D::var(row[5 + c], s * NI + 5 + c);
+        eos_coeffs<D, NC>(P, p);
+        p.e = p.p / (p.r * (p.gam - D(1)));
+        p.e_tot = p.e + D(0.5) * (p.u * p.u + p.v * p.v + p.w * p.w);
+        p.cz = dsqrt(p.gam * p.p / p.r);
+    }
+    double nt[3][3];
+    ref_face_frame(A.normals + 3 * i, nt);
+    Frame<D, NC> f[2];
+    for (int s = 0; s < 2; s++)
+        f[s] = to_frame<D, NC>(pp[s], D(nt[0][0]), D(nt[0][1]), D(nt[0][2]), D(nt[1][0]), D(nt[1][1]),
+                               D(nt[1][2]), D(nt[2][0]), D(nt[2][1]), D(nt[2][2]));
+    D q[NV];
+    flux_x<D, NC, FLUX>(f[0], f[1], q);
+    D o[NV];
+    for (int d = 0; d < 3; d++) o[d] = q[0] * D(nt[0][d]) + q[1] * D(nt[1][d]) + q[2] * D(nt[2][d]);
+    for (int v = 3; v < NV; v++) o[v] = q[v];
+    const double *qb = A.qbar + i * NV;
+    for (int j = 0; j < 2 * NI; j++) {
+        double sj = 0.0;
+        for (int v = 0; v < NV; v++) sj += qb[v] * o[v].d[j];
+        if (j < NI) A.bar_l[i * NI + j] = sj;
+        else A.bar_r[i * NI + j - NI] = sj;
+    }
+}
+
+template <int NC>
+static hipError_t launch_flux_vjp_nc(int flux, const FluxArgs &A, const DevParams &P, hipStream_t st)
+{
+    const dim3 grid((unsigned) ((A.n + 63) / 64)), block(64);
+    switch (flux) {
+    case FLUX_LF: hipLaunchKernelGGL((k_flux_points_vjp<NC, FLUX_LF>), grid, block, 0, st, A, P); break;
+    case FLUX_HLLC: hipLaunchKernelGGL((k_flux_points_vjp<NC, FLUX_HLLC>), grid, block, 0, st, A, P); break;
+    case FLUX_CD: hipLaunchKernelGGL((k_flux_points_vjp<NC, FLUX_CD>), grid, block, 0, st, A, P); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_flux_points_vjp(int nc, int flux, const FluxArgs &A, const DevParams &P, hipStream_t st)
+{
+    if (A.n <= 0) return hipSuccess;
+    switch (nc) {
+    case 1: return launch_flux_vjp_nc<1>(flux, A, P, st);
+    case 2: return launch_flux_vjp_nc<2>(flux, A, P, st);
+    }
+    return hipErrorInvalidValue;
+}
+
 size_t adj_chunk_smem_bytes(int prec, int nc, int max_faces)
 {
     return (size_t) max_faces * (6 * (4 + nc) * (prec ? 4 : 8) + sizeof(int));

@@ -334,6 +334,25 @@ __device__ __forceinline__ Frame<R, NC> sel_frame(bool c, const Frame<R, NC> &a,
     return o;
 }
 
+// the reference's face frame for an arbitrary unit normal (src/charm_fluxes.c:
+// 458-488): rows n, t1 (the first non-degenerate of the three in-plane
+// choices), t2 = n x t1
+__device__ __forceinline__ void ref_face_frame(const double *nn, double nt[3][3])
+{
+    nt[0][0] = nn[0]; nt[0][1] = nn[1]; nt[0][2] = nn[2];
+    nt[1][0] = nt[1][1] = nt[1][2] = 0.0;
+    double ri = sqrt(nn[0] * nn[0] + nn[1] * nn[1]);
+    if (ri > kEps) { nt[1][0] = -nn[1] / ri; nt[1][1] = nn[0] / ri; }
+    else {
+        ri = sqrt(nn[1] * nn[1] + nn[2] * nn[2]);
+        if (ri > kEps) { nt[1][1] = -nn[2] / ri; nt[1][2] = nn[1] / ri; }
+        else { ri = sqrt(nn[0] * nn[0] + nn[2] * nn[2]); if (ri > kEps) { nt[1][0] = -nn[2] / ri; nt[1][2] = nn[0] / ri; } }
+    }
+    nt[2][0] = nt[0][1] * nt[1][2] - nt[0][2] * nt[1][1];
+    nt[2][1] = -nt[0][0] * nt[1][2] + nt[0][2] * nt[1][0];
+    nt[2][2] = nt[0][0] * nt[1][1] - nt[0][1] * nt[1][0];
+}
+
 // x-split HLLC, src/charm_fluxes.c:319-437; q = (qn, qt1, qt2, qe, qc[]).
 // Returns false where the reference aborts (:354-357).
 template <typename R, int NC>

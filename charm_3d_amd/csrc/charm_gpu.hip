@@ -689,6 +689,7 @@ extern "C" int charm_gpu_flux_batch(charm_gpu_ctx_t *c, int flux, int64_t n, con
     if (e == hipSuccess) e = hipMemcpyAsync(dn, normals, nn, hipMemcpyHostToDevice, c->st);
     if (e == hipSuccess) {
         FluxArgs A;
+        memset(&A, 0, sizeof(A));
         A.prim_l = dl; A.prim_r = dr; A.normals = dn; A.out = dq; A.n = n;
         e = launch_flux_points(c->sprec, c->nc, flux, A, c->P, c->st);
     }
@@ -696,6 +697,48 @@ extern "C" int charm_gpu_flux_batch(charm_gpu_ctx_t *c, int flux, int64_t n, con
     if (e == hipSuccess) e = hipStreamSynchronize(c->st);
     (void) hipFree(dl); (void) hipFree(dr); (void) hipFree(dn); (void) hipFree(dq);
     if (e != hipSuccess) { charm_set_error("flux_batch: %s", hipGetErrorString(e)); return CHARM_ERR_HIP; }
+    return CHARM_OK;
+}
+
+extern "C" int charm_gpu_flux_vjp_batch(charm_gpu_ctx_t *c, int flux, int64_t n, const double *prim_l,
+                                        const double *prim_r, const double *normals, const double *qbar,
+                                        double *prim_l_bar, double *prim_r_bar)
+{
+    if (!c || n < 0 || (n > 0 && (!prim_l || !prim_r || !normals || !qbar || !prim_l_bar || !prim_r_bar))) {
+        charm_set_error("flux_vjp_batch: bad arguments");
+        return CHARM_ERR_ARG;
+    }
+    if ((flux != CHARM_FLUX_LF && flux != CHARM_FLUX_HLLC && flux != CHARM_FLUX_CD) || c->nc > 2 || c->ph) {
+        charm_set_error("flux_vjp_batch: LF, HLLC or CD flux, constant cp, 1-2 species");
+        return CHARM_ERR_UNSUPPORTED;
+    }
+    if (n == 0) return CHARM_OK;
+    const size_t np = (size_t) n * (5 + c->nc) * sizeof(double);
+    const size_t nn = (size_t) n * 3 * sizeof(double), nq = (size_t) n * c->nv * sizeof(double);
+    double *dl = NULL, *dr = NULL, *dn = NULL, *dq = NULL, *bl = NULL, *br = NULL;
+    hipError_t e = hipMalloc(&dl, np);
+    if (e == hipSuccess) e = hipMalloc(&dr, np);
+    if (e == hipSuccess) e = hipMalloc(&dn, nn);
+    if (e == hipSuccess) e = hipMalloc(&dq, nq);
+    if (e == hipSuccess) e = hipMalloc(&bl, np);
+    if (e == hipSuccess) e = hipMalloc(&br, np);
+    if (e == hipSuccess) e = hipMemcpyAsync(dl, prim_l, np, hipMemcpyHostToDevice, c->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dr, prim_r, np, hipMemcpyHostToDevice, c->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dn, normals, nn, hipMemcpyHostToDevice, c->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dq, qbar, nq, hipMemcpyHostToDevice, c->st);
+    if (e == hipSuccess) {
+        FluxArgs A;
+        memset(&A, 0, sizeof(A));
+        A.prim_l = dl; A.prim_r = dr; A.normals = dn; A.n = n;
+        A.qbar = dq; A.bar_l = bl; A.bar_r = br;
+        e = launch_flux_points_vjp(c->nc, flux, A, c->P, c->st);
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(prim_l_bar, bl, np, hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(prim_r_bar, br, np, hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    (void) hipStreamSynchronize(c->st);
+    (void) hipFree(dl); (void) hipFree(dr); (void) hipFree(dn); (void) hipFree(dq); (void) hipFree(bl); (void) hipFree(br);
+    if (e != hipSuccess) { charm_set_error("flux_vjp_batch: %s", hipGetErrorString(e)); return CHARM_ERR_HIP; }
     return CHARM_OK;
 }
 

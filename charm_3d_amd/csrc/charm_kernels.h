@@ -48,6 +48,9 @@ struct FluxArgs {
     const double *prim_l, *prim_r, *normals;
     double *out;
     int64_t n;
+    // vector-Jacobian product: qbar [n][V] in, d(qbar . q)/d(prim rows) out
+    const double *qbar;
+    double *bar_l, *bar_r;
 };
 
 // density-gradient error estimator of the adaptation (one thread per leaf)
@@ -155,6 +158,8 @@ const void *stage_kernel(int prec, int nc, int flux, int ph, int bs);
 hipError_t launch_limiter(int prec, int nc, int ph, const LimArgs &A, const DevParams &P, hipStream_t st);
 hipError_t launch_reduce(int prec, int nc, int ph, const RedArgs &A, const DevParams &P, hipStream_t st);
 hipError_t launch_flux_points(int prec, int nc, int flux, const FluxArgs &A, const DevParams &P, hipStream_t st);
+// FP64 pointwise flux VJP (LF, HLLC, CD; 1-2 species) on dual numbers
+hipError_t launch_flux_points_vjp(int nc, int flux, const FluxArgs &A, const DevParams &P, hipStream_t st);
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st);
 // what: 0 chemistry, 1 diffusion coefficients, 2 tau / q, 3 viscous integrals
 hipError_t launch_ns(int prec, int what, int nc, const NsArgs &A, const DevParams &P, hipStream_t st);

@@ -211,20 +211,8 @@ __global__ void k_flux_points(FluxArgs A, DevParams P)
         p.e_tot = p.e + R(0.5) * (p.u * p.u + p.v * p.v + p.w * p.w);
         p.cz = dsqrt(p.gam * p.p / p.r);
     }
-    const double *nn = A.normals + 3 * i;
     double nt[3][3];
-    nt[0][0] = nn[0]; nt[0][1] = nn[1]; nt[0][2] = nn[2];
-    nt[1][0] = nt[1][1] = nt[1][2] = 0.0;
-    double ri = sqrt(nn[0] * nn[0] + nn[1] * nn[1]);
-    if (ri > kEps) { nt[1][0] = -nn[1] / ri; nt[1][1] = nn[0] / ri; }
-    else {
-        ri = sqrt(nn[1] * nn[1] + nn[2] * nn[2]);
-        if (ri > kEps) { nt[1][1] = -nn[2] / ri; nt[1][2] = nn[1] / ri; }
-        else { ri = sqrt(nn[0] * nn[0] + nn[2] * nn[2]); if (ri > kEps) { nt[1][0] = -nn[2] / ri; nt[1][2] = nn[0] / ri; } }
-    }
-    nt[2][0] = nt[0][1] * nt[1][2] - nt[0][2] * nt[1][1];
-    nt[2][1] = -nt[0][0] * nt[1][2] + nt[0][2] * nt[1][0];
-    nt[2][2] = nt[0][0] * nt[1][1] - nt[0][1] * nt[1][0];
+    ref_face_frame(A.normals + 3 * i, nt);
     Frame<R, NC> f[2];
     for (int s = 0; s < 2; s++)
         f[s] = to_frame<R, NC>(pp[s], R(nt[0][0]), R(nt[0][1]), R(nt[0][2]), R(nt[1][0]), R(nt[1][1]),

@@ -280,6 +280,13 @@ int charm_adapt_weighted(charm_forest_t *p4est, const double *weight, int64_t n)
  * (r, u, v, w, p, c[0..n_comp-1]); normals [n][3]; out [n][4 + n_comp] */
 int charm_gpu_flux_batch(charm_gpu_ctx_t *ctx, int flux, int64_t n, const double *prim_l,
                          const double *prim_r, const double *normals, double *out);
+/* its vector-Jacobian product (no counterpart in the reference): for qbar
+ * [n][4 + n_comp], prim_l_bar / prim_r_bar [n][5 + n_comp] = d(qbar . q) /
+ * d(prim rows), FP64 on dual numbers; LF, HLLC and CD, constant cp, 1-2
+ * species */
+int charm_gpu_flux_vjp_batch(charm_gpu_ctx_t *ctx, int flux, int64_t n, const double *prim_l,
+                             const double *prim_r, const double *normals, const double *qbar,
+                             double *prim_l_bar, double *prim_r_bar);
 
 /* VTK output (src/charm_output.c:10-126): <name>_0000.vtu (one
  * UnstructuredGrid piece, raw appended data) and <name>.pvtu; points are the
