@@ -319,10 +319,17 @@ def bench_adjoint(CA, W, fd_check=True):
     dt = ctx.get_dt(1)
     lo, hi = W.C5_BOX
     J, g, th, fwd_ms, adj_ms = ctx.adjoint(dt, 400, lo, hi, ckpt=20, theta_face=0)
+    # the memory-rich variant: every stage state of the forward pass kept
+    # (1,201 states, 25 GB at this size), no recompute; the same gradient
+    ctx.set_state(u0)
+    J0, g0, th0, fwd0, adj0 = ctx.adjoint(dt, 400, lo, hi, ckpt=0, theta_face=0)
     ctx.close()
     out = {"leaves": len(leaves), "steps": 400, "checkpoint_every": 20, "J": J,
            "dJ_dinlet_u_T_p": th.tolist(), "forward_ms": fwd_ms, "adjoint_ms": adj_ms,
-           "adjoint_over_forward": adj_ms / fwd_ms}
+           "adjoint_over_forward": adj_ms / fwd_ms,
+           "all_stage_states_kept": {"forward_ms": fwd0, "adjoint_ms": adj0, "adjoint_over_forward": adj0 / fwd0,
+                                     "same_gradient_bits": bool(J0 == J and np.array_equal(g0, g)
+                                                                and np.array_equal(th0, th))}}
     if fd_check:
         out["fp64_reference"] = adjoint_fd_check(CA, W, dt, g, th)
     return out

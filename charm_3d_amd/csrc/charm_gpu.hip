@@ -895,7 +895,7 @@ static int adj_check(charm_gpu_ctx_t *c, int nsteps, int ckpt, int theta_face)
                         "the LF, HLLC or CD flux");
         return CHARM_ERR_UNSUPPORTED;
     }
-    if (nsteps < 1 || ckpt < 1 || theta_face < -1 || theta_face > 5 ||
+    if (nsteps < 1 || ckpt < 0 || theta_face < -1 || theta_face > 5 ||
         (theta_face >= 0 && c->cfg.bnd_type[theta_face] != CHARM_BND_INLET)) {
         charm_set_error("adjoint: bad step count, checkpoint period or inlet face");
         return CHARM_ERR_ARG;
@@ -926,6 +926,45 @@ static int adj_forward(charm_gpu_ctx_t *c, double dt, int nsteps, int ckpt, cons
         O.partial = part + (size_t) s * nblk;
         HIPCHK(launch_adj_objective(c->sprec, c->nc, O, c->P, c->st));
     }
+    double *h = (double *) malloc((size_t) nsteps * nblk * sizeof(double));
+    if (!h) { charm_set_error("adjoint: out of memory"); return CHARM_ERR_MEM; }
+    hipError_t e = hipMemcpyAsync(h, part, (size_t) nsteps * nblk * sizeof(double), hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    if (e != hipSuccess) { free(h); charm_set_error("adjoint: %s", hipGetErrorString(e)); return CHARM_ERR_HIP; }
+    double j = 0.0;
+    for (int s = 0; s < nsteps; s++) {
+        double q = 0.0;
+        for (int64_t b = 0; b < nblk; b++) q += h[(size_t) s * nblk + b];
+        j += dt * q;
+    }
+    free(h);
+    *J = j;
+    return check_flag(c);
+}
+
+// forward pass that keeps every stage state: slot j of S0 = u^j (slot 0 the
+// initial state), S1 / S2 the stage states of step j, in the segment layout
+// the reverse sweep reads; the same stage launches as the recompute
+static int adj_forward_store(charm_gpu_ctx_t *c, double dt, int nsteps, const double lo[3],
+                             const double hi[3], char *S0, double *part, int64_t nblk, double *J)
+{
+    const size_t ssz = (size_t) c->n * 4 * c->nv * c->esz;
+    char *S1 = S0 + (size_t) (nsteps + 1) * ssz, *S2 = S1 + (size_t) nsteps * ssz;
+    HIPCHK(hipMemcpyAsync(S0, c->B[0], ssz, hipMemcpyDeviceToDevice, c->st));
+    for (int j = 0; j < nsteps; j++) {
+        void *u0 = S0 + (size_t) j * ssz, *u1 = S1 + (size_t) j * ssz, *u2 = S2 + (size_t) j * ssz;
+        void *un = S0 + (size_t) (j + 1) * ssz;
+        StageArgs A = stage_args(c, u0, u0, u1, 1, 0, dt);
+        HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st));
+        A = stage_args(c, u1, u0, u2, 1, 1, dt);
+        HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st));
+        A = stage_args(c, u2, u0, un, 1, 2, dt);
+        HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st));
+        ObjArgs O = obj_args(c, un, lo, hi);
+        O.partial = part + (size_t) j * nblk;
+        HIPCHK(launch_adj_objective(c->sprec, c->nc, O, c->P, c->st));
+    }
+    HIPCHK(hipMemcpyAsync(c->B[0], S0 + (size_t) nsteps * ssz, ssz, hipMemcpyDeviceToDevice, c->st));
     double *h = (double *) malloc((size_t) nsteps * nblk * sizeof(double));
     if (!h) { charm_set_error("adjoint: out of memory"); return CHARM_ERR_MEM; }
     hipError_t e = hipMemcpyAsync(h, part, (size_t) nsteps * nblk * sizeof(double), hipMemcpyDeviceToHost, c->st);
@@ -1093,7 +1132,12 @@ extern "C" int charm_gpu_adjoint(charm_gpu_ctx_t *c, double dt, int nsteps, int 
     const int64_t n = c->n, nblk = (n + 255) / 256;
     const size_t ssz = (size_t) n * 4 * c->nv * c->esz;
     const int64_t m = n * 4 * c->nv;
-    const int ncp = (nsteps + ckpt - 1) / ckpt;
+    // ckpt = 0: every stage state of the forward pass is kept (one segment
+    // of nsteps steps filled by the forward pass, no recompute); else
+    // checkpoints every ckpt steps and segments recomputed in two buffers
+    const bool store_all = ckpt == 0;
+    const int seg_len = store_all ? nsteps : ckpt;
+    const int ncp = store_all ? 1 : (nsteps + ckpt - 1) / ckpt;
     char *cp = NULL, *seg = NULL, *la = NULL, *lb = NULL, *lc = NULL, *segb[2] = {NULL, NULL};
     hipStream_t st2 = NULL;
     hipEvent_t ev_ready[2] = {NULL, NULL}, ev_done[2] = {NULL, NULL};
@@ -1108,14 +1152,14 @@ extern "C" int charm_gpu_adjoint(charm_gpu_ctx_t *c, double dt, int nsteps, int 
                          adj_chunk_smem_bytes(1, c->nc, c->max_faces) <= (size_t) lds_max;
     (void) hipGetLastError();
     hipError_t e = hipMalloc((void **) &cp, (size_t) ncp * ssz);
-    if (e == hipSuccess) e = hipMalloc((void **) &seg, (size_t) 2 * (3 * ckpt + 1) * ssz);
+    if (e == hipSuccess) e = hipMalloc((void **) &seg, (size_t) (store_all ? 1 : 2) * (3 * seg_len + 1) * ssz);
     if (e == hipSuccess) e = hipStreamCreate(&st2);
     for (int i = 0; i < 2 && e == hipSuccess; i++) {
         e = hipEventCreateWithFlags(&ev_ready[i], hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&ev_done[i], hipEventDisableTiming);
     }
     segb[0] = seg;
-    segb[1] = seg ? seg + (size_t) (3 * ckpt + 1) * ssz : NULL;
+    segb[1] = (seg && !store_all) ? seg + (size_t) (3 * seg_len + 1) * ssz : seg;
     if (e == hipSuccess) e = hipMalloc((void **) &la, ssz);
     if (e == hipSuccess) e = hipMalloc((void **) &lb, ssz);
     if (e == hipSuccess) e = hipMalloc((void **) &lc, ssz);
@@ -1129,7 +1173,9 @@ extern "C" int charm_gpu_adjoint(charm_gpu_ctx_t *c, double dt, int nsteps, int 
     }
 #define ADJ_CHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { charm_set_error("adjoint: %s", hipGetErrorString(e_)); rc = CHARM_ERR_HIP; goto done; } } while (0)
     ADJ_CHK(hipEventRecord(c->ev0, c->st));
-    if ((rc = adj_forward(c, dt, nsteps, ckpt, lo, hi, cp, part, nblk, J))) goto done;
+    if (store_all) rc = adj_forward_store(c, dt, nsteps, lo, hi, segb[0], part, nblk, J);
+    else rc = adj_forward(c, dt, nsteps, ckpt, lo, hi, cp, part, nblk, J);
+    if (rc) goto done;
     ADJ_CHK(hipEventRecord(c->ev1, c->st));
     ADJ_CHK(hipEventSynchronize(c->ev1));
     ADJ_CHK(hipEventElapsedTime(&ms_f, c->ev0, c->ev1));
@@ -1147,7 +1193,7 @@ extern "C" int charm_gpu_adjoint(charm_gpu_ctx_t *c, double dt, int nsteps, int 
     ADJ_CHK(hipEventRecord(ev_done[1], c->st));
     for (int sg = ncp - 1; sg >= -1; sg--) {
         // recompute segment sg - 1 ahead (sg = ncp - 1: segments ncp - 1 and ncp - 2)
-        for (int r = (sg == ncp - 1 ? sg : sg - 1); r >= sg - 1 && r >= 0; r--) {
+        for (int r = (sg == ncp - 1 ? sg : sg - 1); !store_all && r >= sg - 1 && r >= 0; r--) {
             const int start = r * ckpt, end = (start + ckpt < nsteps) ? start + ckpt : nsteps, L = end - start;
             char *S0 = segb[r & 1], *S1 = S0 + (size_t) (ckpt + 1) * ssz, *S2 = S1 + (size_t) ckpt * ssz;
             // the buffer's previous user (the adjoint of segment r + 2) is done
@@ -1166,9 +1212,9 @@ extern "C" int charm_gpu_adjoint(charm_gpu_ctx_t *c, double dt, int nsteps, int 
             ADJ_CHK(hipEventRecord(ev_ready[r & 1], st2));
         }
         if (sg < 0) break;
-        const int start = sg * ckpt, end = (start + ckpt < nsteps) ? start + ckpt : nsteps, L = end - start;
-        char *S0 = segb[sg & 1], *S1 = S0 + (size_t) (ckpt + 1) * ssz, *S2 = S1 + (size_t) ckpt * ssz;
-        ADJ_CHK(hipStreamWaitEvent(c->st, ev_ready[sg & 1], 0));
+        const int start = sg * seg_len, end = (start + seg_len < nsteps) ? start + seg_len : nsteps, L = end - start;
+        char *S0 = segb[sg & 1], *S1 = S0 + (size_t) (seg_len + 1) * ssz, *S2 = S1 + (size_t) seg_len * ssz;
+        if (!store_all) ADJ_CHK(hipStreamWaitEvent(c->st, ev_ready[sg & 1], 0));
         for (int j = L - 1; j >= 0; j--) {
             void *u0 = S0 + (size_t) j * ssz, *u1 = S1 + (size_t) j * ssz, *u2 = S2 + (size_t) j * ssz;
             void *un = S0 + (size_t) (j + 1) * ssz;

@@ -241,7 +241,9 @@ int charm_gpu_ns_chem(charm_gpu_ctx_t *ctx, double dt, double *chem_rhs, int64_t
  * dJ/d(inlet u, T, p) of boundary face theta_face (-1: none) by reverse
  * passes over checkpoints stored every ckpt steps (each segment's stage
  * states are recomputed), and the device times of the forward and the
- * reverse pass.  Both leave the state at u^N. */
+ * reverse pass.  ckpt = 0 keeps every stage state of the forward pass
+ * instead (3 nsteps + 1 states of device memory, no recompute).  Both leave
+ * the state at u^N. */
 int charm_gpu_objective(charm_gpu_ctx_t *ctx, double dt, int nsteps, const double lo[3],
                         const double hi[3], double *J);
 /* The stage VJP the adjoint sweeps with, on its own (no counterpart in the

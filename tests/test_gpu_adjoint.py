@@ -64,7 +64,7 @@ def test_initial_state_gradient_matches_directional_differences(adapted):
 
 def test_checkpoint_period_does_not_change_the_gradient():
     out = []
-    for ckpt in (1, 7, STEPS):
+    for ckpt in (1, 7, STEPS, 0):                      # 0: every stage state kept, no recompute
         cfg, lv, u0, per = case()
         out.append(device(cfg, lv, per, u0).adjoint(DT, STEPS, LO, HI, ckpt=ckpt, theta_face=0))
     for o in out[1:]:
