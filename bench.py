@@ -511,6 +511,12 @@ def main():
                                               "linf_rel": float(np.abs(dm).max() / np.abs(r64["state"]).max())}}
         del r64
 
+    # the level-7 contexts are done with: release their memory and HIP
+    # streams before the other workloads (the adjoint overlaps two streams,
+    # which need hardware queues of their own)
+    r32["ctx"].close()
+    rmx["ctx"].close()
+
     if not args.no_extra:
         print("[bench] smooth cases, FP32 against FP64", file=sys.stderr, flush=True)
         out["smooth_errors_vs_fp64"] = smooth_errors(CA, W)
