@@ -913,19 +913,46 @@ static ObjArgs obj_args(charm_gpu_ctx_t *c, const void *U, const double lo[3], c
 
 // forward pass: nsteps steps from B[0]; per-step objective partials into
 // part[nsteps][nblk]; optional checkpoints every ckpt steps
+// forward pass with checkpoints every ckpt steps.  With seg_last, the steps
+// of the last segment keep their stage states there, in the segment layout
+// of the reverse sweep (slot j of S0 = u^(start+j), S1 / S2 the stage
+// states), so that segment needs no recompute; the stage launches are the
+// recompute's, so the states are the same bits.
 static int adj_forward(charm_gpu_ctx_t *c, double dt, int nsteps, int ckpt, const double lo[3],
-                       const double hi[3], char *cp, double *part, int64_t nblk, double *J)
+                       const double hi[3], char *cp, char *seg_last, double *part, int64_t nblk, double *J)
 {
     const size_t ssz = (size_t) c->n * 4 * c->nv * c->esz;
+    const int last = ((nsteps - 1) / ckpt) * ckpt;
+    char *S0 = seg_last, *S1 = seg_last ? S0 + (size_t) (ckpt + 1) * ssz : NULL;
+    char *S2 = seg_last ? S1 + (size_t) ckpt * ssz : NULL;
     for (int s = 0; s < nsteps; s++) {
         if (cp && s % ckpt == 0)
             HIPCHK(hipMemcpyAsync(cp + (size_t) (s / ckpt) * ssz, c->B[0], ssz, hipMemcpyDeviceToDevice, c->st));
-        int rc = step_once(c, dt);
-        if (rc) return rc;
-        ObjArgs O = obj_args(c, c->B[0], lo, hi);
+        const void *ucur;
+        if (seg_last && s >= last) {
+            const int j = s - last;
+            void *u0 = S0 + (size_t) j * ssz, *u1 = S1 + (size_t) j * ssz, *u2 = S2 + (size_t) j * ssz;
+            void *un = S0 + (size_t) (j + 1) * ssz;
+            if (j == 0) HIPCHK(hipMemcpyAsync(u0, c->B[0], ssz, hipMemcpyDeviceToDevice, c->st));
+            StageArgs A = stage_args(c, u0, u0, u1, 1, 0, dt);
+            HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st));
+            A = stage_args(c, u1, u0, u2, 1, 1, dt);
+            HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st));
+            A = stage_args(c, u2, u0, un, 1, 2, dt);
+            HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st));
+            ucur = un;
+        } else {
+            int rc = step_once(c, dt);
+            if (rc) return rc;
+            ucur = c->B[0];
+        }
+        ObjArgs O = obj_args(c, ucur, lo, hi);
         O.partial = part + (size_t) s * nblk;
         HIPCHK(launch_adj_objective(c->sprec, c->nc, O, c->P, c->st));
     }
+    // the context's state ends at u^nsteps, as after step_once
+    if (seg_last)
+        HIPCHK(hipMemcpyAsync(c->B[0], S0 + (size_t) (nsteps - last) * ssz, ssz, hipMemcpyDeviceToDevice, c->st));
     double *h = (double *) malloc((size_t) nsteps * nblk * sizeof(double));
     if (!h) { charm_set_error("adjoint: out of memory"); return CHARM_ERR_MEM; }
     hipError_t e = hipMemcpyAsync(h, part, (size_t) nsteps * nblk * sizeof(double), hipMemcpyDeviceToHost, c->st);
@@ -990,7 +1017,7 @@ extern "C" int charm_gpu_objective(charm_gpu_ctx_t *c, double dt, int nsteps, co
     const int64_t nblk = (c->n + 255) / 256;
     double *part = NULL;
     HIPCHK(hipMalloc((void **) &part, (size_t) nsteps * nblk * sizeof(double)));
-    rc = adj_forward(c, dt, nsteps, 1, lo, hi, NULL, part, nblk, J);
+    rc = adj_forward(c, dt, nsteps, 1, lo, hi, NULL, NULL, part, nblk, J);
     (void) hipFree(part);
     return rc;
 }
@@ -1174,13 +1201,15 @@ extern "C" int charm_gpu_adjoint(charm_gpu_ctx_t *c, double dt, int nsteps, int 
 #define ADJ_CHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { charm_set_error("adjoint: %s", hipGetErrorString(e_)); rc = CHARM_ERR_HIP; goto done; } } while (0)
     ADJ_CHK(hipEventRecord(c->ev0, c->st));
     if (store_all) rc = adj_forward_store(c, dt, nsteps, lo, hi, segb[0], part, nblk, J);
-    else rc = adj_forward(c, dt, nsteps, ckpt, lo, hi, cp, part, nblk, J);
+    else rc = adj_forward(c, dt, nsteps, ckpt, lo, hi, cp, segb[(ncp - 1) & 1], part, nblk, J);
     if (rc) goto done;
     ADJ_CHK(hipEventRecord(c->ev1, c->st));
     ADJ_CHK(hipEventSynchronize(c->ev1));
     ADJ_CHK(hipEventElapsedTime(&ms_f, c->ev0, c->ev1));
 
-    // Reverse sweep.  Segment sg's stage states are recomputed from its
+    // Reverse sweep.  The last segment's stage states were kept by the
+    // forward pass (in segment buffer (ncp - 1) & 1, on the context stream).
+    // Segment sg's are recomputed from its
     // checkpoint into one of two segment buffers on a second stream while
     // the adjoint of segment sg + 1 runs on the context stream from the
     // other buffer (events order the reuse of a buffer); per segment buffer
@@ -1192,8 +1221,10 @@ extern "C" int charm_gpu_adjoint(charm_gpu_ctx_t *c, double dt, int nsteps, int 
     ADJ_CHK(hipEventRecord(ev_done[0], c->st));
     ADJ_CHK(hipEventRecord(ev_done[1], c->st));
     for (int sg = ncp - 1; sg >= -1; sg--) {
-        // recompute segment sg - 1 ahead (sg = ncp - 1: segments ncp - 1 and ncp - 2)
-        for (int r = (sg == ncp - 1 ? sg : sg - 1); !store_all && r >= sg - 1 && r >= 0; r--) {
+        // recompute segment sg - 1 ahead (the last segment's states come
+        // from the forward pass)
+        if (!store_all && sg >= 1) {
+            const int r = sg - 1;
             const int start = r * ckpt, end = (start + ckpt < nsteps) ? start + ckpt : nsteps, L = end - start;
             char *S0 = segb[r & 1], *S1 = S0 + (size_t) (ckpt + 1) * ssz, *S2 = S1 + (size_t) ckpt * ssz;
             // the buffer's previous user (the adjoint of segment r + 2) is done
@@ -1214,7 +1245,7 @@ extern "C" int charm_gpu_adjoint(charm_gpu_ctx_t *c, double dt, int nsteps, int 
         if (sg < 0) break;
         const int start = sg * seg_len, end = (start + seg_len < nsteps) ? start + seg_len : nsteps, L = end - start;
         char *S0 = segb[sg & 1], *S1 = S0 + (size_t) (seg_len + 1) * ssz, *S2 = S1 + (size_t) seg_len * ssz;
-        if (!store_all) ADJ_CHK(hipStreamWaitEvent(c->st, ev_ready[sg & 1], 0));
+        if (!store_all && sg != ncp - 1) ADJ_CHK(hipStreamWaitEvent(c->st, ev_ready[sg & 1], 0));
         for (int j = L - 1; j >= 0; j--) {
             void *u0 = S0 + (size_t) j * ssz, *u1 = S1 + (size_t) j * ssz, *u2 = S2 + (size_t) j * ssz;
             void *un = S0 + (size_t) (j + 1) * ssz;
