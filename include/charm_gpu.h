@@ -240,7 +240,8 @@ int charm_gpu_ns_chem(charm_gpu_ctx_t *ctx, double dt, double *chem_rhs, int64_t
  * charm_gpu_adjoint also returns dJ/du^0 (FP64 [n][V][4], may be NULL) and
  * dJ/d(inlet u, T, p) of boundary face theta_face (-1: none) by reverse
  * passes over checkpoints stored every ckpt steps (each segment's stage
- * states are recomputed), and the device times of the forward and the
+ * states are recomputed, except the last segment's, which the forward pass
+ * keeps), and the device times of the forward and the
  * reverse pass.  ckpt = 0 keeps every stage state of the forward pass
  * instead (3 nsteps + 1 states of device memory, no recompute).  Both leave
  * the state at u^N. */
