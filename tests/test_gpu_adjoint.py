@@ -72,6 +72,24 @@ def test_checkpoint_period_does_not_change_the_gradient():
         assert np.array_equal(o[1], out[0][1]) and np.array_equal(o[2], out[0][2])
 
 
+@pytest.mark.parametrize("precision", [CA.FP64, CA.FP32])
+def test_adjoint_leaves_the_final_state(precision):
+    # the forward pass ends at u^N whether the last segment's stage states
+    # are kept in the segment buffers or the steps run in place: the same
+    # bits as the objective's plain forward pass, for every checkpoint period
+    cfg, lv, u0, per = case(precision)
+    ref = device(cfg, lv, per, u0)
+    J = ref.objective(DT, STEPS, LO, HI)
+    u_end = ref.get_state()
+    assert not np.array_equal(u_end, np.asarray(u0, dtype=np.float64).reshape(u_end.shape))   # the steps moved it
+    for ckpt in (1, 5, 7, STEPS, 0):
+        cfg, lv, u0, per = case(precision)
+        ctx = device(cfg, lv, per, u0)
+        Ja = ctx.adjoint(DT, STEPS, LO, HI, ckpt=ckpt)[0]
+        assert Ja == J, (ckpt, Ja, J)
+        assert np.array_equal(ctx.get_state(), u_end), ckpt
+
+
 def test_fp32_adjoint_close_to_fp64():
     res = {}
     for prec in (CA.FP64, CA.FP32):
