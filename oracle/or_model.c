@@ -564,7 +564,14 @@ void or_limiter(or_sim_t *s)
             psi[v] = 1.;
         }
         or_quad_vertices(d->quad, vtx);
+#ifdef OR_MUTANT_BJ_FACE_CENTRES
+        /* planted mistake (never set by the Makefile): the six face centres
+         * instead of the eight vertices, for tests/test_oracle_mutants.py */
+        for (i = 0; i < 6; i++) for (k = 0; k < 3; k++) vtx[i][k] = d->g.fc[i][k];
+        for (i = 0; i < 6; i++) {
+#else
         for (i = 0; i < 8; i++) {
+#endif
             or_get_fields(s, d, vtx[i], &cj);
             or_cons_get(s, &cj, f);
             for (v = 0; v < s->nv; v++) {

@@ -369,9 +369,17 @@ static void or_flux_cd(const or_config_t *cfg, const or_prim_t prim[2],
     for (i = 0; i < fc; i++) q[i] = 0.5*(ff[i][1]+ff[i][0]);
 }
 
-#define F_HLLC_U(UK, FK, SK, SS, PK, RK, VK) (((SS)*((SK)*(UK)-(FK)) + (SK)*( (PK)+(RK)*((SK)-(VK))*((SS)-(VK)) )) / ((SK)-(SS)))
+/* OR_MUTANT_HLLC_SIGN (never set by the Makefile) flips the sign of the
+ * (S* - u_k) factor: a planted mistake that tests/test_oracle_mutants.py
+ * builds on purpose to show that the star-region checks catch it */
+#ifdef OR_MUTANT_HLLC_SIGN
+#define OR_SS_MINUS_VK(SS, VK) ((VK)-(SS))
+#else
+#define OR_SS_MINUS_VK(SS, VK) ((SS)-(VK))
+#endif
+#define F_HLLC_U(UK, FK, SK, SS, PK, RK, VK) (((SS)*((SK)*(UK)-(FK)) + (SK)*( (PK)+(RK)*((SK)-(VK))*OR_SS_MINUS_VK(SS, VK) )) / ((SK)-(SS)))
 #define F_HLLC_V(UK, FK, SK, SS, PK, RK, VK) (((SS)*((SK)*(UK)-(FK))) / ((SK)-(SS)))
-#define F_HLLC_E(UK, FK, SK, SS, PK, RK, VK) (((SS)*((SK)*(UK)-(FK)) + (SK)*( (PK)+(RK)*((SK)-(VK))*((SS)-(VK)) )*(SS)) / ((SK)-(SS)))
+#define F_HLLC_E(UK, FK, SK, SS, PK, RK, VK) (((SS)*((SK)*(UK)-(FK)) + (SK)*( (PK)+(RK)*((SK)-(VK))*OR_SS_MINUS_VK(SS, VK) )*(SS)) / ((SK)-(SS)))
 
 /* x-split HLLC core, src/charm_fluxes.c:319-437.  Returns nonzero where the
  * reference aborts (wave ordering violated, i.e. NaN input, :354-357). */

@@ -11,7 +11,9 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
-LIB_PATH = os.path.join(ORACLE_DIR, "liboracle.so")
+# CHARM_ORACLE_LIB points the checks at another build of the same sources
+# (tests/test_oracle_mutants.py loads builds with a planted mistake)
+LIB_PATH = os.environ.get("CHARM_ORACLE_LIB") or os.path.join(ORACLE_DIR, "liboracle.so")
 
 NC_MAX = 8
 NV_MAX = 4 + NC_MAX
