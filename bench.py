@@ -260,53 +260,103 @@ def extra_workloads(CA, W):
     return out
 
 
+def common_leaf_errors(leaves_a, ua, leaves_b, ub):
+    """Errors of ua against ub over the leaves both leaf sets hold (two runs
+    whose re-adaptations may part ways); the share of common leaves."""
+    key_b = {tuple(q): i for i, q in enumerate(np.asarray(leaves_b).tolist())}
+    ia, ib = [], []
+    for i, q in enumerate(np.asarray(leaves_a).tolist()):
+        j = key_b.get(tuple(q))
+        if j is not None:
+            ia.append(i)
+            ib.append(j)
+    e = rel_errors(ua[ia], ub[ib])
+    e["common_leaf_share"] = len(ia) / max(len(leaves_a), len(leaves_b))
+    return e
+
+
 def bench_bubbles(CA, W):
     # bubbles: levels 5-8 (about 1M leaves), exact Riemann, two species,
-    # re-adaptation every 20 steps
+    # re-adaptation every 20 steps; FP32 bulk (mixture EOS closure in FP64),
+    # then the FP64 path from the same leaves and state for its errors
     t0 = time.perf_counter()
     cfg, forest, u0 = W.c3(precision=CA.FP32)
     t_init = time.perf_counter() - t0
-    s = CA.Solver(forest, cfg)
-    s.set_state(u0)
-    dt = s.get_dt(1)
-    n0 = forest.n
-    m0 = s.conservation_sums()
-    gpu_ms, adapt_s, dofs = 0.0, 0.0, 0
-    for blk in range(10):
-        if blk:
-            t = time.perf_counter()
-            s.adapt()
-            adapt_s += time.perf_counter() - t
-        dofs += W.dof_updates(forest.n, 6, 3, 20)
-        s.run(dt, 20)
-        gpu_ms += s.last_ms()
-    m1 = s.conservation_sums()
+    leaves0 = forest.leaves()
+    runs = {}
+    for prec in (CA.FP32, CA.FP64):
+        if prec == CA.FP64:
+            cfg = W.c3_config(CA.FP64, cfg.max_err, cfg.min_level, cfg.max_level)
+            forest = CA.Forest.from_leaves(leaves0)
+        s = CA.Solver(forest, cfg)
+        s.set_state(u0)
+        dt = s.get_dt(1) if prec == CA.FP32 else runs[CA.FP32]["dt"]
+        m0 = s.conservation_sums()
+        gpu_ms, adapt_s, dofs = 0.0, 0.0, 0
+        for blk in range(10):
+            if blk:
+                t = time.perf_counter()
+                s.adapt()
+                adapt_s += time.perf_counter() - t
+            dofs += W.dof_updates(forest.n, 6, 3, 20)
+            s.run(dt, 20)
+            gpu_ms += s.last_ms()
+        m1 = s.conservation_sums()
+        runs[prec] = dict(dt=dt, gpu_ms=gpu_ms, adapt_s=adapt_s, dofs=dofs, m0=m0, m1=m1,
+                          leaves=forest.leaves(), state=s.get_state(), n=forest.n)
+        s.close()
+    r = runs[CA.FP32]
     out = {
-        "leaves_initial": n0, "leaves_final": forest.n, "steps": 200, "adapt_every": 20,
-        "dof_updates_per_s_device": dofs / (gpu_ms * 1e-3),
-        "dof_updates_per_s_with_adaptation": dofs / (gpu_ms * 1e-3 + adapt_s),
-        "ms_per_step": gpu_ms / 200, "host_s_per_adaptation": adapt_s / 9, "init_s": t_init,
-        "species_mass_change": [float((m1[4 + i] - m0[4 + i]) / m0[4 + i]) for i in range(2)]}
-    s.close()
+        "leaves_initial": len(leaves0), "leaves_final": r["n"], "steps": 200, "adapt_every": 20,
+        "dof_updates_per_s_device": r["dofs"] / (r["gpu_ms"] * 1e-3),
+        "dof_updates_per_s_with_adaptation": r["dofs"] / (r["gpu_ms"] * 1e-3 + r["adapt_s"]),
+        "ms_per_step": r["gpu_ms"] / 200, "host_s_per_adaptation": r["adapt_s"] / 9, "init_s": t_init,
+        "species_mass_change": [float((r["m1"][4 + i] - r["m0"][4 + i]) / r["m0"][4 + i]) for i in range(2)],
+        "eos": "mixture closure (cp, M, gamma) in FP64 arithmetic on the FP32 path"}
+    q = runs[CA.FP64]
+    mass = lambda m: m[4] + m[5]
+    out["fp64"] = {"ms_per_step": q["gpu_ms"] / 200, "leaves_final": q["n"],
+                   "species_mass_change": [float((q["m1"][4 + i] - q["m0"][4 + i]) / q["m0"][4 + i]) for i in range(2)]}
+    out["errors_vs_fp64"] = common_leaf_errors(r["leaves"], r["state"], q["leaves"], q["state"])
+    # conserved mass: the FP32 run's total against the FP64 run's (both lose
+    # mass through the transmissive faces alike)
+    out["errors_vs_fp64"]["mass_drift"] = float(abs(mass(r["m1"]) - mass(q["m1"])) / abs(mass(q["m0"])))
     return out
 
 
 def bench_hot_spot(CA, W):
     # reacting hot spot: level 6, four species, BR1 viscous terms, diffusion,
-    # chemistry, 100 steps
-    cfg, leaves, u0, per = W.c4(6, CA.FP32)
-    ctx = CA.GpuContext(cfg)
-    ce, cf = CA.recommended_chunks(cfg)
-    ctx.upload_mesh(CA.Forest.uniform(6).flatten(ce, cf))
-    ctx.set_state(u0)
-    dt = ctx.get_dt(1)
-    ctx.run(dt, 2)
-    ctx.set_state(u0)
-    ctx.run(dt, 100)
-    ms = ctx.last_ms()
-    ctx.close()
-    return {"leaves": len(leaves), "steps": 100, "ms_per_step": ms / 100,
-            "dof_updates_per_s": W.dof_updates(len(leaves), 8, 3, 100) / (ms * 1e-3)}
+    # chemistry (A = 1e9, molar energy source), 100 steps; FP32 transport with
+    # the FP64 Newton temperature and chemistry, then the FP64 path for its errors
+    runs = {}
+    for prec in (CA.FP32, CA.FP64):
+        cfg, leaves, u0, per = W.c4(6, prec)
+        ctx = CA.GpuContext(cfg)
+        ce, cf = CA.recommended_chunks(cfg)
+        ctx.upload_mesh(CA.Forest.uniform(6).flatten(ce, cf))
+        ctx.set_state(u0)
+        dt = ctx.get_dt(1) if prec == CA.FP32 else runs[CA.FP32]["dt"]
+        ctx.run(dt, 2)
+        ctx.set_state(u0)
+        m0 = ctx.conservation_sums()
+        ctx.run(dt, 100)
+        ms = ctx.last_ms()
+        runs[prec] = dict(dt=dt, ms=ms, m0=m0, m1=ctx.conservation_sums(), state=ctx.get_state(), u0=u0)
+        ctx.close()
+    r, q = runs[CA.FP32], runs[CA.FP64]
+    mass = lambda m: m[4:].sum()
+    h2o = q["state"][:, 6, 0]
+    out = {"leaves": len(leaves), "steps": 100, "ms_per_step": r["ms"] / 100,
+           "dof_updates_per_s": W.dof_updates(len(leaves), 8, 3, 100) / (r["ms"] * 1e-3),
+           "reaction": "2 H2 + O2 -> 2 H2O, k = A exp(-15000 K / T), A = %g; energy source sum_i w_i M_i h_i "
+                       "(the reference's sum_i w_i h_i mixes mol and kg units and fails within two steps "
+                       "at this A)" % cfg.react[0].a,
+           "fp64": {"ms_per_step": q["ms"] / 100,
+                    "mass_drift": float(abs(mass(q["m1"]) - mass(q["m0"])) / mass(q["m0"])),
+                    "h2o_mass_max": float(h2o.max() / q["state"][:, 4:, 0].sum(1)[np.argmax(h2o)])},
+           "mass_drift": float(abs(mass(r["m1"]) - mass(r["m0"])) / mass(r["m0"])),
+           "errors_vs_fp64": rel_errors(r["state"], q["state"])}
+    return out
 
 
 def bench_adjoint(CA, W, fd_check=True):

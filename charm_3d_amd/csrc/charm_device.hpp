@@ -24,6 +24,7 @@ struct DevParams {
     int    eos;
     double gam, cv, m;                 // single-component ideal gas
     double comp_m[kMaxComp];           // mixture: molar masses
+    double comp_minv[kMaxComp];        //          1 / M_i (FP64, for the FP32 path's closure)
     double comp_cp[kMaxComp];          //          constant cp per species
     int    bnd_type[6];
     double bnd_param[6][5];
@@ -45,7 +46,7 @@ struct DevParams {
     double sig[kMaxComp], ek[kMaxComp];
     double t_ref;
     int    use_visc, use_diff;
-    int    n_react, chem_parity;
+    int    n_react, chem_parity, chem_energy_molar;
     int    r_left[4][3], r_right[4][3];
     double r_a[4], r_e[4], r_n[4];
 };
@@ -178,6 +179,21 @@ __device__ __forceinline__ void eos_coeffs(const DevParams &P, Prim<R, NC> &p)
     }
     if constexpr (NC == 1) {
         p.gam = cst<R>(P.gam, P.gam_f); p.cv = cst<R>(P.cv, P.cv_f); p.m = cst<R>(P.m, P.m_f);
+    } else if constexpr (sizeof(R) == 4) {
+        // FP32 bulk path: the mixture closure in FP64 arithmetic (FP64
+        // EOS), only its results rounded: M^-1 = sum c_i / M_i with the
+        // host's FP64 reciprocals, R / M = R M^-1
+        double cp = 0.0, minv = 0.0;
+        #pragma unroll
+        for (int i = 0; i < NC; i++) {
+            minv += (double) p.c[i] * P.comp_minv[i];
+            cp += (double) p.c[i] * P.comp_cp[i];
+        }
+        const double cv = cp - kGasR * minv;
+        const bool ideal = P.eos == EOS_IDEAL;
+        p.gam = sel(ideal, P.gam_f, (float) (cp / cv));
+        p.cv = sel(ideal, P.cv_f, (float) cv);
+        p.m = sel(ideal, P.m_f, (float) (1.0 / minv));
     } else {
         // the mixture values, then a select on the (uniform) EOS type: with
         // two branches writing the record the compiler kept both candidates

@@ -26,6 +26,13 @@ __device__ __forceinline__ R at(const R *c, R px, R py, R pz)
     return c[0] + px * c[1] + py * c[2] + pz * c[3];
 }
 
+// component a (0, 1, 2; known at run time) of three values, by selects
+template <typename R>
+__device__ __forceinline__ R pick3(int a, R v0, R v1, R v2)
+{
+    return a == 0 ? v0 : (a == 1 ? v1 : v2);
+}
+
 // reference coordinates of face point k on the face of direction d of a leaf
 // (normal coordinate +-1/2, tangential offsets off + scale * gp); xyz order
 template <typename R>

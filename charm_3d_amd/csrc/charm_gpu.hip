@@ -24,6 +24,7 @@ struct charm_gpu_ctx {
     int ph;                   // 1: Navier-Stokes / real-gas kernels (PH = 1)
     size_t esz;
     DevParams P;
+    DevParams *d_P;           // device copy of P (kernels that index it at run time)
     hipStream_t st;
     hipEvent_t ev0, ev1;
     float last_ms;
@@ -136,7 +137,10 @@ extern "C" int charm_gpu_create(const charm_config_t *cfg, charm_gpu_ctx_t **out
     P.m = cfg->comp[0].m;
     P.cv = cp0 - kGasR / P.m;
     P.gam = cp0 / P.cv;
-    for (int i = 0; i < c->nc; i++) { P.comp_m[i] = cfg->comp[i].m; P.comp_cp[i] = cfg->comp[i].cp[0]; }
+    for (int i = 0; i < c->nc; i++) {
+        P.comp_m[i] = cfg->comp[i].m; P.comp_cp[i] = cfg->comp[i].cp[0];
+        P.comp_minv[i] = 1.0 / cfg->comp[i].m;
+    }
     for (int f = 0; f < 6; f++) {
         P.bnd_type[f] = cfg->bnd_type[f];
         for (int j = 0; j < 5; j++) P.bnd_param[f][j] = cfg->bnd_param[f][j];
@@ -163,6 +167,7 @@ extern "C" int charm_gpu_create(const charm_config_t *cfg, charm_gpu_ctx_t **out
     P.use_diff = cfg->use_diff != 0;
     P.n_react = cfg->n_react;
     P.chem_parity = cfg->chem_parity != 0;
+    P.chem_energy_molar = cfg->chem_energy_molar != 0;
     for (int r = 0; r < cfg->n_react; r++) {
         for (int m = 0; m < 3; m++) { P.r_left[r][m] = cfg->react[r].left[m]; P.r_right[r][m] = cfg->react[r].right[m]; }
         P.r_a[r] = cfg->react[r].a; P.r_e[r] = cfg->react[r].e; P.r_n[r] = cfg->react[r].n;
@@ -196,7 +201,10 @@ extern "C" int charm_gpu_create(const charm_config_t *cfg, charm_gpu_ctx_t **out
     if ((e = hipStreamCreate(&c->st)) != hipSuccess || (e = hipEventCreate(&c->ev0)) != hipSuccess ||
         (e = hipEventCreate(&c->ev1)) != hipSuccess || (e = hipMalloc(&c->d_err, sizeof(int))) != hipSuccess ||
         (e = hipMalloc(&c->d_result, 16 * sizeof(double))) != hipSuccess ||
-        (e = hipMemsetAsync(c->d_err, 0, sizeof(int), c->st)) != hipSuccess) {
+        (e = hipMalloc(&c->d_P, sizeof(DevParams))) != hipSuccess ||
+        (e = hipMemcpyAsync(c->d_P, &c->P, sizeof(DevParams), hipMemcpyHostToDevice, c->st)) != hipSuccess ||
+        (e = hipMemsetAsync(c->d_err, 0, sizeof(int), c->st)) != hipSuccess ||
+        (e = hipStreamSynchronize(c->st)) != hipSuccess) {
         charm_set_error("gpu_create: %s", hipGetErrorString(e));
         charm_gpu_destroy(c);
         return CHARM_ERR_HIP;
@@ -212,6 +220,7 @@ extern "C" int charm_gpu_destroy(charm_gpu_ctx_t *c)
     free_mesh(c);
     if (c->d_err) (void) hipFree(c->d_err);
     if (c->d_result) (void) hipFree(c->d_result);
+    if (c->d_P) (void) hipFree(c->d_P);
     if (c->ev0) (void) hipEventDestroy(c->ev0);
     if (c->ev1) (void) hipEventDestroy(c->ev1);
     if (c->st) (void) hipStreamDestroy(c->st);
@@ -346,6 +355,7 @@ static StageArgs stage_args(charm_gpu_ctx_t *c, const void *U, const void *U0, v
     A.n_xcd = c->n_xcd;
     A.mode = mode; A.combine = combine; A.dt = dt;
     A.visc = (c->ph && c->cfg.use_visc) ? c->d_vi : NULL;
+    A.prm = c->d_P;
     return A;
 }
 
@@ -378,6 +388,7 @@ static NsArgs ns_args(charm_gpu_ctx_t *c, const void *U, void *out, double dt)
     A.dcoef = (c->cfg.use_diff && c->nc >= 2) ? c->d_dcoef : NULL;
     A.aux_in = c->d_auxf; A.aux_out = c->d_auxf; A.visc_out = c->d_vi;
     A.n = c->n; A.dt = dt;
+    A.prm = c->d_P;
     return A;
 }
 

@@ -23,6 +23,11 @@ struct StageArgs {
     int combine;              // 0: none, 1: w/4 + 3 u0/4, 2: 2w/3 + u0/3
     double dt;
     const void *visc;         // [16][n] viscous integrals added before A^-1 (or null)
+    // the context's constants in device memory: the Navier-Stokes / real-gas
+    // variant (PH = 1) reads them here, so that its runtime indices (power
+    // series up to n_cp, the BC record of a lane's face) are loads rather
+    // than a private copy of the whole by-value block
+    const DevParams *prm;
 };
 
 struct LimArgs {
@@ -89,6 +94,7 @@ struct NsArgs {
     void *visc_out;           // [16][n] viscous integrals of ru rv rw re
     int64_t n;
     double dt;
+    const DevParams *prm;     // the context's constants in device memory (read by every NS kernel)
 };
 
 // adjoint of one SSP-RK stage (one thread per leaf):

@@ -79,8 +79,9 @@ __device__ __forceinline__ void state_at(const DevParams &P, const R *u, const R
 // then rc_i <- (C^_i M_i, 0, 0, 0).  Other variables pass through.
 template <typename S, int NC>
 __global__ __launch_bounds__(256)
-void k_ns_chem(NsArgs A, DevParams P)
+void k_ns_chem(NsArgs A, DevParams)
 {
+    const DevParams &P = *A.prm;         // reaction tables indexed at run time
     constexpr int NV = 4 + NC;
     const int64_t n = A.n;
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -144,7 +145,9 @@ void k_ns_chem(NsArgs A, DevParams P)
                         }
                         if (P.r_right[r][m] == i) ws += w[r];
                     }
-                rhs += ws * enthalpy(P, i, t);
+                // the reference's molar rate times mass enthalpy, or (corrected
+                // units) times the molar enthalpy M_i h_i
+                rhs += ws * (P.chem_energy_molar ? P.comp_m[i] : 1.0) * enthalpy(P, i, t);
             }
             A.chem_rhs[e] = rhs;
         }
@@ -165,8 +168,9 @@ void k_ns_chem(NsArgs A, DevParams P)
 // coefficients with the fitted collision integral, Wilke's mixture rule
 template <typename S, int NC>
 __global__ __launch_bounds__(256)
-void k_ns_diffusion(NsArgs A, DevParams P)
+void k_ns_diffusion(NsArgs A, DevParams)
 {
+    const DevParams &P = *A.prm;
     constexpr int NV = 4 + NC;
     const int64_t n = A.n;
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -211,8 +215,9 @@ void k_ns_diffusion(NsArgs A, DevParams P)
 // Y_i) with the sign folded as in the reference.
 template <typename S, int NC>
 __global__ __launch_bounds__(128)
-void k_ns_grad(NsArgs A, DevParams P)
+void k_ns_grad(NsArgs A, DevParams)
 {
+    const DevParams &P = *A.prm;         // power series and BC records indexed at run time
     using R = S;
     constexpr int NV = 4 + NC;
     const int64_t n = A.n;
@@ -353,8 +358,9 @@ void k_ns_grad(NsArgs A, DevParams P)
 // centre with the cell-mean velocity: -(-mu v_t / |l|, -q . n).
 template <typename S, int NC>
 __global__ __launch_bounds__(128)
-void k_ns_visc(NsArgs A, DevParams P)
+void k_ns_visc(NsArgs A, DevParams)
 {
+    const DevParams &P = *A.prm;
     using R = S;
     constexpr int NV = 4 + NC;
     const int64_t n = A.n;
@@ -424,13 +430,18 @@ void k_ns_visc(NsArgs A, DevParams P)
                 fu[0] += p.u * t[0] + p.v * t[3] + p.w * t[4];
                 fu[1] += p.u * t[3] + p.v * t[1] + p.w * t[5];
                 fu[2] += p.u * t[4] + p.v * t[5] + p.w * t[2];
-                ft += at<R>(&xx[4 * (6 + a)], pp[0], pp[1], pp[2]);         // q . n, n = +e_a
+                // q . n, n = +e_a: the three components evaluated and one
+                // selected (a runtime index into the register array would
+                // put it in private memory)
+                ft += pick3(a, at<R>(&xx[24], pp[0], pp[1], pp[2]), at<R>(&xx[28], pp[0], pp[1], pp[2]),
+                            at<R>(&xx[32], pp[0], pp[1], pp[2]));
             }
             // avg(tau) . n with n = +e_a: the row a of the averaged tensor
-            const R tr[3][3] = {{ftau[0], ftau[3], ftau[4]}, {ftau[3], ftau[1], ftau[5]}, {ftau[4], ftau[5], ftau[2]}};
             R q[4];
-            q[0] = R(0.5) * tr[0][a]; q[1] = R(0.5) * tr[1][a]; q[2] = R(0.5) * tr[2][a];
-            q[3] = R(0.5) * fu[a] - R(0.5) * ft;
+            q[0] = R(0.5) * pick3(a, ftau[0], ftau[3], ftau[4]);
+            q[1] = R(0.5) * pick3(a, ftau[3], ftau[1], ftau[5]);
+            q[2] = R(0.5) * pick3(a, ftau[4], ftau[5], ftau[2]);
+            q[3] = R(0.5) * pick3(a, fu[0], fu[1], fu[2]) - R(0.5) * ft;
             add(q, ps, sign * wj);
         },
         [&](int d, int face, int k, const R *, R) {
@@ -450,7 +461,8 @@ void k_ns_visc(NsArgs A, DevParams P)
             const R vv[3] = {pm.u, pm.v, pm.w};
             const R vt[3] = {a == 0 ? R(0) : vv[0], a == 1 ? R(0) : vv[1], a == 2 ? R(0) : vv[2]};
             const R ll = R(0.5) * h;
-            const R qn = at<R>(&x[4 * (6 + a)], pc[0], pc[1], pc[2]) * sg;
+            const R qn = pick3(a, at<R>(&x[24], pc[0], pc[1], pc[2]), at<R>(&x[28], pc[0], pc[1], pc[2]),
+                               at<R>(&x[32], pc[0], pc[1], pc[2])) * sg;
             R q[4];
             q[0] = -mu * dv(vt[0], ll); q[1] = -mu * dv(vt[1], ll); q[2] = -mu * dv(vt[2], ll);
             q[3] = -qn;

@@ -266,10 +266,20 @@ __device__ __forceinline__ void volume_integral(const DevParams &P, const R *u, 
 // state, update and combine).  PH = 0: constant-cp Euler terms; PH = 1: any
 // cp (FP64 Newton T) and the viscous integrals of the Navier-Stokes model
 // added before A^-1.
+// PH = 1 reads the constants from device memory (runtime-indexed power
+// series and BC records), PH = 0 from the by-value kernel argument
+template <int PH>
+__device__ __forceinline__ const DevParams &stage_params(const StageArgs &A, const DevParams &Pv)
+{
+    if constexpr (PH == 1) return *A.prm;
+    else return Pv;
+}
+
 template <typename S, typename R, int NC, int FLUX, int BS, int PH>
 __global__ __launch_bounds__(BS)
-void k_stage(StageArgs A, DevParams P)
+void k_stage(StageArgs A, DevParams Pv)
 {
+    const DevParams &P = stage_params<PH>(A, Pv);
     constexpr int NV = 4 + NC;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     R *mom = reinterpret_cast<R *>(smem_raw);                       // [maxf][3][NV]

@@ -186,12 +186,15 @@ def c3(precision=FP32, seed=3, min_level=5, max_level=8, max_err=0.04, width=Non
 # mixture energy (cp(T) - R / M) T non-monotonic, and the reference's Newton
 # temperature then lands on a spurious root.  Sutherland viscosity and
 # conductivity with KP0 = ML0 cp(T0) / 0.72, Lennard-Jones diffusion
-# parameters, and one reaction H2 + O2 -> H2O with k = A exp(-15000 / T).
+# parameters, and one reaction 2 H2 + O2 -> 2 H2O with k = A exp(-15000 / T),
+# A = 1e9 (the reference's left / right species lists hold H2 and H2O twice;
+# the rate is k C_H2^2 C_O2 and mass is conserved).
 # The reference's energy source multiplies molar rates (mol / m^3 / s) by
-# mass enthalpies (J / kg); with A = 1e9 that source releases about twenty
+# mass enthalpies (J / kg); at A = 1e9 that source releases about twenty
 # times the local internal energy per step at the hot-spot peak and the
-# state fails within two steps (on the oracle as on the device), so the case
-# runs with A = 1e5.
+# state fails within two steps (on the oracle as on the device).  The case
+# therefore runs with chem_energy_molar = 1: the rates times the molar
+# enthalpies M_i h_i, whose release is bounded by the fuel present.
 C4_SPECIES = ("H2", "O2", "H2O", "N2")
 C4_M = (0.002016, 0.031999, 0.018015, 0.028014)
 C4_NASA_LOW = ((2.34433112, 7.98052075e-03, -1.94781510e-05, 2.01572094e-08, -7.37611761e-12),
@@ -218,14 +221,14 @@ def c4_cp_over_r():
     return tuple(out)
 
 
-C4_A = 1.0e5
+C4_A = 1.0e9
 C4_H0 = (0.0, 0.0, -241826.0 / 0.018015, 0.0)            # formation enthalpy at 298.15 K, J/kg
 C4_ML = ((8.41e-6, 72.0), (1.919e-5, 127.0), (1.12e-5, 1064.0), (1.663e-5, 111.0))   # mu(273 K), S
 C4_LJ = ((2.92, 38.0), (3.458, 107.4), (2.605, 572.4), (3.621, 97.53))               # sigma A, eps/k K
 C4_Y = (0.02, 0.22, 0.0, 0.76)
 
 
-def c4_config(precision=FP32, limiter=False):
+def c4_config(precision=FP32, limiter=False, a=None, chem_energy_molar=True):
     cps = tuple(tuple(a * GAS_R / m for a in fit) for fit, m in zip(c4_cp_over_r(), C4_M))
     t0 = 273.0
     tr = []
@@ -234,10 +237,11 @@ def c4_config(precision=FP32, limiter=False):
         ml0, ts = C4_ML[i]
         tr.append(dict(ml0=ml0, kp0=ml0 * cp0 / 0.72, t0=t0, ts=ts, ml_type=1, kp_type=1,
                        sig=C4_LJ[i][0], ek=C4_LJ[i][1], h0=C4_H0[i]))
-    rx = [((0, 1, -1), (2, -1, -1), C4_A, 15000.0 * GAS_R, 0.0)]
+    rx = [((0, 0, 1), (2, 2, -1), C4_A if a is None else a, 15000.0 * GAS_R, 0.0)]
     return make_config(precision=precision, model="NS", flux="HLLC", limiter=limiter, eos="MIX",
                        comp_m=C4_M, comp_cp=cps, bnd=("OUTLET",) * 6, cfl=0.3, transport=tr,
-                       use_visc=True, use_diff=True, t_ref=298.15, reactions=rx)
+                       use_visc=True, use_diff=True, t_ref=298.15, reactions=rx,
+                       chem_energy_molar=chem_energy_molar)
 
 
 def c4_state_fn(cfg, r0=0.1, p=101325.0):
@@ -261,11 +265,12 @@ def c4_state_fn(cfg, r0=0.1, p=101325.0):
     return fn
 
 
-def c4(level=6, precision=FP32):
+def c4(level=6, precision=FP32, **kw):
     """Reacting hot spot: level-6 uniform cube (262,144 leaves), four
-    species, HLLC + BR1 viscous terms + diffusion + chemistry, SSP-RK3."""
+    species, HLLC + BR1 viscous terms + diffusion + chemistry, SSP-RK3.
+    kw: c4_config's a (rate constant) and chem_energy_molar."""
     leaves = Forest.uniform(level).leaves()
-    cfg = c4_config(precision)
+    cfg = c4_config(precision, **kw)
     return cfg, leaves, project_p1(leaves, c4_state_fn(cfg), 8), (False, False, False)
 
 

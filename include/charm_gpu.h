@@ -111,6 +111,11 @@ typedef struct {
     charm_reaction_desc_t react[4];
     int    chem_parity;        /* 1: the reference's chem_w[species] index quirk
                                   (src/models/ns/charm_model_ns_chem.c:97) */
+    int    chem_energy_molar;  /* energy source of the chemistry: 0 the reference's
+                                  sum_i w_i h_i, molar rate (mol/m^3/s) times mass
+                                  enthalpy (J/kg) (src/models/ns/charm_model_ns_chem.c:
+                                  66-108); 1 sum_i w_i M_i h_i (J/m^3/s), the rate times
+                                  the molar enthalpy */
     /* output (src/charm_init_yaml.cpp:388, include/charm_globals.h:294) */
     int    write_period;       /* FILE_OUTPUT_STEP: VTK files every that many steps (0: none) */
     char   comp_name[CHARM_MAX_COMP][32];   /* components' names: cell data C_<name> */

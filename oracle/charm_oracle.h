@@ -87,6 +87,8 @@ typedef struct {
     or_reaction_t react[4];
     double cfl;
     int    chem_parity;        /* 1: the reference's chem_w[species] index (see or_ns.c) */
+    int    chem_energy_molar;  /* 1: energy source sum w_i M_i h_i instead of the
+                                  reference's sum w_i h_i (see or_ns.c) */
 } or_config_t;
 
 typedef struct or_sim or_sim_t;

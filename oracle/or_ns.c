@@ -473,7 +473,10 @@ static double or_chem_energy(const or_config_t *cfg, double t, const double *c_)
                 if (i == r->right[m]) ws += w.w[n];
             }
         }
-        rhs += ws*or_comp_calc_enthalpy(&cfg->comp[i], t, cfg->t_ref);
+        /* the reference multiplies the molar rate (mol/m^3/s) by the mass
+         * enthalpy (J/kg); chem_energy_molar takes the molar enthalpy
+         * M_i h_i instead, which gives the source in J/m^3/s */
+        rhs += ws*(cfg->chem_energy_molar ? cfg->comp[i].m : 1.)*or_comp_calc_enthalpy(&cfg->comp[i], t, cfg->t_ref);
     }
     return rhs;
 }

@@ -52,7 +52,7 @@ class Config(C.Structure):
                 ("bnd_type", C.c_int * 6), ("bnd_param", (C.c_double * 16) * 6),
                 ("grav", C.c_double * 3), ("use_visc", C.c_int), ("use_diff", C.c_int),
                 ("t_ref", C.c_double), ("n_react", C.c_int), ("react", Reaction * 4),
-                ("cfl", C.c_double), ("chem_parity", C.c_int)]
+                ("cfl", C.c_double), ("chem_parity", C.c_int), ("chem_energy_molar", C.c_int)]
 
 
 def build():
@@ -118,7 +118,7 @@ def _dptr(a):
 def make_config(n_comp=1, comp_m=(GAS_R,), comp_cp=((3.5,),), eos="IDEAL", flux="LF",
                 limiter=False, model="NS", bnd=("OUTLET",) * 6, bnd_param=None,
                 grav=(0, 0, 0), cfl=0.3, transport=None, use_visc=False, use_diff=False,
-                t_ref=298.15, reactions=(), chem_parity=False):
+                t_ref=298.15, reactions=(), chem_parity=False, chem_energy_molar=False):
     """transport: per species dict with ml0, kp0, t0, ts, sig, ek, h0, ml_type,
     kp_type (Sutherland = 1); reactions: (left[3], right[3], a, e, n) tuples."""
     cfg = Config()
@@ -155,6 +155,7 @@ def make_config(n_comp=1, comp_m=(GAS_R,), comp_cp=((3.5,),), eos="IDEAL", flux=
             cfg.react[j].right[m] = right[m]
         cfg.react[j].a, cfg.react[j].e, cfg.react[j].n = a, e, n
     cfg.chem_parity = int(chem_parity)
+    cfg.chem_energy_molar = int(chem_energy_molar)
     return cfg
 
 

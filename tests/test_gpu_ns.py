@@ -186,11 +186,15 @@ def test_fp32_ns_close_to_fp64():
     assert err < 2e-6, err
 
 
-def test_reacting_hot_spot_small_matches_oracle():
+@pytest.mark.parametrize("level,a,molar", [(3, 1e5, False), (3, 1e9, True), (4, 1e9, True)])
+def test_reacting_hot_spot_small_matches_oracle(level, a, molar):
     """The four-species reacting hot spot (NASA polynomials, Sutherland
-    transport, diffusion, H2 + O2 -> H2O) at level 3, two steps."""
+    transport, diffusion, 2 H2 + O2 -> 2 H2O) at levels 3-4, two steps: the
+    reference's energy source at a rate it survives, and the molar energy
+    source at the configured A = 1e9 (at level 4 the spot ignites)."""
     import charm_3d_amd.workloads as W
-    cfg, lv, u0, per = W.c4(3, CA.FP64)
+    print(f"[hot spot] level {level} A {a:g} molar {molar}", flush=True)
+    cfg, lv, u0, per = W.c4(level, CA.FP64, a=a, chem_energy_molar=molar)
     sim = OL.OracleSim(oracle_cfg(cfg), lv)
     sim.set_state(u0)
     dt = sim.cfl_dt(1)

@@ -37,7 +37,8 @@ def oracle_cfg(cfg):
                           bnd=bnd, bnd_param=params, grav=[cfg.grav[i] for i in range(3)],
                           cfl=cfg.cfl, transport=tr, use_visc=bool(cfg.use_visc),
                           use_diff=bool(cfg.use_diff), t_ref=cfg.t_ref, reactions=rx,
-                          chem_parity=bool(cfg.chem_parity))
+                          chem_parity=bool(cfg.chem_parity),
+                          chem_energy_molar=bool(cfg.chem_energy_molar))
 
 
 def device(cfg, leaves, periodic, u0):

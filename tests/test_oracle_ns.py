@@ -239,6 +239,43 @@ def test_chemistry_energy_source():
     np.testing.assert_allclose(out[True][0], w * (hC - hA), rtol=1e-9)
 
 
+def test_chemistry_molar_energy_source_repeated_species():
+    """2 A + B -> 2 C written as left (A, A, B), right (C, C): rate w = k C_A^2
+    C_B, net molar rates -2w, -w, +2w; with chem_energy_molar the source is
+    sum_i w_i M_i h_i (J/m^3/s), the reference's mode sum_i w_i h_i.  Molar
+    masses with 2 M_A + M_B = 2 M_C conserve mass to the map's third order."""
+    A = 40.0
+    ms = (0.002, 0.032, 0.018)
+    h0 = (0.0, 0.0, -1.3e7)
+    lv = MU.uniform_leaves(1)
+    rx = [((0, 0, 1), (2, 2, -1), A, 0.0, 0.0)]
+    out = {}
+    for molar in (False, True):
+        tr = [dict(h0=h0[i]) for i in range(3)]
+        cfg = OL.make_config(n_comp=3, comp_m=ms, comp_cp=((1000.0,),) * 3, eos="MIX", transport=tr,
+                             reactions=rx, t_ref=298.15, chem_energy_molar=molar)
+        u = mix_state(lv, cfg, (0.05, 0.55, 0.4), 600.0, 1e5)
+        sim = OL.OracleSim(cfg, lv)
+        sim.set_state(u)
+        dt = 1e-5
+        out[molar] = (sim.ns_chem_apply(dt), sim.get_state(), u)
+    rhs_ref, st, u = out[False]
+    ca, cb = st[0, 4, 0] / ms[0], st[0, 5, 0] / ms[1]
+    w = A * ca * ca * cb
+    dh = 1000.0 * (600.0 - 298.15)
+    h = np.array(h0) + dh
+    wi = np.array([-2 * w, -w, 2 * w])
+    np.testing.assert_allclose(rhs_ref, (wi * h).sum(), rtol=1e-9)
+    np.testing.assert_allclose(out[True][0], (wi * np.array(ms) * h).sum(), rtol=1e-9)
+    # the species update is the same in both modes
+    assert np.array_equal(out[True][1], st)
+    # consumption: d C_A = 2 d C_B at first order, mass conserved
+    dca = (st[0, 4, 0] - u[0, 4, 0]) / ms[0]
+    dcb = (st[0, 5, 0] - u[0, 5, 0]) / ms[1]
+    assert abs(dca / dcb - 2.0) < 0.05
+    assert abs(st[0, 4:, 0].sum() - u[0, 4:, 0].sum()) < 1e-4 * u[0, 4:, 0].sum()
+
+
 def test_sutherland_viscosity_of_air_in_stress():
     """Sutherland's law for air (mu0 1.716e-5 Pa s at 273.15 K, S = 110.4 K)
     gives 1.846e-5 Pa s at 300 K (tabulated air viscosity)."""
