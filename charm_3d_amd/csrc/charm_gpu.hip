@@ -571,8 +571,14 @@ extern "C" int charm_gpu_recommended_chunks(const charm_config_t *cfg, int *ce, 
         const int groups = stage_groups_per_cu(cfg, bs);
         size_t budget = groups > 0 ? lds_cu / groups : 40960;
         if (budget > 65536) budget = 65536;
+        // whole KiB: LDS is allocated in blocks, and a share that rounds
+        // up past lds_cu / groups leaves room for one workgroup fewer
+        budget &= ~(size_t) 1023;
         faces = (int) (budget / per_face);
         if (faces > 1024) faces = 1024;
+        // at most three passes of the workgroup over the faces (a fourth,
+        // partial pass runs with most lanes idle)
+        if (faces > 3 * bs) faces = 3 * bs;
         // 128-thread workgroups serve chunks of at most 128 elements
         if (bs == 128 || faces / 3 > 128 || config_physics(cfg)) break;
     }

@@ -22,6 +22,7 @@
 #include <stdint.h>
 #include <type_traits>
 #include "charm_device.hpp"
+#include "charm_dev.hpp"
 #include "charm_kernels.h"
 
 namespace charm {
@@ -75,9 +76,11 @@ __device__ __forceinline__ void to_xyz(int a, R n, R t1, R t2, R &x, R &y, R &z)
 // (u1 unused on a boundary face):
 // out[3][NV] = wJ * (sum q, sum q phi_t1, sum q phi_t2) over the six points,
 // phi_t the tangential Taylor coordinates of the small (or side-0) face.
-template <typename R, int NC, int FLUX, int AX, bool BND, int PH>
+// DEV: u0, u1 hold deviations from the chunk reference *rf (coefficient 0
+// minus the reference) and q is F - F(U_r) (charm_dev.hpp).
+template <typename R, int NC, int FLUX, int AX, bool BND, int PH, bool DEV = false>
 __device__ __forceinline__ bool face_core(const DevParams &P, const R *u0, const R *u1, int info,
-                                          int lown, R *out, int bnd_axis)
+                                          int lown, R *out, int bnd_axis, const DevRef &rf)
 {
     constexpr int NV = 4 + NC;
     constexpr int T1 = (AX == 0) ? 1 : 0, T2 = (AX == 2) ? 1 : 2;
@@ -111,6 +114,27 @@ __device__ __forceinline__ bool face_core(const DevParams &P, const R *u0, const
         #pragma unroll
         for (int v = 0; v < NV; v++)
             t[v] = c0[v] + (o01 + m0 * g1) * u0[4*v+1+T1] + (o02 + m0 * g2) * u0[4*v+1+T2];
+        R q[NV], qg[NV];
+        if constexpr (DEV) {
+            DPrim dl, dr;
+            dev_prim(rf, t, dl);
+            if (BND) {
+                if (P.bnd_type[dir0] == BND_OUTLET) {
+                    dr = dl;                   // transmissive: the exterior state is the interior one
+                } else {
+                    Prim<float, 1> pl, pr;
+                    prim_of_dev(P, rf, dl, pl);
+                    bnd_state<float, 1, 0>(P, dir0, AX, s, pl, pr, bnd_axis);
+                    dev_from_prim(rf, pr, dr);
+                }
+            } else {
+                #pragma unroll
+                for (int v = 0; v < NV; v++)
+                    t[v] = c1[v] + (o11 + m1 * g1) * u1[4*v+1+T1] + (o12 + m1 * g2) * u1[4*v+1+T2];
+                dev_prim(rf, t, dr);
+            }
+            ok &= flux_dev<FLUX>(dev_frame<AX>(dl, s), dev_frame<AX>(dr, s), dev_ref_frame<AX>(rf, s), q);
+        } else {
         Prim<R, NC> pl, pr;
         cons_to_prim<R, NC, PH>(P, t, pl);
         if (BND) {
@@ -133,8 +157,8 @@ __device__ __forceinline__ bool face_core(const DevParams &P, const R *u0, const
             #pragma unroll
             for (int i = 0; i < NC; i++) { fl.c[i] = pl.c[i]; fr.c[i] = pr.c[i]; }
         }
-        R q[NV], qg[NV];
         ok &= flux_x<R, NC, FLUX>(fl, fr, q);
+        }
         qg[AX] = s * q[0]; qg[T1] = q[1]; qg[T2] = q[2];
         #pragma unroll
         for (int v = 3; v < NV; v++) qg[v] = q[v];
@@ -155,20 +179,22 @@ __device__ __forceinline__ bool face_core(const DevParams &P, const R *u0, const
 
 // interior face from the two preloaded element states; the small side's
 // level sets the face size
-template <typename R, int NC, int FLUX, int AX, int PH>
+template <typename R, int NC, int FLUX, int AX, int PH, bool DEV = false>
 __device__ __forceinline__ bool face_moments(const DevParams &P, const R *u0, const R *u1, int info,
-                                             int lev0, R *out)
+                                             int lev0, R *out, const DevRef &rf)
 {
     const int kind = (info >> 3) & 3;
-    return face_core<R, NC, FLUX, AX, false, PH>(P, u0, u1, info, kind == FACE_HANG1 ? lev0 + 1 : lev0, out, AX);
+    return face_core<R, NC, FLUX, AX, false, PH, DEV>(P, u0, u1, info, kind == FACE_HANG1 ? lev0 + 1 : lev0, out,
+                                                      AX, rf);
 }
 
 // Boundary faces: the element state is rotated into the frame (a, t1, t2)
 // of its face (Taylor slopes and momentum components permuted), evaluated
 // with the x-normal code, and the momentum moments are rotated back; one
 // code path serves the three axes.
-template <typename R, int NC, int FLUX, int PH>
-__device__ __forceinline__ bool face_moments_bnd(const DevParams &P, const R *u, int info, int lev, R *out)
+template <typename R, int NC, int FLUX, int PH, bool DEV = false>
+__device__ __forceinline__ bool face_moments_bnd(const DevParams &P, const R *u, int info, int lev, R *out,
+                                                 const DevRef &rf)
 {
     constexpr int NV = 4 + NC;
     const int ax = (info & 7) >> 1;
@@ -194,7 +220,13 @@ __device__ __forceinline__ bool face_moments_bnd(const DevParams &P, const R *u,
         w[4 * v] = c[0]; w[4 * v + 1] = ca; w[4 * v + 2] = ct1; w[4 * v + 3] = ct2;
     }
     R o[3 * NV];
-    const bool ok = face_core<R, NC, FLUX, 0, true, PH>(P, w, w, info, lev, o, ax);
+    bool ok;
+    if constexpr (DEV) {
+        // the chunk reference in the face's frame, like the state
+        ok = face_core<R, NC, FLUX, 0, true, PH, true>(P, w, w, info, lev, o, ax, permute_ref(rf, ax));
+    } else {
+        ok = face_core<R, NC, FLUX, 0, true, PH>(P, w, w, info, lev, o, ax, rf);
+    }
     #pragma unroll
     for (int m = 0; m < 3; m++) {
         const R f0 = o[m * NV + 0], f1 = o[m * NV + 1], f2 = o[m * NV + 2];
@@ -261,6 +293,53 @@ __device__ __forceinline__ void volume_integral(const DevParams &P, const R *u, 
     }
 }
 
+// the volume integral of F - F(U_r) from deviation coefficients u (one
+// species): the momentum, energy and mass fluxes along each axis d as in
+// dev_phys; the gravity term takes the full density
+__device__ __forceinline__ void volume_integral_dev(const DevParams &P, const DevRef &rf, const float *u, float h,
+                                                    float *res)
+{
+    const float ph = 0.28867513459481288225f;          // 1/(2 sqrt 3)
+    const float hinv = rcp(h);
+    const float wj = h * h * h * 0.125f;
+    float SF[3][5];
+    float G[4] = {0.f, 0.f, 0.f, 0.f};
+    #pragma unroll
+    for (int d = 0; d < 3; d++)
+        #pragma unroll
+        for (int v = 0; v < 5; v++) SF[d][v] = 0.f;
+    #pragma unroll
+    for (int g = 0; g < 8; g++) {
+        const float px = (g & 1) ? ph : -ph, py = (g & 2) ? ph : -ph, pz = (g & 4) ? ph : -ph;
+        float t[5];
+        trace<float, 5>(u, px, py, pz, t);
+        DPrim q;
+        dev_prim(rf, t, q);
+        #pragma unroll
+        for (int d = 0; d < 3; d++) {
+            #pragma unroll
+            for (int j = 0; j < 3; j++) SF[d][j] += q.dm[d] * q.u[j] + rf.m[d] * q.du[j] + (d == j ? q.dp : 0.f);
+            SF[d][3] += (q.dE + q.dp) * q.u[d] + rf.Epp * q.du[d];
+            SF[d][4] += q.dm[d];
+        }
+        G[0] += q.r; G[1] += q.r * px; G[2] += q.r * py; G[3] += q.r * pz;
+    }
+    const float f = wj * hinv;
+    #pragma unroll
+    for (int v = 0; v < 5; v++) {
+        res[4*v+0] = 0.f;
+        res[4*v+1] = -SF[0][v] * f;
+        res[4*v+2] = -SF[1][v] * f;
+        res[4*v+3] = -SF[2][v] * f;
+    }
+    #pragma unroll
+    for (int v = 0; v < 3; v++) {
+        const float gv = P.grav_f[v];
+        #pragma unroll
+        for (int k = 0; k < 4; k++) res[4*v+k] -= gv * G[k] * wj;
+    }
+}
+
 // S: storage type of the state; R: arithmetic type of the residual (S = R,
 // or the mixed mode S = double, R = float: FP32 fluxes and integrals, FP64
 // state, update and combine).  PH = 0: constant-cp Euler terms; PH = 1: any
@@ -290,27 +369,37 @@ void k_stage(StageArgs A, DevParams Pv)
     const int fb = A.ch_fbeg[c], fe = A.ch_fbeg[c + 1];
     const int eb = A.ch_ebeg[c], ee = A.ch_ebeg[c + 1];
     const int64_t n = A.n_elem;
+    // FP32, one species: deviation form against the chunk's reference (the
+    // cell mean of its first leaf), the same for every face and leaf of the
+    // chunk
+    constexpr bool DEV = uses_dev<R, NC, FLUX, PH>();
+    DevRef rf{};
+    if constexpr (DEV)
+        rf = make_ref(P, (double) U[eb], (double) (U + 4 * n)[eb], (double) (U + 8 * n)[eb],
+                      (double) (U + 12 * n)[eb], (double) (U + 16 * n)[eb]);
     bool ok = true;
     for (int f = fb + (int) threadIdx.x; f < fe; f += BS) {
         const int4 rec = A.cf4[f];
         R *dst = mom + (size_t) (f - fb) * 3 * NV;
         R ua[4 * NV];
-        load_conv<S, R, NV>(U, n, rec.x, ua);
+        if constexpr (DEV) load_dev<S>(U, n, rec.x, rf, ua);
+        else load_conv<S, R, NV>(U, n, rec.x, ua);
         const int lev0 = A.level[rec.x];
         // faces of a chunk are sorted by (axis, kind): waves are mostly
         // uniform in the branch taken here
         const int ax = (rec.z & 7) >> 1;
         if (((rec.z >> 3) & 3) == FACE_BND) {
-            ok &= face_moments_bnd<R, NC, FLUX, PH>(P, ua, rec.z, lev0, dst);
+            ok &= face_moments_bnd<R, NC, FLUX, PH, DEV>(P, ua, rec.z, lev0, dst, rf);
         } else {
             R ub[4 * NV];
-            load_conv<S, R, NV>(U, n, rec.y, ub);
+            if constexpr (DEV) load_dev<S>(U, n, rec.y, rf, ub);
+            else load_conv<S, R, NV>(U, n, rec.y, ub);
             if (ax == 0)
-                ok &= face_moments<R, NC, FLUX, 0, PH>(P, ua, ub, rec.z, lev0, dst);
+                ok &= face_moments<R, NC, FLUX, 0, PH, DEV>(P, ua, ub, rec.z, lev0, dst, rf);
             else if (ax == 1)
-                ok &= face_moments<R, NC, FLUX, 1, PH>(P, ua, ub, rec.z, lev0, dst);
+                ok &= face_moments<R, NC, FLUX, 1, PH, DEV>(P, ua, ub, rec.z, lev0, dst, rf);
             else
-                ok &= face_moments<R, NC, FLUX, 2, PH>(P, ua, ub, rec.z, lev0, dst);
+                ok &= face_moments<R, NC, FLUX, 2, PH, DEV>(P, ua, ub, rec.z, lev0, dst, rf);
         }
         finfo[f - fb] = rec.z;
     }
@@ -318,10 +407,15 @@ void k_stage(StageArgs A, DevParams Pv)
     const int e = eb + (int) threadIdx.x;
     if (e < ee) {
         R u[4 * NV], res[4 * NV];
-        load_conv<S, R, NV>(U, n, e, u);
         const int lev = A.level[e];
         const R h = ldexp(R(1), -lev);
-        volume_integral<R, NC, PH>(P, u, h, res);
+        if constexpr (DEV) {
+            load_dev<S>(U, n, e, rf, u);
+            volume_integral_dev(P, rf, u, h, res);
+        } else {
+            load_conv<S, R, NV>(U, n, e, u);
+            volume_integral<R, NC, PH>(P, u, h, res);
+        }
         if constexpr (PH == 1) {
             if (A.visc) {
                 const S *__restrict__ VI = reinterpret_cast<const S *>(A.visc);
@@ -373,7 +467,8 @@ void k_stage(StageArgs A, DevParams Pv)
                 #pragma unroll
                 for (int k = 0; k < 4; k++) {
                     const S ai = cst<S>(P.ainv0[k], P.ainv0_f[k]) * sc;
-                    const S uk = std::is_same<S, R>::value ? (S) u[4*v+k] : (U + (4*v+k) * n)[e];
+                    // the stored coefficient (the registers hold deviations in DEV)
+                    const S uk = (std::is_same<S, R>::value && !DEV) ? (S) u[4*v+k] : (U + (4*v+k) * n)[e];
                     const S w = uk - dt * (ai * (S) res[4*v+k]);
                     (out + (4*v+k) * n)[e] = A.combine ? rk_combine<S>(A.combine, w, (U0 + (4*v+k) * n)[e]) : w;
                 }
