@@ -670,6 +670,36 @@ extern "C" int charm_gpu_conservation_sums(charm_gpu_ctx_t *c, double *out)
     return CHARM_OK;
 }
 
+// charm_gpu_conservation_sums for callers that cannot allocate: the same two
+// kernels, the block partials in the library's static per-GPU array instead
+// of the context's buffer, the V sums written to device memory out_dev, the
+// work only enqueued on the context's stream (no allocation, memset, copy or
+// synchronisation).  Every block partial the final pass reads is written by
+// the first kernel of the same call, so nothing needs clearing; calls that
+// overlap on one GPU race on the shared array.
+extern "C" int charm_gpu_conservation_sums_noalloc(charm_gpu_ctx_t *c, double *out_dev)
+{
+    int rc = need_mesh(c, -1);
+    if (rc) return rc;
+    if (!out_dev) { charm_set_error("conservation_sums_noalloc: null output"); return CHARM_ERR_ARG; }
+    void *scr = NULL;
+    size_t bytes = 0;
+    HIPCHK(noalloc_scratch(&scr, &bytes));
+    const int64_t nblk = (c->n + 255) / 256;
+    const size_t need = (size_t) nblk * (size_t) (c->nv + 1) * sizeof(double);
+    if (need > bytes) {
+        charm_set_error("conservation_sums_noalloc: %lld leaves need %zu bytes of block partials, the static "
+                        "array holds %zu (at most %lld leaves)", (long long) c->n, need, bytes,
+                        (long long) (bytes / ((size_t) (c->nv + 1) * sizeof(double)) * 256));
+        return CHARM_ERR_ARG;
+    }
+    RedArgs A;
+    A.U = c->B[0]; A.level = c->d_level; A.partial = (double *) scr; A.result = out_dev;
+    A.n = c->n; A.mode = 1;
+    HIPCHK(launch_reduce_n(c->sprec, c->nc, c->ph, A, c->P, c->st, c->nv));
+    return CHARM_OK;
+}
+
 extern "C" int charm_gpu_state_flag(charm_gpu_ctx_t *c, int *flag)
 {
     if (!c || !flag) { charm_set_error("state_flag: null argument"); return CHARM_ERR_ARG; }

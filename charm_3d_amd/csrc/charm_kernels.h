@@ -163,6 +163,12 @@ size_t stage_smem_bytes(int prec, int nc, int max_faces);
 const void *stage_kernel(int prec, int nc, int flux, int ph, int bs);
 hipError_t launch_limiter(int prec, int nc, int ph, const LimArgs &A, const DevParams &P, hipStream_t st);
 hipError_t launch_reduce(int prec, int nc, int ph, const RedArgs &A, const DevParams &P, hipStream_t st);
+// as launch_reduce, writing the first nout of the V sums and the CFL minimum
+hipError_t launch_reduce_n(int prec, int nc, int ph, const RedArgs &A, const DevParams &P, hipStream_t st,
+                           int nout);
+// device address and size of the static block-partial array of the
+// allocation-free conservation sums, for the current device (no allocation)
+hipError_t noalloc_scratch(void **ptr, size_t *bytes);
 hipError_t launch_flux_points(int prec, int nc, int flux, const FluxArgs &A, const DevParams &P, hipStream_t st);
 // FP64 pointwise flux VJP (LF, HLLC, CD; 1-2 species) on dual numbers
 hipError_t launch_flux_points_vjp(int nc, int flux, const FluxArgs &A, const DevParams &P, hipStream_t st);

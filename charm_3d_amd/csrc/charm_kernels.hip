@@ -4,6 +4,7 @@
 //
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <mutex>
 #include "charm_device.hpp"
 #include "charm_kernels.h"
 #include "charm_stage.hpp"
@@ -166,9 +167,10 @@ void k_reduce(RedArgs A, DevParams P)
     }
 }
 
-// final pass of k_reduce over the block partials (one block)
+// final pass of k_reduce over the block partials (one block); writes the
+// first nout of the nv sums and the CFL minimum
 __global__ __launch_bounds__(256)
-void k_reduce_final(double *partial, int nblk, int nv, double *out)
+void k_reduce_final(const double *partial, int nblk, int nv, double *out, int nout)
 {
     __shared__ double sh[256];
     for (int v = 0; v <= nv; v++) {
@@ -186,7 +188,7 @@ void k_reduce_final(double *partial, int nblk, int nv, double *out)
                                        : sh[threadIdx.x] + sh[threadIdx.x + s];
             __syncthreads();
         }
-        if (threadIdx.x == 0) out[v] = sh[0];
+        if (threadIdx.x == 0 && v < nout) out[v] = sh[0];
         __syncthreads();
     }
 }
@@ -304,7 +306,7 @@ hipError_t launch_limiter(int prec, int nc, int ph, const LimArgs &A, const DevP
 }
 
 template <typename R, int PH>
-static hipError_t launch_reduce_r(int nc, const RedArgs &A, const DevParams &P, hipStream_t st)
+static hipError_t launch_reduce_r(int nc, const RedArgs &A, const DevParams &P, hipStream_t st, int nout)
 {
     const dim3 grid((unsigned) ((A.n + 255) / 256)), block(256);
     switch (nc) {
@@ -314,14 +316,52 @@ static hipError_t launch_reduce_r(int nc, const RedArgs &A, const DevParams &P, 
     case 4: hipLaunchKernelGGL((k_reduce<R, 4, PH>), grid, block, 0, st, A, P); break;
     default: return hipErrorInvalidValue;
     }
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, st, A.partial, (int) grid.x, 4 + nc, A.result);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, st, A.partial, (int) grid.x, 4 + nc, A.result, nout);
     return hipGetLastError();
+}
+
+hipError_t launch_reduce_n(int prec, int nc, int ph, const RedArgs &A, const DevParams &P, hipStream_t st,
+                           int nout)
+{
+    if (ph) return prec ? launch_reduce_r<float, 1>(nc, A, P, st, nout) : launch_reduce_r<double, 1>(nc, A, P, st, nout);
+    return prec ? launch_reduce_r<float, 0>(nc, A, P, st, nout) : launch_reduce_r<double, 0>(nc, A, P, st, nout);
 }
 
 hipError_t launch_reduce(int prec, int nc, int ph, const RedArgs &A, const DevParams &P, hipStream_t st)
 {
-    if (ph) return prec ? launch_reduce_r<float, 1>(nc, A, P, st) : launch_reduce_r<double, 1>(nc, A, P, st);
-    return prec ? launch_reduce_r<float, 0>(nc, A, P, st) : launch_reduce_r<double, 0>(nc, A, P, st);
+    return launch_reduce_n(prec, nc, ph, A, P, st, 4 + nc + 1);
+}
+
+// Block partials of charm_gpu_conservation_sums_noalloc: a static device
+// array, one per GPU the library runs on, held until the process ends
+// (kNoAllocBlocks blocks of 256 leaves, V + 1 <= 9 doubles each: 1.2 MB, up
+// to 4,194,304 leaves).  Host code never passes the array's name to a
+// kernel: that is a host-side stand-in, not a device address.  The device
+// address comes from hipGetSymbolAddress, once per device, with that device
+// current, and is kept per device.
+constexpr int kNoAllocBlocks = 16384;
+static __device__ __align__(256) double scratch[(size_t) kNoAllocBlocks * (4 + 4 + 1)];
+
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[64];
+
+hipError_t noalloc_scratch(void **ptr, size_t *bytes)
+{
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch_ptr[dev] = p;
+    }
+    *ptr = g_scratch_ptr[dev];
+    *bytes = sizeof(scratch);
+    return hipSuccess;
 }
 
 template <typename R, int NC>

@@ -213,6 +213,17 @@ int charm_gpu_time_copy(charm_gpu_ctx_t *ctx, int reps, double *copy_ms, double 
 int charm_gpu_get_dt(charm_gpu_ctx_t *ctx, int mode, double *dt);
 /* FP64 sums_e V_e U[e][v][0] */
 int charm_gpu_conservation_sums(charm_gpu_ctx_t *ctx, double *out);
+/* Allocation-free variant of charm_gpu_conservation_sums (no counterpart in
+ * the reference; for callers that cannot allocate: code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  The V FP64 sums
+ * of the current state go to out_dev (device memory, V doubles); the block
+ * partials live in a static array of the library, one per GPU (1.2 MB, held
+ * until the process ends), which limits the mesh to 4,194,304 leaves (larger:
+ * CHARM_ERR_ARG).  The work is only enqueued on the context's stream: no
+ * allocation, memset, copy or synchronisation; wait on the stream (e.g.
+ * charm_gpu_synchronize) before reading out_dev.  Calls that overlap on one
+ * GPU race on the shared array. */
+int charm_gpu_conservation_sums_noalloc(charm_gpu_ctx_t *ctx, double *out_dev);
 /* 1 if a NaN / non-positive pressure was met since the last call */
 int charm_gpu_state_flag(charm_gpu_ctx_t *ctx, int *flag);
 int charm_gpu_synchronize(charm_gpu_ctx_t *ctx);
