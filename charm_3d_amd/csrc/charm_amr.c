@@ -131,6 +131,20 @@ void charm_adapt(charm_forest_t *p4est, charm_ghost_t *ghost, charm_data_t *ghos
     if (rc) c->err = rc;
 }
 
+/* src/charm_amr.c:309-347: the reference refines recursively by
+ * charm_refine_init_err_estimate, which reads the gradient par.a.grad_u that
+ * nothing has computed before the first charm_adapt (:50-83 against
+ * :581-599), so its initial mesh is undefined.  Here the call records
+ * CHARM_ERR_UNSUPPORTED with that reason; an initial mesh is built by
+ * charm_adapt passes on a projected initial state. */
+void charm_adapt_init(charm_forest_t *p4est)
+{
+    charm_ctx_t *c = p4est ? charm_ctx_of(p4est) : NULL;
+    charm_set_error("adapt_init: the reference's initial refinement reads an estimator gradient it never "
+                    "computes; refine with charm_adapt on a projected initial state");
+    if (c) c->err = CHARM_ERR_UNSUPPORTED;
+}
+
 int charm_adapt_weighted(charm_forest_t *p4est, const double *weight, int64_t n)
 {
     charm_ctx_t *c = charm_ctx_of(p4est);

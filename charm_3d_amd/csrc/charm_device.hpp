@@ -13,7 +13,10 @@ constexpr int kMaxComp = 8;
 constexpr double kGasR = 8.31446261815324;       // src/charm_eos.c:8
 constexpr double kEps = 1.e-12;                   // CHARM_EPS
 enum { FLUX_LF = 0, FLUX_HLLC = 1, FLUX_GODUNOV = 2, FLUX_CD = 3 };
-enum { BND_INLET = 0, BND_OUTLET = 1, BND_WALL_SLIP = 2, BND_WALL_NO_SLIP = 3 };
+enum { BND_INLET = 0, BND_OUTLET = 1, BND_WALL_SLIP = 2, BND_WALL_NO_SLIP = 3, BND_MASS_FLOW = 4 };
+// boundary parameters per face: inlet (u, v, w, T, p); no-slip wall (T); mass
+// flow (flux, static p, total T, 3 direction cosines, outflow p, composition)
+constexpr int kBndParams = 7 + 4;
 enum { EOS_IDEAL = 0, EOS_MIX = 1 };
 enum { FACE_CONF = 0, FACE_HANG0 = 1, FACE_HANG1 = 2, FACE_BND = 3 };
 
@@ -27,14 +30,14 @@ struct DevParams {
     double comp_minv[kMaxComp];        //          1 / M_i (FP64, for the FP32 path's closure)
     double comp_cp[kMaxComp];          //          constant cp per species
     int    bnd_type[6];
-    double bnd_param[6][5];
+    double bnd_param[6][kBndParams];
     double grav[3];
     double cfl;
     double ainv0[4];                   // diagonal of A^-1 at level 0 (FP64 Gauss-Jordan)
     float  ainv0_f[4];
     float  gam_f, cv_f, m_f;
     float  comp_m_f[kMaxComp], comp_cp_f[kMaxComp];
-    float  bnd_param_f[6][5];
+    float  bnd_param_f[6][kBndParams];
     float  grav_f[3];
     // Navier-Stokes / real-gas extensions, read by the PH = 1 kernels only
     int    newton_t;                   // comp 0 has a polynomial cp: T by FP64 Newton
@@ -299,6 +302,51 @@ __device__ __forceinline__ void bnd_state(const DevParams &P, int face, int axis
         vel[1] = sel(axis == 1, va, vel[1]);
         vel[2] = sel(axis == 2, va, vel[2]);
         t = cst<R>(P.bnd_param[face][0], P.bnd_param_f[face][0]);
+    } else if (type == BND_MASS_FLOW) {
+        // src/charm_bnd_cond.c:101-146: param = (mass flux, static p, total T,
+        // direction cosines, outflow p, composition).  Outflow (the direction
+        // leaves through the face): the pressure param[6] where the normal
+        // velocity is subsonic.  The reference also writes the interior
+        // pressure of a supersonic face back into param[6], which later faces
+        // read; that depends on the face-visit order, so param[6] stays fixed
+        // here.  Inflow: the composition of the parameters, EOS flag 0 at the
+        // interior T, the static pressure if supersonic, T from
+        // T + (m (cp - cv) T / (p f))^2 / (2 cp) = T_tot, speed m / (rho f).
+        R cs[3];
+        #pragma unroll
+        for (int i = 0; i < 3; i++) cs[i] = cst<R>(P.bnd_param[face][3 + i], P.bnd_param_f[face][3 + i]);
+        R cf0 = cs[0], cf1 = cs[1], cf2 = cs[2];          // in the state's component order
+        if (frame_axis >= 0) {
+            const int a = frame_axis;
+            cf0 = sel(a == 0, cs[0], sel(a == 1, cs[1], cs[2]));
+            cf1 = sel(a == 0, cs[1], cs[0]);
+            cf2 = sel(a == 2, cs[1], cs[2]);
+        }
+        const R fd = s * sel(axis == 0, cf0, sel(axis == 1, cf1, cf2));      // n . direction
+        const R vn = s * sel(axis == 0, vel[0], sel(axis == 1, vel[1], vel[2]));
+        const R mflow = cst<R>(P.bnd_param[face][0], P.bnd_param_f[face][0]);
+        if (fd > R(0)) {
+            if (vn < in.cz) pr = cst<R>(P.bnd_param[face][6], P.bnd_param_f[face][6]);
+        } else {
+            Prim<R, NC> q = in;
+            #pragma unroll
+            for (int i = 0; i < NC; i++) q.c[i] = cst<R>(P.bnd_param[face][7 + i], P.bnd_param_f[face][7 + i]);
+            if constexpr (PH == 1) eos_coeffs_at<R, NC>(P, q, (double) t);
+            else eos_coeffs<R, NC>(P, q);
+            const R r = in.r < R(kEps) ? R(kEps) : in.r;
+            const R p0 = r * in.e * (q.gam - R(1));
+            const R v2 = vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2];
+            pr = v2 > dv(q.gam * p0, r) ? cst<R>(P.bnd_param[face][1], P.bnd_param_f[face][1]) : p0;
+            const R cv = q.cv, cp = q.gam * cv;
+            const R x = dv(mflow * (cp - cv), pr * fd);
+            const R k = dv(x * x, R(2) * cp);
+            const R ttot = cst<R>(P.bnd_param[face][2], P.bnd_param_f[face][2]);
+            t = dv(R(-1) + dsqrt(R(1) + R(4) * k * ttot), R(2) * k);
+            const R vm = fabs(dv(mflow, r * fd));
+            vel[0] = cf0 * vm; vel[1] = cf1 * vm; vel[2] = cf2 * vm;
+            #pragma unroll
+            for (int i = 0; i < NC; i++) out.c[i] = q.c[i];
+        }
     }
     out.u = vel[0]; out.v = vel[1]; out.w = vel[2];
     if constexpr (PH == 1) eos_coeffs_at<R, NC>(P, out, (double) t);   // flag 3: T given

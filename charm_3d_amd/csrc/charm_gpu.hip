@@ -110,8 +110,7 @@ extern "C" int charm_gpu_create(const charm_config_t *cfg, charm_gpu_ctx_t **out
         }
     for (int i = 0; i < 6; i++) {
         int t = cfg->bnd_type[i];
-        if (t == CHARM_BND_MASS_FLOW) { charm_set_error("gpu_create: BOUND_MASS_FLOW not built"); return CHARM_ERR_UNSUPPORTED; }
-        if (t != CHARM_BND_PERIODIC && (t < 0 || t > 3)) { charm_set_error("gpu_create: bad boundary type %d", t); return CHARM_ERR_ARG; }
+        if (t != CHARM_BND_PERIODIC && (t < 0 || t > 4)) { charm_set_error("gpu_create: bad boundary type %d", t); return CHARM_ERR_ARG; }
     }
     int ndev = 0;
     HIPCHK(hipGetDeviceCount(&ndev));
@@ -143,7 +142,7 @@ extern "C" int charm_gpu_create(const charm_config_t *cfg, charm_gpu_ctx_t **out
     }
     for (int f = 0; f < 6; f++) {
         P.bnd_type[f] = cfg->bnd_type[f];
-        for (int j = 0; j < 5; j++) P.bnd_param[f][j] = cfg->bnd_param[f][j];
+        for (int j = 0; j < kBndParams; j++) P.bnd_param[f][j] = cfg->bnd_param[f][j];
     }
     for (int d = 0; d < 3; d++) P.grav[d] = cfg->grav[d];
     P.cfl = cfg->cfl;
@@ -195,7 +194,7 @@ extern "C" int charm_gpu_create(const charm_config_t *cfg, charm_gpu_ctx_t **out
     P.gam_f = (float) P.gam; P.cv_f = (float) P.cv; P.m_f = (float) P.m;
     for (int i = 0; i < kMaxComp; i++) { P.comp_m_f[i] = (float) P.comp_m[i]; P.comp_cp_f[i] = (float) P.comp_cp[i]; }
     for (int f = 0; f < 6; f++)
-        for (int j = 0; j < 5; j++) P.bnd_param_f[f][j] = (float) P.bnd_param[f][j];
+        for (int j = 0; j < kBndParams; j++) P.bnd_param_f[f][j] = (float) P.bnd_param[f][j];
     for (int d = 0; d < 3; d++) P.grav_f[d] = (float) P.grav[d];
     hipError_t e;
     if ((e = hipStreamCreate(&c->st)) != hipSuccess || (e = hipEventCreate(&c->ev0)) != hipSuccess ||
@@ -932,10 +931,23 @@ extern "C" int charm_gpu_ns_chem(charm_gpu_ctx_t *c, double dt, double *chem_rhs
 // S^T_U(y) = y + (d int / d U)^T (-dt A^-1 y).  The inlet parameters
 // (velocity x, T, p of face theta_face) accumulate their gradient per leaf.
 
+// the boundary types the adjoint's boundary linearisation covers (all but
+// the mass-flow condition)
+static int adj_bnd_supported(const charm_gpu_ctx_t *c)
+{
+    for (int f = 0; f < 6; f++)
+        if (c->cfg.bnd_type[f] == CHARM_BND_MASS_FLOW) {
+            charm_set_error("adjoint: the mass-flow boundary has no adjoint here");
+            return CHARM_ERR_UNSUPPORTED;
+        }
+    return CHARM_OK;
+}
+
 static int adj_check(charm_gpu_ctx_t *c, int nsteps, int ckpt, int theta_face)
 {
     int rc = need_mesh(c, -1);
     if (rc) return rc;
+    if ((rc = adj_bnd_supported(c))) return rc;
     if (c->cfg.model != CHARM_MODEL_NS || c->cfg.limiter || c->ph || c->nc > 2 || c->prec == CHARM_FP32_MIX ||
         c->cfg.flux == CHARM_FLUX_GODUNOV) {
         charm_set_error("adjoint: needs the SSP-RK3 model without limiter, constant cp, 1-2 species and "
@@ -1152,6 +1164,7 @@ extern "C" int charm_gpu_stage_vjp(charm_gpu_ctx_t *c, double dt, const double *
         charm_set_error("stage_vjp: needs no limiter, constant cp, 1-2 species and the LF, HLLC or CD flux");
         return CHARM_ERR_UNSUPPORTED;
     }
+    if ((rc = adj_bnd_supported(c))) return rc;
     const size_t ssz = (size_t) n * 4 * c->nv * c->esz, dsz = (size_t) n * 4 * c->nv * sizeof(double);
     AdjBoundary bnd;
     if ((rc = adj_boundary(c, bnd))) return rc;

@@ -159,6 +159,35 @@ void charm_mat_eos_mix(charm_forest_t *p4est, charm_prim_t *p, int flag)
     if (cfg && p) charm_eos_mix_cfg(cfg, p, flag);
 }
 
+/* src/charm_eos.c:150-154: the reference's table EOS stops the program with
+ * "TABLE EOS IS NOT RELEASED"; here the error is recorded on the forest's
+ * context (or as the last error) and p is left as it is */
+void charm_mat_eos_table(charm_forest_t *p4est, charm_prim_t *p, int flag)
+{
+    charm_ctx_t *c = p4est ? charm_ctx_of(p4est) : NULL;
+    (void) p; (void) flag;
+    charm_set_error("eos: TABLE EOS IS NOT RELEASED");
+    if (c) c->err = CHARM_ERR_UNSUPPORTED;
+}
+
+/* src/charm_eos.c:10 */
+charm_real_t charm_eos_get_r(void)
+{
+    return GAS_R;
+}
+
+/* src/charm_bnd_cond.c:9-19 over the names of src/charm_globals.c:13-19:
+ * the type whose name matches, CHARM_BND_UNKNOWN otherwise */
+int charm_bnd_type_by_name(const char *name)
+{
+    static const char *const names[] = {"BOUND_INLET", "BOUND_OUTLET", "BOUND_WALL_SLIP",
+                                        "BOUND_WALL_NO_SLIP", "BOUND_MASS_FLOW"};
+    int i;
+    for (i = 0; name && i < (int) (sizeof(names) / sizeof(names[0])); i++)
+        if (!strcmp(names[i], name)) return i;
+    return CHARM_BND_UNKNOWN;
+}
+
 /* src/charm_globals.c:226-240 */
 void charm_prim_cpy(charm_prim_t *dest, charm_prim_t *src)
 {
@@ -217,15 +246,47 @@ void charm_bnd_cond_fn_wall_no_slip(charm_prim_t *par_in, charm_prim_t *par_out,
     par_out->t = param[0];
 }
 
-/* src/charm_bnd_cond.c:101-146: the mass-flow condition keeps state between
- * calls through the reference's globals; it is not supported here (the
- * state passes through and the error is recorded) */
+/* the configuration of the boundary being evaluated: set by charm_bnd_cond
+ * before it calls a BC function (the reference's glob_mat and
+ * charm_get_p4est() globals, src/charm_bnd_cond.c:7,28) */
+static __thread const charm_config_t *g_bnd_cfg;
+
+/* src/charm_bnd_cond.c:101-146.  param = (mass flux, static p, total T,
+ * direction cosines, outflow p, composition).  As in the reference, a
+ * supersonic outflow face stores its pressure in param[6], which later
+ * subsonic faces read (charm_bnd_cond passes the context's own parameters);
+ * the device kernels keep param[6] fixed instead. */
 void charm_bnd_cond_fn_mass_flow(charm_prim_t *par_in, charm_prim_t *par_out, int8_t face,
                                  charm_real_t *param, charm_real_t n[3])
 {
-    (void) face; (void) param; (void) n;
+    const charm_config_t *cfg = g_bnd_cfg;
+    double cs[3], fd, vel, v2, k;
+    int ic;
+    (void) face;
     charm_prim_cpy(par_out, par_in);
-    charm_set_error("bnd_cond_fn_mass_flow: BOUND_MASS_FLOW is not supported");
+    if (!cfg) {
+        charm_set_error("bnd_cond_fn_mass_flow: no configuration (call through charm_bnd_cond)");
+        return;
+    }
+    cs[0] = param[3]; cs[1] = param[4]; cs[2] = param[5];
+    fd = n[0] * cs[0] + n[1] * cs[1] + n[2] * cs[2];
+    if (fd > 0.) {
+        vel = par_out->u * n[0] + par_out->v * n[1] + par_out->w * n[2];
+        if (vel < par_out->cz) par_out->p = param[6];
+        else param[6] = par_out->p;
+    } else {
+        for (ic = 0; ic < cfg->n_comp; ic++) par_out->c[ic] = param[7 + ic];
+        if (cfg->eos == CHARM_EOS_IDEAL) charm_eos_ideal_cfg(cfg, par_out, 0);
+        else charm_eos_mix_cfg(cfg, par_out, 0);
+        v2 = par_out->u * par_out->u + par_out->v * par_out->v + par_out->w * par_out->w;
+        if (v2 > par_out->cz * par_out->cz) par_out->p = param[1];
+        k = pow(param[0] * (par_out->cp - par_out->cv) / (par_out->p * fd), 2.) / (2. * par_out->cp);
+        par_out->t = (-1 + sqrt(1.0 + 4.0 * k * param[2])) / (2. * k);
+        vel = fabs(param[0] / (par_out->r * fd));
+        par_out->u = cs[0] * vel;
+        par_out->v = cs[1] * vel;
+        par_out->w = cs[2] * vel;
+    }
 }
 
 /* src/charm_bnd_cond.c:21-33: the BC function of the face's boundary, then
@@ -235,7 +296,7 @@ void charm_bnd_cond(charm_forest_t *p4est, int treeid, int8_t face, charm_prim_t
 {
     charm_ctx_t *c = charm_ctx_of(p4est);
     const charm_config_t *cfg;
-    double param[16];
+    double *param;
     (void) treeid;
     if (!c) return;
     cfg = &c->cfg;
@@ -244,17 +305,23 @@ void charm_bnd_cond(charm_forest_t *p4est, int treeid, int8_t face, charm_prim_t
         c->err = CHARM_ERR_ARG;
         return;
     }
-    memcpy(param, cfg->bnd_param[(int) face], sizeof(param));
+    /* the boundary's own parameters (bnd->params, which the mass-flow
+     * condition updates) */
+    param = c->cfg.bnd_param[(int) face];
+    g_bnd_cfg = cfg;
     switch (cfg->bnd_type[(int) face]) {
     case CHARM_BND_INLET: charm_bnd_cond_fn_inlet(par_in, par_out, face, param, n); break;
     case CHARM_BND_OUTLET: charm_bnd_cond_fn_outlet(par_in, par_out, face, param, n); break;
     case CHARM_BND_WALL_SLIP: charm_bnd_cond_fn_wall_slip(par_in, par_out, face, param, n); break;
     case CHARM_BND_WALL_NO_SLIP: charm_bnd_cond_fn_wall_no_slip(par_in, par_out, face, param, n); break;
+    case CHARM_BND_MASS_FLOW: charm_bnd_cond_fn_mass_flow(par_in, par_out, face, param, n); break;
     default:
-        charm_bnd_cond_fn_mass_flow(par_in, par_out, face, param, n);
-        c->err = CHARM_ERR_UNSUPPORTED;
+        g_bnd_cfg = NULL;
+        charm_set_error("bnd_cond: unknown boundary type %d", cfg->bnd_type[(int) face]);
+        c->err = CHARM_ERR_ARG;
         return;
     }
+    g_bnd_cfg = NULL;
     par_out->mat_id = par_in->mat_id;
     if (cfg->eos == CHARM_EOS_IDEAL) charm_eos_ideal_cfg(cfg, par_out, 3);
     else charm_eos_mix_cfg(cfg, par_out, 3);

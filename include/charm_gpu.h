@@ -52,7 +52,8 @@ enum { CHARM_FP64 = 0, CHARM_FP32 = 1, CHARM_FP32_MIX = 2 };
 enum { CHARM_FLUX_LF = 0, CHARM_FLUX_HLLC = 1, CHARM_FLUX_GODUNOV = 2, CHARM_FLUX_CD = 3 };
 /* boundary types (include/charm_globals.h:432-439) plus periodic links */
 enum { CHARM_BND_INLET = 0, CHARM_BND_OUTLET = 1, CHARM_BND_WALL_SLIP = 2,
-       CHARM_BND_WALL_NO_SLIP = 3, CHARM_BND_MASS_FLOW = 4, CHARM_BND_PERIODIC = 100 };
+       CHARM_BND_WALL_NO_SLIP = 3, CHARM_BND_MASS_FLOW = 4, CHARM_BND_UNKNOWN = 5,
+       CHARM_BND_PERIODIC = 100 };
 /* EOS (src/charm_eos.c:98-154) */
 enum { CHARM_EOS_IDEAL = 0, CHARM_EOS_MIX = 1 };
 /* MODEL name (src/charm_init_yaml.cpp:394-413) */
@@ -419,6 +420,12 @@ void charm_limiter(charm_forest_t *p4est, charm_ghost_t *ghost, charm_data_t *gh
  * (include/charm_amr.h:9-10, src/charm_amr.c:575-610).  The time-step entry
  * points call it when REFINE_PERIOD divides a nonzero step number. */
 void charm_adapt(charm_forest_t *p4est, charm_ghost_t *ghost, charm_data_t *ghost_data);
+/* the reference's initial refinement (include/charm_amr.h:9,
+ * src/charm_amr.c:309-347) reads an estimator gradient that is never
+ * computed before it (:50-83 against :581-599); this records
+ * CHARM_ERR_UNSUPPORTED with that reason.  Build the initial mesh with
+ * charm_adapt passes on a projected initial state. */
+void charm_adapt_init(charm_forest_t *p4est);
 void charm_limiter_bj(charm_forest_t *p4est, charm_ghost_t *ghost, charm_data_t *ghost_data);
 /* Pointwise host functions with the reference's names and arguments, over
  * the forest's configuration, FP64 (the device path does not call them; its
@@ -429,6 +436,16 @@ void charm_limiter_bj(charm_forest_t *p4est, charm_ghost_t *ghost, charm_data_t 
  *  Newton solve for flags 1 and 4 (include/charm_eos.h:14-18) */
 void charm_mat_eos_ideal(charm_forest_t *p4est, charm_prim_t *p, int flag);
 void charm_mat_eos_mix(charm_forest_t *p4est, charm_prim_t *p, int flag);
+/* the table EOS (include/charm_eos.h:18, src/charm_eos.c:150-154), which the
+ * reference leaves unreleased and aborts on: records CHARM_ERR_UNSUPPORTED
+ * ("TABLE EOS IS NOT RELEASED") and leaves p unchanged */
+void charm_mat_eos_table(charm_forest_t *p4est, charm_prim_t *p, int flag);
+/* the gas constant 8.31446261815324 (include/charm_eos.h:20, src/charm_eos.c:10) */
+charm_real_t charm_eos_get_r(void);
+/* BOUND_INLET .. BOUND_MASS_FLOW by name -> CHARM_BND_INLET .. _MASS_FLOW,
+ * anything else CHARM_BND_UNKNOWN (include/charm_bnd_cond.h:33,
+ * src/charm_bnd_cond.c:9-19) */
+int charm_bnd_type_by_name(const char *name);
 /* boundary-condition functions, bnd_fn (include/charm_globals.h:237;
  * include/charm_bnd_cond.h:14-33, src/charm_bnd_cond.c:36-146); the
  * mass-flow one records CHARM_ERR_UNSUPPORTED's message and passes the
@@ -441,6 +458,12 @@ void charm_bnd_cond_fn_wall_slip(charm_prim_t *par_in, charm_prim_t *par_out, in
                                  charm_real_t n[3]);
 void charm_bnd_cond_fn_wall_no_slip(charm_prim_t *par_in, charm_prim_t *par_out, int8_t face,
                                     charm_real_t *param, charm_real_t n[3]);
+/* mass flow, param = (flux, static p, total T, direction cosines, outflow
+ * p, composition): as the reference, a supersonic outflow stores its
+ * pressure in param[6] for later calls (the device kernels keep param[6]
+ * fixed, since that update depends on the face order); it reads the
+ * material of the boundary charm_bnd_cond is evaluating, so a direct call
+ * passes the state through and records an error */
 void charm_bnd_cond_fn_mass_flow(charm_prim_t *par_in, charm_prim_t *par_out, int8_t face,
                                  charm_real_t *param, charm_real_t n[3]);
 /* charm_bnd_cond (src/charm_bnd_cond.c:21-33): the BC function of cube side

@@ -316,6 +316,30 @@ void or_bnd(const or_config_t *cfg, int face, const or_prim_t *in,
         out->t = param[0];
         break;
     }
+    case OR_BND_MASS_FLOW: {
+        /* src/charm_bnd_cond.c:101-146: param = (mass flux, static p,
+         * total T, direction cosines, outflow p, composition).  The outflow
+         * pressure param[6] is read and never written back: the reference
+         * stores the last supersonic face's pressure there, which depends on
+         * the face-visit order */
+        double cs[3] = {param[3], param[4], param[5]};
+        double fd = or_scalar_prod(n, cs);
+        if (fd > 0.) {
+            double vel = out->u*n[0] + out->v*n[1] + out->w*n[2];
+            if (vel < out->cz) out->p = param[6];
+        } else {
+            double v2, k, vm;
+            for (i = 0; i < cfg->n_comp; i++) out->c[i] = param[7+i];
+            or_eos(cfg, out, 0);
+            v2 = out->u*out->u + out->v*out->v + out->w*out->w;
+            if (v2 > out->cz*out->cz) out->p = param[1];
+            k = pow(param[0]*(out->cp - out->cv)/(out->p*fd), 2.) / (2.*out->cp);
+            out->t = (-1. + sqrt(1. + 4.*k*param[2])) / (2.*k);
+            vm = fabs(param[0] / (out->r*fd));
+            out->u = cs[0]*vm; out->v = cs[1]*vm; out->w = cs[2]*vm;
+        }
+        break;
+    }
     default:
         break;
     }

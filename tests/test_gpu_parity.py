@@ -23,7 +23,7 @@ def oracle_cfg(cfg):
     """The oracle's config for a device config (every field this path reads)."""
     kinds = {v: k for k, v in CA.BND.items()}
     bnd = tuple(kinds[cfg.bnd_type[i]] for i in range(6))
-    params = [[cfg.bnd_param[i][j] for j in range(6)] for i in range(6)]
+    params = [[cfg.bnd_param[i][j] for j in range(16)] for i in range(6)]
     comps = [cfg.comp[i] for i in range(cfg.n_comp)]
     tr = [dict(ml0=q.ml0, kp0=q.kp0, t0=q.t0, ts=q.ts, h0=q.h0, sig=q.sig, ek=q.ek,
                ml_type=q.ml_type, kp_type=q.kp_type, cp_type=q.cp_type) for q in comps]
@@ -117,6 +117,40 @@ def test_residual_fp64_matches_oracle(mesh, flux):
     ref = sim.residual()
     tol = 1e-9 if flux == "GODUNOV" else REL64
     assert relerr(got, ref) < tol
+
+
+@pytest.mark.parametrize("nc,flux", [(1, "HLLC"), (1, "LF"), (2, "HLLC")])
+def test_mass_flow_boundary_matches_oracle(nc, flux):
+    """BOUND_MASS_FLOW (src/charm_bnd_cond.c:101-146) on a 2:1 mesh: the
+    direction cosines enter through +x and -y (inflow: the parameters'
+    composition, total temperature and mass flux) and leave through -x and
+    +y (outflow pressure); residual and three SSP-RK3 steps with the
+    limiter against the oracle, FP64, and FP32 against FP64."""
+    leaves = MU.random_adapted(1, 3, seed=11)
+    # (flux, static p, total T, cosines, outflow p, composition)
+    mf = [0.35, 1.3, 1.6, -0.6, 0.8, 0.0, 0.95] + ([1.0] if nc == 1 else [0.75, 0.25])
+    bnd = ("MASS_FLOW", "MASS_FLOW", "MASS_FLOW", "MASS_FLOW", "OUTLET", "WALL_SLIP")
+    params = [mf] * 4 + [None, None]
+    if nc == 1:
+        kw = {}
+    else:
+        kw = dict(eos="MIX", comp_m=(CA.GAS_R, 0.5 * CA.GAS_R), comp_cp=(3.5, 5.0))
+    res = {}
+    for prec in (CA.FP64, CA.FP32):
+        cfg = CA.make_config(precision=prec, flux=flux, bnd=bnd, bnd_param=params, limiter=True, **kw)
+        u0 = random_state(leaves, np.random.default_rng(21), nc=nc)
+        ctx = device(cfg, leaves, (False,) * 3, u0)
+        res[prec] = [ctx.residual()]
+        ctx.run(2e-3, 3)
+        res[prec].append(ctx.get_state())
+        if prec == CA.FP64:
+            sim = OL.OracleSim(oracle_cfg(cfg), leaves)
+            sim.set_state(u0)
+            assert relerr(res[prec][0], sim.residual()) < REL64
+            sim.run(2e-3, 3)
+            assert relerr(res[prec][1], sim.get_state()) < 1e-11
+    assert relerr(res[CA.FP32][0], res[CA.FP64][0]) < 2e-5
+    assert relerr(res[CA.FP32][1], res[CA.FP64][1]) < 2e-5
 
 
 @pytest.mark.parametrize("mesh", ["uniform3", "adapt", "adapt_p"])

@@ -262,6 +262,73 @@ def test_toro_star_values():
     assert abs(PI - 0.30313) < 1e-5 and abs(UI - 0.92745) < 1e-5
 
 
+# ---------------------------------------------------------------- mass flow
+def _mass_flow_cfg(p_out=0.9e5):
+    R = OL.GAS_R
+    gams, ms = (1.4, 1.67), (0.029, 0.004)
+    cps = tuple((g * R / ((g - 1) * m),) for g, m in zip(gams, ms))
+    cos = np.array([0.6, 0.0, -0.8])
+    m, p_stat, t_tot = 3.0, 1.2e5, 420.0
+    par = [m, p_stat, t_tot, *cos, p_out, 0.7, 0.3]
+    cfg = OL.make_config(n_comp=2, comp_m=ms, comp_cp=cps, eos="MIX", bnd=("MASS_FLOW",) * 6,
+                         bnd_param=[par] * 6)
+    return cfg, cps, ms, cos, m, p_stat, t_tot
+
+
+def test_mass_flow_inflow_carries_flux_and_total_temperature():
+    """Inflow (the flow direction enters through the face): with an interior
+    density equal to p / (R_s T*) for the T* that the total temperature
+    implies, the exterior state has temperature T*, the parameters'
+    composition, velocity along the direction cosines, mass flux m into
+    the domain and total temperature T* + |v|^2 / (2 cp) = T_tot
+    (src/charm_bnd_cond.c:101-146)."""
+    cfg, cps, ms, cos, m, p_stat, t_tot = _mass_flow_cfg()
+    y = np.array([0.7, 0.3])
+    cp = y @ np.array([c[0] for c in cps])
+    rs = OL.GAS_R * (y / np.array(ms)).sum()
+    n = np.array([0.0, 0.0, 1.0])                        # outward normal; cos . n = -0.8: inflow
+    fd = cos @ n
+    for t_star, p in ((400.0, 1.0e5), (380.0, 0.8e5)):
+        r_in = p / (rs * t_star)
+        vm = m / (r_in * abs(fd))
+        cfg.bnd_param[5][2] = t_star + vm * vm / (2 * cp)       # the T_tot that implies T*
+        pin = OL.prim_from(cfg, r_in, 5.0, -3.0, 2.0, p, c=(0.7, 0.3))   # subsonic interior
+        out = OL.Prim()
+        OL.lib().or_bnd(C_byref(cfg), 5, C_byref(pin), C_byref(out), n.ctypes.data_as(OL.C.POINTER(OL.C.c_double)))
+        v = np.array([out.u, out.v, out.w])
+        assert abs(out.t - t_star) < 1e-10 * t_star
+        assert abs(out.p - p) < 1e-9 * p
+        assert np.allclose([out.c[0], out.c[1]], [0.7, 0.3], rtol=0, atol=1e-15)
+        assert np.allclose(v / np.linalg.norm(v), cos, atol=1e-14)
+        assert abs(out.r * (v @ n) + m) < 1e-12 * m          # mass flux m into the domain
+        assert abs(out.t + (v @ v) / (2 * cp) - cfg.bnd_param[5][2]) < 1e-10 * t_tot
+    # supersonic interior: the static pressure of the parameters
+    pin = OL.prim_from(cfg, 1.0, 900.0, 0.0, 0.0, 1.0e5, c=(0.7, 0.3))
+    out = OL.Prim()
+    OL.lib().or_bnd(C_byref(cfg), 5, C_byref(pin), C_byref(out), n.ctypes.data_as(OL.C.POINTER(OL.C.c_double)))
+    assert out.p == p_stat
+
+
+def test_mass_flow_outflow_pressure():
+    """Outflow (the direction leaves through the face): a subsonic normal
+    velocity takes the outflow pressure param[6], a supersonic one keeps the
+    interior state; the rest of the interior state passes through."""
+    cfg, *_ = _mass_flow_cfg(p_out=0.9e5)
+    n = np.array([0.0, 0.0, -1.0])                       # cos . n = +0.8: outflow
+    for w, expect in ((-50.0, 0.9e5), (-900.0, None)):
+        pin = OL.prim_from(cfg, 1.1, 3.0, 1.0, w, 1.0e5, c=(0.4, 0.6))
+        out = OL.Prim()
+        OL.lib().or_bnd(C_byref(cfg), 4, C_byref(pin), C_byref(out), n.ctypes.data_as(OL.C.POINTER(OL.C.c_double)))
+        assert out.p == (pin.p if expect is None else expect)
+        assert (out.u, out.v, out.w) == (pin.u, pin.v, pin.w)
+        assert abs(out.t - pin.t) < 1e-12 * pin.t
+        assert (out.c[0], out.c[1]) == (pin.c[0], pin.c[1])
+
+
+def C_byref(x):
+    return OL.C.byref(x)
+
+
 # ---------------------------------------------------------------- mesh traversal
 @pytest.mark.parametrize("level", [1, 2, 3])
 def test_face_counts_uniform(level):

@@ -110,11 +110,43 @@ def test_boundary_functions_closed_forms():
         vo = np.array([out.u, out.v, out.w])
         np.testing.assert_allclose(vo, v - (v @ n) * n, rtol=0, atol=1e-15)
         assert abs(vo @ n) < 1e-14 and out.t == par[0]
-    # mass flow: unsupported, passes the state through and says so
+    # mass flow called directly: it needs the material of the boundary that
+    # charm_bnd_cond evaluates, so it passes the state through and says so
     out = ChPrim()
     L.charm_bnd_cond_fn_mass_flow(C.byref(pin), C.byref(out), 1, dptr(par), dptr(nn))
     assert as_dict(out) == base
-    assert b"MASS_FLOW" in CA.lib().charm_gpu_last_error()
+    assert b"charm_bnd_cond" in CA.lib().charm_gpu_last_error()
+
+
+def test_reference_named_lookups_and_unreleased_entry_points():
+    """charm_eos_get_r (src/charm_eos.c:10), charm_bnd_type_by_name
+    (src/charm_bnd_cond.c:9-19) and the two entry points the reference
+    cannot run as written (charm_mat_eos_table aborts, charm_adapt_init reads
+    an uncomputed gradient): they record the reason instead."""
+    L = CA.lib()
+    L.charm_eos_get_r.restype = C.c_double
+    L.charm_eos_get_r.argtypes = []
+    assert L.charm_eos_get_r() == 8.31446261815324
+    L.charm_bnd_type_by_name.restype = C.c_int
+    L.charm_bnd_type_by_name.argtypes = [C.c_char_p]
+    names = ("BOUND_INLET", "BOUND_OUTLET", "BOUND_WALL_SLIP", "BOUND_WALL_NO_SLIP", "BOUND_MASS_FLOW")
+    for i, nm in enumerate(names):
+        assert L.charm_bnd_type_by_name(nm.encode()) == i
+    for nm in (b"BOUND_PERIODIC", b"bound_inlet", b"", b"BOUND_INLET "):
+        assert L.charm_bnd_type_by_name(nm) == 5                # BOUND_UNKNOWN
+    L.charm_mat_eos_table.restype = None
+    L.charm_mat_eos_table.argtypes = [C.c_void_p, C.POINTER(ChPrim), C.c_int]
+    p = ChPrim()
+    p.r, p.e = 1.0, 2.0
+    L.charm_mat_eos_table(None, C.byref(p), 4)
+    assert (p.r, p.e, p.p) == (1.0, 2.0, 0.0)
+    assert b"TABLE EOS IS NOT RELEASED" in L.charm_gpu_last_error()
+    L.charm_adapt_init.restype = None
+    L.charm_adapt_init.argtypes = [C.c_void_p]
+    f = CA.Forest.uniform(2)
+    L.charm_adapt_init(f.h)
+    assert f.n == 64
+    assert b"adapt_init" in L.charm_gpu_last_error()
 
 
 def configs():
@@ -216,8 +248,11 @@ def test_cons_to_prim_and_back(name):
 def test_bnd_cond_matches_oracle():
     """charm_bnd_cond on every BC type of the sides: the BC function, EOS
     flag 3 and e_tot, against the oracle's boundary state"""
-    bnd = ("INLET", "OUTLET", "WALL_SLIP", "WALL_NO_SLIP", "WALL_SLIP", "OUTLET")
-    params = [[0.3, 0.1, 0.0, 1.1, 1.0], None, None, [1.05], None, None]
+    bnd = ("INLET", "OUTLET", "WALL_SLIP", "WALL_NO_SLIP", "MASS_FLOW", "MASS_FLOW")
+    # mass flow: (flux, static p, total T, cosines, outflow p); the cosine
+    # leaves through -z (outflow) and enters through +z (inflow)
+    mf = [0.4, 1.3, 1.5, 0.6, 0.0, -0.8, 0.95, 1.0]
+    params = [[0.3, 0.1, 0.0, 1.1, 1.0], None, None, [1.05], mf, mf]
     cfg = CA.make_config(precision=CA.FP64, bnd=bnd, bnd_param=params)
     ocfg = OL.make_config(bnd=bnd, bnd_param=params)
     L = bind()
