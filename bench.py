@@ -75,59 +75,86 @@ def run_path(CA, W, level, precision, steps, warmup, u0=None):
                 state=state, m0=m0, m1=m1, chunking=chunking)
 
 
-def cpu_probe(W, budget_s):
-    """One single-threaded FP64 oracle run on the level-4 Riemann box for
-    about budget_s; returns (DOF-updates, seconds, steps)."""
+def cpu_run(W, level, steps, warmup):
+    """Child of the CPU baseline (never touches the GPU): the FP64 oracle
+    build named by CHARM_ORACLE_LIB on the Riemann box at `level`, `warmup`
+    untimed steps, then `steps` timed ones; returns its record."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as OL
     import charm_3d_amd as CA
-    cfg, leaves, u0, _ = W.c2(level=4, precision=CA.FP64)
+    cfg, leaves, u0, _ = W.c2(level=level, precision=CA.FP64)
     ocfg = OL.make_config(flux="HLLC", limiter=True, model="NS", bnd=("OUTLET",) * 6)
+    t0 = time.perf_counter()
     sim = OL.OracleSim(ocfg, leaves)
     sim.set_state(u0)
+    setup = time.perf_counter() - t0
     dt = sim.cfl_dt(1)
-    steps, t = 0, 0.0
+    if warmup:
+        sim.run(dt, warmup)
     t0 = time.perf_counter()
-    while t < budget_s and steps < 10000:
-        sim.run(dt, 1)
-        steps += 1
-        t = time.perf_counter() - t0
-    return W.dof_updates(len(leaves), 5, 3, steps), t, steps
+    sim.run(dt, steps)
+    t = time.perf_counter() - t0
+    return {"dofs": W.dof_updates(len(leaves), 5, 3, steps), "seconds": t, "steps": steps, "level": level,
+            "elements": len(leaves), "setup_s": setup, "lib": os.path.basename(OL.LIB_PATH),
+            "threads": int(os.environ.get("OMP_NUM_THREADS", "1"))}
 
 
-def cpu_baseline(W, budget_s):
-    """The FP64 oracle (C restatement of the reference numerics, one thread
-    per run) on the same workload at level 4: one run alone, then one run
-    per host core at once (child processes that never touch the GPU).  The
-    aggregate of the concurrent runs is the upper bound of the reference's
-    MPI scaling on those cores (each rank would hold 1/P of the leaves)."""
+def host_info():
+    """lscpu's model name, sockets, cores per socket and logical CPUs"""
     import subprocess
-    dofs, t, steps = cpu_probe(W, budget_s)
-    single = {"value": dofs / t, "cores": 1,
-              "sample": "%d steps in %.1f s on one host core" % (steps, t)}
-    procs = int(os.environ.get("CHARM_CPU_PROCS", "0")) or min(16, os.cpu_count() or 1)
-    out = {"value": single["value"], "unit": "DOF-updates/s", "cores": 1, "kind": "port",
-           "sample": "FP64 oracle (C restatement of the reference numerics, AoS records, callback face "
-                     "traversal), Riemann box at level 4 (4,096 hexes), HLLC + BJ, SSP-RK3, " + single["sample"],
-           "single_core": single}
-    if procs > 1:
-        cmd = [sys.executable, os.path.abspath(__file__), "--cpu-probe", str(budget_s)]
-        ps = [subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True)
-              for _ in range(procs)]
-        res = []
-        for p in ps:
-            o, _ = p.communicate(timeout=budget_s * 10 + 120)
-            if p.returncode == 0:
-                res.append(json.loads(o.strip().splitlines()[-1]))
-        if len(res) == procs:
-            rate = sum(r["dofs"] / r["seconds"] for r in res)
-            out.update({"value": rate, "cores": procs,
-                        "sample": "FP64 oracle (C restatement of the reference numerics, AoS records, "
-                                  "callback face traversal), Riemann box at level 4 (4,096 hexes), HLLC + BJ, "
-                                  "SSP-RK3: %d single-threaded runs at once, one per host core, %d-%d steps in "
-                                  "%.1f s each; sum of their rates (the bound of MPI scaling on %d cores)"
-                                  % (procs, min(r["steps"] for r in res), max(r["steps"] for r in res),
-                                     budget_s, procs)})
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    sockets = int(info.get("Socket(s)", "0") or 0)
+    cores = int(info.get("Core(s) per socket", "0") or 0)
+    return {"model": info.get("Model name", "?"), "physical_cores": sockets * cores or None,
+            "logical_cpus": int(info.get("CPU(s)", "0") or 0) or os.cpu_count(),
+            "threads_per_core": info.get("Thread(s) per core")}
+
+
+def cpu_baseline(W, level, threads):
+    """The FP64 oracle (the C restatement of the reference numerics: array-
+    of-structs leaf records, per-basis field recomputation, callback face
+    traversal) on the benchmark's own Riemann box, in child processes that
+    never touch the GPU.  The headline is the OpenMP build (leaf- and
+    face-parallel passes with the serial sums) at the benchmark's level on
+    `threads` threads, 1 warm-up and 3 timed steps; beside it the same build
+    on one thread at level 5, and the reference-faithful build (per-leaf
+    records and field loops sized for the reference's 128 species, about its
+    24 KB per leaf) at level 5 on all threads.  The reference itself needs
+    p4est, libsc and yaml-cpp, which are absent, so it cannot be built."""
+    import subprocess
+
+    def child(lib, lvl, nthr, steps, warmup):
+        env = dict(os.environ, CHARM_ORACLE_LIB=os.path.join(ROOT, "oracle", lib), OMP_NUM_THREADS=str(nthr))
+        cmd = [sys.executable, os.path.abspath(__file__), "--cpu-run", str(lvl), str(steps), str(warmup)]
+        r = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=1800)
+        if r.returncode != 0:
+            return {"error": r.stderr[-400:]}
+        rec = json.loads(r.stdout.strip().splitlines()[-1])
+        rec["value"] = rec["dofs"] / rec["seconds"]
+        return rec
+
+    print("[bench] CPU baseline: level %d, %d threads" % (level, threads), file=sys.stderr, flush=True)
+    main_run = child("liboracle_omp.so", level, threads, 3, 1)
+    print("[bench] CPU baseline: one thread, reference-faithful records", file=sys.stderr, flush=True)
+    single = child("liboracle_omp.so", 5, 1, 3, 1)
+    faithful = child("liboracle_faithful.so", 5, threads, 2, 1)
+    host = host_info()
+    out = {"value": main_run.get("value"), "unit": "DOF-updates/s", "cores": threads, "kind": "port",
+           "sample": "FP64 oracle (C restatement of the reference numerics: AoS leaf records, per-basis field "
+                     "recomputation, callback face traversal), OpenMP build on %d threads, the benchmark's "
+                     "Riemann box at level %d (%s hexes), HLLC + BJ, SSP-RK3: 1 warm-up and 3 timed steps "
+                     "(%.1f s)" % (threads, level, main_run.get("elements"), main_run.get("seconds", 0.0)),
+           "host": host, "run": main_run,
+           "single_thread": single,
+           "reference_faithful": dict(faithful, note="per-leaf records and field loops for the reference's "
+                                                     "128 species (include/charm_def.h:181), level 5")}
     return out
 
 
@@ -475,12 +502,15 @@ def main():
     ap.add_argument("--long-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-level", type=int, default=None,
+                    help="level of the CPU baseline's Riemann box (default: the benchmark's)")
+    ap.add_argument("--cpu-threads", type=int, default=None,
+                    help="OpenMP threads of the CPU baseline (default: OMP_NUM_THREADS, else all CPUs)")
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--no-traffic", action="store_true")
     ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--probe-precision", type=int, default=1, help=argparse.SUPPRESS)
-    ap.add_argument("--cpu-probe", type=float, default=0.0, help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-run", type=int, nargs=3, default=None, help=argparse.SUPPRESS)
     ap.add_argument("--no-fd-check", action="store_true",
                     help="with --only adjoint: the FP32 adjoint and its timing, without the FP64 reference gradient")
     ap.add_argument("--only", choices=("bubbles", "hot_spot", "adjoint"),
@@ -492,9 +522,8 @@ def main():
         sys.exit(2)
 
     import charm_3d_amd.workloads as W
-    if args.cpu_probe > 0:
-        dofs, t, steps = cpu_probe(W, args.cpu_probe)
-        print(json.dumps({"dofs": dofs, "seconds": t, "steps": steps}))
+    if args.cpu_run:
+        print(json.dumps(cpu_run(W, *args.cpu_run)))
         return
     import charm_3d_amd as CA
 
@@ -624,8 +653,8 @@ def main():
                 out["roofline"]["bound"] = "valu"
 
     if not args.no_cpu:
-        print("[bench] CPU baseline", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(W, args.cpu_seconds)
+        threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
+        out["cpu_baseline"] = cpu_baseline(W, args.cpu_level or args.level, threads)
     print(json.dumps(out))
 
 

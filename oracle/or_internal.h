@@ -13,6 +13,22 @@
 #define OR_MAX(X, Y) ((X) > (Y) ? (X) : (Y))
 #define OR_MIN(X, Y) ((X) < (Y) ? (X) : (Y))
 
+/* Species bound of the per-leaf record.  OR_FAITHFUL (the bench's
+ * reference-faithful CPU timing build, never the checker) sizes the record
+ * for the reference's CHARM_MAX_COMPONETS_COUNT = 128 species
+ * (include/charm_def.h:181; about the reference's 24 KB per leaf) and runs
+ * the field evaluation and the RK combine over all of them, as
+ * src/charm_base_func.c:198-211 and src/models/ns/charm_model.c:129-178 do.
+ * The public structs keep OR_NC_MAX. */
+#ifdef OR_FAITHFUL
+#define OR_REC_NC 128
+#define OR_FIELD_NV(s) (4 + OR_REC_NC)
+#else
+#define OR_REC_NC OR_NC_MAX
+#define OR_FIELD_NV(s) ((s)->nv)
+#endif
+#define OR_REC_NV (4 + OR_REC_NC)
+
 extern const double or_gR;
 
 /* per-leaf geometry, the `geom` member of charm_param_t
@@ -37,17 +53,17 @@ typedef struct {
 /* per-leaf record in the shape of charm_data_t (include/charm_globals.h:
  * 143-234); coefficients stored [v][k] with v = ru, rv, rw, re, rc[...] */
 typedef struct {
-    double c[OR_NV_MAX][OR_BF];
-    double c_old[OR_NV_MAX][OR_BF];
+    double c[OR_REC_NV][OR_BF];
+    double c_old[OR_REC_NV][OR_BF];
     or_geom_t g;
     int    mat_id;
     double grav[3];
     struct {
         int    count;
-        double u[OR_NV_MAX][7];
+        double u[OR_REC_NV][7];
     } l;
     double grad_u[3];
-    double integ[OR_NV_MAX][OR_BF];          /* int_ru .. int_rc */
+    double integ[OR_REC_NV][OR_BF];          /* int_ru .. int_rc */
     /* Navier-Stokes auxiliaries (charm_param_t.model.ns) */
     double tau[6][OR_BF];                    /* xx yy zz xy xz yz */
     double q[3][OR_BF];
@@ -77,6 +93,15 @@ struct or_sim {
     int         nv;
     double      dt;          /* ctx->dt: the chemistry's step */
     char        err[256];
+    /* OpenMP builds only (the bench's CPU timing): each leaf's face
+     * incidences in face order, and per face its flux values at the face
+     * points, so that the convective pass runs its faces in parallel and
+     * still adds every leaf's contributions in the serial order */
+    int64_t    *inc_ptr;     /* [n + 1] */
+    int64_t    *inc;         /* face << 4 | side << 3 | subface (7: all four) */
+    int64_t    *fq_off;      /* [nf + 1], in doubles */
+    double     *fq;
+    double     *lq;          /* [nf][2][OR_NV_MAX]: the limiter's per-face neighbour values */
 };
 
 double or_scalar_prod(const double a[3], const double b[3]);

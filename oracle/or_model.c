@@ -12,6 +12,13 @@
  * the non-owning side at the GP shifted by one period.
  */
 #include "or_internal.h"
+#ifdef _OPENMP
+#include <omp.h>
+/* element loops whose iterations touch only their own leaf */
+#define OR_PARFOR _Pragma("omp parallel for schedule(static)")
+#else
+#define OR_PARFOR
+#endif
 
 /* ------------------------------------------------------------ hash map */
 typedef struct { uint64_t *key; int64_t *val; uint64_t mask; } or_hash_t;
@@ -172,6 +179,7 @@ or_sim_t *or_create(const or_config_t *cfg, int64_t n_leaves, const int32_t *lea
     s->nv = 4 + cfg->n_comp;
     s->d = (or_data_t *) calloc((size_t) n_leaves, sizeof(or_data_t));
     if (!s->d) { snprintf(s->err, sizeof(s->err), "out of memory"); return s; }
+    OR_PARFOR
     for (e = 0; e < n_leaves; e++) {
         or_data_t *d = &s->d[e];
         memcpy(d->quad, &leaves[4*e], 4*sizeof(int32_t));
@@ -185,6 +193,7 @@ or_sim_t *or_create(const or_config_t *cfg, int64_t n_leaves, const int32_t *lea
 void or_destroy(or_sim_t *s)
 {
     if (!s) return;
+    free(s->inc_ptr); free(s->inc); free(s->fq_off); free(s->fq); free(s->lq);
     free(s->d); free(s->f); free(s);
 }
 
@@ -248,8 +257,8 @@ void or_geom(const or_sim_t *s, int64_t leaf, double *out)
 void or_get_fields(const or_sim_t *s, const or_data_t *d, const double x[3], or_cons_t *c)
 {
     int v, i;
-    double r[OR_NV_MAX];
-    for (v = 0; v < s->nv; v++) {
+    double r[OR_REC_NV];
+    for (v = 0; v < OR_FIELD_NV(s); v++) {
         r[v] = 0.;
         for (i = 0; i < OR_BF; i++) r[v] += d->c[v][i]*or_bf(&d->g, x, i);
     }
@@ -262,12 +271,12 @@ void or_get_fields(const or_sim_t *s, const or_data_t *d, const double x[3], or_
 void or_get_fields_avg(const or_sim_t *s, const or_data_t *d, or_cons_t *c)
 {
     int igp, v, i;
-    double r[OR_NV_MAX];
-    for (v = 0; v < s->nv; v++) r[v] = 0.;
+    double r[OR_REC_NV];
+    for (v = 0; v < OR_FIELD_NV(s); v++) r[v] = 0.;
     for (igp = 0; igp < OR_QGP; igp++) {
         const double *gx = d->g.quad_gp[igp];
         double gjw = d->g.quad_gj[igp]*d->g.quad_gw[igp]/d->g.volume;
-        for (v = 0; v < s->nv; v++) {
+        for (v = 0; v < OR_FIELD_NV(s); v++) {
             double f = 0.;
             for (i = 0; i < OR_BF; i++) f += d->c[v][i]*or_bf(&d->g, gx, i);
             r[v] += f * gjw;
@@ -431,13 +440,190 @@ static void or_conv_inner(or_sim_t *s, const or_face_t *f)
 static void or_zero(or_sim_t *s)
 {
     int64_t e;
+    OR_PARFOR
     for (e = 0; e < s->n; e++) memset(s->d[e].integ, 0, sizeof(s->d[e].integ));
 }
+
+#ifdef _OPENMP
+/* ------------------------------------------- parallel convective pass
+ * (OpenMP builds: the bench's CPU timing).  Phase 1 evaluates every face's
+ * fluxes at its points in parallel into fq; phase 2 lets every leaf add its
+ * faces' contributions itself, in face order, with the serial pass's
+ * expressions: each accumulator sees the same sums in the same order (a leaf
+ * that is both sides of one face, a one-leaf periodic box, is the only
+ * exception). */
+static int or_nsub(const or_face_t *f)
+{
+    return (f->nsides == 2 && (f->is_hanging[0] || f->is_hanging[1])) ? 4 : 1;
+}
+
+static int or_par_setup(or_sim_t *s)
+{
+    int64_t e, fi, k, *pos;
+    int side, j;
+    if (s->fq) return 0;
+    s->inc_ptr = (int64_t *) calloc((size_t) s->n + 1, sizeof(int64_t));
+    s->fq_off = (int64_t *) malloc(((size_t) s->nf + 1) * sizeof(int64_t));
+    pos = (int64_t *) malloc(((size_t) s->n + 1) * sizeof(int64_t));
+    if (!s->inc_ptr || !s->fq_off || !pos) { free(pos); return 1; }
+    for (fi = 0; fi < s->nf; fi++) {
+        const or_face_t *f = &s->f[fi];
+        for (side = 0; side < f->nsides; side++) {
+            if (f->is_hanging[side]) for (j = 0; j < 4; j++) s->inc_ptr[f->quad[side][j] + 1]++;
+            else s->inc_ptr[f->quad[side][0] + 1]++;
+        }
+    }
+    for (e = 0; e < s->n; e++) s->inc_ptr[e + 1] += s->inc_ptr[e];
+    s->inc = (int64_t *) malloc((size_t) s->inc_ptr[s->n] * sizeof(int64_t));
+    if (!s->inc) { free(pos); return 1; }
+    memcpy(pos, s->inc_ptr, (size_t) s->n * sizeof(int64_t));
+    for (fi = 0; fi < s->nf; fi++) {              /* ascending: each list in face order */
+        const or_face_t *f = &s->f[fi];
+        for (side = 0; side < f->nsides; side++) {
+            if (f->is_hanging[side])
+                for (j = 0; j < 4; j++) s->inc[pos[f->quad[side][j]]++] = fi << 4 | side << 3 | j;
+            else s->inc[pos[f->quad[side][0]]++] = fi << 4 | side << 3 | 7;
+        }
+    }
+    free(pos);
+    for (k = 0, fi = 0; fi < s->nf; fi++) {
+        s->fq_off[fi] = k;
+        k += (int64_t) or_nsub(&s->f[fi]) * OR_FGP * s->nv;
+    }
+    s->fq_off[s->nf] = k;
+    s->fq = (double *) malloc((size_t) k * sizeof(double));
+    s->lq = (double *) malloc(((size_t) s->nf * 2 * OR_NV_MAX + 1) * sizeof(double));
+    return s->fq == NULL || s->lq == NULL;
+}
+
+/* phase 1: the fluxes of or_conv_bnd / or_conv_inner at every face point */
+static void or_face_fluxes(or_sim_t *s, int *bad)
+{
+    int64_t fi;
+    #pragma omp parallel for schedule(dynamic, 256)
+    for (fi = 0; fi < s->nf; fi++) {
+        const or_face_t *f = &s->f[fi];
+        double *out = s->fq + s->fq_off[fi], n[3], xs[2][3];
+        or_cons_t cons[2]; or_prim_t prim[2];
+        or_data_t *ud[2];
+        int i, j, igp, h_side = -1;
+        if (f->nsides != 2) {
+            or_data_t *d = &s->d[f->quad[0][0]];
+            or_face_normal(d, f->face[0], n);
+            for (igp = 0; igp < OR_FGP; igp++) {
+                or_get_fields(s, d, d->g.face_gp[f->face[0]][igp], &cons[0]);
+                or_cons_to_prim(&s->cfg, &cons[0], &prim[0]);
+                or_bnd(&s->cfg, f->bnd, &prim[0], &prim[1], n);
+                if (or_flux_err(&s->cfg, s->cfg.flux, prim, n, out + igp * s->nv)) *bad = 1;
+            }
+        } else if (f->is_hanging[0] || f->is_hanging[1]) {
+            for (j = 0; j < 4; j++) {
+                for (i = 0; i < 2; i++) {
+                    if (f->is_hanging[i]) { ud[i] = &s->d[f->quad[i][j]]; h_side = i; }
+                    else ud[i] = &s->d[f->quad[i][0]];
+                }
+                or_face_normal(ud[0], f->face[0], n);
+                for (igp = 0; igp < OR_FGP; igp++) {
+                    const double *x = ud[h_side]->g.face_gp[f->face[h_side]][igp];
+                    for (i = 0; i < 2; i++) {
+                        or_shift(f, i, h_side, x, xs[i]);
+                        or_get_fields(s, ud[i], xs[i], &cons[i]);
+                        or_cons_to_prim(&s->cfg, &cons[i], &prim[i]);
+                    }
+                    if (or_flux_err(&s->cfg, s->cfg.flux, prim, n, out + (j * OR_FGP + igp) * s->nv)) *bad = 1;
+                }
+            }
+        } else {
+            for (i = 0; i < 2; i++) ud[i] = &s->d[f->quad[i][0]];
+            or_face_normal(ud[0], f->face[0], n);
+            for (igp = 0; igp < OR_FGP; igp++) {
+                const double *x = ud[0]->g.face_gp[f->face[0]][igp];
+                for (i = 0; i < 2; i++) {
+                    or_shift(f, i, 0, x, xs[i]);
+                    or_get_fields(s, ud[i], xs[i], &cons[i]);
+                    or_cons_to_prim(&s->cfg, &cons[i], &prim[i]);
+                }
+                if (or_flux_err(&s->cfg, s->cfg.flux, prim, n, out + igp * s->nv)) *bad = 1;
+            }
+        }
+    }
+}
+
+static void or_accum(or_sim_t *s, or_data_t *d, const double *q, double bfv, int ibf);
+
+/* phase 2: each leaf adds its faces' contributions, in face order */
+static void or_face_gather(or_sim_t *s)
+{
+    int64_t e;
+    #pragma omp parallel for schedule(static)
+    for (e = 0; e < s->n; e++) {
+        or_data_t *d = &s->d[e];
+        int64_t k;
+        for (k = s->inc_ptr[e]; k < s->inc_ptr[e + 1]; k++) {
+            const int64_t fi = s->inc[k] >> 4;
+            const int side = (int) ((s->inc[k] >> 3) & 1), jj = (int) (s->inc[k] & 7);
+            const or_face_t *f = &s->f[fi];
+            const double *q0 = s->fq + s->fq_off[fi];
+            double xs[3];
+            int igp, ibf, j;
+            if (f->nsides != 2) {
+                const int face = f->face[0];
+                for (igp = 0; igp < OR_FGP; igp++) {
+                    const double *x = d->g.face_gp[face][igp];
+                    double gw = d->g.face_gw[face][igp], gj = d->g.face_gj[face][igp];
+                    for (ibf = 0; ibf < OR_BF; ibf++)
+                        or_accum(s, d, q0 + igp * s->nv, or_bf(&d->g, x, ibf) * gw * gj, ibf);
+                }
+            } else if (f->is_hanging[0] || f->is_hanging[1]) {
+                const int h_side = f->is_hanging[0] ? 0 : 1;
+                for (j = (jj == 7 ? 0 : jj); j <= (jj == 7 ? 3 : jj); j++) {
+                    const or_data_t *hs = &s->d[f->quad[h_side][j]];
+                    for (igp = 0; igp < OR_FGP; igp++) {
+                        const double *x = hs->g.face_gp[f->face[h_side]][igp];
+                        double gw = hs->g.face_gw[f->face[h_side]][igp];
+                        double gj = hs->g.face_gj[f->face[h_side]][igp];
+                        or_shift(f, side, h_side, x, xs);
+                        for (ibf = 0; ibf < OR_BF; ibf++)
+                            or_accum(s, d, q0 + (j * OR_FGP + igp) * s->nv,
+                                     (side ? -1. : 1.) * or_bf(&d->g, xs, ibf) * gw * gj, ibf);
+                    }
+                }
+            } else {
+                const or_data_t *d0 = &s->d[f->quad[0][0]];
+                for (igp = 0; igp < OR_FGP; igp++) {
+                    const double *x = d0->g.face_gp[f->face[0]][igp];
+                    double gw = d0->g.face_gw[f->face[0]][igp], gj = d0->g.face_gj[f->face[0]][igp];
+                    or_shift(f, side, 0, x, xs);
+                    for (ibf = 0; ibf < OR_BF; ibf++)
+                        or_accum(s, d, q0 + igp * s->nv, (side ? -1. : 1.) * or_bf(&d->g, xs, ibf) * gw * gj, ibf);
+                }
+            }
+        }
+    }
+}
+
+/* 0: done in parallel; nonzero: the serial pass runs instead */
+static int or_conv_parallel(or_sim_t *s)
+{
+    int64_t e;
+    int bad = 0;
+    if (omp_get_max_threads() < 2 || or_par_setup(s)) return 1;
+    #pragma omp parallel for schedule(static)
+    for (e = 0; e < s->n; e++) or_conv_volume(s, &s->d[e]);
+    or_face_fluxes(s, &bad);
+    or_face_gather(s);
+    if (bad) snprintf(s->err, sizeof(s->err), "HLLC: inequality SL <= S* <= SR is FALSE");
+    return 0;
+}
+#endif
 
 /* charm_timestep_conv, src/models/euler/charm_model.c:388-396 */
 static void or_conv(or_sim_t *s)
 {
     int64_t e, i;
+#ifdef _OPENMP
+    if (!or_conv_parallel(s)) return;
+#endif
     for (e = 0; e < s->n; e++) or_conv_volume(s, &s->d[e]);
     for (i = 0; i < s->nf; i++) {
         if (s->f[i].nsides != 2) or_conv_bnd(s, &s->f[i]);
@@ -471,10 +657,12 @@ void or_conv_residual(or_sim_t *s, double *out)
 /* c -= dt * A^-1 int, src/models/euler/charm_model.c:402-433 */
 static void or_update(or_sim_t *s, double dt)
 {
-    int64_t e; int v, i;
-    double rhs[OR_NV_MAX][4];
+    int64_t e;
+    OR_PARFOR
     for (e = 0; e < s->n; e++) {
         or_data_t *d = &s->d[e];
+        double rhs[OR_NV_MAX][4];
+        int v, i;
         for (v = 0; v < s->nv; v++) or_matr_vect_mult(d->g.a_inv, d->integ[v], rhs[v]);
         for (i = 0; i < OR_BF; i++)
             for (v = 0; v < s->nv; v++) d->c[v][i] -= dt * rhs[v][i];
@@ -491,6 +679,55 @@ static void or_lim_push(or_sim_t *s, or_data_t *d, const double *u)
     d->l.count++;
 }
 
+/* the two values a face contributes to its leaves' neighbour lists,
+ * src/limiters/charm_limiter_bj.c:30-238: the leaves of side i receive
+ * u[1 - i]; a boundary face gives its leaf u[1], the mean pushed through the
+ * boundary condition.  Conforming: the cell means; hanging: the big side's
+ * centre value and the volume average of the four small centre values */
+static void or_lim_face_values(or_sim_t *s, const or_face_t *f, double u[2][OR_NV_MAX])
+{
+    or_cons_t cons[2], cj;
+    int i, j, v;
+    if (f->nsides != 2) {
+        or_data_t *d = &s->d[f->quad[0][0]];
+        double n[3];
+        or_prim_t prim[2];
+        or_face_normal(d, f->face[0], n);
+        or_get_fields_avg(s, d, &cons[0]);
+        or_cons_to_prim(&s->cfg, &cons[0], &prim[0]);
+        or_bnd(&s->cfg, f->bnd, &prim[0], &prim[1], n);
+        or_prim_to_cons(&s->cfg, &cons[1], &prim[1]);
+        or_cons_get(s, &cons[1], u[1]);
+    } else if (f->is_hanging[0] || f->is_hanging[1]) {
+        for (i = 0; i < 2; i++) {
+            if (f->is_hanging[i]) {
+                double svol = 0.;
+                for (v = 0; v < s->nv; v++) u[i][v] = 0.;
+                for (j = 0; j < 4; j++) {
+                    or_data_t *dj = &s->d[f->quad[i][j]];
+                    double vol = dj->g.volume, uj[OR_NV_MAX];
+                    svol += vol;
+                    or_get_fields(s, dj, dj->g.c, &cj);
+                    or_cons_get(s, &cj, uj);
+                    for (v = 0; v < s->nv; v++) u[i][v] += uj[v]*vol;
+                }
+                for (v = 0; v < s->nv; v++) u[i][v] /= svol;
+            } else {
+                or_data_t *di = &s->d[f->quad[i][0]];
+                or_get_fields(s, di, di->g.c, &cons[i]);
+                or_cons_get(s, &cons[i], u[i]);
+            }
+        }
+        /* species entries are stored here too; the reference leaves them
+         * unwritten on this branch (charm_limiter_bj.c:174-193) */
+    } else {
+        for (i = 0; i < 2; i++) {
+            or_get_fields_avg(s, &s->d[f->quad[i][0]], &cons[i]);
+            or_cons_get(s, &cons[i], u[i]);
+        }
+    }
+}
+
 void or_limiter(or_sim_t *s)
 {
     int64_t e, fi;
@@ -498,62 +735,56 @@ void or_limiter(or_sim_t *s)
     double u[2][OR_NV_MAX];
     or_cons_t cons[2], cj;
     if (!s->cfg.limiter) return;
+    OR_PARFOR
     for (e = 0; e < s->n; e++) {           /* :11-27 */
-        or_get_fields_avg(s, &s->d[e], &cons[0]);
-        or_cons_get(s, &cons[0], u[0]);
+        or_cons_t ce;
+        double ue[OR_NV_MAX];
+        or_get_fields_avg(s, &s->d[e], &ce);
+        or_cons_get(s, &ce, ue);
         s->d[e].l.count = 0;
-        or_lim_push(s, &s->d[e], u[0]);
+        or_lim_push(s, &s->d[e], ue);
     }
+#ifdef _OPENMP
+    if (omp_get_max_threads() > 1 && !or_par_setup(s)) {
+        /* the face values in parallel, then every leaf pushes its faces'
+         * values in face order: the serial lists */
+        OR_PARFOR
+        for (fi = 0; fi < s->nf; fi++)
+            or_lim_face_values(s, &s->f[fi], (double (*)[OR_NV_MAX]) (s->lq + (size_t) fi * 2 * OR_NV_MAX));
+        OR_PARFOR
+        for (e = 0; e < s->n; e++) {
+            int64_t kk;
+            for (kk = s->inc_ptr[e]; kk < s->inc_ptr[e + 1]; kk++) {
+                const int64_t f = s->inc[kk] >> 4;
+                const int side = (int) ((s->inc[kk] >> 3) & 1);
+                const double *lu = s->lq + (size_t) f * 2 * OR_NV_MAX;
+                or_lim_push(s, &s->d[e], lu + (s->f[f].nsides != 2 ? 1 : 1 - side) * OR_NV_MAX);
+            }
+        }
+    } else
+#endif
     for (fi = 0; fi < s->nf; fi++) {        /* :30-238 */
         const or_face_t *f = &s->f[fi];
+        or_lim_face_values(s, f, u);
         if (f->nsides != 2) {
-            or_data_t *d = &s->d[f->quad[0][0]];
-            double n[3];
-            or_prim_t prim[2];
-            or_face_normal(d, f->face[0], n);
-            or_get_fields_avg(s, d, &cons[0]);
-            or_cons_to_prim(&s->cfg, &cons[0], &prim[0]);
-            or_bnd(&s->cfg, f->bnd, &prim[0], &prim[1], n);
-            or_prim_to_cons(&s->cfg, &cons[1], &prim[1]);
-            or_cons_get(s, &cons[1], u[1]);
-            or_lim_push(s, d, u[1]);
+            or_lim_push(s, &s->d[f->quad[0][0]], u[1]);
         } else if (f->is_hanging[0] || f->is_hanging[1]) {
             int hs = f->is_hanging[0] ? 0 : 1, fs = 1 - hs;
-            for (i = 0; i < 2; i++) {
-                if (f->is_hanging[i]) {
-                    double svol = 0.;
-                    for (v = 0; v < s->nv; v++) u[i][v] = 0.;
-                    for (j = 0; j < 4; j++) {
-                        or_data_t *dj = &s->d[f->quad[i][j]];
-                        double vol = dj->g.volume, uj[OR_NV_MAX];
-                        svol += vol;
-                        or_get_fields(s, dj, dj->g.c, &cj);
-                        or_cons_get(s, &cj, uj);
-                        for (v = 0; v < s->nv; v++) u[i][v] += uj[v]*vol;
-                    }
-                    for (v = 0; v < s->nv; v++) u[i][v] /= svol;
-                } else {
-                    or_data_t *di = &s->d[f->quad[i][0]];
-                    or_get_fields(s, di, di->g.c, &cons[i]);
-                    or_cons_get(s, &cons[i], u[i]);
-                }
-            }
-            /* species entries are stored here too; the reference leaves
-             * them unwritten on this branch (charm_limiter_bj.c:174-193) */
             or_lim_push(s, &s->d[f->quad[fs][0]], u[hs]);
             for (j = 0; j < 4; j++) or_lim_push(s, &s->d[f->quad[hs][j]], u[fs]);
         } else {
             for (i = 0; i < 2; i++) {
-                or_get_fields_avg(s, &s->d[f->quad[i][0]], &cons[i]);
-                or_cons_get(s, &cons[i], u[i]);
                 k = (i + 1) % 2;
                 or_lim_push(s, &s->d[f->quad[k][0]], u[i]);
             }
         }
     }
+    OR_PARFOR
     for (e = 0; e < s->n; e++) {            /* :241-315 */
         or_data_t *d = &s->d[e];
         double umin[OR_NV_MAX], umax[OR_NV_MAX], psi[OR_NV_MAX], vtx[8][3], f[OR_NV_MAX];
+        or_cons_t cj;
+        int i, v, k;
         if (d->l.count != 7) { snprintf(s->err, sizeof(s->err), "limiter: count != 7"); }
         for (v = 0; v < s->nv; v++) {
             umin[v] = umax[v] = d->l.u[v][0];
@@ -617,24 +848,29 @@ static void or_ns_stage(or_sim_t *s, double dt)
  * 119-184, 244-266 */
 void or_ns_step(or_sim_t *s, double dt)
 {
-    int64_t e; int v, i;
+    int64_t e;
     s->dt = dt;                                   /* ctx->dt of the chemistry */
+    OR_PARFOR
     for (e = 0; e < s->n; e++) memcpy(s->d[e].c_old, s->d[e].c, sizeof(s->d[e].c));
     or_ns_stage(s, dt);
     or_ns_stage(s, dt);
+    OR_PARFOR
     for (e = 0; e < s->n; e++) {
         or_data_t *d = &s->d[e];
+        int v, i;
         for (i = 0; i < OR_BF; i++)
-            for (v = 0; v < s->nv; v++) {
+            for (v = 0; v < OR_FIELD_NV(s); v++) {
                 d->c[v][i] *= 0.25;
                 d->c[v][i] += 0.75*d->c_old[v][i];
             }
     }
     or_ns_stage(s, dt);
+    OR_PARFOR
     for (e = 0; e < s->n; e++) {
         or_data_t *d = &s->d[e];
+        int v, i;
         for (i = 0; i < OR_BF; i++)
-            for (v = 0; v < s->nv; v++) {
+            for (v = 0; v < OR_FIELD_NV(s); v++) {
                 d->c[v][i] *= 2.;
                 d->c[v][i] /= 3.;
                 d->c[v][i] += d->c_old[v][i] / 3.;
