@@ -578,6 +578,14 @@ extern "C" int charm_gpu_recommended_chunks(const charm_config_t *cfg, int *ce, 
         // at most three passes of the workgroup over the faces (a fourth,
         // partial pass runs with most lanes idle)
         if (faces > 3 * bs) faces = 3 * bs;
+        // the FP32 stage adjoint runs on the same chunks with wider LDS
+        // records: keep two of its workgroups per CU (one halves the
+        // adjoint's occupancy; the forward stage time does not change
+        // between 640 and 768 faces)
+        if (cfg->precision == CHARM_FP32) {
+            const int adj_faces = (int) (lds_cu / 2 / adj_chunk_smem_bytes(1, cfg->n_comp, 1));
+            if (faces > adj_faces) faces = adj_faces;
+        }
         // 128-thread workgroups serve chunks of at most 128 elements
         if (bs == 128 || faces / 3 > 128 || config_physics(cfg)) break;
     }
