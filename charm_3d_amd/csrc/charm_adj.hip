@@ -264,9 +264,10 @@ void k_adj_stage(AdjArgs A, DevParams P)
         }
     }
 
-    R *__restrict__ out = reinterpret_cast<R *>(A.OUT);
-    const R *__restrict__ acc = reinterpret_cast<const R *>(A.ACC);
-    const R *__restrict__ acc2 = reinterpret_cast<const R *>(A.ACC2);
+    // not __restrict__: the last stage adjoint accumulates into its own output
+    R *out = reinterpret_cast<R *>(A.OUT);
+    const R *acc = reinterpret_cast<const R *>(A.ACC);
+    const R *acc2 = reinterpret_cast<const R *>(A.ACC2);
     #pragma unroll
     for (int i = 0; i < 4 * NV; i++) {
         const R sb = R(A.c_in) * (X + i * n)[e];
@@ -453,9 +454,10 @@ void k_adj_chunk(AdjArgs A, DevParams P)
             for (int j = 0; j < 3; j++) A.theta_bar[3 * (size_t) e + j] += th[j];
         }
     }
-    R *__restrict__ out = reinterpret_cast<R *>(A.OUT);
-    const R *__restrict__ acc = reinterpret_cast<const R *>(A.ACC);
-    const R *__restrict__ acc2 = reinterpret_cast<const R *>(A.ACC2);
+    // not __restrict__: the last stage adjoint accumulates into its own output
+    R *out = reinterpret_cast<R *>(A.OUT);
+    const R *acc = reinterpret_cast<const R *>(A.ACC);
+    const R *acc2 = reinterpret_cast<const R *>(A.ACC2);
     #pragma unroll
     for (int i = 0; i < 4 * NV; i++) {
         const R sb = R(A.c_in) * (X + i * n)[e];
@@ -560,7 +562,7 @@ void k_adj_bnd_sum(AdjArgs A)
             for (int j = 0; j < 3; j++) th[j] += A.bnd_theta[3 * (size_t) p + j];
         }
     }
-    R *__restrict__ out = reinterpret_cast<R *>(A.OUT);
+    R *out = reinterpret_cast<R *>(A.OUT);
     const int64_t n = A.n;
     #pragma unroll
     for (int j = 0; j < 4 * NV; j++) (out + j * n)[e] += R(A.c_out) * acc[j];

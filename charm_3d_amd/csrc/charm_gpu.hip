@@ -993,7 +993,8 @@ static int adj_forward(charm_gpu_ctx_t *c, double dt, int nsteps, int ckpt, cons
     char *S0 = seg_last, *S1 = seg_last ? S0 + (size_t) (ckpt + 1) * ssz : NULL;
     char *S2 = seg_last ? S1 + (size_t) ckpt * ssz : NULL;
     for (int s = 0; s < nsteps; s++) {
-        if (cp && s % ckpt == 0)
+        // the last segment's checkpoint is not stored: its states stay in seg_last
+        if (cp && s % ckpt == 0 && s < last)
             HIPCHK(hipMemcpyAsync(cp + (size_t) (s / ckpt) * ssz, c->B[0], ssz, hipMemcpyDeviceToDevice, c->st));
         const void *ucur;
         if (seg_last && s >= last) {
@@ -1231,8 +1232,13 @@ extern "C" int charm_gpu_adjoint(charm_gpu_ctx_t *c, double dt, int nsteps, int 
     // of nsteps steps filled by the forward pass, no recompute); else
     // checkpoints every ckpt steps and segments recomputed in two buffers
     const bool store_all = ckpt == 0;
+    // a period beyond the run is one segment of the whole run
+    if (ckpt > nsteps) ckpt = nsteps;
     const int seg_len = store_all ? nsteps : ckpt;
     const int ncp = store_all ? 1 : (nsteps + ckpt - 1) / ckpt;
+    // checkpoints 0 .. ncp - 2 are stored (the last segment's states are kept
+    // by the forward pass); two segment buffers only when a segment is recomputed
+    const int nseg_buf = ncp > 1 ? 2 : 1;
     char *cp = NULL, *seg = NULL, *la = NULL, *lb = NULL, *lc = NULL, *segb[2] = {NULL, NULL};
     hipStream_t st2 = NULL;
     hipEvent_t ev_ready[2] = {NULL, NULL}, ev_done[2] = {NULL, NULL};
@@ -1246,15 +1252,15 @@ extern "C" int charm_gpu_adjoint(charm_gpu_ctx_t *c, double dt, int nsteps, int 
                          hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
                          adj_chunk_smem_bytes(1, c->nc, c->max_faces) <= (size_t) lds_max;
     (void) hipGetLastError();
-    hipError_t e = hipMalloc((void **) &cp, (size_t) ncp * ssz);
-    if (e == hipSuccess) e = hipMalloc((void **) &seg, (size_t) (store_all ? 1 : 2) * (3 * seg_len + 1) * ssz);
+    hipError_t e = ncp > 1 ? hipMalloc((void **) &cp, (size_t) (ncp - 1) * ssz) : hipSuccess;
+    if (e == hipSuccess) e = hipMalloc((void **) &seg, (size_t) nseg_buf * (3 * seg_len + 1) * ssz);
     if (e == hipSuccess) e = hipStreamCreate(&st2);
     for (int i = 0; i < 2 && e == hipSuccess; i++) {
         e = hipEventCreateWithFlags(&ev_ready[i], hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&ev_done[i], hipEventDisableTiming);
     }
     segb[0] = seg;
-    segb[1] = (seg && !store_all) ? seg + (size_t) (3 * seg_len + 1) * ssz : seg;
+    segb[1] = (seg && nseg_buf == 2) ? seg + (size_t) (3 * seg_len + 1) * ssz : seg;
     if (e == hipSuccess) e = hipMalloc((void **) &la, ssz);
     if (e == hipSuccess) e = hipMalloc((void **) &lb, ssz);
     if (e == hipSuccess) e = hipMalloc((void **) &lc, ssz);

@@ -461,7 +461,8 @@ void k_stage(StageArgs A, DevParams Pv)
             // integral widened, and the update and combine run in FP64)
             const S sc = ldexp(S(1), 3 * lev);
             const S dt = S(A.dt);
-            const S *__restrict__ U0 = reinterpret_cast<const S *>(A.U0);
+            // not __restrict__: the last RK stage writes its output over u^n
+            const S *U0 = reinterpret_cast<const S *>(A.U0);
             #pragma unroll
             for (int v = 0; v < NV; v++)
                 #pragma unroll

@@ -64,7 +64,9 @@ def test_initial_state_gradient_matches_directional_differences(adapted):
 
 def test_checkpoint_period_does_not_change_the_gradient():
     out = []
-    for ckpt in (1, 7, STEPS, 0):                      # 0: every stage state kept, no recompute
+    # 0: every stage state kept, no recompute; 3 * STEPS: a period beyond the
+    # run is one segment of the whole run
+    for ckpt in (1, 7, STEPS, 3 * STEPS, 0):
         cfg, lv, u0, per = case()
         out.append(device(cfg, lv, per, u0).adjoint(DT, STEPS, LO, HI, ckpt=ckpt, theta_face=0))
     for o in out[1:]:
@@ -82,7 +84,7 @@ def test_adjoint_leaves_the_final_state(precision):
     J = ref.objective(DT, STEPS, LO, HI)
     u_end = ref.get_state()
     assert not np.array_equal(u_end, np.asarray(u0, dtype=np.float64).reshape(u_end.shape))   # the steps moved it
-    for ckpt in (1, 5, 7, STEPS, 0):
+    for ckpt in (1, 5, 7, STEPS, 3 * STEPS, 0):
         cfg, lv, u0, per = case(precision)
         ctx = device(cfg, lv, per, u0)
         Ja = ctx.adjoint(DT, STEPS, LO, HI, ckpt=ckpt)[0]
