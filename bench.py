@@ -372,12 +372,13 @@ def bench_bubbles(CA, W):
 
 
 def bench_hot_spot(CA, W):
-    # reacting hot spot: level 6, four species, BR1 viscous terms, diffusion,
+    # reacting hot spot: level 6, four species, BR2 viscous terms, diffusion,
     # chemistry (A = 1e9, molar energy source), 100 steps; FP32 transport with
-    # the FP64 Newton temperature and chemistry, then the FP64 path for its errors
+    # the FP64 Newton temperature and chemistry, then the FP64 path for its
+    # errors, then the FP32 path with the reference's BR1 viscous flux
     runs = {}
-    for prec in (CA.FP32, CA.FP64):
-        cfg, leaves, u0, per = W.c4(6, prec)
+    for prec, eta in ((CA.FP32, W.C4_BR2_ETA), (CA.FP64, W.C4_BR2_ETA), ("br1", 0.0)):
+        cfg, leaves, u0, per = W.c4(6, CA.FP32 if prec == "br1" else prec, visc_br2_eta=eta)
         ctx = CA.GpuContext(cfg)
         ce, cf = CA.recommended_chunks(cfg)
         ctx.upload_mesh(CA.Forest.uniform(6).flatten(ce, cf))
@@ -390,7 +391,7 @@ def bench_hot_spot(CA, W):
         ms = ctx.last_ms()
         runs[prec] = dict(dt=dt, ms=ms, m0=m0, m1=ctx.conservation_sums(), state=ctx.get_state(), u0=u0)
         ctx.close()
-    r, q = runs[CA.FP32], runs[CA.FP64]
+    r, q, b = runs[CA.FP32], runs[CA.FP64], runs["br1"]
     mass = lambda m: m[4:].sum()
     h2o = q["state"][:, 6, 0]
     out = {"leaves": len(leaves), "steps": 100, "ms_per_step": r["ms"] / 100,
@@ -398,6 +399,8 @@ def bench_hot_spot(CA, W):
            "reaction": "2 H2 + O2 -> 2 H2O, k = A exp(-15000 K / T), A = %g; energy source sum_i w_i M_i h_i "
                        "(the reference's sum_i w_i h_i mixes mol and kg units and fails within two steps "
                        "at this A)" % cfg.react[0].a,
+           "viscous_flux": "BR2, eta = %g (the reference's is BR1)" % W.C4_BR2_ETA,
+           "br1": {"ms_per_step": b["ms"] / 100, "l2_rel_vs_br2": rel_errors(b["state"], r["state"])["l2_rel"]},
            "fp64": {"ms_per_step": q["ms"] / 100,
                     "mass_drift": float(abs(mass(q["m1"]) - mass(q["m0"])) / mass(q["m0"])),
                     "h2o_mass_max": float(h2o.max() / q["state"][:, 4:, 0].sum(1)[np.argmax(h2o)])},

@@ -50,7 +50,8 @@ class Config(C.Structure):
                 ("max_err", C.c_double), ("min_level", C.c_int), ("max_level", C.c_int),
                 ("refine_period", C.c_int), ("use_visc", C.c_int), ("use_diff", C.c_int),
                 ("t_ref", C.c_double), ("n_react", C.c_int), ("react", ReactionDesc * 4),
-                ("chem_parity", C.c_int), ("chem_energy_molar", C.c_int), ("write_period", C.c_int),
+                ("chem_parity", C.c_int), ("chem_energy_molar", C.c_int),
+                ("visc_br2_eta", C.c_double), ("write_period", C.c_int),
                 ("comp_name", (C.c_char * 32) * MAX_COMP), ("out_prefix", C.c_char * 256)]
 
 
@@ -194,12 +195,14 @@ def make_config(precision=FP32, model="NS", flux="HLLC", limiter=False, eos="IDE
                 grav=(0.0, 0.0, 0.0), cfl=0.3, dt=0.0, tmax=0.0, chunk_elems=0, chunk_faces=0,
                 max_err=0.0, min_level=0, max_level=0, refine_period=0, transport=None,
                 use_visc=False, use_diff=False, t_ref=298.15, reactions=(), chem_parity=False,
-                chem_energy_molar=False, write_period=0, comp_names=(), out_prefix=""):
+                chem_energy_molar=False, visc_br2_eta=0.0, write_period=0, comp_names=(),
+                out_prefix=""):
     """charm_config_t from task.yaml-like keys.  comp_cp entries are a constant
     cp or a tuple of power-series coefficients (polynomial cp, which as in
     the reference takes effect with Sutherland viscosity, ml_type 1);
     transport: per species dict of charm_comp_desc_t fields (ml0, kp0, t0,
-    ts, h0, ml_type, kp_type, sig, ek); reactions: (left[3], right[3], a, e, n)."""
+    ts, h0, ml_type, kp_type, sig, ek); reactions: (left[3], right[3], a, e, n);
+    visc_br2_eta > 0: BR2 viscous flux with that penalty factor (0: BR1)."""
     cfg = Config()
     cfg.precision = precision
     cfg.model = MODEL[model]
@@ -239,6 +242,7 @@ def make_config(precision=FP32, model="NS", flux="HLLC", limiter=False, eos="IDE
         cfg.react[j].a, cfg.react[j].e, cfg.react[j].n = a, e, n
     cfg.chem_parity = int(chem_parity)
     cfg.chem_energy_molar = int(chem_energy_molar)
+    cfg.visc_br2_eta = visc_br2_eta
     cfg.write_period = write_period
     for i, nm in enumerate(comp_names):
         cfg.comp_name[i].value = nm.encode()[:31]

@@ -52,6 +52,8 @@ struct DevParams {
     int    n_react, chem_parity, chem_energy_molar;
     int    r_left[4][3], r_right[4][3];
     double r_a[4], r_e[4], r_n[4];
+    double br2_eta;                    // BR2 penalty factor (0: BR1)
+    float  br2_eta_f;
 };
 
 // c ? a : b on values (the dual-number overload selects component-wise, so

@@ -41,6 +41,7 @@ struct charm_gpu_ctx {
     // Navier-Stokes buffers (allocated when ph)
     void *d_auxf;             // [36][n] tau, q coefficients
     void *d_vi;               // [16][n] viscous integrals
+    void *d_auxo;             // [36][n] BR2: the own-trace tau, q (allocated with visc_br2_eta > 0)
     double *d_dcoef;          // [nc][n] diffusion coefficients
     double *d_chem;           // [n] chemistry energy source
     // state
@@ -74,7 +75,7 @@ static void free_mesh(charm_gpu_ctx_t *c)
                   (void **) &c->d_ch_ebeg, (void **) &c->d_ch_fbeg, (void **) &c->d_slot_ptr,
                   (void **) &c->d_nbr, (void **) &c->d_slots, &c->B[0], &c->B[1], &c->B[2],
                   (void **) &c->d_stage64, (void **) &c->d_partial, (void **) &c->d_q,
-                  &c->d_auxf, &c->d_vi, (void **) &c->d_dcoef, (void **) &c->d_chem};
+                  &c->d_auxf, &c->d_vi, &c->d_auxo, (void **) &c->d_dcoef, (void **) &c->d_chem};
     for (size_t i = 0; i < sizeof(p) / sizeof(p[0]); i++)
         if (*p[i]) { (void) hipFree(*p[i]); *p[i] = NULL; }
     free(c->hq);
@@ -111,6 +112,10 @@ extern "C" int charm_gpu_create(const charm_config_t *cfg, charm_gpu_ctx_t **out
     for (int i = 0; i < 6; i++) {
         int t = cfg->bnd_type[i];
         if (t != CHARM_BND_PERIODIC && (t < 0 || t > 4)) { charm_set_error("gpu_create: bad boundary type %d", t); return CHARM_ERR_ARG; }
+    }
+    if (!(cfg->visc_br2_eta >= 0.0 && cfg->visc_br2_eta < 1e6)) {
+        charm_set_error("gpu_create: BR2 penalty factor %g (0 for BR1, else positive)", cfg->visc_br2_eta);
+        return CHARM_ERR_ARG;
     }
     int ndev = 0;
     HIPCHK(hipGetDeviceCount(&ndev));
@@ -167,6 +172,8 @@ extern "C" int charm_gpu_create(const charm_config_t *cfg, charm_gpu_ctx_t **out
     P.n_react = cfg->n_react;
     P.chem_parity = cfg->chem_parity != 0;
     P.chem_energy_molar = cfg->chem_energy_molar != 0;
+    P.br2_eta = cfg->visc_br2_eta;
+    P.br2_eta_f = (float) cfg->visc_br2_eta;
     for (int r = 0; r < cfg->n_react; r++) {
         for (int m = 0; m < 3; m++) { P.r_left[r][m] = cfg->react[r].left[m]; P.r_right[r][m] = cfg->react[r].right[m]; }
         P.r_a[r] = cfg->react[r].a; P.r_e[r] = cfg->react[r].e; P.r_n[r] = cfg->react[r].n;
@@ -278,6 +285,7 @@ extern "C" int charm_gpu_upload_mesh(charm_gpu_ctx_t *c, const charm_mesh_t *m)
     if (c->ph) {
         DMALLOC(c->d_auxf, (size_t) m->n * 36 * c->esz);
         DMALLOC(c->d_vi, (size_t) m->n * 16 * c->esz);
+        if (c->cfg.use_visc && c->cfg.visc_br2_eta > 0.0) DMALLOC(c->d_auxo, (size_t) m->n * 36 * c->esz);
         DMALLOC(c->d_dcoef, (size_t) m->n * c->nc * sizeof(double));
         DMALLOC(c->d_chem, (size_t) m->n * sizeof(double));
     }
@@ -385,10 +393,25 @@ static NsArgs ns_args(charm_gpu_ctx_t *c, const void *U, void *out, double dt)
     A.U = U; A.out = out; A.level = c->d_level; A.nbr = c->d_nbr; A.aux = c->d_aux;
     A.chem_rhs = c->d_chem;
     A.dcoef = (c->cfg.use_diff && c->nc >= 2) ? c->d_dcoef : NULL;
-    A.aux_in = c->d_auxf; A.aux_out = c->d_auxf; A.visc_out = c->d_vi;
+    A.aux_in = c->d_auxf; A.aux_out = c->d_auxf; A.visc_out = c->d_vi; A.aux_own = c->d_auxo;
     A.n = c->n; A.dt = dt;
     A.prm = c->d_P;
     return A;
+}
+
+// The viscous passes on the state A.U: diffusion coefficients (DIFFUSION and
+// two or more species), the BR1 auxiliary tau / q, with BR2 also the
+// own-trace auxiliary field, then the viscous integrals (BR1 or BR2 surface
+// flux).  Launch codes of launch_ns: 1 diffusion, 2 / 4 auxiliary field
+// (BR1 / own trace), 3 / 5 viscous integrals (BR1 / BR2).
+static hipError_t ns_viscous(charm_gpu_ctx_t *c, const NsArgs &A)
+{
+    hipError_t e = hipSuccess;
+    const bool br2 = c->cfg.visc_br2_eta > 0.0;
+    if (A.dcoef && (e = launch_ns(c->sprec, 1, c->nc, A, c->P, c->st)) != hipSuccess) return e;
+    if ((e = launch_ns(c->sprec, 2, c->nc, A, c->P, c->st)) != hipSuccess) return e;
+    if (br2 && (e = launch_ns(c->sprec, 4, c->nc, A, c->P, c->st)) != hipSuccess) return e;
+    return launch_ns(c->sprec, br2 ? 5 : 3, c->nc, A, c->P, c->st);
 }
 
 // The Navier-Stokes pieces in front of a stage (src/models/ns/charm_model.c:
@@ -408,9 +431,7 @@ static hipError_t ns_prepare(charm_gpu_ctx_t *c, const void *in, void *chem_out,
     }
     if (c->cfg.use_visc) {
         NsArgs A = ns_args(c, *stage_in, NULL, dt);
-        if (A.dcoef && (e = launch_ns(c->sprec, 1, c->nc, A, c->P, c->st)) != hipSuccess) return e;
-        if ((e = launch_ns(c->sprec, 2, c->nc, A, c->P, c->st)) != hipSuccess) return e;
-        if ((e = launch_ns(c->sprec, 3, c->nc, A, c->P, c->st)) != hipSuccess) return e;
+        if ((e = ns_viscous(c, A)) != hipSuccess) return e;
     }
     return hipSuccess;
 }
@@ -421,10 +442,7 @@ extern "C" int charm_gpu_residual(charm_gpu_ctx_t *c, double *out, int64_t n)
     if (rc) return rc;
     if (c->ph && c->cfg.model == CHARM_MODEL_NS && c->cfg.use_visc) {
         // viscous passes on the current state, without the chemistry
-        NsArgs N = ns_args(c, c->B[0], NULL, 0.0);
-        if (N.dcoef) HIPCHK(launch_ns(c->sprec, 1, c->nc, N, c->P, c->st));
-        HIPCHK(launch_ns(c->sprec, 2, c->nc, N, c->P, c->st));
-        HIPCHK(launch_ns(c->sprec, 3, c->nc, N, c->P, c->st));
+        HIPCHK(ns_viscous(c, ns_args(c, c->B[0], NULL, 0.0)));
     }
     StageArgs A = stage_args(c, c->B[0], c->B[0], c->B[1], 0, 0, 0.0);
     HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st));

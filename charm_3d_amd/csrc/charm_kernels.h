@@ -92,6 +92,7 @@ struct NsArgs {
     const void *aux_in;       // [36][n] tau, q coefficients (viscous integrals)
     void *aux_out;            // [36][n] (gradient pass)
     void *visc_out;           // [16][n] viscous integrals of ru rv rw re
+    void *aux_own;            // [36][n] BR2: own-trace tau, q (written by code 4, read by code 5)
     int64_t n;
     double dt;
     const DevParams *prm;     // the context's constants in device memory (read by every NS kernel)

@@ -69,6 +69,40 @@ __device__ __forceinline__ void state_at(const DevParams &P, const R *u, const R
     cons_to_prim<R, NC, 1>(P, t, p);
 }
 
+// velocity of the P1 state at reference point ph, formed as cons_to_prim
+// forms it (the momenta over rho = sum of the partial densities)
+template <typename R, int NC>
+__device__ __forceinline__ void velocity_at(const R *u, const R ph[3], R vel[3])
+{
+    R r = at<R>(&u[16], ph[0], ph[1], ph[2]);
+    #pragma unroll
+    for (int i = 1; i < NC; i++) r += at<R>(&u[4 * (4 + i)], ph[0], ph[1], ph[2]);
+    const R ri = rcp(r);
+    #pragma unroll
+    for (int d = 0; d < 3; d++) vel[d] = at<R>(&u[4 * d], ph[0], ph[1], ph[2]) * ri;
+}
+
+// face terms of the gradient pass at one point
+// (src/models/ns/charm_model_ns_diff_grad.c:279-298): q6 = the stress of the
+// velocity (fu, fv, fw) along nn, qq = (k T + sum_i h_i(T) rho D_i Y_i) nn, at
+// the coefficients lp, lm, mu, kt, dm
+template <typename R, int NC>
+__device__ __forceinline__ void grad_terms(const DevParams &P, R lp, R lm, R mu, R kt, R fu, R fv, R fw, R ft,
+                                           R fr, const R *fc, const double *dm, const R nn[3], R q6[6], R qq[3])
+{
+    q6[0] = lp * fu * nn[0] + lm * fv * nn[1] + lm * fw * nn[2];
+    q6[1] = lm * fu * nn[0] + lp * fv * nn[1] + lm * fw * nn[2];
+    q6[2] = lm * fu * nn[0] + lm * fv * nn[1] + lp * fw * nn[2];
+    q6[3] = mu * (fu * nn[1] + fv * nn[0]);
+    q6[4] = mu * (fw * nn[0] + fu * nn[2]);
+    q6[5] = mu * (fw * nn[1] + fv * nn[2]);
+    double sp = 0.0;
+    #pragma unroll
+    for (int i = 0; i < NC; i++) sp += enthalpy(P, i, (double) ft) * (double) fr * dm[i] * (double) fc[i];
+    #pragma unroll
+    for (int d = 0; d < 3; d++) qq[d] = kt * ft * nn[d] + (R) sp * nn[d];
+}
+
 } // namespace
 
 // Chemistry on the cell means (_charm_model_ns_chem_iter_fn,
@@ -212,8 +246,10 @@ void k_ns_diffusion(NsArgs A, DevParams)
 // 21-507): int = -int f(u) . grad phi + sum_faces int avg(f) . n phi, then
 // A^-1 (diagonal on the cube octree).  tau = (lambda + 4 mu / 3, lambda - 2 mu
 // / 3, mu) applied to the velocity (lambda = 0), q = -(k T + sum_i h_i rho D_i
-// Y_i) with the sign folded as in the reference.
-template <typename S, int NC>
+// Y_i) with the sign folded as in the reference.  OWN (the BR2 option): the
+// interior face terms from the leaf's own trace at the same face-averaged
+// coefficients, into aux_own: tau_own = tau_h less the interior liftings.
+template <typename S, int NC, bool OWN>
 __global__ __launch_bounds__(128)
 void k_ns_grad(NsArgs A, DevParams)
 {
@@ -277,17 +313,7 @@ void k_ns_grad(NsArgs A, DevParams)
     };
     auto terms = [&](R lp, R lm, R mu, R kt, R fu, R fv, R fw, R ft, R fr, const R *fc, const double *dm,
                      const R nn[3], R q6[6], R qq[3]) {
-        q6[0] = lp * fu * nn[0] + lm * fv * nn[1] + lm * fw * nn[2];
-        q6[1] = lm * fu * nn[0] + lp * fv * nn[1] + lm * fw * nn[2];
-        q6[2] = lm * fu * nn[0] + lm * fv * nn[1] + lp * fw * nn[2];
-        q6[3] = mu * (fu * nn[1] + fv * nn[0]);
-        q6[4] = mu * (fw * nn[0] + fu * nn[2]);
-        q6[5] = mu * (fw * nn[1] + fv * nn[2]);
-        double sp = 0.0;
-        #pragma unroll
-        for (int i = 0; i < NC; i++) sp += enthalpy(P, i, (double) ft) * (double) fr * dm[i] * (double) fc[i];
-        #pragma unroll
-        for (int d = 0; d < 3; d++) qq[d] = kt * ft * nn[d] + (R) sp * nn[d];
+        grad_terms<R, NC>(P, lp, lm, mu, kt, fu, fv, fw, ft, fr, fc, dm, nn, q6, qq);
     };
     face_points<R>(A.nbr, A.aux, A.level, e, h,
         [&](int d, int nb, const R ps[3], const R pn[3], R wj, R sign) {
@@ -313,8 +339,11 @@ void k_ns_grad(NsArgs A, DevParams)
             const int a = d >> 1;
             const R nn[3] = {a == 0 ? R(1) : R(0), a == 1 ? R(1) : R(0), a == 2 ? R(1) : R(0)};
             R q6[6], qq[3];
-            terms(lp, lm, fmu, kt, R(0.5) * (pa.u + pb.u), R(0.5) * (pa.v + pb.v), R(0.5) * (pa.w + pb.w),
-                  ft, R(0.5) * (pa.r + pb.r), fc, dm, nn, q6, qq);
+            if constexpr (OWN)
+                terms(lp, lm, fmu, kt, pa.u, pa.v, pa.w, dv(pa.e, pa.cv), pa.r, pa.c, dm, nn, q6, qq);
+            else
+                terms(lp, lm, fmu, kt, R(0.5) * (pa.u + pb.u), R(0.5) * (pa.v + pb.v), R(0.5) * (pa.w + pb.w),
+                      ft, R(0.5) * (pa.r + pb.r), fc, dm, nn, q6, qq);
             add(q6, qq, ps, sign * wj);
         },
         [&](int d, int face, int, const R ps[3], R wj) {
@@ -342,7 +371,7 @@ void k_ns_grad(NsArgs A, DevParams)
             terms(lp, lm, mu, kt, fu, fv, fw, ft, R(0.5) * (pa.r + pb.r), fc, ds, nn, q6, qq);
             add(q6, qq, ps, wj);
         });
-    R *__restrict__ aux = reinterpret_cast<R *>(A.aux_out);
+    R *__restrict__ aux = reinterpret_cast<R *>(OWN ? A.aux_own : A.aux_out);
     #pragma unroll
     for (int k = 0; k < 4; k++) {
         const R ai = cst<R>(P.ainv0[k], P.ainv0_f[k]) * ldexp(R(1), 3 * lev);
@@ -356,7 +385,14 @@ void k_ns_grad(NsArgs A, DevParams)
 // energy) - sum_faces int (avg(tau) . n, avg(tau u) . n - avg(q) . n) phi;
 // boundary faces contribute only on no-slip walls, one point at the face
 // centre with the cell-mean velocity: -(-mu v_t / |l|, -q . n).
-template <typename S, int NC>
+// BR2 (no counterpart in the reference): on an interior (sub)face each
+// side's stress and heat flux are its own-trace auxiliary field plus eta
+// times the (sub)face's lifting
+//   l_s = A_s^-1 sum_points sgn_s [Q(avg) - Q(own_s)] phi_s wJ,
+// Q the gradient pass's face term at the face-averaged coefficients, from
+// the averaged and from side s's own trace; the volume and boundary terms
+// are BR1's.  Of q only the normal component enters the flux.
+template <typename S, int NC, bool BR2>
 __global__ __launch_bounds__(128)
 void k_ns_visc(NsArgs A, DevParams)
 {
@@ -409,8 +445,114 @@ void k_ns_visc(NsArgs A, DevParams)
             #pragma unroll
             for (int v = 0; v < 4; v++) acc[v][k] -= q[v] * phi[k] * wj;
     };
+    if constexpr (BR2) {
+        double ds[NC];
+        #pragma unroll
+        for (int i = 0; i < NC; i++) ds[i] = A.dcoef ? A.dcoef[i * n + e] : 0.0;
+        const R *__restrict__ XO = reinterpret_cast<const R *>(A.aux_own);
+        const R eta = cst<R>(P.br2_eta, P.br2_eta_f);
+        face_subfaces<R>(A.nbr, A.aux, A.level, e, h,
+            [&](int d, int nb, R so1, R so2, R sm, R no1, R no2, R nm, R wj, R sign) {
+                const int a = d >> 1;
+                R un[4 * NV];
+                load_coeffs<R, NV>(U, n, nb, un);
+                const R nn[3] = {a == 0 ? R(1) : R(0), a == 1 ? R(1) : R(0), a == 2 ? R(1) : R(0)};
+                // [side][field][coefficient]: side 0 = e, 1 = nb; fields tau xx yy zz
+                // xy xz yz and q_a.  First the liftings, then tau_own + eta l.
+                R cs[2][7][4];
+                #pragma unroll
+                for (int s = 0; s < 2; s++)
+                    #pragma unroll
+                    for (int f = 0; f < 7; f++)
+                        #pragma unroll
+                        for (int k = 0; k < 4; k++) cs[s][f][k] = R(0);
+                for (int k = 0; k < 6; k++) {
+                    R ps[3], pn[3];
+                    face_point<R>(d, k, so1, so2, sm, ps);
+                    face_point<R>(d ^ 1, k, no1, no2, nm, pn);
+                    Prim<R, NC> pa, pb;
+                    state_at<R, NC>(P, u, ps, pa);
+                    state_at<R, NC>(P, un, pn, pb);
+                    R mua, kta, mub, ktb;
+                    transport<R, NC>(P, pa, mua, kta);
+                    transport<R, NC>(P, pb, mub, ktb);
+                    const R fmu = R(0.5) * (mua + mub), kt = R(0.5) * (kta + ktb);
+                    const R lp = R(4.0 / 3.0) * fmu, lm = R(-2.0 / 3.0) * fmu;
+                    const R ta = dv(pa.e, pa.cv), tb = dv(pb.e, pb.cv);
+                    R fc[NC];
+                    double dm[NC];
+                    #pragma unroll
+                    for (int i = 0; i < NC; i++) {
+                        fc[i] = R(0.5) * (pa.c[i] + pb.c[i]);
+                        dm[i] = 0.5 * (ds[i] + (A.dcoef ? A.dcoef[i * n + nb] : 0.0));
+                    }
+                    R q6[6], qq[3], q6a[6], qqa[3], q6b[6], qqb[3];
+                    grad_terms<R, NC>(P, lp, lm, fmu, kt, R(0.5) * (pa.u + pb.u), R(0.5) * (pa.v + pb.v),
+                                      R(0.5) * (pa.w + pb.w), R(0.5) * (ta + tb), R(0.5) * (pa.r + pb.r),
+                                      fc, dm, nn, q6, qq);
+                    grad_terms<R, NC>(P, lp, lm, fmu, kt, pa.u, pa.v, pa.w, ta, pa.r, pa.c, dm, nn, q6a, qqa);
+                    grad_terms<R, NC>(P, lp, lm, fmu, kt, pb.u, pb.v, pb.w, tb, pb.r, pb.c, dm, nn, q6b, qqb);
+                    const R qn = pick3(a, qq[0], qq[1], qq[2]);
+                    const R qna = pick3(a, qqa[0], qqa[1], qqa[2]), qnb = pick3(a, qqb[0], qqb[1], qqb[2]);
+                    const R phs[4] = {R(1), ps[0], ps[1], ps[2]}, phn[4] = {R(1), pn[0], pn[1], pn[2]};
+                    #pragma unroll
+                    for (int c = 0; c < 4; c++) {
+                        // e adds with its side's sign, nb with the other
+                        const R ba = phs[c] * sign * wj, bb = -phn[c] * sign * wj;
+                        #pragma unroll
+                        for (int f = 0; f < 6; f++) {
+                            cs[0][f][c] += (q6[f] - q6a[f]) * ba;
+                            cs[1][f][c] += (q6[f] - q6b[f]) * bb;
+                        }
+                        cs[0][6][c] -= (qn - qna) * ba;
+                        cs[1][6][c] -= (qn - qnb) * bb;
+                    }
+                }
+                // coefficients of tau_own + eta l on each side (A^-1 of the side's level)
+                const int lv[2] = {lev, (int) A.level[nb]};
+                const int el[2] = {e, nb};
+                #pragma unroll
+                for (int s = 0; s < 2; s++)
+                    #pragma unroll
+                    for (int c = 0; c < 4; c++) {
+                        const R ai = cst<R>(P.ainv0[c], P.ainv0_f[c]) * ldexp(R(1), 3 * lv[s]) * eta;
+                        #pragma unroll
+                        for (int f = 0; f < 6; f++) cs[s][f][c] = (XO + (4 * f + c) * n)[el[s]] + cs[s][f][c] * ai;
+                        cs[s][6][c] = (XO + (24 + 4 * a + c) * n)[el[s]] + cs[s][6][c] * ai;
+                    }
+                // the BR1 surface flux with these stresses
+                for (int k = 0; k < 6; k++) {
+                    R ps[3], pn[3];
+                    face_point<R>(d, k, so1, so2, sm, ps);
+                    face_point<R>(d ^ 1, k, no1, no2, nm, pn);
+                    R ftau[6], fu[3] = {R(0), R(0), R(0)}, ft = R(0);
+                    #pragma unroll
+                    for (int f = 0; f < 6; f++) ftau[f] = R(0);
+                    #pragma unroll
+                    for (int s = 0; s < 2; s++) {
+                        const R *pp = s ? pn : ps;
+                        R vel[3];
+                        velocity_at<R, NC>(s ? un : u, pp, vel);
+                        R t[6];
+                        #pragma unroll
+                        for (int f = 0; f < 6; f++) { t[f] = at<R>(cs[s][f], pp[0], pp[1], pp[2]); ftau[f] += t[f]; }
+                        fu[0] += vel[0] * t[0] + vel[1] * t[3] + vel[2] * t[4];
+                        fu[1] += vel[0] * t[3] + vel[1] * t[1] + vel[2] * t[5];
+                        fu[2] += vel[0] * t[4] + vel[1] * t[5] + vel[2] * t[2];
+                        ft += at<R>(cs[s][6], pp[0], pp[1], pp[2]);
+                    }
+                    R q[4];
+                    q[0] = R(0.5) * pick3(a, ftau[0], ftau[3], ftau[4]);
+                    q[1] = R(0.5) * pick3(a, ftau[3], ftau[1], ftau[5]);
+                    q[2] = R(0.5) * pick3(a, ftau[4], ftau[5], ftau[2]);
+                    q[3] = R(0.5) * pick3(a, fu[0], fu[1], fu[2]) - R(0.5) * ft;
+                    add(q, ps, sign * wj);
+                }
+            });
+    }
     face_points<R>(A.nbr, A.aux, A.level, e, h,
         [&](int d, int nb, const R ps[3], const R pn[3], R wj, R sign) {
+            if constexpr (BR2) return;               // the interior faces are done above
             R un[4 * NV], xn[36];
             load_coeffs<R, NV>(U, n, nb, un);
             #pragma unroll
@@ -484,8 +626,10 @@ static hipError_t launch_ns_r(int what, int nc, const NsArgs &A, const DevParams
     case NCV:                                                                                      \
         if (what == 0) hipLaunchKernelGGL((k_ns_chem<S, NCV>), grid, block, 0, st, A, P);          \
         else if (what == 1) hipLaunchKernelGGL((k_ns_diffusion<S, NCV>), grid, block, 0, st, A, P);\
-        else if (what == 2) hipLaunchKernelGGL((k_ns_grad<S, NCV>), grid, block, 0, st, A, P);     \
-        else hipLaunchKernelGGL((k_ns_visc<S, NCV>), grid, block, 0, st, A, P);                    \
+        else if (what == 2) hipLaunchKernelGGL((k_ns_grad<S, NCV, false>), grid, block, 0, st, A, P); \
+        else if (what == 4) hipLaunchKernelGGL((k_ns_grad<S, NCV, true>), grid, block, 0, st, A, P);  \
+        else if (what == 3) hipLaunchKernelGGL((k_ns_visc<S, NCV, false>), grid, block, 0, st, A, P); \
+        else hipLaunchKernelGGL((k_ns_visc<S, NCV, true>), grid, block, 0, st, A, P);               \
         break;
     switch (nc) {
     CHARM_NS_CASE(1)

@@ -226,9 +226,12 @@ C4_H0 = (0.0, 0.0, -241826.0 / 0.018015, 0.0)            # formation enthalpy at
 C4_ML = ((8.41e-6, 72.0), (1.919e-5, 127.0), (1.12e-5, 1064.0), (1.663e-5, 111.0))   # mu(273 K), S
 C4_LJ = ((2.92, 38.0), (3.458, 107.4), (2.605, 572.4), (3.621, 97.53))               # sigma A, eps/k K
 C4_Y = (0.02, 0.22, 0.0, 0.76)
+# BR2 viscous flux (the case names a BR2-type flux; the reference's own is
+# BR1): penalty factor = the number of faces of a hexahedron
+C4_BR2_ETA = 6.0
 
 
-def c4_config(precision=FP32, limiter=False, a=None, chem_energy_molar=True):
+def c4_config(precision=FP32, limiter=False, a=None, chem_energy_molar=True, visc_br2_eta=C4_BR2_ETA):
     cps = tuple(tuple(a * GAS_R / m for a in fit) for fit, m in zip(c4_cp_over_r(), C4_M))
     t0 = 273.0
     tr = []
@@ -241,7 +244,7 @@ def c4_config(precision=FP32, limiter=False, a=None, chem_energy_molar=True):
     return make_config(precision=precision, model="NS", flux="HLLC", limiter=limiter, eos="MIX",
                        comp_m=C4_M, comp_cp=cps, bnd=("OUTLET",) * 6, cfl=0.3, transport=tr,
                        use_visc=True, use_diff=True, t_ref=298.15, reactions=rx,
-                       chem_energy_molar=chem_energy_molar)
+                       chem_energy_molar=chem_energy_molar, visc_br2_eta=visc_br2_eta)
 
 
 def c4_state_fn(cfg, r0=0.1, p=101325.0):
@@ -267,8 +270,9 @@ def c4_state_fn(cfg, r0=0.1, p=101325.0):
 
 def c4(level=6, precision=FP32, **kw):
     """Reacting hot spot: level-6 uniform cube (262,144 leaves), four
-    species, HLLC + BR1 viscous terms + diffusion + chemistry, SSP-RK3.
-    kw: c4_config's a (rate constant) and chem_energy_molar."""
+    species, HLLC + BR2 viscous terms + diffusion + chemistry, SSP-RK3.
+    kw: c4_config's a (rate constant), chem_energy_molar and visc_br2_eta
+    (0: the reference's BR1)."""
     leaves = Forest.uniform(level).leaves()
     cfg = c4_config(precision, **kw)
     return cfg, leaves, project_p1(leaves, c4_state_fn(cfg), 8), (False, False, False)

@@ -117,6 +117,11 @@ typedef struct {
                                   enthalpy (J/kg) (src/models/ns/charm_model_ns_chem.c:
                                   66-108); 1 sum_i w_i M_i h_i (J/m^3/s), the rate times
                                   the molar enthalpy */
+    double visc_br2_eta;       /* viscous flux on interior faces: 0 the reference's BR1
+                                  (src/models/ns/charm_model_ns_diff_integrals.c:202-382);
+                                  > 0 BR2 with this penalty factor: each side's stress is
+                                  its own-trace auxiliary field plus eta times the face's
+                                  local lifting (no counterpart in the reference) */
     /* output (src/charm_init_yaml.cpp:388, include/charm_globals.h:294) */
     int    write_period;       /* FILE_OUTPUT_STEP: VTK files every that many steps (0: none) */
     char   comp_name[CHARM_MAX_COMP][32];   /* components' names: cell data C_<name> */

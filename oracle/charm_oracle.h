@@ -89,6 +89,8 @@ typedef struct {
     int    chem_parity;        /* 1: the reference's chem_w[species] index (see or_ns.c) */
     int    chem_energy_molar;  /* 1: energy source sum w_i M_i h_i instead of the
                                   reference's sum w_i h_i (see or_ns.c) */
+    double visc_br2_eta;       /* 0: the reference's BR1 viscous flux; > 0: BR2
+                                  lifting with this penalty factor (see or_ns.c) */
 } or_config_t;
 
 typedef struct or_sim or_sim_t;
@@ -116,6 +118,8 @@ void or_run(or_sim_t *s, double dt, int nsteps);
  * coefficients [n][n_comp], chemistry applied to the state (chem_rhs [n]) */
 void or_ns_residual(or_sim_t *s, double *out);
 void or_ns_aux(or_sim_t *s, double *out);
+void or_ns_aux_own(or_sim_t *s, double *out);
+void or_ns_visc(or_sim_t *s, double *out);
 void or_ns_diff_coeffs(or_sim_t *s, double *out);
 void or_ns_chem_apply(or_sim_t *s, double dt, double *chem_rhs);
 

@@ -71,6 +71,12 @@ typedef struct {
     double chem_rhs;
     double int_tau[6][OR_BF];
     double int_q[3][OR_BF];
+    /* BR2 option: the auxiliary field with every interior face term taken
+     * from the leaf's own trace (tau_h less the interior face liftings) */
+    double tau_own[6][OR_BF];
+    double q_own[3][OR_BF];
+    double int_tau_own[6][OR_BF];
+    double int_q_own[3][OR_BF];
     int32_t quad[4];                         /* x y z level */
 } or_data_t;
 

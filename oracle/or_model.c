@@ -644,6 +644,17 @@ void or_ns_residual(or_sim_t *s, double *out)
             for (k = 0; k < OR_BF; k++) out[(e*s->nv + v)*OR_BF + k] = s->d[e].integ[v][k];
 }
 
+/* the viscous integrals alone: [n][4][OR_BF] for ru rv rw re */
+void or_ns_visc(or_sim_t *s, double *out)
+{
+    int64_t e; int v, k;
+    or_zero(s);
+    or_ns_diff(s);
+    for (e = 0; e < s->n; e++)
+        for (v = 0; v < 4; v++)
+            for (k = 0; k < OR_BF; k++) out[(e*4 + v)*OR_BF + k] = s->d[e].integ[v][k];
+}
+
 void or_conv_residual(or_sim_t *s, double *out)
 {
     int64_t e; int v, k;

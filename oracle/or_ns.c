@@ -11,6 +11,19 @@
  *   chemistry                                src/models/ns/charm_model_ns_chem.c:18-202
  * on the oracle's own face traversal.  Periodic faces shift the far side's
  * evaluation point as in the convective pass.
+ *
+ * BR2 option (visc_br2_eta > 0; the reference has BR1 only).  Bassi and
+ * Rebay's second scheme with the local lifting of each interior (sub)face,
+ * written in the reference's tau / q projection.  The gradient pass's face
+ * term at a point is Q(state) . n with the face-averaged coefficients (mu,
+ * k, D); BR1 takes the averaged state.  With Q(own_i), the same term from
+ * side i's own trace, side i's lifting of face f is
+ *   l_i,f = A_i^-1 sum_points sgn_i [Q(avg) - Q(own_i)] phi_i wJ,
+ * and tau_h = tau_own + sum_f l_f, where tau_own takes every interior face
+ * term from the leaf's own trace (volume and boundary terms as BR1).  The
+ * volume integrals keep tau_h; an interior face uses tau_own + eta l_f on
+ * each side in the BR1 surface flux (averages of tau, tau u and q).  The
+ * boundary terms (the no-slip wall's) are the reference's.
  */
 #include "or_internal.h"
 
@@ -64,6 +77,15 @@ static void or_q_at(const or_data_t *d, const double x[3], double q[3])
     }
 }
 
+/* value of one field (OR_BF coefficients) of leaf d at x */
+static double or_coef_at(const or_data_t *d, const double c[OR_BF], const double x[3])
+{
+    double v = 0.;
+    int k;
+    for (k = 0; k < OR_BF; k++) v += c[k] * or_bf(&d->g, x, k);
+    return v;
+}
+
 /* ------------------------------------------------------------ diffusion */
 
 /* src/models/ns/charm_model_ns_diff.c:18-71 */
@@ -110,6 +132,14 @@ static void or_grad_add(or_data_t *d, int ibf, const double qt6[6], const double
     int i;
     for (i = 0; i < 6; i++) d->int_tau[i][ibf] += qt6[i] * bfv;
     for (i = 0; i < 3; i++) d->int_q[i][ibf] -= qq[i] * bfv;
+}
+
+/* the same into the BR2 own-trace accumulators */
+static void or_grad_add_own(or_data_t *d, int ibf, const double qt6[6], const double qq[3], double bfv)
+{
+    int i;
+    for (i = 0; i < 6; i++) d->int_tau_own[i][ibf] += qt6[i] * bfv;
+    for (i = 0; i < 3; i++) d->int_q_own[i][ibf] -= qq[i] * bfv;
 }
 
 /* src/models/ns/charm_model_ns_diff_grad.c:21-74 */
@@ -201,44 +231,76 @@ static void or_grad_bnd(or_sim_t *s, const or_face_t *f)
         fr = (prim[0].r + prim[1].r)*0.5;
         for (i = 0; i < s->cfg.n_comp; i++) fc[i] = (prim[0].c[i] + prim[1].c[i])*0.5;
         or_grad_face_terms(s, lp, lm, mu, kt, fu, fv, fw, ft, fr, fc, d->d, n, q6, qq);
-        for (ibf = 0; ibf < OR_BF; ibf++)
+        for (ibf = 0; ibf < OR_BF; ibf++) {
             or_grad_add(d, ibf, q6, qq, or_bf(&d->g, x, ibf) * gw * gj);
+            if (s->cfg.visc_br2_eta > 0.) or_grad_add_own(d, ibf, q6, qq, or_bf(&d->g, x, ibf) * gw * gj);
+        }
     }
 }
 
 /* one face point of an interior face, src/models/ns/charm_model_ns_diff_grad.c:
  * 253-330 (hanging) and 358-429 (conforming) */
-static void or_grad_point(or_sim_t *s, or_data_t *ud[2], const double xs[2][3], const double n[3],
-                          double wj)
+/* one interior face point of the gradient pass: both traces, the averaged
+ * coefficients and the face terms q6, qq from the averaged state */
+typedef struct {
+    or_prim_t prim[2];
+    double    lp, lm, fmu, kt, dm[OR_NC_MAX];
+    double    q6[6], qq[3];
+} or_gpt_t;
+
+static void or_grad_point_terms(const or_sim_t *s, or_data_t *ud[2], const double xs[2][3],
+                                const double n[3], or_gpt_t *t)
 {
-    or_cons_t cons[2]; or_prim_t prim[2];
-    double mu[2], lambda[2], kt = 0., fmu, flambda, lp, lm, fr, ft, fu, fv, fw;
-    double fc[OR_NC_MAX], dm[OR_NC_MAX], q6[6], qq[3];
-    int i, ibf;
+    or_cons_t cons[2];
+    double mu[2], lambda[2], kt = 0., flambda, fr, ft, fu, fv, fw, fc[OR_NC_MAX];
+    int i;
     for (i = 0; i < 2; i++) {
         or_get_fields(s, ud[i], xs[i], &cons[i]);
-        or_cons_to_prim(&s->cfg, &cons[i], &prim[i]);
+        or_cons_to_prim(&s->cfg, &cons[i], &t->prim[i]);
         lambda[i] = 0.;
         mu[i] = or_visc_mu(s, ud[i], xs[i]);
         kt += or_heat_k(s, ud[i], xs[i]);
     }
-    fmu = 0.5*(mu[0] + mu[1]);
+    t->fmu = 0.5*(mu[0] + mu[1]);
     flambda = 0.5*(lambda[0] + lambda[1]);
-    lp = (flambda + 4.*fmu/3.); lm = (flambda - 2.*fmu/3.);
-    kt *= 0.5;
-    fr = (prim[0].r + prim[1].r)*0.5;
-    ft = (prim[0].t + prim[1].t)*0.5;
-    fu = (prim[0].u + prim[1].u)*0.5;
-    fv = (prim[0].v + prim[1].v)*0.5;
-    fw = (prim[0].w + prim[1].w)*0.5;
+    t->lp = (flambda + 4.*t->fmu/3.); t->lm = (flambda - 2.*t->fmu/3.);
+    t->kt = kt * 0.5;
+    fr = (t->prim[0].r + t->prim[1].r)*0.5;
+    ft = (t->prim[0].t + t->prim[1].t)*0.5;
+    fu = (t->prim[0].u + t->prim[1].u)*0.5;
+    fv = (t->prim[0].v + t->prim[1].v)*0.5;
+    fw = (t->prim[0].w + t->prim[1].w)*0.5;
     for (i = 0; i < s->cfg.n_comp; i++) {
-        fc[i] = (prim[0].c[i] + prim[1].c[i])*0.5;
-        dm[i] = (ud[0]->d[i] + ud[1]->d[i])*0.5;
+        fc[i] = (t->prim[0].c[i] + t->prim[1].c[i])*0.5;
+        t->dm[i] = (ud[0]->d[i] + ud[1]->d[i])*0.5;
     }
-    or_grad_face_terms(s, lp, lm, fmu, kt, fu, fv, fw, ft, fr, fc, dm, n, q6, qq);
+    or_grad_face_terms(s, t->lp, t->lm, t->fmu, t->kt, fu, fv, fw, ft, fr, fc, t->dm, n, t->q6, t->qq);
+}
+
+/* BR2: the face terms from side i's own trace, at the averaged coefficients */
+static void or_grad_own_terms(const or_sim_t *s, const or_gpt_t *t, int i, const double n[3],
+                              double q6[6], double qq[3])
+{
+    const or_prim_t *p = &t->prim[i];
+    or_grad_face_terms(s, t->lp, t->lm, t->fmu, t->kt, p->u, p->v, p->w, p->t, p->r, p->c, t->dm, n, q6, qq);
+}
+
+static void or_grad_point(or_sim_t *s, or_data_t *ud[2], const double xs[2][3], const double n[3],
+                          double wj)
+{
+    or_gpt_t t;
+    double q6o[2][6], qqo[2][3];
+    int i, ibf;
+    const int br2 = s->cfg.visc_br2_eta > 0.;
+    or_grad_point_terms(s, ud, xs, n, &t);
+    if (br2)
+        for (i = 0; i < 2; i++) or_grad_own_terms(s, &t, i, n, q6o[i], qqo[i]);
     for (ibf = 0; ibf < OR_BF; ibf++)
-        for (i = 0; i < 2; i++)
-            or_grad_add(ud[i], ibf, q6, qq, (i ? -1. : 1.) * or_bf(&ud[i]->g, xs[i], ibf) * wj);
+        for (i = 0; i < 2; i++) {
+            const double bfv = (i ? -1. : 1.) * or_bf(&ud[i]->g, xs[i], ibf) * wj;
+            or_grad_add(ud[i], ibf, t.q6, t.qq, bfv);
+            if (br2) or_grad_add_own(ud[i], ibf, q6o[i], qqo[i], bfv);
+        }
 }
 
 /* visits the points of an interior face (conforming, or the four sub-faces
@@ -274,6 +336,11 @@ static void or_ns_grad(or_sim_t *s)
         memset(s->d[e].int_q, 0, sizeof(s->d[e].int_q));
     }
     for (e = 0; e < s->n; e++) or_grad_volume(s, &s->d[e]);
+    if (s->cfg.visc_br2_eta > 0.)                      /* BR2: the volume terms are shared */
+        for (e = 0; e < s->n; e++) {
+            memcpy(s->d[e].int_tau_own, s->d[e].int_tau, sizeof(s->d[e].int_tau));
+            memcpy(s->d[e].int_q_own, s->d[e].int_q, sizeof(s->d[e].int_q));
+        }
     for (i = 0; i < s->nf; i++) {
         if (s->f[i].nsides != 2) or_grad_bnd(s, &s->f[i]);
         else or_inner_points(s, &s->f[i], or_grad_point);
@@ -282,6 +349,10 @@ static void or_ns_grad(or_sim_t *s)
         or_data_t *d = &s->d[e];
         for (k = 0; k < 6; k++) or_matr_vect_mult(d->g.a_inv, d->int_tau[k], d->tau[k]);
         for (k = 0; k < 3; k++) or_matr_vect_mult(d->g.a_inv, d->int_q[k], d->q[k]);
+        if (s->cfg.visc_br2_eta > 0.) {
+            for (k = 0; k < 6; k++) or_matr_vect_mult(d->g.a_inv, d->int_tau_own[k], d->tau_own[k]);
+            for (k = 0; k < 3; k++) or_matr_vect_mult(d->g.a_inv, d->int_q_own[k], d->q_own[k]);
+        }
     }
 }
 
@@ -350,18 +421,18 @@ static void or_visc_bnd(or_sim_t *s, const or_face_t *f)
 
 /* src/models/ns/charm_model_ns_diff_integrals.c:232-292 (hanging) and
  * 330-382 (conforming) */
-static void or_visc_point(or_sim_t *s, or_data_t *ud[2], const double xs[2][3], const double n[3],
-                          double wj)
+/* the surface flux at one point from each side's stress t[i] and heat flux
+ * qs[i] at its trace */
+static void or_visc_flux(or_sim_t *s, or_data_t *ud[2], const double xs[2][3], const double n[3],
+                         double wj, double ts[2][6], double qs[2][3])
 {
     double ftau[6] = {0, 0, 0, 0, 0, 0}, fu = 0., fv = 0., fw = 0., ft = 0., qu, qv, qw, qe;
     int i, k, ibf;
     for (i = 0; i < 2; i++) {
         or_cons_t c; or_prim_t p;
-        double t[6], qt[3];
+        const double *t = ts[i], *qt = qs[i];
         or_get_fields(s, ud[i], xs[i], &c);
         or_cons_to_prim(&s->cfg, &c, &p);
-        or_q_at(ud[i], xs[i], qt);
-        or_tau_at(ud[i], xs[i], t);
         for (k = 0; k < 6; k++) ftau[k] += t[k];
         fu += p.u*t[0] + p.v*t[3] + p.w*t[4];
         fv += p.u*t[3] + p.v*t[1] + p.w*t[5];
@@ -384,6 +455,81 @@ static void or_visc_point(or_sim_t *s, or_data_t *ud[2], const double xs[2][3], 
         }
 }
 
+/* BR1: each side's auxiliary tau_h, q_h at its trace */
+static void or_visc_point(or_sim_t *s, or_data_t *ud[2], const double xs[2][3], const double n[3],
+                          double wj)
+{
+    double ts[2][6], qs[2][3];
+    int i;
+    for (i = 0; i < 2; i++) {
+        or_q_at(ud[i], xs[i], qs[i]);
+        or_tau_at(ud[i], xs[i], ts[i]);
+    }
+    or_visc_flux(s, ud, xs, n, wj, ts, qs);
+}
+
+/* BR2 surface term of one interior face, (sub)face by (sub)face: each side's
+ * lifting l_i (9 fields: tau xx yy zz xy xz yz, q x y z) from the points of
+ * the (sub)face, then the flux with tau_own_i + eta l_i at every point.
+ * Two switches the Makefile never sets plant plausible mistakes for
+ * tests/test_oracle_mutants.py: OR_MUTANT_BR2_TAU_H takes BR1's tau_h for
+ * tau_own, OR_MUTANT_BR2_LIFT_SIGN builds the lifting from Q(own) - Q(avg). */
+#ifdef OR_MUTANT_BR2_TAU_H
+#define OR_BR2_TAU(D) ((D)->tau)
+#define OR_BR2_Q(D) ((D)->q)
+#else
+#define OR_BR2_TAU(D) ((D)->tau_own)
+#define OR_BR2_Q(D) ((D)->q_own)
+#endif
+#ifdef OR_MUTANT_BR2_LIFT_SIGN
+#define OR_BR2_JUMP(AVG, OWN) ((OWN) - (AVG))
+#else
+#define OR_BR2_JUMP(AVG, OWN) ((AVG) - (OWN))
+#endif
+
+static void or_visc_face_br2(or_sim_t *s, const or_face_t *f)
+{
+    const double eta = s->cfg.visc_br2_eta;
+    int i, j, k, igp, ibf, nj = (f->is_hanging[0] || f->is_hanging[1]) ? 4 : 1, h_side = 0;
+    for (i = 0; i < 2; i++) if (f->is_hanging[i]) h_side = i;
+    for (j = 0; j < nj; j++) {
+        or_data_t *ud[2];
+        double n[3], xs[OR_FGP][2][3], wj[OR_FGP], lint[2][9][OR_BF], lift[2][9][OR_BF];
+        for (i = 0; i < 2; i++) ud[i] = &s->d[f->is_hanging[i] ? f->quad[i][j] : f->quad[i][0]];
+        or_face_normal(ud[0], f->face[0], n);
+        memset(lint, 0, sizeof(lint));
+        for (igp = 0; igp < OR_FGP; igp++) {
+            const double *x = ud[h_side]->g.face_gp[f->face[h_side]][igp];
+            or_gpt_t t;
+            wj[igp] = ud[h_side]->g.face_gw[f->face[h_side]][igp] * ud[h_side]->g.face_gj[f->face[h_side]][igp];
+            for (i = 0; i < 2; i++) or_shift(f, i, h_side, x, xs[igp][i]);
+            or_grad_point_terms(s, ud, (const double (*)[3]) xs[igp], n, &t);
+            for (i = 0; i < 2; i++) {
+                double q6o[6], qqo[3];
+                or_grad_own_terms(s, &t, i, n, q6o, qqo);
+                for (ibf = 0; ibf < OR_BF; ibf++) {
+                    const double bfv = (i ? -1. : 1.) * or_bf(&ud[i]->g, xs[igp][i], ibf) * wj[igp];
+                    for (k = 0; k < 6; k++) lint[i][k][ibf] += OR_BR2_JUMP(t.q6[k], q6o[k]) * bfv;
+                    for (k = 0; k < 3; k++) lint[i][6 + k][ibf] -= OR_BR2_JUMP(t.qq[k], qqo[k]) * bfv;
+                }
+            }
+        }
+        for (i = 0; i < 2; i++)
+            for (k = 0; k < 9; k++) or_matr_vect_mult(ud[i]->g.a_inv, lint[i][k], lift[i][k]);
+        for (igp = 0; igp < OR_FGP; igp++) {
+            double ts[2][6], qs[2][3];
+            for (i = 0; i < 2; i++) {
+                const double *xi = xs[igp][i];
+                for (k = 0; k < 6; k++)
+                    ts[i][k] = or_coef_at(ud[i], OR_BR2_TAU(ud[i])[k], xi) + eta * or_coef_at(ud[i], lift[i][k], xi);
+                for (k = 0; k < 3; k++)
+                    qs[i][k] = or_coef_at(ud[i], OR_BR2_Q(ud[i])[k], xi) + eta * or_coef_at(ud[i], lift[i][6 + k], xi);
+            }
+            or_visc_flux(s, ud, (const double (*)[3]) xs[igp], n, wj[igp], ts, qs);
+        }
+    }
+}
+
 /* src/models/ns/charm_model_ns_diff_integrals.c:399-408 */
 static void or_ns_integrals(or_sim_t *s)
 {
@@ -391,6 +537,7 @@ static void or_ns_integrals(or_sim_t *s)
     for (e = 0; e < s->n; e++) or_visc_volume(s, &s->d[e]);
     for (i = 0; i < s->nf; i++) {
         if (s->f[i].nsides != 2) or_visc_bnd(s, &s->f[i]);
+        else if (s->cfg.visc_br2_eta > 0.) or_visc_face_br2(s, &s->f[i]);
         else or_inner_points(s, &s->f[i], or_visc_point);
     }
 }
@@ -536,6 +683,19 @@ void or_ns_aux(or_sim_t *s, double *out)
     for (e = 0; e < s->n; e++) {
         for (k = 0; k < 6; k++) memcpy(&out[(e*9 + k)*4], s->d[e].tau[k], 4*sizeof(double));
         for (k = 0; k < 3; k++) memcpy(&out[(e*9 + 6 + k)*4], s->d[e].q[k], 4*sizeof(double));
+    }
+}
+
+/* BR2's tau_own, q_own in the layout of or_ns_aux (visc_br2_eta > 0) */
+void or_ns_aux_own(or_sim_t *s, double *out)
+{
+    int64_t e;
+    int k;
+    or_ns_diffusion(s);
+    or_ns_grad(s);
+    for (e = 0; e < s->n; e++) {
+        for (k = 0; k < 6; k++) memcpy(&out[(e*9 + k)*4], s->d[e].tau_own[k], 4*sizeof(double));
+        for (k = 0; k < 3; k++) memcpy(&out[(e*9 + 6 + k)*4], s->d[e].q_own[k], 4*sizeof(double));
     }
 }
 

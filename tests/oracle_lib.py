@@ -52,7 +52,8 @@ class Config(C.Structure):
                 ("bnd_type", C.c_int * 6), ("bnd_param", (C.c_double * 16) * 6),
                 ("grav", C.c_double * 3), ("use_visc", C.c_int), ("use_diff", C.c_int),
                 ("t_ref", C.c_double), ("n_react", C.c_int), ("react", Reaction * 4),
-                ("cfl", C.c_double), ("chem_parity", C.c_int), ("chem_energy_molar", C.c_int)]
+                ("cfl", C.c_double), ("chem_parity", C.c_int), ("chem_energy_molar", C.c_int),
+                ("visc_br2_eta", C.c_double)]
 
 
 def build():
@@ -98,7 +99,7 @@ def lib():
         L.or_amr_indicator.argtypes = [vp, dp, dp]
         L.or_amr_flags.argtypes = [vp, dp, dp, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int8)]
         L.or_amr_project.argtypes = [C.c_int, i64, ip, dp, i64, ip, C.POINTER(C.c_int64), dp]
-        for fn in ("or_ns_residual", "or_ns_aux", "or_ns_diff_coeffs"):
+        for fn in ("or_ns_residual", "or_ns_aux", "or_ns_aux_own", "or_ns_visc", "or_ns_diff_coeffs"):
             getattr(L, fn).argtypes = [vp, dp]
         L.or_ns_chem_apply.argtypes = [vp, C.c_double, dp]
         L.or_gas_r.restype = C.c_double
@@ -118,9 +119,11 @@ def _dptr(a):
 def make_config(n_comp=1, comp_m=(GAS_R,), comp_cp=((3.5,),), eos="IDEAL", flux="LF",
                 limiter=False, model="NS", bnd=("OUTLET",) * 6, bnd_param=None,
                 grav=(0, 0, 0), cfl=0.3, transport=None, use_visc=False, use_diff=False,
-                t_ref=298.15, reactions=(), chem_parity=False, chem_energy_molar=False):
+                t_ref=298.15, reactions=(), chem_parity=False, chem_energy_molar=False,
+                visc_br2_eta=0.0):
     """transport: per species dict with ml0, kp0, t0, ts, sig, ek, h0, ml_type,
-    kp_type (Sutherland = 1); reactions: (left[3], right[3], a, e, n) tuples."""
+    kp_type (Sutherland = 1); reactions: (left[3], right[3], a, e, n) tuples;
+    visc_br2_eta > 0 selects the BR2 viscous flux with that penalty factor."""
     cfg = Config()
     cfg.n_comp = n_comp
     for i in range(n_comp):
@@ -156,6 +159,7 @@ def make_config(n_comp=1, comp_m=(GAS_R,), comp_cp=((3.5,),), eos="IDEAL", flux=
         cfg.react[j].a, cfg.react[j].e, cfg.react[j].n = a, e, n
     cfg.chem_parity = int(chem_parity)
     cfg.chem_energy_molar = int(chem_energy_molar)
+    cfg.visc_br2_eta = visc_br2_eta
     return cfg
 
 
@@ -209,6 +213,19 @@ class OracleSim:
         """tau (xx yy zz xy xz yz) and q (x y z) coefficients: [n][9][4]"""
         out = np.zeros((self.n, 9, 4))
         lib().or_ns_aux(self.h, _dptr(out))
+        return out
+
+    def ns_aux_own(self):
+        """BR2: the auxiliary field with the interior face terms from the own trace, [n][9][4]"""
+        out = np.zeros((self.n, 9, 4))
+        lib().or_ns_aux_own(self.h, _dptr(out))
+        return out
+
+    def ns_visc(self):
+        """the viscous integrals alone: ru rv rw re, [n][4][4]"""
+        out = np.zeros((self.n, 4, 4))
+        lib().or_ns_visc(self.h, _dptr(out))
+        self.check()
         return out
 
     def ns_diff_coeffs(self):

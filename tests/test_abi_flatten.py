@@ -166,6 +166,14 @@ def test_level_mass_inverse_matches_oracle(level):
         assert np.allclose(g["a_inv"], ai, rtol=1e-13, atol=1e-13 * ai.max())
 
 
+@pytest.mark.parametrize("eta", [-1.0, float("nan"), float("inf")])
+def test_bad_br2_penalty_factor_is_rejected(eta):
+    """the context is refused before any device call (CPU)"""
+    cfg = CA.make_config(model="NS", use_visc=True, visc_br2_eta=eta)
+    with pytest.raises(CA.CharmError, match="BR2 penalty factor"):
+        CA.GpuContext(cfg)
+
+
 def test_chunk_faces_below_one_leaf_is_an_error():
     """a chunk must hold every face of a leaf (six on a uniform mesh)"""
     with pytest.raises(CA.CharmError, match="chunk_faces"):

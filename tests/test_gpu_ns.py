@@ -203,3 +203,49 @@ def test_reacting_hot_spot_small_matches_oracle(level, a, molar):
     ctx.run(dt, 2)
     err = var_err(ctx.get_state(), sim.get_state())
     assert err <= 1e-11, err
+
+
+# --- BR2 option (no counterpart in the reference; the oracle's BR2 is pinned
+# by closed forms in tests/test_oracle_br2.py) ------------------------------
+
+@pytest.mark.parametrize("name", sorted(cases()))
+def test_br2_residual_matches_oracle(name):
+    cfg, lv, per, u0 = setup(name)
+    cfg.visc_br2_eta = 6.0
+    got = device(cfg, lv, per, u0).residual()
+    sim = OL.OracleSim(oracle_cfg(cfg), lv)
+    sim.set_state(u0)
+    want = sim.ns_residual()
+    assert var_err(got, want) <= 1e-12, var_err(got, want)
+    # and the BR1 residual of the same state differs
+    cfg.visc_br2_eta = 0.0
+    br1 = device(cfg, lv, per, u0).residual()
+    assert np.abs(br1 - got).max() > 1e-9 * np.abs(got).max()
+
+
+@pytest.mark.parametrize("name,lim,eta", [("const_walls", False, 3.0), ("mix_walls", True, 6.0)])
+def test_br2_steps_match_oracle(name, lim, eta):
+    cfg, lv, per, u0 = setup(name, seed=3)
+    cfg.limiter = int(lim)
+    cfg.visc_br2_eta = eta
+    dt = 1e-3 if not name.startswith("mix") else 2e-5
+    ctx = device(cfg, lv, per, u0)
+    ctx.run(dt, 4)
+    sim = OL.OracleSim(oracle_cfg(cfg), lv)
+    sim.set_state(u0)
+    sim.run(dt, 4)
+    err = var_err(ctx.get_state(), sim.get_state())
+    assert err <= 1e-11, err
+
+
+def test_br2_fp32_close_to_fp64():
+    cfg, lv, per, u0 = setup("mix_walls", seed=4)
+    out = {}
+    for prec in (CA.FP64, CA.FP32):
+        c = cfg_mix(prec=prec, bnd=WALLS, params=cases()["mix_walls"][1]["params"], visc_br2_eta=6.0)
+        c.limiter = 1
+        ctx = device(c, lv, per, u0)
+        ctx.run(2e-5, 4)
+        out[prec] = ctx.get_state()
+    err = np.sqrt(((out[CA.FP32] - out[CA.FP64]) ** 2).sum() / (out[CA.FP64] ** 2).sum())
+    assert err < 2e-6, err

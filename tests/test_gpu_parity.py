@@ -38,7 +38,8 @@ def oracle_cfg(cfg):
                           cfl=cfg.cfl, transport=tr, use_visc=bool(cfg.use_visc),
                           use_diff=bool(cfg.use_diff), t_ref=cfg.t_ref, reactions=rx,
                           chem_parity=bool(cfg.chem_parity),
-                          chem_energy_molar=bool(cfg.chem_energy_molar))
+                          chem_energy_molar=bool(cfg.chem_energy_molar),
+                          visc_br2_eta=cfg.visc_br2_eta)
 
 
 def device(cfg, leaves, periodic, u0):
