@@ -46,6 +46,9 @@ struct DevParams {
     double cp_c[kMaxComp][8];
     int    ml_type[kMaxComp], kp_type[kMaxComp];
     double ml0[kMaxComp], kp0[kMaxComp], t0[kMaxComp], ts[kMaxComp], h0[kMaxComp];
+    double t0inv[kMaxComp];            // 1 / t0 (Sutherland)
+    double hc[kMaxComp][8];            // enthalpy as a polynomial: h(T) = hH0 + T (hc0 + T (hc1 + ...)),
+    double hH0[kMaxComp];              // hc_k = cp_k / (k + 1), hH0 = h0 - sum_k hc_k T_ref^(k+1)
     double sig[kMaxComp], ek[kMaxComp];
     double t_ref;
     int    use_visc, use_diff;
@@ -121,7 +124,7 @@ __device__ __forceinline__ double newton_t(const DevParams &P, double e, const d
 {
     double tt = 1.0, m_ = 0.0;
     #pragma unroll
-    for (int i = 0; i < NC; i++) m_ += c[i] / P.comp_m[i];
+    for (int i = 0; i < NC; i++) m_ += c[i] * P.comp_minv[i];
     const double rm = kGasR * m_;
     for (int j = 0; j < 100; j++) {
         double cp = 0.0, cp_dt = 0.0;
@@ -156,7 +159,7 @@ __device__ __forceinline__ void eos_coeffs_at(const DevParams &P, Prim<R, NC> &p
         cp = 0.0;
         #pragma unroll
         for (int i = 0; i < NC; i++) {
-            M_ += (double) p.c[i] / P.comp_m[i];
+            M_ += (double) p.c[i] * P.comp_minv[i];
             cp += (double) p.c[i] * comp_cp(P, i, t);
         }
         m = 1.0 / M_;

@@ -102,10 +102,12 @@ __device__ __forceinline__ void face_points(const int *__restrict__ nbr, const i
 }
 
 // Enumerates the interior (sub)faces of leaf e, one call per (sub)face:
-// gfn(d, nb, so1, so2, sm, no1, no2, nm, wj, sign).  Point k of the
+// gfn(d, nb, so1, so2, sm, no1, no2, nm, wj, sign, js, jn).  Point k of the
 // (sub)face is face_point(d, k, so1, so2, sm) on e and face_point(d ^ 1, k,
-// no1, no2, nm) on nb; wj and sign as in face_points.  Boundary faces are
-// skipped.  The (sub)faces and their points are those face_points visits.
+// no1, no2, nm) on nb; wj and sign as in face_points.  js / jn number the
+// (sub)face among the sub-faces of e's face d / nb's face d ^ 1 (0 on a face
+// that is not split on that side).  Boundary faces are skipped.  The
+// (sub)faces and their points are those face_points visits.
 template <typename R, typename G>
 __device__ __forceinline__ void face_subfaces(const int *__restrict__ nbr, const int4 *__restrict__ aux,
                                               const uint8_t *__restrict__ level, int e, R h, G gfn)
@@ -116,21 +118,22 @@ __device__ __forceinline__ void face_subfaces(const int *__restrict__ nbr, const
         const R sign = (d & 1) ? R(1) : R(-1);
         if (nb >= 0) {
             R o1 = R(0), o2 = R(0), m = R(1);
+            int j = 0;
             if (level[nb] != lev) {
                 const int4 grp = aux[-kNbrAuxBase - nbr[6 * nb + (d ^ 1)]];
-                const int j = (grp.x == e) ? 0 : ((grp.y == e) ? 1 : ((grp.z == e) ? 2 : 3));
+                j = (grp.x == e) ? 0 : ((grp.y == e) ? 1 : ((grp.z == e) ? 2 : 3));
                 o1 = (j & 1) ? R(0.25) : R(-0.25);
                 o2 = (j & 2) ? R(0.25) : R(-0.25);
                 m = R(0.5);
             }
-            gfn(d, nb, R(0), R(0), R(1), o1, o2, m, h * h * R(1.0 / 6.0), sign);
+            gfn(d, nb, R(0), R(0), R(1), o1, o2, m, h * h * R(1.0 / 6.0), sign, 0, j);
         } else if (nb <= -kNbrAuxBase) {
             const int4 grp = aux[-kNbrAuxBase - nb];
             for (int j = 0; j < 4; j++) {
                 // by selects: the loop need not unroll around a large body
                 const int qj = j == 0 ? grp.x : (j == 1 ? grp.y : (j == 2 ? grp.z : grp.w));
                 const R o1 = (j & 1) ? R(0.25) : R(-0.25), o2 = (j & 2) ? R(0.25) : R(-0.25);
-                gfn(d, qj, o1, o2, R(0.5), R(0), R(0), R(1), R(0.25) * h * h * R(1.0 / 6.0), sign);
+                gfn(d, qj, o1, o2, R(0.5), R(0), R(0), R(1), R(0.25) * h * h * R(1.0 / 6.0), sign, j, 0);
             }
         }
     }

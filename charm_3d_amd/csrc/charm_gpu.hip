@@ -42,6 +42,7 @@ struct charm_gpu_ctx {
     void *d_auxf;             // [36][n] tau, q coefficients
     void *d_vi;               // [16][n] viscous integrals
     void *d_auxo;             // [36][n] BR2: the own-trace tau, q (allocated with visc_br2_eta > 0)
+    void *d_lift;             // [24][28][n] BR2: the (sub)face liftings
     double *d_dcoef;          // [nc][n] diffusion coefficients
     double *d_chem;           // [n] chemistry energy source
     // state
@@ -75,7 +76,7 @@ static void free_mesh(charm_gpu_ctx_t *c)
                   (void **) &c->d_ch_ebeg, (void **) &c->d_ch_fbeg, (void **) &c->d_slot_ptr,
                   (void **) &c->d_nbr, (void **) &c->d_slots, &c->B[0], &c->B[1], &c->B[2],
                   (void **) &c->d_stage64, (void **) &c->d_partial, (void **) &c->d_q,
-                  &c->d_auxf, &c->d_vi, &c->d_auxo, (void **) &c->d_dcoef, (void **) &c->d_chem};
+                  &c->d_auxf, &c->d_vi, &c->d_auxo, &c->d_lift, (void **) &c->d_dcoef, (void **) &c->d_chem};
     for (size_t i = 0; i < sizeof(p) / sizeof(p[0]); i++)
         if (*p[i]) { (void) hipFree(*p[i]); *p[i] = NULL; }
     free(c->hq);
@@ -164,7 +165,16 @@ extern "C" int charm_gpu_create(const charm_config_t *cfg, charm_gpu_ctx_t **out
         for (int k = 0; k < 8; k++) P.cp_c[i][k] = k < q.n_cp ? q.cp[k] : 0.0;
         P.ml_type[i] = q.ml_type; P.kp_type[i] = q.kp_type;
         P.ml0[i] = q.ml0; P.kp0[i] = q.kp0; P.t0[i] = q.t0; P.ts[i] = q.ts; P.h0[i] = q.h0;
+        P.t0inv[i] = q.t0 != 0.0 ? 1.0 / q.t0 : 0.0;
         P.sig[i] = q.sig; P.ek[i] = q.ek;
+        // enthalpy polynomial: hc_k = cp_k / (k+1), hH0 = h0 - sum_k hc_k T_ref^(k+1)
+        double hr = 0.0, tp = cfg->t_ref;
+        for (int k = 0; k < 8; k++) {
+            P.hc[i][k] = k < q.n_cp ? q.cp[k] / (double) (k + 1) : 0.0;
+            hr += P.hc[i][k] * tp;
+            tp *= cfg->t_ref;
+        }
+        P.hH0[i] = q.h0 - hr;
     }
     P.t_ref = cfg->t_ref;
     P.use_visc = cfg->use_visc != 0;
@@ -285,7 +295,10 @@ extern "C" int charm_gpu_upload_mesh(charm_gpu_ctx_t *c, const charm_mesh_t *m)
     if (c->ph) {
         DMALLOC(c->d_auxf, (size_t) m->n * 36 * c->esz);
         DMALLOC(c->d_vi, (size_t) m->n * 16 * c->esz);
-        if (c->cfg.use_visc && c->cfg.visc_br2_eta > 0.0) DMALLOC(c->d_auxo, (size_t) m->n * 36 * c->esz);
+        if (c->cfg.use_visc && c->cfg.visc_br2_eta > 0.0) {
+            DMALLOC(c->d_auxo, (size_t) m->n * 36 * c->esz);
+            DMALLOC(c->d_lift, (size_t) m->n * kLiftSlots * kLiftFields * c->esz);
+        }
         DMALLOC(c->d_dcoef, (size_t) m->n * c->nc * sizeof(double));
         DMALLOC(c->d_chem, (size_t) m->n * sizeof(double));
     }
@@ -394,23 +407,23 @@ static NsArgs ns_args(charm_gpu_ctx_t *c, const void *U, void *out, double dt)
     A.chem_rhs = c->d_chem;
     A.dcoef = (c->cfg.use_diff && c->nc >= 2) ? c->d_dcoef : NULL;
     A.aux_in = c->d_auxf; A.aux_out = c->d_auxf; A.visc_out = c->d_vi; A.aux_own = c->d_auxo;
+    A.lift = c->d_lift;
     A.n = c->n; A.dt = dt;
     A.prm = c->d_P;
     return A;
 }
 
 // The viscous passes on the state A.U: diffusion coefficients (DIFFUSION and
-// two or more species), the BR1 auxiliary tau / q, with BR2 also the
-// own-trace auxiliary field, then the viscous integrals (BR1 or BR2 surface
-// flux).  Launch codes of launch_ns: 1 diffusion, 2 / 4 auxiliary field
-// (BR1 / own trace), 3 / 5 viscous integrals (BR1 / BR2).
+// two or more species), the auxiliary tau / q, then the viscous integrals.
+// Launch codes of launch_ns: 1 diffusion; 2 / 4 auxiliary field (BR1 / BR2,
+// which also writes each (sub)face's lifting and the own-trace field); 3 / 5
+// viscous integrals (BR1 / BR2 surface flux).
 static hipError_t ns_viscous(charm_gpu_ctx_t *c, const NsArgs &A)
 {
     hipError_t e = hipSuccess;
     const bool br2 = c->cfg.visc_br2_eta > 0.0;
     if (A.dcoef && (e = launch_ns(c->sprec, 1, c->nc, A, c->P, c->st)) != hipSuccess) return e;
-    if ((e = launch_ns(c->sprec, 2, c->nc, A, c->P, c->st)) != hipSuccess) return e;
-    if (br2 && (e = launch_ns(c->sprec, 4, c->nc, A, c->P, c->st)) != hipSuccess) return e;
+    if ((e = launch_ns(c->sprec, br2 ? 4 : 2, c->nc, A, c->P, c->st)) != hipSuccess) return e;
     return launch_ns(c->sprec, br2 ? 5 : 3, c->nc, A, c->P, c->st);
 }
 

@@ -80,7 +80,12 @@ struct ProjArgs {
     int nvk;                     // 4 V coefficients per leaf
 };
 
-// Navier-Stokes kernels (one thread per leaf)
+// Navier-Stokes kernels (one thread per leaf).  BR2 liftings: per leaf 24
+// (sub)face slots (direction d * 4 + sub-face j) of kLiftFields coefficients
+// (tau xx yy zz xy xz yz and q along the face axis, 4 each)
+constexpr int kLiftSlots = 24;
+constexpr int kLiftFields = 28;
+
 struct NsArgs {
     const void *U;            // state
     void *out;                // chemistry: state with the reacted species means
@@ -93,6 +98,7 @@ struct NsArgs {
     void *aux_out;            // [36][n] (gradient pass)
     void *visc_out;           // [16][n] viscous integrals of ru rv rw re
     void *aux_own;            // [36][n] BR2: own-trace tau, q (written by code 4, read by code 5)
+    void *lift;               // [24][kLiftFields][n] BR2: each (sub)face slot's lifting (codes 4, 5)
     int64_t n;
     double dt;
     const DevParams *prm;     // the context's constants in device memory (read by every NS kernel)

@@ -29,15 +29,14 @@ __device__ __forceinline__ void transport(const DevParams &P, const Prim<R, NC> 
     double m = 0.0, s = 0.0, k = 0.0;
     #pragma unroll
     for (int i = 0; i < NC; i++) {
-        const double cm = (double) p.c[i] / P.comp_m[i];
+        const double cm = (double) p.c[i] * P.comp_minv[i];
         double ml = P.ml0[i], kp = P.kp0[i];
-        if (P.ml_type[i] == 1) {
-            const double r = t / P.t0[i];
-            ml = P.ml0[i] * sqrt(r * r * r) * (P.t0[i] + P.ts[i]) / (t + P.ts[i]);
-        }
-        if (P.kp_type[i] == 1) {
-            const double r = t / P.t0[i];
-            kp = P.kp0[i] * sqrt(r * r * r) * (P.t0[i] + P.ts[i]) / (t + P.ts[i]);
+        if (P.ml_type[i] == 1 || P.kp_type[i] == 1) {
+            // one Sutherland factor for both (same T0, S per species)
+            const double r = t * P.t0inv[i];
+            const double f = sqrt(r * r * r) * (P.t0[i] + P.ts[i]) / (t + P.ts[i]);
+            if (P.ml_type[i] == 1) ml = P.ml0[i] * f;
+            if (P.kp_type[i] == 1) kp = P.kp0[i] * f;
         }
         s += cm;
         m += cm * ml;
@@ -48,15 +47,15 @@ __device__ __forceinline__ void transport(const DevParams &P, const Prim<R, NC> 
 }
 
 // enthalpy h0 + sum_k cp_k (T^(k+1) - T_ref^(k+1)) / (k+1),
-// charm_comp_calc_enthalpy (src/charm_globals.c:679-694)
+// charm_comp_calc_enthalpy (src/charm_globals.c:679-694), as the polynomial
+// hH0 + T (hc0 + T (hc1 + ...)) with hc_k = cp_k / (k+1) and the T_ref terms
+// folded into hH0 on the host (the coefficients beyond n_cp are zero)
 __device__ __forceinline__ double enthalpy(const DevParams &P, int i, double t)
 {
-    double h = P.h0[i], tp = t, rp = P.t_ref;
-    for (int k = 0; k < P.n_cp[i]; k++) {
-        h += P.cp_c[i][k] * (tp - rp) / (double) (k + 1);
-        tp *= t; rp *= P.t_ref;
-    }
-    return h;
+    double h = P.hc[i][7];
+    #pragma unroll
+    for (int k = 6; k >= 0; k--) h = fma(h, t, P.hc[i][k]);
+    return fma(h, t, P.hH0[i]);
 }
 
 template <typename R, int NC>
@@ -246,10 +245,15 @@ void k_ns_diffusion(NsArgs A, DevParams)
 // 21-507): int = -int f(u) . grad phi + sum_faces int avg(f) . n phi, then
 // A^-1 (diagonal on the cube octree).  tau = (lambda + 4 mu / 3, lambda - 2 mu
 // / 3, mu) applied to the velocity (lambda = 0), q = -(k T + sum_i h_i rho D_i
-// Y_i) with the sign folded as in the reference.  OWN (the BR2 option): the
-// interior face terms from the leaf's own trace at the same face-averaged
-// coefficients, into aux_own: tau_own = tau_h less the interior liftings.
-template <typename S, int NC, bool OWN>
+// Y_i) with the sign folded as in the reference.
+// BR2 (the option; no counterpart in the reference) also writes, for each
+// interior (sub)face slot of the leaf (d * 4 + j: face direction d, sub-face
+// j of a face with four small neighbours, else 0), the leaf's lifting
+//   l = A^-1 sum_points sgn [Q(avg) - Q(own)] phi wJ
+// in 7 fields (tau xx yy zz xy xz yz, q along the face axis: the lifting's
+// other q components are zero), and the own-trace auxiliary field
+// tau_own = tau_h - sum of the leaf's liftings.
+template <typename S, int NC, bool BR2>
 __global__ __launch_bounds__(128)
 void k_ns_grad(NsArgs A, DevParams)
 {
@@ -315,38 +319,48 @@ void k_ns_grad(NsArgs A, DevParams)
                      const R nn[3], R q6[6], R qq[3]) {
         grad_terms<R, NC>(P, lp, lm, mu, kt, fu, fv, fw, ft, fr, fc, dm, nn, q6, qq);
     };
-    face_points<R>(A.nbr, A.aux, A.level, e, h,
-        [&](int d, int nb, const R ps[3], const R pn[3], R wj, R sign) {
-            R un[4 * NV];
-            load_coeffs<R, NV>(U, n, nb, un);
-            Prim<R, NC> pa, pb;
-            state_at<R, NC>(P, u, ps, pa);
-            state_at<R, NC>(P, un, pn, pb);
-            R mua, kta, mub, ktb;
-            transport<R, NC>(P, pa, mua, kta);
-            transport<R, NC>(P, pb, mub, ktb);
-            const R fmu = R(0.5) * (mua + mub), kt = R(0.5) * (kta + ktb);
-            const R lp = R(4.0 / 3.0) * fmu, lm = R(-2.0 / 3.0) * fmu;
-            const R ft = R(0.5) * (dv(pa.e, pa.cv) + dv(pb.e, pb.cv));
-            R fc[NC];
-            double dm[NC];
+    // one interior face point: the BR1 term into acc; with BR2 also the
+    // lifting integrand [Q(avg) - Q(own)] sgn phi wJ into L
+    auto inner = [&](int d, const R *un, int nb, const R ps[3], const R pn[3], R wj, R sign, R (*L)[4]) {
+        Prim<R, NC> pa, pb;
+        state_at<R, NC>(P, u, ps, pa);
+        state_at<R, NC>(P, un, pn, pb);
+        R mua, kta, mub, ktb;
+        transport<R, NC>(P, pa, mua, kta);
+        transport<R, NC>(P, pb, mub, ktb);
+        const R fmu = R(0.5) * (mua + mub), kt = R(0.5) * (kta + ktb);
+        const R lp = R(4.0 / 3.0) * fmu, lm = R(-2.0 / 3.0) * fmu;
+        const R ta = dv(pa.e, pa.cv);
+        const R ft = R(0.5) * (ta + dv(pb.e, pb.cv));
+        R fc[NC];
+        double dm[NC];
+        #pragma unroll
+        for (int i = 0; i < NC; i++) {
+            fc[i] = R(0.5) * (pa.c[i] + pb.c[i]);
+            dm[i] = 0.5 * (ds[i] + (A.dcoef ? A.dcoef[i * n + nb] : 0.0));
+        }
+        // the face normal is +e_a for both sides; e adds with its side's sign
+        const int a = d >> 1;
+        const R nn[3] = {a == 0 ? R(1) : R(0), a == 1 ? R(1) : R(0), a == 2 ? R(1) : R(0)};
+        R q6[6], qq[3];
+        terms(lp, lm, fmu, kt, R(0.5) * (pa.u + pb.u), R(0.5) * (pa.v + pb.v), R(0.5) * (pa.w + pb.w),
+              ft, R(0.5) * (pa.r + pb.r), fc, dm, nn, q6, qq);
+        add(q6, qq, ps, sign * wj);
+        if constexpr (BR2) {
+            R q6o[6], qqo[3];
+            terms(lp, lm, fmu, kt, pa.u, pa.v, pa.w, ta, pa.r, pa.c, dm, nn, q6o, qqo);
+            const R dq = pick3(a, qq[0], qq[1], qq[2]) - pick3(a, qqo[0], qqo[1], qqo[2]);
+            const R phi[4] = {R(1), ps[0], ps[1], ps[2]};
             #pragma unroll
-            for (int i = 0; i < NC; i++) {
-                fc[i] = R(0.5) * (pa.c[i] + pb.c[i]);
-                dm[i] = 0.5 * (ds[i] + (A.dcoef ? A.dcoef[i * n + nb] : 0.0));
+            for (int c = 0; c < 4; c++) {
+                const R b = phi[c] * sign * wj;
+                #pragma unroll
+                for (int f = 0; f < 6; f++) L[f][c] += (q6[f] - q6o[f]) * b;
+                L[6][c] -= dq * b;
             }
-            // the face normal is +e_a for both sides; e adds with its side's sign
-            const int a = d >> 1;
-            const R nn[3] = {a == 0 ? R(1) : R(0), a == 1 ? R(1) : R(0), a == 2 ? R(1) : R(0)};
-            R q6[6], qq[3];
-            if constexpr (OWN)
-                terms(lp, lm, fmu, kt, pa.u, pa.v, pa.w, dv(pa.e, pa.cv), pa.r, pa.c, dm, nn, q6, qq);
-            else
-                terms(lp, lm, fmu, kt, R(0.5) * (pa.u + pb.u), R(0.5) * (pa.v + pb.v), R(0.5) * (pa.w + pb.w),
-                      ft, R(0.5) * (pa.r + pb.r), fc, dm, nn, q6, qq);
-            add(q6, qq, ps, sign * wj);
-        },
-        [&](int d, int face, int, const R ps[3], R wj) {
+        }
+    };
+    auto bnd = [&](int d, int face, int, const R ps[3], R wj) {
             const int a = d >> 1;
             const R sg = (d & 1) ? R(1) : R(-1);
             Prim<R, NC> pa, pb;
@@ -370,13 +384,74 @@ void k_ns_grad(NsArgs A, DevParams)
             R q6[6], qq[3];
             terms(lp, lm, mu, kt, fu, fv, fw, ft, R(0.5) * (pa.r + pb.r), fc, ds, nn, q6, qq);
             add(q6, qq, ps, wj);
-        });
-    R *__restrict__ aux = reinterpret_cast<R *>(OWN ? A.aux_own : A.aux_out);
+    };
+    R ai[4];
     #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const R ai = cst<R>(P.ainv0[k], P.ainv0_f[k]) * ldexp(R(1), 3 * lev);
+    for (int k = 0; k < 4; k++) ai[k] = cst<R>(P.ainv0[k], P.ainv0_f[k]) * ldexp(R(1), 3 * lev);
+    R *__restrict__ lift = reinterpret_cast<R *>(A.lift);
+    if constexpr (!BR2) {
+        face_points<R>(A.nbr, A.aux, A.level, e, h,
+            [&](int d, int nb, const R ps[3], const R pn[3], R wj, R sign) {
+                R un[4 * NV];
+                load_coeffs<R, NV>(U, n, nb, un);
+                inner(d, un, nb, ps, pn, wj, sign, nullptr);
+            }, bnd);
+    } else {
+        face_points<R>(A.nbr, A.aux, A.level, e, h, [&](int, int, const R *, const R *, R, R) {}, bnd);
+        face_subfaces<R>(A.nbr, A.aux, A.level, e, h,
+            [&](int d, int nb, R so1, R so2, R sm, R no1, R no2, R nm, R wj, R sign, int js, int) {
+                R un[4 * NV];
+                load_coeffs<R, NV>(U, n, nb, un);
+                R L[7][4];
+                #pragma unroll
+                for (int f = 0; f < 7; f++)
+                    #pragma unroll
+                    for (int c = 0; c < 4; c++) L[f][c] = R(0);
+                for (int k = 0; k < 6; k++) {
+                    R ps[3], pn[3];
+                    face_point<R>(d, k, so1, so2, sm, ps);
+                    face_point<R>(d ^ 1, k, no1, no2, nm, pn);
+                    inner(d, un, nb, ps, pn, wj, sign, L);
+                }
+                const int slot = 4 * d + js;
+                #pragma unroll
+                for (int f = 0; f < 7; f++)
+                    #pragma unroll
+                    for (int c = 0; c < 4; c++) (lift + ((size_t) kLiftFields * slot + 4 * f + c) * n)[e] = L[f][c] * ai[c];
+            });
+    }
+    R *__restrict__ aux = reinterpret_cast<R *>(A.aux_out);
+    #pragma unroll
+    for (int k = 0; k < 4; k++)
         #pragma unroll
-        for (int f = 0; f < 9; f++) (aux + (4 * f + k) * n)[e] = acc[f][k] * ai;
+        for (int f = 0; f < 9; f++) {
+            acc[f][k] *= ai[k];
+            (aux + (4 * f + k) * n)[e] = acc[f][k];
+        }
+    if constexpr (BR2) {
+        // tau_own = tau_h less the liftings this thread has just written
+        for (int d = 0; d < 6; d++) {
+            const int nb = A.nbr[6 * e + d];
+            const int ns = nb >= 0 ? 1 : (nb <= -kNbrAuxBase ? 4 : 0);
+            const int a = d >> 1;
+            for (int j = 0; j < ns; j++) {
+                const R *l = lift + (size_t) kLiftFields * (4 * d + j) * n + e;
+                #pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    #pragma unroll
+                    for (int f = 0; f < 6; f++) acc[f][c] -= l[(4 * f + c) * n];
+                    const R lq = l[(24 + c) * n];
+                    acc[6][c] -= a == 0 ? lq : R(0);
+                    acc[7][c] -= a == 1 ? lq : R(0);
+                    acc[8][c] -= a == 2 ? lq : R(0);
+                }
+            }
+        }
+        R *__restrict__ own = reinterpret_cast<R *>(A.aux_own);
+        #pragma unroll
+        for (int k = 0; k < 4; k++)
+            #pragma unroll
+            for (int f = 0; f < 9; f++) (own + (4 * f + k) * n)[e] = acc[f][k];
     }
 }
 
@@ -445,111 +520,6 @@ void k_ns_visc(NsArgs A, DevParams)
             #pragma unroll
             for (int v = 0; v < 4; v++) acc[v][k] -= q[v] * phi[k] * wj;
     };
-    if constexpr (BR2) {
-        double ds[NC];
-        #pragma unroll
-        for (int i = 0; i < NC; i++) ds[i] = A.dcoef ? A.dcoef[i * n + e] : 0.0;
-        const R *__restrict__ XO = reinterpret_cast<const R *>(A.aux_own);
-        const R eta = cst<R>(P.br2_eta, P.br2_eta_f);
-        face_subfaces<R>(A.nbr, A.aux, A.level, e, h,
-            [&](int d, int nb, R so1, R so2, R sm, R no1, R no2, R nm, R wj, R sign) {
-                const int a = d >> 1;
-                R un[4 * NV];
-                load_coeffs<R, NV>(U, n, nb, un);
-                const R nn[3] = {a == 0 ? R(1) : R(0), a == 1 ? R(1) : R(0), a == 2 ? R(1) : R(0)};
-                // [side][field][coefficient]: side 0 = e, 1 = nb; fields tau xx yy zz
-                // xy xz yz and q_a.  First the liftings, then tau_own + eta l.
-                R cs[2][7][4];
-                #pragma unroll
-                for (int s = 0; s < 2; s++)
-                    #pragma unroll
-                    for (int f = 0; f < 7; f++)
-                        #pragma unroll
-                        for (int k = 0; k < 4; k++) cs[s][f][k] = R(0);
-                for (int k = 0; k < 6; k++) {
-                    R ps[3], pn[3];
-                    face_point<R>(d, k, so1, so2, sm, ps);
-                    face_point<R>(d ^ 1, k, no1, no2, nm, pn);
-                    Prim<R, NC> pa, pb;
-                    state_at<R, NC>(P, u, ps, pa);
-                    state_at<R, NC>(P, un, pn, pb);
-                    R mua, kta, mub, ktb;
-                    transport<R, NC>(P, pa, mua, kta);
-                    transport<R, NC>(P, pb, mub, ktb);
-                    const R fmu = R(0.5) * (mua + mub), kt = R(0.5) * (kta + ktb);
-                    const R lp = R(4.0 / 3.0) * fmu, lm = R(-2.0 / 3.0) * fmu;
-                    const R ta = dv(pa.e, pa.cv), tb = dv(pb.e, pb.cv);
-                    R fc[NC];
-                    double dm[NC];
-                    #pragma unroll
-                    for (int i = 0; i < NC; i++) {
-                        fc[i] = R(0.5) * (pa.c[i] + pb.c[i]);
-                        dm[i] = 0.5 * (ds[i] + (A.dcoef ? A.dcoef[i * n + nb] : 0.0));
-                    }
-                    R q6[6], qq[3], q6a[6], qqa[3], q6b[6], qqb[3];
-                    grad_terms<R, NC>(P, lp, lm, fmu, kt, R(0.5) * (pa.u + pb.u), R(0.5) * (pa.v + pb.v),
-                                      R(0.5) * (pa.w + pb.w), R(0.5) * (ta + tb), R(0.5) * (pa.r + pb.r),
-                                      fc, dm, nn, q6, qq);
-                    grad_terms<R, NC>(P, lp, lm, fmu, kt, pa.u, pa.v, pa.w, ta, pa.r, pa.c, dm, nn, q6a, qqa);
-                    grad_terms<R, NC>(P, lp, lm, fmu, kt, pb.u, pb.v, pb.w, tb, pb.r, pb.c, dm, nn, q6b, qqb);
-                    const R qn = pick3(a, qq[0], qq[1], qq[2]);
-                    const R qna = pick3(a, qqa[0], qqa[1], qqa[2]), qnb = pick3(a, qqb[0], qqb[1], qqb[2]);
-                    const R phs[4] = {R(1), ps[0], ps[1], ps[2]}, phn[4] = {R(1), pn[0], pn[1], pn[2]};
-                    #pragma unroll
-                    for (int c = 0; c < 4; c++) {
-                        // e adds with its side's sign, nb with the other
-                        const R ba = phs[c] * sign * wj, bb = -phn[c] * sign * wj;
-                        #pragma unroll
-                        for (int f = 0; f < 6; f++) {
-                            cs[0][f][c] += (q6[f] - q6a[f]) * ba;
-                            cs[1][f][c] += (q6[f] - q6b[f]) * bb;
-                        }
-                        cs[0][6][c] -= (qn - qna) * ba;
-                        cs[1][6][c] -= (qn - qnb) * bb;
-                    }
-                }
-                // coefficients of tau_own + eta l on each side (A^-1 of the side's level)
-                const int lv[2] = {lev, (int) A.level[nb]};
-                const int el[2] = {e, nb};
-                #pragma unroll
-                for (int s = 0; s < 2; s++)
-                    #pragma unroll
-                    for (int c = 0; c < 4; c++) {
-                        const R ai = cst<R>(P.ainv0[c], P.ainv0_f[c]) * ldexp(R(1), 3 * lv[s]) * eta;
-                        #pragma unroll
-                        for (int f = 0; f < 6; f++) cs[s][f][c] = (XO + (4 * f + c) * n)[el[s]] + cs[s][f][c] * ai;
-                        cs[s][6][c] = (XO + (24 + 4 * a + c) * n)[el[s]] + cs[s][6][c] * ai;
-                    }
-                // the BR1 surface flux with these stresses
-                for (int k = 0; k < 6; k++) {
-                    R ps[3], pn[3];
-                    face_point<R>(d, k, so1, so2, sm, ps);
-                    face_point<R>(d ^ 1, k, no1, no2, nm, pn);
-                    R ftau[6], fu[3] = {R(0), R(0), R(0)}, ft = R(0);
-                    #pragma unroll
-                    for (int f = 0; f < 6; f++) ftau[f] = R(0);
-                    #pragma unroll
-                    for (int s = 0; s < 2; s++) {
-                        const R *pp = s ? pn : ps;
-                        R vel[3];
-                        velocity_at<R, NC>(s ? un : u, pp, vel);
-                        R t[6];
-                        #pragma unroll
-                        for (int f = 0; f < 6; f++) { t[f] = at<R>(cs[s][f], pp[0], pp[1], pp[2]); ftau[f] += t[f]; }
-                        fu[0] += vel[0] * t[0] + vel[1] * t[3] + vel[2] * t[4];
-                        fu[1] += vel[0] * t[3] + vel[1] * t[1] + vel[2] * t[5];
-                        fu[2] += vel[0] * t[4] + vel[1] * t[5] + vel[2] * t[2];
-                        ft += at<R>(cs[s][6], pp[0], pp[1], pp[2]);
-                    }
-                    R q[4];
-                    q[0] = R(0.5) * pick3(a, ftau[0], ftau[3], ftau[4]);
-                    q[1] = R(0.5) * pick3(a, ftau[3], ftau[1], ftau[5]);
-                    q[2] = R(0.5) * pick3(a, ftau[4], ftau[5], ftau[2]);
-                    q[3] = R(0.5) * pick3(a, fu[0], fu[1], fu[2]) - R(0.5) * ft;
-                    add(q, ps, sign * wj);
-                }
-            });
-    }
     face_points<R>(A.nbr, A.aux, A.level, e, h,
         [&](int d, int nb, const R ps[3], const R pn[3], R wj, R sign) {
             if constexpr (BR2) return;               // the interior faces are done above
@@ -610,6 +580,62 @@ void k_ns_visc(NsArgs A, DevParams)
             q[3] = -qn;
             add(q, pc, h * h);
         });
+    if constexpr (BR2) {        // after the walls: x (tau_h) is dead here
+        const R *__restrict__ XO = reinterpret_cast<const R *>(A.aux_own);
+        const R *__restrict__ LF = reinterpret_cast<const R *>(A.lift);
+        const R eta = cst<R>(P.br2_eta, P.br2_eta_f);
+        face_subfaces<R>(A.nbr, A.aux, A.level, e, h,
+            [&](int d, int nb, R so1, R so2, R sm, R no1, R no2, R nm, R wj, R sign, int js, int jn) {
+                const int a = d >> 1;
+                R un[4 * NV];
+                load_coeffs<R, NV>(U, n, nb, un);
+                // [side][field][coefficient], side 0 = e, 1 = nb; fields tau xx yy
+                // zz xy xz yz and q_a: tau_own + eta l, l the side's lifting of
+                // this (sub)face from the gradient pass
+                R cs[2][7][4];
+                const int el[2] = {e, nb};
+                const int slot[2] = {4 * d + js, 4 * (d ^ 1) + jn};
+                #pragma unroll
+                for (int s = 0; s < 2; s++) {
+                    const R *l = LF + (size_t) kLiftFields * slot[s] * n + el[s];
+                    #pragma unroll
+                    for (int c = 0; c < 4; c++) {
+                        #pragma unroll
+                        for (int f = 0; f < 6; f++)
+                            cs[s][f][c] = (XO + (4 * f + c) * n)[el[s]] + eta * l[(4 * f + c) * n];
+                        cs[s][6][c] = (XO + (24 + 4 * a + c) * n)[el[s]] + eta * l[(24 + c) * n];
+                    }
+                }
+                // the BR1 surface flux with these stresses
+                for (int k = 0; k < 6; k++) {
+                    R ps[3], pn[3];
+                    face_point<R>(d, k, so1, so2, sm, ps);
+                    face_point<R>(d ^ 1, k, no1, no2, nm, pn);
+                    R ftau[6], fu[3] = {R(0), R(0), R(0)}, ft = R(0);
+                    #pragma unroll
+                    for (int f = 0; f < 6; f++) ftau[f] = R(0);
+                    #pragma unroll
+                    for (int s = 0; s < 2; s++) {
+                        const R *pp = s ? pn : ps;
+                        R vel[3];
+                        velocity_at<R, NC>(s ? un : u, pp, vel);
+                        R t[6];
+                        #pragma unroll
+                        for (int f = 0; f < 6; f++) { t[f] = at<R>(cs[s][f], pp[0], pp[1], pp[2]); ftau[f] += t[f]; }
+                        fu[0] += vel[0] * t[0] + vel[1] * t[3] + vel[2] * t[4];
+                        fu[1] += vel[0] * t[3] + vel[1] * t[1] + vel[2] * t[5];
+                        fu[2] += vel[0] * t[4] + vel[1] * t[5] + vel[2] * t[2];
+                        ft += at<R>(cs[s][6], pp[0], pp[1], pp[2]);
+                    }
+                    R q[4];
+                    q[0] = R(0.5) * pick3(a, ftau[0], ftau[3], ftau[4]);
+                    q[1] = R(0.5) * pick3(a, ftau[3], ftau[1], ftau[5]);
+                    q[2] = R(0.5) * pick3(a, ftau[4], ftau[5], ftau[2]);
+                    q[3] = R(0.5) * pick3(a, fu[0], fu[1], fu[2]) - R(0.5) * ft;
+                    add(q, ps, sign * wj);
+                }
+            });
+    }
     R *__restrict__ vi = reinterpret_cast<R *>(A.visc_out);
     #pragma unroll
     for (int v = 0; v < 4; v++)
