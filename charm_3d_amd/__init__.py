@@ -136,6 +136,13 @@ def lib():
         "charm_conn_patch": (C.c_int, [vp, C.c_int, C.POINTER(ConnPatch)]),
         "charm_conn_bind_task": (C.c_int, [vp, C.POINTER(Task), i32p, i32p]),
         "charm_task_forest": (C.c_int, [C.POINTER(Task), C.POINTER(vp)]),
+        "charm_gmesh_build": (C.c_int, [i64, dp, i64, i32p, i32p, C.c_int, C.POINTER(vp)]),
+        "charm_gmesh_destroy": (None, [vp]),
+        "charm_gmesh_info": (C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+        "charm_gmesh_leaves": (C.c_int, [vp, i32p]),
+        "charm_gmesh_geometry": (C.c_int, [vp, dp]),
+        "charm_gmesh_faces": (C.c_int, [vp, i32p]),
+        "charm_gpu_upload_gmesh": (C.c_int, [vp, vp]),
         "charm_task_initial_state": (C.c_int, [C.POINTER(Task), vp, dp, i64]),
         "charm_gpu_amr_indicator": (C.c_int, [vp, dp, dp, i64]),
         "charm_gpu_ns_aux": (C.c_int, [vp, dp, i64]),
@@ -270,6 +277,53 @@ def task_initial_state(task, forest):
     u = np.zeros((n, 4 + task.cfg.n_comp, 4))
     check(lib().charm_task_initial_state(C.byref(task), forest.h, _dptr(u), n))
     return u
+
+
+class GMesh:
+    """charm_gmesh_build: a conforming multi-tree forest refined uniformly to
+    `level`, with every leaf's trilinear geometry (FP64, host).  vertices
+    [nv][3], tree_to_vertex [nt][8] (z-order corners), tree_bnd [nt][6] (the
+    boundary slot of a tree face, -1 where it meets another tree)."""
+
+    def __init__(self, vertices, tree_to_vertex, tree_bnd, level):
+        v = np.ascontiguousarray(vertices, dtype=np.float64)
+        t2v = np.ascontiguousarray(tree_to_vertex, dtype=np.int32)
+        tb = np.ascontiguousarray(tree_bnd, dtype=np.int32)
+        self.h = C.c_void_p()
+        check(lib().charm_gmesh_build(len(v), _dptr(v), len(t2v), _i32ptr(t2v), _i32ptr(tb), level,
+                                      C.byref(self.h)))
+        n, nf = C.c_int64(), C.c_int64()
+        check(lib().charm_gmesh_info(self.h, C.byref(n), C.byref(nf)))
+        self.n, self.nf = n.value, nf.value
+
+    def leaves(self):
+        """[n][5] tree, x, y, z, level"""
+        out = np.zeros((self.n, 5), dtype=np.int32)
+        check(lib().charm_gmesh_leaves(self.h, _i32ptr(out)))
+        return out
+
+    def geometry(self):
+        """dict of centre [n][3], dh [n][3], volume [n], a_inv [n][4][4]"""
+        g = np.zeros((self.n, 23))
+        check(lib().charm_gmesh_geometry(self.h, _dptr(g)))
+        return {"c": g[:, 0:3], "dh": g[:, 3:6], "volume": g[:, 6], "a_inv": g[:, 7:23].reshape(-1, 4, 4)}
+
+    def faces(self):
+        """[nf][4]: side-0 leaf, side-1 leaf or -1 - slot, side-0 face, side-1 face"""
+        out = np.zeros((self.nf, 4), dtype=np.int32)
+        check(lib().charm_gmesh_faces(self.h, _i32ptr(out)))
+        return out
+
+    def close(self):
+        if self.h:
+            lib().charm_gmesh_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Connectivity:
@@ -465,6 +519,11 @@ class GpuContext:
     def upload_mesh(self, mesh):
         check(lib().charm_gpu_upload_mesh(self.h, mesh.h))
         self.n = mesh.n
+
+    def upload_gmesh(self, gmesh):
+        """a multi-tree mesh (GMesh) in place of the octree"""
+        check(lib().charm_gpu_upload_gmesh(self.h, gmesh.h))
+        self.n = gmesh.n
 
     def set_state(self, u):
         u = np.ascontiguousarray(u, dtype=np.float64)

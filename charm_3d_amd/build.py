@@ -16,10 +16,10 @@ LIB = os.path.join(PKG, "libcharm3d.so")
 EXE = os.path.join(PKG, "charm_3d")          # the charm_3d program (task.yaml driver)
 
 C_SOURCES = ["charm_forest.c", "charm_geom.c", "charm_model.c", "charm_amr.c", "charm_output.c", "charm_yaml.c", "charm_conn.c",
-             "charm_pointwise.c"]
+             "charm_pointwise.c", "charm_gmesh.c"]
 HIP_SOURCES = ["charm_stage_f32_ph0.hip", "charm_stage_f32_ph1.hip", "charm_stage_f64_ph0.hip",
                "charm_stage_f64_ph1.hip", "charm_stage_mix_ph0.hip", "charm_kernels.hip", "charm_gpu.hip", "charm_amr.hip",
-               "charm_ns.hip", "charm_adj.hip"]
+               "charm_ns.hip", "charm_adj.hip", "charm_gmesh.hip"]
 ARCH = os.environ.get("CHARM_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

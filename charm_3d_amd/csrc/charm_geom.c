@@ -15,7 +15,7 @@
 
 /* Gauss-Jordan inverse of a 4x4 matrix, partial pivoting on |a[r][i]| with
  * ties to the later row; returns 0 on a zero pivot */
-static int gj_inverse4(const double src[4][4], double inv[4][4])
+int charm_gj_inverse4(const double src[4][4], double inv[4][4])
 {
     double a[4][4];
     int i, j, r;
@@ -65,7 +65,7 @@ int charm_level_mass_inverse(int level, double a[4][4], double ainv[4][4])
         for (i = 0; i < 4; i++)
             for (j = 0; j < 4; j++) a[i][j] += wj * phi[i] * phi[j];
     }
-    if (!gj_inverse4((const double (*)[4]) a, ainv)) {
+    if (!charm_gj_inverse4((const double (*)[4]) a, ainv)) {
         charm_set_error("mass matrix of level %d is singular", level);
         return CHARM_ERR_ARG;
     }

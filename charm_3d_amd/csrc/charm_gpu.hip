@@ -45,6 +45,12 @@ struct charm_gpu_ctx {
     void *d_lift;             // [24][28][n] BR2: the (sub)face liftings
     double *d_dcoef;          // [nc][n] diffusion coefficients
     double *d_chem;           // [n] chemistry energy source
+    // multi-tree mesh (charm_gpu_upload_gmesh; gm = 1): the general kernels
+    int gm;
+    int64_t gnf;
+    void *d_glg, *d_gfg, *d_gfq;
+    double *d_gfn;
+    int *d_gfe, *d_glnb, *d_ginc_ptr, *d_ginc;
     // state
     void *B[3];
     double *d_stage64;        // FP64 staging for host transfers
@@ -76,9 +82,12 @@ static void free_mesh(charm_gpu_ctx_t *c)
                   (void **) &c->d_ch_ebeg, (void **) &c->d_ch_fbeg, (void **) &c->d_slot_ptr,
                   (void **) &c->d_nbr, (void **) &c->d_slots, &c->B[0], &c->B[1], &c->B[2],
                   (void **) &c->d_stage64, (void **) &c->d_partial, (void **) &c->d_q,
-                  &c->d_auxf, &c->d_vi, &c->d_auxo, &c->d_lift, (void **) &c->d_dcoef, (void **) &c->d_chem};
+                  &c->d_auxf, &c->d_vi, &c->d_auxo, &c->d_lift, (void **) &c->d_dcoef, (void **) &c->d_chem,
+                  &c->d_glg, &c->d_gfg, &c->d_gfq, (void **) &c->d_gfn, (void **) &c->d_gfe,
+                  (void **) &c->d_glnb, (void **) &c->d_ginc_ptr, (void **) &c->d_ginc};
     for (size_t i = 0; i < sizeof(p) / sizeof(p[0]); i++)
         if (*p[i]) { (void) hipFree(*p[i]); *p[i] = NULL; }
+    c->gm = 0; c->gnf = 0;
     free(c->hq);
     c->hq = NULL;
     c->n = 0;
@@ -327,6 +336,80 @@ extern "C" int charm_gpu_upload_mesh(charm_gpu_ctx_t *c, const charm_mesh_t *m)
     return CHARM_OK;
 }
 
+// device copy of a FP64 host array in the state precision
+static hipError_t upload_real(charm_gpu_ctx_t *c, void *dst, const double *src, size_t count)
+{
+    if (!c->sprec) return hipMemcpy(dst, src, count * sizeof(double), hipMemcpyHostToDevice);
+    float *tmp = (float *) malloc(count * sizeof(float) + 1);
+    if (!tmp) return hipErrorOutOfMemory;
+    for (size_t i = 0; i < count; i++) tmp[i] = (float) src[i];
+    const hipError_t e = hipMemcpy(dst, tmp, count * sizeof(float), hipMemcpyHostToDevice);
+    free(tmp);
+    return e;
+}
+
+extern "C" int charm_gpu_upload_gmesh(charm_gpu_ctx_t *c, const charm_gmesh_t *m)
+{
+    charm_gmesh_arrays_t g;
+    if (!c || !m) { charm_set_error("upload_gmesh: null argument"); return CHARM_ERR_ARG; }
+    if (c->ph || c->prec == CHARM_FP32_MIX || c->cfg.use_visc || c->cfg.n_react > 0) {
+        charm_set_error("upload_gmesh: multi-tree meshes run the constant-cp Euler terms without viscosity or "
+                        "chemistry, in FP32 or FP64");
+        return CHARM_ERR_UNSUPPORTED;
+    }
+    int rc = charm_gmesh_arrays(m, &g);
+    if (rc) return rc;
+    for (int64_t i = 0; i < g.nf; i++)
+        if (g.fe[g.nf + i] < 0 && (-1 - g.fe[g.nf + i] > 5 || c->cfg.bnd_type[-1 - g.fe[g.nf + i]] == CHARM_BND_PERIODIC)) {
+            charm_set_error("upload_gmesh: boundary slot %d is not a boundary type of the configuration",
+                            -1 - g.fe[g.nf + i]);
+            return CHARM_ERR_ARG;
+        }
+    HIPCHK(hipStreamSynchronize(c->st));
+    free_mesh(c);
+    const int64_t n = g.n, nf = g.nf;
+    const size_t ssz = (size_t) n * 4 * c->nv * c->esz;
+    for (int i = 0; i < 3; i++) DMALLOC(c->B[i], ssz);
+    DMALLOC(c->d_stage64, (size_t) n * 4 * c->nv * sizeof(double));
+    DMALLOC(c->d_partial, (size_t) ((n + 255) / 256 + 1) * (c->nv + 1) * sizeof(double));
+    DMALLOC(c->d_glg, (size_t) n * CHARM_GM_LEAFQ * c->esz);
+    DMALLOC(c->d_gfg, (size_t) nf * CHARM_GM_FACEQ * c->esz);
+    DMALLOC(c->d_gfq, (size_t) nf * 2 * 4 * c->nv * c->esz);
+    DMALLOC(c->d_gfn, (size_t) nf * 3 * sizeof(double));
+    DMALLOC(c->d_gfe, (size_t) nf * 2 * sizeof(int));
+    DMALLOC(c->d_glnb, (size_t) n * 6 * sizeof(int));
+    DMALLOC(c->d_ginc_ptr, (size_t) (n + 1) * sizeof(int));
+    DMALLOC(c->d_ginc, (size_t) g.inc_ptr[n] * sizeof(int));
+    hipError_t e = upload_real(c, c->d_glg, g.lg, (size_t) n * CHARM_GM_LEAFQ);
+    if (e == hipSuccess) e = upload_real(c, c->d_gfg, g.fg, (size_t) nf * CHARM_GM_FACEQ);
+    if (e == hipSuccess) e = hipMemcpy(c->d_gfn, g.fn, (size_t) nf * 3 * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c->d_gfe, g.fe, (size_t) nf * 2 * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c->d_glnb, g.lnb, (size_t) n * 6 * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c->d_ginc_ptr, g.inc_ptr, (size_t) (n + 1) * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess && g.inc_ptr[n] > 0)
+        e = hipMemcpy(c->d_ginc, g.inc, (size_t) g.inc_ptr[n] * sizeof(int), hipMemcpyHostToDevice);
+    for (int i = 0; i < 3 && e == hipSuccess; i++) e = hipMemsetAsync(c->B[i], 0, ssz, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    if (e != hipSuccess) {
+        charm_set_error("upload_gmesh: %s", hipGetErrorString(e));
+        free_mesh(c);
+        return CHARM_ERR_HIP;
+    }
+    c->n = n; c->gnf = nf; c->gm = 1;
+    c->nch = 0; c->ncf = 0; c->nslot = 0; c->naux = 0; c->max_faces = 0; c->chunk_elems = 0;
+    return CHARM_OK;
+}
+
+// the entry points that need the cube octree's arrays
+static int need_octree(const charm_gpu_ctx_t *c, const char *what)
+{
+    if (c && c->gm) {
+        charm_set_error("%s: not available on a multi-tree mesh (charm_gpu_upload_gmesh)", what);
+        return CHARM_ERR_UNSUPPORTED;
+    }
+    return CHARM_OK;
+}
+
 static int need_mesh(const charm_gpu_ctx_t *c, int64_t n)
 {
     if (!c) { charm_set_error("null context"); return CHARM_ERR_ARG; }
@@ -449,6 +532,33 @@ static hipError_t ns_prepare(charm_gpu_ctx_t *c, const void *in, void *chem_out,
     return hipSuccess;
 }
 
+// one stage / one limiter pass on the context's mesh: the chunked octree
+// kernels, or the multi-tree face + element passes
+static GArgs gargs(charm_gpu_ctx_t *c, const void *U, const void *U0, void *out, int mode, int combine, double dt)
+{
+    GArgs G;
+    G.U = U; G.U0 = U0; G.out = out; G.fq = c->d_gfq;
+    G.lg = c->d_glg; G.fg = c->d_gfg; G.fn = c->d_gfn;
+    G.fe = c->d_gfe; G.lnb = c->d_glnb; G.inc_ptr = c->d_ginc_ptr; G.inc = c->d_ginc;
+    G.err = c->d_err; G.n = c->n; G.nf = c->gnf;
+    G.mode = mode; G.combine = combine; G.dt = dt;
+    return G;
+}
+
+static hipError_t stage_launch(charm_gpu_ctx_t *c, const StageArgs &A)
+{
+    if (!c->gm) return launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st);
+    return launch_gstage(c->sprec, c->nc, c->cfg.flux, gargs(c, A.U, A.U0, A.out, A.mode, A.combine, A.dt),
+                         c->P, c->st);
+}
+
+static hipError_t limiter_launch(charm_gpu_ctx_t *c, const LimArgs &L)
+{
+    if (!c->gm) return launch_limiter(c->sprec, c->nc, c->ph, L, c->P, c->st);
+    if (!L.limit) return hipErrorInvalidValue;      // every multi-tree call limits
+    return launch_glimiter(c->sprec, c->nc, gargs(c, L.W, L.U0, L.out, 1, L.combine, 0.0), c->P, c->st);
+}
+
 extern "C" int charm_gpu_residual(charm_gpu_ctx_t *c, double *out, int64_t n)
 {
     int rc = need_mesh(c, n);
@@ -458,7 +568,7 @@ extern "C" int charm_gpu_residual(charm_gpu_ctx_t *c, double *out, int64_t n)
         HIPCHK(ns_viscous(c, ns_args(c, c->B[0], NULL, 0.0)));
     }
     StageArgs A = stage_args(c, c->B[0], c->B[0], c->B[1], 0, 0, 0.0);
-    HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st));
+    HIPCHK(stage_launch(c, A));
     rc = check_flag(c);
     if (rc) return rc;
     return copy_out(c, c->B[1], out, n);
@@ -468,6 +578,7 @@ extern "C" int charm_gpu_limiter_bj(charm_gpu_ctx_t *c)
 {
     int rc = need_mesh(c, -1);
     if (rc) return rc;
+    if ((rc = need_octree(c, "limiter_bj"))) return rc;
     LimArgs A = lim_args(c, c->B[0], c->B[0], c->B[1], 1, 0);
     HIPCHK(launch_limiter(c->sprec, c->nc, c->ph, A, c->P, c->st));
     void *t = c->B[0]; c->B[0] = c->B[1]; c->B[1] = t;
@@ -483,10 +594,10 @@ static int step_once(charm_gpu_ctx_t *c, double dt)
     if (c->cfg.model == CHARM_MODEL_EULER) {
         // forward Euler + limiter, src/models/euler/charm_model.c:519-535
         StageArgs A = stage_args(c, B0, B0, B1, 1, 0, dt);
-        HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st));
+        HIPCHK(stage_launch(c, A));
         if (lim) {
             LimArgs L = lim_args(c, B1, B0, B0, 1, 0);
-            HIPCHK(launch_limiter(c->sprec, c->nc, c->ph, L, c->P, c->st));
+            HIPCHK(limiter_launch(c, L));
         } else {
             c->B[0] = B1; c->B[1] = B0;
         }
@@ -500,29 +611,29 @@ static int step_once(charm_gpu_ctx_t *c, double dt)
     if (lim) {
         HIPCHK(ns_prepare(c, B0, B2, dt, &X));
         StageArgs A = stage_args(c, X, B0, B1, 1, 0, dt);
-        HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st));
+        HIPCHK(stage_launch(c, A));
         LimArgs L = lim_args(c, B1, B0, B2, 1, 0);
-        HIPCHK(launch_limiter(c->sprec, c->nc, c->ph, L, c->P, c->st));
+        HIPCHK(limiter_launch(c, L));
         HIPCHK(ns_prepare(c, B2, B2, dt, &X));
         A = stage_args(c, X, B0, B1, 1, 0, dt);
-        HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st));
+        HIPCHK(stage_launch(c, A));
         L = lim_args(c, B1, B0, B2, 1, 1);
-        HIPCHK(launch_limiter(c->sprec, c->nc, c->ph, L, c->P, c->st));
+        HIPCHK(limiter_launch(c, L));
         HIPCHK(ns_prepare(c, B2, B2, dt, &X));
         A = stage_args(c, X, B0, B1, 1, 0, dt);
-        HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st));
+        HIPCHK(stage_launch(c, A));
         L = lim_args(c, B1, B0, B0, 1, 2);
-        HIPCHK(launch_limiter(c->sprec, c->nc, c->ph, L, c->P, c->st));
+        HIPCHK(limiter_launch(c, L));
     } else {
         HIPCHK(ns_prepare(c, B0, B2, dt, &X));
         StageArgs A = stage_args(c, X, B0, B1, 1, 0, dt);
-        HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st));
+        HIPCHK(stage_launch(c, A));
         HIPCHK(ns_prepare(c, B1, B1, dt, &X));
         A = stage_args(c, X, B0, B2, 1, 1, dt);
-        HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st));
+        HIPCHK(stage_launch(c, A));
         HIPCHK(ns_prepare(c, B2, B2, dt, &X));
         A = stage_args(c, X, B0, B0, 1, 2, dt);
-        HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st));
+        HIPCHK(stage_launch(c, A));
     }
     return CHARM_OK;
 }
@@ -652,14 +763,14 @@ extern "C" int charm_gpu_time_kernels(charm_gpu_ctx_t *c, double dt, int reps, d
     float ms = 0.f;
     StageArgs A = stage_args(c, c->B[0], c->B[0], c->B[1], 1, 0, dt);
     HIPCHK(hipEventRecord(c->ev0, c->st));
-    for (int r = 0; r < reps; r++) HIPCHK(launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st));
+    for (int r = 0; r < reps; r++) HIPCHK(stage_launch(c, A));
     HIPCHK(hipEventRecord(c->ev1, c->st));
     HIPCHK(hipEventSynchronize(c->ev1));
     HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     if (stage_ms) *stage_ms = ms / reps;
     LimArgs L = lim_args(c, c->B[1], c->B[0], c->B[2], 1, 1);
     HIPCHK(hipEventRecord(c->ev0, c->st));
-    for (int r = 0; r < reps; r++) HIPCHK(launch_limiter(c->sprec, c->nc, c->ph, L, c->P, c->st));
+    for (int r = 0; r < reps; r++) HIPCHK(limiter_launch(c, L));
     HIPCHK(hipEventRecord(c->ev1, c->st));
     HIPCHK(hipEventSynchronize(c->ev1));
     HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
@@ -690,6 +801,7 @@ extern "C" int charm_gpu_get_dt(charm_gpu_ctx_t *c, int mode, double *dt)
 {
     int rc = need_mesh(c, -1);
     if (rc) return rc;
+    if ((rc = need_octree(c, "get_dt"))) return rc;
     if (!dt || (mode != 0 && mode != 1)) { charm_set_error("get_dt: bad arguments"); return CHARM_ERR_ARG; }
     double r[16];
     if ((rc = reduce(c, mode, r))) return rc;
@@ -701,6 +813,7 @@ extern "C" int charm_gpu_conservation_sums(charm_gpu_ctx_t *c, double *out)
 {
     int rc = need_mesh(c, -1);
     if (rc) return rc;
+    if ((rc = need_octree(c, "conservation_sums"))) return rc;
     if (!out) { charm_set_error("conservation_sums: null output"); return CHARM_ERR_ARG; }
     double r[16];
     if ((rc = reduce(c, 1, r))) return rc;
@@ -719,6 +832,7 @@ extern "C" int charm_gpu_conservation_sums_noalloc(charm_gpu_ctx_t *c, double *o
 {
     int rc = need_mesh(c, -1);
     if (rc) return rc;
+    if ((rc = need_octree(c, "conservation_sums_noalloc"))) return rc;
     if (!out_dev) { charm_set_error("conservation_sums_noalloc: null output"); return CHARM_ERR_ARG; }
     void *scr = NULL;
     size_t bytes = 0;
@@ -833,6 +947,7 @@ extern "C" int charm_gpu_amr_indicator(charm_gpu_ctx_t *c, double *err2, double 
 {
     int rc = need_mesh(c, n);
     if (rc) return rc;
+    if ((rc = need_octree(c, "amr_indicator"))) return rc;
     if (!err2 || !rho_c) { charm_set_error("amr_indicator: null output"); return CHARM_ERR_ARG; }
     AmrArgs A;
     A.U = c->B[0]; A.level = c->d_level; A.nbr = c->d_nbr; A.aux = c->d_aux;
@@ -898,6 +1013,7 @@ extern "C" int charm_gpu_remesh(charm_gpu_ctx_t *c, const charm_mesh_t *m, const
 {
     int rc = need_mesh(c, -1);
     if (rc) return rc;
+    if ((rc = need_octree(c, "remesh"))) return rc;
     if (!m || !map || n_map != m->n) { charm_set_error("remesh: map length must equal the new leaf count"); return CHARM_ERR_ARG; }
     if ((rc = check_map(c->hq, c->n, m->q, m->n, map))) return rc;
     HIPCHK(hipStreamSynchronize(c->st));
@@ -935,6 +1051,7 @@ extern "C" int charm_gpu_ns_aux(charm_gpu_ctx_t *c, double *aux, int64_t n)
 {
     int rc = need_mesh(c, n);
     if (rc) return rc;
+    if ((rc = need_octree(c, "ns_aux"))) return rc;
     if (!c->ph || !aux) { charm_set_error("ns_aux: no Navier-Stokes terms in this configuration"); return CHARM_ERR_ARG; }
     NsArgs N = ns_args(c, c->B[0], NULL, 0.0);
     if (N.dcoef) HIPCHK(launch_ns(c->sprec, 1, c->nc, N, c->P, c->st));
@@ -950,6 +1067,7 @@ extern "C" int charm_gpu_ns_chem(charm_gpu_ctx_t *c, double dt, double *chem_rhs
 {
     int rc = need_mesh(c, n);
     if (rc) return rc;
+    if ((rc = need_octree(c, "ns_chem"))) return rc;
     if (!c->ph || c->cfg.n_react <= 0) { charm_set_error("ns_chem: no reactions in this configuration"); return CHARM_ERR_ARG; }
     NsArgs N = ns_args(c, c->B[0], c->B[0], dt);
     HIPCHK(launch_ns(c->sprec, 0, c->nc, N, c->P, c->st));
@@ -986,6 +1104,7 @@ static int adj_check(charm_gpu_ctx_t *c, int nsteps, int ckpt, int theta_face)
 {
     int rc = need_mesh(c, -1);
     if (rc) return rc;
+    if ((rc = need_octree(c, "adjoint"))) return rc;
     if ((rc = adj_bnd_supported(c))) return rc;
     if (c->cfg.model != CHARM_MODEL_NS || c->cfg.limiter || c->ph || c->nc > 2 || c->prec == CHARM_FP32_MIX ||
         c->cfg.flux == CHARM_FLUX_GODUNOV) {
@@ -1199,6 +1318,7 @@ extern "C" int charm_gpu_stage_vjp(charm_gpu_ctx_t *c, double dt, const double *
 {
     int rc = need_mesh(c, n);
     if (rc) return rc;
+    if ((rc = need_octree(c, "stage_vjp"))) return rc;
     if (!u || !x || !out || !(dt >= 0)) { charm_set_error("stage_vjp: bad argument"); return CHARM_ERR_ARG; }
     if (c->cfg.limiter || c->ph || c->nc > 2 || c->prec == CHARM_FP32_MIX || c->cfg.flux == CHARM_FLUX_GODUNOV) {
         charm_set_error("stage_vjp: needs no limiter, constant cp, 1-2 species and the LF, HLLC or CD flux");

@@ -72,6 +72,25 @@ void   charm_eos_mix_cfg(const charm_config_t *cfg, charm_prim_t *p, int flag);
 uint64_t charm_morton(int32_t x, int32_t y, int32_t z);
 int64_t  charm_forest_find(const charm_forest_t *f, int32_t x, int32_t y, int32_t z);
 void     charm_set_error(const char *fmt, ...);
+/* FP64 Gauss-Jordan inverse with partial pivoting (charm_geom.c); 0 on a zero pivot */
+int      charm_gj_inverse4(const double src[4][4], double inv[4][4]);
+
+/* Multi-tree meshes (charm_gmesh.c): the device arrays, structure of arrays
+ * [q][n] / [q][nf].  Per leaf (CHARM_GM_LEAFQ): 1/dh (0-2), basis values at the
+ * 8 volume points (3-26), their weights wJ (27-34), A^-1 row-major (35-50),
+ * basis values at the 8 vertices (51-74), the leaf means of the 3 linear
+ * basis functions (75-77).  Per face (CHARM_GM_FACEQ): basis values of side 0
+ * (0-17) and side 1 (18-35) at the 6 points, the points' weights wJ (36-41). */
+#define CHARM_GM_LEAFQ 78
+#define CHARM_GM_FACEQ 42
+typedef struct {
+    int64_t n, nf;
+    const double *lg, *fg, *fn;   /* fn: [3][nf] side-0 outward normals */
+    const int32_t *fe;            /* [2][nf] side-0 leaf, side-1 leaf or -1 - boundary slot */
+    const int32_t *lnb;           /* [6][n] neighbour leaf or -1 - boundary face index */
+    const int32_t *inc_ptr, *inc; /* per leaf its faces: face << 1 | side */
+} charm_gmesh_arrays_t;
+int charm_gmesh_arrays(const charm_gmesh_t *m, charm_gmesh_arrays_t *a);
 
 #ifdef __cplusplus
 }

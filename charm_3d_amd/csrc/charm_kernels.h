@@ -80,6 +80,25 @@ struct ProjArgs {
     int nvk;                     // 4 V coefficients per leaf
 };
 
+// Multi-tree (general hexahedral) meshes, charm_gmesh.hip: the stage as a
+// face pass (FQ[2][4V][nf]) and an element pass; the limiter.  Geometry
+// arrays in the state precision, layouts of charm_internal.h (CHARM_GM_*).
+struct GArgs {
+    const void *U;            // stage: the state the residual is evaluated on; limiter: W
+    const void *U0;           // SSP-RK combine base
+    void *out;                // mode 0: the residual integrals; else the new state
+    void *fq;                 // [2][4V][nf] face moments
+    const void *lg, *fg;      // [CHARM_GM_LEAFQ][n], [CHARM_GM_FACEQ][nf]
+    const double *fn;         // [3][nf] side-0 outward normals (FP64: the reference's face frame)
+    const int *fe, *lnb, *inc_ptr, *inc;
+    int *err;
+    int64_t n, nf;
+    int mode, combine;
+    double dt;
+};
+hipError_t launch_gstage(int prec, int nc, int flux, const GArgs &A, const DevParams &P, hipStream_t st);
+hipError_t launch_glimiter(int prec, int nc, const GArgs &A, const DevParams &P, hipStream_t st);
+
 // Navier-Stokes kernels (one thread per leaf).  BR2 liftings: per leaf 24
 // (sub)face slots (direction d * 4 + sub-face j) of kLiftFields coefficients
 // (tau xx yy zz xy xz yz and q along the face axis, 4 each)

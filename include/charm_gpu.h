@@ -382,6 +382,31 @@ int  charm_conn_patch(const charm_conn_t *c, int i, charm_conn_patch_t *p);
  * a patch, tree_reg [nt] = index into task->region; a name the task lacks is
  * CHARM_ERR_MESH with the reference's message */
 int  charm_conn_bind_task(const charm_conn_t *c, const charm_task_t *task, int32_t *tree_bnd, int32_t *tree_reg);
+/* ------------------------------------------------- multi-tree meshes
+ * The forest the reference builds over a connectivity, refined uniformly
+ * (p4est_new_ext at MIN_LEVEL, src/charm_3d.c:36-37), with every leaf's
+ * trilinear geometry (charm_geom_quad_calc, src/charm_geom.c:12-172,496-530)
+ * in FP64 on the host.  Conforming: no refinement across trees.
+ * vertices [nv][3], tree_to_vertex [nt][8] (z-order corners), tree_bnd
+ * [nt][6] = the boundary slot (0..5, an index into charm_config_t's
+ * bnd_type / bnd_param) of a tree face, or -1 where it meets another tree;
+ * level 0..6.  Leaves are numbered tree by tree, Morton order inside a tree. */
+typedef struct charm_gmesh charm_gmesh_t;
+int  charm_gmesh_build(int64_t n_vertices, const double *vertices, int64_t n_trees, const int32_t *tree_to_vertex,
+                       const int32_t *tree_bnd, int level, charm_gmesh_t **out);
+void charm_gmesh_destroy(charm_gmesh_t *m);
+int  charm_gmesh_info(const charm_gmesh_t *m, int64_t *n_leaves, int64_t *n_faces);
+/* per leaf: tree, x, y, z, level (tree coordinates in units of 2^-19) */
+int  charm_gmesh_leaves(const charm_gmesh_t *m, int32_t *leaves);
+/* per leaf 23 doubles: centre[3], dh[3], volume, A^-1[4][4] */
+int  charm_gmesh_geometry(const charm_gmesh_t *m, double *out);
+/* per face: side-0 leaf, side-1 leaf or -1 - boundary slot, side-0 face, side-1 face (-1 on a boundary) */
+int  charm_gmesh_faces(const charm_gmesh_t *m, int32_t *faces);
+/* the context runs on the mesh from now on (replacing a charm_gpu_upload_mesh
+ * octree): the Euler and SSP-RK3 models with constant cp, no viscosity and no
+ * chemistry, every flux, BJ, every boundary type.  The state layout is the
+ * octree's, [n][V][4], in this mesh's leaf order. */
+int  charm_gpu_upload_gmesh(charm_gpu_ctx_t *c, const charm_gmesh_t *m);
 /* ------------------------------------------------ reference entry points
  * (include/charm_models.h:15-23, charm_limiter.h:13-15, charm_fluxes.h:19-32,
  * charm_amr.h:9-10)

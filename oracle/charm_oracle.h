@@ -97,6 +97,11 @@ typedef struct or_sim or_sim_t;
 
 /* lifecycle: leaves given as int32 [n][4] = (x, y, z, level) in root units */
 or_sim_t *or_create(const or_config_t *cfg, int64_t n_leaves, const int32_t *leaves);
+/* multi-tree forest refined uniformly to `level`: tree_vertices [n_trees][8][3]
+ * (z-order corners), tree_bnd [n_trees][6] = the boundary slot of a tree face
+ * or -1 where it meets another tree (conforming meshes) */
+or_sim_t *or_create_trees(const or_config_t *cfg, int64_t n_trees, const double *tree_vertices,
+                          const int32_t *tree_bnd, int level);
 void      or_destroy(or_sim_t *s);
 int       or_nvar(const or_sim_t *s);
 int64_t   or_n_faces(const or_sim_t *s);

@@ -78,6 +78,7 @@ typedef struct {
     double int_tau_own[6][OR_BF];
     double int_q_own[3][OR_BF];
     int32_t quad[4];                         /* x y z level */
+    int32_t tree;                            /* multi-tree forests: the leaf's tree */
 } or_data_t;
 
 /* one face visit in the sense of p4est_iter_face_info_t */
@@ -108,7 +109,11 @@ struct or_sim {
     int64_t    *fq_off;      /* [nf + 1], in doubles */
     double     *fq;
     double     *lq;          /* [nf][2][OR_NV_MAX]: the limiter's per-face neighbour values */
+    double     *tv;          /* multi-tree forests: [n_trees][8][3] tree vertices (NULL: the unit cube) */
 };
+
+/* the 8 vertices of leaf d (its tree's trilinear map) */
+void or_leaf_vertices(const or_sim_t *s, const or_data_t *d, double v[8][3]);
 
 double or_scalar_prod(const double a[3], const double b[3]);
 double or_vect_length(const double v[3]);
@@ -124,6 +129,9 @@ void   or_prim_to_cons(const or_config_t *cfg, or_cons_t *c, const or_prim_t *p)
 int    or_flux_err(const or_config_t *cfg, int flux, const or_prim_t prim[2],
                    const double n[3], double *q);
 void   or_quad_vertices(const int32_t q[4], double v[8][3]);
+extern const double or_unit_tree[8][3];
+void   or_quad_vertices_tv(const double tv[8][3], const int32_t q[4], double v[8][3]);
+void   or_geom_calc_tv(const double tv[8][3], const int32_t q[4], or_geom_t *g);
 double or_bf(const or_geom_t *g, const double x[3], int k);
 double or_bf_dx(const or_geom_t *g, int k);
 double or_bf_dy(const or_geom_t *g, int k);

@@ -190,10 +190,129 @@ or_sim_t *or_create(const or_config_t *cfg, int64_t n_leaves, const int32_t *lea
     return s;
 }
 
+void or_leaf_vertices(const or_sim_t *s, const or_data_t *d, double v[8][3])
+{
+    if (s->tv) or_quad_vertices_tv((const double (*)[3]) (s->tv + 24 * (size_t) d->tree), d->quad, v);
+    else or_quad_vertices(d->quad, v);
+}
+
+/* Multi-tree forest: every tree refined uniformly to `level` (leaves in
+ * tree order, Morton order inside a tree, the order of p4est's global
+ * numbering), geometry from each tree's vertices (src/charm_geom.c with
+ * p4est_qcoord_to_vertex over the tree's own corners), faces: inside a tree
+ * as the single-tree walk; on a tree face with a boundary slot
+ * (tree_bnd >= 0) a boundary face; every other tree-face leaf face is paired
+ * with the one leaf face elsewhere whose centre coincides with it (brute
+ * force over those faces, tolerance 1e-9 of the face's size).  Side 0 of a
+ * pair is the leaf with the smaller index.  Conforming forests only. */
+or_sim_t *or_create_trees(const or_config_t *cfg, int64_t n_trees, const double *tree_vertices,
+                          const int32_t *tree_bnd, int level)
+{
+    or_sim_t *s = (or_sim_t *) calloc(1, sizeof(or_sim_t));
+    const int64_t nt = (int64_t) 1 << (3 * level), nside = (int64_t) 1 << level;
+    const int32_t len = 1 << (OR_MAXLEVEL - level);
+    int64_t e, cap = 0, nc = 0, i, j;
+    int a;
+    struct cand { int64_t e; int f; double x[3], tol; int used; } *cd = NULL;
+    if (!s) return NULL;
+    s->cfg = *cfg;
+    s->n = n_trees * nt;
+    s->nv = 4 + cfg->n_comp;
+    s->d = (or_data_t *) calloc((size_t) s->n, sizeof(or_data_t));
+    s->tv = (double *) malloc((size_t) n_trees * 24 * sizeof(double));
+    cd = (struct cand *) malloc((size_t) (s->n * 6 + 1) * sizeof(*cd));
+    if (!s->d || !s->tv || !cd || level < 0 || level > 8) {
+        snprintf(s->err, sizeof(s->err), "out of memory or bad level");
+        free(cd);
+        return s;
+    }
+    memcpy(s->tv, tree_vertices, (size_t) n_trees * 24 * sizeof(double));
+    for (e = 0; e < s->n; e++) {
+        or_data_t *d = &s->d[e];
+        const int64_t m = e % nt;
+        int32_t c[3] = {0, 0, 0};
+        int b;
+        for (b = 0; b < level; b++)
+            for (a = 0; a < 3; a++) c[a] |= (int32_t) (((m >> (3 * b + a)) & 1) << b);
+        d->tree = (int32_t) (e / nt);
+        d->quad[0] = c[0] * len; d->quad[1] = c[1] * len; d->quad[2] = c[2] * len; d->quad[3] = level;
+        or_geom_calc_tv((const double (*)[3]) (s->tv + 24 * (size_t) d->tree), d->quad, &d->g);
+        d->grav[0] = cfg->grav[0]; d->grav[1] = cfg->grav[1]; d->grav[2] = cfg->grav[2];
+    }
+    for (e = 0; e < s->n; e++) {
+        const or_data_t *d = &s->d[e];
+        const int64_t t = d->tree, base = t * nt;
+        for (a = 0; a < 3; a++) {
+            int side;
+            for (side = 0; side < 2; side++) {
+                const int f = 2 * a + side;
+                const int32_t ci = d->quad[a] / len;
+                or_face_t fr;
+                memset(&fr, 0, sizeof(fr));
+                fr.wrap_axis = -1;
+                if ((side == 1 && ci + 1 < nside) || (side == 0 && ci > 0)) {
+                    if (side == 1) {                    /* inside the tree, from the lower leaf */
+                        int32_t p[3] = {d->quad[0] / len, d->quad[1] / len, d->quad[2] / len};
+                        int64_t mm = 0;
+                        int b;
+                        p[a] += 1;
+                        for (b = 0; b < level; b++)
+                            for (j = 0; j < 3; j++) mm |= (int64_t) ((p[j] >> b) & 1) << (3 * b + j);
+                        fr.nsides = 2; fr.face[0] = f; fr.face[1] = f - 1;
+                        fr.quad[0][0] = (int) e; fr.quad[1][0] = (int) (base + mm);
+                        if (or_face_push(s, &cap, &fr)) goto fail;
+                    }
+                    continue;
+                }
+                if (tree_bnd[6 * t + f] >= 0) {
+                    fr.nsides = 1; fr.face[0] = f; fr.quad[0][0] = (int) e; fr.bnd = tree_bnd[6 * t + f];
+                    if (or_face_push(s, &cap, &fr)) goto fail;
+                    continue;
+                }
+                cd[nc].e = e; cd[nc].f = f; cd[nc].used = 0;
+                memcpy(cd[nc].x, d->g.fc[f], sizeof(cd[nc].x));
+                cd[nc].tol = 1e-9 * sqrt(d->g.area[f]);
+                nc++;
+            }
+        }
+    }
+    for (i = 0; i < nc; i++) {
+        int64_t hit = -1;
+        if (cd[i].used) continue;
+        for (j = i + 1; j < nc; j++) {
+            double dd = 0.;
+            int k;
+            if (cd[j].used) continue;
+            for (k = 0; k < 3; k++) dd = OR_MAX(dd, fabs(cd[i].x[k] - cd[j].x[k]));
+            if (dd <= cd[i].tol) {
+                if (hit >= 0) { snprintf(s->err, sizeof(s->err), "tree face: two matches"); goto fail; }
+                hit = j;
+            }
+        }
+        if (hit < 0) { snprintf(s->err, sizeof(s->err), "tree face without a partner or a boundary"); goto fail; }
+        cd[i].used = cd[hit].used = 1;
+        {
+            or_face_t fr;
+            memset(&fr, 0, sizeof(fr));
+            fr.wrap_axis = -1;
+            fr.nsides = 2;
+            fr.face[0] = cd[i].f; fr.face[1] = cd[hit].f;          /* cd is in leaf order */
+            fr.quad[0][0] = (int) cd[i].e; fr.quad[1][0] = (int) cd[hit].e;
+            if (or_face_push(s, &cap, &fr)) goto fail;
+        }
+    }
+    free(cd);
+    return s;
+fail:
+    free(cd);
+    if (!s->err[0]) snprintf(s->err, sizeof(s->err), "face build failed");
+    return s;
+}
+
 void or_destroy(or_sim_t *s)
 {
     if (!s) return;
-    free(s->inc_ptr); free(s->inc); free(s->fq_off); free(s->fq); free(s->lq);
+    free(s->inc_ptr); free(s->inc); free(s->fq_off); free(s->fq); free(s->lq); free(s->tv);
     free(s->d); free(s->f); free(s);
 }
 
@@ -805,7 +924,7 @@ void or_limiter(or_sim_t *s)
             }
             psi[v] = 1.;
         }
-        or_quad_vertices(d->quad, vtx);
+        or_leaf_vertices(s, d, vtx);
 #ifdef OR_MUTANT_BJ_FACE_CENTRES
         /* planted mistake (never set by the Makefile): the six face centres
          * instead of the eight vertices, for tests/test_oracle_mutants.py */

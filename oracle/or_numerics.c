@@ -697,15 +697,17 @@ void or_flux(const or_config_t *cfg, int flux, const or_prim_t prim[2],
 }
 
 /* --------------------------------------------------------------- geometry
- * The tree is the unit cube [0,1]^3 with vertices in z-order.  Vertex
- * coordinates follow p4est_qcoord_to_vertex (p4est a9910fd,
- * src/p4est_connectivity.c): trilinear blend of the 8 tree vertices with
- * weights w[1] = q / root_len, w[0] = 1 - w[1], summed z, y, x outer to
- * inner.  Called by the reference at src/charm_geom.c:26,36,196,267,283. */
-static void or_qcoord_to_vertex(int32_t x, int32_t y, int32_t z, double v[3])
+ * A tree has 8 vertices in z-order (the unit cube [0,1]^3 for the
+ * single-tree forests).  Vertex coordinates follow p4est_qcoord_to_vertex
+ * (p4est a9910fd, src/p4est_connectivity.c): trilinear blend of the 8 tree
+ * vertices with weights w[1] = q / root_len, w[0] = 1 - w[1], summed z, y, x
+ * outer to inner.  Called by the reference at src/charm_geom.c:26,36,196,
+ * 267,283. */
+const double or_unit_tree[8][3] = {
+    {0,0,0},{1,0,0},{0,1,0},{1,1,0},{0,0,1},{1,0,1},{0,1,1},{1,1,1}};
+
+static void or_qcoord_to_vertex(const double tv[8][3], int32_t x, int32_t y, int32_t z, double v[3])
 {
-    static const double tv[8][3] = {
-        {0,0,0},{1,0,0},{0,1,0},{1,1,0},{0,0,1},{1,0,1},{0,1,1},{1,1,1}};
     double wx[2], wy[2], wz[2], xf, yf;
     int xi, yi, zi, vi = 0;
     v[0] = v[1] = v[2] = 0.;
@@ -726,14 +728,14 @@ static void or_qcoord_to_vertex(int32_t x, int32_t y, int32_t z, double v[3])
 }
 
 /* src/charm_geom.c:12-31 */
-void or_quad_vertices(const int32_t q[4], double v[8][3])
+void or_quad_vertices_tv(const double tv[8][3], const int32_t q[4], double v[8][3])
 {
     int32_t l = 1 << (OR_MAXLEVEL - q[3]);
     int i = 0, ix, iy, iz;
     for (iz = 0; iz < 2; iz++)
         for (iy = 0; iy < 2; iy++)
             for (ix = 0; ix < 2; ix++)
-                or_qcoord_to_vertex(q[0] + ix*l, q[1] + iy*l, q[2] + iz*l, v[i++]);
+                or_qcoord_to_vertex(tv, q[0] + ix*l, q[1] + iy*l, q[2] + iz*l, v[i++]);
 }
 
 /* src/charm_geom.c:40-128 */
@@ -811,7 +813,7 @@ double or_bf_dz(const or_geom_t *g, int k) { return k == 3 ? 1./g->dh[2] : 0.; }
 
 /* charm_geom_quad_calc and helpers, src/charm_geom.c:130-211,255-268,
  * 443-530 */
-void or_geom_calc(const int32_t q[4], or_geom_t *g)
+void or_geom_calc_tv(const double tv[8][3], const int32_t q[4], or_geom_t *g)
 {
     int32_t l = 1 << (OR_MAXLEVEL - q[3]), l2 = l / 2;
     double v[8][3], x[4][3], e[2][3], nl;
@@ -823,9 +825,9 @@ void or_geom_calc(const int32_t q[4], or_geom_t *g)
     const int32_t fc[6][3] = {{0,l2,l2},{l,l2,l2},{l2,0,l2},{l2,l,l2},{l2,l2,0},{l2,l2,l}};
 
     for (f = 0; f < 6; f++) {
-        or_qcoord_to_vertex(q[0]+fc[f][0], q[1]+fc[f][1], q[2]+fc[f][2], g->fc[f]);
+        or_qcoord_to_vertex(tv, q[0]+fc[f][0], q[1]+fc[f][1], q[2]+fc[f][2], g->fc[f]);
         for (i = 0; i < 4; i++)
-            or_qcoord_to_vertex(q[0] + l*(or_ftv[f][i]%2), q[1] + l*((or_ftv[f][i]/2)%2),
+            or_qcoord_to_vertex(tv, q[0] + l*(or_ftv[f][i]%2), q[1] + l*((or_ftv[f][i]/2)%2),
                                 q[2] + l*(or_ftv[f][i]/4), x[i]);
         for (i = 0; i < 3; i++) { e[0][i] = x[1][i]-x[0][i]; e[1][i] = x[2][i]-x[0][i]; }
         or_vect_prod(e[0], e[1], g->n[f]);
@@ -856,8 +858,8 @@ void or_geom_calc(const int32_t q[4], or_geom_t *g)
         g->area[f] = 0.;
         for (j = 0; j < OR_FGP; j++) g->area[f] += g->face_gw[f][j]*g->face_gj[f][j];
     }
-    or_qcoord_to_vertex(q[0]+l2, q[1]+l2, q[2]+l2, g->c);
-    or_quad_vertices(q, v);
+    or_qcoord_to_vertex(tv, q[0]+l2, q[1]+l2, q[2]+l2, g->c);
+    or_quad_vertices_tv(tv, q, v);
     for (i = 0; i < OR_QGP; i++) {
         or_quad_gp_at_point(v, tq[i], g->quad_gp[i], &g->quad_gj[i]);
         g->quad_gw[i] = 1.;
@@ -879,4 +881,15 @@ void or_geom_calc(const int32_t q[4], or_geom_t *g)
                         * or_bf(g, g->quad_gp[ig], i) * or_bf(g, g->quad_gp[ig], j);
     }
     or_matr_inv(g->a, g->a_inv);
+}
+
+/* the single-tree forests: the unit cube */
+void or_quad_vertices(const int32_t q[4], double v[8][3])
+{
+    or_quad_vertices_tv(or_unit_tree, q, v);
+}
+
+void or_geom_calc(const int32_t q[4], or_geom_t *g)
+{
+    or_geom_calc_tv(or_unit_tree, q, g);
 }

@@ -73,6 +73,8 @@ def lib():
         vp, i64, dp, ip = C.c_void_p, C.c_int64, C.POINTER(C.c_double), C.POINTER(C.c_int32)
         L.or_create.restype = vp
         L.or_create.argtypes = [C.POINTER(Config), i64, ip]
+        L.or_create_trees.restype = vp
+        L.or_create_trees.argtypes = [C.POINTER(Config), i64, dp, ip, C.c_int]
         L.or_destroy.argtypes = [vp]
         L.or_nvar.argtypes = [vp]
         L.or_n_faces.restype = i64
@@ -166,13 +168,25 @@ def make_config(n_comp=1, comp_m=(GAS_R,), comp_cp=((3.5,),), eos="IDEAL", flux=
 class OracleSim:
     """One oracle simulation over a fixed leaf set."""
 
-    def __init__(self, cfg, leaves):
+    def __init__(self, cfg, leaves, trees=None):
+        """leaves: int32 [n][4] of the unit-cube forest; or trees = (tree_vertices
+        [nt][8][3], tree_bnd [nt][6], level) for a multi-tree forest refined
+        uniformly (leaves is then ignored)."""
         self.cfg = cfg
-        self.leaves = np.ascontiguousarray(leaves, dtype=np.int32)
-        self.n = len(self.leaves)
         L = lib()
-        self.h = L.or_create(C.byref(cfg), self.n,
-                             self.leaves.ctypes.data_as(C.POINTER(C.c_int32)))
+        if trees is None:
+            self.leaves = np.ascontiguousarray(leaves, dtype=np.int32)
+            self.n = len(self.leaves)
+            self.h = L.or_create(C.byref(cfg), self.n,
+                                 self.leaves.ctypes.data_as(C.POINTER(C.c_int32)))
+        else:
+            tv, tb, level = trees
+            self._tv = np.ascontiguousarray(tv, dtype=np.float64)
+            self._tb = np.ascontiguousarray(tb, dtype=np.int32)
+            self.h = L.or_create_trees(C.byref(cfg), len(self._tv), _dptr(self._tv),
+                                       self._tb.ctypes.data_as(C.POINTER(C.c_int32)), level)
+            self.n = len(self._tv) * 8 ** level
+            self.leaves = None
         self.nv = L.or_nvar(self.h)
         err = L.or_error(self.h)
         if err:
