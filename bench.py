@@ -11,7 +11,8 @@ value = DOF-updates/s of the FP32 path (E * 4 * V * 3 stages * K / wall).
 
 The JSON line also carries the other configurations at full size (adaptive
 bubbles, reacting hot spot, adjoint of the pulse case with its FP64 reference
-gradient checked against central differences) under "workloads",
+gradient checked against central differences, the Riemann box on a multi-tree
+mesh of distorted hexahedra) under "workloads",
 the FP32 errors of the smooth cases, the FP64 path's own sensitivity to the
 FP32 rounding of the initial state, and the stage kernel's HBM traffic per
 launch from two rocprofv3 counter passes (child processes).
@@ -293,12 +294,13 @@ def measure_traffic(level, copy_bytes, precision):
 
 def extra_workloads(CA, W):
     """The other configurations at their full sizes (FP32), each a few
-    seconds: the adaptive bubble case, the reacting hot spot, and the
-    adjoint of the pulse case (adjoint / forward wall-time ratio).  A case
+    seconds: the adaptive bubble case, the reacting hot spot, the adjoint of
+    the pulse case (adjoint / forward wall-time ratio) and the Riemann box on
+    a multi-tree mesh of distorted hexahedra.  A case
     that fails reports its error instead of a measurement."""
     out = {}
     for name, fn in (("bubbles_adaptive", bench_bubbles), ("reacting_hot_spot", bench_hot_spot),
-                     ("pulse_adjoint", bench_adjoint)):
+                     ("pulse_adjoint", bench_adjoint), ("multi_tree_riemann_box", bench_multi_tree)):
         print("[bench]   %s" % name, file=sys.stderr, flush=True)
         try:
             out[name] = fn(CA, W)
@@ -409,6 +411,59 @@ def bench_hot_spot(CA, W):
     return out
 
 
+def gmesh_stage_bytes(nv, real_bytes, faces_per_leaf):
+    """HBM bytes per leaf of one multi-tree stage (face pass + element pass)
+    by design: per face the 42 basis / weight words, the FP64 normal, two
+    leaf indices, both sides' states and both sides' moment sums; per leaf
+    its 51 geometry words, the state read and written, its faces' moment
+    sums and its incidence list."""
+    s = real_bytes * 4 * nv
+    face = real_bytes * 42 + 8 * 3 + 4 * 2 + 2 * s + 2 * s
+    leaf = real_bytes * 51 + 2 * s + 2 * faces_per_leaf * s + 4 + 4 * 2 * faces_per_leaf
+    return faces_per_leaf * face + leaf
+
+
+def bench_multi_tree(CA, W, steps=20):
+    # the Riemann box on 4^3 distorted trees at level 5 (2,097,152 non-affine
+    # hexahedra, the octree benchmark's size): FP32 and FP64, 20 steps
+    t0 = time.perf_counter()
+    runs = {}
+    for prec in (CA.FP32, CA.FP64):
+        cfg, gm, u0, dt = W.c2_multi_tree(precision=prec)
+        t_build = time.perf_counter() - t0
+        ctx = CA.GpuContext(cfg)
+        ctx.upload_gmesh(gm)
+        ctx.set_state(u0)
+        ctx.run(dt, 2)
+        ctx.set_state(u0)
+        ctx.synchronize()
+        t = time.perf_counter()
+        ctx.run(dt, steps)
+        wall = time.perf_counter() - t
+        ms = ctx.last_ms()
+        state = ctx.get_state()
+        ctx.set_state(u0)
+        stage_ms, lim_ms = ctx.time_kernels(dt, 10)
+        runs[prec] = dict(wall=wall, ms=ms, state=state, stage_ms=stage_ms, lim_ms=lim_ms, n=gm.n, nf=gm.nf,
+                          build_s=t_build)
+        ctx.close()
+        gm.close()
+        t0 = time.perf_counter()
+    r, q = runs[CA.FP32], runs[CA.FP64]
+    n, fpl = r["n"], r["nf"] / r["n"]
+    out = {"leaves": n, "faces": r["nf"], "trees": 64, "level": 5, "steps": steps,
+           "mesh": "unit box of 4^3 trees, vertices displaced by up to 0.1 of the tree size, refined to level 5",
+           "host_mesh_build_s": r["build_s"]}
+    for name, x, rb in (("fp32", r, 4), ("fp64", q, 8)):
+        b = gmesh_stage_bytes(5, rb, fpl)
+        out[name] = {"dof_updates_per_s": W.dof_updates(n, 5, 3, steps) / x["wall"],
+                     "ms_per_step": 1e3 * x["wall"] / steps, "device_ms_per_step": x["ms"] / steps,
+                     "stage_ms": x["stage_ms"], "limiter_ms": x["lim_ms"], "stage_bytes_per_leaf": b,
+                     "stage_hbm_frac": b * n / (x["stage_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    out["fp32"]["errors_vs_fp64"] = rel_errors(r["state"], q["state"])
+    return out
+
+
 def bench_adjoint(CA, W, fd_check=True):
     # adjoint of the pulse case: level 6, 400 steps, checkpoints every 20
     cfg, leaves, u0, per = W.c5_pulse(6, CA.FP32)
@@ -516,7 +571,7 @@ def main():
     ap.add_argument("--cpu-run", type=int, nargs=3, default=None, help=argparse.SUPPRESS)
     ap.add_argument("--no-fd-check", action="store_true",
                     help="with --only adjoint: the FP32 adjoint and its timing, without the FP64 reference gradient")
-    ap.add_argument("--only", choices=("bubbles", "hot_spot", "adjoint"),
+    ap.add_argument("--only", choices=("bubbles", "hot_spot", "adjoint", "multi_tree"),
                     help="run one of the other configurations alone and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -534,7 +589,7 @@ def main():
         traffic_probe(CA, W, args.level, args.probe_precision)
         return
     if args.only:
-        fn = {"bubbles": bench_bubbles, "hot_spot": bench_hot_spot,
+        fn = {"bubbles": bench_bubbles, "hot_spot": bench_hot_spot, "multi_tree": bench_multi_tree,
               "adjoint": lambda CA, W: bench_adjoint(CA, W, fd_check=not args.no_fd_check)}[args.only]
         print(json.dumps(fn(CA, W)))
         return

@@ -76,6 +76,52 @@ def c2(level=7, precision=1, seed=2):
     return cfg, leaves, u, (False, False, False)
 
 
+def distorted_box_trees(n=4, distort=0.1, seed=2):
+    """A unit box of n^3 hexahedral trees whose vertices are displaced by up
+    to `distort` of the tree size (interior vertices in 3-D, vertices on a
+    side inside that side's plane), so leaves are non-affine hexahedra with
+    non-planar faces.  Returns vertices [nv][3], tree_to_vertex [nt][8]
+    (z-order corners), tree_bnd [nt][6] (the box side's slot 0-5 on the
+    box's sides, -1 between trees)."""
+    rng = np.random.default_rng(seed)
+    g = np.linspace(0.0, 1.0, n + 1)
+    idx = np.stack(np.meshgrid(np.arange(n + 1), np.arange(n + 1), np.arange(n + 1), indexing="ij"), -1)
+    v = g[idx].reshape(-1, 3).copy()
+    free = ((idx > 0) & (idx < n)).reshape(-1, 3)         # axes along which a vertex may move
+    v += np.where(free, distort / n * rng.uniform(-1, 1, v.shape), 0.0)
+    vid = np.arange((n + 1) ** 3).reshape(n + 1, n + 1, n + 1)
+    t = np.stack(np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij"), -1).reshape(-1, 3)
+    t2v = np.stack([vid[t[:, 0] + (c & 1), t[:, 1] + ((c >> 1) & 1), t[:, 2] + (c >> 2)] for c in range(8)], 1)
+    tb = np.full((len(t), 6), -1, np.int32)
+    for f in range(6):
+        a, hi = f // 2, f % 2
+        tb[t[:, a] == (n - 1 if hi else 0), f] = f
+    return v, t2v.astype(np.int32), tb
+
+
+def c2_multi_tree(n=4, level=5, precision=1, seed=2, distort=0.1):
+    """The Riemann box of c2 on a multi-tree mesh: n^3 distorted trees
+    (distorted_box_trees) refined uniformly to `level` (n = 4, level 5:
+    2,097,152 leaves, the size of the level-7 octree), cell means from the
+    leaf centres, P1 slopes 0, HLLC + BJ, SSP-RK3, transmissive boundaries.
+    dt = 0.3 min(dh) / max(|u| + c), dh the leaves' bounding-box extents.
+    Returns (cfg, GMesh, u0, dt)."""
+    from . import GMesh
+    v, t2v, tb = distorted_box_trees(n, distort, seed)
+    gm = GMesh(v, t2v, tb, level)
+    c = gm.geometry()["c"]
+    rng = np.random.default_rng(seed)
+    left = c[:, 0] < 0.5
+    rho = np.where(left, 1.0, 0.125) * (1 + 0.01 * rng.uniform(-1, 1, len(c)))
+    p = np.where(left, 1.0, 0.1) * (1 + 0.01 * rng.uniform(-1, 1, len(c)))
+    u = np.zeros((len(c), 5, 4))
+    u[:, :, 0] = prim_to_cons(rho, np.zeros((len(c), 3)), p)
+    cfg = make_config(precision=precision, model="NS", flux="HLLC", limiter=True,
+                      bnd=("OUTLET",) * 6, comp_m=(GAS_R,), comp_cp=(3.5,), cfl=0.3)
+    dt = 0.3 * gm.geometry()["dh"].min() / np.sqrt(GAMMA * p / rho).max()
+    return cfg, gm, u, dt
+
+
 C5_BOX = ((0.6, 0.6, 0.6), (0.8, 0.8, 0.8))
 
 
