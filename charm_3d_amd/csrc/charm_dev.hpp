@@ -32,6 +32,7 @@ struct DevRef {
     float r, u[3], p;
     float m[3], E, Epp;          // rounded rho_r u_r, E_r, E_r + p_r (multiply small terms only)
     double c0[5];                // exact (m_r[3], E_r, rho_r): subtracted from coefficient 0
+    float c0h[5], c0l[5];        // c0 as the sum of two floats, c0h = c0 rounded
     float gam, gm1, gq;          // gamma, gamma - 1, (gamma + 1) / (2 gamma)
 };
 
@@ -59,6 +60,11 @@ __device__ __forceinline__ DevRef make_ref(const DevParams &P, double m0, double
     R.c0[4] = (double) R.r;
     #pragma unroll
     for (int d = 0; d < 3; d++) R.m[d] = (float) R.c0[d];
+    #pragma unroll
+    for (int i = 0; i < 5; i++) {
+        R.c0h[i] = (float) R.c0[i];
+        R.c0l[i] = (float) (R.c0[i] - (double) R.c0h[i]);
+    }
     R.E = (float) R.c0[3];
     R.Epp = (float) (R.c0[3] + (double) R.p);
     return R;
@@ -82,6 +88,14 @@ __device__ __forceinline__ DevRef permute_ref(const DevRef &R, int a)
     Q.r = R.r; Q.p = R.p; Q.E = R.E; Q.Epp = R.Epp;
     Q.gam = R.gam; Q.gm1 = R.gm1; Q.gq = R.gq;
     Q.c0[3] = R.c0[3]; Q.c0[4] = R.c0[4];
+    #pragma unroll
+    for (int i = 3; i < 5; i++) { Q.c0h[i] = R.c0h[i]; Q.c0l[i] = R.c0l[i]; }
+    Q.c0h[0] = vpick3(a, R.c0h[0], R.c0h[1], R.c0h[2]);
+    Q.c0h[1] = vpick3(a, R.c0h[1], R.c0h[0], R.c0h[0]);
+    Q.c0h[2] = vpick3(a, R.c0h[2], R.c0h[2], R.c0h[1]);
+    Q.c0l[0] = vpick3(a, R.c0l[0], R.c0l[1], R.c0l[2]);
+    Q.c0l[1] = vpick3(a, R.c0l[1], R.c0l[0], R.c0l[0]);
+    Q.c0l[2] = vpick3(a, R.c0l[2], R.c0l[2], R.c0l[1]);
     Q.u[0] = vpick3(a, R.u[0], R.u[1], R.u[2]);
     Q.u[1] = vpick3(a, R.u[1], R.u[0], R.u[0]);
     Q.u[2] = vpick3(a, R.u[2], R.u[2], R.u[1]);
@@ -94,15 +108,20 @@ __device__ __forceinline__ DevRef permute_ref(const DevRef &R, int a)
     return Q;
 }
 
-// element coefficients as deviations: coefficient 0 minus the reference
-// (in FP64, one rounding of the difference), slopes as they are
+// element coefficients as deviations: coefficient 0 minus the reference,
+// slopes as they are.  An FP64 state subtracts in FP64 (one rounding of the
+// difference); an FP32 state subtracts the two-float reference c0h + c0l in
+// FP32, (x - c0h) - c0l, two roundings, each relative to the deviation.
 template <typename S>
 __device__ __forceinline__ void load_dev(const S *__restrict__ U, int64_t n, int e, const DevRef &R, float *u)
 {
     #pragma unroll
     for (int i = 0; i < 20; i++) {
-        const S x = (U + i * n)[e];
-        u[i] = (i & 3) ? (float) x : (float) ((double) x - R.c0[i >> 2]);
+        const S x = ldp<S>(U, n, i, e);
+        if constexpr (sizeof(S) == 4)
+            u[i] = (i & 3) ? x : (x - R.c0h[i >> 2]) - R.c0l[i >> 2];
+        else
+            u[i] = (i & 3) ? (float) x : (float) ((double) x - R.c0[i >> 2]);
     }
 }
 
@@ -162,7 +181,7 @@ __device__ __forceinline__ void prim_of_dev(const DevParams &P, const DevRef &R,
 
 // face-frame view (normal axis AX with sign s, tangents T1 < T2)
 struct DFrame {
-    float r, un, ut1, ut2, p, E, cz;
+    float r, ri, un, ut1, ut2, p, E, cz;
     float dr, dmn, dmt1, dmt2, dun, dE, dp;
 };
 
@@ -171,7 +190,7 @@ __device__ __forceinline__ DFrame dev_frame(const DPrim &q, float s)
 {
     constexpr int T1 = (AX == 0) ? 1 : 0, T2 = (AX == 2) ? 1 : 2;
     DFrame f;
-    f.r = q.r; f.p = q.p; f.E = q.E; f.cz = q.cz;
+    f.r = q.r; f.ri = q.ri; f.p = q.p; f.E = q.E; f.cz = q.cz;
     f.un = s * q.u[AX]; f.ut1 = q.u[T1]; f.ut2 = q.u[T2];
     f.dr = q.dr; f.dE = q.dE; f.dp = q.dp;
     f.dmn = s * q.dm[AX]; f.dmt1 = q.dm[T1]; f.dmt2 = q.dm[T2];
@@ -181,7 +200,7 @@ __device__ __forceinline__ DFrame dev_frame(const DPrim &q, float s)
 
 // reference values in the same frame
 struct DRefFrame {
-    float mn, mt1, mt2, Epp, gq;
+    float mn, mt1, mt2, Epp, gq, gam;
 };
 
 template <int AX>
@@ -189,14 +208,14 @@ __device__ __forceinline__ DRefFrame dev_ref_frame(const DevRef &R, float s)
 {
     constexpr int T1 = (AX == 0) ? 1 : 0, T2 = (AX == 2) ? 1 : 2;
     DRefFrame f;
-    f.mn = s * R.m[AX]; f.mt1 = R.m[T1]; f.mt2 = R.m[T2]; f.Epp = R.Epp; f.gq = R.gq;
+    f.mn = s * R.m[AX]; f.mt1 = R.m[T1]; f.mt2 = R.m[T2]; f.Epp = R.Epp; f.gq = R.gq; f.gam = R.gam;
     return f;
 }
 
 __device__ __forceinline__ DFrame sel_dframe(bool c, const DFrame &a, const DFrame &b)
 {
     DFrame o;
-    o.r = sel(c, a.r, b.r); o.un = sel(c, a.un, b.un); o.ut1 = sel(c, a.ut1, b.ut1); o.ut2 = sel(c, a.ut2, b.ut2);
+    o.r = sel(c, a.r, b.r); o.ri = sel(c, a.ri, b.ri); o.un = sel(c, a.un, b.un); o.ut1 = sel(c, a.ut1, b.ut1); o.ut2 = sel(c, a.ut2, b.ut2);
     o.p = sel(c, a.p, b.p); o.E = sel(c, a.E, b.E); o.cz = sel(c, a.cz, b.cz);
     o.dr = sel(c, a.dr, b.dr); o.dmn = sel(c, a.dmn, b.dmn); o.dmt1 = sel(c, a.dmt1, b.dmt1);
     o.dmt2 = sel(c, a.dmt2, b.dmt2); o.dun = sel(c, a.dun, b.dun); o.dE = sel(c, a.dE, b.dE);
@@ -223,10 +242,10 @@ __device__ __forceinline__ bool hllc_dev(const DFrame &L, const DFrame &Rt, cons
 {
     const float p_pvrs = 0.5f * (L.p + Rt.p) - 0.5f * (Rt.un - L.un) * 0.25f * (L.r + Rt.r) * (L.cz + Rt.cz);
     const float p_star = p_pvrs > 0.f ? p_pvrs : 0.f;
-    const float ql = (p_star <= L.p) ? 1.f : dsqrt(1.f + rf.gq * (dv(p_star, L.p) - 1.f));
-    const float qr = (p_star <= Rt.p) ? 1.f : dsqrt(1.f + rf.gq * (dv(p_star, Rt.p) - 1.f));
-    float sl = L.un - L.cz * ql;
-    float sr = Rt.un + Rt.cz * qr;
+    // c_K q_K with q_K = sqrt(1 + gq (p* / p_K - 1)) for p* > p_K, else 1, as
+    // one root: c_K q_K = sqrt(gamma / rho_K (p_K + gq max(p* - p_K, 0)))
+    float sl = L.un - dsqrt(rf.gam * L.ri * (L.p + rf.gq * fmaxf(p_star - L.p, 0.f)));
+    float sr = Rt.un + dsqrt(rf.gam * Rt.ri * (Rt.p + rf.gq * fmaxf(p_star - Rt.p, 0.f)));
     if (sl > sr) { const float t = sl; sl = sr; sr = t; }
     const float al = L.r * (sl - L.un), ar = Rt.r * (sr - Rt.un);
     const float dpn = Rt.dp - L.dp, dun = Rt.dun - L.dun;
@@ -237,21 +256,21 @@ __device__ __forceinline__ bool hllc_dev(const DFrame &L, const DFrame &Rt, cons
     if (s_star < sl) { s_star = sl; dl = sl - L.un; dr = sl - Rt.un; }
     if (s_star > sr) { s_star = sr; dl = sr - L.un; dr = sr - Rt.un; }
     const bool ok = (sl <= s_star) && (s_star <= sr);
-    if (sl >= 0.f || sr <= 0.f) {
-        dev_phys(sel_dframe(sl >= 0.f, L, Rt), rf, q);
-    } else {
-        const bool left = s_star >= 0.f;
-        const DFrame K = sel_dframe(left, L, Rt);
-        const float sk = left ? sl : sr, dk = left ? dl : dr;
-        dev_phys(K, rf, q);
-        const float sg = sk * dk * rcp(sk - s_star);          // S_K g
-        const float ak = K.r * (sk - K.un);
-        q[0] += sg * (K.r * sk);
-        q[1] += sg * (K.r * K.ut1);
-        q[2] += sg * (K.r * K.ut2);
-        q[3] += sg * (K.E + ak * s_star + K.p);
-        q[4] += sg * K.r;
-    }
+    // one branch-free body: the upwind side of a supersonic face (sl >= 0:
+    // left, sr <= 0: right) with no star term, else the star region of the
+    // side s* falls on
+    const bool sup = (sl >= 0.f) || (sr <= 0.f);
+    const bool left = (sl >= 0.f) || (!(sr <= 0.f) && s_star >= 0.f);
+    const DFrame K = sel_dframe(left, L, Rt);
+    const float sk = left ? sl : sr, dk = left ? dl : dr;
+    dev_phys(K, rf, q);
+    const float sg = sup ? 0.f : sk * dk * rcp(sk - s_star);          // S_K g
+    const float ak = K.r * (sk - K.un);
+    q[0] += sg * (K.r * sk);
+    q[1] += sg * (K.r * K.ut1);
+    q[2] += sg * (K.r * K.ut2);
+    q[3] += sg * (K.E + ak * s_star + K.p);
+    q[4] += sg * K.r;
     return ok;
 }
 

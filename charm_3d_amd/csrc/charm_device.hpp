@@ -79,6 +79,42 @@ __device__ __forceinline__ int xcd_block(int n_xcd)
     return ((int) blockIdx.x % n_xcd) * per + (int) blockIdx.x / n_xcd;
 }
 
+// Coefficient (v, k) of a state U[(4 v + k) n + e] through a buffer
+// resource per variable v: its base and size sit in scalar registers, the
+// coefficient's plane offset k n sizeof(S) is a scalar offset, and a lane
+// supplies only its 32-bit byte offset, so a load or store costs no 64-bit
+// address arithmetic on the VALU.  The host keeps 4 n sizeof(S) below 2^31.
+template <typename S>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t var_rsrc(const S *U, int64_t n, int v)
+{
+    return __builtin_amdgcn_make_buffer_rsrc((void *) (U + 4 * v * n), 0, (int) (4 * n * (int64_t) sizeof(S)),
+                                             0x00020000);
+}
+
+// i = 4 v + k
+template <typename S>
+__device__ __forceinline__ S ldp(const S *U, int64_t n, int i, int e)
+{
+    const __amdgpu_buffer_rsrc_t r = var_rsrc(U, n, i >> 2);
+    const int so = (int) ((i & 3) * n * (int64_t) sizeof(S));
+    if constexpr (sizeof(S) == 8)
+        return __builtin_bit_cast(S, __builtin_amdgcn_raw_buffer_load_b64(r, e * 8, so, 0));
+    else
+        return __builtin_bit_cast(S, __builtin_amdgcn_raw_buffer_load_b32(r, e * 4, so, 0));
+}
+
+template <typename S>
+__device__ __forceinline__ void stp(S *U, int64_t n, int i, int e, S x)
+{
+    const __amdgpu_buffer_rsrc_t r = var_rsrc<S>(U, n, i >> 2);
+    const int so = (int) ((i & 3) * n * (int64_t) sizeof(S));
+    if constexpr (sizeof(S) == 8)
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, x), r,
+                                              e * 8, so, 0);
+    else
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, e * 4, so, 0);
+}
+
 // Reciprocal, quotient and square root.  The FP32 bulk path uses the
 // hardware v_rcp_f32 / v_sqrt_f32 (about 1 ulp; operands here are normal
 // numbers), the FP64 path IEEE division and sqrt.

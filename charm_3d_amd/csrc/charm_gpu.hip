@@ -271,6 +271,13 @@ extern "C" int charm_gpu_upload_mesh(charm_gpu_ctx_t *c, const charm_mesh_t *m)
         charm_set_error("upload_mesh: chunk_elems %d outside [1, 256] (one thread per element)", m->chunk_elems);
         return CHARM_ERR_ARG;
     }
+    if (4 * (int64_t) m->n * (int64_t) c->esz >= ((int64_t) 1 << 31)) {
+        // the stage kernel addresses one variable's 4 coefficient planes with
+        // 32-bit byte offsets
+        charm_set_error("upload_mesh: %lld leaves: one variable of the state exceeds 2^31 bytes",
+                        (long long) m->n);
+        return CHARM_ERR_ARG;
+    }
     const size_t smem = stage_smem_bytes(c->prec, c->nc, m->chunk_faces);
     if (smem > 65536) {
         charm_set_error("upload_mesh: chunk_faces %d needs %zu B of LDS per workgroup (limit 65536)",

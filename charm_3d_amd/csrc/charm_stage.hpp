@@ -44,7 +44,7 @@ template <typename R, int NV>
 __device__ __forceinline__ void load_elem(const R *__restrict__ U, int64_t n, int e, R *u)
 {
     #pragma unroll
-    for (int i = 0; i < 4 * NV; i++) u[i] = (U + i * n)[e];
+    for (int i = 0; i < 4 * NV; i++) u[i] = ldp<R>(U, n, i, e);
 }
 
 // load a state stored in S into the arithmetic type R
@@ -52,7 +52,7 @@ template <typename S, typename R, int NV>
 __device__ __forceinline__ void load_conv(const S *__restrict__ U, int64_t n, int e, R *u)
 {
     #pragma unroll
-    for (int i = 0; i < 4 * NV; i++) u[i] = (R) (U + i * n)[e];
+    for (int i = 0; i < 4 * NV; i++) u[i] = (R) ldp<S>(U, n, i, e);
 }
 
 template <typename R, int NV>
@@ -99,6 +99,8 @@ __device__ __forceinline__ bool face_core(const DevParams &P, const R *u0, const
     const R o01 = big0 ? d1 : R(0), o02 = big0 ? d2 : R(0);
     const R o11 = big1 ? d1 : R(0), o12 = big1 ? d2 : R(0);
     R S0[NV], S1[NV], S2[NV];
+    #pragma unroll
+    for (int v = 0; v < NV; v++) { S0[v] = R(0); S1[v] = R(0); S2[v] = R(0); }
     // the normal-direction part of each trace is the same at all six points
     R c0[NV], c1[NV];
     #pragma unroll
@@ -107,7 +109,9 @@ __device__ __forceinline__ bool face_core(const DevParams &P, const R *u0, const
         c1[v] = BND ? R(0) : u1[4*v] + pa1 * u1[4*v+1+AX];
     }
     bool ok = true;
-    #pragma unroll
+    // a rolled loop over the points (k is wave-uniform: the point's constants
+    // are scalar selects): a fifth of the unrolled code and fewer registers
+    #pragma unroll 1
     for (int k = 0; k < 6; k++) {
         const R g1 = gp_t1<R>(k), g2 = gp_t2<R>(k);
         R t[NV];
@@ -163,10 +167,7 @@ __device__ __forceinline__ bool face_core(const DevParams &P, const R *u0, const
         #pragma unroll
         for (int v = 3; v < NV; v++) qg[v] = q[v];
         #pragma unroll
-        for (int v = 0; v < NV; v++) {
-            if (k == 0) { S0[v] = qg[v]; S1[v] = qg[v] * g1; S2[v] = qg[v] * g2; }
-            else { S0[v] += qg[v]; S1[v] += qg[v] * g1; S2[v] += qg[v] * g2; }
-        }
+        for (int v = 0; v < NV; v++) { S0[v] += qg[v]; S1[v] += qg[v] * g1; S2[v] += qg[v] * g2; }
     }
     #pragma unroll
     for (int v = 0; v < NV; v++) {
@@ -354,8 +355,17 @@ __device__ __forceinline__ const DevParams &stage_params(const StageArgs &A, con
     else return Pv;
 }
 
+// The LDS of a chunk allows three 256-thread workgroups per CU, three waves
+// per SIMD: the FP32 deviation-form instances are held to the registers of
+// that occupancy (168 VGPRs)
+template <typename R, int NC, int FLUX, int PH>
+constexpr int stage_min_waves()
+{
+    return uses_dev<R, NC, FLUX, PH>() ? 3 : 1;
+}
+
 template <typename S, typename R, int NC, int FLUX, int BS, int PH>
-__global__ __launch_bounds__(BS)
+__global__ __launch_bounds__(BS, (stage_min_waves<R, NC, FLUX, PH>()))
 void k_stage(StageArgs A, DevParams Pv)
 {
     const DevParams &P = stage_params<PH>(A, Pv);
@@ -451,7 +461,7 @@ void k_stage(StageArgs A, DevParams Pv)
         S *out = reinterpret_cast<S *>(A.out);
         if (A.mode == 0) {
             #pragma unroll
-            for (int i = 0; i < 4 * NV; i++) (out + i * n)[e] = (S) res[i];
+            for (int i = 0; i < 4 * NV; i++) stp<S>(out, n, i, e, (S) res[i]);
         } else {
             // c -= dt * A^-1 int (src/models/euler/charm_model.c:402-433); A^-1
             // of a level-l cube is the FP64 level-0 inverse times 8^l (exact
@@ -469,9 +479,9 @@ void k_stage(StageArgs A, DevParams Pv)
                 for (int k = 0; k < 4; k++) {
                     const S ai = cst<S>(P.ainv0[k], P.ainv0_f[k]) * sc;
                     // the stored coefficient (the registers hold deviations in DEV)
-                    const S uk = (std::is_same<S, R>::value && !DEV) ? (S) u[4*v+k] : (U + (4*v+k) * n)[e];
+                    const S uk = (std::is_same<S, R>::value && !DEV) ? (S) u[4*v+k] : ldp<S>(U, n, 4*v+k, e);
                     const S w = uk - dt * (ai * (S) res[4*v+k]);
-                    (out + (4*v+k) * n)[e] = A.combine ? rk_combine<S>(A.combine, w, (U0 + (4*v+k) * n)[e]) : w;
+                    stp<S>(out, n, 4*v+k, e, A.combine ? rk_combine<S>(A.combine, w, ldp<S>(U0, n, 4*v+k, e)) : w);
                 }
         }
     }
