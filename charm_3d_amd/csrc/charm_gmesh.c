@@ -52,8 +52,6 @@ struct charm_gmesh {
     int32_t *fe;        /* [2][nf] side-0 leaf, side-1 leaf or -1 - boundary slot */
     int32_t *ff;        /* [2][nf] local faces (side 1: -1 on a boundary) */
     int32_t *lnb;       /* [6][n] neighbour leaf, or -1 - face index of a boundary face */
-    int32_t *inc_ptr;   /* [n + 1] */
-    int32_t *inc;       /* face << 1 | side */
 };
 
 typedef struct {
@@ -210,7 +208,7 @@ void charm_gmesh_destroy(charm_gmesh_t *m)
 {
     if (!m) return;
     free(m->leaf); free(m->geo); free(m->lg); free(m->fg); free(m->fn); free(m->fe); free(m->ff);
-    free(m->lnb); free(m->inc_ptr); free(m->inc);
+    free(m->lnb);
     free(m);
 }
 
@@ -355,22 +353,24 @@ int charm_gmesh_build(int64_t n_vertices, const double *vertices, int64_t n_tree
         pe[6 * b->leaf + b->face] = a->leaf; pf[6 * b->leaf + b->face] = (int8_t) a->face;
         i++;
     }
-    /* faces in (side-0 leaf, face) order */
+    /* faces grouped by (side-0 local face, boundary or not), leaf order
+     * inside a group: the face pass's threads of a wave then read and write
+     * the slots of consecutive leaves on one local face, and take one branch */
+    int64_t next[12] = {0};
     for (e = 0; e < m->n; e++)
         for (int f = 0; f < 6; f++) {
             const int64_t p = pe[6 * e + f];
-            if (p < 0 || p > e || (p == e && pf[6 * e + f] > f)) nf++;
+            if (p < 0 || p > e || (p == e && pf[6 * e + f] > f)) next[2 * f + (p < 0)]++;
         }
+    for (int b = 0; b < 12; b++) { const int64_t c = next[b]; next[b] = nf; nf += c; }
     m->nf = nf;
     m->fg = (double *) malloc((size_t) nf * CHARM_GM_FACEQ * sizeof(double));
     m->fn = (double *) malloc((size_t) nf * 3 * sizeof(double));
     m->fe = (int32_t *) malloc((size_t) nf * 2 * sizeof(int32_t));
     m->ff = (int32_t *) malloc((size_t) nf * 2 * sizeof(int32_t));
-    m->inc_ptr = (int32_t *) calloc((size_t) m->n + 1, sizeof(int32_t));
-    m->inc = (int32_t *) malloc((size_t) (2 * nf + 1) * sizeof(int32_t));
-    if (!m->fg || !m->fn || !m->fe || !m->ff || !m->inc_ptr || !m->inc) goto oom;
+    if (!m->fg || !m->fn || !m->fe || !m->ff) goto oom;
     {
-        int64_t fi = 0;
+        int64_t fi;
         for (e = 0; e < m->n; e++) {
             gm_geom_t g;
             int have = 0;
@@ -379,6 +379,7 @@ int charm_gmesh_build(int64_t n_vertices, const double *vertices, int64_t n_tree
                 int pt, a, k;
                 double nn[3];
                 if (!(p < 0 || p > e || (p == e && pf[6 * e + f] > f))) continue;
+                fi = next[2 * f + (p < 0)]++;
                 if (!have) {
                     gm_geom((const double (*)[3]) tvs[m->leaf[5 * e]], m->leaf + 5 * e + 1, &g);
                     have = 1;
@@ -403,21 +404,7 @@ int charm_gmesh_build(int64_t n_vertices, const double *vertices, int64_t n_tree
                     m->fg[(size_t) (36 + pt) * nf + fi] = g.fwj[f][pt];
                 }
                 (void) k;
-                m->inc_ptr[e + 1]++;
-                if (p >= 0) m->inc_ptr[p + 1]++;
-                fi++;
             }
-        }
-        for (e = 0; e < m->n; e++) m->inc_ptr[e + 1] += m->inc_ptr[e];
-        {
-            int32_t *pos = (int32_t *) malloc((size_t) m->n * sizeof(int32_t));
-            if (!pos) goto oom;
-            memcpy(pos, m->inc_ptr, (size_t) m->n * sizeof(int32_t));
-            for (fi = 0; fi < nf; fi++) {
-                m->inc[pos[m->fe[fi]]++] = (int32_t) (fi << 1);
-                if (m->fe[nf + fi] >= 0) m->inc[pos[m->fe[nf + fi]]++] = (int32_t) ((fi << 1) | 1);
-            }
-            free(pos);
         }
         /* limiter neighbours: the leaf across each face, or -1 - the boundary face */
         for (fi = 0; fi < nf; fi++) {
@@ -475,7 +462,6 @@ int charm_gmesh_arrays(const charm_gmesh_t *m, charm_gmesh_arrays_t *a)
 {
     if (!m || !a) { charm_set_error("gmesh_arrays: null argument"); return CHARM_ERR_ARG; }
     a->n = m->n; a->nf = m->nf;
-    a->lg = m->lg; a->fg = m->fg; a->fn = m->fn; a->fe = m->fe; a->lnb = m->lnb;
-    a->inc_ptr = m->inc_ptr; a->inc = m->inc;
+    a->lg = m->lg; a->fg = m->fg; a->fn = m->fn; a->fe = m->fe; a->ff = m->ff; a->lnb = m->lnb;
     return CHARM_OK;
 }

@@ -5,11 +5,13 @@
 //
 // Face pass, one thread per face: at its 6 points both traces (side 1 from
 // the boundary condition on a boundary face), the flux along the side-0
-// normal in the reference's face frame, and the three-moment sums of both
-// sides, written to FQ[2][4V][nf].  Element pass, one thread per leaf: the
-// volume integral at 8 points (flux tensor against grad phi = e / dh, gravity
-// on the momentum), the leaf's face sums in face order (no atomics), the full
-// 4x4 A^-1, the update and the SSP-RK combine.  The arithmetic is the
+// normal in the reference's face frame, and the moment sums of both sides,
+// each written to its leaf's slot for that local face, FQ[6][4V][n] (side 1
+// negated; on a conforming forest every local face of a leaf has exactly one
+// face).  Element pass, one thread per leaf: the volume integral at 8 points
+// (flux tensor against grad phi = e / dh, gravity on the momentum), its six
+// face slots (coalesced, no atomics), the full 4x4 A^-1, the update and the
+// SSP-RK combine; ahead of the limiter also the cell means of the new state.  The arithmetic is the
 // reference's Euler model (src/models/euler/charm_model.c:66-212, 215-372,
 // 402-433) with the geometry of src/charm_geom.c.
 #include <hip/hip_runtime.h>
@@ -159,14 +161,18 @@ void k_gface(GArgs A, DevParams P)
             #pragma unroll
             for (int k = 0; k < 4; k++) { s0[v][k] += g[v] * ph0[k]; s1[v][k] += g[v] * ph1[k]; }
     }
-    R *__restrict__ FQ = reinterpret_cast<R *>(A.fq) + f;
+    R *__restrict__ FQ = reinterpret_cast<R *>(A.fq);
+    const int l0 = A.ff[f], l1 = A.ff[nf + f];
     #pragma unroll
     for (int v = 0; v < NV; v++)
         #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            FQ[(4 * v + k) * nf] = s0[v][k];
-            FQ[(4 * NV + 4 * v + k) * nf] = s1[v][k];
-        }
+        for (int k = 0; k < 4; k++) (FQ + (int64_t) (4 * NV * l0 + 4 * v + k) * n)[e0] = s0[v][k];
+    if (e1 >= 0) {
+        #pragma unroll
+        for (int v = 0; v < NV; v++)
+            #pragma unroll
+            for (int k = 0; k < 4; k++) (FQ + (int64_t) (4 * NV * l1 + 4 * v + k) * n)[e1] = -s1[v][k];
+    }
     if (!ok) A.err[0] = 1;
 }
 
@@ -175,7 +181,7 @@ __global__ __launch_bounds__(256)
 void k_gelem(GArgs A, DevParams P)
 {
     constexpr int NV = 4 + NC;
-    const int64_t n = A.n, nf = A.nf;
+    const int64_t n = A.n;
     const int64_t e = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
     const R *__restrict__ U = reinterpret_cast<const R *>(A.U);
@@ -216,17 +222,12 @@ void k_gelem(GArgs A, DevParams P)
             for (int k = 0; k < 4; k++) in[d][k] -= p.r * grav[d] * phi[k];
     }
     const R *__restrict__ FQ = reinterpret_cast<const R *>(A.fq);
-    for (int j = A.inc_ptr[e]; j < A.inc_ptr[e + 1]; j++) {
-        const int rec = A.inc[j], face = rec >> 1, side = rec & 1;
-        const R *q = FQ + (int64_t) side * 4 * NV * nf + face;
+    #pragma unroll 1
+    for (int lf = 0; lf < 6; lf++)
         #pragma unroll
         for (int v = 0; v < NV; v++)
             #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const R x = q[(4 * v + k) * nf];
-                in[v][k] += side ? -x : x;
-            }
-    }
+            for (int k = 0; k < 4; k++) in[v][k] += (FQ + (int64_t) (4 * NV * lf + 4 * v + k) * n)[e];
     R *__restrict__ out = reinterpret_cast<R *>(A.out);
     if (A.mode == 0) {
         #pragma unroll
@@ -238,23 +239,30 @@ void k_gelem(GArgs A, DevParams P)
     // c -= dt A^-1 int, then the SSP-RK combine (U0 is read before out is written)
     const R dt = R(A.dt);
     const R *U0 = reinterpret_cast<const R *>(A.U0);
+    R *MB = reinterpret_cast<R *>(A.mb);
+    const R mb[3] = {LG[75 * n], LG[76 * n], LG[77 * n]};
     #pragma unroll
-    for (int v = 0; v < NV; v++)
+    for (int v = 0; v < NV; v++) {
+        R w[4];
         #pragma unroll
         for (int k = 0; k < 4; k++) {
             R du = R(0);
             #pragma unroll
             for (int j = 0; j < 4; j++) du += LG[(35 + 4 * k + j) * n] * in[v][j];
-            const R w = u[4 * v + k] - dt * du;
-            (out + (4 * v + k) * n)[e] = A.combine ? rk_combine<R>(A.combine, w, (U0 + (4 * v + k) * n)[e]) : w;
+            w[k] = u[4 * v + k] - dt * du;
+            (out + (4 * v + k) * n)[e] = A.combine ? rk_combine<R>(A.combine, w[k], (U0 + (4 * v + k) * n)[e]) : w[k];
         }
+        // the limiter's cell mean of this variable (the basis means of the leaf)
+        if (MB) (MB + v * n)[e] = w[0] + w[1] * mb[0] + w[2] * mb[1] + w[3] * mb[2];
+    }
 }
 
 // Barth-Jespersen on the general mesh (src/limiters/charm_limiter_bj.c:
 // 11-332): the leaf's mean and its six neighbours' (a boundary face gives
 // the mean pushed through the boundary condition), psi from the 8 vertex
 // values, the slopes scaled, then the SSP-RK combine.  A leaf's mean is
-// U0 + sum_d U_d mb_d (mb: the leaf means of the linear basis functions).
+// U0 + sum_d U_d mb_d (mb: the leaf means of the linear basis functions),
+// read from the element pass's means when it wrote them for this state.
 template <typename R, int NC>
 __global__ __launch_bounds__(256)
 void k_glimiter(GArgs A, DevParams P)
@@ -268,7 +276,9 @@ void k_glimiter(GArgs A, DevParams P)
     R u[4 * NV];
     #pragma unroll
     for (int i = 0; i < 4 * NV; i++) u[i] = (W + i * n)[e];
+    const R *__restrict__ MB = reinterpret_cast<const R *>(A.mb);
     auto mean = [&](int64_t j, int v) {
+        if (MB) return (MB + v * n)[j];
         return (W + 4 * v * n)[j] + (W + (4 * v + 1) * n)[j] * (LG + 75 * n)[j] +
                (W + (4 * v + 2) * n)[j] * (LG + 76 * n)[j] + (W + (4 * v + 3) * n)[j] * (LG + 77 * n)[j];
     };

@@ -48,9 +48,10 @@ struct charm_gpu_ctx {
     // multi-tree mesh (charm_gpu_upload_gmesh; gm = 1): the general kernels
     int gm;
     int64_t gnf;
-    void *d_glg, *d_gfg, *d_gfq;
+    void *d_glg, *d_gfg, *d_gfq, *d_gmb;
     double *d_gfn;
-    int *d_gfe, *d_glnb, *d_ginc_ptr, *d_ginc;
+    int *d_gfe, *d_gff, *d_glnb;
+    const void *gmb_of;             // the state whose cell means d_gmb holds (multi-tree)
     // state
     void *B[3];
     double *d_stage64;        // FP64 staging for host transfers
@@ -83,11 +84,11 @@ static void free_mesh(charm_gpu_ctx_t *c)
                   (void **) &c->d_nbr, (void **) &c->d_slots, &c->B[0], &c->B[1], &c->B[2],
                   (void **) &c->d_stage64, (void **) &c->d_partial, (void **) &c->d_q,
                   &c->d_auxf, &c->d_vi, &c->d_auxo, &c->d_lift, (void **) &c->d_dcoef, (void **) &c->d_chem,
-                  &c->d_glg, &c->d_gfg, &c->d_gfq, (void **) &c->d_gfn, (void **) &c->d_gfe,
-                  (void **) &c->d_glnb, (void **) &c->d_ginc_ptr, (void **) &c->d_ginc};
+                  &c->d_glg, &c->d_gfg, &c->d_gfq, &c->d_gmb, (void **) &c->d_gfn, (void **) &c->d_gfe,
+                  (void **) &c->d_gff, (void **) &c->d_glnb};
     for (size_t i = 0; i < sizeof(p) / sizeof(p[0]); i++)
         if (*p[i]) { (void) hipFree(*p[i]); *p[i] = NULL; }
-    c->gm = 0; c->gnf = 0;
+    c->gm = 0; c->gnf = 0; c->gmb_of = NULL;
     free(c->hq);
     c->hq = NULL;
     c->n = 0;
@@ -381,20 +382,18 @@ extern "C" int charm_gpu_upload_gmesh(charm_gpu_ctx_t *c, const charm_gmesh_t *m
     DMALLOC(c->d_partial, (size_t) ((n + 255) / 256 + 1) * (c->nv + 1) * sizeof(double));
     DMALLOC(c->d_glg, (size_t) n * CHARM_GM_LEAFQ * c->esz);
     DMALLOC(c->d_gfg, (size_t) nf * CHARM_GM_FACEQ * c->esz);
-    DMALLOC(c->d_gfq, (size_t) nf * 2 * 4 * c->nv * c->esz);
+    DMALLOC(c->d_gfq, (size_t) n * 6 * 4 * c->nv * c->esz);
+    DMALLOC(c->d_gmb, (size_t) n * c->nv * c->esz);
     DMALLOC(c->d_gfn, (size_t) nf * 3 * sizeof(double));
     DMALLOC(c->d_gfe, (size_t) nf * 2 * sizeof(int));
+    DMALLOC(c->d_gff, (size_t) nf * 2 * sizeof(int));
     DMALLOC(c->d_glnb, (size_t) n * 6 * sizeof(int));
-    DMALLOC(c->d_ginc_ptr, (size_t) (n + 1) * sizeof(int));
-    DMALLOC(c->d_ginc, (size_t) g.inc_ptr[n] * sizeof(int));
     hipError_t e = upload_real(c, c->d_glg, g.lg, (size_t) n * CHARM_GM_LEAFQ);
     if (e == hipSuccess) e = upload_real(c, c->d_gfg, g.fg, (size_t) nf * CHARM_GM_FACEQ);
     if (e == hipSuccess) e = hipMemcpy(c->d_gfn, g.fn, (size_t) nf * 3 * sizeof(double), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(c->d_gfe, g.fe, (size_t) nf * 2 * sizeof(int), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(c->d_glnb, g.lnb, (size_t) n * 6 * sizeof(int), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(c->d_ginc_ptr, g.inc_ptr, (size_t) (n + 1) * sizeof(int), hipMemcpyHostToDevice);
-    if (e == hipSuccess && g.inc_ptr[n] > 0)
-        e = hipMemcpy(c->d_ginc, g.inc, (size_t) g.inc_ptr[n] * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c->d_gff, g.ff, (size_t) nf * 2 * sizeof(int), hipMemcpyHostToDevice);
     for (int i = 0; i < 3 && e == hipSuccess; i++) e = hipMemsetAsync(c->B[i], 0, ssz, c->st);
     if (e == hipSuccess) e = hipStreamSynchronize(c->st);
     if (e != hipSuccess) {
@@ -544,9 +543,9 @@ static hipError_t ns_prepare(charm_gpu_ctx_t *c, const void *in, void *chem_out,
 static GArgs gargs(charm_gpu_ctx_t *c, const void *U, const void *U0, void *out, int mode, int combine, double dt)
 {
     GArgs G;
-    G.U = U; G.U0 = U0; G.out = out; G.fq = c->d_gfq;
+    G.U = U; G.U0 = U0; G.out = out; G.fq = c->d_gfq; G.mb = NULL;
     G.lg = c->d_glg; G.fg = c->d_gfg; G.fn = c->d_gfn;
-    G.fe = c->d_gfe; G.lnb = c->d_glnb; G.inc_ptr = c->d_ginc_ptr; G.inc = c->d_ginc;
+    G.fe = c->d_gfe; G.ff = c->d_gff; G.lnb = c->d_glnb;
     G.err = c->d_err; G.n = c->n; G.nf = c->gnf;
     G.mode = mode; G.combine = combine; G.dt = dt;
     return G;
@@ -555,15 +554,22 @@ static GArgs gargs(charm_gpu_ctx_t *c, const void *U, const void *U0, void *out,
 static hipError_t stage_launch(charm_gpu_ctx_t *c, const StageArgs &A)
 {
     if (!c->gm) return launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st);
-    return launch_gstage(c->sprec, c->nc, c->cfg.flux, gargs(c, A.U, A.U0, A.out, A.mode, A.combine, A.dt),
-                         c->P, c->st);
+    GArgs G = gargs(c, A.U, A.U0, A.out, A.mode, A.combine, A.dt);
+    // a stage that a limiter may follow (new state, no combine) also writes
+    // the limiter's cell means of that state
+    const bool means = A.mode != 0 && A.combine == 0;
+    G.mb = means ? c->d_gmb : NULL;
+    c->gmb_of = means ? A.out : NULL;
+    return launch_gstage(c->sprec, c->nc, c->cfg.flux, G, c->P, c->st);
 }
 
 static hipError_t limiter_launch(charm_gpu_ctx_t *c, const LimArgs &L)
 {
     if (!c->gm) return launch_limiter(c->sprec, c->nc, c->ph, L, c->P, c->st);
     if (!L.limit) return hipErrorInvalidValue;      // every multi-tree call limits
-    return launch_glimiter(c->sprec, c->nc, gargs(c, L.W, L.U0, L.out, 1, L.combine, 0.0), c->P, c->st);
+    GArgs G = gargs(c, L.W, L.U0, L.out, 1, L.combine, 0.0);
+    G.mb = c->gmb_of == L.W ? c->d_gmb : NULL;
+    return launch_glimiter(c->sprec, c->nc, G, c->P, c->st);
 }
 
 extern "C" int charm_gpu_residual(charm_gpu_ctx_t *c, double *out, int64_t n)

@@ -87,8 +87,8 @@ typedef struct {
     int64_t n, nf;
     const double *lg, *fg, *fn;   /* fn: [3][nf] side-0 outward normals */
     const int32_t *fe;            /* [2][nf] side-0 leaf, side-1 leaf or -1 - boundary slot */
+    const int32_t *ff;            /* [2][nf] each side's local face 0-5 (side 1: -1 on a boundary) */
     const int32_t *lnb;           /* [6][n] neighbour leaf or -1 - boundary face index */
-    const int32_t *inc_ptr, *inc; /* per leaf its faces: face << 1 | side */
 } charm_gmesh_arrays_t;
 int charm_gmesh_arrays(const charm_gmesh_t *m, charm_gmesh_arrays_t *a);
 

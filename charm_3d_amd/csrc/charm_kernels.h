@@ -87,10 +87,11 @@ struct GArgs {
     const void *U;            // stage: the state the residual is evaluated on; limiter: W
     const void *U0;           // SSP-RK combine base
     void *out;                // mode 0: the residual integrals; else the new state
-    void *fq;                 // [2][4V][nf] face moments
+    void *fq;                 // [6][4V][n] each leaf's face sums by local face (side 1 negated)
+    void *mb;                 // [V][n] cell means of the stage output for the limiter (or null)
     const void *lg, *fg;      // [CHARM_GM_LEAFQ][n], [CHARM_GM_FACEQ][nf]
     const double *fn;         // [3][nf] side-0 outward normals (FP64: the reference's face frame)
-    const int *fe, *lnb, *inc_ptr, *inc;
+    const int *fe, *ff, *lnb;
     int *err;
     int64_t n, nf;
     int mode, combine;
