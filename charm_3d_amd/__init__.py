@@ -142,6 +142,11 @@ def lib():
         "charm_gmesh_leaves": (C.c_int, [vp, i32p]),
         "charm_gmesh_geometry": (C.c_int, [vp, dp]),
         "charm_gmesh_faces": (C.c_int, [vp, i32p]),
+        "charm_gmesh_leaf_corners": (C.c_int, [vp, dp]),
+        "charm_gmesh_write_vtk": (C.c_int, [C.c_char_p, vp, dp, C.POINTER(Config)]),
+        "charm_task_gmesh": (C.c_int, [C.POINTER(Task), C.c_char_p, C.POINTER(vp)]),
+        "charm_task_initial_state_gmesh": (C.c_int, [C.POINTER(Task), vp, dp, i64]),
+        "charm_gmesh_timesteps": (C.c_int, [vp, vp, C.POINTER(Config)]),
         "charm_gpu_upload_gmesh": (C.c_int, [vp, vp]),
         "charm_task_initial_state": (C.c_int, [C.POINTER(Task), vp, dp, i64]),
         "charm_gpu_amr_indicator": (C.c_int, [vp, dp, dp, i64]),
@@ -285,7 +290,13 @@ class GMesh:
     [nv][3], tree_to_vertex [nt][8] (z-order corners), tree_bnd [nt][6] (the
     boundary slot of a tree face, -1 where it meets another tree)."""
 
-    def __init__(self, vertices, tree_to_vertex, tree_bnd, level):
+    def __init__(self, vertices, tree_to_vertex, tree_bnd, level, _handle=None):
+        if _handle is not None:
+            self.h = _handle
+            n, nf = C.c_int64(), C.c_int64()
+            check(lib().charm_gmesh_info(self.h, C.byref(n), C.byref(nf)))
+            self.n, self.nf = n.value, nf.value
+            return
         v = np.ascontiguousarray(vertices, dtype=np.float64)
         t2v = np.ascontiguousarray(tree_to_vertex, dtype=np.int32)
         tb = np.ascontiguousarray(tree_bnd, dtype=np.int32)
@@ -307,6 +318,33 @@ class GMesh:
         g = np.zeros((self.n, 23))
         check(lib().charm_gmesh_geometry(self.h, _dptr(g)))
         return {"c": g[:, 0:3], "dh": g[:, 3:6], "volume": g[:, 6], "a_inv": g[:, 7:23].reshape(-1, 4, 4)}
+
+    @classmethod
+    def from_task(cls, task, directory):
+        """charm_task_gmesh: a Gmsh task's mesh (mesh.name relative to
+        `directory`) bound to the task's boundaries and regions, at
+        MIN_LEVEL."""
+        h = C.c_void_p()
+        check(lib().charm_task_gmesh(C.byref(task), str(directory).encode(), C.byref(h)))
+        return cls(None, None, None, None, _handle=h)
+
+    def corners(self):
+        """[n][8][3] leaf corners in z-order"""
+        out = np.zeros((self.n, 8, 3))
+        check(lib().charm_gmesh_leaf_corners(self.h, _dptr(out)))
+        return out
+
+    def write_vtk(self, name, u, cfg):
+        """charm_gmesh_write_vtk: <name>_0000.vtu + <name>.pvtu"""
+        uu = np.ascontiguousarray(u, dtype=np.float64)
+        check(lib().charm_gmesh_write_vtk(str(name).encode(), self.h, _dptr(uu), C.byref(cfg)))
+
+    def initial_state(self, task):
+        """charm_task_initial_state_gmesh: FP64 [n][V][4], each leaf the
+        region of its tree"""
+        u = np.zeros((self.n, 4 + task.cfg.n_comp, 4))
+        check(lib().charm_task_initial_state_gmesh(C.byref(task), self.h, _dptr(u), self.n))
+        return u
 
     def faces(self):
         """[nf][4]: side-0 leaf, side-1 leaf or -1 - slot, side-0 face, side-1 face"""
@@ -524,6 +562,11 @@ class GpuContext:
         """a multi-tree mesh (GMesh) in place of the octree"""
         check(lib().charm_gpu_upload_gmesh(self.h, gmesh.h))
         self.n = gmesh.n
+
+    def gmesh_timesteps(self, gmesh, cfg):
+        """charm_gmesh_timesteps: the time loop to cfg.tmax on the uploaded
+        multi-tree mesh (dt = cfg.dt, VTK every cfg.write_period steps)"""
+        check(lib().charm_gmesh_timesteps(self.h, gmesh.h, C.byref(cfg)))
 
     def set_state(self, u):
         u = np.ascontiguousarray(u, dtype=np.float64)

@@ -52,6 +52,8 @@ struct charm_gmesh {
     int32_t *fe;        /* [2][nf] side-0 leaf, side-1 leaf or -1 - boundary slot */
     int32_t *ff;        /* [2][nf] local faces (side 1: -1 on a boundary) */
     int32_t *lnb;       /* [6][n] neighbour leaf, or -1 - face index of a boundary face */
+    double (*tv)[8][3]; /* [ntree] tree corners (z-order) */
+    int32_t *treg;      /* [ntree] region index of each tree (a task's), or NULL */
 };
 
 typedef struct {
@@ -208,7 +210,7 @@ void charm_gmesh_destroy(charm_gmesh_t *m)
 {
     if (!m) return;
     free(m->leaf); free(m->geo); free(m->lg); free(m->fg); free(m->fn); free(m->fe); free(m->ff);
-    free(m->lnb);
+    free(m->lnb); free(m->tv); free(m->treg);
     free(m);
 }
 
@@ -413,7 +415,8 @@ int charm_gmesh_build(int64_t n_vertices, const double *vertices, int64_t n_tree
             if (e1 >= 0) m->lnb[(size_t) m->ff[nf + fi] * m->n + e1] = e0;
         }
     }
-    free(pe); free(pf); free(keys); free(tvs);
+    free(pe); free(pf); free(keys);
+    m->tv = tvs;
     *out = m;
     return CHARM_OK;
 oom:
@@ -443,6 +446,36 @@ int charm_gmesh_geometry(const charm_gmesh_t *m, double *out)
 {
     if (!m || !out) { charm_set_error("gmesh_geometry: null argument"); return CHARM_ERR_ARG; }
     memcpy(out, m->geo, (size_t) m->n * 23 * sizeof(double));
+    return CHARM_OK;
+}
+
+int charm_gmesh_set_regions(charm_gmesh_t *m, const int32_t *tree_reg)
+{
+    if (!m || !tree_reg) { charm_set_error("gmesh_set_regions: null argument"); return CHARM_ERR_ARG; }
+    free(m->treg);
+    m->treg = (int32_t *) malloc((size_t) (m->ntree + 1) * sizeof(int32_t));
+    if (!m->treg) { charm_set_error("gmesh_set_regions: out of memory"); return CHARM_ERR_MEM; }
+    memcpy(m->treg, tree_reg, (size_t) m->ntree * sizeof(int32_t));
+    return CHARM_OK;
+}
+
+const int32_t *charm_gmesh_regions(const charm_gmesh_t *m)
+{
+    return m ? m->treg : NULL;
+}
+
+/* each leaf's 8 corners through its tree's trilinear map, [n][8][3] in
+ * z-order (corner x + 2 y + 4 z) */
+int charm_gmesh_leaf_corners(const charm_gmesh_t *m, double *out)
+{
+    if (!m || !out) { charm_set_error("gmesh_leaf_corners: null argument"); return CHARM_ERR_ARG; }
+    for (int64_t e = 0; e < m->n; e++) {
+        const int32_t *q = m->leaf + 5 * e;
+        const int64_t len = (int64_t) GM_ROOT >> q[4];
+        for (int k = 0; k < 8; k++)
+            gm_tree_point((const double (*)[3]) m->tv[q[0]], q[1] + (k & 1) * len, q[2] + ((k >> 1) & 1) * len,
+                          q[3] + (k >> 2) * len, out + 3 * (8 * e + k));
+    }
     return CHARM_OK;
 }
 

@@ -91,6 +91,9 @@ typedef struct {
     const int32_t *lnb;           /* [6][n] neighbour leaf or -1 - boundary face index */
 } charm_gmesh_arrays_t;
 int charm_gmesh_arrays(const charm_gmesh_t *m, charm_gmesh_arrays_t *a);
+/* a task's region index per tree (charm_task_gmesh), or NULL */
+int charm_gmesh_set_regions(charm_gmesh_t *m, const int32_t *tree_reg);
+const int32_t *charm_gmesh_regions(const charm_gmesh_t *m);
 
 #ifdef __cplusplus
 }

@@ -142,6 +142,51 @@ void charm_timesteps(charm_forest_t *p4est)
     }
 }
 
+int charm_gmesh_timesteps(charm_gpu_ctx_t *g, const charm_gmesh_t *m, const charm_config_t *cfg)
+{
+    /* the reference's loop, t += dt while t < TMAX, with dt from get_dt (TAU,
+     * src/models/euler/charm_model.c:30-38); the solution is written before
+     * every step FILE_OUTPUT_STEP divides (src/models/euler/charm_model.c:
+     * 507-510); runs of steps between outputs and log lines go to the device
+     * in one call */
+    const double dt = cfg ? cfg->dt : 0.0;
+    double t = 0.0;
+    int64_t n = 0, step = 0;
+    double *u = NULL;
+    int rc = CHARM_OK;
+    if (!g || !m || !cfg || !(dt > 0.0)) { charm_set_error("gmesh_timesteps: bad arguments"); return CHARM_ERR_ARG; }
+    charm_gmesh_info(m, &n, NULL);
+    if (cfg->write_period > 0) {
+        u = (double *) malloc((size_t) (n + 1) * 4 * (4 + cfg->n_comp) * sizeof(double));
+        if (!u) { charm_set_error("gmesh_timesteps: out of memory"); return CHARM_ERR_MEM; }
+    }
+    while (t < cfg->tmax) {
+        int64_t k = 0;
+        double ms = 0.0;
+        if (cfg->write_period > 0 && step % cfg->write_period == 0) {
+            char name[512];
+            snprintf(name, sizeof(name), "%.480s_%08lld", cfg->out_prefix[0] ? cfg->out_prefix : "charm_3d",
+                     (long long) step);
+            if ((rc = charm_gpu_get_state(g, u, n)) || (rc = charm_gmesh_write_vtk(name, m, u, cfg))) break;
+        }
+        /* steps up to the next output or log line, or to TMAX */
+        do {
+            t += dt;
+            k++;
+        } while (t < cfg->tmax && !(cfg->write_period > 0 && (step + k) % cfg->write_period == 0) &&
+                 !(cfg->log_period > 0 && (step + k - 1) % cfg->log_period == 0));
+        if ((rc = charm_gpu_step(g, cfg->model, dt, (int) k))) break;
+        charm_gpu_last_timing(g, &ms, NULL);
+        /* the reference logs after step s when LOG_OUTPUT_STEP divides s */
+        if (cfg->log_period > 0 && (step + k - 1) % cfg->log_period == 0)
+            printf("STEP = %8lld\tTIME = %16.8e\tTAU = %16.8e\tELAPSED_TIME = %16.8e\n", (long long) (step + k - 1),
+                   t - dt, dt, ms * 1e-3 / (double) k);
+        step += k;
+    }
+    free(u);
+    return rc;
+}
+
 /* src/limiters/charm_limiter.c:9-16 */
 void charm_limiter(charm_forest_t *p4est, charm_ghost_t *ghost, charm_data_t *ghost_data)
 {

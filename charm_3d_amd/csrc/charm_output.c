@@ -75,67 +75,26 @@ static int put_block(FILE *fp, const void *data, uint64_t bytes)
            (bytes == 0 || fwrite(data, 1, (size_t) bytes, fp) == (size_t) bytes);
 }
 
-int charm_write_vtk(const char *name, int64_t n, const int32_t *leaves, const double *u,
-                    const charm_config_t *cfg)
+/* the piece <name>_0000.vtu and its index <name>.pvtu: np points [np][3],
+ * n hexahedra conn [n][8] (VTK corner order), cell data treeid, level,
+ * mpirank (zero) and the nf fields fld [nf][n] */
+static int write_pieces(const char *name, int64_t n, int64_t np, const double *pts, const int64_t *conn,
+                        const int32_t *itree, const int32_t *ilev, const double *fld, int nf,
+                        const charm_config_t *cfg)
 {
-    const double inv = 1.0 / (double) CHARM_ROOT_LEN;
-    const int nc = cfg ? cfg->n_comp : 0, nv = 4 + nc, nf = 8 + nc;
-    /* hexahedron corner order (VTK_HEXAHEDRON): x fastest on the bottom
-     * face counter-clockwise, then the top face */
-    static const int hx[8] = {0, 1, 1, 0, 0, 1, 1, 0}, hy[8] = {0, 0, 1, 1, 0, 0, 1, 1},
-                     hz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
-    uint64_t *keys = NULL, *ukeys = NULL;
-    int64_t *conn = NULL, *offs = NULL, np = 0, i;
-    double *pts = NULL, *fld = NULL;
-    uint8_t *types = NULL;
-    int32_t *ilev = NULL, *izero = NULL;
+    int64_t *offs = (int64_t *) malloc((size_t) (n + 1) * sizeof(int64_t));
+    uint8_t *types = (uint8_t *) malloc((size_t) (n + 1));
+    int32_t *izero = (int32_t *) calloc((size_t) (n + 1), sizeof(int32_t));
     char path[4096];
     FILE *fp = NULL;
     int rc = CHARM_ERR_MEM, k, j;
     uint64_t off = 0;
-    if (!name || !cfg || n < 0 || (n > 0 && (!leaves || !u)) || nc < 1 || nc > CHARM_MAX_COMP) {
-        charm_set_error("write_vtk: bad arguments");
-        return CHARM_ERR_ARG;
-    }
-    keys = (uint64_t *) malloc((size_t) (8 * n + 1) * sizeof(uint64_t));
-    ukeys = (uint64_t *) malloc((size_t) (8 * n + 1) * sizeof(uint64_t));
-    conn = (int64_t *) malloc((size_t) (8 * n + 1) * sizeof(int64_t));
-    offs = (int64_t *) malloc((size_t) (n + 1) * sizeof(int64_t));
-    types = (uint8_t *) malloc((size_t) (n + 1));
-    ilev = (int32_t *) malloc((size_t) (n + 1) * sizeof(int32_t));
-    izero = (int32_t *) calloc((size_t) (n + 1), sizeof(int32_t));
-    fld = (double *) malloc((size_t) (n * nf + 1) * sizeof(double));
-    if (!keys || !ukeys || !conn || !offs || !types || !ilev || !izero || !fld) {
-        charm_set_error("write_vtk: out of memory");
-        goto done;
-    }
+    int64_t i;
+    if (!offs || !types || !izero) { charm_set_error("write_vtk: out of memory"); goto done; }
     for (i = 0; i < n; i++) {
-        const int32_t *q = leaves + 4 * i;
-        const int32_t len = CHARM_ROOT_LEN >> q[3];
-        for (k = 0; k < 8; k++)
-            keys[8 * i + k] = corner_key(q[0] + hx[k] * len, q[1] + hy[k] * len, q[2] + hz[k] * len);
         offs[i] = 8 * (i + 1);
         types[i] = 12;                       /* VTK_HEXAHEDRON */
-        ilev[i] = q[3];
     }
-    memcpy(ukeys, keys, (size_t) (8 * n) * sizeof(uint64_t));
-    qsort(ukeys, (size_t) (8 * n), sizeof(uint64_t), cmp_u64);
-    for (i = 0; i < 8 * n; i++)
-        if (np == 0 || ukeys[i] != ukeys[np - 1]) ukeys[np++] = ukeys[i];
-    for (i = 0; i < 8 * n; i++) conn[i] = find_key(ukeys, np, keys[i]);
-    pts = (double *) malloc((size_t) (3 * np + 1) * sizeof(double));
-    if (!pts) { charm_set_error("write_vtk: out of memory"); goto done; }
-    for (i = 0; i < np; i++) {
-        pts[3 * i + 0] = (double) (ukeys[i] >> 42) * inv;
-        pts[3 * i + 1] = (double) ((ukeys[i] >> 21) & 0x1FFFFF) * inv;
-        pts[3 * i + 2] = (double) (ukeys[i] & 0x1FFFFF) * inv;
-    }
-    for (i = 0; i < n; i++) {
-        double row[8 + CHARM_MAX_COMP];
-        cell_fields(cfg, u + (size_t) i * nv * 4, row);
-        for (j = 0; j < nf; j++) fld[(size_t) j * n + i] = row[j];
-    }
-
     /* piece */
     snprintf(path, sizeof(path), "%s_0000.vtu", name);
     fp = fopen(path, "wb");
@@ -181,7 +140,7 @@ int charm_write_vtk(const char *name, int64_t n, const int32_t *leaves, const do
         !put_block(fp, conn, (uint64_t) n * 8 * sizeof(int64_t)) ||
         !put_block(fp, offs, (uint64_t) n * sizeof(int64_t)) ||
         !put_block(fp, types, (uint64_t) n) ||
-        !put_block(fp, izero, (uint64_t) n * sizeof(int32_t)) ||
+        !put_block(fp, itree ? itree : izero, (uint64_t) n * sizeof(int32_t)) ||
         !put_block(fp, ilev, (uint64_t) n * sizeof(int32_t)) ||
         !put_block(fp, izero, (uint64_t) n * sizeof(int32_t))) {
         charm_set_error("write_vtk: write failed (%s)", path);
@@ -228,8 +187,110 @@ int charm_write_vtk(const char *name, int64_t n, const int32_t *leaves, const do
     rc = CHARM_OK;
 done:
     if (fp) fclose(fp);
-    free(keys); free(ukeys); free(conn); free(offs); free(types); free(ilev); free(izero);
-    free(fld); free(pts);
+    free(offs); free(types); free(izero);
+    return rc;
+}
+
+int charm_write_vtk(const char *name, int64_t n, const int32_t *leaves, const double *u,
+                    const charm_config_t *cfg)
+{
+    const double inv = 1.0 / (double) CHARM_ROOT_LEN;
+    const int nc = cfg ? cfg->n_comp : 0, nv = 4 + nc, nf = 8 + nc;
+    /* hexahedron corner order (VTK_HEXAHEDRON): x fastest on the bottom
+     * face counter-clockwise, then the top face */
+    static const int hx[8] = {0, 1, 1, 0, 0, 1, 1, 0}, hy[8] = {0, 0, 1, 1, 0, 0, 1, 1},
+                     hz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+    uint64_t *keys = NULL, *ukeys = NULL;
+    int64_t *conn = NULL, np = 0, i;
+    double *pts = NULL, *fld = NULL;
+    int32_t *ilev = NULL;
+    int rc = CHARM_ERR_MEM, k, j;
+    if (!name || !cfg || n < 0 || (n > 0 && (!leaves || !u)) || nc < 1 || nc > CHARM_MAX_COMP) {
+        charm_set_error("write_vtk: bad arguments");
+        return CHARM_ERR_ARG;
+    }
+    keys = (uint64_t *) malloc((size_t) (8 * n + 1) * sizeof(uint64_t));
+    ukeys = (uint64_t *) malloc((size_t) (8 * n + 1) * sizeof(uint64_t));
+    conn = (int64_t *) malloc((size_t) (8 * n + 1) * sizeof(int64_t));
+    ilev = (int32_t *) malloc((size_t) (n + 1) * sizeof(int32_t));
+    fld = (double *) malloc((size_t) (n * nf + 1) * sizeof(double));
+    if (!keys || !ukeys || !conn || !ilev || !fld) {
+        charm_set_error("write_vtk: out of memory");
+        goto done;
+    }
+    for (i = 0; i < n; i++) {
+        const int32_t *q = leaves + 4 * i;
+        const int32_t len = CHARM_ROOT_LEN >> q[3];
+        for (k = 0; k < 8; k++)
+            keys[8 * i + k] = corner_key(q[0] + hx[k] * len, q[1] + hy[k] * len, q[2] + hz[k] * len);
+        ilev[i] = q[3];
+    }
+    memcpy(ukeys, keys, (size_t) (8 * n) * sizeof(uint64_t));
+    qsort(ukeys, (size_t) (8 * n), sizeof(uint64_t), cmp_u64);
+    for (i = 0; i < 8 * n; i++)
+        if (np == 0 || ukeys[i] != ukeys[np - 1]) ukeys[np++] = ukeys[i];
+    for (i = 0; i < 8 * n; i++) conn[i] = find_key(ukeys, np, keys[i]);
+    pts = (double *) malloc((size_t) (3 * np + 1) * sizeof(double));
+    if (!pts) { charm_set_error("write_vtk: out of memory"); goto done; }
+    for (i = 0; i < np; i++) {
+        pts[3 * i + 0] = (double) (ukeys[i] >> 42) * inv;
+        pts[3 * i + 1] = (double) ((ukeys[i] >> 21) & 0x1FFFFF) * inv;
+        pts[3 * i + 2] = (double) (ukeys[i] & 0x1FFFFF) * inv;
+    }
+    for (i = 0; i < n; i++) {
+        double row[8 + CHARM_MAX_COMP];
+        cell_fields(cfg, u + (size_t) i * nv * 4, row);
+        for (j = 0; j < nf; j++) fld[(size_t) j * n + i] = row[j];
+    }
+
+    rc = write_pieces(name, n, np, pts, conn, NULL, ilev, fld, nf, cfg);
+done:
+    free(keys); free(ukeys); free(conn); free(ilev); free(fld); free(pts);
+    return rc;
+}
+
+int charm_gmesh_write_vtk(const char *name, const charm_gmesh_t *m, const double *u, const charm_config_t *cfg)
+{
+    /* VTK_HEXAHEDRON corner k is the z-order corner zk[k] */
+    static const int zk[8] = {0, 1, 3, 2, 4, 5, 7, 6};
+    const int nc = cfg ? cfg->n_comp : 0, nv = 4 + nc, nf = 8 + nc;
+    int64_t n = 0, i;
+    double *cor = NULL, *pts = NULL, *fld = NULL;
+    int64_t *conn = NULL;
+    int32_t *lv = NULL, *itree = NULL, *ilev = NULL;
+    int rc = CHARM_ERR_MEM, k, j;
+    if (!name || !m || !cfg || !u || nc < 1 || nc > CHARM_MAX_COMP) {
+        charm_set_error("gmesh_write_vtk: bad arguments");
+        return CHARM_ERR_ARG;
+    }
+    charm_gmesh_info(m, &n, NULL);
+    cor = (double *) malloc((size_t) (24 * n + 1) * sizeof(double));
+    pts = (double *) malloc((size_t) (24 * n + 1) * sizeof(double));
+    conn = (int64_t *) malloc((size_t) (8 * n + 1) * sizeof(int64_t));
+    lv = (int32_t *) malloc((size_t) (5 * n + 1) * sizeof(int32_t));
+    itree = (int32_t *) malloc((size_t) (n + 1) * sizeof(int32_t));
+    ilev = (int32_t *) malloc((size_t) (n + 1) * sizeof(int32_t));
+    fld = (double *) malloc((size_t) (n * nf + 1) * sizeof(double));
+    if (!cor || !pts || !conn || !lv || !itree || !ilev || !fld) {
+        charm_set_error("gmesh_write_vtk: out of memory");
+        goto done;
+    }
+    if ((rc = charm_gmesh_leaf_corners(m, cor)) || (rc = charm_gmesh_leaves(m, lv))) goto done;
+    /* each leaf its own 8 points (the trees' corners are not merged) */
+    for (i = 0; i < n; i++) {
+        double row[8 + CHARM_MAX_COMP];
+        for (k = 0; k < 8; k++) {
+            for (j = 0; j < 3; j++) pts[3 * (8 * i + k) + j] = cor[3 * (8 * i + zk[k]) + j];
+            conn[8 * i + k] = 8 * i + k;
+        }
+        itree[i] = lv[5 * i];
+        ilev[i] = lv[5 * i + 4];
+        cell_fields(cfg, u + (size_t) i * nv * 4, row);
+        for (j = 0; j < nf; j++) fld[(size_t) j * n + i] = row[j];
+    }
+    rc = write_pieces(name, n, 8 * n, pts, conn, itree, ilev, fld, nf, cfg);
+done:
+    free(cor); free(pts); free(conn); free(lv); free(itree); free(ilev); free(fld);
     return rc;
 }
 

@@ -537,15 +537,19 @@ static void load_task(yctx_t *C, const yn_t *root, charm_task_t *T)
         for (i = 0; i < 3; i++) T->mesh_periodic[i] = ynum_node(C, pr->kid[i], "a periodic flag") != 0.0;
     }
 
-    /* boundaries (_charm_init_fetch_bnd): by name onto the cube sides */
+    /* boundaries (_charm_init_fetch_bnd): by name onto the cube sides of a
+     * unit-cube mesh; a Gmsh mesh's boundaries take the slots 0-5 in the
+     * order of the file (the mesh's patches bind to them by name) */
     for (i = 0; i < 6; i++) cfg->bnd_type[i] = T->mesh_periodic[i / 2] ? CHARM_BND_PERIODIC : -1;
     bnds = yget(root, "boundaries");
     if (bnds && bnds->type == YN_SEQ) {
+        const int cube = !strcmp(T->mesh_files_type, "unit_cube");
+        if (!cube && bnds->n > 6) { cerr(C, "task.yaml: %d boundaries (at most 6)%s", bnds->n, ""); return; }
         static const char *const bn[5] = {"BOUND_INLET", "BOUND_OUTLET", "BOUND_WALL_SLIP", "BOUND_WALL_NO_SLIP", "BOUND_MASS_FLOW"};
         for (i = 0; i < bnds->n; i++) {
             const yn_t *b = bnds->kid[i], *par = yget(b, "parameters");
             const char *name = ystr(C, b, "name", "a boundary");
-            const int side = cube_side(name);
+            const int side = cube ? cube_side(name) : i;
             int type;
             s = ystr(C, b, "type", "a boundary");
             if ((type = type_of(s, bn, 5)) < 0) { cerr(C, "task.yaml: unknown boundary type '%s'%s", s, ""); return; }
@@ -650,12 +654,104 @@ int charm_task_forest(const charm_task_t *task, charm_forest_t **out)
     return charm_forest_new_uniform(task->cfg.min_level, task->mesh_periodic, out);
 }
 
+/* one region's state as coefficients [V][4]: (T, p) -> (r, cz, e) by EOS
+ * flag 3 (src/charm_eos.c:57-96), then charm_param_prim_to_cons
+ * (src/charm_globals.c:211-223) into coefficient 0, the slopes zero
+ * (src/charm_init.c:47-66) */
+static void region_coeffs(const charm_config_t *cfg, const charm_region_desc_t *g, double *w)
+{
+    const int nv = 4 + cfg->n_comp;
+    charm_prim_t p;
+    int i;
+    memset(&p, 0, sizeof(p));
+    p.mat_id = g->mat_id;
+    p.p = g->p;
+    p.t = g->t;
+    p.u = g->v[0]; p.v = g->v[1]; p.w = g->v[2];
+    for (i = 0; i < cfg->n_comp; i++) p.c[i] = g->c[i];
+    if (cfg->eos == CHARM_EOS_IDEAL) charm_eos_ideal_cfg(cfg, &p, 3);
+    else charm_eos_mix_cfg(cfg, &p, 3);
+    for (i = 0; i < nv * 4; i++) w[i] = 0.0;
+    w[0] = p.r * p.u;
+    w[4] = p.r * p.v;
+    w[8] = p.r * p.w;
+    w[12] = p.r * (p.e + 0.5 * (p.u * p.u + p.v * p.v + p.w * p.w));
+    for (i = 0; i < cfg->n_comp; i++) w[4 * (4 + i)] = p.r * p.c[i];
+}
+
+int charm_task_gmesh(const charm_task_t *task, const char *dir, charm_gmesh_t **out)
+{
+    char path[600];
+    charm_conn_t *conn = NULL;
+    int64_t nv = 0, nt = 0;
+    int np = 0, rc;
+    double *v = NULL;
+    int32_t *ttv = NULL, *tb = NULL, *tr = NULL;
+    if (!task || !out) { charm_set_error("task_gmesh: null argument"); return CHARM_ERR_ARG; }
+    *out = NULL;
+    if (strcmp(task->mesh_files_type, "gmsh_msh")) {
+        charm_set_error("task_gmesh: mesh type '%s' is not a Gmsh mesh (files_type gmsh_msh)", task->mesh_files_type);
+        return CHARM_ERR_ARG;
+    }
+    if (task->mesh_name[0] == '/' || !dir || !dir[0]) snprintf(path, sizeof(path), "%s", task->mesh_name);
+    else snprintf(path, sizeof(path), "%.300s/%s", dir, task->mesh_name);
+    if ((rc = charm_conn_read_msh(path, &conn))) return rc;
+    if ((rc = charm_conn_info(conn, &nv, &nt, &np))) goto done;
+    v = (double *) malloc((size_t) (3 * nv + 1) * sizeof(double));
+    ttv = (int32_t *) malloc((size_t) (8 * nt + 1) * sizeof(int32_t));
+    tb = (int32_t *) malloc((size_t) (6 * nt + 1) * sizeof(int32_t));
+    tr = (int32_t *) malloc((size_t) (nt + 1) * sizeof(int32_t));
+    if (!v || !ttv || !tb || !tr) { charm_set_error("task_gmesh: out of memory"); rc = CHARM_ERR_MEM; goto done; }
+    if ((rc = charm_conn_get(conn, v, ttv, NULL, NULL, NULL, NULL))) goto done;
+    if ((rc = charm_conn_bind_task(conn, task, tb, tr))) goto done;
+    /* p4est_new_ext(..., min_level, fill_uniform = 1, ...), src/charm_3d.c:36-37 */
+    if ((rc = charm_gmesh_build(nv, v, nt, ttv, tb, task->cfg.min_level, out))) goto done;
+    rc = charm_gmesh_set_regions(*out, tr);
+    if (rc) { charm_gmesh_destroy(*out); *out = NULL; }
+done:
+    charm_conn_destroy(conn);
+    free(v); free(ttv); free(tb); free(tr);
+    return rc;
+}
+
+int charm_task_initial_state_gmesh(const charm_task_t *task, const charm_gmesh_t *m, double *u, int64_t n)
+{
+    int64_t nl = 0, q;
+    int32_t *lv = NULL;
+    const int32_t *treg;
+    int nv;
+    if (!task || !m || !u) { charm_set_error("task_initial_state_gmesh: null argument"); return CHARM_ERR_ARG; }
+    charm_gmesh_info(m, &nl, NULL);
+    if (n != nl) {
+        charm_set_error("task_initial_state_gmesh: %lld leaves, state for %lld", (long long) nl, (long long) n);
+        return CHARM_ERR_ARG;
+    }
+    treg = charm_gmesh_regions(m);
+    if (!treg) { charm_set_error("task_initial_state_gmesh: the mesh has no regions (charm_task_gmesh binds them)"); return CHARM_ERR_ARG; }
+    lv = (int32_t *) malloc((size_t) (5 * n + 1) * sizeof(int32_t));
+    if (!lv) { charm_set_error("task_initial_state_gmesh: out of memory"); return CHARM_ERR_MEM; }
+    charm_gmesh_leaves(m, lv);
+    nv = 4 + task->cfg.n_comp;
+    /* each leaf takes the region of its tree (the tree attribute's region,
+     * src/charm_init.c:26-27) */
+    for (q = 0; q < n; q++) {
+        const int g = treg[lv[5 * q]];
+        if (g < 0 || g >= task->n_regions) {
+            free(lv);
+            charm_set_error("task_initial_state_gmesh: tree %d has region %d of %d", lv[5 * q], g, task->n_regions);
+            return CHARM_ERR_ARG;
+        }
+        region_coeffs(&task->cfg, &task->region[g], u + (size_t) q * nv * 4);
+    }
+    free(lv);
+    return CHARM_OK;
+}
+
 int charm_task_initial_state(const charm_task_t *task, const charm_forest_t *f, double *u, int64_t n)
 {
     const charm_config_t *cfg;
     const charm_region_desc_t *g;
-    charm_prim_t p;
-    int nv, i, k;
+    int nv, i;
     int64_t q;
     if (!task || !f || !u) { charm_set_error("task_initial_state: null argument"); return CHARM_ERR_ARG; }
     if (n != charm_forest_num_leaves(f)) {
@@ -669,26 +765,6 @@ int charm_task_initial_state(const charm_task_t *task, const charm_forest_t *f, 
     g = &task->region[0];
     for (i = 0; i < task->n_regions; i++)
         if (task->region[i].cell_type == 0) { g = &task->region[i]; break; }
-    /* (T, p) -> (r, cz, e) by EOS flag 3 (src/charm_eos.c:57-96), then
-     * charm_param_prim_to_cons (src/charm_globals.c:211-223) into
-     * coefficient 0, the slopes zero (src/charm_init.c:47-66) */
-    memset(&p, 0, sizeof(p));
-    p.mat_id = g->mat_id;
-    p.p = g->p;
-    p.t = g->t;
-    p.u = g->v[0]; p.v = g->v[1]; p.w = g->v[2];
-    for (i = 0; i < cfg->n_comp; i++) p.c[i] = g->c[i];
-    if (cfg->eos == CHARM_EOS_IDEAL) charm_eos_ideal_cfg(cfg, &p, 3);
-    else charm_eos_mix_cfg(cfg, &p, 3);
-    p.e_tot = p.e + 0.5 * (p.u * p.u + p.v * p.v + p.w * p.w);
-    for (q = 0; q < n; q++) {
-        double *w = u + (size_t) q * nv * 4;
-        for (i = 0; i < nv * 4; i++) w[i] = 0.0;
-        w[0] = p.r * p.u;
-        w[4] = p.r * p.v;
-        w[8] = p.r * p.w;
-        w[12] = p.r * (p.e + 0.5 * (p.u * p.u + p.v * p.v + p.w * p.w));
-        for (k = 0; k < cfg->n_comp; k++) w[4 * (4 + k)] = p.r * p.c[k];
-    }
+    for (q = 0; q < n; q++) region_coeffs(cfg, g, u + (size_t) q * nv * 4);
     return CHARM_OK;
 }

@@ -340,14 +340,15 @@ typedef struct {
     int    mesh_periodic[3];           /* unit_cube: mesh.periodic [x, y, z] */
 } charm_task_t;
 /* Parse a task.yaml (the reference's keys; yaml-cpp's role is taken by a
- * parser of the YAML subset task files use).  Boundaries are bound to the
- * unit cube's sides by name: LEFT -x, RIGHT +x, FRONT -y, BACK +y, BOTTOM -z,
- * TOP +z.  This build's own keys: control.PRECISION (FP64, FP32, FP32_MIX;
+ * parser of the YAML subset task files use).  Boundaries of a unit-cube mesh
+ * are bound to its sides by name: LEFT -x, RIGHT +x, FRONT -y, BACK +y,
+ * BOTTOM -z, TOP +z; a Gmsh mesh's boundaries (at most 6) take the
+ * configuration's slots 0-5 in the order of the file.  This build's own keys: control.PRECISION (FP64, FP32, FP32_MIX;
  * default FP64), mesh.files_type unit_cube with mesh.periodic.  Errors
  * (with the reference's messages) are returned, never fatal. */
 int charm_task_load_yaml(const char *path, charm_task_t *task);
-/* the uniform forest at MIN_LEVEL (src/charm_3d.c:36-37); gmsh meshes are
- * CHARM_ERR_UNSUPPORTED */
+/* the uniform forest at MIN_LEVEL (src/charm_3d.c:36-37) of a unit-cube
+ * task; a Gmsh task's mesh is charm_task_gmesh (CHARM_ERR_UNSUPPORTED here) */
 int charm_task_forest(const charm_task_t *task, charm_forest_t **out);
 /* charm_init_initial_condition (src/charm_init.c:12-73): the region of cell
  * type 0 through EOS flag 3 into coefficient 0, slopes zero; u FP64 [n][V][4] */
@@ -407,6 +408,29 @@ int  charm_gmesh_faces(const charm_gmesh_t *m, int32_t *faces);
  * chemistry, every flux, BJ, every boundary type.  The state layout is the
  * octree's, [n][V][4], in this mesh's leaf order. */
 int  charm_gpu_upload_gmesh(charm_gpu_ctx_t *c, const charm_gmesh_t *m);
+/* each leaf's 8 corners through its tree's trilinear map, [n][8][3] in
+ * z-order (corner x + 2 y + 4 z) */
+int  charm_gmesh_leaf_corners(const charm_gmesh_t *m, double *corners);
+/* VTK output of a multi-tree mesh in charm_write_vtk's format: each leaf a
+ * hexahedron of its own 8 corners, cell data treeid, level, mpirank and the
+ * cell-centre fields of u (FP64 [n][V][4]) */
+int  charm_gmesh_write_vtk(const char *name, const charm_gmesh_t *m, const double *u, const charm_config_t *cfg);
+/* A Gmsh task's mesh (files_type gmsh_msh; src/charm_3d.c:32-37): mesh.name
+ * (relative to dir unless absolute) read by charm_conn_read_msh, its patches
+ * and regions bound to the task's boundary and region names
+ * (charm_conn_bind_task), refined uniformly to MIN_LEVEL.  The mesh keeps
+ * each tree's region for charm_task_initial_state_gmesh. */
+int  charm_task_gmesh(const charm_task_t *task, const char *dir, charm_gmesh_t **out);
+/* charm_init_initial_condition (src/charm_init.c:12-73) on such a mesh:
+ * each leaf takes the region of its tree; u FP64 [n][V][4] */
+int  charm_task_initial_state_gmesh(const charm_task_t *task, const charm_gmesh_t *m, double *u, int64_t n);
+/* charm_timesteps (src/charm_timesteps.c:23-51) on a context that runs a
+ * multi-tree mesh: dt = TAU (the models' get_dt), t += dt while t < TMAX,
+ * the solution written before every step FILE_OUTPUT_STEP divides
+ * (<out_prefix>_<step %08d>, charm_gmesh_write_vtk), a log line after every
+ * step LOG_OUTPUT_STEP divides.  Adaptation across trees is not built:
+ * REFINE_PERIOD is not acted on. */
+int  charm_gmesh_timesteps(charm_gpu_ctx_t *g, const charm_gmesh_t *m, const charm_config_t *cfg);
 /* ------------------------------------------------ reference entry points
  * (include/charm_models.h:15-23, charm_limiter.h:13-15, charm_fluxes.h:19-32,
  * charm_amr.h:9-10)

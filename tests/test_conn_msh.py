@@ -48,8 +48,11 @@ def zcorner(bits):
     return bits[0] + 2 * bits[1] + 4 * bits[2]
 
 
-def write_box(path, n, rng, rotate=True, extra_lines=(), binary=False, dup_hex=False, stray_quad=False):
-    """n = (nx, ny, nz) unit hexahedra; returns nothing, writes a Gmsh 2.2 file"""
+def write_box(path, n, rng, rotate=True, extra_lines=(), binary=False, dup_hex=False, stray_quad=False,
+              split_x=None):
+    """n = (nx, ny, nz) unit hexahedra; returns nothing, writes a Gmsh 2.2 file.
+    Hexahedra are in the physical volume "BOX", or with split_x those at
+    i >= split_x in a second one, "BOX2"."""
     nx, ny, nz = n
     pts = [(i, j, k) for k in range(nz + 1) for j in range(ny + 1) for i in range(nx + 1)]
     ids = rng.permutation(len(pts)) * 3 + 7                      # sparse, shuffled node ids
@@ -67,7 +70,7 @@ def write_box(path, n, rng, rotate=True, extra_lines=(), binary=False, dup_hex=F
                     rb = (r @ (2 * b - 1) + 1) // 2                 # rotated corner bits
                     corner[c] = nid[(i + rb[0], j + rb[1], k + rb[2])]
                 gm = [corner[Z_TO_GMSH[g]] for g in range(8)]
-                elems.append((5, 1, gm))
+                elems.append((5, 9 if split_x is not None and i >= split_x else 1, gm))
     if dup_hex:
         elems.append(elems[0])
     for side in range(6):
@@ -88,10 +91,13 @@ def write_box(path, n, rng, rotate=True, extra_lines=(), binary=False, dup_hex=F
         elems.append((3, 2, [nid[(0, 0, 0)], nid[(1, 0, 0)], nid[(1, 1, 1)], nid[(0, 1, 1)]]))
     with open(path, "w") as f:
         f.write("$MeshFormat\n2.2 %d 8\n$EndMeshFormat\n" % (1 if binary else 0))
-        f.write("$PhysicalNames\n7\n")
+        f.write("$PhysicalNames\n%d\n" % (7 if split_x is None else 8))
         for s, name in enumerate(SIDES):
             f.write('2 %d "%s"\n' % (2 + s, name))
-        f.write('3 1 "BOX"\n$EndPhysicalNames\n')
+        f.write('3 1 "BOX"\n')
+        if split_x is not None:
+            f.write('3 9 "BOX2"\n')
+        f.write('$EndPhysicalNames\n')
         f.write("$Nodes\n%d\n" % len(pts))
         for p in pts:
             f.write("%d %g %g %g\n" % (nid[p], *p))

@@ -323,7 +323,7 @@ def test_program_reports_task_errors(tmp_path):
     assert r.returncode == 1 and "cannot open task.yaml" in r.stderr
     write(tmp_path, EULER_TASK.replace("files_type: unit_cube", "files_type: gmsh_msh"))
     r = run_program(tmp_path)
-    assert r.returncode == 1 and "mesh type 'gmsh_msh' is not supported" in r.stderr
+    assert r.returncode == 1 and "charm_3d: mesh: Failed to open" in r.stderr
     p = write(tmp_path, EULER_TASK.replace("LIMITER: BJ", "LIMITER: MINMOD"), "other.yaml")
     r = run_program(tmp_path, p)
     assert r.returncode == 1 and "unknown limiter type 'MINMOD'" in r.stderr
