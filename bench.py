@@ -544,6 +544,33 @@ def adjoint_fd_check(CA, W, dt, g32, th32, ndir=32, steps=(1e-6, 1e-7)):
                           "median_rel_diff_du0": float(np.median(rel)),
                           "worst_diff_over_rms_du0": float(np.abs(ad - fd).max() / np.sqrt((ad ** 2).mean())),
                           "worst_rel_diff_dinlet": worst_th}
+        if h == steps[0]:
+            # the kinks behind the worst directions: the face points whose HLLC
+            # branch differs between the +h and -h runs, at the start of each
+            # of the 400 steps, for the three worst and the three best
+            # directions
+            order = np.argsort(rel)
+            rows = []
+            for i in list(order[-3:][::-1]) + list(order[:3]):
+                cp, _ = device(u0 + h * dirs[i], inlet)
+                cm, _ = device(u0 - h * dirs[i], inlet)
+                tot, npts, bits, first = 0, 0, np.zeros(9, dtype=np.int64), None
+                for _ in range(400):
+                    pts, fl, pb = cp.hllc_branch_flips(cm, per_bit=True)
+                    if first is None:
+                        first = fl / pts
+                    tot += fl
+                    npts += pts
+                    bits += pb
+                    cp.run(dt, 1)
+                    cm.run(dt, 1)
+                cp.close()
+                cm.close()
+                rows.append({"rel_diff": float(rel[i]), "flipped_fraction": tot / npts,
+                             "flipped_fraction_step0": first,
+                             "by_condition": dict(zip(("p*>pL", "p*>pR", "swap", "SL>=0", "SR<=0", "S*>=0",
+                                                       "S*=SL", "S*=SR", "pvrs<=0"), (bits / npts).tolist()))})
+            cd["h=%g" % h]["hllc_branch_flips_worst3_best3"] = rows
     ctx.close()
     return {"J": J, "dJ_dinlet_u_T_p": th.tolist(), "forward_ms": fwd_ms, "adjoint_ms": adj_ms,
             "central_differences": cd,

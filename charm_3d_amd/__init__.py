@@ -147,6 +147,8 @@ def lib():
         "charm_task_gmesh": (C.c_int, [C.POINTER(Task), C.c_char_p, C.POINTER(vp)]),
         "charm_task_initial_state_gmesh": (C.c_int, [C.POINTER(Task), vp, dp, i64]),
         "charm_gmesh_timesteps": (C.c_int, [vp, vp, C.POINTER(Config)]),
+        "charm_gpu_hllc_branch_flips": (C.c_int, [vp, vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                                  C.POINTER(C.c_int64)]),
         "charm_gpu_upload_gmesh": (C.c_int, [vp, vp]),
         "charm_task_initial_state": (C.c_int, [C.POINTER(Task), vp, dp, i64]),
         "charm_gpu_amr_indicator": (C.c_int, [vp, dp, dp, i64]),
@@ -562,6 +564,15 @@ class GpuContext:
         """a multi-tree mesh (GMesh) in place of the octree"""
         check(lib().charm_gpu_upload_gmesh(self.h, gmesh.h))
         self.n = gmesh.n
+
+    def hllc_branch_flips(self, other, per_bit=False):
+        """charm_gpu_hllc_branch_flips: (interior face points, points whose
+        HLLC branch differs between this context's state and other's[, the
+        points where each of the 9 branch conditions differs])"""
+        a, b = C.c_int64(), C.c_int64()
+        bits = (C.c_int64 * 9)()
+        check(lib().charm_gpu_hllc_branch_flips(self.h, other.h, C.byref(a), C.byref(b), bits))
+        return (a.value, b.value, list(bits)) if per_bit else (a.value, b.value)
 
     def gmesh_timesteps(self, gmesh, cfg):
         """charm_gmesh_timesteps: the time loop to cfg.tmax on the uploaded

@@ -791,6 +791,39 @@ extern "C" int charm_gpu_time_kernels(charm_gpu_ctx_t *c, double dt, int reps, d
     return check_flag(c);
 }
 
+extern "C" int charm_gpu_hllc_branch_flips(charm_gpu_ctx_t *a, charm_gpu_ctx_t *b, int64_t *points,
+                                           int64_t *flips, int64_t *per_bit)
+{
+    int rc = need_mesh(a, -1);
+    if (rc) return rc;
+    if ((rc = need_mesh(b, -1))) return rc;
+    if ((rc = need_octree(a, "hllc_branch_flips")) || (rc = need_octree(b, "hllc_branch_flips"))) return rc;
+    if (!points || !flips) { charm_set_error("hllc_branch_flips: null output"); return CHARM_ERR_ARG; }
+    if (a->prec != CHARM_FP64 || b->prec != CHARM_FP64 || a->ph || b->ph || a->nc != b->nc || a->n != b->n ||
+        a->nch != b->nch || a->ncf != b->ncf) {
+        charm_set_error("hllc_branch_flips: two FP64 constant-cp contexts on the same mesh");
+        return CHARM_ERR_ARG;
+    }
+    unsigned long long host[2 + kHllcBits] = {0}, *dev = NULL;
+    HIPCHK(hipStreamSynchronize(b->st));
+    HIPCHK(hipMalloc((void **) &dev, sizeof(host)));
+    FlipArgs F;
+    F.Ua = (const double *) a->B[0]; F.Ub = (const double *) b->B[0];
+    F.cf4 = a->d_cf4; F.ch_ebeg = a->d_ch_ebeg; F.ch_fbeg = a->d_ch_fbeg;
+    F.n = a->n; F.n_chunks = (int) a->nch; F.out = dev;
+    hipError_t e = hipMemcpyAsync(dev, host, sizeof(host), hipMemcpyHostToDevice, a->st);
+    if (e == hipSuccess) e = launch_hllc_flips(a->nc, F, a->P, a->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(host, dev, sizeof(host), hipMemcpyDeviceToHost, a->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(a->st);
+    (void) hipFree(dev);
+    if (e != hipSuccess) { charm_set_error("hllc_branch_flips: %s", hipGetErrorString(e)); return CHARM_ERR_HIP; }
+    *points = (int64_t) host[0];
+    *flips = (int64_t) host[1];
+    if (per_bit)
+        for (int i = 0; i < kHllcBits; i++) per_bit[i] = (int64_t) host[2 + i];
+    return CHARM_OK;
+}
+
 extern "C" int charm_gpu_last_timing(charm_gpu_ctx_t *c, double *stage_ms, double *limiter_ms)
 {
     if (!c) { charm_set_error("null context"); return CHARM_ERR_ARG; }

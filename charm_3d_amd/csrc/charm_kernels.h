@@ -205,5 +205,17 @@ hipError_t launch_ns(int prec, int what, int nc, const NsArgs &A, const DevParam
 hipError_t launch_amr_indicator(int prec, int nc, int ph, const AmrArgs &A, const DevParams &P, hipStream_t st);
 hipError_t launch_amr_project(int prec, const ProjArgs &A, hipStream_t st);
 hipError_t launch_convert(int prec, int to_f64, const void *src, void *dst, int64_t n, int nvk, hipStream_t st);
+// HLLC branch comparison of two FP64 states on one chunked octree mesh
+constexpr int kHllcBits = 9;
+struct FlipArgs {
+    const double *Ua, *Ub;    // the two states, [(4 v + k) n + e]
+    const int4 *cf4;
+    const int *ch_ebeg, *ch_fbeg;
+    int64_t n;
+    int n_chunks;
+    unsigned long long *out;  // [2 + kHllcBits]: interior face points, points whose branch
+                              // differs, points where each branch bit differs
+};
+hipError_t launch_hllc_flips(int nc, const FlipArgs &A, const DevParams &P, hipStream_t st);
 
 } // namespace charm

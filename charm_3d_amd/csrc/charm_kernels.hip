@@ -395,6 +395,131 @@ hipError_t launch_flux_points(int prec, int nc, int flux, const FluxArgs &A, con
     return prec ? launch_flux_r<float>(nc, flux, A, P, st) : launch_flux_r<double>(nc, flux, A, P, st);
 }
 
+// The branch hllc_x takes at one face point, as kHllcBits bits: 0 p* > p_L (shock
+// factor on the left), 1 p* > p_R, 2 the wave speeds swapped, 3 S_L >= 0,
+// 4 S_R <= 0, 5 S* >= 0, 6 S* clamped to S_L, 7 S* clamped to S_R, 8 the
+// PVRS pressure clamped to zero.  The conditions restate hllc_x's
+// (src/charm_fluxes.c:319-437) in the same arithmetic.
+template <int NC>
+__device__ int hllc_branch(const Frame<double, NC> &L, const Frame<double, NC> &Rt)
+{
+    const double p_pvrs = 0.5 * (L.p + Rt.p) - 0.5 * (Rt.un - L.un) * 0.25 * (L.r + Rt.r) * (L.cz + Rt.cz);
+    const double p_star = p_pvrs > 0.0 ? p_pvrs : 0.0;
+    const double gl = (L.gam + 1.0) / (2.0 * L.gam), gr = (Rt.gam + 1.0) / (2.0 * Rt.gam);
+    const double ql = (p_star <= L.p) ? 1.0 : sqrt(1.0 + gl * (p_star / L.p - 1.0));
+    const double qr = (p_star <= Rt.p) ? 1.0 : sqrt(1.0 + gr * (p_star / Rt.p - 1.0));
+    double sl = L.un - L.cz * ql, sr = Rt.un + Rt.cz * qr;
+    const bool swap = sl > sr;
+    if (swap) { const double t = sl; sl = sr; sr = t; }
+    double s_star = Rt.p - L.p;
+    s_star += L.r * L.un * (sl - L.un);
+    s_star -= Rt.r * Rt.un * (sr - Rt.un);
+    s_star = s_star / (L.r * (sl - L.un) - Rt.r * (sr - Rt.un));
+    const bool lo = s_star < sl;
+    if (lo) s_star = sl;
+    const bool hi = s_star > sr;
+    if (hi) s_star = sr;
+    return (p_star > L.p) | (p_star > Rt.p) << 1 | swap << 2 | (sl >= 0.0) << 3 | (sr <= 0.0) << 4 |
+           (s_star >= 0.0) << 5 | lo << 6 | hi << 7 | !(p_pvrs > 0.0) << 8;
+}
+
+// the HLLC branch at the six points of an interior face for one state
+// (the traces and face frame of face_core)
+template <int NC, int AX>
+__device__ void face_branches(const DevParams &P, const double *u0, const double *u1, int info, int *code)
+{
+    constexpr int NV = 4 + NC;
+    constexpr int T1 = (AX == 0) ? 1 : 0, T2 = (AX == 2) ? 1 : 2;
+    const int dir0 = info & 7, kind = (info >> 3) & 3, sub = (info >> 5) & 3;
+    const double s = (dir0 & 1) ? 1.0 : -1.0;
+    const double pa0 = 0.5 * s, pa1 = -pa0;
+    const double d1 = (sub & 1) ? 0.25 : -0.25, d2 = (sub & 2) ? 0.25 : -0.25;
+    const bool big0 = (kind == FACE_HANG1), big1 = (kind == FACE_HANG0);
+    const double m0 = big0 ? 0.5 : 1.0, m1 = big1 ? 0.5 : 1.0;
+    const double o01 = big0 ? d1 : 0.0, o02 = big0 ? d2 : 0.0, o11 = big1 ? d1 : 0.0, o12 = big1 ? d2 : 0.0;
+    for (int k = 0; k < 6; k++) {
+        const double g1 = gp_t1<double>(k), g2 = gp_t2<double>(k);
+        double ta[NV], tb[NV];
+        #pragma unroll
+        for (int v = 0; v < NV; v++) {
+            ta[v] = u0[4*v] + pa0 * u0[4*v+1+AX] + (o01 + m0 * g1) * u0[4*v+1+T1] + (o02 + m0 * g2) * u0[4*v+1+T2];
+            tb[v] = u1[4*v] + pa1 * u1[4*v+1+AX] + (o11 + m1 * g1) * u1[4*v+1+T1] + (o12 + m1 * g2) * u1[4*v+1+T2];
+        }
+        Prim<double, NC> pl, pr;
+        cons_to_prim<double, NC, 0>(P, ta, pl);
+        cons_to_prim<double, NC, 0>(P, tb, pr);
+        Frame<double, NC> fl, fr;
+        const double vl[3] = {pl.u, pl.v, pl.w}, vr[3] = {pr.u, pr.v, pr.w};
+        fl.r = pl.r; fl.e_tot = pl.e_tot; fl.p = pl.p; fl.cz = pl.cz; fl.gam = pl.gam;
+        fr.r = pr.r; fr.e_tot = pr.e_tot; fr.p = pr.p; fr.cz = pr.cz; fr.gam = pr.gam;
+        fl.un = s * vl[AX]; fr.un = s * vr[AX];
+        fl.ut1 = vl[T1]; fr.ut1 = vr[T1];
+        fl.ut2 = vl[T2]; fr.ut2 = vr[T2];
+        #pragma unroll
+        for (int i = 0; i < NC; i++) { fl.c[i] = pl.c[i]; fr.c[i] = pr.c[i]; }
+        code[k] = hllc_branch<NC>(fl, fr);
+    }
+}
+
+// one workgroup per chunk; each interior face counted in the chunk that
+// holds its side-0 element (once)
+template <int NC>
+__global__ __launch_bounds__(256) void k_hllc_flips(FlipArgs A, DevParams P)
+{
+    constexpr int NV = 4 + NC;
+    constexpr int NQ = 2 + kHllcBits;
+    __shared__ unsigned long long red[NQ][256];
+    const int c = blockIdx.x;
+    const int fb = A.ch_fbeg[c], fe = A.ch_fbeg[c + 1], eb = A.ch_ebeg[c], ee = A.ch_ebeg[c + 1];
+    unsigned long long q[NQ];
+    #pragma unroll
+    for (int i = 0; i < NQ; i++) q[i] = 0;
+    for (int f = fb + (int) threadIdx.x; f < fe; f += 256) {
+        const int4 rec = A.cf4[f];
+        if (((rec.z >> 3) & 3) == FACE_BND || rec.x < eb || rec.x >= ee) continue;
+        double a0[4 * NV], a1[4 * NV], b0[4 * NV], b1[4 * NV];
+        load_elem<double, NV>(A.Ua, A.n, rec.x, a0);
+        load_elem<double, NV>(A.Ua, A.n, rec.y, a1);
+        load_elem<double, NV>(A.Ub, A.n, rec.x, b0);
+        load_elem<double, NV>(A.Ub, A.n, rec.y, b1);
+        int ca[6], cb[6];
+        const int ax = (rec.z & 7) >> 1;
+        if (ax == 0) { face_branches<NC, 0>(P, a0, a1, rec.z, ca); face_branches<NC, 0>(P, b0, b1, rec.z, cb); }
+        else if (ax == 1) { face_branches<NC, 1>(P, a0, a1, rec.z, ca); face_branches<NC, 1>(P, b0, b1, rec.z, cb); }
+        else { face_branches<NC, 2>(P, a0, a1, rec.z, ca); face_branches<NC, 2>(P, b0, b1, rec.z, cb); }
+        for (int k = 0; k < 6; k++) {
+            const int x = ca[k] ^ cb[k];
+            q[0]++;
+            q[1] += x != 0;
+            #pragma unroll
+            for (int i = 0; i < kHllcBits; i++) q[2 + i] += (x >> i) & 1;
+        }
+    }
+    #pragma unroll
+    for (int i = 0; i < NQ; i++) red[i][threadIdx.x] = q[i];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int) threadIdx.x < w)
+            for (int i = 0; i < NQ; i++) red[i][threadIdx.x] += red[i][threadIdx.x + w];
+        __syncthreads();
+    }
+    if ((int) threadIdx.x < NQ) atomicAdd(A.out + threadIdx.x, red[threadIdx.x][0]);
+}
+
+hipError_t launch_hllc_flips(int nc, const FlipArgs &A, const DevParams &P, hipStream_t st)
+{
+    if (A.n_chunks <= 0) return hipSuccess;
+    const dim3 grid((unsigned) A.n_chunks), block(256);
+    switch (nc) {
+    case 1: hipLaunchKernelGGL(k_hllc_flips<1>, grid, block, 0, st, A, P); break;
+    case 2: hipLaunchKernelGGL(k_hllc_flips<2>, grid, block, 0, st, A, P); break;
+    case 3: hipLaunchKernelGGL(k_hllc_flips<3>, grid, block, 0, st, A, P); break;
+    case 4: hipLaunchKernelGGL(k_hllc_flips<4>, grid, block, 0, st, A, P); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_convert(int prec, int to_f64, const void *src, void *dst, int64_t n, int nvk, hipStream_t st)
 {
     const dim3 grid((unsigned) ((n + 255) / 256)), block(256);

@@ -313,6 +313,20 @@ int charm_gpu_flux_vjp_batch(charm_gpu_ctx_t *ctx, int flux, int64_t n, const do
                              const double *prim_r, const double *normals, const double *qbar,
                              double *prim_l_bar, double *prim_r_bar);
 
+/* No counterpart in the reference; for callers that check the adjoint's
+ * central differences, whose objective has kinks where a perturbation moves
+ * a face point across a branch of the HLLC flux.  For two FP64 constant-cp
+ * contexts on the same octree mesh: at every interior face point the branch
+ * hllc_x takes (the shock factor on either side, the wave-speed swap, the
+ * upwind and star regions, the S* clamps, the PVRS clamp) for each context's
+ * current state; points = the interior face points, flips = those whose
+ * branch differs between the two states, per_bit [9] (may be NULL) = those
+ * where each part differs: 0 p* > p_L, 1 p* > p_R, 2 the wave speeds
+ * swapped, 3 S_L >= 0, 4 S_R <= 0, 5 S* >= 0, 6 S* clamped to S_L, 7 to S_R,
+ * 8 p_pvrs <= 0. */
+int charm_gpu_hllc_branch_flips(charm_gpu_ctx_t *a, charm_gpu_ctx_t *b, int64_t *points, int64_t *flips,
+                                int64_t *per_bit);
+
 /* VTK output (src/charm_output.c:10-126): <name>_0000.vtu (one
  * UnstructuredGrid piece, raw appended data) and <name>.pvtu; points are the
  * shared leaf corners, cells hexahedra with cell data treeid, level,

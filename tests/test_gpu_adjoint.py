@@ -196,3 +196,37 @@ def test_fp32_chunked_adjoint_close_to_fp64(name):
     err = np.sqrt(((g32 - g64) ** 2).sum() / (g64 ** 2).sum())
     print("%s: FP32 vs FP64 gradient, relative L2 %.2e" % (name, err))
     assert err < 1e-4, err
+
+
+def test_hllc_branch_flips_count_points_across_a_branch():
+    """charm_gpu_hllc_branch_flips on a random subsonic state (no point near
+    a branch switch): equal states flip no point; a leaf made supersonic
+    flips points only on its own six faces; a 1e-13 relative perturbation
+    flips none"""
+    cfg, leaves, _, per = W.c5_pulse(3, CA.FP64)
+    rng = np.random.default_rng(5)
+    n = len(leaves)
+    u0 = np.zeros((n, 5, 4))
+    u0[:, :, 0] = W.prim_to_cons(1 + 0.1 * rng.uniform(-1, 1, n), 0.3 * rng.normal(size=(n, 3)),
+                                 1 + 0.1 * rng.uniform(-1, 1, n))
+    ce, cf = CA.recommended_chunks(cfg)
+    mesh = CA.Forest.from_leaves(leaves, per).flatten(ce, cf)
+    a, b = CA.GpuContext(cfg), CA.GpuContext(cfg)
+    for c in (a, b):
+        c.upload_mesh(mesh)
+        c.set_state(u0)
+    pts, flips = a.hllc_branch_flips(b)
+    assert pts == 6 * 3 * 8 * 8 * 7 and flips == 0          # every interior face of the level-3 box, once
+    u1 = u0.copy()
+    e = 200
+    r = u1[e, 4, 0]
+    u1[e, 3, 0] -= 0.5 * (u1[e, 0:3, 0] ** 2).sum() / r
+    u1[e, 0:3, 0] = [3.0 * r * np.sqrt(1.4 * 1.1 / 0.9), 0.0, 0.0]    # x velocity above 3 c: supersonic
+    u1[e, 3, 0] += 0.5 * u1[e, 0, 0] ** 2 / r
+    b.set_state(u1)
+    pts1, flips1 = a.hllc_branch_flips(b)
+    assert pts1 == pts and 1 <= flips1 <= 36, flips1
+    b.set_state(u0 * (1 + 1e-13 * rng.uniform(-1, 1, u0.shape)))
+    assert a.hllc_branch_flips(b)[1] == 0
+    a.close()
+    b.close()
