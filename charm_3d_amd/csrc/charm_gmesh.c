@@ -378,7 +378,7 @@ int charm_gmesh_build(int64_t n_vertices, const double *vertices, int64_t n_tree
             int have = 0;
             for (int f = 0; f < 6; f++) {
                 const int64_t p = pe[6 * e + f];
-                int pt, a, k;
+                int pt, a;
                 double nn[3];
                 if (!(p < 0 || p > e || (p == e && pf[6 * e + f] > f))) continue;
                 fi = next[2 * f + (p < 0)]++;
@@ -405,7 +405,6 @@ int charm_gmesh_build(int64_t n_vertices, const double *vertices, int64_t n_tree
                     }
                     m->fg[(size_t) (36 + pt) * nf + fi] = g.fwj[f][pt];
                 }
-                (void) k;
             }
         }
         /* limiter neighbours: the leaf across each face, or -1 - the boundary face */

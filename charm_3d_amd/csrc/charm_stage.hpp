@@ -6,8 +6,9 @@
 // variable v = ru rv rw re rc[..], Taylor coefficient k (1, (x-xc)/h,
 // (y-yc)/h, (z-zc)/h): a wave's per-element accesses are unit-stride.
 //
-// One RK stage = k_stage over workgroup chunks (<= 128 Morton-consecutive
-// elements each):
+// One RK stage = k_stage over workgroup chunks (at most one element per
+// thread of the 128- or 256-thread workgroup, Morton-consecutive, and a cap
+// on the chunk's faces that sizes its LDS):
 //   phase 1  one thread per (sub)face touching the chunk: both traces at the
 //            six face Gauss points, the Riemann flux, and the three flux
 //            moments per variable (sum q wJ, sum q phi_t1 wJ, sum q phi_t2 wJ in
@@ -355,9 +356,10 @@ __device__ __forceinline__ const DevParams &stage_params(const StageArgs &A, con
     else return Pv;
 }
 
-// The LDS of a chunk allows three 256-thread workgroups per CU, three waves
-// per SIMD: the FP32 deviation-form instances are held to the registers of
-// that occupancy (168 VGPRs)
+// The FP32 deviation-form instances are held to at most 168 VGPRs (three
+// waves per SIMD) so that a change of the code cannot drop them below the
+// occupancy their LDS allows; with the rolled point loop they need 117, and
+// four 40 KB workgroups share a CU
 template <typename R, int NC, int FLUX, int PH>
 constexpr int stage_min_waves()
 {
