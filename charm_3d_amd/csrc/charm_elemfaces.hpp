@@ -47,11 +47,13 @@ __device__ __forceinline__ void face_point(int d, int k, R o1, R o2, R scale, R 
     ph[2] = (a == 2) ? pn : p2;                           // t2 = 2 unless a = 2
 }
 
-// Enumerates the face points of leaf e: fn(d, nb, self_ph, nb_ph, wj, sign)
-// for interior points (d the face direction of e; sign +1 when e is side 0
-// of the face, i.e. its face lies on the upper side of axis d / 2) and
-// bfn(d, bc_face, k, self_ph, wj) for boundary faces.  Hanging faces are
-// visited at the small face's points.
+// Enumerates the face points of leaf e: fn(d, nb, self_ph, nb_ph, wj, sign,
+// k, js, jn) for interior points (d the face direction of e; sign +1 when e
+// is side 0 of the face, i.e. its face lies on the upper side of axis d / 2;
+// point k of the (sub)face numbered js among the sub-faces of e's face d and
+// jn among those of nb's face d ^ 1, 0 on a face that is not split on that
+// side) and bfn(d, bc_face, k, self_ph, wj) for boundary faces.  Hanging
+// faces are visited at the small face's points.
 template <typename R, typename F, typename B>
 __device__ __forceinline__ void face_points(const int *__restrict__ nbr, const int4 *__restrict__ aux,
                                             const uint8_t *__restrict__ level, int e, R h, F fn, B bfn)
@@ -62,12 +64,13 @@ __device__ __forceinline__ void face_points(const int *__restrict__ nbr, const i
         const R sign = (d & 1) ? R(1) : R(-1);
         if (nb >= 0) {
             R o1 = R(0), o2 = R(0), m = R(1);
+            int jn = 0;
             if (level[nb] != lev) {
-                // e is sub-face j of the coarser neighbour's face
+                // e is sub-face jn of the coarser neighbour's face
                 const int4 grp = aux[-kNbrAuxBase - nbr[6 * nb + (d ^ 1)]];
-                const int j = (grp.x == e) ? 0 : ((grp.y == e) ? 1 : ((grp.z == e) ? 2 : 3));
-                o1 = (j & 1) ? R(0.25) : R(-0.25);
-                o2 = (j & 2) ? R(0.25) : R(-0.25);
+                jn = (grp.x == e) ? 0 : ((grp.y == e) ? 1 : ((grp.z == e) ? 2 : 3));
+                o1 = (jn & 1) ? R(0.25) : R(-0.25);
+                o2 = (jn & 2) ? R(0.25) : R(-0.25);
                 m = R(0.5);
             }
             const R wj = h * h * R(1.0 / 6.0);
@@ -75,7 +78,7 @@ __device__ __forceinline__ void face_points(const int *__restrict__ nbr, const i
                 R ps[3], pn[3];
                 face_point<R>(d, k, R(0), R(0), R(1), ps);
                 face_point<R>(d ^ 1, k, o1, o2, m, pn);
-                fn(d, nb, ps, pn, wj, sign);
+                fn(d, nb, ps, pn, wj, sign, k, 0, jn);
             }
         } else if (nb > -kNbrAuxBase) {
             const R wj = h * h * R(1.0 / 6.0);
@@ -94,7 +97,7 @@ __device__ __forceinline__ void face_points(const int *__restrict__ nbr, const i
                     R ps[3], pn[3];
                     face_point<R>(d, k, o1, o2, R(0.5), ps);
                     face_point<R>(d ^ 1, k, R(0), R(0), R(1), pn);
-                    fn(d, q4[j], ps, pn, wj, sign);
+                    fn(d, q4[j], ps, pn, wj, sign, k, j, 0);
                 }
             }
         }

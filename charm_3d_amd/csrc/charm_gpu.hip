@@ -45,6 +45,11 @@ struct charm_gpu_ctx {
     void *d_lift;             // [24][28][n] BR2: the (sub)face liftings
     double *d_dcoef;          // [nc][n] diffusion coefficients
     double *d_chem;           // [n] chemistry energy source
+    // point temperatures (allocated when ph and a species has a polynomial
+    // cp): [24 * 6][n] own face points, [8][n] volume points; temps_of is
+    // the state they hold, for the stage in progress (reset after it)
+    double *d_tpt, *d_tvol;
+    const void *temps_of;
     // multi-tree mesh (charm_gpu_upload_gmesh; gm = 1): the general kernels
     int gm;
     int64_t gnf;
@@ -84,11 +89,12 @@ static void free_mesh(charm_gpu_ctx_t *c)
                   (void **) &c->d_nbr, (void **) &c->d_slots, &c->B[0], &c->B[1], &c->B[2],
                   (void **) &c->d_stage64, (void **) &c->d_partial, (void **) &c->d_q,
                   &c->d_auxf, &c->d_vi, &c->d_auxo, &c->d_lift, (void **) &c->d_dcoef, (void **) &c->d_chem,
+                  (void **) &c->d_tpt, (void **) &c->d_tvol,
                   &c->d_glg, &c->d_gfg, &c->d_gfq, &c->d_gmb, (void **) &c->d_gfn, (void **) &c->d_gfe,
                   (void **) &c->d_gff, (void **) &c->d_glnb};
     for (size_t i = 0; i < sizeof(p) / sizeof(p[0]); i++)
         if (*p[i]) { (void) hipFree(*p[i]); *p[i] = NULL; }
-    c->gm = 0; c->gnf = 0; c->gmb_of = NULL;
+    c->gm = 0; c->gnf = 0; c->gmb_of = NULL; c->temps_of = NULL;
     free(c->hq);
     c->hq = NULL;
     c->n = 0;
@@ -318,6 +324,10 @@ extern "C" int charm_gpu_upload_mesh(charm_gpu_ctx_t *c, const charm_mesh_t *m)
         }
         DMALLOC(c->d_dcoef, (size_t) m->n * c->nc * sizeof(double));
         DMALLOC(c->d_chem, (size_t) m->n * sizeof(double));
+        if (c->P.newton_t) {
+            DMALLOC(c->d_tpt, (size_t) m->n * kLiftSlots * 6 * sizeof(double));
+            DMALLOC(c->d_tvol, (size_t) m->n * 8 * sizeof(double));
+        }
     }
     c->hq = (int32_t *) malloc((size_t) (m->n ? m->n : 1) * 4 * sizeof(int32_t));
     if (!c->hq) { charm_set_error("upload_mesh: out of memory"); free_mesh(c); return CHARM_ERR_MEM; }
@@ -465,6 +475,7 @@ static StageArgs stage_args(charm_gpu_ctx_t *c, const void *U, const void *U0, v
     A.mode = mode; A.combine = combine; A.dt = dt;
     A.visc = (c->ph && c->cfg.use_visc) ? c->d_vi : NULL;
     A.prm = c->d_P;
+    A.tpt = NULL; A.tvol = NULL;
     return A;
 }
 
@@ -497,9 +508,23 @@ static NsArgs ns_args(charm_gpu_ctx_t *c, const void *U, void *out, double dt)
     A.dcoef = (c->cfg.use_diff && c->nc >= 2) ? c->d_dcoef : NULL;
     A.aux_in = c->d_auxf; A.aux_out = c->d_auxf; A.visc_out = c->d_vi; A.aux_own = c->d_auxo;
     A.lift = c->d_lift;
+    const bool fresh = c->d_tpt && c->temps_of == U;
+    A.tpt = fresh ? c->d_tpt : NULL; A.tvol = fresh ? c->d_tvol : NULL;
     A.n = c->n; A.dt = dt;
     A.prm = c->d_P;
     return A;
+}
+
+// the point temperatures of the state U (launch code 6), for the viscous
+// passes and the stage that read U next; a no-op without a polynomial cp
+static hipError_t ns_temps(charm_gpu_ctx_t *c, const void *U)
+{
+    if (!c->d_tpt || c->temps_of == U) return hipSuccess;
+    NsArgs A = ns_args(c, U, NULL, 0.0);
+    A.tpt = c->d_tpt; A.tvol = c->d_tvol;
+    hipError_t e = launch_ns(c->sprec, 6, c->nc, A, c->P, c->st);
+    c->temps_of = e == hipSuccess ? U : NULL;
+    return e;
 }
 
 // The viscous passes on the state A.U: diffusion coefficients (DIFFUSION and
@@ -532,6 +557,7 @@ static hipError_t ns_prepare(charm_gpu_ctx_t *c, const void *in, void *chem_out,
         *stage_in = chem_out;
     }
     if (c->cfg.use_visc) {
+        if ((e = ns_temps(c, *stage_in)) != hipSuccess) return e;
         NsArgs A = ns_args(c, *stage_in, NULL, dt);
         if ((e = ns_viscous(c, A)) != hipSuccess) return e;
     }
@@ -553,7 +579,20 @@ static GArgs gargs(charm_gpu_ctx_t *c, const void *U, const void *U0, void *out,
 
 static hipError_t stage_launch(charm_gpu_ctx_t *c, const StageArgs &A)
 {
-    if (!c->gm) return launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st);
+    if (!c->gm) {
+        if (!c->d_tpt) return launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st);
+        // PH = 1 with a polynomial cp: the stage reads the point temperatures
+        // of its state (computed here unless the viscous passes just did);
+        // they are valid for this stage only, as the buffers are rewritten
+        StageArgs B = A;
+        hipError_t e = ns_temps(c, A.U);
+        if (e == hipSuccess) {
+            B.tpt = c->d_tpt; B.tvol = c->d_tvol;
+            e = launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, B, c->P, c->st);
+        }
+        c->temps_of = NULL;
+        return e;
+    }
     GArgs G = gargs(c, A.U, A.U0, A.out, A.mode, A.combine, A.dt);
     // a stage that a limiter may follow (new state, no combine) also writes
     // the limiter's cell means of that state
@@ -578,6 +617,7 @@ extern "C" int charm_gpu_residual(charm_gpu_ctx_t *c, double *out, int64_t n)
     if (rc) return rc;
     if (c->ph && c->cfg.model == CHARM_MODEL_NS && c->cfg.use_visc) {
         // viscous passes on the current state, without the chemistry
+        HIPCHK(ns_temps(c, c->B[0]));
         HIPCHK(ns_viscous(c, ns_args(c, c->B[0], NULL, 0.0)));
     }
     StageArgs A = stage_args(c, c->B[0], c->B[0], c->B[1], 0, 0, 0.0);

@@ -58,14 +58,28 @@ __device__ __forceinline__ double enthalpy(const DevParams &P, int i, double t)
     return fma(h, t, P.hH0[i]);
 }
 
+// tk: the point's temperature from the temperature pass, or null
 template <typename R, int NC>
-__device__ __forceinline__ void state_at(const DevParams &P, const R *u, const R ph[3], Prim<R, NC> &p)
+__device__ __forceinline__ void state_at(const DevParams &P, const R *u, const R ph[3], Prim<R, NC> &p,
+                                         const double *tk = nullptr)
 {
     constexpr int NV = 4 + NC;
     R t[NV];
     #pragma unroll
     for (int v = 0; v < NV; v++) t[v] = at<R>(&u[4 * v], ph[0], ph[1], ph[2]);
-    cons_to_prim<R, NC, 1>(P, t, p);
+    cons_to_prim<R, NC, 1>(P, t, p, tk);
+}
+
+// the temperature pass's value at point k of (sub)face slot 4 d + j of leaf
+// e, or at volume point g (null when the pass did not run on this state)
+__device__ __forceinline__ const double *tpt_at(const NsArgs &A, int slot, int k, int e)
+{
+    return A.tpt ? A.tpt + ((int64_t) (6 * slot + k) * A.n + e) : nullptr;
+}
+
+__device__ __forceinline__ const double *tvol_at(const NsArgs &A, int g, int e)
+{
+    return A.tvol ? A.tvol + ((int64_t) g * A.n + e) : nullptr;
 }
 
 // velocity of the P1 state at reference point ph, formed as cons_to_prim
@@ -282,7 +296,7 @@ void k_ns_grad(NsArgs A, DevParams)
     for (int g = 0; g < 8; g++) {
         const R ph[3] = {(g & 1) ? gph : -gph, (g & 2) ? gph : -gph, (g & 4) ? gph : -gph};
         Prim<R, NC> p;
-        state_at<R, NC>(P, u, ph, p);
+        state_at<R, NC>(P, u, ph, p, tvol_at(A, g, e));
         R mu, kt;
         transport<R, NC>(P, p, mu, kt);
         const R lp = R(4.0 / 3.0) * mu, lm = R(-2.0 / 3.0) * mu;
@@ -321,10 +335,11 @@ void k_ns_grad(NsArgs A, DevParams)
     };
     // one interior face point: the BR1 term into acc; with BR2 also the
     // lifting integrand [Q(avg) - Q(own)] sgn phi wJ into L
-    auto inner = [&](int d, const R *un, int nb, const R ps[3], const R pn[3], R wj, R sign, R (*L)[4]) {
+    auto inner = [&](int d, const R *un, int nb, const R ps[3], const R pn[3], R wj, R sign, R (*L)[4], int k,
+                     int js, int jn) {
         Prim<R, NC> pa, pb;
-        state_at<R, NC>(P, u, ps, pa);
-        state_at<R, NC>(P, un, pn, pb);
+        state_at<R, NC>(P, u, ps, pa, tpt_at(A, 4 * d + js, k, e));
+        state_at<R, NC>(P, un, pn, pb, tpt_at(A, 4 * (d ^ 1) + jn, k, nb));
         R mua, kta, mub, ktb;
         transport<R, NC>(P, pa, mua, kta);
         transport<R, NC>(P, pb, mub, ktb);
@@ -360,11 +375,11 @@ void k_ns_grad(NsArgs A, DevParams)
             }
         }
     };
-    auto bnd = [&](int d, int face, int, const R ps[3], R wj) {
+    auto bnd = [&](int d, int face, int k, const R ps[3], R wj) {
             const int a = d >> 1;
             const R sg = (d & 1) ? R(1) : R(-1);
             Prim<R, NC> pa, pb;
-            state_at<R, NC>(P, u, ps, pa);
+            state_at<R, NC>(P, u, ps, pa, tpt_at(A, 4 * d, k, e));
             bnd_state<R, NC, 1>(P, face, a, sg, pa, pb, -1);
             R mu, kt;
             transport<R, NC>(P, pa, mu, kt);
@@ -391,15 +406,15 @@ void k_ns_grad(NsArgs A, DevParams)
     R *__restrict__ lift = reinterpret_cast<R *>(A.lift);
     if constexpr (!BR2) {
         face_points<R>(A.nbr, A.aux, A.level, e, h,
-            [&](int d, int nb, const R ps[3], const R pn[3], R wj, R sign) {
+            [&](int d, int nb, const R ps[3], const R pn[3], R wj, R sign, int k, int js, int jn) {
                 R un[4 * NV];
                 load_coeffs<R, NV>(U, n, nb, un);
-                inner(d, un, nb, ps, pn, wj, sign, nullptr);
+                inner(d, un, nb, ps, pn, wj, sign, nullptr, k, js, jn);
             }, bnd);
     } else {
-        face_points<R>(A.nbr, A.aux, A.level, e, h, [&](int, int, const R *, const R *, R, R) {}, bnd);
+        face_points<R>(A.nbr, A.aux, A.level, e, h, [&](int, int, const R *, const R *, R, R, int, int, int) {}, bnd);
         face_subfaces<R>(A.nbr, A.aux, A.level, e, h,
-            [&](int d, int nb, R so1, R so2, R sm, R no1, R no2, R nm, R wj, R sign, int js, int) {
+            [&](int d, int nb, R so1, R so2, R sm, R no1, R no2, R nm, R wj, R sign, int js, int jn) {
                 R un[4 * NV];
                 load_coeffs<R, NV>(U, n, nb, un);
                 R L[7][4];
@@ -411,7 +426,7 @@ void k_ns_grad(NsArgs A, DevParams)
                     R ps[3], pn[3];
                     face_point<R>(d, k, so1, so2, sm, ps);
                     face_point<R>(d ^ 1, k, no1, no2, nm, pn);
-                    inner(d, un, nb, ps, pn, wj, sign, L);
+                    inner(d, un, nb, ps, pn, wj, sign, L, k, js, jn);
                 }
                 const int slot = 4 * d + js;
                 #pragma unroll
@@ -496,7 +511,7 @@ void k_ns_visc(NsArgs A, DevParams)
     for (int g = 0; g < 8; g++) {
         const R ph[3] = {(g & 1) ? gph : -gph, (g & 2) ? gph : -gph, (g & 4) ? gph : -gph};
         Prim<R, NC> p;
-        state_at<R, NC>(P, u, ph, p);
+        state_at<R, NC>(P, u, ph, p, tvol_at(A, g, e));
         R t[6];
         #pragma unroll
         for (int f = 0; f < 6; f++) t[f] = at<R>(&x[4 * f], ph[0], ph[1], ph[2]);
@@ -521,7 +536,7 @@ void k_ns_visc(NsArgs A, DevParams)
             for (int v = 0; v < 4; v++) acc[v][k] -= q[v] * phi[k] * wj;
     };
     face_points<R>(A.nbr, A.aux, A.level, e, h,
-        [&](int d, int nb, const R ps[3], const R pn[3], R wj, R sign) {
+        [&](int d, int nb, const R ps[3], const R pn[3], R wj, R sign, int k, int js, int jn) {
             if constexpr (BR2) return;               // the interior faces are done above
             R un[4 * NV], xn[36];
             load_coeffs<R, NV>(U, n, nb, un);
@@ -535,7 +550,7 @@ void k_ns_visc(NsArgs A, DevParams)
             for (int side = 0; side < 2; side++) {
                 const R *uu = side ? un : u, *xx = side ? xn : x, *pp = side ? pn : ps;
                 Prim<R, NC> p;
-                state_at<R, NC>(P, uu, pp, p);
+                state_at<R, NC>(P, uu, pp, p, side ? tpt_at(A, 4 * (d ^ 1) + jn, k, nb) : tpt_at(A, 4 * d + js, k, e));
                 R t[6];
                 #pragma unroll
                 for (int f = 0; f < 6; f++) { t[f] = at<R>(&xx[4 * f], pp[0], pp[1], pp[2]); ftau[f] += t[f]; }
@@ -643,10 +658,57 @@ void k_ns_visc(NsArgs A, DevParams)
         for (int k = 0; k < 4; k++) (vi + (4 * v + k) * n)[e] = acc[v][k];
 }
 
+// The point temperatures of the state U, one thread per leaf: the FP64 Newton
+// solve of _calc_t (src/charm_eos.c:12-54) at its 8 volume points and at its
+// own face points (the points face_points and face_subfaces visit: a
+// (sub)face slot 4 d + j, points k = 0..5), for the PH = 1 EOS closures of
+// the viscous passes and the stage kernel on the same state.  Each interior
+// face point is solved once per side instead of once per kernel and side.
+template <typename S, int NC>
+__global__ __launch_bounds__(256)
+void k_ns_temps(NsArgs A, DevParams)
+{
+    const DevParams &P = *A.prm;
+    using R = S;
+    constexpr int NV = 4 + NC;
+    const int64_t n = A.n;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const R *__restrict__ U = reinterpret_cast<const R *>(A.U);
+    R u[4 * NV];
+    load_coeffs<R, NV>(U, n, e, u);
+    auto temp = [&](const R ph[3]) {
+        R t[NV];
+        #pragma unroll
+        for (int v = 0; v < NV; v++) t[v] = at<R>(&u[4 * v], ph[0], ph[1], ph[2]);
+        return temperature_of<R, NC>(P, t);
+    };
+    const R gph = R(0.28867513459481288225);
+    for (int g = 0; g < 8; g++) {
+        const R ph[3] = {(g & 1) ? gph : -gph, (g & 2) ? gph : -gph, (g & 4) ? gph : -gph};
+        A.tvol[(int64_t) g * n + e] = temp(ph);
+    }
+    for (int d = 0; d < 6; d++) {
+        const int nb = A.nbr[6 * e + d];
+        // the big side of a hanging face: its four sub-faces' points
+        const int nsub = nb <= -kNbrAuxBase ? 4 : 1;
+        for (int j = 0; j < nsub; j++) {
+            const R o1 = nsub == 4 ? ((j & 1) ? R(0.25) : R(-0.25)) : R(0);
+            const R o2 = nsub == 4 ? ((j & 2) ? R(0.25) : R(-0.25)) : R(0);
+            const R m = nsub == 4 ? R(0.5) : R(1);
+            for (int k = 0; k < 6; k++) {
+                R ph[3];
+                face_point<R>(d, k, o1, o2, m, ph);
+                A.tpt[(int64_t) (6 * (4 * d + j) + k) * n + e] = temp(ph);
+            }
+        }
+    }
+}
+
 template <typename S>
 static hipError_t launch_ns_r(int what, int nc, const NsArgs &A, const DevParams &P, hipStream_t st)
 {
-    const unsigned bs = (what >= 2) ? 128u : 256u;
+    const unsigned bs = (what >= 2 && what != 6) ? 128u : 256u;
     const dim3 grid((unsigned) ((A.n + bs - 1) / bs)), block(bs);
 #define CHARM_NS_CASE(NCV)                                                                         \
     case NCV:                                                                                      \
@@ -655,6 +717,7 @@ static hipError_t launch_ns_r(int what, int nc, const NsArgs &A, const DevParams
         else if (what == 2) hipLaunchKernelGGL((k_ns_grad<S, NCV, false>), grid, block, 0, st, A, P); \
         else if (what == 4) hipLaunchKernelGGL((k_ns_grad<S, NCV, true>), grid, block, 0, st, A, P);  \
         else if (what == 3) hipLaunchKernelGGL((k_ns_visc<S, NCV, false>), grid, block, 0, st, A, P); \
+        else if (what == 6) hipLaunchKernelGGL((k_ns_temps<S, NCV>), grid, block, 0, st, A, P);      \
         else hipLaunchKernelGGL((k_ns_visc<S, NCV, true>), grid, block, 0, st, A, P);               \
         break;
     switch (nc) {
