@@ -658,12 +658,14 @@ void k_ns_visc(NsArgs A, DevParams)
         for (int k = 0; k < 4; k++) (vi + (4 * v + k) * n)[e] = acc[v][k];
 }
 
-// The point temperatures of the state U, one thread per leaf: the FP64 Newton
-// solve of _calc_t (src/charm_eos.c:12-54) at its 8 volume points and at its
-// own face points (the points face_points and face_subfaces visit: a
-// (sub)face slot 4 d + j, points k = 0..5), for the PH = 1 EOS closures of
-// the viscous passes and the stage kernel on the same state.  Each interior
-// face point is solved once per side instead of once per kernel and side.
+// The point temperatures of the state U: the FP64 Newton solve of _calc_t
+// (src/charm_eos.c:12-54) at each leaf's 8 volume points and at its own face
+// points (the points face_points and face_subfaces visit: a (sub)face slot
+// 4 d + j, points k = 0..5), for the PH = 1 EOS closures of the viscous
+// passes and the stage kernel on the same state.  Each interior face point
+// is solved once per side instead of once per kernel and side.  One thread
+// per leaf and part (blockIdx.y: face d = 0..5, or 6 the volume points), so
+// that the long dependent Newton chains of a leaf run in parallel.
 template <typename S, int NC>
 __global__ __launch_bounds__(256)
 void k_ns_temps(NsArgs A, DevParams)
@@ -674,6 +676,7 @@ void k_ns_temps(NsArgs A, DevParams)
     const int64_t n = A.n;
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
+    const int part = blockIdx.y;
     const R *__restrict__ U = reinterpret_cast<const R *>(A.U);
     R u[4 * NV];
     load_coeffs<R, NV>(U, n, e, u);
@@ -683,24 +686,26 @@ void k_ns_temps(NsArgs A, DevParams)
         for (int v = 0; v < NV; v++) t[v] = at<R>(&u[4 * v], ph[0], ph[1], ph[2]);
         return temperature_of<R, NC>(P, t);
     };
-    const R gph = R(0.28867513459481288225);
-    for (int g = 0; g < 8; g++) {
-        const R ph[3] = {(g & 1) ? gph : -gph, (g & 2) ? gph : -gph, (g & 4) ? gph : -gph};
-        A.tvol[(int64_t) g * n + e] = temp(ph);
+    if (part == 6) {
+        const R gph = R(0.28867513459481288225);
+        for (int g = 0; g < 8; g++) {
+            const R ph[3] = {(g & 1) ? gph : -gph, (g & 2) ? gph : -gph, (g & 4) ? gph : -gph};
+            A.tvol[(int64_t) g * n + e] = temp(ph);
+        }
+        return;
     }
-    for (int d = 0; d < 6; d++) {
-        const int nb = A.nbr[6 * e + d];
-        // the big side of a hanging face: its four sub-faces' points
-        const int nsub = nb <= -kNbrAuxBase ? 4 : 1;
-        for (int j = 0; j < nsub; j++) {
-            const R o1 = nsub == 4 ? ((j & 1) ? R(0.25) : R(-0.25)) : R(0);
-            const R o2 = nsub == 4 ? ((j & 2) ? R(0.25) : R(-0.25)) : R(0);
-            const R m = nsub == 4 ? R(0.5) : R(1);
-            for (int k = 0; k < 6; k++) {
-                R ph[3];
-                face_point<R>(d, k, o1, o2, m, ph);
-                A.tpt[(int64_t) (6 * (4 * d + j) + k) * n + e] = temp(ph);
-            }
+    const int d = part;
+    const int nb = A.nbr[6 * e + d];
+    // the big side of a hanging face: its four sub-faces' points
+    const int nsub = nb <= -kNbrAuxBase ? 4 : 1;
+    for (int j = 0; j < nsub; j++) {
+        const R o1 = nsub == 4 ? ((j & 1) ? R(0.25) : R(-0.25)) : R(0);
+        const R o2 = nsub == 4 ? ((j & 2) ? R(0.25) : R(-0.25)) : R(0);
+        const R m = nsub == 4 ? R(0.5) : R(1);
+        for (int k = 0; k < 6; k++) {
+            R ph[3];
+            face_point<R>(d, k, o1, o2, m, ph);
+            A.tpt[(int64_t) (6 * (4 * d + j) + k) * n + e] = temp(ph);
         }
     }
 }
@@ -709,7 +714,8 @@ template <typename S>
 static hipError_t launch_ns_r(int what, int nc, const NsArgs &A, const DevParams &P, hipStream_t st)
 {
     const unsigned bs = (what >= 2 && what != 6) ? 128u : 256u;
-    const dim3 grid((unsigned) ((A.n + bs - 1) / bs)), block(bs);
+    // the temperature pass runs a thread per leaf and part (6 faces, volume)
+    const dim3 grid((unsigned) ((A.n + bs - 1) / bs), what == 6 ? 7u : 1u), block(bs);
 #define CHARM_NS_CASE(NCV)                                                                         \
     case NCV:                                                                                      \
         if (what == 0) hipLaunchKernelGGL((k_ns_chem<S, NCV>), grid, block, 0, st, A, P);          \
