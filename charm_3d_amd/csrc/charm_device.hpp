@@ -42,6 +42,7 @@ struct DevParams {
     // Navier-Stokes / real-gas extensions, read by the PH = 1 kernels only
     int    newton_t;                   // comp 0 has a polynomial cp: T by FP64 Newton
     int    cp_poly[kMaxComp];          // cp(T) is the power series
+    int    cp_dt_rest;                 // a species with cp_poly 0 has a series of n_cp > 1
     int    n_cp[kMaxComp];
     double cp_c[kMaxComp][8];
     int    ml_type[kMaxComp], kp_type[kMaxComp];
@@ -154,7 +155,14 @@ __device__ __forceinline__ double comp_cp_dt(const DevParams &P, int i, double t
 
 // temperature of the polynomial-cp mixture by Newton from T = 1 K,
 // (cp(T) - R sum c_i / M_i) T = e, at most 100 steps, stop at dT^2 < 1e-12
-// (_calc_t, src/charm_eos.c:12-54); FP64 whatever the storage precision
+// (_calc_t, src/charm_eos.c:12-54); FP64 whatever the storage precision.
+// The mixture's cp(T) = sum_i c_i cp_i(T) is one power series whose
+// coefficients sum_i c_i a_ik are formed once, so that an iteration
+// evaluates one polynomial and its derivative (Horner) instead of two per
+// species; the iterates are the reference's up to rounding.  As
+// charm_comp_calc_cp_dt, the derivative takes every species' stored series:
+// a species whose cp is cp[0] (cp_poly false) adds its series' derivative
+// in a branch uniform over the launch (P.cp_dt_rest).
 template <int NC>
 __device__ __forceinline__ double newton_t(const DevParams &P, double e, const double *c)
 {
@@ -162,10 +170,29 @@ __device__ __forceinline__ double newton_t(const DevParams &P, double e, const d
     #pragma unroll
     for (int i = 0; i < NC; i++) m_ += c[i] * P.comp_minv[i];
     const double rm = kGasR * m_;
+    double a[8];
+    #pragma unroll
+    for (int k = 0; k < 8; k++) a[k] = 0.0;
+    #pragma unroll
+    for (int i = 0; i < NC; i++) {
+        if (P.cp_poly[i]) {
+            #pragma unroll
+            for (int k = 0; k < 8; k++) a[k] = fma(c[i], P.cp_c[i][k], a[k]);
+        } else {
+            a[0] = fma(c[i], P.comp_cp[i], a[0]);
+        }
+    }
     for (int j = 0; j < 100; j++) {
-        double cp = 0.0, cp_dt = 0.0;
+        double cp = a[7], cp_dt = 7.0 * a[7];
         #pragma unroll
-        for (int i = 0; i < NC; i++) { cp += c[i] * comp_cp(P, i, tt); cp_dt += c[i] * comp_cp_dt(P, i, tt); }
+        for (int k = 6; k >= 0; k--) cp = fma(cp, tt, a[k]);
+        #pragma unroll
+        for (int k = 6; k >= 1; k--) cp_dt = fma(cp_dt, tt, k * a[k]);
+        if (P.cp_dt_rest) {
+            #pragma unroll
+            for (int i = 0; i < NC; i++)
+                if (!P.cp_poly[i]) cp_dt += c[i] * comp_cp_dt(P, i, tt);
+        }
         const double ft = (cp - rm) * tt - e, ft_dt = (cp - rm) + cp_dt * tt;
         const double tt1 = tt - ft / ft_dt;
         const double d = tt1 - tt;

@@ -179,6 +179,9 @@ extern "C" int charm_gpu_create(const charm_config_t *cfg, charm_gpu_ctx_t **out
         any_poly |= P.cp_poly[i];
         P.n_cp[i] = q.n_cp;
         for (int k = 0; k < 8; k++) P.cp_c[i][k] = k < q.n_cp ? q.cp[k] : 0.0;
+        // its cp is cp[0], while dcp/dT is its series' derivative
+        // (src/charm_globals.c:628-649)
+        P.cp_dt_rest |= !P.cp_poly[i] && q.n_cp > 1;
         P.ml_type[i] = q.ml_type; P.kp_type[i] = q.kp_type;
         P.ml0[i] = q.ml0; P.kp0[i] = q.kp0; P.t0[i] = q.t0; P.ts[i] = q.ts; P.h0[i] = q.h0;
         P.t0inv[i] = q.t0 != 0.0 ? 1.0 / q.t0 : 0.0;
