@@ -268,7 +268,7 @@ void k_ns_diffusion(NsArgs A, DevParams)
 // other q components are zero), and the own-trace auxiliary field
 // tau_own = tau_h - sum of the leaf's liftings.
 template <typename S, int NC, bool BR2>
-__global__ __launch_bounds__(128)
+__global__ __launch_bounds__(128, (sizeof(S) == 4 ? 2 : 1))
 void k_ns_grad(NsArgs A, DevParams)
 {
     const DevParams &P = *A.prm;         // power series and BC records indexed at run time
@@ -404,37 +404,33 @@ void k_ns_grad(NsArgs A, DevParams)
     #pragma unroll
     for (int k = 0; k < 4; k++) ai[k] = cst<R>(P.ainv0[k], P.ainv0_f[k]) * ldexp(R(1), 3 * lev);
     R *__restrict__ lift = reinterpret_cast<R *>(A.lift);
-    if constexpr (!BR2) {
-        face_points<R>(A.nbr, A.aux, A.level, e, h,
-            [&](int d, int nb, const R ps[3], const R pn[3], R wj, R sign, int k, int js, int jn) {
-                R un[4 * NV];
-                load_coeffs<R, NV>(U, n, nb, un);
-                inner(d, un, nb, ps, pn, wj, sign, nullptr, k, js, jn);
-            }, bnd);
-    } else {
-        face_points<R>(A.nbr, A.aux, A.level, e, h, [&](int, int, const R *, const R *, R, R, int, int, int) {}, bnd);
-        face_subfaces<R>(A.nbr, A.aux, A.level, e, h,
-            [&](int d, int nb, R so1, R so2, R sm, R no1, R no2, R nm, R wj, R sign, int js, int jn) {
-                R un[4 * NV];
-                load_coeffs<R, NV>(U, n, nb, un);
-                R L[7][4];
+    // boundary points, then each interior (sub)face with the neighbour's
+    // coefficients loaded once for its six points
+    face_points<R>(A.nbr, A.aux, A.level, e, h, [&](int, int, const R *, const R *, R, R, int, int, int) {}, bnd);
+    face_subfaces<R>(A.nbr, A.aux, A.level, e, h,
+        [&](int d, int nb, R so1, R so2, R sm, R no1, R no2, R nm, R wj, R sign, int js, int jn) {
+            R un[4 * NV];
+            load_coeffs<R, NV>(U, n, nb, un);
+            R L[7][4];
+            #pragma unroll
+            for (int f = 0; f < 7; f++)
                 #pragma unroll
-                for (int f = 0; f < 7; f++)
-                    #pragma unroll
-                    for (int c = 0; c < 4; c++) L[f][c] = R(0);
-                for (int k = 0; k < 6; k++) {
-                    R ps[3], pn[3];
-                    face_point<R>(d, k, so1, so2, sm, ps);
-                    face_point<R>(d ^ 1, k, no1, no2, nm, pn);
-                    inner(d, un, nb, ps, pn, wj, sign, L, k, js, jn);
-                }
+                for (int c = 0; c < 4; c++) L[f][c] = R(0);
+            for (int k = 0; k < 6; k++) {
+                R ps[3], pn[3];
+                face_point<R>(d, k, so1, so2, sm, ps);
+                face_point<R>(d ^ 1, k, no1, no2, nm, pn);
+                inner(d, un, nb, ps, pn, wj, sign, L, k, js, jn);
+            }
+            if constexpr (BR2) {
                 const int slot = 4 * d + js;
                 #pragma unroll
                 for (int f = 0; f < 7; f++)
                     #pragma unroll
-                    for (int c = 0; c < 4; c++) (lift + ((size_t) kLiftFields * slot + 4 * f + c) * n)[e] = L[f][c] * ai[c];
-            });
-    }
+                    for (int c = 0; c < 4; c++)
+                        (lift + ((size_t) kLiftFields * slot + 4 * f + c) * n)[e] = L[f][c] * ai[c];
+            }
+        });
     R *__restrict__ aux = reinterpret_cast<R *>(A.aux_out);
     #pragma unroll
     for (int k = 0; k < 4; k++)
