@@ -193,6 +193,7 @@ int charm_forest_new_uniform(int level, const int periodic[3], charm_forest_t **
         f->key[i] = charm_morton(f->q[4*i], f->q[4*i+1], f->q[4*i+2]);
     }
     for (i = 0; i < 3; i++) f->periodic[i] = periodic ? (periodic[i] != 0) : 0;
+    f->balanced = 1;
     *out = f;
     return CHARM_OK;
 }
@@ -279,26 +280,42 @@ void charm_forest_set_user_pointer(charm_forest_t *f, void *p) { if (f) f->user 
 
 /* rebuild with per-leaf actions: +1 refine into 8 children, -1 coarsen (the
  * caller guarantees whole families, first child carries the action), 0 keep,
- * -2 drop (other members of a coarsened family) */
-static int forest_rebuild(charm_forest_t *f, const int8_t *act, int64_t **map_out, int64_t *n_map)
-{
-    int64_t i, n_new = 0, o = 0;
+ * -2 drop (other members of a coarsened family).  `fresh` (may be NULL)
+ * receives for every new leaf the balance checks it needs: child_code for a
+ * child of a refined leaf, FRESH_BOTH for a coarsened parent, 0 when kept.
+ * Host threads: a counting pass and a filling pass over the same contiguous
+ * ranges of old leaves. */
+#define FRESH_FINE 1        /* check the leaf as the fine side of a violation */
+#define FRESH_BOTH 2        /* ... and as the coarse side */
+typedef struct {
+    const charm_forest_t *f;
+    const int8_t *act;
     int32_t *q;
     uint64_t *key;
-    int64_t *map = NULL;
-    for (i = 0; i < f->n; i++)
-        n_new += act[i] == 1 ? 8 : (act[i] == -2 ? 0 : 1);
-    q = (int32_t *) malloc((size_t) n_new * 4 * sizeof(int32_t));
-    key = (uint64_t *) malloc((size_t) n_new * sizeof(uint64_t));
-    if (map_out) map = (int64_t *) malloc((size_t) n_new * sizeof(int64_t));
-    if (!q || !key || (map_out && !map)) {
-        free(q); free(key); free(map);
-        charm_set_error("forest: out of memory");
-        return CHARM_ERR_MEM;
-    }
-    for (i = 0; i < f->n; i++) {
+    int64_t *map;
+    int8_t *fresh;
+    int8_t child_code;
+    int64_t cnt[64], off[64];
+} rebuild_arg_t;
+
+static void rebuild_count(void *arg, int64_t lo, int64_t hi, int part)
+{
+    rebuild_arg_t *r = (rebuild_arg_t *) arg;
+    int64_t i, c = 0;
+    for (i = lo; i < hi; i++) c += r->act[i] == 1 ? 8 : (r->act[i] == -2 ? 0 : 1);
+    r->cnt[part] = c;
+}
+
+static void rebuild_fill(void *arg, int64_t lo, int64_t hi, int part)
+{
+    rebuild_arg_t *r = (rebuild_arg_t *) arg;
+    const charm_forest_t *f = r->f;
+    int32_t *q = r->q;
+    int64_t i, o = r->off[part];
+    for (i = lo; i < hi; i++) {
         const int32_t *p = &f->q[4*i];
-        if (act[i] == 1) {
+        const int8_t act = r->act[i];
+        if (act == 1) {
             int32_t h = (CHARM_ROOT_LEN >> p[3]) / 2;
             int c;
             for (c = 0; c < 8; c++) {
@@ -306,35 +323,89 @@ static int forest_rebuild(charm_forest_t *f, const int8_t *act, int64_t **map_ou
                 q[4*o+1] = p[1] + ((c >> 1) & 1) * h;
                 q[4*o+2] = p[2] + ((c >> 2) & 1) * h;
                 q[4*o+3] = p[3] + 1;
-                key[o] = charm_morton(q[4*o], q[4*o+1], q[4*o+2]);
-                if (map) map[o] = i;
+                r->key[o] = charm_morton(q[4*o], q[4*o+1], q[4*o+2]);
+                if (r->map) r->map[o] = i;
+                if (r->fresh) r->fresh[o] = r->child_code;
                 o++;
             }
-        } else if (act[i] == -1) {
+        } else if (act == -1) {
             int32_t h = (CHARM_ROOT_LEN >> (p[3] - 1));
             q[4*o+0] = (p[0] / h) * h;
             q[4*o+1] = (p[1] / h) * h;
             q[4*o+2] = (p[2] / h) * h;
             q[4*o+3] = p[3] - 1;
-            key[o] = charm_morton(q[4*o], q[4*o+1], q[4*o+2]);
-            if (map) map[o] = -1 - i;
+            r->key[o] = charm_morton(q[4*o], q[4*o+1], q[4*o+2]);
+            if (r->map) r->map[o] = -1 - i;
+            if (r->fresh) r->fresh[o] = FRESH_BOTH;
             o++;
-        } else if (act[i] == 0) {
+        } else if (act == 0) {
             memcpy(&q[4*o], p, 4 * sizeof(int32_t));
-            key[o] = f->key[i];
-            if (map) map[o] = i;
+            r->key[o] = f->key[i];
+            if (r->map) r->map[o] = i;
+            if (r->fresh) r->fresh[o] = 0;
             o++;
         }
     }
+}
+
+static int forest_rebuild(charm_forest_t *f, const int8_t *act, int64_t **map_out, int64_t *n_map,
+                          int8_t **fresh_out, int8_t child_code)
+{
+    rebuild_arg_t r;
+    int64_t n_new = 0;
+    int p, parts = par_parts(f->n, 16384);
+    memset(&r, 0, sizeof(r));
+    r.f = f; r.act = act; r.child_code = child_code;
+    par_for(f->n, parts, rebuild_count, &r);
+    for (p = 0; p < parts; p++) { r.off[p] = n_new; n_new += r.cnt[p]; }
+    r.q = (int32_t *) malloc((size_t) (n_new ? n_new : 1) * 4 * sizeof(int32_t));
+    r.key = (uint64_t *) malloc((size_t) (n_new ? n_new : 1) * sizeof(uint64_t));
+    if (map_out) r.map = (int64_t *) malloc((size_t) (n_new ? n_new : 1) * sizeof(int64_t));
+    if (fresh_out) r.fresh = (int8_t *) malloc((size_t) (n_new ? n_new : 1));
+    if (!r.q || !r.key || (map_out && !r.map) || (fresh_out && !r.fresh)) {
+        free(r.q); free(r.key); free(r.map); free(r.fresh);
+        charm_set_error("forest: out of memory");
+        return CHARM_ERR_MEM;
+    }
+    par_for(f->n, parts, rebuild_fill, &r);
     free(f->q); free(f->key);
-    f->q = q; f->key = key; f->n = n_new;
-    if (map_out) { *map_out = map; *n_map = n_new; }
+    f->q = r.q; f->key = r.key; f->n = n_new;
+    if (map_out) { *map_out = r.map; *n_map = n_new; }
+    if (fresh_out) *fresh_out = r.fresh;
     return CHARM_OK;
 }
 
 /* 2:1 face balance by ripple refinement: a leaf at level l forces the leaf
- * across each of its faces to level >= l - 1 */
-typedef struct { const charm_forest_t *f; int8_t *act; int8_t found[64]; } bal_arg_t;
+ * across each of its faces to level >= l - 1.  A pass marks the coarse side
+ * of every violation for refinement.  Without `fresh` it scans every leaf
+ * from the fine side: the leaf containing the point just across each face
+ * at the leaf's anchor corner (inside the coarser neighbour's face).  With
+ * `fresh` it scans only the leaves a rebuild has just made (a violation of
+ * a forest that was balanced, or of one whose previous pass refined every
+ * coarse side, involves such a leaf): from the fine side as above, and for
+ * FRESH_BOTH leaves also from the coarse side, marking the leaf itself when
+ * a leaf across a face is two levels finer.  Such a leaf fills one of the
+ * four quarters of the face's neighbour region, and the leaf containing that
+ * quarter's centre point is then at least as fine; the four centres are
+ * checked.  A child made by the adaptation's refinement (FRESH_FINE) cannot
+ * be the coarse side: a leaf two levels finer than it was three levels finer
+ * than its balanced parent, or is itself a new child, one level finer.
+ * Coarsened parents and the children of a balance pass can (a parent's
+ * neighbour may have been refined in the same adaptation).  Both forms mark
+ * the same leaves. */
+typedef struct { const charm_forest_t *f; const int8_t *fresh; int8_t *act; int8_t found[64]; } bal_arg_t;
+
+/* the point (x) just across face (a, s) of the leaf at p with size len,
+ * wrapped on a periodic axis; 0 when it leaves a non-periodic domain */
+static int across(const charm_forest_t *f, const int32_t *p, int32_t len, int a, int s, int32_t x[3])
+{
+    x[a] = s ? p[a] + len : p[a] - 1;
+    if (x[a] < 0 || x[a] >= CHARM_ROOT_LEN) {
+        if (!f->periodic[a]) return 0;
+        x[a] = (x[a] + CHARM_ROOT_LEN) % CHARM_ROOT_LEN;
+    }
+    return 1;
+}
 
 static void balance_scan(void *arg, int64_t lo, int64_t hi, int part)
 {
@@ -343,46 +414,75 @@ static void balance_scan(void *arg, int64_t lo, int64_t hi, int part)
     int64_t i;
     for (i = lo; i < hi; i++) {
         const int32_t *p = &f->q[4*i];
-        int32_t len = CHARM_ROOT_LEN >> p[3];
+        const int32_t len = CHARM_ROOT_LEN >> p[3];
         int a, s;
-        if (p[3] < 2) continue;
+        if (b->fresh && !b->fresh[i]) continue;
         for (a = 0; a < 3; a++)
             for (s = 0; s < 2; s++) {
                 int32_t x[3] = {p[0], p[1], p[2]};
                 int64_t nb;
-                x[a] += s ? len : -1;
-                if (x[a] < 0 || x[a] >= CHARM_ROOT_LEN) {
-                    if (!f->periodic[a]) continue;
-                    x[a] = (x[a] + CHARM_ROOT_LEN) % CHARM_ROOT_LEN;
+                if (p[3] >= 2 && across(f, p, len, a, s, x)) {
+                    nb = find_near(f, charm_morton(x[0], x[1], x[2]), i);
+                    if (nb >= 0 && f->q[4*nb+3] < p[3] - 1) {
+                        __atomic_store_n(&b->act[nb], (int8_t) 1, __ATOMIC_RELAXED);
+                        b->found[part] = 1;
+                    }
                 }
-                nb = find_near(f, charm_morton(x[0], x[1], x[2]), i);
-                if (nb >= 0 && f->q[4*nb+3] < p[3] - 1) {
-                    __atomic_store_n(&b->act[nb], (int8_t) 1, __ATOMIC_RELAXED);
-                    b->found[part] = 1;
+                if (b->fresh && b->fresh[i] == FRESH_BOTH && len >= 4) {
+                    const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
+                    int j;
+                    for (j = 0; j < 4; j++) {
+                        int32_t y[3] = {p[0], p[1], p[2]};
+                        if (!across(f, p, len, a, s, y)) break;
+                        y[t1] = p[t1] + ((j & 1) ? 3 * (len / 4) : len / 4);
+                        y[t2] = p[t2] + ((j & 2) ? 3 * (len / 4) : len / 4);
+                        nb = find_near(f, charm_morton(y[0], y[1], y[2]), i);
+                        if (nb >= 0 && f->q[4*nb+3] > p[3] + 1) {
+                            __atomic_store_n(&b->act[i], (int8_t) 1, __ATOMIC_RELAXED);
+                            b->found[part] = 1;
+                            break;
+                        }
+                    }
                 }
             }
     }
 }
 
-static int forest_balance_map(charm_forest_t *f, int64_t **map)
+typedef struct { int64_t *m2; const int64_t *m; } compose_arg_t;
+
+static void compose_maps(void *arg, int64_t lo, int64_t hi, int part)
+{
+    compose_arg_t *c = (compose_arg_t *) arg;
+    int64_t j;
+    (void) part;
+    for (j = lo; j < hi; j++) c->m2[j] = c->m[c->m2[j]];
+}
+
+/* fresh: the leaves the caller's rebuild made (taken over and freed), or
+ * NULL for a full first pass */
+static int forest_balance_map(charm_forest_t *f, int64_t **map, int8_t *fresh)
 {
     for (;;) {
         bal_arg_t b;
         int p, parts = par_parts(f->n, 4096), any = 0;
         memset(&b, 0, sizeof(b));
         b.f = f;
-        b.act = (int8_t *) calloc((size_t) f->n, 1);
-        if (!b.act) { charm_set_error("forest: out of memory"); return CHARM_ERR_MEM; }
+        b.fresh = fresh;
+        b.act = (int8_t *) calloc((size_t) (f->n ? f->n : 1), 1);
+        if (!b.act) { free(fresh); charm_set_error("forest: out of memory"); return CHARM_ERR_MEM; }
         par_for(f->n, parts, balance_scan, &b);
+        free(fresh);
+        fresh = NULL;
         for (p = 0; p < parts; p++) any |= b.found[p];
-        if (!any) { free(b.act); return CHARM_OK; }
+        if (!any) { free(b.act); f->balanced = 1; return CHARM_OK; }
         {
-            int64_t *m2 = NULL, nm2, j;
-            int rc = forest_rebuild(f, b.act, map ? &m2 : NULL, &nm2);
+            int64_t *m2 = NULL, nm2;
+            int rc = forest_rebuild(f, b.act, map ? &m2 : NULL, &nm2, &fresh, FRESH_BOTH);
             free(b.act);
             if (rc) return rc;
             if (map && *map) {           /* compose maps: new -> old original */
-                for (j = 0; j < nm2; j++) m2[j] = (*map)[m2[j]];
+                compose_arg_t c = {m2, *map};
+                par_for(nm2, par_parts(nm2, 16384), compose_maps, &c);
                 free(*map);
             }
             if (map) *map = m2;
@@ -393,14 +493,14 @@ static int forest_balance_map(charm_forest_t *f, int64_t **map)
 int charm_forest_balance(charm_forest_t *f)
 {
     if (!f) { charm_set_error("forest_balance: null forest"); return CHARM_ERR_ARG; }
-    return forest_balance_map(f, NULL);
+    return forest_balance_map(f, NULL, NULL);
 }
 
 int charm_forest_adapt(charm_forest_t *f, const int8_t *flags, int max_level, int min_level,
                        int64_t **map, int64_t *n_map)
 {
     int64_t i, nm = 0;
-    int8_t *act;
+    int8_t *act, *fresh = NULL;
     int64_t *m = NULL;
     int rc;
     if (!f || !flags) { charm_set_error("forest_adapt: bad arguments"); return CHARM_ERR_ARG; }
@@ -425,10 +525,16 @@ int charm_forest_adapt(charm_forest_t *f, const int8_t *flags, int max_level, in
             }
         }
     }
-    rc = forest_rebuild(f, act, map ? &m : NULL, &nm);
-    free(act);
-    if (rc) return rc;
-    rc = forest_balance_map(f, map ? &m : NULL);
+    {
+        const int was_balanced = f->balanced;
+        f->balanced = 0;
+        rc = forest_rebuild(f, act, map ? &m : NULL, &nm, &fresh, FRESH_FINE);
+        free(act);
+        if (rc) return rc;
+        /* the fresh-leaf passes need a forest that was balanced before */
+        if (!was_balanced) { free(fresh); fresh = NULL; }
+    }
+    rc = forest_balance_map(f, map ? &m : NULL, fresh);
     if (rc) { free(m); return rc; }
     if (map) { *map = m; *n_map = f->n; }
     return CHARM_OK;

@@ -24,6 +24,7 @@ struct charm_forest {
     int32_t  *q;            /* [n][4] x y z level, Morton order */
     uint64_t *key;          /* Morton key of each anchor */
     int       periodic[3];
+    int       balanced;     /* known 2:1 face balanced (uniform, or balanced since) */
     void     *user;
 };
 

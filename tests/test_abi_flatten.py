@@ -108,6 +108,57 @@ def test_balance_matches_bruteforce():
     assert np.array_equal(f.leaves(), MU.balance(raw))
 
 
+def flags_applied(leaves, flags, max_level, min_level):
+    """refine (flag > 0 below max_level), then coarsen every complete family
+    of eight leaves flagged < 0 above min_level (charm_forest_adapt's rules,
+    src/charm_amr.c:601-606), by sets"""
+    fl = {tuple(int(v) for v in q): int(g) for q, g in zip(leaves, flags)}
+
+    def parent(q):
+        h = MU.ROOT >> (q[3] - 1)
+        return (q[0] // h * h, q[1] // h * h, q[2] // h * h, q[3] - 1)
+
+    fams = {}
+    for q in fl:
+        if q[3] > 0:
+            fams.setdefault(parent(q), []).append(q)
+    gone = {p for p, kids in fams.items()
+            if len(kids) == 8 and kids[0][3] > min_level and all(fl[k] < 0 for k in kids)}
+    out = list(gone)
+    for q, g in fl.items():
+        if g > 0 and q[3] < max_level:
+            out.extend(MU._children(q))
+        elif not (q[3] > 0 and parent(q) in gone):
+            out.append(q)
+    return MU.sort_morton(np.array(out))
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_adapt_rounds_match_bruteforce_balance(periodic):
+    """Rounds of mixed random refine and coarsen flags on a forest that the
+    adaptation keeps balanced (its balance passes visit only the leaves each
+    rebuild made) give the leaves of flags -> brute-force balance, and the
+    same leaves and old -> new map as the same flags on a forest rebuilt from
+    the same leaves (whose first balance pass visits every leaf)"""
+    rng = np.random.default_rng(7 + periodic)
+    per = (periodic,) * 3
+    f = CA.Forest.uniform(2, per)
+    coarsened = refined = 0
+    for _ in range(6):
+        lv = f.leaves()
+        flags = rng.choice(np.array([-1, 0, 1], dtype=np.int8), size=len(lv), p=[0.55, 0.3, 0.15])
+        slow = CA.Forest.from_leaves(lv, per)
+        m_fast = f.adapt(flags, 5, 1)
+        m_slow = slow.adapt(flags, 5, 1)
+        got = f.leaves()
+        np.testing.assert_array_equal(got, slow.leaves())
+        np.testing.assert_array_equal(m_fast, m_slow)
+        np.testing.assert_array_equal(got, MU.balance(flags_applied(lv, flags, 5, 1), per))
+        coarsened += int((m_fast < 0).sum())
+        refined += int(len(got) > len(lv))
+    assert coarsened > 0 and refined > 0
+
+
 def test_neighbor_table_covers_every_face():
     leaves = MU.random_adapted(1, 3, seed=31)
     mesh = CA.Forest.from_leaves(leaves).flatten()
