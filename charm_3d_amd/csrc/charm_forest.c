@@ -650,42 +650,106 @@ static int mesh_faces(const charm_forest_t *f, charm_mesh_t *m)
     return CHARM_OK;
 }
 
-static int mesh_neighbors(charm_mesh_t *m)
+/* Neighbour table from the face list, on host threads: (1) every face
+ * writes the slots it fills (each slot has one writer; the four faces of a
+ * split face mark the big side's slot with the same sentinel); (2) the
+ * split slots are numbered as hanging groups in (leaf, direction) order;
+ * (3) every hanging face writes its small leaf into its group. */
+#define NBR_HANG_MARK (INT32_MIN + 1)
+
+typedef struct {
+    charm_mesh_t *m;
+    int64_t cnt[64], off[64];
+    int bad[64];
+} nbr_arg_t;
+
+static void nbr_slots(void *arg, int64_t lo, int64_t hi, int part)
 {
-    int64_t i, naux = 0, cap = 64;
-    m->nbr = (int32_t *) malloc((size_t) m->n * 6 * sizeof(int32_t));
-    m->aux = (int32_t *) malloc((size_t) cap * 4 * sizeof(int32_t));
-    if (!m->nbr || !m->aux) { charm_set_error("flatten: out of memory"); return CHARM_ERR_MEM; }
-    for (i = 0; i < m->n * 6; i++) m->nbr[i] = INT32_MIN;
-    for (i = 0; i < m->nf; i++) {
-        int32_t e0 = m->f4[4*i], e1 = m->f4[4*i+1], info = m->f4[4*i+2];
-        int d0 = info & 7, kind = (info >> 3) & 3, sub = (info >> 5) & 3;
-        int32_t big = -1, small = -1, bdir = 0;
+    charm_mesh_t *m = ((nbr_arg_t *) arg)->m;
+    int64_t i;
+    (void) part;
+    for (i = lo; i < hi; i++) {
+        const int32_t e0 = m->f4[4*i], e1 = m->f4[4*i+1], info = m->f4[4*i+2];
+        const int d0 = info & 7, kind = (info >> 3) & 3;
         switch (kind) {
         case CHARM_FACE_BND:  m->nbr[6*e0 + d0] = e1; break;
         case CHARM_FACE_CONF: m->nbr[6*e0 + d0] = e1; m->nbr[6*e1 + (d0 ^ 1)] = e0; break;
-        case CHARM_FACE_HANG0: small = e0; big = e1; bdir = d0 ^ 1; m->nbr[6*e0 + d0] = e1; break;
-        case CHARM_FACE_HANG1: small = e1; big = e0; bdir = d0; m->nbr[6*e1 + (d0 ^ 1)] = e0; break;
-        }
-        if (big >= 0) {
-            int32_t *slot = &m->nbr[6*big + bdir];
-            if (*slot == INT32_MIN) {
-                if (naux == cap) {
-                    int32_t *t;
-                    cap *= 2;
-                    t = (int32_t *) realloc(m->aux, (size_t) cap * 4 * sizeof(int32_t));
-                    if (!t) { charm_set_error("flatten: out of memory"); return CHARM_ERR_MEM; }
-                    m->aux = t;
-                }
-                *slot = -CHARM_NBR_AUX_BASE - (int32_t) naux;
-                m->aux[4*naux] = m->aux[4*naux+1] = m->aux[4*naux+2] = m->aux[4*naux+3] = -1;
-                naux++;
-            }
-            m->aux[4 * (-CHARM_NBR_AUX_BASE - *slot) + sub] = small;
+        case CHARM_FACE_HANG0:
+            m->nbr[6*e0 + d0] = e1;
+            __atomic_store_n(&m->nbr[6*e1 + (d0 ^ 1)], NBR_HANG_MARK, __ATOMIC_RELAXED);
+            break;
+        case CHARM_FACE_HANG1:
+            m->nbr[6*e1 + (d0 ^ 1)] = e0;
+            __atomic_store_n(&m->nbr[6*e0 + d0], NBR_HANG_MARK, __ATOMIC_RELAXED);
+            break;
         }
     }
-    for (i = 0; i < m->n * 6; i++)
-        if (m->nbr[i] == INT32_MIN) { charm_set_error("flatten: element face %lld not covered", (long long) i); return CHARM_ERR_MESH; }
+}
+
+static void nbr_count(void *arg, int64_t lo, int64_t hi, int part)
+{
+    nbr_arg_t *r = (nbr_arg_t *) arg;
+    int64_t i, c = 0;
+    for (i = 6 * lo; i < 6 * hi; i++) c += r->m->nbr[i] == NBR_HANG_MARK;
+    r->cnt[part] = c;
+}
+
+static void nbr_number(void *arg, int64_t lo, int64_t hi, int part)
+{
+    nbr_arg_t *r = (nbr_arg_t *) arg;
+    charm_mesh_t *m = r->m;
+    int64_t i, g = r->off[part];
+    for (i = 6 * lo; i < 6 * hi; i++)
+        if (m->nbr[i] == NBR_HANG_MARK) {
+            m->aux[4*g] = m->aux[4*g+1] = m->aux[4*g+2] = m->aux[4*g+3] = -1;
+            m->nbr[i] = -CHARM_NBR_AUX_BASE - (int32_t) g++;
+        }
+}
+
+static void nbr_groups(void *arg, int64_t lo, int64_t hi, int part)
+{
+    charm_mesh_t *m = ((nbr_arg_t *) arg)->m;
+    int64_t i;
+    (void) part;
+    for (i = lo; i < hi; i++) {
+        const int32_t e0 = m->f4[4*i], e1 = m->f4[4*i+1], info = m->f4[4*i+2];
+        const int d0 = info & 7, kind = (info >> 3) & 3, sub = (info >> 5) & 3;
+        int32_t slot;
+        if (kind == CHARM_FACE_HANG0) slot = m->nbr[6*e1 + (d0 ^ 1)];
+        else if (kind == CHARM_FACE_HANG1) slot = m->nbr[6*e0 + d0];
+        else continue;
+        m->aux[4 * (-CHARM_NBR_AUX_BASE - slot) + sub] = kind == CHARM_FACE_HANG0 ? e0 : e1;
+    }
+}
+
+static void nbr_check(void *arg, int64_t lo, int64_t hi, int part)
+{
+    nbr_arg_t *r = (nbr_arg_t *) arg;
+    int64_t i;
+    for (i = 6 * lo; i < 6 * hi; i++)
+        if (r->m->nbr[i] == INT32_MIN) { r->bad[part] = 1; return; }
+}
+
+static int mesh_neighbors(charm_mesh_t *m)
+{
+    nbr_arg_t r;
+    int64_t i, naux = 0;
+    int p, fparts = par_parts(m->nf, 16384), lparts = par_parts(m->n, 8192);
+    memset(&r, 0, sizeof(r));
+    r.m = m;
+    m->nbr = (int32_t *) malloc((size_t) (m->n ? m->n : 1) * 6 * sizeof(int32_t));
+    if (!m->nbr) { charm_set_error("flatten: out of memory"); return CHARM_ERR_MEM; }
+    for (i = 0; i < m->n * 6; i++) m->nbr[i] = INT32_MIN;
+    par_for(m->nf, fparts, nbr_slots, &r);
+    par_for(m->n, lparts, nbr_count, &r);
+    for (p = 0; p < lparts; p++) { r.off[p] = naux; naux += r.cnt[p]; }
+    m->aux = (int32_t *) malloc((size_t) (naux ? naux : 1) * 4 * sizeof(int32_t));
+    if (!m->aux) { charm_set_error("flatten: out of memory"); return CHARM_ERR_MEM; }
+    par_for(m->n, lparts, nbr_number, &r);
+    par_for(m->nf, fparts, nbr_groups, &r);
+    par_for(m->n, lparts, nbr_check, &r);
+    for (p = 0; p < lparts; p++)
+        if (r.bad[p]) { charm_set_error("flatten: an element face is not covered"); return CHARM_ERR_MESH; }
     for (i = 0; i < naux * 4; i++)
         if (m->aux[i] < 0) { charm_set_error("flatten: hanging face group %lld incomplete", (long long) (i / 4)); return CHARM_ERR_MESH; }
     m->naux = naux;
@@ -870,31 +934,74 @@ static void chunk_merge(void *arg, int64_t lo, int64_t hi, int part)
     }
 }
 
+typedef struct { const charm_mesh_t *m; int32_t *inc_ptr, *inc, *fill; } inc_arg_t;
+
+static void inc_count(void *arg, int64_t lo, int64_t hi, int part)
+{
+    inc_arg_t *I = (inc_arg_t *) arg;
+    int64_t i;
+    (void) part;
+    for (i = lo; i < hi; i++) {
+        __atomic_fetch_add(&I->inc_ptr[I->m->f4[4*i] + 1], 1, __ATOMIC_RELAXED);
+        if (I->m->f4[4*i+1] >= 0) __atomic_fetch_add(&I->inc_ptr[I->m->f4[4*i+1] + 1], 1, __ATOMIC_RELAXED);
+    }
+}
+
+static void inc_fill(void *arg, int64_t lo, int64_t hi, int part)
+{
+    inc_arg_t *I = (inc_arg_t *) arg;
+    int64_t i;
+    (void) part;
+    for (i = lo; i < hi; i++) {
+        I->inc[__atomic_fetch_add(&I->fill[I->m->f4[4*i]], 1, __ATOMIC_RELAXED)] = (int32_t) i;
+        if (I->m->f4[4*i+1] >= 0)
+            I->inc[__atomic_fetch_add(&I->fill[I->m->f4[4*i+1]], 1, __ATOMIC_RELAXED)] = (int32_t) i;
+    }
+}
+
+/* insertion sort of each leaf's incident faces (at most a few dozen) */
+static void inc_sort(void *arg, int64_t lo, int64_t hi, int part)
+{
+    inc_arg_t *I = (inc_arg_t *) arg;
+    int64_t e;
+    (void) part;
+    for (e = lo; e < hi; e++) {
+        int32_t *a = I->inc + I->inc_ptr[e];
+        const int32_t k = I->inc_ptr[e+1] - I->inc_ptr[e];
+        for (int32_t j = 1; j < k; j++) {
+            const int32_t v = a[j];
+            int32_t t = j - 1;
+            while (t >= 0 && a[t] > v) { a[t+1] = a[t]; t--; }
+            a[t+1] = v;
+        }
+    }
+}
+
 static int mesh_chunks(charm_mesh_t *m, int ch_e, int ch_f)
 {
     const int64_t n = m->n, nf = m->nf, nblk = (n + CHUNK_BLOCK - 1) / CHUNK_BLOCK;
     int32_t *inc_ptr = NULL, *inc = NULL, *fill = NULL;
-    int64_t i, e, b, *ch_off = NULL, *cf_off = NULL;
+    int64_t e, b, *ch_off = NULL, *cf_off = NULL;
     chunk_job_t J;
     int rc = CHARM_ERR_MEM;
     memset(&J, 0, sizeof(J));
 
     /* leaf -> incident faces, in face order (a face joining a leaf to
-     * itself appears twice, adjacent) */
+     * itself appears twice, adjacent): counted and filled by host threads
+     * over the faces, then each leaf's short list sorted */
     inc_ptr = (int32_t *) calloc((size_t) n + 1, sizeof(int32_t));
     if (!inc_ptr) goto done;
-    for (i = 0; i < nf; i++) {
-        inc_ptr[m->f4[4*i] + 1]++;
-        if (m->f4[4*i+1] >= 0) inc_ptr[m->f4[4*i+1] + 1]++;
-    }
-    for (e = 0; e < n; e++) inc_ptr[e+1] += inc_ptr[e];
-    inc = (int32_t *) malloc((size_t) (inc_ptr[n] + 1) * sizeof(int32_t));
-    fill = (int32_t *) malloc((size_t) (n + 1) * sizeof(int32_t));
-    if (!inc || !fill) goto done;
-    memcpy(fill, inc_ptr, (size_t) n * sizeof(int32_t));
-    for (i = 0; i < nf; i++) {
-        inc[fill[m->f4[4*i]]++] = (int32_t) i;
-        if (m->f4[4*i+1] >= 0) inc[fill[m->f4[4*i+1]]++] = (int32_t) i;
+    {
+        inc_arg_t I = {m, inc_ptr, NULL, NULL};
+        par_for(nf, par_parts(nf, 16384), inc_count, &I);
+        for (e = 0; e < n; e++) inc_ptr[e+1] += inc_ptr[e];
+        inc = (int32_t *) malloc((size_t) (inc_ptr[n] + 1) * sizeof(int32_t));
+        fill = (int32_t *) malloc((size_t) (n + 1) * sizeof(int32_t));
+        if (!inc || !fill) goto done;
+        memcpy(fill, inc_ptr, (size_t) n * sizeof(int32_t));
+        I.inc = inc; I.fill = fill;
+        par_for(nf, par_parts(nf, 16384), inc_fill, &I);
+        par_for(n, par_parts(n, 8192), inc_sort, &I);
     }
 
     m->slot_ptr = (int32_t *) malloc((size_t) (n + 1) * sizeof(int32_t));
