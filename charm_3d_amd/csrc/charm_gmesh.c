@@ -37,6 +37,7 @@
 
 #define GM_MAXLEVEL 19
 #define GM_ROOT ((int64_t) 1 << GM_MAXLEVEL)
+#define GM_FACE_BLOCK 8192   /* side-0 leaves per face group block */
 
 /* corners of each face in z-order (src/charm_geom.c:182-188) */
 static const int gm_ftv[6][4] = {{0, 2, 4, 6}, {1, 3, 5, 7}, {0, 1, 4, 5}, {2, 3, 6, 7}, {0, 1, 2, 3}, {4, 5, 6, 7}};
@@ -220,6 +221,7 @@ int charm_gmesh_build(int64_t n_vertices, const double *vertices, int64_t n_tree
     charm_gmesh_t *m;
     int64_t e, t, nper, side_n, nkey = 0, i, nf = 0;
     int64_t *pe = NULL;            /* [n][6]: partner leaf (>= 0), or -1 - slot */
+    int64_t *next = NULL;          /* face group offsets */
     int8_t *pf = NULL;             /* [n][6]: partner face */
     gm_key_t *keys = NULL;
     double (*tvs)[8][3] = NULL;
@@ -355,16 +357,21 @@ int charm_gmesh_build(int64_t n_vertices, const double *vertices, int64_t n_tree
         pe[6 * b->leaf + b->face] = a->leaf; pf[6 * b->leaf + b->face] = (int8_t) a->face;
         i++;
     }
-    /* faces grouped by (side-0 local face, boundary or not), leaf order
-     * inside a group: the face pass's threads of a wave then read and write
-     * the slots of consecutive leaves on one local face, and take one branch */
-    int64_t next[12] = {0};
+    /* faces grouped by (block of GM_FACE_BLOCK side-0 leaves, side-0 local
+     * face, boundary or not), leaf order inside a group: the face pass's
+     * threads of a wave then read and write the slots of consecutive leaves
+     * on one local face and take one branch, and the six sweeps of a block
+     * find its leaves' states in L2 (one XCD walks a contiguous face range) */
+    const int64_t nblk = (m->n + GM_FACE_BLOCK - 1) / GM_FACE_BLOCK;
+    next = (int64_t *) calloc((size_t) nblk * 12, sizeof(int64_t));
+    if (!next) goto oom;
     for (e = 0; e < m->n; e++)
         for (int f = 0; f < 6; f++) {
             const int64_t p = pe[6 * e + f];
-            if (p < 0 || p > e || (p == e && pf[6 * e + f] > f)) next[2 * f + (p < 0)]++;
+            if (p < 0 || p > e || (p == e && pf[6 * e + f] > f))
+                next[12 * (e / GM_FACE_BLOCK) + 2 * f + (p < 0)]++;
         }
-    for (int b = 0; b < 12; b++) { const int64_t c = next[b]; next[b] = nf; nf += c; }
+    for (int64_t b = 0; b < nblk * 12; b++) { const int64_t c = next[b]; next[b] = nf; nf += c; }
     m->nf = nf;
     m->fg = (double *) malloc((size_t) nf * CHARM_GM_FACEQ * sizeof(double));
     m->fn = (double *) malloc((size_t) nf * 3 * sizeof(double));
@@ -381,7 +388,7 @@ int charm_gmesh_build(int64_t n_vertices, const double *vertices, int64_t n_tree
                 int pt, a;
                 double nn[3];
                 if (!(p < 0 || p > e || (p == e && pf[6 * e + f] > f))) continue;
-                fi = next[2 * f + (p < 0)]++;
+                fi = next[12 * (e / GM_FACE_BLOCK) + 2 * f + (p < 0)]++;
                 if (!have) {
                     gm_geom((const double (*)[3]) tvs[m->leaf[5 * e]], m->leaf + 5 * e + 1, &g);
                     have = 1;
@@ -414,14 +421,14 @@ int charm_gmesh_build(int64_t n_vertices, const double *vertices, int64_t n_tree
             if (e1 >= 0) m->lnb[(size_t) m->ff[nf + fi] * m->n + e1] = e0;
         }
     }
-    free(pe); free(pf); free(keys);
+    free(pe); free(pf); free(keys); free(next);
     m->tv = tvs;
     *out = m;
     return CHARM_OK;
 oom:
     charm_set_error("gmesh_build: out of memory");
 fail:
-    free(pe); free(pf); free(keys); free(tvs);
+    free(pe); free(pf); free(keys); free(tvs); free(next);
     charm_gmesh_destroy(m);
     return CHARM_ERR_MEM;
 }

@@ -103,6 +103,12 @@ __device__ __forceinline__ void prim_to_cons(const Prim<R, NC> &p, R *c)
     for (int i = 0; i < NC; i++) c[4 + i] = p.r * p.c[i];
 }
 
+// blocks rounded up to a multiple of the XCD count
+static inline int64_t xcd_grid(int64_t blocks, int n_xcd)
+{
+    return (blocks + n_xcd - 1) / n_xcd * n_xcd;
+}
+
 } // namespace
 
 template <typename R, int NC, int FLUX>
@@ -111,7 +117,7 @@ void k_gface(GArgs A, DevParams P)
 {
     constexpr int NV = 4 + NC;
     const int64_t nf = A.nf, n = A.n;
-    const int64_t f = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t f = (int64_t) xcd_block(A.n_xcd) * blockDim.x + threadIdx.x;
     if (f >= nf) return;
     const R *__restrict__ U = reinterpret_cast<const R *>(A.U);
     const R *__restrict__ FG = reinterpret_cast<const R *>(A.fg) + f;
@@ -269,7 +275,7 @@ void k_glimiter(GArgs A, DevParams P)
 {
     constexpr int NV = 4 + NC;
     const int64_t n = A.n, nf = A.nf;
-    const int64_t e = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t e = (int64_t) xcd_block(A.n_xcd) * blockDim.x + threadIdx.x;
     if (e >= n) return;
     const R *__restrict__ W = reinterpret_cast<const R *>(A.U);
     const R *__restrict__ LG = reinterpret_cast<const R *>(A.lg);
@@ -338,7 +344,9 @@ void k_glimiter(GArgs A, DevParams P)
 template <typename R, int NC>
 static hipError_t launch_gstage_r(int flux, const GArgs &A, const DevParams &P, hipStream_t st)
 {
-    const dim3 block(256), gf((unsigned) ((A.nf + 255) / 256)), ge((unsigned) ((A.n + 255) / 256));
+    // the face grid is a multiple of the XCD count (xcd_block's bijection);
+    // the padding blocks return
+    const dim3 block(256), gf((unsigned) xcd_grid((A.nf + 255) / 256, A.n_xcd)), ge((unsigned) ((A.n + 255) / 256));
     switch (flux) {
     case FLUX_LF: hipLaunchKernelGGL((k_gface<R, NC, FLUX_LF>), gf, block, 0, st, A, P); break;
     case FLUX_HLLC: hipLaunchKernelGGL((k_gface<R, NC, FLUX_HLLC>), gf, block, 0, st, A, P); break;
@@ -370,7 +378,7 @@ hipError_t launch_gstage(int prec, int nc, int flux, const GArgs &A, const DevPa
 hipError_t launch_glimiter(int prec, int nc, const GArgs &A, const DevParams &P, hipStream_t st)
 {
     if (A.n <= 0) return hipSuccess;
-    const dim3 block(256), ge((unsigned) ((A.n + 255) / 256));
+    const dim3 block(256), ge((unsigned) xcd_grid((A.n + 255) / 256, A.n_xcd));
 #define CHARM_G_CASE(NCV)                                                                              \
     case NCV:                                                                                          \
         if (prec) hipLaunchKernelGGL((k_glimiter<float, NCV>), ge, block, 0, st, A, P);               \

@@ -577,6 +577,7 @@ static GArgs gargs(charm_gpu_ctx_t *c, const void *U, const void *U0, void *out,
     G.fe = c->d_gfe; G.ff = c->d_gff; G.lnb = c->d_glnb;
     G.err = c->d_err; G.n = c->n; G.nf = c->gnf;
     G.mode = mode; G.combine = combine; G.dt = dt;
+    G.n_xcd = c->n_xcd;
     return G;
 }
 

@@ -98,6 +98,7 @@ struct GArgs {
     int *err;
     int64_t n, nf;
     int mode, combine;
+    int n_xcd;                // XCDs: the face pass and limiter give each a contiguous range
     double dt;
 };
 hipError_t launch_gstage(int prec, int nc, int flux, const GArgs &A, const DevParams &P, hipStream_t st);
