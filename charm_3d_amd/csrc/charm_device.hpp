@@ -10,6 +10,10 @@
 namespace charm {
 
 constexpr int kMaxComp = 8;
+// rows of the point pass's EOS closures per leaf: 24 (sub)face slots x 6
+// points, then the 8 volume points; planes m, cv, gamma of kMemoRows rows
+constexpr int kMemoVolRow = 24 * 6;
+constexpr int kMemoRows = kMemoVolRow + 8;
 constexpr double kGasR = 8.31446261815324;       // src/charm_eos.c:8
 constexpr double kEps = 1.e-12;                   // CHARM_EPS
 enum { FLUX_LF = 0, FLUX_HLLC = 1, FLUX_GODUNOV = 2, FLUX_CD = 3 };
@@ -306,12 +310,13 @@ __device__ __forceinline__ R gp_t2(int k)
 // charm_param_cons_to_prim + EOS flag 4, src/charm_globals.c:183-208 and
 // src/charm_eos.c:86-91: velocities and energy use the unfloored density,
 // pressure and sound speed the floored one.
-// tk (PH = 1 only, may be null): the temperature of this state, solved
-// earlier by temperature_of on the same trace (the EOS closure then skips
-// its Newton solve; the same value, computed once per point)
+// tk (PH = 1 only, may be null): this state's EOS closure from the point
+// pass, m, cv and gamma at tk[0], tk[qs], tk[2 qs], as this function's own
+// closure computes them from the same trace (the FP64 Newton temperature
+// and the mixture's cp at it: the same values, evaluated once per point)
 template <typename R, int NC, int PH = 0>
 __device__ __forceinline__ void cons_to_prim(const DevParams &P, const R *u, Prim<R, NC> &p,
-                                             const double *tk = nullptr)
+                                             const R *tk = nullptr, int64_t qs = 0)
 {
     R r = u[4];
     #pragma unroll
@@ -323,7 +328,7 @@ __device__ __forceinline__ void cons_to_prim(const DevParams &P, const R *u, Pri
     #pragma unroll
     for (int i = 0; i < NC; i++) p.c[i] = clamp01(u[4 + i] * ri);
     if constexpr (PH == 1) {
-        if (tk) eos_coeffs_at<R, NC>(P, p, *tk);
+        if (tk) { p.m = tk[0]; p.cv = tk[qs]; p.gam = tk[2 * qs]; }
         else eos_coeffs<R, NC, PH>(P, p);
     } else {
         eos_coeffs<R, NC, PH>(P, p);
@@ -334,26 +339,6 @@ __device__ __forceinline__ void cons_to_prim(const DevParams &P, const R *u, Pri
     p.p = r * p.e * gm1;
     // c^2 = gamma p / r = gamma (gamma - 1) e with the same (floored) r
     p.cz = dsqrt(p.gam * gm1 * p.e);
-}
-
-// The temperature the PH = 1 EOS closure of cons_to_prim takes for the
-// conserved point state u: the FP64 Newton solve of _calc_t on the same
-// internal energy and mass fractions (0 when no species has a polynomial cp,
-// as eos_coeffs passes then)
-template <typename R, int NC>
-__device__ __forceinline__ double temperature_of(const DevParams &P, const R *u)
-{
-    if (!P.newton_t) return 0.0;
-    R r = u[4];
-    #pragma unroll
-    for (int i = 1; i < NC; i++) r += u[4 + i];
-    const R ri = rcp(r);
-    const R pu = u[0] * ri, pv = u[1] * ri, pw = u[2] * ri;
-    const R e = u[3] * ri - R(0.5) * (pu * pu + pv * pv + pw * pw);
-    double c[NC];
-    #pragma unroll
-    for (int i = 0; i < NC; i++) c[i] = (double) clamp01(u[4 + i] * ri);
-    return newton_t<NC>(P, (double) e, c);
 }
 
 // Exterior state of a boundary face, src/charm_bnd_cond.c:21-93: the BC

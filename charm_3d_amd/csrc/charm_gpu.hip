@@ -45,10 +45,11 @@ struct charm_gpu_ctx {
     void *d_lift;             // [24][28][n] BR2: the (sub)face liftings
     double *d_dcoef;          // [nc][n] diffusion coefficients
     double *d_chem;           // [n] chemistry energy source
-    // point temperatures (allocated when ph and a species has a polynomial
-    // cp): [24 * 6][n] own face points, [8][n] volume points; temps_of is
-    // the state they hold, for the stage in progress (reset after it)
-    double *d_tpt, *d_tvol;
+    // point EOS closures (allocated when ph and a species has a polynomial
+    // cp): [3][kMemoRows][n] in the state's precision, planes m, cv, gamma,
+    // rows the own face points then the volume points; temps_of is the state
+    // they hold, for the stage in progress (reset after it)
+    void *d_tpt;
     const void *temps_of;
     // multi-tree mesh (charm_gpu_upload_gmesh; gm = 1): the general kernels
     int gm;
@@ -89,7 +90,7 @@ static void free_mesh(charm_gpu_ctx_t *c)
                   (void **) &c->d_nbr, (void **) &c->d_slots, &c->B[0], &c->B[1], &c->B[2],
                   (void **) &c->d_stage64, (void **) &c->d_partial, (void **) &c->d_q,
                   &c->d_auxf, &c->d_vi, &c->d_auxo, &c->d_lift, (void **) &c->d_dcoef, (void **) &c->d_chem,
-                  (void **) &c->d_tpt, (void **) &c->d_tvol,
+                  (void **) &c->d_tpt,
                   &c->d_glg, &c->d_gfg, &c->d_gfq, &c->d_gmb, (void **) &c->d_gfn, (void **) &c->d_gfe,
                   (void **) &c->d_gff, (void **) &c->d_glnb};
     for (size_t i = 0; i < sizeof(p) / sizeof(p[0]); i++)
@@ -328,8 +329,7 @@ extern "C" int charm_gpu_upload_mesh(charm_gpu_ctx_t *c, const charm_mesh_t *m)
         DMALLOC(c->d_dcoef, (size_t) m->n * c->nc * sizeof(double));
         DMALLOC(c->d_chem, (size_t) m->n * sizeof(double));
         if (c->P.newton_t) {
-            DMALLOC(c->d_tpt, (size_t) m->n * kLiftSlots * 6 * sizeof(double));
-            DMALLOC(c->d_tvol, (size_t) m->n * 8 * sizeof(double));
+            DMALLOC(c->d_tpt, (size_t) m->n * 3 * kMemoRows * c->esz);
         }
     }
     c->hq = (int32_t *) malloc((size_t) (m->n ? m->n : 1) * 4 * sizeof(int32_t));
@@ -478,7 +478,7 @@ static StageArgs stage_args(charm_gpu_ctx_t *c, const void *U, const void *U0, v
     A.mode = mode; A.combine = combine; A.dt = dt;
     A.visc = (c->ph && c->cfg.use_visc) ? c->d_vi : NULL;
     A.prm = c->d_P;
-    A.tpt = NULL; A.tvol = NULL;
+    A.tpt = NULL;
     return A;
 }
 
@@ -512,19 +512,19 @@ static NsArgs ns_args(charm_gpu_ctx_t *c, const void *U, void *out, double dt)
     A.aux_in = c->d_auxf; A.aux_out = c->d_auxf; A.visc_out = c->d_vi; A.aux_own = c->d_auxo;
     A.lift = c->d_lift;
     const bool fresh = c->d_tpt && c->temps_of == U;
-    A.tpt = fresh ? c->d_tpt : NULL; A.tvol = fresh ? c->d_tvol : NULL;
+    A.tpt = fresh ? c->d_tpt : NULL;
     A.n = c->n; A.dt = dt;
     A.prm = c->d_P;
     return A;
 }
 
-// the point temperatures of the state U (launch code 6), for the viscous
+// the point EOS closures of the state U (launch code 6), for the viscous
 // passes and the stage that read U next; a no-op without a polynomial cp
 static hipError_t ns_temps(charm_gpu_ctx_t *c, const void *U)
 {
     if (!c->d_tpt || c->temps_of == U) return hipSuccess;
     NsArgs A = ns_args(c, U, NULL, 0.0);
-    A.tpt = c->d_tpt; A.tvol = c->d_tvol;
+    A.tpt = c->d_tpt;
     hipError_t e = launch_ns(c->sprec, 6, c->nc, A, c->P, c->st);
     c->temps_of = e == hipSuccess ? U : NULL;
     return e;
@@ -585,13 +585,13 @@ static hipError_t stage_launch(charm_gpu_ctx_t *c, const StageArgs &A)
 {
     if (!c->gm) {
         if (!c->d_tpt) return launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, A, c->P, c->st);
-        // PH = 1 with a polynomial cp: the stage reads the point temperatures
+        // PH = 1 with a polynomial cp: the stage reads the point EOS closures
         // of its state (computed here unless the viscous passes just did);
         // they are valid for this stage only, as the buffers are rewritten
         StageArgs B = A;
         hipError_t e = ns_temps(c, A.U);
         if (e == hipSuccess) {
-            B.tpt = c->d_tpt; B.tvol = c->d_tvol;
+            B.tpt = c->d_tpt;
             e = launch_stage(c->prec, c->nc, c->cfg.flux, c->ph, B, c->P, c->st);
         }
         c->temps_of = NULL;

@@ -23,9 +23,9 @@ struct StageArgs {
     int combine;              // 0: none, 1: w/4 + 3 u0/4, 2: 2w/3 + u0/3
     double dt;
     const void *visc;         // [16][n] viscous integrals added before A^-1 (or null)
-    // the point temperatures of U (NsArgs::tpt / tvol) for the PH = 1
-    // variant's EOS closures, or null (each point solves its own)
-    const double *tpt, *tvol;
+    // the point EOS closures of U (NsArgs::tpt) for the PH = 1 variant, or
+    // null (each point evaluates its own)
+    const void *tpt;
     // the context's constants in device memory: the Navier-Stokes / real-gas
     // variant (PH = 1) reads them here, so that its runtime indices (power
     // series up to n_cp, the BC record of a lane's face) are loads rather
@@ -123,11 +123,12 @@ struct NsArgs {
     void *visc_out;           // [16][n] viscous integrals of ru rv rw re
     void *aux_own;            // [36][n] BR2: own-trace tau, q (written by code 4, read by code 5)
     void *lift;               // [24][kLiftFields][n] BR2: each (sub)face slot's lifting (codes 4, 5)
-    // point temperatures of the state U (code 6 writes them; the viscous
-    // kernels read them where non-null): [(4 d + j) * 6 + k][n] at point k of
-    // (sub)face slot 4 d + j of each leaf's own face points, and [g][n] at
-    // the 8 volume points
-    double *tpt, *tvol;
+    // point EOS closures of the state U, in its precision (code 6 writes
+    // them; the viscous kernels and the stage read them where non-null):
+    // planes m, cv, gamma of kMemoRows rows, row (4 d + j) * 6 + k at point
+    // k of (sub)face slot 4 d + j of each leaf's own face points, row
+    // kMemoVolRow + g at volume point g: [3][kMemoRows][n]
+    void *tpt;
     int64_t n;
     double dt;
     const DevParams *prm;     // the context's constants in device memory (read by every NS kernel)
@@ -210,7 +211,7 @@ hipError_t launch_flux_points(int prec, int nc, int flux, const FluxArgs &A, con
 hipError_t launch_flux_points_vjp(int nc, int flux, const FluxArgs &A, const DevParams &P, hipStream_t st);
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st);
 // what: 0 chemistry, 1 diffusion coefficients, 2 / 4 tau / q (BR1 / BR2),
-// 3 / 5 viscous integrals (BR1 / BR2), 6 point temperatures
+// 3 / 5 viscous integrals (BR1 / BR2), 6 point EOS closures
 hipError_t launch_ns(int prec, int what, int nc, const NsArgs &A, const DevParams &P, hipStream_t st);
 hipError_t launch_amr_indicator(int prec, int nc, int ph, const AmrArgs &A, const DevParams &P, hipStream_t st);
 hipError_t launch_amr_project(int prec, const ProjArgs &A, hipStream_t st);

@@ -58,28 +58,31 @@ __device__ __forceinline__ double enthalpy(const DevParams &P, int i, double t)
     return fma(h, t, P.hH0[i]);
 }
 
-// tk: the point's temperature from the temperature pass, or null
+// tk: the point's EOS closure from the point pass (planes kMemoRows * n
+// apart), or null
 template <typename R, int NC>
 __device__ __forceinline__ void state_at(const DevParams &P, const R *u, const R ph[3], Prim<R, NC> &p,
-                                         const double *tk = nullptr)
+                                         const R *tk = nullptr, int64_t qs = 0)
 {
     constexpr int NV = 4 + NC;
     R t[NV];
     #pragma unroll
     for (int v = 0; v < NV; v++) t[v] = at<R>(&u[4 * v], ph[0], ph[1], ph[2]);
-    cons_to_prim<R, NC, 1>(P, t, p, tk);
+    cons_to_prim<R, NC, 1>(P, t, p, tk, qs);
 }
 
-// the temperature pass's value at point k of (sub)face slot 4 d + j of leaf
-// e, or at volume point g (null when the pass did not run on this state)
-__device__ __forceinline__ const double *tpt_at(const NsArgs &A, int slot, int k, int e)
+// the point pass's row at point k of (sub)face slot 4 d + j of leaf e, or
+// at volume point g (null when the pass did not run on this state)
+template <typename R>
+__device__ __forceinline__ const R *tpt_at(const NsArgs &A, int slot, int k, int e)
 {
-    return A.tpt ? A.tpt + ((int64_t) (6 * slot + k) * A.n + e) : nullptr;
+    return A.tpt ? reinterpret_cast<const R *>(A.tpt) + ((int64_t) (6 * slot + k) * A.n + e) : nullptr;
 }
 
-__device__ __forceinline__ const double *tvol_at(const NsArgs &A, int g, int e)
+template <typename R>
+__device__ __forceinline__ const R *tvol_at(const NsArgs &A, int g, int e)
 {
-    return A.tvol ? A.tvol + ((int64_t) g * A.n + e) : nullptr;
+    return A.tpt ? reinterpret_cast<const R *>(A.tpt) + ((int64_t) (kMemoVolRow + g) * A.n + e) : nullptr;
 }
 
 // velocity of the P1 state at reference point ph, formed as cons_to_prim
@@ -296,7 +299,7 @@ void k_ns_grad(NsArgs A, DevParams)
     for (int g = 0; g < 8; g++) {
         const R ph[3] = {(g & 1) ? gph : -gph, (g & 2) ? gph : -gph, (g & 4) ? gph : -gph};
         Prim<R, NC> p;
-        state_at<R, NC>(P, u, ph, p, tvol_at(A, g, e));
+        state_at<R, NC>(P, u, ph, p, tvol_at<R>(A, g, e), kMemoRows * n);
         R mu, kt;
         transport<R, NC>(P, p, mu, kt);
         const R lp = R(4.0 / 3.0) * mu, lm = R(-2.0 / 3.0) * mu;
@@ -338,8 +341,8 @@ void k_ns_grad(NsArgs A, DevParams)
     auto inner = [&](int d, const R *un, int nb, const R ps[3], const R pn[3], R wj, R sign, R (*L)[4], int k,
                      int js, int jn) {
         Prim<R, NC> pa, pb;
-        state_at<R, NC>(P, u, ps, pa, tpt_at(A, 4 * d + js, k, e));
-        state_at<R, NC>(P, un, pn, pb, tpt_at(A, 4 * (d ^ 1) + jn, k, nb));
+        state_at<R, NC>(P, u, ps, pa, tpt_at<R>(A, 4 * d + js, k, e), kMemoRows * n);
+        state_at<R, NC>(P, un, pn, pb, tpt_at<R>(A, 4 * (d ^ 1) + jn, k, nb), kMemoRows * n);
         R mua, kta, mub, ktb;
         transport<R, NC>(P, pa, mua, kta);
         transport<R, NC>(P, pb, mub, ktb);
@@ -379,7 +382,7 @@ void k_ns_grad(NsArgs A, DevParams)
             const int a = d >> 1;
             const R sg = (d & 1) ? R(1) : R(-1);
             Prim<R, NC> pa, pb;
-            state_at<R, NC>(P, u, ps, pa, tpt_at(A, 4 * d, k, e));
+            state_at<R, NC>(P, u, ps, pa, tpt_at<R>(A, 4 * d, k, e), kMemoRows * n);
             bnd_state<R, NC, 1>(P, face, a, sg, pa, pb, -1);
             R mu, kt;
             transport<R, NC>(P, pa, mu, kt);
@@ -507,7 +510,7 @@ void k_ns_visc(NsArgs A, DevParams)
     for (int g = 0; g < 8; g++) {
         const R ph[3] = {(g & 1) ? gph : -gph, (g & 2) ? gph : -gph, (g & 4) ? gph : -gph};
         Prim<R, NC> p;
-        state_at<R, NC>(P, u, ph, p, tvol_at(A, g, e));
+        state_at<R, NC>(P, u, ph, p, tvol_at<R>(A, g, e), kMemoRows * n);
         R t[6];
         #pragma unroll
         for (int f = 0; f < 6; f++) t[f] = at<R>(&x[4 * f], ph[0], ph[1], ph[2]);
@@ -546,7 +549,8 @@ void k_ns_visc(NsArgs A, DevParams)
             for (int side = 0; side < 2; side++) {
                 const R *uu = side ? un : u, *xx = side ? xn : x, *pp = side ? pn : ps;
                 Prim<R, NC> p;
-                state_at<R, NC>(P, uu, pp, p, side ? tpt_at(A, 4 * (d ^ 1) + jn, k, nb) : tpt_at(A, 4 * d + js, k, e));
+                state_at<R, NC>(P, uu, pp, p, side ? tpt_at<R>(A, 4 * (d ^ 1) + jn, k, nb) : tpt_at<R>(A, 4 * d + js, k, e),
+                                 kMemoRows * n);
                 R t[6];
                 #pragma unroll
                 for (int f = 0; f < 6; f++) { t[f] = at<R>(&xx[4 * f], pp[0], pp[1], pp[2]); ftau[f] += t[f]; }
@@ -654,14 +658,16 @@ void k_ns_visc(NsArgs A, DevParams)
         for (int k = 0; k < 4; k++) (vi + (4 * v + k) * n)[e] = acc[v][k];
 }
 
-// The point temperatures of the state U: the FP64 Newton solve of _calc_t
-// (src/charm_eos.c:12-54) at each leaf's 8 volume points and at its own face
-// points (the points face_points and face_subfaces visit: a (sub)face slot
-// 4 d + j, points k = 0..5), for the PH = 1 EOS closures of the viscous
-// passes and the stage kernel on the same state.  Each interior face point
-// is solved once per side instead of once per kernel and side.  One thread
-// per leaf and part (blockIdx.y: face d = 0..5, or 6 the volume points), so
-// that the long dependent Newton chains of a leaf run in parallel.
+// The point EOS closures of the state U: at each leaf's 8 volume points and
+// at its own face points (the points face_points and face_subfaces visit: a
+// (sub)face slot 4 d + j, points k = 0..5), the FP64 Newton temperature of
+// _calc_t (src/charm_eos.c:12-54) and the mixture's m, cv and gamma at it
+// (src/charm_eos.c:98-148), stored in the state's precision for the PH = 1
+// closures of the viscous passes and the stage kernel on the same state.
+// Each interior face point is evaluated once per side instead of once per
+// kernel and side.  One thread per leaf and part (blockIdx.y: face d = 0..5,
+// or 6 the volume points), so that the long dependent Newton chains of a
+// leaf run in parallel.
 template <typename S, int NC>
 __global__ __launch_bounds__(256)
 void k_ns_temps(NsArgs A, DevParams)
@@ -676,17 +682,25 @@ void k_ns_temps(NsArgs A, DevParams)
     const R *__restrict__ U = reinterpret_cast<const R *>(A.U);
     R u[4 * NV];
     load_coeffs<R, NV>(U, n, e, u);
-    auto temp = [&](const R ph[3]) {
+    // the EOS closure (m, cv, gamma) of the point state, as cons_to_prim
+    // forms it (the FP64 Newton temperature, the mixture's cp at it), into
+    // row `row` of the three planes
+    R *__restrict__ W = reinterpret_cast<R *>(A.tpt);
+    const int64_t qs = (int64_t) kMemoRows * n;
+    auto closure = [&](const R ph[3], int row) {
         R t[NV];
         #pragma unroll
         for (int v = 0; v < NV; v++) t[v] = at<R>(&u[4 * v], ph[0], ph[1], ph[2]);
-        return temperature_of<R, NC>(P, t);
+        Prim<R, NC> p;
+        cons_to_prim<R, NC, 1>(P, t, p);
+        R *w = W + (int64_t) row * n + e;
+        w[0] = p.m; w[qs] = p.cv; w[2 * qs] = p.gam;
     };
     if (part == 6) {
         const R gph = R(0.28867513459481288225);
         for (int g = 0; g < 8; g++) {
             const R ph[3] = {(g & 1) ? gph : -gph, (g & 2) ? gph : -gph, (g & 4) ? gph : -gph};
-            A.tvol[(int64_t) g * n + e] = temp(ph);
+            closure(ph, kMemoVolRow + g);
         }
         return;
     }
@@ -701,7 +715,7 @@ void k_ns_temps(NsArgs A, DevParams)
         for (int k = 0; k < 6; k++) {
             R ph[3];
             face_point<R>(d, k, o1, o2, m, ph);
-            A.tpt[(int64_t) (6 * (4 * d + j) + k) * n + e] = temp(ph);
+            closure(ph, 6 * (4 * d + j) + k);
         }
     }
 }

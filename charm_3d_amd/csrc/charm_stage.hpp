@@ -79,13 +79,14 @@ __device__ __forceinline__ void to_xyz(int a, R n, R t1, R t2, R &x, R &y, R &z)
 // phi_t the tangential Taylor coordinates of the small (or side-0) face.
 // DEV: u0, u1 hold deviations from the chunk reference *rf (coefficient 0
 // minus the reference) and q is F - F(U_r) (charm_dev.hpp).
-// tl / tr (PH = 1, may be null): the temperature of side 0 / 1 at point k
-// from the point-temperature pass is tl[k * ts] / tr[k * ts] (global memory:
-// the point loop is not unrolled, so a local array would sit in scratch)
+// tl / tr (PH = 1, may be null): the EOS closure of side 0 / 1 at point k
+// from the point pass starts at tl[k * ts] / tr[k * ts], its planes
+// kMemoRows * ts apart (global memory: the point loop is not unrolled, so a
+// local array would sit in scratch)
 template <typename R, int NC, int FLUX, int AX, bool BND, int PH, bool DEV = false>
 __device__ __forceinline__ bool face_core(const DevParams &P, const R *u0, const R *u1, int info,
                                           int lown, R *out, int bnd_axis, const DevRef &rf,
-                                          const double *tl = nullptr, const double *tr = nullptr,
+                                          const R *tl = nullptr, const R *tr = nullptr,
                                           int64_t ts = 0)
 {
     constexpr int NV = 4 + NC;
@@ -146,14 +147,14 @@ __device__ __forceinline__ bool face_core(const DevParams &P, const R *u0, const
             ok &= flux_dev<FLUX>(dev_frame<AX>(dl, s), dev_frame<AX>(dr, s), dev_ref_frame<AX>(rf, s), q);
         } else {
         Prim<R, NC> pl, pr;
-        cons_to_prim<R, NC, PH>(P, t, pl, tl ? tl + k * ts : nullptr);
+        cons_to_prim<R, NC, PH>(P, t, pl, tl ? tl + k * ts : nullptr, kMemoRows * ts);
         if (BND) {
             bnd_state<R, NC, PH>(P, dir0, AX, s, pl, pr, bnd_axis);
         } else {
             #pragma unroll
             for (int v = 0; v < NV; v++)
                 t[v] = c1[v] + (o11 + m1 * g1) * u1[4*v+1+T1] + (o12 + m1 * g2) * u1[4*v+1+T2];
-            cons_to_prim<R, NC, PH>(P, t, pr, tr ? tr + k * ts : nullptr);
+            cons_to_prim<R, NC, PH>(P, t, pr, tr ? tr + k * ts : nullptr, kMemoRows * ts);
         }
         // face frame (s e_AX, e_T1, e_T2)
         Frame<R, NC> fl, fr;
@@ -188,8 +189,8 @@ __device__ __forceinline__ bool face_core(const DevParams &P, const R *u0, const
 // level sets the face size
 template <typename R, int NC, int FLUX, int AX, int PH, bool DEV = false>
 __device__ __forceinline__ bool face_moments(const DevParams &P, const R *u0, const R *u1, int info,
-                                             int lev0, R *out, const DevRef &rf, const double *tl = nullptr,
-                                             const double *tr = nullptr, int64_t ts = 0)
+                                             int lev0, R *out, const DevRef &rf, const R *tl = nullptr,
+                                             const R *tr = nullptr, int64_t ts = 0)
 {
     const int kind = (info >> 3) & 3;
     return face_core<R, NC, FLUX, AX, false, PH, DEV>(P, u0, u1, info, kind == FACE_HANG1 ? lev0 + 1 : lev0, out,
@@ -202,7 +203,7 @@ __device__ __forceinline__ bool face_moments(const DevParams &P, const R *u0, co
 // code path serves the three axes.
 template <typename R, int NC, int FLUX, int PH, bool DEV = false>
 __device__ __forceinline__ bool face_moments_bnd(const DevParams &P, const R *u, int info, int lev, R *out,
-                                                 const DevRef &rf, const double *tl = nullptr, int64_t ts = 0)
+                                                 const DevRef &rf, const R *tl = nullptr, int64_t ts = 0)
 {
     constexpr int NV = 4 + NC;
     const int ax = (info & 7) >> 1;
@@ -252,10 +253,11 @@ __device__ __forceinline__ bool face_moments_bnd(const DevParams &P, const R *u,
 // basis on a cube, grad phi_k = e_k / h and phi_k(gp) = +-1/(2 sqrt 3), so
 // int[v][1+d] -= (wJ/h) sum_gp F_d,v and the gravity term adds
 // -g_v sum_gp rho phi_k wJ.
-// tv (PH = 1, may be null): the temperature at volume point g is tv[g * ts]
+// tv (PH = 1, may be null): the EOS closure at volume point g starts at
+// tv[g * ts], its planes kMemoRows * ts apart
 template <typename R, int NC, int PH>
 __device__ __forceinline__ void volume_integral(const DevParams &P, const R *u, R h, R *res,
-                                                const double *tv = nullptr, int64_t ts = 0)
+                                                const R *tv = nullptr, int64_t ts = 0)
 {
     constexpr int NV = 4 + NC;
     const R ph = R(0.28867513459481288225);           // 1/(2 sqrt 3)
@@ -273,7 +275,7 @@ __device__ __forceinline__ void volume_integral(const DevParams &P, const R *u, 
         R t[NV];
         trace<R, NV>(u, px, py, pz, t);
         Prim<R, NC> p;
-        cons_to_prim<R, NC, PH>(P, t, p, tv ? tv + g * ts : nullptr);
+        cons_to_prim<R, NC, PH>(P, t, p, tv ? tv + g * ts : nullptr, kMemoRows * ts);
         const R m[3] = {t[0], t[1], t[2]};
         const R vel[3] = {p.u, p.v, p.w};
         #pragma unroll
@@ -408,15 +410,16 @@ void k_stage(StageArgs A, DevParams Pv)
         // faces of a chunk are sorted by (axis, kind): waves are mostly
         // uniform in the branch taken here
         const int ax = (rec.z & 7) >> 1;
-        // PH = 1: each side's point temperatures from the point-temperature
-        // pass, at the side's own (sub)face slot 4 d + j (point k at k * n)
-        const double *pl = nullptr, *pr = nullptr;
+        // PH = 1: each side's point EOS closures from the point pass, at
+        // the side's own (sub)face slot 4 d + j (point k at k * n)
+        const R *pl = nullptr, *pr = nullptr;
         if (PH == 1 && A.tpt) {
+            const R *TP = reinterpret_cast<const R *>(A.tpt);
             const int dir0 = rec.z & 7, kind = (rec.z >> 3) & 3, sub = (rec.z >> 5) & 3;
             const int s0 = 4 * dir0 + (kind == FACE_HANG1 ? sub : 0);
             const int s1 = 4 * (dir0 ^ 1) + (kind == FACE_HANG0 ? sub : 0);
-            pl = A.tpt + (int64_t) 6 * s0 * n + rec.x;
-            if (kind != FACE_BND) pr = A.tpt + (int64_t) 6 * s1 * n + rec.y;
+            pl = TP + (int64_t) 6 * s0 * n + rec.x;
+            if (kind != FACE_BND) pr = TP + (int64_t) 6 * s1 * n + rec.y;
         }
         if (((rec.z >> 3) & 3) == FACE_BND) {
             ok &= face_moments_bnd<R, NC, FLUX, PH, DEV>(P, ua, rec.z, lev0, dst, rf, pl, n);
@@ -444,7 +447,9 @@ void k_stage(StageArgs A, DevParams Pv)
             volume_integral_dev(P, rf, u, h, res);
         } else {
             load_conv<S, R, NV>(U, n, e, u);
-            volume_integral<R, NC, PH>(P, u, h, res, (PH == 1 && A.tvol) ? A.tvol + e : nullptr, n);
+            volume_integral<R, NC, PH>(P, u, h, res,
+                                       (PH == 1 && A.tpt) ? reinterpret_cast<const R *>(A.tpt) + (int64_t) kMemoVolRow * n + e
+                                                          : nullptr, n);
         }
         if constexpr (PH == 1) {
             if (A.visc) {
