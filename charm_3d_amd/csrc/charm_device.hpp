@@ -10,10 +10,12 @@
 namespace charm {
 
 constexpr int kMaxComp = 8;
-// rows of the point pass's EOS closures per leaf: 24 (sub)face slots x 6
-// points, then the 8 volume points; planes m, cv, gamma of kMemoRows rows
+// rows of the point pass per leaf: 24 (sub)face slots x 6 points, then the
+// 8 volume points; planes m, cv, gamma (the EOS closure), mu, k (transport)
+// of kMemoRows rows each
 constexpr int kMemoVolRow = 24 * 6;
 constexpr int kMemoRows = kMemoVolRow + 8;
+constexpr int kMemoPlanes = 5;
 constexpr double kGasR = 8.31446261815324;       // src/charm_eos.c:8
 constexpr double kEps = 1.e-12;                   // CHARM_EPS
 enum { FLUX_LF = 0, FLUX_HLLC = 1, FLUX_GODUNOV = 2, FLUX_CD = 3 };

@@ -45,10 +45,11 @@ struct charm_gpu_ctx {
     void *d_lift;             // [24][28][n] BR2: the (sub)face liftings
     double *d_dcoef;          // [nc][n] diffusion coefficients
     double *d_chem;           // [n] chemistry energy source
-    // point EOS closures (allocated when ph and a species has a polynomial
-    // cp): [3][kMemoRows][n] in the state's precision, planes m, cv, gamma,
-    // rows the own face points then the volume points; temps_of is the state
-    // they hold, for the stage in progress (reset after it)
+    // point EOS closures and transport (allocated when ph and a species has
+    // a polynomial cp): [kMemoPlanes][kMemoRows][n] in the state's precision,
+    // planes m, cv, gamma, mu, k, rows the own face points then the volume
+    // points; temps_of is the state they hold, for the stage in progress
+    // (reset after it)
     void *d_tpt;
     const void *temps_of;
     // multi-tree mesh (charm_gpu_upload_gmesh; gm = 1): the general kernels
@@ -329,7 +330,7 @@ extern "C" int charm_gpu_upload_mesh(charm_gpu_ctx_t *c, const charm_mesh_t *m)
         DMALLOC(c->d_dcoef, (size_t) m->n * c->nc * sizeof(double));
         DMALLOC(c->d_chem, (size_t) m->n * sizeof(double));
         if (c->P.newton_t) {
-            DMALLOC(c->d_tpt, (size_t) m->n * 3 * kMemoRows * c->esz);
+            DMALLOC(c->d_tpt, (size_t) m->n * kMemoPlanes * kMemoRows * c->esz);
         }
     }
     c->hq = (int32_t *) malloc((size_t) (m->n ? m->n : 1) * 4 * sizeof(int32_t));

@@ -123,11 +123,12 @@ struct NsArgs {
     void *visc_out;           // [16][n] viscous integrals of ru rv rw re
     void *aux_own;            // [36][n] BR2: own-trace tau, q (written by code 4, read by code 5)
     void *lift;               // [24][kLiftFields][n] BR2: each (sub)face slot's lifting (codes 4, 5)
-    // point EOS closures of the state U, in its precision (code 6 writes
-    // them; the viscous kernels and the stage read them where non-null):
-    // planes m, cv, gamma of kMemoRows rows, row (4 d + j) * 6 + k at point
-    // k of (sub)face slot 4 d + j of each leaf's own face points, row
-    // kMemoVolRow + g at volume point g: [3][kMemoRows][n]
+    // point EOS closures and transport of the state U, in its precision
+    // (code 6 writes them; the viscous kernels and the stage read them where
+    // non-null): planes m, cv, gamma, mu, k of kMemoRows rows, row
+    // (4 d + j) * 6 + k at point k of (sub)face slot 4 d + j of each leaf's
+    // own face points, row kMemoVolRow + g at volume point g:
+    // [kMemoPlanes][kMemoRows][n]
     void *tpt;
     int64_t n;
     double dt;

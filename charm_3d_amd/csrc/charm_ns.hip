@@ -46,6 +46,16 @@ __device__ __forceinline__ void transport(const DevParams &P, const Prim<R, NC> 
     kt = (R) k;
 }
 
+// the transport of a point state: the point pass's mu and k (planes 3 and
+// 4 of its row tk, qs apart) when tk is set, else evaluated
+template <typename R, int NC>
+__device__ __forceinline__ void transport_at(const DevParams &P, const Prim<R, NC> &p, const R *tk, int64_t qs,
+                                             R &mu, R &kt)
+{
+    if (tk) { mu = tk[3 * qs]; kt = tk[4 * qs]; }
+    else transport<R, NC>(P, p, mu, kt);
+}
+
 // enthalpy h0 + sum_k cp_k (T^(k+1) - T_ref^(k+1)) / (k+1),
 // charm_comp_calc_enthalpy (src/charm_globals.c:679-694), as the polynomial
 // hH0 + T (hc0 + T (hc1 + ...)) with hc_k = cp_k / (k+1) and the T_ref terms
@@ -299,9 +309,10 @@ void k_ns_grad(NsArgs A, DevParams)
     for (int g = 0; g < 8; g++) {
         const R ph[3] = {(g & 1) ? gph : -gph, (g & 2) ? gph : -gph, (g & 4) ? gph : -gph};
         Prim<R, NC> p;
-        state_at<R, NC>(P, u, ph, p, tvol_at<R>(A, g, e), kMemoRows * n);
+        const R *tg = tvol_at<R>(A, g, e);
+        state_at<R, NC>(P, u, ph, p, tg, kMemoRows * n);
         R mu, kt;
-        transport<R, NC>(P, p, mu, kt);
+        transport_at<R, NC>(P, p, tg, kMemoRows * n, mu, kt);
         const R lp = R(4.0 / 3.0) * mu, lm = R(-2.0 / 3.0) * mu;
         const R vel[3] = {p.u, p.v, p.w};
         const double t = (double) p.e / (double) p.cv;
@@ -341,11 +352,12 @@ void k_ns_grad(NsArgs A, DevParams)
     auto inner = [&](int d, const R *un, int nb, const R ps[3], const R pn[3], R wj, R sign, R (*L)[4], int k,
                      int js, int jn) {
         Prim<R, NC> pa, pb;
-        state_at<R, NC>(P, u, ps, pa, tpt_at<R>(A, 4 * d + js, k, e), kMemoRows * n);
-        state_at<R, NC>(P, un, pn, pb, tpt_at<R>(A, 4 * (d ^ 1) + jn, k, nb), kMemoRows * n);
+        const R *ta_ = tpt_at<R>(A, 4 * d + js, k, e), *tb_ = tpt_at<R>(A, 4 * (d ^ 1) + jn, k, nb);
+        state_at<R, NC>(P, u, ps, pa, ta_, kMemoRows * n);
+        state_at<R, NC>(P, un, pn, pb, tb_, kMemoRows * n);
         R mua, kta, mub, ktb;
-        transport<R, NC>(P, pa, mua, kta);
-        transport<R, NC>(P, pb, mub, ktb);
+        transport_at<R, NC>(P, pa, ta_, kMemoRows * n, mua, kta);
+        transport_at<R, NC>(P, pb, tb_, kMemoRows * n, mub, ktb);
         const R fmu = R(0.5) * (mua + mub), kt = R(0.5) * (kta + ktb);
         const R lp = R(4.0 / 3.0) * fmu, lm = R(-2.0 / 3.0) * fmu;
         const R ta = dv(pa.e, pa.cv);
@@ -382,10 +394,11 @@ void k_ns_grad(NsArgs A, DevParams)
             const int a = d >> 1;
             const R sg = (d & 1) ? R(1) : R(-1);
             Prim<R, NC> pa, pb;
-            state_at<R, NC>(P, u, ps, pa, tpt_at<R>(A, 4 * d, k, e), kMemoRows * n);
+            const R *tk = tpt_at<R>(A, 4 * d, k, e);
+            state_at<R, NC>(P, u, ps, pa, tk, kMemoRows * n);
             bnd_state<R, NC, 1>(P, face, a, sg, pa, pb, -1);
             R mu, kt;
-            transport<R, NC>(P, pa, mu, kt);
+            transport_at<R, NC>(P, pa, tk, kMemoRows * n, mu, kt);
             const R lp = R(4.0 / 3.0) * mu, lm = R(-2.0 / 3.0) * mu;
             R fu, fv, fw, ft;
             if (P.bnd_type[face] == BND_WALL_NO_SLIP) {
@@ -662,8 +675,10 @@ void k_ns_visc(NsArgs A, DevParams)
 // at its own face points (the points face_points and face_subfaces visit: a
 // (sub)face slot 4 d + j, points k = 0..5), the FP64 Newton temperature of
 // _calc_t (src/charm_eos.c:12-54) and the mixture's m, cv and gamma at it
-// (src/charm_eos.c:98-148), stored in the state's precision for the PH = 1
-// closures of the viscous passes and the stage kernel on the same state.
+// (src/charm_eos.c:98-148), and the viscosity and conductivity there
+// (src/charm_globals.c:515-561), stored in the state's precision for the
+// PH = 1 closures and transport of the viscous passes and the stage kernel
+// on the same state.
 // Each interior face point is evaluated once per side instead of once per
 // kernel and side.  One thread per leaf and part (blockIdx.y: face d = 0..5,
 // or 6 the volume points), so that the long dependent Newton chains of a
@@ -683,8 +698,8 @@ void k_ns_temps(NsArgs A, DevParams)
     R u[4 * NV];
     load_coeffs<R, NV>(U, n, e, u);
     // the EOS closure (m, cv, gamma) of the point state, as cons_to_prim
-    // forms it (the FP64 Newton temperature, the mixture's cp at it), into
-    // row `row` of the three planes
+    // forms it (the FP64 Newton temperature, the mixture's cp at it), and
+    // its transport coefficients (mu, k), into row `row` of the five planes
     R *__restrict__ W = reinterpret_cast<R *>(A.tpt);
     const int64_t qs = (int64_t) kMemoRows * n;
     auto closure = [&](const R ph[3], int row) {
@@ -693,8 +708,10 @@ void k_ns_temps(NsArgs A, DevParams)
         for (int v = 0; v < NV; v++) t[v] = at<R>(&u[4 * v], ph[0], ph[1], ph[2]);
         Prim<R, NC> p;
         cons_to_prim<R, NC, 1>(P, t, p);
+        R mu, kt;
+        transport<R, NC>(P, p, mu, kt);
         R *w = W + (int64_t) row * n + e;
-        w[0] = p.m; w[qs] = p.cv; w[2 * qs] = p.gam;
+        w[0] = p.m; w[qs] = p.cv; w[2 * qs] = p.gam; w[3 * qs] = mu; w[4 * qs] = kt;
     };
     if (part == 6) {
         const R gph = R(0.28867513459481288225);
