@@ -741,7 +741,7 @@ template <typename S>
 static hipError_t launch_ns_r(int what, int nc, const NsArgs &A, const DevParams &P, hipStream_t st)
 {
     const unsigned bs = (what >= 2 && what != 6) ? 128u : 256u;
-    // the temperature pass runs a thread per leaf and part (6 faces, volume)
+    // the point pass runs a thread per leaf and part (6 faces, volume)
     const dim3 grid((unsigned) ((A.n + bs - 1) / bs), what == 6 ? 7u : 1u), block(bs);
 #define CHARM_NS_CASE(NCV)                                                                         \
     case NCV:                                                                                      \
