@@ -413,13 +413,13 @@ def bench_hot_spot(CA, W):
 
 def gmesh_stage_bytes(nv, real_bytes, faces_per_leaf):
     """HBM bytes per leaf of one multi-tree stage (face pass + element pass)
-    by design: per face the 42 basis / weight words, the FP64 normal, two
+    by design: per face the 30 basis / weight words, the FP64 normal, two
     leaf and two local-face indices and both sides' states; per leaf its 54
     geometry words (51 and the 3 basis means), the state read and written,
     its six face-sum slots written by the face pass and read by the element
     pass, and its cell means for the limiter."""
     s = real_bytes * 4 * nv
-    face = real_bytes * 42 + 8 * 3 + 4 * 2 + 4 * 2 + 2 * s
+    face = real_bytes * 30 + 8 * 3 + 4 * 2 + 4 * 2 + 2 * s
     leaf = real_bytes * 54 + 2 * s + 6 * s + 6 * s + real_bytes * nv
     return faces_per_leaf * face + leaf
 

@@ -401,16 +401,14 @@ int charm_gmesh_build(int64_t n_vertices, const double *vertices, int64_t n_tree
                 for (a = 0; a < 3; a++) m->fn[(size_t) a * nf + fi] = nn[a];
                 for (pt = 0; pt < 6; pt++) {
                     const double *x = g.fgp[f][pt];
-                    for (a = 0; a < 3; a++) {
-                        m->fg[(size_t) (3 * pt + a) * nf + fi] = (x[a] - g.c[a]) / g.dh[a];
-                        if (p >= 0) {
-                            const double *g1 = m->geo + 23 * (size_t) p;
-                            m->fg[(size_t) (18 + 3 * pt + a) * nf + fi] = (x[a] - g1[a]) / g1[3 + a];
-                        } else {
-                            m->fg[(size_t) (18 + 3 * pt + a) * nf + fi] = 0.0;
-                        }
-                    }
-                    m->fg[(size_t) (36 + pt) * nf + fi] = g.fwj[f][pt];
+                    for (a = 0; a < 3; a++) m->fg[(size_t) (3 * pt + a) * nf + fi] = (x[a] - g.c[a]) / g.dh[a];
+                    m->fg[(size_t) (24 + pt) * nf + fi] = g.fwj[f][pt];
+                }
+                /* side 1's basis (x - c1) / dh1 = b0 dh0 / dh1 + (c0 - c1) / dh1 */
+                for (a = 0; a < 3; a++) {
+                    const double *g1 = p >= 0 ? m->geo + 23 * (size_t) p : NULL;
+                    m->fg[(size_t) (18 + a) * nf + fi] = g1 ? g.dh[a] / g1[3 + a] : 0.0;
+                    m->fg[(size_t) (21 + a) * nf + fi] = g1 ? (g.c[a] - g1[a]) / g1[3 + a] : 0.0;
                 }
             }
         }

@@ -136,10 +136,12 @@ void k_gface(GArgs A, DevParams P)
         #pragma unroll
         for (int k = 0; k < 4; k++) { s0[v][k] = R(0); s1[v][k] = R(0); }
     bool ok = true;
+    // side 1's basis is the affine image of side 0's: b1 = b0 r + o per axis
+    const R r1[3] = {FG[18 * nf], FG[19 * nf], FG[20 * nf]}, o1[3] = {FG[21 * nf], FG[22 * nf], FG[23 * nf]};
     for (int p = 0; p < 6; p++) {
         const R b0[3] = {FG[(3 * p) * nf], FG[(3 * p + 1) * nf], FG[(3 * p + 2) * nf]};
-        const R b1[3] = {FG[(18 + 3 * p) * nf], FG[(19 + 3 * p) * nf], FG[(20 + 3 * p) * nf]};
-        const R wj = FG[(36 + p) * nf];
+        const R b1[3] = {b0[0] * r1[0] + o1[0], b0[1] * r1[1] + o1[1], b0[2] * r1[2] + o1[2]};
+        const R wj = FG[(24 + p) * nf];
         R t[NV];
         Prim<R, NC> pa, pb;
         trace_at<R, NV>(u0, b0, t);

@@ -81,9 +81,11 @@ int      charm_gj_inverse4(const double src[4][4], double inv[4][4]);
  * 8 volume points (3-26), their weights wJ (27-34), A^-1 row-major (35-50),
  * basis values at the 8 vertices (51-74), the leaf means of the 3 linear
  * basis functions (75-77).  Per face (CHARM_GM_FACEQ): basis values of side 0
- * (0-17) and side 1 (18-35) at the 6 points, the points' weights wJ (36-41). */
+ * at the 6 points (0-17); side 1's as the affine image of side 0's, b1_a =
+ * b0_a r_a + o_a with r_a = dh0_a / dh1_a (18-20) and o_a = (c0_a - c1_a) /
+ * dh1_a (21-23), zero on a boundary face; the points' weights wJ (24-29). */
 #define CHARM_GM_LEAFQ 78
-#define CHARM_GM_FACEQ 42
+#define CHARM_GM_FACEQ 30
 typedef struct {
     int64_t n, nf;
     const double *lg, *fg, *fn;   /* fn: [3][nf] side-0 outward normals */
