@@ -369,14 +369,11 @@ __device__ __forceinline__ const DevParams &stage_params(const StageArgs &A, con
 // The FP32 deviation-form instances are held to at most 168 VGPRs (three
 // waves per SIMD) so that a change of the code cannot drop them below the
 // occupancy their LDS allows; with the rolled point loop they need 117, and
-// four 40 KB workgroups share a CU.  The FP32 Navier-Stokes instances (PH =
-// 1) are held there too: they need 210 VGPRs at two waves, and at three,
-// with 152 bytes of their least-used values in scratch, the reacting hot
-// spot's stage takes 0.26 ms instead of 0.30.
+// four 40 KB workgroups share a CU
 template <typename R, int NC, int FLUX, int PH>
 constexpr int stage_min_waves()
 {
-    return (uses_dev<R, NC, FLUX, PH>() || (PH == 1 && sizeof(R) == 4)) ? 3 : 1;
+    return uses_dev<R, NC, FLUX, PH>() ? 3 : 1;
 }
 
 template <typename S, typename R, int NC, int FLUX, int BS, int PH>
